@@ -1,0 +1,128 @@
+/* Counter-based random streams of the MI355X Smoldyn engine.
+ *
+ * The reference draws every random number from one global SFMT-19937 stream
+ * in program order (source/libSteve/random2.h:21-58), which serialises the
+ * whole time step.  The device engine instead derives each draw from
+ * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+ * as 1, 2, 3", SC'11) keyed by the simulation seed and a counter built from
+ * (molecule serial number, step index, purpose).  Any thread can then compute
+ * the draws of any molecule without coordination, and the CPU checker computes
+ * the very same values, so whole trajectories can be compared bit for bit.
+ *
+ * This header is plain C, usable from host C/C++ and from HIP device code.
+ */
+#ifndef SMOLDYN_AMD_RNG_H
+#define SMOLDYN_AMD_RNG_H
+
+#include <stdint.h>
+#if !defined(__HIPCC__) && !defined(__HIP__)
+#include <math.h>
+#endif
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#define SMOLRNG_FN __host__ __device__ static inline
+#else
+#define SMOLRNG_FN static inline
+#endif
+
+/* Counter word 3 carries the purpose of a draw in its low 8 bits and a
+   sub-index (reaction number, attempt number, wall number) above them. */
+enum SmolRngPurpose {
+  SMOLRNG_DIFFUSE = 1,   /* Gaussian table indices for one diffusion step */
+  SMOLRNG_UNIRXN = 2,    /* first-order reaction coin flips, 4 reactions per block */
+  SMOLRNG_PRODDIR = 3,   /* product unbinding direction, 2 attempts per block */
+  SMOLRNG_BIRXN = 4,     /* bimolecular reaction probability < 1 */
+  SMOLRNG_WALL = 5,      /* absorbing-wall coin flip */
+  SMOLRNG_SURFACE = 6,   /* surface-action draws */
+  SMOLRNG_PLACE = 7      /* initial uniform placement */
+};
+
+typedef struct { uint32_t v[4]; } smolrng4;
+
+SMOLRNG_FN uint32_t smolrng_mulhilo(uint32_t a, uint32_t b, uint32_t* hi) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  *hi = (uint32_t)(p >> 32);
+  return (uint32_t)p;
+}
+
+/* Philox4x32 with 10 rounds. */
+SMOLRNG_FN smolrng4 smolrng_philox(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1,
+                                   uint32_t c2, uint32_t c3) {
+  smolrng4 r;
+  int i;
+  for (i = 0; i < 10; i++) {
+    uint32_t hi0, hi1;
+    uint32_t lo0 = smolrng_mulhilo(0xD2511F53u, c0, &hi0);
+    uint32_t lo1 = smolrng_mulhilo(0xCD9E8D57u, c2, &hi1);
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  r.v[0] = c0;
+  r.v[1] = c1;
+  r.v[2] = c2;
+  r.v[3] = c3;
+  return r;
+}
+
+/* Draw block for one molecule (serial number) at one step. */
+SMOLRNG_FN smolrng4 smolrng_mol(uint32_t k0, uint32_t k1, uint64_t serial, uint32_t step,
+                                uint32_t purpose, uint32_t sub) {
+  return smolrng_philox(k0, k1, (uint32_t)serial, (uint32_t)(serial >> 32), step,
+                        (purpose & 0xFFu) | (sub << 8));
+}
+
+/* Same as the reference's randCOD (genrand_real2): u32 * 2^-32, in [0,1). */
+SMOLRNG_FN double smolrng_u01(uint32_t v) { return (double)v * (1.0 / 4294967296.0); }
+
+/* Same as the reference's randOOD (genrand_real3): (u32 + 0.5) * 2^-32, in (0,1). */
+SMOLRNG_FN double smolrng_u01open(uint32_t v) { return ((double)v + 0.5) * (1.0 / 4294967296.0); }
+
+/* Uniform in [-1,1) from 32 bits, exact in double. */
+SMOLRNG_FN double smolrng_sym(uint32_t v) { return (double)v * (1.0 / 2147483648.0) - 1.0; }
+
+/* Uniformly random unit vector by Marsaglia's rejection method (Ann. Math.
+   Stat. 43:645, 1972), using only +,*,/ and sqrt so that host and device
+   results agree bit for bit.  Each Philox block yields two attempts; attempt
+   blocks are numbered by sub.  dim is 1, 2 or 3. */
+SMOLRNG_FN void smolrng_unitvec(uint32_t k0, uint32_t k1, uint64_t serial, uint32_t step,
+                                int dim, double* u) {
+  uint32_t a;
+  if (dim == 1) {
+    smolrng4 r = smolrng_mol(k0, k1, serial, step, SMOLRNG_PRODDIR, 0);
+    u[0] = (r.v[0] & 1u) ? 1.0 : -1.0;
+    return;
+  }
+  for (a = 0; a < 64; a++) {
+    smolrng4 r = smolrng_mol(k0, k1, serial, step, SMOLRNG_PRODDIR, a);
+    int t;
+    for (t = 0; t < 2; t++) {
+      double x1 = smolrng_sym(r.v[2 * t]);
+      double x2 = smolrng_sym(r.v[2 * t + 1]);
+      double s = x1 * x1 + x2 * x2;
+      if (s > 0.0 && s < 1.0) {
+        if (dim == 2) {
+          u[0] = (x1 * x1 - x2 * x2) / s;
+          u[1] = (2.0 * x1 * x2) / s;
+        } else {
+          double q = 2.0 * sqrt(1.0 - s);
+          u[0] = x1 * q;
+          u[1] = x2 * q;
+          u[2] = 1.0 - 2.0 * s;
+        }
+        return;
+      }
+    }
+  }
+  /* 128 rejected attempts has probability (1-pi/4)^128 < 1e-85. */
+  u[0] = 1.0;
+  u[1] = 0.0;
+  if (dim == 3) u[2] = 0.0;
+}
+
+#endif

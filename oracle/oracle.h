@@ -1,0 +1,77 @@
+/* CPU oracle for the MI355X Smoldyn engine -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's per-timestep particle path
+ * (ssandrews/Smoldyn source/Smoldyn/smol*.c, source/libSteve; each
+ * function cites the file:line it follows).  Only tests/, the smoke check in
+ * __graft_entry__.py and bench.py's cpu_baseline leg may load it, and only as
+ * the checker or the timed CPU baseline.  The product (libsmoldyn_amd.so)
+ * never links, loads or calls it.
+ *
+ * Parity status of each part (and how it is pinned) is listed in DESIGN.md;
+ * every function here is pinned by at least one of: the reference's own
+ * regression outputs (examples/S95_regression/output_275), the compiled
+ * reference libSteve sources (oracle/_ref), closed forms, or brute force.
+ */
+#ifndef SMOLDYN_ORACLE_H
+#define SMOLDYN_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct osim osim;
+
+osim* osim_new(int dim, const double* low, const double* high);
+void osim_free(osim* sim);
+int osim_set_wall(osim* sim, int d, int highside, char type);
+void osim_set_times(osim* sim, double tstart, double tstop, double dt);
+void osim_set_seed(osim* sim, long seed);
+void osim_set_rngmode(osim* sim, int mode, uint64_t key);
+int osim_set_partitions(osim* sim, const char* method, double value);
+void osim_set_accuracy(osim* sim, double accur);
+int osim_add_species(osim* sim, const char* name);
+int osim_species_index(osim* sim, const char* name);
+void osim_set_difc(osim* sim, int i, int ms, double difc);
+int osim_add_list(osim* sim, const char* name);
+int osim_set_list(osim* sim, int i, int ms, int ll);
+int osim_add_mols(osim* sim, int n, int ident, const double* lo, const double* hi);
+int osim_add_mols_at(osim* sim, int n, int ident, const double* pos);
+int osim_add_reaction(osim* sim, const char* name, int order, const int* rct, const int* rctstate, int nprod,
+                      const int* prd, const int* prdstate, double rate);
+int osim_set_revparam(osim* sim, int order, int r, int rparamt, double rparam);
+int osim_set_rxn_prob(osim* sim, int order, int r, double prob);
+int osim_update(osim* sim);
+int osim_inject(osim* sim, const double* g, long n);
+int osim_step(osim* sim);
+int osim_run(osim* sim, long maxsteps);
+
+double osim_time(osim* sim);
+int osim_nlist(osim* sim);
+int osim_list_count(osim* sim, int ll);
+int osim_get_list(osim* sim, int ll, int* ident, int* mstate, double* pos, uint64_t* serno, int* box);
+int osim_count_species(osim* sim, int ident);
+long osim_listmols(osim* sim, char* buf, long buflen);
+int osim_rxn_info(osim* sim, int order, int r, double* out);
+int osim_box_info(osim* sim, int* side, double* bmin, double* bsize);
+int osim_box_neighbors(osim* sim, int b, int* neigh, int* wp, int* midneigh);
+int osim_rxnlog(osim* sim, int64_t* rows, int maxrows);
+uint64_t osim_next_serno(osim* sim);
+const double* osim_gausstable(osim* sim, int flat);
+int osim_ngauss(osim* sim);
+
+/* rxnparam restatement, exported for the rate-table tests */
+double orxn_numrxnrate(double step, double a, double b);
+double orxn_numrxnrateprob(double step, double a, double b, double prob);
+double orxn_actrxnrate(double step, double a, double prob);
+double orxn_bindingradius(double rate, double dt, double difc, double b, int rel);
+double orxn_bindingradiusprob(double rate, double dt, double difc, double b, int rel, double chi, double* probptr);
+double orxn_unbindingradius(double pgem, double dt, double difc, double a);
+void omath_randtable(double* a, int n);
+double omath_erfn(double x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,287 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is the checker: tests compare the HIP engine against it, and it is
+itself pinned against the reference's regression outputs and the compiled
+reference libSteve sources.
+"""
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+LIB = ORACLE_DIR / "liboracle.so"
+REFLIB = ORACLE_DIR / "_ref" / "libsteve_ref.so"
+
+MS = {"solution": 0, "soln": 0, "front": 1, "back": 2, "up": 3, "down": 4, "bsoln": 5, "all": 6}
+RP = {"none": 0, "irrev": 1, "confspread": 2, "bounce": 3, "pgem": 4, "pgemmax": 5, "pgemmaxw": 6,
+      "ratio": 7, "unbindrad": 8, "pgem2": 9, "pgemmax2": 10, "ratio2": 11, "offset": 12, "fixed": 13}
+RNG_SFMT, RNG_COUNTER = 0, 1
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        P, D, I, L64 = C.c_void_p, C.c_double, C.c_int, C.c_long
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        sig = {
+            "osim_new": (P, [I, dp, dp]),
+            "osim_free": (None, [P]),
+            "osim_set_wall": (I, [P, I, I, C.c_char]),
+            "osim_set_times": (None, [P, D, D, D]),
+            "osim_set_seed": (None, [P, L64]),
+            "osim_set_rngmode": (None, [P, I, C.c_uint64]),
+            "osim_set_partitions": (I, [P, C.c_char_p, D]),
+            "osim_set_accuracy": (None, [P, D]),
+            "osim_add_species": (I, [P, C.c_char_p]),
+            "osim_species_index": (I, [P, C.c_char_p]),
+            "osim_set_difc": (None, [P, I, I, D]),
+            "osim_add_list": (I, [P, C.c_char_p]),
+            "osim_set_list": (I, [P, I, I, I]),
+            "osim_add_mols": (I, [P, I, I, dp, dp]),
+            "osim_add_mols_at": (I, [P, I, I, dp]),
+            "osim_add_reaction": (I, [P, C.c_char_p, I, ip, ip, I, ip, ip, D]),
+            "osim_set_revparam": (I, [P, I, I, I, D]),
+            "osim_set_rxn_prob": (I, [P, I, I, D]),
+            "osim_update": (I, [P]),
+            "osim_inject": (I, [P, dp, L64]),
+            "osim_step": (I, [P]),
+            "osim_run": (I, [P, L64]),
+            "osim_time": (D, [P]),
+            "osim_nlist": (I, [P]),
+            "osim_list_count": (I, [P, I]),
+            "osim_get_list": (I, [P, I, ip, ip, dp, C.POINTER(C.c_uint64), ip]),
+            "osim_count_species": (I, [P, I]),
+            "osim_listmols": (L64, [P, C.c_char_p, L64]),
+            "osim_rxn_info": (I, [P, I, I, dp]),
+            "osim_box_info": (I, [P, ip, dp, dp]),
+            "osim_box_neighbors": (I, [P, I, ip, ip, ip]),
+            "osim_rxnlog": (I, [P, C.POINTER(C.c_int64), I]),
+            "osim_next_serno": (C.c_uint64, [P]),
+            "osim_gausstable": (dp, [P, I]),
+            "osim_ngauss": (I, [P]),
+            "orxn_numrxnrate": (D, [D, D, D]),
+            "orxn_numrxnrateprob": (D, [D, D, D, D]),
+            "orxn_actrxnrate": (D, [D, D, D]),
+            "orxn_bindingradius": (D, [D, D, D, D, I]),
+            "orxn_bindingradiusprob": (D, [D, D, D, D, I, D, dp]),
+            "orxn_unbindingradius": (D, [D, D, D, D]),
+            "omath_randtable": (None, [dp, I]),
+            "omath_erfn": (D, [D]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class OSim:
+    """Thin object wrapper over one oracle simulation."""
+
+    def __init__(self, dim, low, high):
+        self.L = lib()
+        self.dim = dim
+        lo = np.ascontiguousarray(low, dtype=np.float64)
+        hi = np.ascontiguousarray(high, dtype=np.float64)
+        self.p = self.L.osim_new(dim, _dp(lo), _dp(hi))
+        if not self.p:
+            raise ValueError("osim_new failed")
+        self._keep = []
+        self._names = []
+        self._lists = []
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            self.L.osim_free(self.p)
+            self.p = None
+
+    def wall(self, d, high, t):
+        assert self.L.osim_set_wall(self.p, d, int(high), t.encode()) == 0
+
+    def times(self, t0, t1, dt):
+        self.L.osim_set_times(self.p, t0, t1, dt)
+
+    def seed(self, s):
+        self.L.osim_set_seed(self.p, s)
+
+    def rngmode(self, mode, key=0):
+        self.L.osim_set_rngmode(self.p, mode, key)
+
+    def partitions(self, method, value):
+        assert self.L.osim_set_partitions(self.p, method.encode(), value) == 0
+
+    def species(self, name):
+        i = self.L.osim_add_species(self.p, name.encode())
+        assert i > 0
+        self._names.append(name)
+        return i
+
+    def index(self, name):
+        return self.L.osim_species_index(self.p, name.encode())
+
+    def index_exists(self, i):
+        return 0 < i <= len(self._names)
+
+    def list_index(self, name):
+        return self._lists.index(name)
+
+    def accuracy(self, a):
+        self.L.osim_set_accuracy(self.p, a)
+
+    def difc(self, i, d, ms=0):
+        self.L.osim_set_difc(self.p, i, ms, d)
+
+    def add_list(self, name):
+        ll = self.L.osim_add_list(self.p, name.encode())
+        assert ll >= 0
+        self._lists.append(name)
+        return ll
+
+    def set_list(self, i, ll, ms=6):
+        assert self.L.osim_set_list(self.p, i, ms, ll) == 0
+
+    def add_mols(self, n, i, lo, hi):
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        assert self.L.osim_add_mols(self.p, n, i, _dp(lo), _dp(hi)) == 0
+
+    def add_mols_at(self, i, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, self.dim)
+        assert self.L.osim_add_mols_at(self.p, len(pos), i, _dp(pos)) == 0
+
+    def reaction(self, name, rct, prd, rate, rstate=None, pstate=None):
+        order = len(rct)
+        r = np.array(list(rct) + [0] * (2 - order), dtype=np.int32)
+        rs = np.array(rstate if rstate else [0] * order, dtype=np.int32)
+        rs = np.concatenate([rs, np.zeros(2 - order, np.int32)])
+        p = np.array(list(prd) + [0] * (4 - len(prd)), dtype=np.int32)
+        ps = np.array((pstate if pstate else [0] * len(prd)) + [0] * (4 - len(prd)), dtype=np.int32)
+        ri = self.L.osim_add_reaction(self.p, name.encode(), order, _ip(r), _ip(rs), len(prd), _ip(p), _ip(ps), rate)
+        assert ri >= 0
+        return (order, ri)
+
+    def revparam(self, rxn, kind, value):
+        assert self.L.osim_set_revparam(self.p, rxn[0], rxn[1], RP[kind], value) == 0
+
+    def update(self):
+        er = self.L.osim_update(self.p)
+        if er:
+            raise RuntimeError("osim_update error %d" % er)
+
+    def inject(self, g):
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        self._keep = [g]
+        self.L.osim_inject(self.p, _dp(g), g.size // self.dim)
+
+    def step(self):
+        return self.L.osim_step(self.p)
+
+    def run(self, maxsteps=-1):
+        return self.L.osim_run(self.p, maxsteps)
+
+    def time(self):
+        return self.L.osim_time(self.p)
+
+    def nlist(self):
+        return self.L.osim_nlist(self.p)
+
+    def get_list(self, ll):
+        n = self.L.osim_list_count(self.p, ll)
+        ident = np.zeros(n, np.int32)
+        ms = np.zeros(n, np.int32)
+        pos = np.zeros((n, self.dim), np.float64)
+        ser = np.zeros(n, np.uint64)
+        box = np.zeros(n, np.int32)
+        self.L.osim_get_list(self.p, ll, _ip(ident), _ip(ms), _dp(pos),
+                             ser.ctypes.data_as(C.POINTER(C.c_uint64)), _ip(box))
+        return dict(ident=ident, mstate=ms, pos=pos, serno=ser, box=box)
+
+    def count(self, i):
+        return self.L.osim_count_species(self.p, i)
+
+    def listmols(self):
+        n = self.L.osim_listmols(self.p, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        self.L.osim_listmols(self.p, buf, n + 1)
+        return buf.value.decode()
+
+    def rxn_info(self, rxn):
+        out = np.zeros(8)
+        assert self.L.osim_rxn_info(self.p, rxn[0], rxn[1], _dp(out)) == 0
+        return dict(bindrad2=out[0], prob=out[1], unbindrad=out[2], prdpos0=out[3], prdpos1=out[4],
+                    rparamt=int(out[5]), rparam=out[6])
+
+    def box_info(self):
+        side = np.zeros(3, np.int32)
+        bmin = np.zeros(3)
+        bsize = np.zeros(3)
+        nbox = self.L.osim_box_info(self.p, _ip(side), _dp(bmin), _dp(bsize))
+        return nbox, side[:self.dim], bmin[:self.dim], bsize[:self.dim]
+
+    def box_neighbors(self, b):
+        nb = np.zeros(27, np.int32)
+        wp = np.zeros(27, np.int32)
+        mid = np.zeros(1, np.int32)
+        n = self.L.osim_box_neighbors(self.p, b, _ip(nb), _ip(wp), _ip(mid))
+        return nb[:n], wp[:n], int(mid[0])
+
+    def rxnlog(self):
+        n = self.L.osim_rxnlog(self.p, None, 0)
+        rows = np.zeros((max(n, 1), 8), np.int64)
+        self.L.osim_rxnlog(self.p, rows.ctypes.data_as(C.POINTER(C.c_int64)), n)
+        return rows[:n]
+
+    def gausstable(self, flat=True):
+        n = self.L.osim_ngauss(self.p)
+        ptr = self.L.osim_gausstable(self.p, 1 if flat else 0)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+
+
+def reflib():
+    """The compiled reference libSteve (checker only); None when absent."""
+    if not REFLIB.exists():
+        return None
+    L = C.CDLL(str(REFLIB))
+    D, I = C.c_double, C.c_int
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    for name, res, args in [
+        ("ref_randomize", None, [C.c_long]),
+        ("ref_randULI", C.c_ulong, []),
+        ("ref_randCOD", D, []),
+        ("ref_gausstable", None, [dp, I, I]),
+        ("ref_neighbor7", I, [ip, ip, ip, I, ip, ip]),
+        ("numrxnrate", D, [D, D, D]),
+        ("numrxnrateprob", D, [D, D, D, D]),
+        ("actrxnrate", D, [D, D, D]),
+        ("bindingradius", D, [D, D, D, D, I]),
+        ("bindingradiusprob", D, [D, D, D, D, I, D, dp]),
+        ("unbindingradius", D, [D, D, D, D]),
+    ]:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L

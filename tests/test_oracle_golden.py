@@ -1,0 +1,41 @@
+"""The oracle reproduces the reference's own regression outputs
+(examples/S95_regression, golden files of release 2.75) byte for byte.
+
+These runs use the reference's random source (SFMT-19937 in program order), so
+they pin the whole CPU path: Gaussian-table construction and shuffle, molecule
+placement, diffusion, reflective and periodic walls, virtual-box grid and
+neighbor lists (including wrap codes), live-list and dead-list ordering,
+first- and second-order reactions with their greedy pair order, product
+placement and serial numbers."""
+import gzip
+from pathlib import Path
+
+import pytest
+
+from oracle_bind import OSim
+from smolcfg import read_config
+
+S95 = Path(__file__).resolve().parent / "golden" / "s95"
+
+
+def run_model(name):
+    sim = read_config((S95 / (name + ".txt")).read_text(), lambda d, lo, hi: OSim(d, lo, hi))
+    sim.update()
+    code = sim.run()
+    return code, sim.listmols()
+
+
+@pytest.mark.parametrize("name", ["bounce2", "molecule", "lotvolt"])
+def test_s95_output_identical(name):
+    code, out = run_model(name)
+    assert code == 1
+    gold = gzip.open(S95 / (name + "_275_out.txt.gz")).read().decode()
+    assert out == gold
+
+
+@pytest.mark.xfail(reason="absorbing-wall path not reproduced; the device engine rejects absorbing walls",
+                   strict=True)
+def test_s95_bounds1_absorbing_walls():
+    code, out = run_model("bounds1")
+    gold = gzip.open(S95 / "bounds1_275_out.txt.gz").read().decode()
+    assert out == gold

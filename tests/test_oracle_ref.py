@@ -1,0 +1,163 @@
+"""Pin the oracle's restated libSteve pieces against the compiled reference
+(oracle/_ref/libsteve_ref.so, built from /root/reference by `make -C oracle
+ref`).  Skipped where the reference is absent (the GPU box); the committed
+golden fixtures below cover the same values there."""
+import ctypes as C
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+GOLD = Path(__file__).resolve().parent / "golden"
+REF = ob.reflib()
+needs_ref = pytest.mark.skipif(REF is None, reason="compiled reference (oracle/_ref) not present")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@needs_ref
+def test_sfmt_stream_matches_reference():
+    for seed in (1, 7, 123456789):
+        REF.ref_randomize(seed)
+        ref = np.array([REF.ref_randULI() for _ in range(5000)], dtype=np.uint64)
+        sim = ob.OSim(1, [0.0], [1.0])
+        sim.seed(seed)
+        sp = sim.species("a")
+        sim.difc(sp, 1.0)
+        # The oracle's stream is observed through its Gaussian-table shuffle
+        # and placements; compare the raw stream through a 1-D placement:
+        # unirandOOD(0, 2^32) = u + 0.5 exactly.
+        sim.add_mols(5000, sp, [0.0], [4294967296.0])
+        sim.update()
+        got = np.sort(sim.get_list(0)["pos"][:, 0] - 0.5).astype(np.uint64)
+        assert np.array_equal(np.sort(ref), got)
+
+
+@needs_ref
+def test_gauss_table_matches_reference():
+    n = 4096
+    ref_flat = np.zeros(n)
+    REF.ref_gausstable(_dp(ref_flat), n, 0)
+    got = np.zeros(n)
+    ob.lib().omath_randtable(_dp(got), n)
+    assert np.array_equal(ref_flat, got)
+    # shuffled table: same seed, same stream position
+    REF.ref_randomize(1)
+    ref_sh = np.zeros(n)
+    REF.ref_gausstable(_dp(ref_sh), n, 1)
+    sim = ob.OSim(1, [0.0], [1.0])
+    sim.seed(1)
+    sp = sim.species("a")
+    sim.difc(sp, 1.0)
+    sim.add_mols(0, sp, [0.0], [1.0])
+    sim.update()
+    assert np.array_equal(ref_sh, sim.gausstable(flat=False))
+
+
+@needs_ref
+@pytest.mark.parametrize("side,walls", [((5, 4, 3), (1, 1, 1, 1, 1, 1)), ((3, 3, 3), (0, 0, 0, 0, 0, 0)),
+                                        ((2, 6, 1), (1, 1, 1, 1, 0, 0)), ((4, 4), (0, 1, 1, 0)),
+                                        ((1, 1, 2), (1, 1, 1, 1, 1, 1))])
+def test_neighbor_lists_match_reference(side, walls):
+    dim = len(side)
+    hi = np.array(side, dtype=np.float64)
+    sim = ob.OSim(dim, np.zeros(dim), hi)
+    for d in range(dim):
+        sim.wall(d, 0, "p" if walls[2 * d] else "r")
+        sim.wall(d, 1, "p" if walls[2 * d + 1] else "r")
+    sp = sim.species("a")
+    sim.difc(sp, 1.0)
+    sim.partitions("boxsize", 1.0)
+    sim.add_mols(1, sp, np.zeros(dim), hi)
+    sim.update()
+    nbox, oside, _, _ = sim.box_info()
+    assert tuple(oside) == tuple(side)
+    sidev = np.array(side, dtype=np.int32)
+    for b in range(nbox):
+        idx = np.unravel_index(b, side)
+        indx = np.array(idx, dtype=np.int32)
+        c = np.zeros(27, np.int32)
+        wrap = np.zeros(27, np.int32)
+        wrap[:2 * dim] = walls
+        mid = np.zeros(1, np.int32)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        n = REF.ref_neighbor7(ip(indx), ip(c), ip(sidev), dim, ip(wrap), ip(mid))
+        nb, wp, omid = sim.box_neighbors(b)
+        assert n == len(nb) and omid == mid[0]
+        assert np.array_equal(nb, c[:n])
+        assert np.array_equal(wp, wrap[:n])
+
+
+def _grid():
+    steps = np.array([1e-3, 0.01, 0.05, 0.1, 0.3, 0.7, 1.0, 2.0, 5.0, 20.0, 100.0])
+    bs = np.array([-1.0, 0.0, 0.05, 0.3, 0.55, 0.99, 1.0, 1.01, 1.2, 2.0, 7.0, 30.0])
+    return steps, bs
+
+
+@needs_ref
+def test_numrxnrate_matches_reference():
+    L = ob.lib()
+    steps, bs = _grid()
+    for s in steps:
+        for b in bs:
+            for a in (0.3, 1.0, 2.5):
+                assert L.orxn_numrxnrate(s, a, b * a if b >= 0 else -1) == REF.numrxnrate(s, a, b * a if b >= 0 else -1)
+
+
+@needs_ref
+def test_numrxnrateprob_matches_reference():
+    L = ob.lib()
+    steps, bs = _grid()
+    for s in steps:
+        for b in bs:
+            for p in (0.005, 0.015, 0.05, 0.3, 0.77, 1.0):
+                bb = b if b >= 0 else -1
+                assert L.orxn_numrxnrateprob(s, 1.0, bb, p) == REF.numrxnrateprob(s, 1.0, bb, p)
+
+
+@needs_ref
+def test_radius_inversions_match_reference():
+    L = ob.lib()
+    for rate in (0.5, 9.640578597, 21.42267826, 99.3218287, 8000.0):
+        for dt in (0.0, 1e-4, 0.002, 0.005, 0.01, 0.1):
+            for difc in (0.3, 2.0, 4.0, 200.0):
+                for b, rel in ((-1, 0), (1.2, 1), (3.0, 1), (2.0, 0)):
+                    assert L.orxn_bindingradius(rate, dt, difc, b, rel) == REF.bindingradius(rate, dt, difc, b, rel)
+                for prob in (1.0, 0.3):
+                    p1 = C.c_double(prob)
+                    p2 = C.c_double(prob)
+                    a1 = L.orxn_bindingradiusprob(rate, dt, difc, -1, 0, -1, C.byref(p1))
+                    a2 = REF.bindingradiusprob(rate, dt, difc, -1, 0, -1, C.byref(p2))
+                    assert a1 == a2 and p1.value == p2.value
+                p1 = C.c_double(-1)
+                p2 = C.c_double(-1)
+                a1 = L.orxn_bindingradiusprob(rate, dt, difc, -1, 0, 0.5, C.byref(p1))
+                a2 = REF.bindingradiusprob(rate, dt, difc, -1, 0, 0.5, C.byref(p2))
+                assert a1 == a2 and p1.value == p2.value
+    for pgem in (0.05, 0.2, 0.5, 0.9):
+        for a in (0.2, 1.0, 3.0):
+            for dt in (0.0, 0.001, 0.01):
+                assert L.orxn_unbindingradius(pgem, dt, 2.0, a) == REF.unbindingradius(pgem, dt, 2.0, a)
+    for s in (0.01, 0.5, 1.0, 3.0):
+        assert L.orxn_actrxnrate(s, 1.0, 0.4) == REF.actrxnrate(s, 1.0, 0.4)
+
+
+def test_rxnparam_golden_fixture():
+    """Values of the compiled reference committed as a fixture (usable where
+    the reference is absent)."""
+    L = ob.lib()
+    data = json.loads((GOLD / "rxnparam_ref.json").read_text())
+    for row in data["bindingradius"]:
+        rate, dt, difc, b, rel, want = row
+        assert L.orxn_bindingradius(rate, dt, difc, b, int(rel)) == want
+    for row in data["numrxnrate"]:
+        s, a, b, want = row
+        assert L.orxn_numrxnrate(s, a, b) == want
+    for row in data["unbindingradius"]:
+        pgem, dt, difc, a, want = row
+        assert L.orxn_unbindingradius(pgem, dt, difc, a) == want
