@@ -1,0 +1,170 @@
+"""Benchmark of the MI355X Smoldyn particle step.
+
+Workload (the headline configuration): reversible A+B<->C in a periodic cube of
+side 1000 with 500k A + 500k B (1M molecules), D = 1/1/0.5, dt = 0.005,
+forward rate 99.3218287 (binding radius ~1.0 by the reaction-parameter solver)
+with unbinding radius 1.2x the binding radius, virtual boxes of ~4 molecules,
+FP64 positions.  One "step" is one reference time step (diffusion, walls,
+re-binning, first-order reactions, bimolecular pair search and reaction,
+list maintenance) of that whole system, with inputs resident in HBM.
+
+Prints one JSON line: molecule-steps/s (sum over the timed steps of the live
+molecules at step start, divided by the timed wall time), the dominant
+kernel's HBM roofline fraction (HIP-event timing, algorithmic bytes), and the
+CPU oracle's throughput on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic HBM bytes per molecule per launch of each kernel, FP64 SoA:
+#   k_diffuse  read pos 24 + species 4 + serial 8, write pos 24 + box 4 + rank 4,
+#              histogram atomic 4                                        = 72
+#   k_scatter  read box 4 + rank 4 + scan offsets 8, write slot 4         = 20
+#   k_boxsort  read slot 4 + box 4 + species 4 + pos 24 + box range 8,
+#              write slot 4 + box 4 + species 4 + pos 24                  = 80
+#   k_pairs    read own species 4 + slot 4 + box 4 + pos 24 + serial 8
+#              + box ranges 8 (neighbor entries are cache reuse)          = 52
+ALGO_BYTES = {"diffuse": 72, "scatter": 20, "boxsort": 80, "pairs": 52}
+KERNEL_NAMES = ["diffuse", "scan", "scatter", "boxsort", "pairs", "resolve"]
+HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_scan1", "scatter": "k_scatter", "boxsort": "k_boxsort",
+                     "pairs": "k_pairs", "resolve": "k_resolve"}
+
+
+def c2_spec(n_per_species=500000, side=1000.0, seed=2026, kback=1.0):
+    return dict(low=[0.0] * 3, high=[side] * 3, walls="ppp", dt=0.005, seed=seed,
+                species={"A": 1.0, "B": 1.0, "C": 0.5},
+                reactions=[("fwd", ["A", "B"], ["C"], 99.3218287), ("back", ["C"], ["A", "B"], kback)],
+                placement={"back": ("ratio", 1.2)},
+                molecules=[("A", n_per_species), ("B", n_per_species)])
+
+
+def build_device(spec):
+    import smoldyn_amd as sa
+    sim = sa.Simulation(spec["low"], spec["high"])
+    for d, w in enumerate(spec["walls"]):
+        sim.boundary(d, 0, w)
+        sim.boundary(d, 1, w)
+    sim.times(0.0, 1e30, spec["dt"])
+    sim.seed(spec["seed"])
+    for name, d in spec["species"].items():
+        sim.species(name)
+        sim.mobility(name, d)
+    for name, rct, prd, rate in spec["reactions"]:
+        sim.reaction(name, rct, prd, rate)
+    for name, (kind, param) in spec["placement"].items():
+        sim.product_placement(name, kind, param)
+    for sp, n in spec["molecules"]:
+        sim.add_molecules(sp, n)
+    sim.update()
+    return sim
+
+
+def cpu_baseline(spec, steps):
+    """The CPU oracle (a restatement of the reference CPU path, 1 thread) on the
+    same workload: setup excluded, `steps` steps timed."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import simspec as ss
+    osim, _ = ss.build_oracle(dict(spec, tmax=1e30))
+    nmol = sum(len(x["ident"]) for x in ss.oracle_state(osim))
+    t0 = time.perf_counter()
+    molsteps = 0
+    for _ in range(steps):
+        molsteps += nmol
+        osim.step()
+        nmol = sum(osim.L.osim_list_count(osim.p, ll) for ll in range(osim.nlist()))
+    dt = time.perf_counter() - t0
+    return molsteps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--n", type=int, default=500000, help="molecules per reactant species")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import smoldyn_amd as sa
+    spec = c2_spec(n_per_species=args.n)
+    t_setup = time.perf_counter()
+    sim = build_device(spec)
+    setup_s = time.perf_counter() - t_setup
+    use_graph = not args.no_graph
+    if args.warmup > 0:
+        sim.run_steps(args.warmup, graph=use_graph)
+    d0 = sim.diagnostics()
+    # timed region: all steps captured as HIP-graph replays on one stream; the
+    # device accumulates the live molecule count at the start of every step
+    t0 = time.perf_counter()
+    dev_ms = sim.run_steps(args.steps, graph=use_graph)
+    wall = time.perf_counter() - t0
+    d1 = sim.diagnostics()
+    if d1["err"]:
+        raise SystemExit("device engine error bits 0x%x" % d1["err"])
+    molsteps = d1["molsteps"] - d0["molsteps"]
+    value = molsteps / wall
+    # per-kernel device time with HIP events on the engine's stream
+    kt = sim.profile_steps(args.profile_steps) if args.profile_steps > 0 else np.zeros(6)
+    nmol = sim.diagnostics()["nmol"]
+    times = dict(zip(KERNEL_NAMES, [float(x) for x in kt]))
+    dom = max(ALGO_BYTES, key=lambda k: times[k])
+    achieved = ALGO_BYTES[dom] * nmol / (times[dom] * 1e-3) / 1e9 if times[dom] > 0 else 0.0
+    step_bytes = sum(ALGO_BYTES.values()) * nmol
+    counts = {s: sim.count(s) for s in ("A", "B", "C")}
+    out = {
+        "metric": "molecule-steps/s at 1M molecules",
+        "value": value,
+        "unit": "molecule-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "reversible A+B<->C, periodic cube 1000, 500k A + 500k B, dt 0.005, "
+                               "a~1.0, b/a 1.2, ~4 molecules per virtual box, FP64",
+                   "molecules": 2 * args.n, "graph": use_graph},
+        "device_ms_per_step": dev_ms / args.steps,
+        "kernel_ms": times,
+        "roofline": {"bound": "hbm", "kernel": HIP_KERNEL_SYMBOL[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "algo_bytes_per_molecule": ALGO_BYTES[dom], "molecules_per_launch": nmol,
+                     "traffic": None},
+        "step_algo_bytes": step_bytes,
+        "step_frac_of_hbm": step_bytes / (dev_ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "final_counts": counts,
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu:
+        cv, cdt = cpu_baseline(spec, args.cpu_steps)
+        out["cpu_baseline"] = {"value": cv, "unit": "molecule-steps/s", "cores": 1, "kind": "port",
+                               "sample": "same 1M-molecule workload, %d steps after setup (%.1f s), CPU oracle "
+                                         "(C restatement of the reference path), 1 thread" % (args.cpu_steps, cdt)}
+        out["speedup_vs_cpu"] = value / cv
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+/* libsmoldyn_amd: the MI355X (gfx950) engine for Smoldyn's per-timestep
+ * particle path behind the libsmoldyn C API.
+ *
+ * The functions below keep the names, argument meaning, enum values and error
+ * behaviour of the reference's libsmoldyn (source/Smoldyn/libsmoldyn.h), so a
+ * C/C++ caller that uses this subset links against libsmoldyn_amd.so instead
+ * of libsmoldyn.so.  simptr is an opaque handle here: the reference exposes
+ * its simstruct, but no caller on the particle path needs its fields (state is
+ * read back through smolGetMoleculeCount and the smolamd* inspection calls).
+ *
+ * Each declaration cites the reference declaration it replaces.
+ */
+#ifndef LIBSMOLDYN_AMD_H
+#define LIBSMOLDYN_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* enum values as source/Smoldyn/libsmoldyn.h:13-26 */
+enum ErrorCode {
+  ECok = 0,
+  ECnotify = -1,
+  ECwarning = -2,
+  ECnonexist = -3,
+  ECall = -4,
+  ECmissing = -5,
+  ECbounds = -6,
+  ECsyntax = -7,
+  ECerror = -8,
+  ECmemory = -9,
+  ECbug = -10,
+  ECsame = -11,
+  ECwildcard = -12
+};
+
+/* source/Smoldyn/smoldyn.h:251-261 */
+enum MolecState { MSsoln, MSfront, MSback, MSup, MSdown, MSbsoln, MSall, MSnone, MSsome };
+
+/* source/Smoldyn/smoldyn.h:358-373 */
+enum RevParam {
+  RPnone, RPirrev, RPconfspread, RPbounce, RPpgem, RPpgemmax, RPpgemmaxw, RPratio, RPunbindrad,
+  RPpgem2, RPpgemmax2, RPratio2, RPoffset, RPfixed
+};
+
+/* source/Smoldyn/smoldyn.h:484-500 */
+enum PanelFace { PFfront, PFback, PFnone, PFboth };
+enum PanelShape { PSrect, PStri, PSsph, PScyl, PShemi, PSdisk, PSall, PSnone };
+
+/* source/Smoldyn/smoldyn.h:503-516 */
+enum SrfAction { SAreflect, SAtrans, SAabsorb, SAjump, SAport, SAmult, SAno, SAnone, SAadsorb, SArevdes, SAirrevdes, SAflip };
+
+typedef struct smolamd_sim* simptr;
+
+/* ---------------- miscellaneous and errors (libsmoldyn.h:34-45) ---------------- */
+double smolGetVersion(void);                                                   /* libsmoldyn.h:34 */
+void smolSetError(const char* errorfunction, enum ErrorCode errorcode, const char* errorstring,
+                  const char* flags);                                          /* libsmoldyn.h:40 */
+enum ErrorCode smolGetError(char* errorfunction, char* errorstring, int clearerror); /* libsmoldyn.h:42 */
+void smolClearError(void);                                                     /* libsmoldyn.h:43 */
+void smolSetDebugMode(int debugmode);                                          /* libsmoldyn.h:44 */
+char* smolErrorCodeToString(enum ErrorCode erc, char* string);                 /* libsmoldyn.h:45 */
+
+/* ---------------- sim structure (libsmoldyn.h:49-55) ---------------- */
+simptr smolNewSim(int dim, double* lowbounds, double* highbounds);             /* libsmoldyn.h:49 */
+enum ErrorCode smolUpdateSim(simptr sim);                                      /* libsmoldyn.h:50 */
+enum ErrorCode smolRunTimeStep(simptr sim);                                    /* libsmoldyn.h:51 */
+enum ErrorCode smolRunSim(simptr sim);                                         /* libsmoldyn.h:52 */
+enum ErrorCode smolRunSimUntil(simptr sim, double breaktime);                  /* libsmoldyn.h:53 */
+enum ErrorCode smolFreeSim(simptr sim);                                        /* libsmoldyn.h:54 */
+
+/* ---------------- simulation settings (libsmoldyn.h:65-72) ---------------- */
+enum ErrorCode smolSetSimTimes(simptr sim, double timestart, double timestop, double timestep); /* :66 */
+enum ErrorCode smolSetTimeStart(simptr sim, double timestart);                 /* libsmoldyn.h:67 */
+enum ErrorCode smolSetTimeStop(simptr sim, double timestop);                   /* libsmoldyn.h:68 */
+enum ErrorCode smolSetTimeNow(simptr sim, double timenow);                     /* libsmoldyn.h:69 */
+enum ErrorCode smolSetTimeStep(simptr sim, double timestep);                   /* libsmoldyn.h:70 */
+enum ErrorCode smolSetRandomSeed(simptr sim, long int seed);                   /* libsmoldyn.h:71 */
+enum ErrorCode smolSetPartitions(simptr sim, const char* method, double value); /* libsmoldyn.h:72 */
+
+/* ---------------- molecules (libsmoldyn.h:99-118) ---------------- */
+enum ErrorCode smolAddSpecies(simptr sim, const char* species, const char* mollist); /* :99 */
+int smolGetSpeciesIndex(simptr sim, const char* species);                     /* libsmoldyn.h:100 */
+int smolGetSpeciesIndexNT(simptr sim, const char* species);                   /* libsmoldyn.h:101 */
+void smolGetSpeciesName(simptr sim, int speciesindex, char* species);         /* libsmoldyn.h:102 */
+enum ErrorCode smolSetSpeciesMobility(simptr sim, const char* species, enum MolecState state, double difc,
+                                      double* drift, double* difmatrix);      /* libsmoldyn.h:103 */
+enum ErrorCode smolAddMolList(simptr sim, const char* mollist);               /* libsmoldyn.h:108 */
+int smolGetMolListIndex(simptr sim, const char* mollist);                     /* libsmoldyn.h:109 */
+int smolGetMolListIndexNT(simptr sim, const char* mollist);                   /* libsmoldyn.h:110 */
+enum ErrorCode smolSetMolList(simptr sim, const char* species, enum MolecState state,
+                              const char* mollist);                           /* libsmoldyn.h:112 */
+enum ErrorCode smolSetMaxMolecules(simptr sim, int maxmolecules);             /* libsmoldyn.h:113 */
+enum ErrorCode smolAddSolutionMolecules(simptr sim, const char* species, int number, double* lowposition,
+                                        double* highposition);                /* libsmoldyn.h:114 */
+int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state); /* libsmoldyn.h:117 */
+
+/* ---------------- boundaries (libsmoldyn.h:122) ---------------- */
+enum ErrorCode smolSetBoundaryType(simptr sim, int dimension, int highside, char type); /* :122 */
+
+/* ---------------- reactions (libsmoldyn.h:152-158) ---------------- */
+enum ErrorCode smolAddReaction(simptr sim, const char* reaction, const char* reactant1, enum MolecState rstate1,
+                               const char* reactant2, enum MolecState rstate2, int nproduct,
+                               const char** productspecies, enum MolecState* productstates,
+                               double rate);                                  /* libsmoldyn.h:152 */
+int smolGetReactionIndex(simptr sim, int* orderptr, const char* reaction);    /* libsmoldyn.h:153 */
+int smolGetReactionIndexNT(simptr sim, int* orderptr, const char* reaction);  /* libsmoldyn.h:154 */
+enum ErrorCode smolSetReactionRate(simptr sim, const char* reaction, double rate, int type); /* :156 */
+enum ErrorCode smolSetReactionProducts(simptr sim, const char* reaction, enum RevParam method, double parameter,
+                                       const char* product, double* position); /* libsmoldyn.h:158 */
+
+/* ---------------- extensions: no counterpart in the reference ----------------
+ * Inspection and parity hooks used by callers that compare the device engine
+ * with a CPU path, and the engine's own controls. */
+
+/* Initial molecules at given positions (n*dim doubles), serial numbers in order. */
+enum ErrorCode smolamdAddMoleculesAt(simptr sim, const char* species, int number, const double* positions);
+/* Replace the diffusion draws of the next step by caller-supplied standard
+   normal deviates: 3 per diffusing molecule, in the reference's diffuse loop
+   order (lists in order, live order within a list).  Like swapping
+   sim->diffusefn in the reference (smolsim.cpp:325). */
+enum ErrorCode smolamdInjectDisplacements(simptr sim, const double* gauss, long nmol);
+/* Molecules of list `list` in live order; returns the count (or <0). Any of
+   the output pointers may be NULL. pos is n*3 doubles. box is the virtual box
+   each molecule was assigned to in the last step. */
+int smolamdGetMolecules(simptr sim, int list, int* species, double* pos, unsigned long long* serno, int* box,
+                        int maxmol);
+int smolamdGetListCount(simptr sim);
+/* Reaction events of the last step in the order the reference executes them:
+   rows of 6 long long: order, reaction index, reactant serials (2), product
+   serials (2). Returns the event count. */
+int smolamdGetReactionLog(simptr sim, long long* rows, int maxrows);
+/* Reaction parameters after smolUpdateSim: out[0..5] = bindrad2, prob,
+   unbindrad, prdpos[0][0], prdpos[1][0], rparamt. */
+enum ErrorCode smolamdGetReactionParams(simptr sim, int order, int reaction, double* out);
+/* Virtual-box grid: side[3], min[3], size[3]; returns the box count. */
+int smolamdGetBoxes(simptr sim, int* side, double* boxmin, double* boxsize);
+/* Run nsteps time steps (no stopping at time_stop) with the step captured in
+   a HIP graph when use_graph != 0; *ms receives the device time (HIP events). */
+enum ErrorCode smolamdRunSteps(simptr sim, long nsteps, int use_graph, double* ms);
+/* Per-kernel average device time (ms) over nsteps timed steps:
+   out[0..5] = diffuse, box scan, scatter, box sort, pair search, resolve. */
+enum ErrorCode smolamdProfileSteps(simptr sim, long nsteps, double* out);
+/* Engine diagnostics: out[0]=live molecules, [1]=edges max, [2]=matching rounds
+   of last step, [3]=error bits, [4]=next serial, [5]=step index,
+   [6]=molecule-steps run on the device so far. */
+enum ErrorCode smolamdGetDiagnostics(simptr sim, double* out);
+
+/* Reaction-parameter solver of the host setup (rxnparam.c of the reference). */
+double smolamdBindingRadius(double rate, double dt, double difc, double b, int rel);
+double smolamdNumRxnRate(double step, double a, double b);
+double smolamdUnbindingRadius(double pgem, double dt, double difc, double a);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,273 @@
+"""Python access to libsmoldyn_amd.so, the MI355X engine for Smoldyn's
+per-timestep particle path behind the libsmoldyn C API.
+
+The shared library is the product; this module only binds its C-ABI with
+ctypes (argument meaning and ErrorCode values as in the reference's
+source/Smoldyn/libsmoldyn.h).  It never falls back to a CPU path: loading
+fails loudly when the library is missing, and the first time step fails with
+the HIP runtime's error when no GPU is present.
+"""
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIBPATH = _HERE / "libsmoldyn_amd.so"
+
+# enum ErrorCode (libsmoldyn.h:13-26)
+ECok, ECnotify, ECwarning, ECnonexist, ECall, ECmissing, ECbounds, ECsyntax, ECerror, ECmemory, ECbug, ECsame, \
+    ECwildcard = 0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10, -11, -12
+# enum MolecState
+MSsoln, MSfront, MSback, MSup, MSdown, MSbsoln, MSall, MSnone, MSsome = range(9)
+# enum RevParam
+RP = {"none": 0, "irrev": 1, "confspread": 2, "bounce": 3, "pgem": 4, "pgemmax": 5, "pgemmaxw": 6, "ratio": 7,
+      "unbindrad": 8, "pgem2": 9, "pgemmax2": 10, "ratio2": 11, "offset": 12, "fixed": 13}
+
+_lib = None
+
+
+class SmoldynError(RuntimeError):
+    pass
+
+
+def build():
+    """Compile libsmoldyn_amd.so for gfx950 in-tree (hipcc cross-compiles without a GPU)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", str(_HERE / "csrc"), "-j4"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBPATH.exists():
+        raise OSError("libsmoldyn_amd.so is not built (run smoldyn_amd.build())")
+    L = C.CDLL(str(LIBPATH))
+    P, D, I, L64 = C.c_void_p, C.c_double, C.c_int, C.c_long
+    dp, ip, cp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_char_p
+    sig = {
+        "smolGetVersion": (D, []),
+        "smolGetError": (I, [cp, cp, I]),
+        "smolClearError": (None, []),
+        "smolSetDebugMode": (None, [I]),
+        "smolErrorCodeToString": (cp, [I, cp]),
+        "smolNewSim": (P, [I, dp, dp]),
+        "smolUpdateSim": (I, [P]),
+        "smolRunTimeStep": (I, [P]),
+        "smolRunSim": (I, [P]),
+        "smolRunSimUntil": (I, [P, D]),
+        "smolFreeSim": (I, [P]),
+        "smolSetSimTimes": (I, [P, D, D, D]),
+        "smolSetTimeStart": (I, [P, D]),
+        "smolSetTimeStop": (I, [P, D]),
+        "smolSetTimeNow": (I, [P, D]),
+        "smolSetTimeStep": (I, [P, D]),
+        "smolSetRandomSeed": (I, [P, L64]),
+        "smolSetPartitions": (I, [P, cp, D]),
+        "smolAddSpecies": (I, [P, cp, cp]),
+        "smolGetSpeciesIndex": (I, [P, cp]),
+        "smolGetSpeciesIndexNT": (I, [P, cp]),
+        "smolGetSpeciesName": (None, [P, I, cp]),
+        "smolSetSpeciesMobility": (I, [P, cp, I, D, dp, dp]),
+        "smolAddMolList": (I, [P, cp]),
+        "smolGetMolListIndex": (I, [P, cp]),
+        "smolGetMolListIndexNT": (I, [P, cp]),
+        "smolSetMolList": (I, [P, cp, I, cp]),
+        "smolSetMaxMolecules": (I, [P, I]),
+        "smolAddSolutionMolecules": (I, [P, cp, I, dp, dp]),
+        "smolGetMoleculeCount": (I, [P, cp, I]),
+        "smolSetBoundaryType": (I, [P, I, I, C.c_char]),
+        "smolAddReaction": (I, [P, cp, cp, I, cp, I, I, C.POINTER(cp), ip, D]),
+        "smolGetReactionIndex": (I, [P, ip, cp]),
+        "smolGetReactionIndexNT": (I, [P, ip, cp]),
+        "smolSetReactionRate": (I, [P, cp, D, I]),
+        "smolSetReactionProducts": (I, [P, cp, I, D, cp, dp]),
+        "smolamdAddMoleculesAt": (I, [P, cp, I, dp]),
+        "smolamdInjectDisplacements": (I, [P, dp, L64]),
+        "smolamdGetMolecules": (I, [P, I, ip, dp, C.POINTER(C.c_ulonglong), ip, I]),
+        "smolamdGetListCount": (I, [P]),
+        "smolamdGetReactionLog": (I, [P, C.POINTER(C.c_longlong), I]),
+        "smolamdGetReactionParams": (I, [P, I, I, dp]),
+        "smolamdGetBoxes": (I, [P, ip, dp, dp]),
+        "smolamdRunSteps": (I, [P, L64, I, dp]),
+        "smolamdProfileSteps": (I, [P, L64, dp]),
+        "smolamdGetDiagnostics": (I, [P, dp]),
+        "smolamdBindingRadius": (D, [D, D, D, D, I]),
+        "smolamdNumRxnRate": (D, [D, D, D]),
+        "smolamdUnbindingRadius": (D, [D, D, D, D]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    L.smolSetDebugMode(0)
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def last_error():
+    fn = C.create_string_buffer(512)
+    msg = C.create_string_buffer(512)
+    code = lib().smolGetError(fn, msg, 1)
+    return code, fn.value.decode(), msg.value.decode()
+
+
+def _check(code, what):
+    if code < ECwarning:
+        ecode, fn, msg = last_error()
+        raise SmoldynError("%s failed (%d): %s: %s" % (what, code, fn, msg))
+    return code
+
+
+class Simulation:
+    """One simulation behind the C-ABI, mirroring the libsmoldyn call sequence."""
+
+    def __init__(self, low, high):
+        self.L = lib()
+        lo = np.ascontiguousarray(low, np.float64)
+        hi = np.ascontiguousarray(high, np.float64)
+        self.dim = len(lo)
+        self.p = self.L.smolNewSim(self.dim, _dp(lo), _dp(hi))
+        if not self.p:
+            raise SmoldynError("smolNewSim: %s" % (last_error(),))
+
+    def close(self):
+        if getattr(self, "p", None):
+            self.L.smolFreeSim(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()
+
+    def boundary(self, d, high, t):
+        _check(self.L.smolSetBoundaryType(self.p, d, high, t.encode()), "smolSetBoundaryType")
+
+    def times(self, t0, t1, dt):
+        _check(self.L.smolSetSimTimes(self.p, t0, t1, dt), "smolSetSimTimes")
+
+    def seed(self, s):
+        _check(self.L.smolSetRandomSeed(self.p, s), "smolSetRandomSeed")
+
+    def partitions(self, method, value):
+        _check(self.L.smolSetPartitions(self.p, method.encode(), value), "smolSetPartitions")
+
+    def species(self, name, mollist=None):
+        _check(self.L.smolAddSpecies(self.p, name.encode(), mollist.encode() if mollist else None), "smolAddSpecies")
+        return self.L.smolGetSpeciesIndexNT(self.p, name.encode())
+
+    def mobility(self, name, difc, state=MSsoln):
+        _check(self.L.smolSetSpeciesMobility(self.p, name.encode(), state, difc, None, None), "smolSetSpeciesMobility")
+
+    def add_list(self, name):
+        _check(self.L.smolAddMolList(self.p, name.encode()), "smolAddMolList")
+
+    def set_list(self, species, mollist, state=MSall):
+        _check(self.L.smolSetMolList(self.p, species.encode(), state, mollist.encode()), "smolSetMolList")
+
+    def max_molecules(self, n):
+        _check(self.L.smolSetMaxMolecules(self.p, n), "smolSetMaxMolecules")
+
+    def add_molecules(self, species, n, lo=None, hi=None):
+        lo = None if lo is None else np.ascontiguousarray(lo, np.float64)
+        hi = None if hi is None else np.ascontiguousarray(hi, np.float64)
+        _check(self.L.smolAddSolutionMolecules(self.p, species.encode(), n, None if lo is None else _dp(lo),
+                                               None if hi is None else _dp(hi)), "smolAddSolutionMolecules")
+
+    def add_molecules_at(self, species, pos):
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        _check(self.L.smolamdAddMoleculesAt(self.p, species.encode(), len(pos), _dp(pos)), "smolamdAddMoleculesAt")
+
+    def reaction(self, name, reactants, products, rate):
+        r1 = reactants[0].encode() if len(reactants) > 0 else None
+        r2 = reactants[1].encode() if len(reactants) > 1 else None
+        arr = (C.c_char_p * max(1, len(products)))(*[p.encode() for p in products])
+        st = (C.c_int * max(1, len(products)))(*([MSsoln] * len(products)))
+        _check(self.L.smolAddReaction(self.p, name.encode(), r1, MSsoln, r2, MSsoln, len(products), arr, st, rate),
+               "smolAddReaction")
+
+    def product_placement(self, name, kind, param=0.0):
+        _check(self.L.smolSetReactionProducts(self.p, name.encode(), RP[kind], param, None, None),
+               "smolSetReactionProducts")
+
+    def reaction_index(self, name):
+        order = C.c_int(-1)
+        r = self.L.smolGetReactionIndexNT(self.p, C.byref(order), name.encode())
+        return order.value, r
+
+    def update(self):
+        _check(self.L.smolUpdateSim(self.p), "smolUpdateSim")
+
+    def run_time_step(self):
+        return _check(self.L.smolRunTimeStep(self.p), "smolRunTimeStep")
+
+    def run(self):
+        return _check(self.L.smolRunSim(self.p), "smolRunSim")
+
+    def run_until(self, t):
+        return _check(self.L.smolRunSimUntil(self.p, t), "smolRunSimUntil")
+
+    def inject_step(self, g):
+        g = np.ascontiguousarray(g, np.float64).reshape(-1, 3)
+        return _check(self.L.smolamdInjectDisplacements(self.p, _dp(g), len(g)), "smolamdInjectDisplacements")
+
+    def run_steps(self, n, graph=True):
+        ms = C.c_double(0)
+        _check(self.L.smolamdRunSteps(self.p, n, 1 if graph else 0, C.byref(ms)), "smolamdRunSteps")
+        return ms.value
+
+    def profile_steps(self, n):
+        out = np.zeros(6)
+        _check(self.L.smolamdProfileSteps(self.p, n, _dp(out)), "smolamdProfileSteps")
+        return out
+
+    def count(self, species=None, state=MSsoln):
+        n = self.L.smolGetMoleculeCount(self.p, species.encode() if species else None, state)
+        _check(n if n < 0 else 0, "smolGetMoleculeCount")
+        return n
+
+    def nlists(self):
+        return _check(self.L.smolamdGetListCount(self.p), "smolamdGetListCount")
+
+    def molecules(self, ll, maxmol=1 << 26):
+        n = self.L.smolamdGetMolecules(self.p, ll, None, None, None, None, 0)
+        _check(n if n < 0 else 0, "smolamdGetMolecules")
+        ident = np.zeros(n, np.int32)
+        pos = np.zeros((n, 3), np.float64)
+        ser = np.zeros(n, np.uint64)
+        box = np.zeros(n, np.int32)
+        self.L.smolamdGetMolecules(self.p, ll, ident.ctypes.data_as(C.POINTER(C.c_int)), _dp(pos),
+                                   ser.ctypes.data_as(C.POINTER(C.c_ulonglong)),
+                                   box.ctypes.data_as(C.POINTER(C.c_int)), n)
+        return dict(ident=ident, pos=pos, serno=ser, box=box)
+
+    def reaction_log(self):
+        n = self.L.smolamdGetReactionLog(self.p, None, 0)
+        rows = np.zeros((max(n, 1), 6), np.int64)
+        self.L.smolamdGetReactionLog(self.p, rows.ctypes.data_as(C.POINTER(C.c_longlong)), n)
+        return rows[:n]
+
+    def reaction_params(self, order, r):
+        out = np.zeros(6)
+        _check(self.L.smolamdGetReactionParams(self.p, order, r, _dp(out)), "smolamdGetReactionParams")
+        return dict(bindrad2=out[0], prob=out[1], unbindrad=out[2], prdpos0=out[3], prdpos1=out[4],
+                    rparamt=int(out[5]))
+
+    def boxes(self):
+        side = np.zeros(3, np.int32)
+        bmin = np.zeros(3)
+        bsize = np.zeros(3)
+        n = self.L.smolamdGetBoxes(self.p, side.ctypes.data_as(C.POINTER(C.c_int)), _dp(bmin), _dp(bsize))
+        _check(n if n < 0 else 0, "smolamdGetBoxes")
+        return n, side, bmin, bsize
+
+    def diagnostics(self):
+        out = np.zeros(8)
+        _check(self.L.smolamdGetDiagnostics(self.p, _dp(out)), "smolamdGetDiagnostics")
+        return dict(nmol=int(out[0]), maxedges=int(out[1]), rounds=int(out[2]), err=int(out[3]),
+                    serno_next=int(out[4]), step=int(out[5]), molsteps=int(out[6]))

@@ -1,0 +1,1396 @@
+// libsmoldyn-compatible C API over the MI355X particle engine.
+//
+// Host side of the drop-in boundary: simulation definition calls keep the
+// reference's names, argument meaning and ErrorCode behaviour
+// (source/Smoldyn/libsmoldyn.cpp), the setup reproduces simupdate
+// (source/Smoldyn/smolsim.cpp:2367-2473) for the supported feature set, and
+// every time step runs on the device (engine.hip).  There is no CPU fallback:
+// a simulation that uses a feature the device path does not implement is
+// rejected with ECerror at smolUpdateSim.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "../../include/libsmoldyn_amd.h"
+#include "../../include/smoldyn_amd_rng.h"
+#include "engine.h"
+#include "rxnparam.h"
+
+using namespace smolamd;
+
+namespace {
+
+constexpr int kMS = 5;  // MSMAX
+constexpr int kMS1 = 6; // MSMAX1
+constexpr int kStrLen = 512;
+
+// last-error state, as the reference keeps it in process globals
+// (libsmoldyn.cpp:33-38)
+ErrorCode g_errcode = ECok;
+ErrorCode g_warncode = ECok;
+char g_errfunc[kStrLen] = "";
+char g_errstr[kStrLen] = "";
+int g_debugmode = 1;
+
+struct Rxn {
+  std::string name;
+  int order = 0;
+  int rct[2] = {0, 0};
+  int rctstate[2] = {MSsoln, MSsoln};
+  int permit[kMS1 * kMS1] = {0};
+  int nprod = 0;
+  int prd[kMaxProd] = {0};
+  int prdstate[kMaxProd] = {0};
+  double rate = -1;
+  int multiplicity = -1;
+  double bindrad2 = -1, prob = -1, chi = -1;
+  int rparamt = RPnone;
+  double rparam = 0, unbindrad = -1;
+  double prdpos[kMaxProd][3] = {{0}};
+  int disable = 0;
+};
+
+struct PendingMol {
+  int ident;
+  double pos[3];
+  uint64_t serno;
+};
+
+}  // namespace
+
+struct smolamd_sim {
+  int dim = 3;
+  double low[3] = {0, 0, 0}, high[3] = {1, 1, 1};
+  char wtype[6] = {'t', 't', 't', 't', 't', 't'};
+  double tmin = 0, tmax = 100, dt = 0.01, time = 0, tbreak = 1e300;
+  uint32_t stepindex = 0;
+  uint64_t key = 0;
+  double mpbox = -1, boxsize = -1, accur = 10;
+  int maxmol = -1;
+  std::vector<std::string> spname{"empty"};
+  std::vector<std::array<double, kMS>> difc{{0, 0, 0, 0, 0}};
+  std::vector<std::array<int, kMS>> listlookup{{-1, -1, -1, -1, -1}};
+  std::vector<std::string> listname;
+  std::vector<PendingMol> pending;
+  uint64_t serno = 1;
+  std::vector<Rxn> rxn[3];
+  std::vector<std::vector<int>> table[3];  // [order][ident pack] -> reaction indices
+  bool ok = false;
+  bool started = false;
+  DeviceState* dev = nullptr;
+  std::vector<int> diffuselist;
+  Geometry geo;
+  DevTables tab;
+  std::vector<double> gtable;
+  int counts[kMaxSpecies] = {0};
+};
+
+namespace {
+
+void set_error(const char* func, ErrorCode code, const char* fmt, ...) {
+  char msg[kStrLen];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(msg, sizeof(msg), fmt, ap);
+  va_end(ap);
+  smolSetError(func, code, msg, "");
+}
+
+#define LCHECK(cond, func, code, ...)     \
+  do {                                    \
+    if (!(cond)) {                        \
+      set_error(func, code, __VA_ARGS__); \
+      if (code < ECwarning) return code;  \
+    }                                     \
+  } while (0)
+
+int species_index(simptr sim, const char* name) {
+  if (!name) return -1;
+  for (size_t i = 1; i < sim->spname.size(); i++)
+    if (sim->spname[i] == name) return (int)i;
+  return -1;
+}
+
+int list_index(simptr sim, const char* name) {
+  for (size_t l = 0; l < sim->listname.size(); l++)
+    if (sim->listname[l] == name) return (int)l;
+  return -1;
+}
+
+void invalidate(simptr sim) { sim->ok = false; }
+
+bool ismobile(simptr sim, int i, int ms) { return sim->difc[i][ms] > 0; }
+
+// RxnSetPermit (smolreact.c:1899-1922)
+void set_permit(Rxn& r) {
+  int nms = r.order == 1 ? kMS1 : kMS1 * kMS1;
+  for (int ms = 0; ms < nms; ms++) {
+    int msl[2];
+    if (r.order == 1) msl[0] = ms;
+    else {
+      msl[0] = ms / kMS1;
+      msl[1] = ms % kMS1;
+    }
+    bool set = true;
+    for (int o = 0; o < r.order && set; o++)
+      if (!(r.rctstate[o] == MSall || r.rctstate[o] == msl[o])) set = false;
+    if (set) r.permit[ms] = 1;
+  }
+  if (r.order == 2 && r.rct[0] == r.rct[1] && r.rctstate[0] != r.rctstate[1]) {
+    for (int ms = 0; ms < nms; ms++) {
+      int a = ms / kMS1, b = ms % kMS1;
+      if ((r.rctstate[1] == MSall || r.rctstate[1] == a) && (r.rctstate[0] == MSall || r.rctstate[0] == b))
+        r.permit[ms] = 1;
+    }
+  }
+}
+
+int pack(int order, const int* id) { return order == 1 ? id[0] : id[0] * kMaxSpecies + id[1]; }
+
+// findreverserxn (smolreact.c:345-381)
+int find_reverse(simptr sim, int order, int r, int* optr, int* rptr) {
+  Rxn& rx = sim->rxn[order][r];
+  int orderr = 0, rret = 0, rev = 0;
+  if (rx.nprod == 0 || rx.nprod >= 3 || sim->rxn[rx.nprod].empty()) {
+    rev = 0;
+  } else {
+    orderr = rx.nprod;
+    int identr = pack(orderr, rx.prd);
+    int mstater = orderr == 1 ? rx.prdstate[0] : rx.prdstate[0] * kMS1 + rx.prdstate[1];
+    const std::vector<int>& tr = sim->table[orderr][identr];
+    for (size_t j = 0; j < tr.size(); j++) {
+      int rr = tr[j];
+      Rxn& rxr = sim->rxn[orderr][rr];
+      if (!rxr.permit[mstater]) continue;
+      if (rev != 1 && rxr.nprod == order) {
+        bool same = order == 1 ? rx.rct[0] == rxr.prd[0]
+                               : ((rx.rct[0] == rxr.prd[0] && rx.rct[1] == rxr.prd[1]) ||
+                                  (rx.rct[0] == rxr.prd[1] && rx.rct[1] == rxr.prd[0]));
+        if (same) {
+          int identrprd = pack(order, rxr.prd);
+          int mstaterprd = order == 1 ? rxr.prdstate[0] : rxr.prdstate[0] * kMS1 + rxr.prdstate[1];
+          const std::vector<int>& tf = sim->table[order][identrprd];
+          for (size_t jr = 0; jr < tf.size(); jr++)
+            if (tf[jr] == r && sim->rxn[order][r].permit[mstaterprd]) {
+              rev = 1;
+              rret = rr;
+            }
+        }
+      }
+      if (!rev) {
+        rev = 2;
+        rret = rr;
+      }
+    }
+  }
+  *optr = orderr;
+  *rptr = rret;
+  return rev;
+}
+
+double difcsum(simptr sim, int i1, int ms1, int i2, int ms2) {
+  double sum = 0;
+  if (i1) sum += sim->difc[i1][ms1 >= kMS ? MSsoln : ms1];
+  if (i2) sum += sim->difc[i2][ms2 >= kMS ? MSsoln : ms2];
+  return sum;
+}
+
+// rxnreactantstate, order 2, converting bsoln to soln (smolreact.c:277-343)
+bool rctstate2(const Rxn& r, int* ms1, int* ms2) {
+  const int* p = r.permit;
+  if (p[MSsoln * kMS1 + MSsoln] || p[MSsoln * kMS1 + MSbsoln] || p[MSbsoln * kMS1 + MSsoln] ||
+      p[MSbsoln * kMS1 + MSbsoln]) {
+    *ms1 = *ms2 = MSsoln;
+    return true;
+  }
+  for (int a = 0; a < kMS1; a++)
+    if (p[a * kMS1 + MSsoln]) {
+      *ms1 = a == MSbsoln ? MSsoln : a;
+      *ms2 = MSsoln;
+      return true;
+    }
+  for (int b = 0; b < kMS1; b++)
+    if (p[MSsoln * kMS1 + b]) {
+      *ms1 = MSsoln;
+      *ms2 = b == MSbsoln ? MSsoln : b;
+      return true;
+    }
+  for (int mp = 0; mp < kMS1 * kMS1; mp++)
+    if (p[mp]) {
+      *ms1 = mp / kMS1;
+      *ms2 = mp % kMS1;
+      if (*ms1 == MSbsoln) *ms1 = MSsoln;
+      if (*ms2 == MSbsoln) *ms2 = MSsoln;
+      return true;
+    }
+  *ms1 = *ms2 = MSnone;
+  return false;
+}
+
+// rxnsetrate (smolreact.c:1306-1412)
+int set_rate(simptr sim, int order, int r) {
+  Rxn& rx = sim->rxn[order][r];
+  int o2, r2, rev;
+  if (rx.disable) {
+    rx.prob = 0;
+  } else if (rx.rparamt == RPconfspread) {
+    if (rx.rate < 0) return 1;
+    rx.prob = 1.0 - exp(-sim->dt * rx.rate * rx.multiplicity);
+  } else if (order == 1) {
+    int ms;
+    for (ms = 0; ms < kMS && !rx.permit[ms]; ms++) {
+    }
+    if (rx.rate < 0) return 1;
+    if (rx.rate == 0 || ms == kMS) {
+      rx.prob = 0;
+    } else {
+      const std::vector<int>& t = sim->table[1][rx.rct[0]];
+      double sum = 0;
+      for (int rr : t) {
+        Rxn& x2 = sim->rxn[1][rr];
+        if (x2.permit[ms] && x2.rate > 0) sum += x2.rate * x2.multiplicity;
+      }
+      rx.prob = rx.rate * rx.multiplicity / sum * (1.0 - exp(-sum * sim->dt));
+      double product = 1.0;
+      for (int rr : t) {
+        Rxn& x2 = sim->rxn[1][rr];
+        if (&x2 == &rx) break;
+        if (x2.permit[ms]) {
+          double prob = x2.prob * product;
+          product *= 1.0 - prob;
+        }
+      }
+      rx.prob /= product;
+      if (!(rx.prob >= 0 && rx.prob <= 1)) return 5;
+    }
+    rev = find_reverse(sim, 1, r, &o2, &r2);
+    if (rev > 0 && o2 == 2 && rx.rparamt == RPnone) {
+      rx.rparamt = RPpgemmaxw;
+      rx.rparam = 0.2;
+    }
+  } else {
+    if (rx.rate < 0) {
+      if (rx.prob < 0) rx.prob = 1;
+      return 1;
+    }
+    int ms1, ms2;
+    bool permit = rctstate2(rx, &ms1, &ms2);
+    double rate3 = rx.rate * rx.multiplicity;
+    if (rx.rct[0] == rx.rct[1]) rate3 *= 2;
+    if ((ms1 == MSsoln && ms2 != MSsoln) || (ms1 != MSsoln && ms2 == MSsoln)) rate3 *= 2;
+    double dsum = difcsum(sim, rx.rct[0], ms1, rx.rct[1], ms2);
+    rev = find_reverse(sim, 2, r, &o2, &r2);
+    if (rev > 0 && o2 == 2 && rx.rparamt == RPnone) {
+      rx.rparamt = RPpgemmaxw;
+      rx.rparam = 0.2;
+    }
+    int rparamt = RPnone;
+    double rparam = 0;
+    if (rev == 1) {
+      Rxn& rv = sim->rxn[o2][r2];
+      if (rv.rparamt == RPnone) {
+        rv.rparamt = RPpgemmaxw;
+        rv.rparam = 0.2;
+      }
+      rparamt = rv.rparamt;
+      rparam = rv.rparam;
+    }
+    if (rx.prob < 0) rx.prob = 1;
+    double dtt = sim->dt;
+    if (!permit) rx.bindrad2 = 0;
+    else if (rate3 <= 0) rx.bindrad2 = 0;
+    else if (dsum <= 0) return 4;
+    else if (rparamt == RPunbindrad)
+      rx.bindrad2 = rxnparam::bindingradiusprob(rate3, dtt, dsum, rparam, 0, rx.chi, &rx.prob);
+    else if (rparamt == RPratio)
+      rx.bindrad2 = rxnparam::bindingradiusprob(rate3, dtt, dsum, rparam, 1, rx.chi, &rx.prob);
+    else if (rparamt == RPpgem)
+      rx.bindrad2 = rxnparam::bindingradiusprob(rate3 * (1.0 - rparam), dtt, dsum, -1, 0, rx.chi, &rx.prob);
+    else if (rparamt == RPpgemmax || rparamt == RPpgemmaxw) {
+      rx.bindrad2 = rxnparam::bindingradiusprob(rate3, dtt, dsum, 0, 0, rx.chi, &rx.prob);
+      double ub = rxnparam::unbindingradius(rparam, dtt, dsum, rx.bindrad2);
+      if (ub > 0) rx.bindrad2 = rxnparam::bindingradiusprob(rate3 * (1.0 - rparam), dtt, dsum, -1, 0, rx.chi, &rx.prob);
+    } else
+      rx.bindrad2 = rxnparam::bindingradiusprob(rate3, dtt, dsum, -1, 0, rx.chi, &rx.prob);
+    rx.bindrad2 *= rx.bindrad2;
+  }
+  return 0;
+}
+
+// rxnsetproduct (smolreact.c:1431-1575), except bounce placement
+int set_product(simptr sim, int order, int r) {
+  Rxn& rx = sim->rxn[order][r];
+  int nprod = rx.nprod, rparamt = rx.rparamt, dim = sim->dim;
+  double rpar = rx.rparam;
+  if (nprod == 0) {
+    if (rparamt == RPnone || rparamt == RPirrev || rparamt == RPconfspread) rx.unbindrad = 0;
+    else return 1;
+  } else if (rparamt == RPoffset || rparamt == RPfixed) {
+    return 12;  // not supported on the device path
+  } else if (nprod == 1) {
+    if (rparamt == RPnone || rparamt == RPirrev || rparamt == RPconfspread) {
+      rx.unbindrad = 0;
+      for (int d = 0; d < dim; d++) rx.prdpos[0][d] = 0;
+    } else
+      return 2;
+  } else if (rparamt == RPbounce) {
+    return 11;
+  } else {
+    int ms1 = rx.prdstate[0] == MSbsoln ? MSsoln : rx.prdstate[0];
+    int ms2 = rx.prdstate[1] == MSbsoln ? MSsoln : rx.prdstate[1];
+    double dc1 = difcsum(sim, rx.prd[0], ms1, 0, MSnone), dc2 = difcsum(sim, rx.prd[1], ms2, 0, MSnone);
+    double dsum = dc1 + dc2;
+    if (dsum == 0) dc1 = dc2 = 0.5;
+    else {
+      dc1 /= dsum;
+      dc2 /= dsum;
+    }
+    int orderr, rr;
+    int rev = find_reverse(sim, order, r, &orderr, &rr);
+    if (rparamt == RPirrev || rparamt == RPconfspread) {
+      rx.unbindrad = 0;
+      for (int p = 0; p < nprod; p++)
+        for (int d = 0; d < dim; d++) rx.prdpos[p][d] = 0;
+    } else if (rev == 0) {
+      if (rparamt == RPpgem || rparamt == RPpgemmax || rparamt == RPpgemmaxw || rparamt == RPratio ||
+          rparamt == RPpgem2 || rparamt == RPpgemmax2 || rparamt == RPratio2)
+        return 3;
+      if (rparamt == RPunbindrad) {
+        rx.unbindrad = rpar;
+        rx.prdpos[0][0] = rpar * dc1;
+        rx.prdpos[1][0] = -rpar * dc2;
+      } else {
+        rx.unbindrad = 0;
+        rx.prdpos[0][0] = 0;
+        rx.prdpos[1][0] = 0;
+      }
+    } else {
+      Rxn& rxr = sim->rxn[orderr][rr];
+      double bindradr = rxr.bindrad2 >= 0 ? sqrt(rxr.bindrad2) : -1;
+      if (rparamt == RPnone) return 5;
+      if (rparamt == RPunbindrad) {
+        rx.unbindrad = rpar;
+        rx.prdpos[0][0] = rpar * dc1;
+        rx.prdpos[1][0] = -rpar * dc2;
+      } else if (rxr.bindrad2 < 0) {
+        return 6;
+      } else if (rxr.bindrad2 == 0) {
+        rx.unbindrad = 0;
+        rx.prdpos[0][0] = 0;
+        rx.prdpos[1][0] = 0;
+      } else if (rparamt == RPratio || rparamt == RPratio2) {
+        rx.unbindrad = rpar * bindradr;
+        rx.prdpos[0][0] = rpar * bindradr * dc1;
+        rx.prdpos[1][0] = -rpar * bindradr * dc2;
+      } else if (dsum <= 0) {
+        return 4;
+      } else if (rparamt == RPpgem || rparamt == RPpgem2) {
+        rpar = rxnparam::unbindingradius(rpar, sim->dt, dsum, bindradr);
+        if (rpar < 0) return 7;
+        rx.unbindrad = rpar;
+        rx.prdpos[0][0] = rpar * dc1;
+        rx.prdpos[1][0] = -rpar * dc2;
+      } else if (rparamt == RPpgemmax || rparamt == RPpgemmaxw || rparamt == RPpgemmax2) {
+        rpar = rxnparam::unbindingradius(rpar, sim->dt, dsum, bindradr);
+        if (rpar == -2) return 9;
+        if (rpar <= 0) rx.unbindrad = 0;
+        else {
+          rx.unbindrad = rpar;
+          rx.prdpos[0][0] = rpar * dc1;
+          rx.prdpos[1][0] = -rpar * dc2;
+        }
+      } else
+        return 10;
+    }
+  }
+  return 0;
+}
+
+// Gaussian quantile table (random2.c:339-353 with math2.c's erfn/inversefn):
+// the same construction as the reference, unshuffled because every index is
+// drawn from the counter stream.
+// the reference's gammalnD returns this constant for x == 0.5 (math2.c:268)
+constexpr double kLnGammaHalf = 0.572364942;
+
+double gammp_half(double x) {
+  const double a = 0.5;
+  double sum, del, ap, eps = 3e-7;
+  double gold = 0, g = 1, fac = 1, b1 = 1, b0 = 0, anf, ana, an, a1, a0 = 1;
+  if (x < 0) return -1;
+  if (x == 0) return 0;
+  if (x < a + 1) {
+    ap = a;
+    del = sum = 1.0 / a;
+    while (fabs(del) > fabs(sum) * eps && ap - a < 100) {
+      ap += 1;
+      del *= x / ap;
+      sum += del;
+    }
+    return sum * exp(-x + a * log(x) - kLnGammaHalf);
+  }
+  a1 = x;
+  for (an = 1; an < 100; an++) {
+    ana = an - a;
+    a0 = (a1 + a0 * ana) * fac;
+    b0 = (b1 + b0 * ana) * fac;
+    anf = an * fac;
+    a1 = x * a0 + anf * a1;
+    b1 = x * b0 + anf * b1;
+    if (a1) {
+      fac = 1.0 / a1;
+      g = b1 * fac;
+      if (fabs((g - gold) / g) < eps) return 1.0 - exp(-x + a * log(x) - kLnGammaHalf) * g;
+      gold = g;
+    }
+  }
+  return -1;
+}
+
+double erfn(double x) { return x < 0 ? -gammp_half(x * x) : gammp_half(x * x); }
+
+void gauss_table(std::vector<double>& a, int n) {
+  a.assign(n, 0.0);
+  double dy = 2.0 / n;
+  for (int i = 0; i < n / 2; i++) {
+    double y = (i + 0.5) * dy - 1.0;
+    double x1 = -20, x2 = 20, dx, y2;
+    if (erfn(x1) < erfn(x2)) dx = x2 - x1;
+    else {
+      dx = x1 - x2;
+      x1 = x2;
+    }
+    for (int k = 30; k > 0; k--) {
+      dx *= 0.5;
+      x2 = x1 + dx;
+      y2 = erfn(x2);
+      if (y2 < y) x1 = x2;
+    }
+    a[i] = 1.41421356237 * (x1 + 0.5 * dx);
+  }
+  for (int i = n / 2; i < n; i++) a[i] = -a[n - i - 1];
+}
+
+ErrorCode do_update(simptr sim, const char* fn) {
+  if (sim->ok) return ECok;
+  LCHECK(!sim->started, fn, ECerror, "the simulation definition changed after stepping started");
+  LCHECK(sim->dim == 3, fn, ECerror, "the device engine runs 3-D systems");
+  int nsp = (int)sim->spname.size();
+  LCHECK(nsp <= kMaxSpecies, fn, ECbounds, "too many species for the device tables (%d)", kMaxSpecies);
+  for (int w = 0; w < 6; w++)
+    LCHECK(sim->wtype[w] != 'a', fn, ECerror, "absorbing walls are not implemented on the device path");
+  // molsupdatelists (smolmolec.c:2587-2651)
+  std::vector<std::array<int, kMS>> lookup = sim->listlookup;
+  std::vector<std::string> lists = sim->listname;
+  if (lists.empty() && !sim->pending.empty() && nsp > 1) {
+    bool ndif = false, nfix = false;
+    for (int i = 1; i < nsp; i++)
+      for (int ms = 0; ms < kMS; ms++) (ismobile(sim, i, ms) ? ndif : nfix) = true;
+    if (ndif) {
+      lists.push_back("diffuselist");
+      for (int i = 1; i < nsp; i++)
+        for (int ms = 0; ms < kMS; ms++)
+          if (ismobile(sim, i, ms)) lookup[i][ms] = (int)lists.size() - 1;
+    }
+    if (nfix) {
+      lists.push_back("fixedlist");
+      for (int i = 1; i < nsp; i++)
+        for (int ms = 0; ms < kMS; ms++)
+          if (!ismobile(sim, i, ms)) lookup[i][ms] = (int)lists.size() - 1;
+    }
+  }
+  bool unset = false;
+  for (int i = 1; i < nsp; i++)
+    for (int ms = 0; ms < kMS; ms++)
+      if (lookup[i][ms] < 0) unset = true;
+  if (unset) {
+    int ull = -1;
+    for (size_t l = 0; l < lists.size(); l++)
+      if (lists[l] == "unassignedlist") ull = (int)l;
+    if (ull < 0) {
+      lists.push_back("unassignedlist");
+      ull = (int)lists.size() - 1;
+    }
+    for (int i = 1; i < nsp; i++)
+      for (int ms = 0; ms < kMS; ms++)
+        if (lookup[i][ms] < 0) lookup[i][ms] = ull;
+  }
+  int nlist = (int)lists.size();
+  LCHECK(nlist >= 1 && nlist <= kMaxList, fn, ECbounds, "molecule list count %d outside 1..%d", nlist, kMaxList);
+  sim->listname = lists;
+  sim->listlookup = lookup;
+  sim->diffuselist.assign(nlist, 0);
+  for (int i = 0; i < nsp; i++)
+    for (int ms = 0; ms < kMS; ms++)
+      if (ismobile(sim, i, ms) && lookup[i][ms] >= 0) sim->diffuselist[lookup[i][ms]] = 1;
+  gauss_table(sim->gtable, 4096);
+
+  // boxesupdatelists (smolboxes.c:717-819)
+  Geometry& g = sim->geo;
+  memset(&g, 0, sizeof(g));
+  double mpbox = sim->mpbox;
+  if (mpbox < 0 && sim->boxsize < 0) mpbox = 4;
+  if (mpbox < 0) mpbox = 0;
+  if (mpbox <= 0 && sim->boxsize <= 0) mpbox = 5;
+  double flt1;
+  if (mpbox > 0) {
+    double vol = 1.0;
+    for (int d = 0; d < 3; d++) vol *= sim->high[d] - sim->low[d];
+    flt1 = pow((double)sim->pending.size() / mpbox / vol, 1.0 / 3);
+  } else {
+    flt1 = 1.0 / sim->boxsize;
+  }
+  long nbox = 1;
+  for (int d = 0; d < 3; d++) {
+    g.low[d] = sim->low[d];
+    g.high[d] = sim->high[d];
+    g.width[d] = sim->high[d] - sim->low[d];
+    g.bmin[d] = sim->low[d];
+    g.side[d] = (int)ceil((sim->high[d] - sim->low[d]) * flt1);
+    if (!g.side[d]) g.side[d] = 1;
+    g.bsize[d] = (sim->high[d] - sim->low[d]) / g.side[d];
+    nbox *= g.side[d];
+  }
+  LCHECK(nbox < (1L << 26), fn, ECbounds, "virtual box grid too large (%ld boxes)", nbox);
+  g.nbox = (int)nbox;
+  for (int w = 0; w < 6; w++)
+    g.wall[w] = sim->wtype[w] == 'r' ? kWallReflect : sim->wtype[w] == 'p' ? kWallPeriodic : kWallTransparent;
+  // neighborZV type 7 reads the first dim entries of the per-wall periodic
+  // flags as per-dimension flags (Zn.c:263, smolboxes.c:772-776)
+  LCHECK(sim->accur >= 9, fn, ECerror, "accuracy below 9 selects neighbor lists the device path does not build");
+  for (int d = 0; d < 3; d++) g.wrapflag[d] = (sim->wtype[d] == 'p');
+
+  // reactions (rxnsupdate, smolreact.c:2384-2401)
+  for (int order = 1; order <= 2; order++)
+    for (size_t r = 0; r < sim->rxn[order].size(); r++) {
+      int er = set_rate(sim, order, (int)r);
+      LCHECK(er <= 1, fn, ECerror, "error setting rate for reaction %s (code %d)", sim->rxn[order][r].name.c_str(), er);
+    }
+  for (int order = 1; order <= 2; order++)
+    for (size_t r = 0; r < sim->rxn[order].size(); r++) {
+      int er = set_product(sim, order, (int)r);
+      LCHECK(er == 0, fn, ECerror, "product placement of reaction %s is not supported here (code %d)",
+             sim->rxn[order][r].name.c_str(), er);
+    }
+  LCHECK(sim->rxn[0].empty(), fn, ECerror, "zeroth-order reactions are not implemented on the device path");
+
+  // device tables
+  DevTables& t = sim->tab;
+  memset(&t, 0, sizeof(t));
+  t.nspecies = nsp;
+  t.nlist = nlist;
+  t.dim = 3;
+  t.ngauss = 4096;
+  for (int i = 0; i < nsp; i++) {
+    t.difc[i] = sim->difc[i][MSsoln];
+    t.difstep[i] = sqrt(2.0 * sim->difc[i][MSsoln] * sim->dt);
+    t.listof[i] = i ? lookup[i][MSsoln] : 0;
+  }
+  for (int l = 0; l < nlist; l++) t.diffuselist[l] = sim->diffuselist[l];
+  for (int i = 1; i < nsp; i++) {
+    const std::vector<int>& tb = sim->table[1].size() > (size_t)i ? sim->table[1][i] : std::vector<int>();
+    LCHECK((int)tb.size() <= kMaxRxnPerSet, fn, ECbounds, "too many first-order reactions for species %d", i);
+    t.nuni[i] = (int)tb.size();
+    for (size_t j = 0; j < tb.size(); j++) t.uni[i][j] = tb[j];
+  }
+  for (int i1 = 1; i1 < nsp; i1++)
+    for (int i2 = 1; i2 < nsp; i2++) {
+      int k = i1 * kMaxSpecies + i2;
+      const std::vector<int>& tb = sim->table[2].size() > (size_t)k ? sim->table[2][k] : std::vector<int>();
+      LCHECK((int)tb.size() <= kMaxRxnPerSet, fn, ECbounds, "too many reactions for a species pair");
+      t.nbi[k] = (int)tb.size();
+      for (size_t j = 0; j < tb.size(); j++) t.bi[k][j] = tb[j];
+    }
+  LCHECK(sim->rxn[1].size() <= (size_t)kMaxRxn && sim->rxn[2].size() <= (size_t)kMaxRxn, fn, ECbounds,
+         "too many reactions");
+  for (int order = 1; order <= 2; order++)
+    for (size_t r = 0; r < sim->rxn[order].size(); r++) {
+      Rxn& rx = sim->rxn[order][r];
+      DevRxn& dr = order == 1 ? t.rxn1[r] : t.rxn2[r];
+      dr.order = order;
+      dr.rct0 = rx.rct[0];
+      dr.rct1 = order == 2 ? rx.rct[1] : 0;
+      dr.nprod = rx.nprod;
+      dr.bindrad2 = rx.bindrad2;
+      dr.prob = rx.prob;
+      dr.confspread = rx.rparamt == RPconfspread;
+      LCHECK(!dr.confspread, fn, ECerror, "conformational spread reactions are not implemented on the device path");
+      dr.permit_soln = order == 1 ? rx.permit[MSsoln] : rx.permit[MSsoln * kMS1 + MSsoln];
+      for (int p = 0; p < rx.nprod; p++) {
+        dr.prd[p] = rx.prd[p];
+        LCHECK(rx.prdstate[p] == MSsoln || rx.prdstate[p] == MSbsoln, fn, ECerror,
+               "surface-bound products need the surface path");
+        dr.prdlist[p] = lookup[rx.prd[p]][MSsoln];
+        dr.prdpos[p] = rx.prdpos[p][0];
+        if (rx.prdpos[p][0] != 0) dr.rotate = 1;
+        LCHECK(rx.prdpos[p][1] == 0 && rx.prdpos[p][2] == 0, fn, ECerror, "off-axis product offsets");
+      }
+    }
+  // rxnsupdatelists order 2 (smolreact.c:2352-2377) and the (ll1,ll2) loop rank
+  for (size_t r = 0; r < sim->rxn[2].size(); r++) {
+    Rxn& rx = sim->rxn[2][r];
+    for (int ms1 = 0; ms1 < kMS1; ms1++)
+      for (int ms2 = 0; ms2 < kMS1; ms2++)
+        if (rx.permit[ms1 * kMS1 + ms2] && rx.prob != 0 && (rx.rate > 0 || rx.bindrad2 > 0)) {
+          int l1 = lookup[rx.rct[0]][ms1 == MSbsoln ? MSsoln : ms1];
+          int l2 = lookup[rx.rct[1]][ms2 == MSbsoln ? MSsoln : ms2];
+          t.rxnmol2[l1 * kMaxList + l2] = 1;
+          t.rxnmol2[l2 * kMaxList + l1] = 1;
+        }
+  }
+  {
+    int rank = 0;
+    for (int l1 = 0; l1 < nlist; l1++)
+      for (int l2 = l1; l2 < nlist; l2++) t.pairindex[l1 * kMaxList + l2] = rank++;
+    LCHECK(rank <= 16, fn, ECbounds, "too many molecule lists for the pair key");
+    for (int l1 = 0; l1 < nlist; l1++)
+      for (int l2 = l1; l2 < nlist; l2++)
+        LCHECK(!t.rxnmol2[l1 * kMaxList + l2] || (t.diffuselist[l1] && t.diffuselist[l2]), fn, ECerror,
+               "bimolecular reactions with an immobile list are not implemented on the device path");
+  }
+  // initial live order: molsort appends the molecules in the order they sit
+  // in the dead list's resurrected region, i.e. newest first
+  std::vector<std::vector<int>> ident(nlist);
+  std::vector<std::vector<double>> pos(nlist);
+  std::vector<std::vector<uint64_t>> ser(nlist);
+  memset(sim->counts, 0, sizeof(sim->counts));
+  for (size_t k = sim->pending.size(); k-- > 0;) {
+    PendingMol& pm = sim->pending[k];
+    int l = lookup[pm.ident][MSsoln];
+    ident[l].push_back(pm.ident);
+    pos[l].insert(pos[l].end(), pm.pos, pm.pos + 3);
+    ser[l].push_back(pm.serno);
+    sim->counts[pm.ident]++;
+  }
+  long ntot = (long)sim->pending.size();
+  EngineConfig cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.nlist = nlist;
+  long capl = sim->maxmol > 0 ? sim->maxmol : ntot + (ntot / 4 > 4096 ? ntot / 4 : 4096);
+  LCHECK(capl * nlist < (1L << 27), fn, ECbounds, "too many molecules for the device key layout");
+  for (int l = 0; l < nlist; l++) cfg.cap[l] = (int)capl;
+  long captot = capl * nlist;
+  cfg.maxedges = (int)(captot / 2 > 65536 ? captot / 2 : 65536);
+  cfg.maxevents = (int)(captot / 16 > 16384 ? captot / 16 : 16384);
+  if (sim->dev) engine_destroy(sim->dev);
+  char err[256] = "";
+  sim->dev = engine_create(cfg, t, g, sim->gtable.data(), 4096, (uint32_t)sim->key, (uint32_t)(sim->key >> 32), err,
+                           sizeof(err));
+  LCHECK(sim->dev, fn, ECmemory, "device engine: %s", err);
+  for (int l = 0; l < nlist; l++) {
+    int er = engine_upload_list(sim->dev, l, (int)ident[l].size(), ident[l].data(), pos[l].data(), ser[l].data());
+    LCHECK(er == 0, fn, ECmemory, "upload of list %d failed", l);
+  }
+  engine_set_counters(sim->dev, sim->serno, sim->stepindex, sim->counts);
+  sim->ok = true;
+  return ECok;
+}
+
+ErrorCode engine_error(const char* fn, int bits) {
+  if (bits == 0) return ECok;
+  if (bits == -2) {
+    set_error(fn, ECbounds, "fewer injected displacements than diffusing molecules");
+    return ECbounds;
+  }
+  if (bits < 0) {
+    set_error(fn, ECerror, "HIP runtime error");
+    return ECerror;
+  }
+  set_error(fn, ECerror, "device engine overflow (bits 0x%x: 1 edges, 2 events, 4 molecule capacity, 8 sort)", bits);
+  return ECerror;
+}
+
+// one simulatetimestep; returns the reference's code (0, 1 at tmax, 10 at tbreak) or <0
+int run_one(simptr sim, const double* inject, long ninject, const char* fn, ErrorCode* ec) {
+  *ec = do_update(sim, fn);
+  if (*ec != ECok) return -1;
+  sim->started = true;
+  int bits = engine_step(sim->dev, inject, ninject);
+  if (bits) {
+    *ec = engine_error(fn, bits);
+    return -1;
+  }
+  sim->time += sim->dt;
+  sim->stepindex++;
+  if (sim->time >= sim->tmax) return 1;
+  if (sim->time >= sim->tbreak) return 10;
+  return 0;
+}
+
+// run until time reaches tmax or tbreak, as smolsimulate does, in batches of
+// device steps between host checks
+ErrorCode run_until(simptr sim, const char* fn) {
+  ErrorCode ec = do_update(sim, fn);
+  if (ec != ECok) return ec;
+  sim->started = true;
+  for (;;) {
+    // count the steps the reference would take: time += dt until a stop
+    long n = 0;
+    double tt = sim->time;
+    int code = 0;
+    while (n < 256) {
+      tt += sim->dt;
+      n++;
+      if (tt >= sim->tmax) {
+        code = 1;
+        break;
+      }
+      if (tt >= sim->tbreak) {
+        code = 10;
+        break;
+      }
+    }
+    int bits = engine_run(sim->dev, n, 1);
+    if (bits) return engine_error(fn, bits);
+    sim->time = tt;
+    sim->stepindex += (uint32_t)n;
+    if (code == 1) {
+      set_error(fn, ECnotify, "Simulation complete");
+      return ECok;
+    }
+    if (code == 10) return ECok;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// exported API
+
+extern "C" {
+
+double smolGetVersion(void) { return 2.76; }
+
+void smolSetError(const char* errorfunction, enum ErrorCode errorcode, const char* errorstring, const char* flags) {
+  (void)flags;
+  if (errorcode != ECsame) {
+    g_errcode = errorcode;
+    g_warncode = errorcode >= ECwarning ? errorcode : ECok;
+    if (errorstring) {
+      strncpy(g_errstr, errorstring, kStrLen - 1);
+      g_errstr[kStrLen - 1] = 0;
+    } else
+      g_errstr[0] = 0;
+  }
+  if (errorfunction) {
+    strncpy(g_errfunc, errorfunction, kStrLen - 1);
+    g_errfunc[kStrLen - 1] = 0;
+  } else
+    g_errfunc[0] = 0;
+  if (g_debugmode && g_errfunc[0] && g_errcode < ECwarning) {
+    char s[64];
+    fprintf(stderr, "Libsmoldyn '%s' error in %s: %s\n", smolErrorCodeToString(g_errcode, s), g_errfunc, g_errstr);
+  }
+}
+
+enum ErrorCode smolGetError(char* errorfunction, char* errorstring, int clearerror) {
+  ErrorCode erc = g_errcode;
+  if (errorfunction) strcpy(errorfunction, g_errfunc);
+  if (errorstring) strcpy(errorstring, g_errstr);
+  if (clearerror) smolClearError();
+  return erc;
+}
+
+void smolClearError(void) {
+  g_errcode = ECok;
+  g_warncode = ECok;
+  g_errfunc[0] = 0;
+  g_errstr[0] = 0;
+}
+
+void smolSetDebugMode(int debugmode) { g_debugmode = debugmode; }
+
+char* smolErrorCodeToString(enum ErrorCode erc, char* string) {
+  static const char* names[] = {"ok", "notify", "warning", "nonexistent", "all", "missing", "bounds",
+                                "syntax", "error", "memory", "bug", "same", "wildcard"};
+  int k = -(int)erc;
+  strcpy(string, (k >= 0 && k <= 12) ? names[k] : "undefined");
+  return string;
+}
+
+simptr smolNewSim(int dim, double* lowbounds, double* highbounds) {
+  const char* fn = "smolNewSim";
+  if (dim <= 0 || dim > 3) {
+    set_error(fn, ECbounds, "dim must be between 1 and 3");
+    return nullptr;
+  }
+  if (!lowbounds || !highbounds) {
+    set_error(fn, ECmissing, "missing bounds");
+    return nullptr;
+  }
+  for (int d = 0; d < dim; d++)
+    if (!(lowbounds[d] < highbounds[d])) {
+      set_error(fn, ECbounds, "lowbounds must be < highbounds");
+      return nullptr;
+    }
+  simptr sim = new smolamd_sim();
+  sim->dim = dim;
+  for (int d = 0; d < dim; d++) {
+    sim->low[d] = lowbounds[d];
+    sim->high[d] = highbounds[d];
+  }
+  return sim;
+}
+
+enum ErrorCode smolUpdateSim(simptr sim) {
+  const char* fn = "smolUpdateSim";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  return do_update(sim, fn);
+}
+
+enum ErrorCode smolRunTimeStep(simptr sim) {
+  const char* fn = "smolRunTimeStep";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  ErrorCode ec;
+  int code = run_one(sim, nullptr, 0, fn, &ec);
+  if (code < 0) return ec;
+  LCHECK(code != 1, fn, ECnotify, "Simulation complete");
+  return g_warncode;
+}
+
+enum ErrorCode smolRunSim(simptr sim) {
+  const char* fn = "smolRunSim";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  ErrorCode ec = run_until(sim, fn);
+  if (ec < ECwarning) return ec;
+  if (g_warncode == ECnotify) g_warncode = ECok;
+  return g_warncode;
+}
+
+enum ErrorCode smolRunSimUntil(simptr sim, double breaktime) {
+  const char* fn = "smolRunSimUntil";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  double keep = sim->tbreak;
+  sim->tbreak = breaktime;
+  ErrorCode ec = run_until(sim, fn);
+  sim->tbreak = keep;
+  if (ec < ECwarning) return ec;
+  return g_warncode;
+}
+
+enum ErrorCode smolFreeSim(simptr sim) {
+  if (sim) {
+    if (sim->dev) engine_destroy(sim->dev);
+    delete sim;
+  }
+  return ECok;
+}
+
+enum ErrorCode smolSetSimTimes(simptr sim, double timestart, double timestop, double timestep) {
+  const char* fn = "smolSetSimTimes";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(timestep > 0, fn, ECbounds, "timestep must be >0");
+  sim->tmin = sim->time = timestart;
+  sim->tmax = timestop;
+  sim->dt = timestep;
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolSetTimeStart(simptr sim, double timestart) {
+  const char* fn = "smolSetTimeStart";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  sim->tmin = timestart;
+  return ECok;
+}
+
+enum ErrorCode smolSetTimeStop(simptr sim, double timestop) {
+  const char* fn = "smolSetTimeStop";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  sim->tmax = timestop;
+  return ECok;
+}
+
+enum ErrorCode smolSetTimeNow(simptr sim, double timenow) {
+  const char* fn = "smolSetTimeNow";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  sim->time = timenow;
+  return ECok;
+}
+
+enum ErrorCode smolSetTimeStep(simptr sim, double timestep) {
+  const char* fn = "smolSetTimeStep";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(timestep > 0, fn, ECbounds, "timestep must be >0");
+  LCHECK(!sim->started, fn, ECerror, "the time step is fixed once stepping started");
+  sim->dt = timestep;
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolSetRandomSeed(simptr sim, long int seed) {
+  const char* fn = "smolSetRandomSeed";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(!sim->started, fn, ECerror, "the seed is fixed once stepping started");
+  sim->key = (uint64_t)seed;
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolSetPartitions(simptr sim, const char* method, double value) {
+  const char* fn = "smolSetPartitions";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(method, fn, ECmissing, "missing method");
+  LCHECK(value > 0, fn, ECbounds, "value must be >0");
+  if (!strcmp(method, "molperbox")) sim->mpbox = value;
+  else if (!strcmp(method, "boxsize")) sim->boxsize = value;
+  else LCHECK(0, fn, ECsyntax, "method not recognized");
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolAddSpecies(simptr sim, const char* species, const char* mollist) {
+  const char* fn = "smolAddSpecies";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(species && species[0], fn, ECmissing, "missing species");
+  LCHECK(species_index(sim, species) < 0, fn, ECerror, "species is already in system");
+  LCHECK(strcmp(species, "all") != 0 && strcmp(species, "empty") != 0, fn, ECsyntax, "species name not allowed");
+  LCHECK((int)sim->spname.size() < kMaxSpecies, fn, ECbounds, "too many species");
+  int ll = -1;
+  if (mollist && mollist[0]) {
+    ll = list_index(sim, mollist);
+    LCHECK(ll >= 0, fn, ECnonexist, "molecule list not found");
+  }
+  sim->spname.push_back(species);
+  sim->difc.push_back({0, 0, 0, 0, 0});
+  sim->listlookup.push_back({ll, ll, ll, ll, ll});
+  invalidate(sim);
+  return ECok;
+}
+
+int smolGetSpeciesIndexNT(simptr sim, const char* species) {
+  if (!sim || !species) return (int)ECmissing;
+  if (!strcmp(species, "all")) return (int)ECall;
+  int i = species_index(sim, species);
+  return i < 0 ? (int)ECnonexist : i;
+}
+
+int smolGetSpeciesIndex(simptr sim, const char* species) {
+  int i = smolGetSpeciesIndexNT(sim, species);
+  if (i < 0) smolSetError("smolGetSpeciesIndex", (ErrorCode)i, "species not found", "");
+  return i;
+}
+
+void smolGetSpeciesName(simptr sim, int speciesindex, char* species) {
+  if (!sim || !species) return;
+  if (speciesindex < 0 || speciesindex >= (int)sim->spname.size()) {
+    species[0] = 0;
+    smolSetError("smolGetSpeciesName", ECbounds, "invalid species index", "");
+    return;
+  }
+  strcpy(species, sim->spname[speciesindex].c_str());
+}
+
+enum ErrorCode smolSetSpeciesMobility(simptr sim, const char* species, enum MolecState state, double difc,
+                                      double* drift, double* difmatrix) {
+  const char* fn = "smolSetSpeciesMobility";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(difc >= 0, fn, ECbounds, "negative diffusion coefficient");
+  LCHECK(!drift && !difmatrix, fn, ECerror, "drift and anisotropic diffusion are not implemented on the device path");
+  LCHECK(!sim->started, fn, ECerror, "mobility is fixed once stepping started");
+  std::vector<int> idx;
+  if (species && !strcmp(species, "all")) {
+    for (size_t i = 1; i < sim->spname.size(); i++) idx.push_back((int)i);
+  } else {
+    int i = species_index(sim, species);
+    LCHECK(i > 0, fn, ECnonexist, "species not found");
+    idx.push_back(i);
+  }
+  for (int i : idx) {
+    if (state == MSall)
+      for (int ms = 0; ms < kMS; ms++) sim->difc[i][ms] = difc;
+    else if (state == MSbsoln || state == MSsoln)
+      sim->difc[i][MSsoln] = difc;
+    else if (state < kMS)
+      sim->difc[i][state] = difc;
+    else
+      LCHECK(0, fn, ECbounds, "invalid state");
+  }
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolAddMolList(simptr sim, const char* mollist) {
+  const char* fn = "smolAddMolList";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(mollist && mollist[0], fn, ECmissing, "missing mollist");
+  LCHECK(list_index(sim, mollist) < 0, fn, ECerror, "molecule list already exists");
+  LCHECK((int)sim->listname.size() < kMaxList, fn, ECbounds, "too many molecule lists");
+  sim->listname.push_back(mollist);
+  invalidate(sim);
+  return ECok;
+}
+
+int smolGetMolListIndexNT(simptr sim, const char* mollist) {
+  if (!sim || !mollist) return (int)ECmissing;
+  int l = list_index(sim, mollist);
+  return l < 0 ? (int)ECnonexist : l;
+}
+
+int smolGetMolListIndex(simptr sim, const char* mollist) {
+  int l = smolGetMolListIndexNT(sim, mollist);
+  if (l < 0) smolSetError("smolGetMolListIndex", (ErrorCode)l, "list not found", "");
+  return l;
+}
+
+enum ErrorCode smolSetMolList(simptr sim, const char* species, enum MolecState state, const char* mollist) {
+  const char* fn = "smolSetMolList";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  int i = species_index(sim, species);
+  LCHECK(i > 0, fn, ECnonexist, "species not found");
+  int ll = list_index(sim, mollist);
+  LCHECK(ll >= 0, fn, ECnonexist, "molecule list not found");
+  if (state == MSall)
+    for (int ms = 0; ms < kMS; ms++) sim->listlookup[i][ms] = ll;
+  else
+    sim->listlookup[i][state == MSbsoln ? MSsoln : state] = ll;
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolSetMaxMolecules(simptr sim, int maxmolecules) {
+  const char* fn = "smolSetMaxMolecules";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(maxmolecules > 0, fn, ECbounds, "maxmolecules must be >0");
+  sim->maxmol = maxmolecules;
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolAddSolutionMolecules(simptr sim, const char* species, int number, double* lowposition,
+                                        double* highposition) {
+  const char* fn = "smolAddSolutionMolecules";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(!sim->started, fn, ECerror, "molecules can only be added before stepping starts");
+  int i = species_index(sim, species);
+  LCHECK(i > 0, fn, ECnonexist, "species not found");
+  LCHECK(number >= 0, fn, ECbounds, "number cannot be negative");
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = lowposition ? lowposition[d] : sim->low[d];
+    hi[d] = highposition ? highposition[d] : sim->high[d];
+  }
+  // placement in the counter stream: unirandOOD per coordinate, keyed by the
+  // new molecule's serial number (addmol, smolmolec.c:2717-2740)
+  for (int k = 0; k < number; k++) {
+    PendingMol pm;
+    pm.ident = i;
+    pm.serno = sim->serno++;
+    smolrng4 r = smolrng_mol((uint32_t)sim->key, (uint32_t)(sim->key >> 32), pm.serno, 0, SMOLRNG_PLACE, 0);
+    for (int d = 0; d < 3; d++) pm.pos[d] = smolrng_u01open(r.v[d]) * (hi[d] - lo[d]) + lo[d];
+    sim->pending.push_back(pm);
+  }
+  invalidate(sim);
+  return ECok;
+}
+
+int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state) {
+  const char* fn = "smolGetMoleculeCount";
+  if (!sim) {
+    set_error(fn, ECmissing, "missing sim");
+    return (int)ECmissing;
+  }
+  if (state != MSsoln && state != MSall) return 0;  // the device path holds solution molecules only
+  int counts[kMaxSpecies] = {0};
+  if (sim->dev && sim->ok) {
+    Control c;
+    if (engine_sync_control(sim->dev, &c)) {
+      set_error(fn, ECerror, "HIP runtime error");
+      return (int)ECerror;
+    }
+    memcpy(counts, c.count, sizeof(counts));
+  } else {
+    for (auto& pm : sim->pending) counts[pm.ident]++;
+  }
+  if (!species || !strcmp(species, "all")) {
+    int s = 0;
+    for (int i = 1; i < kMaxSpecies; i++) s += counts[i];
+    return s;
+  }
+  int i = species_index(sim, species);
+  if (i < 0) {
+    set_error(fn, ECnonexist, "species not found");
+    return (int)ECnonexist;
+  }
+  return counts[i];
+}
+
+enum ErrorCode smolSetBoundaryType(simptr sim, int dimension, int highside, char type) {
+  const char* fn = "smolSetBoundaryType";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(dimension < sim->dim, fn, ECbounds, "dimension cannot exceed system dimensionality");
+  LCHECK(highside < 2, fn, ECbounds, "highside must be -1, 0, or 1");
+  LCHECK(type == 'r' || type == 'p' || type == 'a' || type == 't', fn, ECsyntax, "invalid type");
+  for (int d = 0; d < sim->dim; d++) {
+    if (dimension >= 0 && d != dimension) continue;
+    for (int side = 0; side < 2; side++) {
+      if (highside >= 0 && side != highside) continue;
+      sim->wtype[2 * d + side] = type;
+    }
+  }
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolAddReaction(simptr sim, const char* reaction, const char* reactant1, enum MolecState rstate1,
+                               const char* reactant2, enum MolecState rstate2, int nproduct,
+                               const char** productspecies, enum MolecState* productstates, double rate) {
+  const char* fn = "smolAddReaction";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(reaction && reaction[0], fn, ECmissing, "missing reaction name");
+  LCHECK(!sim->started, fn, ECerror, "reactions are fixed once stepping started");
+  for (int o = 0; o < 3; o++)
+    for (auto& r : sim->rxn[o]) LCHECK(r.name != reaction, fn, ECerror, "reaction name already used");
+  Rxn r;
+  r.name = reaction;
+  int order = 0;
+  if (reactant1 && reactant1[0]) {
+    int i = species_index(sim, reactant1);
+    LCHECK(i > 0, fn, ECnonexist, "reactant 1 not found");
+    r.rct[order] = i;
+    r.rctstate[order] = rstate1;
+    order++;
+  }
+  if (reactant2 && reactant2[0]) {
+    int i = species_index(sim, reactant2);
+    LCHECK(i > 0, fn, ECnonexist, "reactant 2 not found");
+    r.rct[order] = i;
+    r.rctstate[order] = rstate2;
+    order++;
+  }
+  r.order = order;
+  LCHECK(nproduct >= 0 && nproduct <= kMaxProd, fn, ECbounds, "nproduct out of range");
+  r.nprod = nproduct;
+  for (int p = 0; p < nproduct; p++) {
+    int i = species_index(sim, productspecies[p]);
+    LCHECK(i > 0, fn, ECnonexist, "product not found");
+    r.prd[p] = i;
+    r.prdstate[p] = productstates ? productstates[p] : MSsoln;
+  }
+  set_permit(r);
+  if (rate >= 0) {
+    r.rate = rate;
+    r.multiplicity = 1;
+  }
+  int idx = (int)sim->rxn[order].size();
+  sim->rxn[order].push_back(r);
+  // nrxn/table over distinct permutations of the reactants (RxnAddReaction,
+  // smolreact.c:2118-2135)
+  if (order >= 1) {
+    if (sim->table[order].empty()) sim->table[order].resize(order == 1 ? kMaxSpecies : kMaxSpecies * kMaxSpecies);
+    sim->table[order][pack(order, r.rct)].push_back(idx);
+    if (order == 2 && r.rct[0] != r.rct[1]) {
+      int sw[2] = {r.rct[1], r.rct[0]};
+      sim->table[order][pack(order, sw)].push_back(idx);
+    }
+  }
+  invalidate(sim);
+  return ECok;
+}
+
+int smolGetReactionIndexNT(simptr sim, int* orderptr, const char* reaction) {
+  if (!sim || !reaction) return (int)ECmissing;
+  for (int o = 0; o < 3; o++) {
+    if (orderptr && *orderptr >= 0 && *orderptr != o) continue;
+    for (size_t r = 0; r < sim->rxn[o].size(); r++)
+      if (sim->rxn[o][r].name == reaction) {
+        if (orderptr) *orderptr = o;
+        return (int)r;
+      }
+  }
+  return (int)ECnonexist;
+}
+
+int smolGetReactionIndex(simptr sim, int* orderptr, const char* reaction) {
+  int r = smolGetReactionIndexNT(sim, orderptr, reaction);
+  if (r < 0) smolSetError("smolGetReactionIndex", (ErrorCode)r, "reaction not found", "");
+  return r;
+}
+
+enum ErrorCode smolSetReactionRate(simptr sim, const char* reaction, double rate, int type) {
+  const char* fn = "smolSetReactionRate";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(!sim->started, fn, ECerror, "rates are fixed once stepping started");
+  int order = -1;
+  int r = smolGetReactionIndexNT(sim, &order, reaction);
+  LCHECK(r >= 0, fn, ECnonexist, "reaction not found");
+  Rxn& rx = sim->rxn[order][r];
+  if (type == 0) {
+    LCHECK(rate >= 0, fn, ECbounds, "rate must be >=0");
+    rx.rate = rate;
+    if (rx.multiplicity < 0) rx.multiplicity = 1;
+  } else if (type == 1) {
+    rx.bindrad2 = rate * rate;
+  } else if (type == 2) {
+    rx.prob = rate;
+  } else {
+    LCHECK(0, fn, ECbounds, "type out of range");
+  }
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolSetReactionProducts(simptr sim, const char* reaction, enum RevParam method, double parameter,
+                                       const char* product, double* position) {
+  const char* fn = "smolSetReactionProducts";
+  (void)product;
+  (void)position;
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  int order = -1;
+  int r = smolGetReactionIndexNT(sim, &order, reaction);
+  LCHECK(r >= 0, fn, ECnonexist, "reaction not found");
+  LCHECK(method != RPoffset && method != RPfixed && method != RPbounce && method != RPconfspread, fn, ECerror,
+         "this product placement is not implemented on the device path");
+  Rxn& rx = sim->rxn[order][r];
+  rx.rparamt = method;
+  if (method != RPnone && method != RPirrev) rx.rparam = parameter;
+  invalidate(sim);
+  return ECok;
+}
+
+// ---------------------------------------------------------------------------
+// extensions
+
+enum ErrorCode smolamdAddMoleculesAt(simptr sim, const char* species, int number, const double* positions) {
+  const char* fn = "smolamdAddMoleculesAt";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(!sim->started, fn, ECerror, "molecules can only be added before stepping starts");
+  int i = species_index(sim, species);
+  LCHECK(i > 0, fn, ECnonexist, "species not found");
+  LCHECK(number >= 0 && (number == 0 || positions), fn, ECmissing, "missing positions");
+  for (int k = 0; k < number; k++) {
+    PendingMol pm;
+    pm.ident = i;
+    pm.serno = sim->serno++;
+    for (int d = 0; d < 3; d++) pm.pos[d] = positions[3 * k + d];
+    sim->pending.push_back(pm);
+  }
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolamdInjectDisplacements(simptr sim, const double* gauss, long nmol) {
+  const char* fn = "smolamdInjectDisplacements";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  ErrorCode ec;
+  int code = run_one(sim, gauss, nmol, fn, &ec);
+  if (code < 0) return ec;
+  if (code == 1) set_error(fn, ECnotify, "Simulation complete");
+  return ECok;
+}
+
+int smolamdGetListCount(simptr sim) {
+  if (!sim) return (int)ECmissing;
+  if (do_update(sim, "smolamdGetListCount") != ECok) return (int)ECerror;
+  return (int)sim->listname.size();
+}
+
+int smolamdGetMolecules(simptr sim, int list, int* species, double* pos, unsigned long long* serno, int* box,
+                        int maxmol) {
+  if (!sim) return (int)ECmissing;
+  if (do_update(sim, "smolamdGetMolecules") != ECok) return (int)ECerror;
+  return engine_get_list(sim->dev, list, species, pos, (uint64_t*)serno, box, maxmol);
+}
+
+int smolamdGetReactionLog(simptr sim, long long* rows, int maxrows) {
+  if (!sim || !sim->dev) return 0;
+  std::vector<LogRow> lr(maxrows > 0 ? maxrows : 1);
+  int n = engine_get_log(sim->dev, lr.data(), maxrows);
+  for (int k = 0; k < n && k < maxrows; k++) {
+    rows[6 * k + 0] = lr[k].order;
+    rows[6 * k + 1] = lr[k].r;
+    rows[6 * k + 2] = (long long)lr[k].rct[0];
+    rows[6 * k + 3] = (long long)lr[k].rct[1];
+    rows[6 * k + 4] = (long long)lr[k].prd[0];
+    rows[6 * k + 5] = (long long)lr[k].prd[1];
+  }
+  return n;
+}
+
+enum ErrorCode smolamdGetReactionParams(simptr sim, int order, int reaction, double* out) {
+  const char* fn = "smolamdGetReactionParams";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  ErrorCode ec = do_update(sim, fn);
+  if (ec != ECok) return ec;
+  LCHECK(order >= 0 && order <= 2 && reaction >= 0 && reaction < (int)sim->rxn[order].size(), fn, ECbounds,
+         "no such reaction");
+  Rxn& rx = sim->rxn[order][reaction];
+  out[0] = rx.bindrad2;
+  out[1] = rx.prob;
+  out[2] = rx.unbindrad;
+  out[3] = rx.prdpos[0][0];
+  out[4] = rx.nprod > 1 ? rx.prdpos[1][0] : 0;
+  out[5] = rx.rparamt;
+  return ECok;
+}
+
+int smolamdGetBoxes(simptr sim, int* side, double* boxmin, double* boxsize) {
+  if (!sim) return (int)ECmissing;
+  if (do_update(sim, "smolamdGetBoxes") != ECok) return (int)ECerror;
+  for (int d = 0; d < 3; d++) {
+    side[d] = sim->geo.side[d];
+    boxmin[d] = sim->geo.bmin[d];
+    boxsize[d] = sim->geo.bsize[d];
+  }
+  return sim->geo.nbox;
+}
+
+enum ErrorCode smolamdRunSteps(simptr sim, long nsteps, int use_graph, double* ms) {
+  const char* fn = "smolamdRunSteps";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  ErrorCode ec = do_update(sim, fn);
+  if (ec != ECok) return ec;
+  sim->started = true;
+  int bits = engine_run(sim->dev, nsteps, use_graph);
+  if (bits) return engine_error(fn, bits);
+  for (long k = 0; k < nsteps; k++) sim->time += sim->dt;
+  sim->stepindex += (uint32_t)nsteps;
+  if (ms) *ms = engine_last_ms(sim->dev);
+  return ECok;
+}
+
+enum ErrorCode smolamdProfileSteps(simptr sim, long nsteps, double* out) {
+  const char* fn = "smolamdProfileSteps";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  ErrorCode ec = do_update(sim, fn);
+  if (ec != ECok) return ec;
+  sim->started = true;
+  int bits = engine_profile(sim->dev, nsteps, out);
+  if (bits) return engine_error(fn, bits);
+  for (long k = 0; k < nsteps; k++) sim->time += sim->dt;
+  sim->stepindex += (uint32_t)nsteps;
+  return ECok;
+}
+
+enum ErrorCode smolamdGetDiagnostics(simptr sim, double* out) {
+  const char* fn = "smolamdGetDiagnostics";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(sim->dev, fn, ECerror, "simulation not set up");
+  Control c;
+  LCHECK(engine_sync_control(sim->dev, &c) == 0, fn, ECerror, "HIP runtime error");
+  double n = 0;
+  for (size_t l = 0; l < sim->listname.size(); l++) n += c.nl[l];
+  out[0] = n;
+  out[1] = c.maxedges_seen;
+  out[2] = c.rounds;
+  out[3] = c.err;
+  out[4] = (double)c.serno_next;
+  out[5] = c.step;
+  out[6] = (double)c.molsteps;
+  return ECok;
+}
+
+double smolamdBindingRadius(double rate, double dt, double difc, double b, int rel) {
+  return rxnparam::bindingradius(rate, dt, difc, b, rel);
+}
+double smolamdNumRxnRate(double step, double a, double b) { return rxnparam::numrxnrate(step, a, b); }
+double smolamdUnbindingRadius(double pgem, double dt, double difc, double a) {
+  return rxnparam::unbindingradius(pgem, dt, difc, a);
+}
+
+}  // extern "C"

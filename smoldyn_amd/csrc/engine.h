@@ -1,0 +1,143 @@
+// Device engine of the MI355X Smoldyn particle step: data layout shared by the
+// host library (capi.cpp) and the HIP kernels (engine.hip).
+//
+// Molecules live in structure-of-arrays form in HBM, one contiguous segment
+// per molecule list, in the reference's live-list order (the order that
+// source/Smoldyn/smolmolec.c:2943-3023 molsort maintains).  Keeping that
+// order on the device is what lets the greedy pair resolution of bireact
+// (smolreact.c:3201-3296) be reproduced exactly.
+#pragma once
+
+#include <stdint.h>
+
+namespace smolamd {
+
+constexpr int kMaxSpecies = 32;
+constexpr int kMaxList = 8;
+constexpr int kMaxRxnPerSet = 4;  // reactions per reactant (pair)
+constexpr int kMaxRxn = 32;
+constexpr int kMaxProd = 4;
+constexpr uint64_t kNoKey = ~0ull;
+
+// wall kinds, per wall 2*d+side
+enum WallKind : int { kWallTransparent = 0, kWallReflect = 1, kWallPeriodic = 2, kWallAbsorb = 3 };
+
+struct DevRxn {
+  int order;
+  int rct0;        // species of the first reactant (rctident[0])
+  int rct1;        // species of the second reactant (order 2)
+  int nprod;
+  int prd[kMaxProd];
+  int prdlist[kMaxProd];   // molecule list of each product
+  double bindrad2;
+  double prob;
+  double prdpos[kMaxProd];  // axial product offset (prdpos[prd][0])
+  int rotate;               // 1 if any product offset is nonzero
+  int permit_soln;          // permitted for solution reactant(s)
+  int confspread;
+};
+
+struct DevTables {
+  int nspecies;
+  int nlist;
+  int dim;
+  int ngauss;
+  double difc[kMaxSpecies];
+  double difstep[kMaxSpecies];
+  int listof[kMaxSpecies];  // list of the solution state
+  int nuni[kMaxSpecies];
+  int uni[kMaxSpecies][kMaxRxnPerSet];
+  int nbi[kMaxSpecies * kMaxSpecies];
+  int bi[kMaxSpecies * kMaxSpecies][kMaxRxnPerSet];
+  int rxnmol2[kMaxList * kMaxList];
+  int pairindex[kMaxList * kMaxList];  // rank of (ll1,ll2), ll1<=ll2, in the reference's loop order
+  int diffuselist[kMaxList];
+  DevRxn rxn1[kMaxRxn];
+  DevRxn rxn2[kMaxRxn];
+};
+
+struct Geometry {
+  double low[3], high[3], width[3];
+  int wall[6];
+  int side[3];
+  int wrapflag[3];  // neighbor wrap flags as neighborZV type 7 reads them
+  double bmin[3], bsize[3];
+  int nbox;
+};
+
+// Device control block: counters updated by the kernels themselves, so a
+// whole time step runs without host synchronisation.
+struct Control {
+  int nl[kMaxList];
+  int pfx[kMaxList];       // diffuse-order offset of each list (injection index)
+  uint64_t serno_next;
+  uint32_t step;           // step index, a Philox counter word
+  int nedges;
+  int nuni;
+  int naccept;
+  int nlog;
+  int err;                 // bit flags, see ErrBits
+  int count[kMaxSpecies];  // molecules per species
+  int maxedges_seen;
+  int rounds;              // matching rounds of the last step
+  unsigned long long molsteps;  // sum over steps of the live molecules at step start
+};
+
+enum ErrBits : int {
+  kErrEdgeOverflow = 1,
+  kErrEventOverflow = 2,
+  kErrCapacity = 4,
+  kErrSortOverflow = 8,
+};
+
+struct Edge {
+  uint64_t key;  // pass | pair | p1 | neighbor | q2, reference iteration order
+  int s1, s2;    // slots of the iterating molecule and its partner
+  int r;         // reaction index (order 2)
+  int wrap;      // periodic wrap code of the neighbor box (wallcalcdist2)
+};
+
+struct UniEvent {
+  int slot;
+  int r;
+};
+
+// One reaction event in creation order (for the reaction log)
+struct LogRow {
+  int order, r;
+  uint64_t rct[2];
+  uint64_t prd[2];
+};
+
+struct DeviceState;  // engine.hip
+
+struct EngineConfig {
+  int nlist;
+  int cap[kMaxList];
+  int maxedges;
+  int maxevents;
+};
+
+DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const Geometry& geo,
+                           const double* gtable, int ngauss, uint32_t key0, uint32_t key1, char* err, int errlen);
+void engine_destroy(DeviceState* s);
+// Upload molecules of list ll in live order (host arrays), replacing its content.
+int engine_upload_list(DeviceState* s, int ll, int n, const int* ident, const double* pos, const uint64_t* serno);
+int engine_set_counters(DeviceState* s, uint64_t serno_next, uint32_t step, const int* counts);
+// One reference time step (simulatetimestep).  inject: optional host array of
+// 3 Gaussian deviates per diffusing molecule in diffuse order (NULL = Philox).
+int engine_step(DeviceState* s, const double* inject, long ninject);
+// Many steps; returns 0 or an error bit set.
+int engine_run(DeviceState* s, long nsteps, int use_graph);
+// Steps with HIP events between the kernels; out[0..5] = average ms per step of
+// diffuse, box scan, scatter, box sort, pair search, resolve.
+int engine_profile(DeviceState* s, long nsteps, double* out);
+int engine_sync_control(DeviceState* s, Control* out);
+int engine_get_list(DeviceState* s, int ll, int* ident, double* pos, uint64_t* serno, int* box, int max);
+int engine_get_log(DeviceState* s, LogRow* rows, int maxrows);
+double engine_last_ms(DeviceState* s);
+// Timing of the dominant kernels of the last engine_run (average ms per launch).
+int engine_kernel_times(DeviceState* s, double* out, int nmax);
+int engine_device_bytes(DeviceState* s, double* out);
+
+}  // namespace smolamd

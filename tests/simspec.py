@@ -1,0 +1,139 @@
+"""Build the same simulation in the CPU oracle and in the device engine.
+
+A spec is a dict: low/high bounds, walls ('r','p','t' per dimension), dt,
+seed, species {name: difc}, optional lists {list: [species]}, reactions
+[(name, [reactants], [products], rate)], product placements
+{name: (kind, param)}, molecules [(species, n)] placed uniformly from the
+counter stream, or explicit positions [(species, array)].
+"""
+import numpy as np
+
+import oracle_bind as ob
+
+
+def build_oracle(spec):
+    sim = ob.OSim(3, spec["low"], spec["high"])
+    for d, w in enumerate(spec["walls"]):
+        sim.wall(d, 0, w)
+        sim.wall(d, 1, w)
+    sim.times(0.0, spec.get("tmax", 1e9), spec["dt"])
+    sim.rngmode(ob.RNG_COUNTER, spec["seed"])
+    idx = {}
+    for name, d in spec["species"].items():
+        idx[name] = sim.species(name)
+        sim.difc(idx[name], d, 0)
+    for ll, (lname, members) in enumerate(spec.get("lists", {}).items()):
+        sim.add_list(lname)
+        for m in members:
+            sim.set_list(idx[m], ll, 6)
+    if "molperbox" in spec:
+        sim.partitions("molperbox", spec["molperbox"])
+    if "boxsize" in spec:
+        sim.partitions("boxsize", spec["boxsize"])
+    rx = {}
+    for name, rct, prd, rate in spec.get("reactions", []):
+        rx[name] = sim.reaction(name, [idx[r] for r in rct], [idx[p] for p in prd], rate)
+    for name, (kind, param) in spec.get("placement", {}).items():
+        sim.revparam(rx[name], kind, param)
+    for sp, n in spec.get("molecules", []):
+        if isinstance(n, int):
+            sim.add_mols(n, idx[sp], spec["low"], spec["high"])
+        else:
+            sim.add_mols_at(idx[sp], n)
+    sim.update()
+    return sim, rx
+
+
+def build_device(spec):
+    import smoldyn_amd as sa
+    sim = sa.Simulation(spec["low"], spec["high"])
+    for d, w in enumerate(spec["walls"]):
+        sim.boundary(d, 0, w)
+        sim.boundary(d, 1, w)
+    sim.times(0.0, spec.get("tmax", 1e9), spec["dt"])
+    sim.seed(spec["seed"])
+    for lname in spec.get("lists", {}):
+        sim.add_list(lname)
+    for name, d in spec["species"].items():
+        sim.species(name)
+        sim.mobility(name, d)
+    for lname, members in spec.get("lists", {}).items():
+        for m in members:
+            sim.set_list(m, lname)
+    if "molperbox" in spec:
+        sim.partitions("molperbox", spec["molperbox"])
+    if "boxsize" in spec:
+        sim.partitions("boxsize", spec["boxsize"])
+    for name, rct, prd, rate in spec.get("reactions", []):
+        sim.reaction(name, rct, prd, rate)
+    for name, (kind, param) in spec.get("placement", {}).items():
+        sim.product_placement(name, kind, param)
+    for sp, n in spec.get("molecules", []):
+        if isinstance(n, int):
+            sim.add_molecules(sp, n)
+        else:
+            sim.add_molecules_at(sp, n)
+    sim.update()
+    return sim
+
+
+def oracle_state(osim):
+    out = []
+    for ll in range(osim.nlist()):
+        out.append(osim.get_list(ll))
+    return out
+
+
+def device_state(dsim):
+    return [dsim.molecules(ll) for ll in range(dsim.nlists())]
+
+
+def compare_states(o, d, check_box=True):
+    """Bit-exact comparison of live lists (order, species, serials, FP64 positions, boxes)."""
+    assert len(o) == len(d)
+    for ll, (a, b) in enumerate(zip(o, d)):
+        assert len(a["ident"]) == len(b["ident"]), "list %d length %d vs %d" % (ll, len(a["ident"]), len(b["ident"]))
+        assert np.array_equal(a["serno"], b["serno"]), "list %d serial order differs" % ll
+        assert np.array_equal(a["ident"], b["ident"]), "list %d species differ" % ll
+        da = a["pos"].view(np.uint64)
+        db = b["pos"].view(np.uint64)
+        if not np.array_equal(da, db):
+            bad = np.argwhere(da != db)
+            k = bad[0][0]
+            raise AssertionError("list %d position bits differ at %d: %r vs %r (max |diff| %g)" % (
+                ll, k, a["pos"][k], b["pos"][k], np.abs(a["pos"] - b["pos"]).max()))
+        if check_box:
+            assert np.array_equal(a["box"], b["box"]), "list %d box indices differ" % ll
+
+
+def canonical_pairs(rows):
+    """Reacted-pair set: sorted tuples (order, reaction, sorted reactant serials, product serials)."""
+    out = []
+    for r in rows:
+        rct = tuple(sorted(int(x) for x in r[2:4]))
+        out.append((int(r[0]), int(r[1]), rct, int(r[4]), int(r[5])))
+    return sorted(out)
+
+
+def c1_spec(n=10000, seed=11, **kw):
+    """Irreversible A+B->C in a reflective cube of side 100; D=1/1/0.5, dt 0.01,
+    k giving a binding radius of about 0.5 (rxnparam)."""
+    spec = dict(low=[0, 0, 0], high=[100, 100, 100], walls="rrr", dt=0.01, seed=seed,
+                species={"A": 1.0, "B": 1.0, "C": 0.5},
+                reactions=[("fwd", ["A", "B"], ["C"], 9.640578597)],
+                molecules=[("A", n), ("B", n)])
+    spec.update(kw)
+    return spec
+
+
+def c2_spec(n=500000, side=1000.0, seed=7, kback=1.0, **kw):
+    """Reversible A+B<->C in a periodic cube; D=1/1/0.5, dt 0.005, forward rate
+    giving binding radius ~1.0 with unbinding radius 1.2x (product_placement
+    ratio 1.2 on the reverse reaction)."""
+    spec = dict(low=[0, 0, 0], high=[side] * 3, walls="ppp", dt=0.005, seed=seed,
+                species={"A": 1.0, "B": 1.0, "C": 0.5},
+                reactions=[("fwd", ["A", "B"], ["C"], 99.3218287), ("back", ["C"], ["A", "B"], kback)],
+                placement={"back": ("ratio", 1.2)},
+                molecules=[("A", n), ("B", n)])
+    spec.update(kw)
+    return spec
