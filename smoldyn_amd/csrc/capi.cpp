@@ -631,6 +631,35 @@ ErrorCode do_update(simptr sim, const char* fn) {
         LCHECK(rx.prdpos[p][1] == 0 && rx.prdpos[p][2] == 0, fn, ECerror, "off-axis product offsets");
       }
     }
+  // pair-search screening tables: which partner species react, and the
+  // largest binding radius of each species
+  for (int i1 = 1; i1 < nsp; i1++) {
+    t.reactmask[i1] = 0;
+    t.maxbind2[i1] = 0;
+    for (int i2 = 1; i2 < nsp; i2++) {
+      int k = i1 * kMaxSpecies + i2;
+      for (int j = 0; j < t.nbi[k]; j++) {
+        const DevRxn& dr = t.rxn2[t.bi[k][j]];
+        if (dr.bindrad2 > 0 && dr.prob > 0 && dr.permit_soln) {
+          t.reactmask[i1] |= 1u << i2;
+          if (dr.bindrad2 > t.maxbind2[i1]) t.maxbind2[i1] = dr.bindrad2;
+        }
+      }
+    }
+  }
+  {
+    bool transparent = false;
+    double ext = 0;
+    for (int w = 0; w < 6; w++) transparent |= sim->wtype[w] == 't';
+    for (int d = 0; d < 3; d++) {
+      ext = fmax(ext, fabs(sim->low[d]));
+      ext = fmax(ext, fabs(sim->high[d]));
+      ext = fmax(ext, sim->high[d] - sim->low[d]);
+    }
+    // FP32 rounding of coordinates of magnitude <= 2*ext perturbs a distance by
+    // less than 5 * 2^-23 * 2*ext; the margin is three times that
+    g.prefilter_margin = transparent ? -1.0 : 32.0 * ext * ldexp(1.0, -23);
+  }
   // rxnsupdatelists order 2 (smolreact.c:2352-2377) and the (ll1,ll2) loop rank
   for (size_t r = 0; r < sim->rxn[2].size(); r++) {
     Rxn& rx = sim->rxn[2][r];
@@ -671,10 +700,20 @@ ErrorCode do_update(simptr sim, const char* fn) {
   EngineConfig cfg;
   memset(&cfg, 0, sizeof(cfg));
   cfg.nlist = nlist;
+  // a list can only hold molecules placed at setup or created by reactions;
+  // lists that receive neither keep a token capacity
+  std::vector<int> reachable(nlist, 0);
+  for (int l = 0; l < nlist; l++) reachable[l] = !ident[l].empty();
+  for (int order = 1; order <= 2; order++)
+    for (auto& rx : sim->rxn[order])
+      for (int p = 0; p < rx.nprod; p++) reachable[lookup[rx.prd[p]][MSsoln]] = 1;
   long capl = sim->maxmol > 0 ? sim->maxmol : ntot + (ntot / 4 > 4096 ? ntot / 4 : 4096);
-  LCHECK(capl * nlist < (1L << 27), fn, ECbounds, "too many molecules for the device key layout");
-  for (int l = 0; l < nlist; l++) cfg.cap[l] = (int)capl;
-  long captot = capl * nlist;
+  long captot = 0;
+  for (int l = 0; l < nlist; l++) {
+    cfg.cap[l] = reachable[l] ? (int)capl : 256;
+    captot += cfg.cap[l];
+  }
+  LCHECK(captot < (1L << 27), fn, ECbounds, "too many molecules for the device key layout");
   cfg.maxedges = (int)(captot / 2 > 65536 ? captot / 2 : 65536);
   cfg.maxevents = (int)(captot / 16 > 16384 ? captot / 16 : 16384);
   if (sim->dev) engine_destroy(sim->dev);

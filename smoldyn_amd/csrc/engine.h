@@ -52,6 +52,8 @@ struct DevTables {
   int rxnmol2[kMaxList * kMaxList];
   int pairindex[kMaxList * kMaxList];  // rank of (ll1,ll2), ll1<=ll2, in the reference's loop order
   int diffuselist[kMaxList];
+  uint32_t reactmask[kMaxSpecies];  // bit i2 set when species (i, i2) has a bimolecular reaction
+  double maxbind2[kMaxSpecies];     // largest squared binding radius over those reactions
   DevRxn rxn1[kMaxRxn];
   DevRxn rxn2[kMaxRxn];
 };
@@ -63,6 +65,11 @@ struct Geometry {
   int wrapflag[3];  // neighbor wrap flags as neighborZV type 7 reads them
   double bmin[3], bsize[3];
   int nbox;
+  // FP32 pre-filter of the pair search: a pair is tested exactly (FP64) when
+  // its FP32 distance is within binding radius + prefilter_margin; the margin
+  // bounds the FP32 rounding of coordinates up to the system extent.  A
+  // negative margin turns the pre-filter off (transparent walls).
+  double prefilter_margin;
 };
 
 // Device control block: counters updated by the kernels themselves, so a

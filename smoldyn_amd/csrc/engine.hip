@@ -68,11 +68,11 @@ struct Binned {
   int* count;   // [nlist*nbox+1], zeroed by k_scan1 after reading
   int* part;    // block-local exclusive prefix
   int* boff;    // exclusive prefix of block sums
+  int* bstart;  // flat exclusive prefix: first box-ordered entry of (list, box)
   int* tmp;     // slots in box order, unordered within a box
   int* sslot;   // slots in box order, live order within a box
   int* sbox;
-  int* sident;
-  double *sx, *sy, *sz;
+  float4* spk;  // FP32 position + species bits, box order (pair pre-filter)
 };
 
 // Product record built per event before the list compaction.
@@ -166,7 +166,7 @@ __device__ __forceinline__ int atomic_load_i32(int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ int gstart(const Binned& bn, int i) { return bn.part[i] + bn.boff[i / kScanTile]; }
+__device__ __forceinline__ int gstart(const Binned& bn, int i) { return bn.bstart[i]; }
 
 // ---------------------------------------------------------------------------
 // k_diffuse: diffusion, walls, box, histogram, first-order reaction coins
@@ -302,6 +302,12 @@ __global__ __launch_bounds__(kScanBlock) void k_scan2(int* __restrict__ bsum, in
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ part, const int* __restrict__ boff,
+                                                  int* __restrict__ bstart, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) bstart[i] = part[i] + boff[i / kScanTile];
+}
+
 __global__ __launch_bounds__(kBlock) void k_scatter(Layout L, const Control* ctl, Mols m, Binned bn, int nbox) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= L.captot) return;
@@ -327,10 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_boxsort(Layout L, const Control* ctl
   int dst = i0 + r;
   bn.sslot[dst] = t;
   bn.sbox[dst] = b;
-  bn.sident[dst] = m.ident[t];
-  bn.sx[dst] = m.px[t];
-  bn.sy[dst] = m.py[t];
-  bn.sz[dst] = m.pz[t];
+  bn.spk[dst] = make_float4((float)m.px[t], (float)m.py[t], (float)m.pz[t], __int_as_float(m.ident[t]));
 }
 
 // ---------------------------------------------------------------------------
@@ -370,106 +373,152 @@ __device__ __forceinline__ int first_reaction(const DevTables* tab, int i1, int 
   return -1;
 }
 
+// Exact FP64 test of one candidate (the reference's dist2 in its operation
+// order, smolreact.c:3238-3240 and wallcalcdist2 smolwall.c:100-116) and the
+// edge for the first reaction that accepts it.
+__device__ __forceinline__ void exact_pair(const Geometry& g, const DevTables* tab, Control* ctl, const Mols& m,
+                                           Edge* edges, int maxedges, uint32_t key0, uint32_t key1, uint32_t step,
+                                           int t1, const double* p1, int i1, uint64_t ser1, int t2, int i2, int wrap,
+                                           uint64_t key) {
+  double p2[3] = {m.px[t2], m.py[t2], m.pz[t2]};
+  double dist2 = 0;
+  if (wrap) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      int cc = (wrap >> (2 * d)) & 3;
+      double v;
+      if (cc == 0) v = p2[d] - p1[d];
+      else if (cc == 1) v = p2[d] - p1[d] - g.width[d];
+      else v = p2[d] - p1[d] + g.width[d];
+      dist2 += v * v;
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; d++) dist2 += (p1[d] - p2[d]) * (p1[d] - p2[d]);
+  }
+  int r = first_reaction(tab, i1, i2, dist2, key0, key1, ser1, m.serno[t2], step);
+  if (r >= 0) emit_edge(ctl, edges, maxedges, key, t1, t2, r, wrap);
+}
+
+// One thread per molecule in box order.  Candidates are screened in FP32 from
+// one packed 16-byte load each; only pairs within binding radius plus the
+// rounding margin reach the exact FP64 test.
 __global__ __launch_bounds__(kBlock) void k_pairs(Layout L, Geometry g, const DevTables* __restrict__ tab,
                                                   Control* ctl, Mols m, Binned bn, int nbox, int nscan,
                                                   Edge* __restrict__ edges, int maxedges, uint32_t key0,
                                                   uint32_t key1) {
   int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= gstart(bn, nscan - 1)) return;  // live total
-  int i1 = bn.sident[u];
+  if (u >= bn.bstart[nscan - 1]) return;  // live total
+  float4 me = bn.spk[u];
+  int i1 = __float_as_int(me.w);
   if (i1 == 0) return;
+  uint32_t mask = tab->reactmask[i1];
+  if (!mask) return;
   int t1 = bn.sslot[u];
   int ll1 = slot_list(L, t1);
   int nl = tab->nlist;
   int b = bn.sbox[u];
-  double x1 = bn.sx[u], y1 = bn.sy[u], z1 = bn.sz[u];
-  uint64_t ser1 = m.serno[t1];
+  float r2f;
+  if (g.prefilter_margin >= 0) {
+    double rr = sqrt(tab->maxbind2[i1]) + g.prefilter_margin;
+    r2f = (float)(rr * rr);
+  } else {
+    r2f = 3.0e38f;
+  }
+  uint64_t ser1 = 0;
+  double p1[3];
+  bool loaded = false;
   uint32_t step = ctl->step;
-  int p1 = t1 - L.base[ll1];
-  int q1 = u - gstart(bn, ll1 * nbox + b);
+  int p1pos = t1 - L.base[ll1];
   int bx = b / (g.side[1] * g.side[2]);
   int by = (b / g.side[2]) % g.side[1];
   int bz = b % g.side[2];
+  float wf[3] = {(float)g.width[0], (float)g.width[1], (float)g.width[2]};
   for (int ll2 = ll1; ll2 < nl; ll2++) {
     if (!tab->rxnmol2[ll1 * kMaxList + ll2]) continue;
     uint64_t pairbits = (uint64_t)tab->pairindex[ll1 * kMaxList + ll2] << 59;
     // same box (pass 0): for ll1==ll2 only entries before m1
     {
       int j0 = gstart(bn, ll2 * nbox + b);
-      int j1 = gstart(bn, ll2 * nbox + b + 1);
-      int jend = (ll1 == ll2) ? j0 + q1 : j1;
+      int jend = (ll1 == ll2) ? u : gstart(bn, ll2 * nbox + b + 1);
       for (int k = j0; k < jend; k++) {
-        int i2 = bn.sident[k];
-        if (i2 == 0 || tab->nbi[i1 * kMaxSpecies + i2] == 0) continue;
-        double dx = x1 - bn.sx[k], dy = y1 - bn.sy[k], dz = z1 - bn.sz[k];
-        double dist2 = 0;
-        dist2 += dx * dx;
-        dist2 += dy * dy;
-        dist2 += dz * dz;
-        int t2 = bn.sslot[k];
-        int r = first_reaction(tab, i1, i2, dist2, key0, key1, ser1, m.serno[t2], step);
-        if (r >= 0) {
-          uint64_t key = pairbits | ((uint64_t)p1 << 32) | (uint64_t)(k - j0);
-          emit_edge(ctl, edges, maxedges, key, t1, t2, r, 0);
+        float4 o = bn.spk[k];
+        int i2 = __float_as_int(o.w);
+        if (!((mask >> i2) & 1u)) continue;
+        float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+        float d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > r2f) continue;
+        if (!loaded) {
+          p1[0] = m.px[t1];
+          p1[1] = m.py[t1];
+          p1[2] = m.pz[t1];
+          ser1 = m.serno[t1];
+          loaded = true;
         }
+        uint64_t key = pairbits | ((uint64_t)p1pos << 32) | (uint64_t)(k - j0);
+        exact_pair(g, tab, ctl, m, edges, maxedges, key0, key1, step, t1, p1, i1, ser1, bn.sslot[k], i2, 0, key);
       }
     }
     // neighbor boxes (pass 1): 3^3 stencil in add2indx3ZV order, half for ll1==ll2
     for (int nb = 0; nb < 27; nb++) {
       if (nb == 13) continue;
       if (ll1 == ll2 && nb > 13) break;
-      int o[3] = {nb / 9 - 1, (nb / 3) % 3 - 1, nb % 3 - 1};
-      int c[3] = {bx + o[0], by + o[1], bz + o[2]};
-      int wrap = 0, valid = 1;
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        if (c[d] < 0) {
-          if (g.wrapflag[d]) {
-            c[d] = g.side[d] - 1;
-            wrap += 1 << (2 * d);
-          } else
-            valid = 0;
-        } else if (c[d] >= g.side[d]) {
-          if (g.wrapflag[d]) {
-            c[d] = 0;
-            wrap += 2 << (2 * d);
-          } else
-            valid = 0;
-        }
+      int c0 = bx + nb / 9 - 1, c1 = by + (nb / 3) % 3 - 1, c2 = bz + nb % 3 - 1;
+      int wrap = 0;
+      float sh[3] = {0.f, 0.f, 0.f};
+      if (c0 < 0) {
+        if (!g.wrapflag[0]) continue;
+        c0 = g.side[0] - 1;
+        wrap += 1;
+        sh[0] = -wf[0];
+      } else if (c0 >= g.side[0]) {
+        if (!g.wrapflag[0]) continue;
+        c0 = 0;
+        wrap += 2;
+        sh[0] = wf[0];
       }
-      if (!valid) continue;
-      int b2 = (c[0] * g.side[1] + c[1]) * g.side[2] + c[2];
+      if (c1 < 0) {
+        if (!g.wrapflag[1]) continue;
+        c1 = g.side[1] - 1;
+        wrap += 1 << 2;
+        sh[1] = -wf[1];
+      } else if (c1 >= g.side[1]) {
+        if (!g.wrapflag[1]) continue;
+        c1 = 0;
+        wrap += 2 << 2;
+        sh[1] = wf[1];
+      }
+      if (c2 < 0) {
+        if (!g.wrapflag[2]) continue;
+        c2 = g.side[2] - 1;
+        wrap += 1 << 4;
+        sh[2] = -wf[2];
+      } else if (c2 >= g.side[2]) {
+        if (!g.wrapflag[2]) continue;
+        c2 = 0;
+        wrap += 2 << 4;
+        sh[2] = wf[2];
+      }
+      int b2 = (c0 * g.side[1] + c1) * g.side[2] + c2;
       int j0 = gstart(bn, ll2 * nbox + b2);
       int j1 = gstart(bn, ll2 * nbox + b2 + 1);
       for (int k = j0; k < j1; k++) {
-        int i2 = bn.sident[k];
-        if (i2 == 0 || tab->nbi[i1 * kMaxSpecies + i2] == 0) continue;
-        double dist2 = 0;
-        if (wrap) {
-          double p2[3] = {bn.sx[k], bn.sy[k], bn.sz[k]};
-          double p1v[3] = {x1, y1, z1};
-#pragma unroll
-          for (int d = 0; d < 3; d++) {
-            int cc = (wrap >> (2 * d)) & 3;
-            double v;
-            if (cc == 0) v = p2[d] - p1v[d];
-            else if (cc == 1) v = p2[d] - p1v[d] - g.width[d];
-            else v = p2[d] - p1v[d] + g.width[d];
-            dist2 += v * v;
-          }
-        } else {
-          double dx = x1 - bn.sx[k], dy = y1 - bn.sy[k], dz = z1 - bn.sz[k];
-          dist2 += dx * dx;
-          dist2 += dy * dy;
-          dist2 += dz * dz;
+        if (k == u) continue;  // a box that is its own periodic neighbor
+        float4 o = bn.spk[k];
+        int i2 = __float_as_int(o.w);
+        if (!((mask >> i2) & 1u)) continue;
+        float dx = (o.x + sh[0]) - me.x, dy = (o.y + sh[1]) - me.y, dz = (o.z + sh[2]) - me.z;
+        float d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > r2f) continue;
+        if (!loaded) {
+          p1[0] = m.px[t1];
+          p1[1] = m.py[t1];
+          p1[2] = m.pz[t1];
+          ser1 = m.serno[t1];
+          loaded = true;
         }
-        int t2 = bn.sslot[k];
-        if (t2 == t1) continue;  // a box that is its own periodic neighbor
-        int r = first_reaction(tab, i1, i2, dist2, key0, key1, ser1, m.serno[t2], step);
-        if (r >= 0) {
-          uint64_t key = (1ull << 63) | pairbits | ((uint64_t)p1 << 32) | ((uint64_t)nb << 27) | (uint64_t)(k - j0);
-          emit_edge(ctl, edges, maxedges, key, t1, t2, r, wrap);
-        }
+        uint64_t key = (1ull << 63) | pairbits | ((uint64_t)p1pos << 32) | ((uint64_t)nb << 27) | (uint64_t)(k - j0);
+        exact_pair(g, tab, ctl, m, edges, maxedges, key0, key1, step, t1, p1, i1, ser1, bn.sslot[k], i2, wrap, key);
       }
     }
   }
@@ -951,10 +1000,10 @@ DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const 
   ok |= dalloc(&s->m.px, captot) | dalloc(&s->m.py, captot) | dalloc(&s->m.pz, captot);
   ok |= dalloc(&s->m.ident, captot) | dalloc(&s->m.serno, captot) | dalloc(&s->m.box, captot) |
         dalloc(&s->m.rank, captot);
-  ok |= dalloc(&s->bn.count, s->nscan) | dalloc(&s->bn.part, s->nscan) | dalloc(&s->bn.boff, s->nscanblk);
+  ok |= dalloc(&s->bn.count, s->nscan) | dalloc(&s->bn.part, s->nscan) | dalloc(&s->bn.boff, s->nscanblk) |
+        dalloc(&s->bn.bstart, s->nscan);
   ok |= dalloc(&s->bn.tmp, captot) | dalloc(&s->bn.sslot, captot) | dalloc(&s->bn.sbox, captot) |
-        dalloc(&s->bn.sident, captot);
-  ok |= dalloc(&s->bn.sx, captot) | dalloc(&s->bn.sy, captot) | dalloc(&s->bn.sz, captot);
+        dalloc(&s->bn.spk, captot);
   ok |= dalloc(&s->edges, cfg.maxedges) | dalloc(&s->estate, cfg.maxedges) | dalloc(&s->accepted, cfg.maxedges);
   ok |= dalloc(&s->bestkey, captot) | dalloc(&s->killed, captot);
   ok |= dalloc(&s->uniev, cfg.maxevents) | dalloc(&s->log, s->sortcap);
@@ -984,7 +1033,7 @@ DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const 
 void engine_destroy(DeviceState* s) {
   if (!s) return;
   void* ptrs[] = {s->m.px, s->m.py, s->m.pz, s->m.ident, s->m.serno, s->m.box, s->m.rank, s->bn.count, s->bn.part,
-                  s->bn.boff, s->bn.tmp, s->bn.sslot, s->bn.sbox, s->bn.sident, s->bn.sx, s->bn.sy, s->bn.sz,
+                  s->bn.boff, s->bn.bstart, s->bn.tmp, s->bn.sslot, s->bn.sbox, s->bn.spk,
                   s->edges, s->estate, s->accepted, s->bestkey, s->killed, s->uniev, s->log, s->sortkey,
                   s->sortval, s->killslot, s->ctl, s->tab, s->gtab, s->inj, s->prods};
   for (void* p : ptrs)
@@ -1078,6 +1127,8 @@ static void launch_step(DeviceState* s, const double* dinj, int timed) {
   hipLaunchKernelGGL(k_scan1, dim3(s->nscanblk), dim3(kScanBlock), 0, st, s->bn.count, s->bn.part, s->bn.boff,
                      s->nscan);
   hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kScanBlock), 0, st, s->bn.boff, s->nscanblk);
+  hipLaunchKernelGGL(k_scan3, dim3((s->nscan + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s->bn.part, s->bn.boff,
+                     s->bn.bstart, s->nscan);
   if (timed) (void)hipEventRecord(s->ev[2], st);
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, st, s->lay, s->ctl, s->m, s->bn, nbox);
   if (timed) (void)hipEventRecord(s->ev[3], st);

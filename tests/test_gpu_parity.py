@@ -87,3 +87,37 @@ def test_injection_identity_with_counter_stream():
     dsim.run_time_step()
     osim.step()
     ss.compare_states(ss.oracle_state(osim), ss.device_state(dsim))
+
+
+def test_c2_full_size_bit_exact_three_steps():
+    """The 1M-molecule headline workload: three steps compared with the oracle
+    bit for bit (positions, order, serials, boxes, reaction events)."""
+    spec = ss.c2_spec(n=500000, seed=2026, kback=1.0)
+    nev = _run_pair(spec, 3, check_every=3)
+    assert nev > 100
+
+
+def test_c2_full_size_invariants_over_many_steps():
+    """Properties that hold at any size: mass balance of A+B<->C, every
+    molecule inside the periodic box, unique serial numbers, box index equals
+    pos2box of the position after the walls, and counts from the device's
+    counters equal the counts in the molecule lists."""
+    spec = ss.c2_spec(n=500000, seed=77, kback=5.0)
+    dsim = ss.build_device(spec)
+    dsim.run_steps(400)
+    st = dsim.molecules(0)
+    ident = st["ident"]
+    a, b, c = (int((ident == k).sum()) for k in (1, 2, 3))
+    assert a + c == 500000 and b + c == 500000
+    assert c > 0
+    assert (dsim.count("A"), dsim.count("B"), dsim.count("C")) == (a, b, c)
+    pos = st["pos"]
+    assert (pos >= 0).all() and (pos <= 1000).all()
+    assert len(np.unique(st["serno"])) == len(st["serno"])
+    n, side, bmin, bsize = dsim.boxes()
+    idx = np.clip(((pos - bmin) / bsize).astype(np.int64), 0, side - 1)
+    # products created in the last step carry the box of their final position;
+    # all others the box computed after diffusion, which is their position now
+    assert np.array_equal((idx[:, 0] * side[1] + idx[:, 1]) * side[2] + idx[:, 2], st["box"])
+    d = dsim.diagnostics()
+    assert d["err"] == 0
