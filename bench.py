@@ -27,17 +27,18 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-# Algorithmic HBM bytes per molecule per launch of each kernel, FP64 SoA:
-#   k_diffuse  read pos 24 + species 4 + serial 8, write pos 24 + box 4 + rank 4,
-#              histogram atomic 4                                        = 72
-#   k_scatter  read box 4 + rank 4 + scan offsets 8, write slot 4         = 20
-#   k_boxsort  read slot 4 + box 4 + species 4 + pos 24 + box range 8,
-#              write slot 4 + box 4 + species 4 + pos 24                  = 80
-#   k_pairs    read own species 4 + slot 4 + box 4 + pos 24 + serial 8
-#              + box ranges 8 (neighbor entries are cache reuse)          = 52
-ALGO_BYTES = {"diffuse": 72, "scatter": 20, "boxsort": 80, "pairs": 52}
-KERNEL_NAMES = ["diffuse", "scan", "scatter", "boxsort", "pairs", "resolve"]
-HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_scan1", "scatter": "k_scatter", "boxsort": "k_boxsort",
+# Algorithmic HBM bytes per molecule per launch of each kernel (molecule
+# records in box order, FP64 positions):
+#   k_diffuse  read pos 24 + species 4 + live position 4 + serial 8,
+#              write pos 24 + box 4 + rank 4                            = 72
+#   k_scatter  read box 4 + rank 4 + species 4 + pos 24 + serial 8 + live
+#              position 4 + box start 4; write pos 24 + species 4 + serial 8
+#              + live position 4 + box 4 + packed FP32 position 16      = 112
+#   k_pairs    read packed position 16 + live position 4 + box 4 once (the
+#              neighboring boxes' records are re-read from L2, not HBM) = 24
+ALGO_BYTES = {"diffuse": 72, "scatter": 112, "pairs": 24}
+KERNEL_NAMES = ["diffuse", "scan", "scatter", "unused", "pairs", "resolve"]
+HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_scan1", "scatter": "k_scatter", "unused": "-",
                      "pairs": "k_pairs", "resolve": "k_resolve"}
 
 

@@ -714,6 +714,11 @@ ErrorCode do_update(simptr sim, const char* fn) {
     captot += cfg.cap[l];
   }
   LCHECK(captot < (1L << 27), fn, ECbounds, "too many molecules for the device key layout");
+  {
+    long maxl = 1;
+    for (int l = 0; l < nlist; l++) maxl = (long)ident[l].size() > maxl ? (long)ident[l].size() : maxl;
+    cfg.expected_per_box = (double)maxl / (double)g.nbox;
+  }
   cfg.maxedges = (int)(captot / 2 > 65536 ? captot / 2 : 65536);
   cfg.maxevents = (int)(captot / 16 > 16384 ? captot / 16 : 16384);
   if (sim->dev) engine_destroy(sim->dev);

@@ -88,6 +88,8 @@ struct Control {
   int maxedges_seen;
   int rounds;              // matching rounds of the last step
   unsigned long long molsteps;  // sum over steps of the live molecules at step start
+  int nrec;                // molecule records in the current buffer (live, dead-marked, new)
+  int nbinned;             // records binned in the last step
 };
 
 enum ErrBits : int {
@@ -98,14 +100,15 @@ enum ErrBits : int {
 };
 
 struct Edge {
-  uint64_t key;  // pass | pair | p1 | neighbor | q2, reference iteration order
-  int s1, s2;    // slots of the iterating molecule and its partner
+  uint64_t key;  // pass | list pair | live pos 1 | neighbor | live pos 2: the reference's loop order
+  int s1, s2;    // records (destination buffer) of the iterating molecule and its partner
   int r;         // reaction index (order 2)
   int wrap;      // periodic wrap code of the neighbor box (wallcalcdist2)
 };
 
 struct UniEvent {
-  int slot;
+  int rec;   // record index in the source buffer of the step
+  int slot;  // list base + live position
   int r;
 };
 
@@ -123,6 +126,7 @@ struct EngineConfig {
   int cap[kMaxList];
   int maxedges;
   int maxevents;
+  double expected_per_box;  // molecules per virtual box per list, for the pair-search tile
 };
 
 DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const Geometry& geo,

@@ -1,23 +1,29 @@
 // HIP kernels of the MI355X Smoldyn particle step (gfx950).
 //
-// One reference time step (source/Smoldyn/smolsim.cpp:2607-2690
-// simulatetimestep, no surfaces) runs as seven launches with no host
-// synchronisation:
+// Data layout.  Molecule records (FP64 position, species, serial number and
+// the molecule's position in the reference's live list) are stored in
+// virtual-box order, double-buffered, so every per-molecule pass is a
+// coalesced HBM stream.  The reference's ordering semantics only depend on
+// live-list positions (molsort, source/Smoldyn/smolmolec.c:2943-3023, keeps
+// them; reassignmolecs, smolboxes.c:896-954, fills box lists in live order),
+// so the live position is kept as a record attribute plus an inverse map
+// (list slot -> record) instead of as the storage order.
 //
-//   k_diffuse   diffuse (smolmolec.c:3115-3168) + checkwalls (smolwall.c:368-424)
-//               + pos2box (smolboxes.c:67-79) + box histogram + unireact coins
-//               (smolreact.c:2997-3054); one thread per molecule, SoA streams
-//   k_scan1/2   exclusive scan of the box histogram (reassignmolecs' box lists)
-//   k_scatter   counting-sort scatter into box order
-//   k_boxsort   restore live order inside each box (the reference appends in
-//               live order, smolboxes.c:908-930) and gather the box-ordered
-//               position payload
-//   k_pairs     bireact candidate search (smolreact.c:3201-3296): same box and
-//               half/full neighbor stencil, FP64 distance tests, emits edges
-//               keyed by the reference's loop order
-//   k_resolve   one workgroup: greedy matching in the reference's edge order,
-//               product creation (doreact, smolreact.c:2662-2901), molsort
-//               compaction (smolmolec.c:2943-3023) and product append order
+// One reference time step (smolsim.cpp:2607-2690 simulatetimestep, no
+// surfaces) runs as seven launches and no host synchronisation:
+//
+//   k_diffuse   diffuse (smolmolec.c:3115-3168), checkwalls (smolwall.c:368-424),
+//               pos2box (smolboxes.c:67-79), unireact coins (smolreact.c:2997-3054),
+//               box histogram; one thread per record of the source buffer
+//   k_scan1/2/3 exclusive scan of the histogram (box lists of reassignmolecs)
+//   k_scatter   counting-sort scatter into the destination buffer + inverse map
+//               + packed FP32 payload for the pair search
+//   k_pairs     bireact candidates (smolreact.c:3201-3296): same box and half
+//               or full neighbor stencil, FP32 screen then the reference's FP64
+//               distance test; edges keyed by the reference's loop order
+//   k_resolve   one workgroup: greedy matching in key order, products
+//               (doreact, smolreact.c:2662-2901), molsort compaction and the
+//               dead-list product append order
 //
 // Arithmetic follows the reference operation by operation and the library is
 // built with -ffp-contract=off, so FP64 positions match the CPU path bit for
@@ -52,39 +58,43 @@ constexpr int kLdsSort = 2048;  // entries sorted in LDS by k_resolve
 
 struct Layout {
   int nlist;
-  int captot;
-  int base[kMaxList + 1];
+  int captot;               // list slots (sum of list capacities)
+  int reccap;               // records per buffer
+  int base[kMaxList + 1];   // first slot of each list
 };
 
-struct Mols {
+// One buffer of molecule records.
+struct Recs {
   double *px, *py, *pz;
-  int* ident;
+  int* ident;     // species, 0 for a record whose molecule died
   uint64_t* serno;
-  int* box;
-  int* rank;
+  int* lpos;      // position in the reference's live list of its list
+  int* box;       // virtual box (-1: not binned this step)
+  int* rank;      // arrival rank within the box
 };
 
 struct Binned {
   int* count;   // [nlist*nbox+1], zeroed by k_scan1 after reading
   int* part;    // block-local exclusive prefix
   int* boff;    // exclusive prefix of block sums
-  int* bstart;  // flat exclusive prefix: first box-ordered entry of (list, box)
-  int* tmp;     // slots in box order, unordered within a box
-  int* sslot;   // slots in box order, live order within a box
-  int* sbox;
-  float4* spk;  // FP32 position + species bits, box order (pair pre-filter)
+  int* bstart;  // first destination record of (list, box); bstart[nscan-1] = records binned
+  float4* spk;  // FP32 position + species bits of the destination records
 };
 
-// Product record built per event before the list compaction.
 struct Prod {
   int ident;
   int list;
   double x, y, z;
 };
 
+// Per-species screening constants of the pair search, passed by value.
+struct PairConst {
+  uint32_t reactmask[kMaxSpecies];  // partner species with a bimolecular reaction
+  float r2f[kMaxSpecies];           // (largest binding radius + FP32 margin)^2
+};
+
 struct DeviceState {
   EngineConfig cfg;
-  Prod* prods;
   Layout lay;
   Geometry geo;
   DevTables* tab;
@@ -92,9 +102,11 @@ struct DeviceState {
   double* gtab;
   int ngauss;
   uint32_t key0, key1;
-  Mols m;
+  Recs buf[2];
+  int cur;  // buffer holding the current state
+  int* inv; // list slot -> record of the current buffer
   Binned bn;
-  int nscan;   // nlist*nbox + 1
+  int nscan;
   int nscanblk;
   Edge* edges;
   uint64_t* bestkey;
@@ -103,30 +115,24 @@ struct DeviceState {
   int* accepted;
   UniEvent* uniev;
   LogRow* log;
-  uint64_t* sortkey;  // global fallback sort storage
+  uint64_t* sortkey;
   int* sortval;
   int sortcap;
   int* killslot;
+  Prod* prods;
   Control* ctl;
+  PairConst pc;
   double* inj;
   long injcap;
   hipStream_t stream;
-  hipGraphExec_t graph;
-  int graph_ready;
+  hipGraphExec_t graph[2];
+  int graph_ready[2];
   hipEvent_t ev[16];
-  double ktime[8];
-  int ktime_n;
   double last_ms;
 };
 
 // ---------------------------------------------------------------------------
 // device helpers
-
-__device__ __forceinline__ int slot_list(const Layout& L, int t) {
-  int ll = 0;
-  while (ll + 1 < L.nlist && t >= L.base[ll + 1]) ll++;
-  return ll;
-}
 
 __device__ __forceinline__ int pos2box(const Geometry& g, double x, double y, double z) {
   int b = 0;
@@ -166,86 +172,112 @@ __device__ __forceinline__ int atomic_load_i32(int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ int gstart(const Binned& bn, int i) { return bn.bstart[i]; }
-
 // ---------------------------------------------------------------------------
-// k_diffuse: diffusion, walls, box, histogram, first-order reaction coins
+// k_diffuse: diffusion, walls, box, first-order reaction coins, histogram
+
+// Histogram increment aggregated over runs of equal keys in a wavefront:
+// records are stored in box order, so neighbouring lanes usually share a box
+// counter; one atomic per run returns the run's base and each lane takes
+// base + its offset in the run.  Every lane of the wave must call this.
+__device__ __forceinline__ int wave_hist_rank(int* bcount, int key) {
+  int lane = threadIdx.x & 63;
+  int prev = __shfl_up(key, 1);
+  bool start = (lane == 0) || (prev != key);
+  unsigned long long starts = __ballot(start);
+  unsigned long long le = starts & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+  int leader = 63 - __clzll(le);
+  unsigned long long above = (leader == 63) ? 0ull : (starts & ~((2ull << leader) - 1));
+  int next = above ? (__ffsll((long long)above) - 1) : 64;
+  int base = 0;
+  if (lane == leader && key >= 0) base = atomicAdd(&bcount[key], next - leader);
+  base = __shfl(base, leader);
+  return base + (lane - leader);
+}
 
 __global__ __launch_bounds__(kBlock) void k_diffuse(Layout L, Geometry g, const DevTables* __restrict__ tab,
-                                                    Control* ctl, Mols m, int* __restrict__ bcount,
+                                                    Control* ctl, Recs a, int* __restrict__ bcount,
                                                     const double* __restrict__ gtab, int gmask,
                                                     const double* __restrict__ inj, uint32_t key0, uint32_t key1,
                                                     UniEvent* __restrict__ uniev, int maxevents, int nbox) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.captot) return;
-  int ll = slot_list(L, t);
-  int p = t - L.base[ll];
-  if (p >= ctl->nl[ll]) return;
-  int i = m.ident[t];
-  uint64_t ser = m.serno[t];
-  uint32_t step = ctl->step;
-  double pos[3] = {m.px[t], m.py[t], m.pz[t]};
-  if (tab->diffuselist[ll]) {
-    double ds = tab->difstep[i];
-    double gg[3];
-    if (inj) {
-      long k = (long)ctl->pfx[ll] + p;
-      gg[0] = inj[3 * k];
-      gg[1] = inj[3 * k + 1];
-      gg[2] = inj[3 * k + 2];
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  int hkey = -1;  // histogram key (list*nbox + box), -1 when not binned
+  if (r < ctl->nrec) {
+    int i = a.ident[r];
+    if (i == 0) {
+      a.box[r] = -1;
     } else {
-      smolrng4 r = smolrng_mol(key0, key1, ser, step, SMOLRNG_DIFFUSE, 0);
-      gg[0] = gtab[r.v[0] & (uint32_t)gmask];
-      gg[1] = gtab[r.v[1] & (uint32_t)gmask];
-      gg[2] = gtab[r.v[2] & (uint32_t)gmask];
-    }
-    pos[0] += ds * gg[0];
-    pos[1] += ds * gg[1];
-    pos[2] += ds * gg[2];
-    apply_walls(g, pos);
-    m.px[t] = pos[0];
-    m.py[t] = pos[1];
-    m.pz[t] = pos[2];
-  }
-  int b = pos2box(g, pos[0], pos[1], pos[2]);
-  m.box[t] = b;
-  m.rank[t] = atomicAdd(&bcount[ll * nbox + b], 1);
-  // unireact: coin flips in table order, first success reacts
-  int nu = tab->nuni[i];
-  if (nu > 0) {
-    smolrng4 rb;
-    int blk = -1;
-    for (int j = 0; j < nu; j++) {
-      int r = tab->uni[i][j];
-      const DevRxn& rx = tab->rxn1[r];
-      if (!rx.permit_soln) continue;
-      if (blk != j / 4) {
-        blk = j / 4;
-        rb = smolrng_mol(key0, key1, ser, step, SMOLRNG_UNIRXN, (uint32_t)blk);
-      }
-      if (smolrng_u01(rb.v[j & 3]) < rx.prob) {
-        int e = atomicAdd(&ctl->nuni, 1);
-        if (e < maxevents) {
-          uniev[e].slot = t;
-          uniev[e].r = r;
+      int ll = tab->listof[i];
+      int lp = a.lpos[r];
+      uint64_t ser = a.serno[r];
+      uint32_t step = ctl->step;
+      double pos[3] = {a.px[r], a.py[r], a.pz[r]};
+      if (tab->diffuselist[ll]) {
+        double ds = tab->difstep[i];
+        double gg[3];
+        if (inj) {
+          long k = (long)ctl->pfx[ll] + lp;
+          gg[0] = inj[3 * k];
+          gg[1] = inj[3 * k + 1];
+          gg[2] = inj[3 * k + 2];
         } else {
-          atomicOr(&ctl->err, kErrEventOverflow);
+          smolrng4 rn = smolrng_mol(key0, key1, ser, step, SMOLRNG_DIFFUSE, 0);
+          gg[0] = gtab[rn.v[0] & (uint32_t)gmask];
+          gg[1] = gtab[rn.v[1] & (uint32_t)gmask];
+          gg[2] = gtab[rn.v[2] & (uint32_t)gmask];
         }
-        m.ident[t] = 0;
-        break;
+        pos[0] += ds * gg[0];
+        pos[1] += ds * gg[1];
+        pos[2] += ds * gg[2];
+        apply_walls(g, pos);
+        a.px[r] = pos[0];
+        a.py[r] = pos[1];
+        a.pz[r] = pos[2];
       }
+      int b = pos2box(g, pos[0], pos[1], pos[2]);
+      hkey = ll * nbox + b;
+      // unireact: coin flips in table order, first success reacts; a
+      // molecule that reacts takes no part in the pair search, so it is not
+      // binned
+      int nu = tab->nuni[i];
+      if (nu > 0) {
+        smolrng4 rb;
+        int blk = -1;
+        for (int j = 0; j < nu; j++) {
+          int rx = tab->uni[i][j];
+          const DevRxn& d = tab->rxn1[rx];
+          if (!d.permit_soln) continue;
+          if (blk != j / 4) {
+            blk = j / 4;
+            rb = smolrng_mol(key0, key1, ser, step, SMOLRNG_UNIRXN, (uint32_t)blk);
+          }
+          if (smolrng_u01(rb.v[j & 3]) < d.prob) {
+            int e = atomicAdd(&ctl->nuni, 1);
+            if (e < maxevents) {
+              uniev[e].rec = r;
+              uniev[e].slot = L.base[ll] + lp;
+              uniev[e].r = rx;
+            } else {
+              atomicOr(&ctl->err, kErrEventOverflow);
+            }
+            hkey = -1;
+            break;
+          }
+        }
+      }
+      a.box[r] = hkey < 0 ? -1 : b;
     }
   }
+  int rank = wave_hist_rank(bcount, hkey);
+  if (hkey >= 0) a.rank[r] = rank;
 }
 
 // ---------------------------------------------------------------------------
-// box-list scan: part[i] = exclusive prefix within a 4096-entry tile
+// histogram scan: part[i] = exclusive prefix within a 4096-entry tile
 
 __global__ __launch_bounds__(kScanBlock) void k_scan1(int* __restrict__ count, int* __restrict__ part,
                                                       int* __restrict__ bsum, int n) {
   __shared__ int s[kScanBlock];
-  int tile0 = blockIdx.x * kScanTile;
-  int base = tile0 + threadIdx.x * kScanPer;
+  int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
   int v[kScanPer];
   int sum = 0;
 #pragma unroll
@@ -308,32 +340,42 @@ __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ part, 
   if (i < n) bstart[i] = part[i] + boff[i / kScanTile];
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(Layout L, const Control* ctl, Mols m, Binned bn, int nbox) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= L.captot) return;
-  int ll = slot_list(L, t);
-  if (t - L.base[ll] >= ctl->nl[ll]) return;
-  int u = gstart(bn, ll * nbox + m.box[t]) + m.rank[t];
-  bn.tmp[u] = t;
+// counting-sort scatter of the binned records into the destination buffer
+// The inverse map (list slot -> record) is written only for the last
+// `window` slots of each list: compaction in k_resolve reads tail slots only,
+// and the kills of one step never exceed the window.
+__global__ __launch_bounds__(kBlock) void k_scatter(Layout L, const DevTables* __restrict__ tab,
+                                                    const Control* __restrict__ ctl, Recs a, Recs b, Binned bn,
+                                                    int* __restrict__ inv, int nbox, int window) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ctl->nrec) return;
+  int bx = a.box[r];
+  if (bx < 0) return;
+  int i = a.ident[r];
+  int ll = tab->listof[i];
+  int dst = bn.bstart[ll * nbox + bx] + a.rank[r];
+  double x = a.px[r], y = a.py[r], z = a.pz[r];
+  int lp = a.lpos[r];
+  b.px[dst] = x;
+  b.py[dst] = y;
+  b.pz[dst] = z;
+  b.ident[dst] = i;
+  b.serno[dst] = a.serno[r];
+  b.lpos[dst] = lp;
+  b.box[dst] = bx;
+  bn.spk[dst] = make_float4((float)x, (float)y, (float)z, __int_as_float(i));
+  if (lp >= ctl->nl[ll] - window) inv[L.base[ll] + lp] = dst;
 }
 
-// Stable order inside each box: rank an entry among its box by slot, which
-// is the live-list order of the list segment.
-__global__ __launch_bounds__(kBlock) void k_boxsort(Layout L, const Control* ctl, Mols m, Binned bn, int nbox,
-                                                    int nscan) {
-  int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= gstart(bn, nscan - 1)) return;  // live total
-  int t = bn.tmp[u];
-  int b = m.box[t];
-  int ll = slot_list(L, t);
-  int i0 = gstart(bn, ll * nbox + b);
-  int i1 = gstart(bn, ll * nbox + b + 1);
-  int r = 0;
-  for (int k = i0; k < i1; k++) r += (bn.tmp[k] < t);
-  int dst = i0 + r;
-  bn.sslot[dst] = t;
-  bn.sbox[dst] = b;
-  bn.spk[dst] = make_float4((float)m.px[t], (float)m.py[t], (float)m.pz[t], __int_as_float(m.ident[t]));
+// Full inverse map of the current buffer, for reading the state back.
+__global__ __launch_bounds__(kBlock) void k_rebuild_inv(Layout L, const DevTables* __restrict__ tab,
+                                                        const Control* __restrict__ ctl, Recs a,
+                                                        int* __restrict__ inv) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ctl->nrec) return;
+  int i = a.ident[r];
+  if (i == 0) return;
+  inv[L.base[tab->listof[i]] + a.lpos[r]] = r;
 }
 
 // ---------------------------------------------------------------------------
@@ -374,13 +416,13 @@ __device__ __forceinline__ int first_reaction(const DevTables* tab, int i1, int 
 }
 
 // Exact FP64 test of one candidate (the reference's dist2 in its operation
-// order, smolreact.c:3238-3240 and wallcalcdist2 smolwall.c:100-116) and the
-// edge for the first reaction that accepts it.
-__device__ __forceinline__ void exact_pair(const Geometry& g, const DevTables* tab, Control* ctl, const Mols& m,
+// order, smolreact.c:3238-3240, and wallcalcdist2, smolwall.c:100-116) and
+// the edge for the first reaction that accepts it.
+__device__ __forceinline__ void exact_pair(const Geometry& g, const DevTables* tab, Control* ctl, const Recs& b,
                                            Edge* edges, int maxedges, uint32_t key0, uint32_t key1, uint32_t step,
                                            int t1, const double* p1, int i1, uint64_t ser1, int t2, int i2, int wrap,
                                            uint64_t key) {
-  double p2[3] = {m.px[t2], m.py[t2], m.pz[t2]};
+  double p2[3] = {b.px[t2], b.py[t2], b.pz[t2]};
   double dist2 = 0;
   if (wrap) {
 #pragma unroll
@@ -396,129 +438,125 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevTables* t
 #pragma unroll
     for (int d = 0; d < 3; d++) dist2 += (p1[d] - p2[d]) * (p1[d] - p2[d]);
   }
-  int r = first_reaction(tab, i1, i2, dist2, key0, key1, ser1, m.serno[t2], step);
+  int r = first_reaction(tab, i1, i2, dist2, key0, key1, ser1, b.serno[t2], step);
   if (r >= 0) emit_edge(ctl, edges, maxedges, key, t1, t2, r, wrap);
 }
 
-// One thread per molecule in box order.  Candidates are screened in FP32 from
-// one packed 16-byte load each; only pairs within binding radius plus the
-// rounding margin reach the exact FP64 test.
+// Pair search, one thread per binned record (box order), molecule m1.
+// The stencil is walked as z-runs: in box order the boxes (x', y', z-1..z+1)
+// of one neighbor column are consecutive records, so each column is one or
+// two contiguous ranges (two when z wraps).  Half stencil (same list): columns
+// (-1,-1),(-1,0),(-1,1),(0,-1) with dz -1..1 and column (0,0) with dz -1..0;
+// full stencil (two lists): all nine columns, dz -1..1.  dz = 0 in column
+// (0,0) is the molecule's own box, bireact's same-box pass.  Candidates pass
+// an FP32 screen, then the reference's FP64 test; only then is their stencil
+// index looked up for the edge key.
 __global__ __launch_bounds__(kBlock) void k_pairs(Layout L, Geometry g, const DevTables* __restrict__ tab,
-                                                  Control* ctl, Mols m, Binned bn, int nbox, int nscan,
-                                                  Edge* __restrict__ edges, int maxedges, uint32_t key0,
+                                                  PairConst pc, Control* ctl, Recs b, Binned bn, int nbox,
+                                                  int nscan, Edge* __restrict__ edges, int maxedges, uint32_t key0,
                                                   uint32_t key1) {
-  int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= bn.bstart[nscan - 1]) return;  // live total
-  float4 me = bn.spk[u];
-  int i1 = __float_as_int(me.w);
-  if (i1 == 0) return;
-  uint32_t mask = tab->reactmask[i1];
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= bn.bstart[nscan - 1]) return;  // records binned
+  const float4 me = bn.spk[u];
+  const int i1 = __float_as_int(me.w);
+  const uint32_t mask = pc.reactmask[i1];
   if (!mask) return;
-  int t1 = bn.sslot[u];
-  int ll1 = slot_list(L, t1);
-  int nl = tab->nlist;
-  int b = bn.sbox[u];
-  float r2f;
-  if (g.prefilter_margin >= 0) {
-    double rr = sqrt(tab->maxbind2[i1]) + g.prefilter_margin;
-    r2f = (float)(rr * rr);
-  } else {
-    r2f = 3.0e38f;
-  }
-  uint64_t ser1 = 0;
+  const float r2f = pc.r2f[i1];
+  const int ll1 = tab->listof[i1];
+  const int nl = tab->nlist;
+  const int bb = b.box[u];
+  const int sz = g.side[2];
+  const int bx = bb / (g.side[1] * sz);
+  const int by = (bb / sz) % g.side[1];
+  const int bz = bb % sz;
+  const int lp1 = b.lpos[u];
+  const uint32_t step = ctl->step;
   double p1[3];
+  uint64_t ser1 = 0;
   bool loaded = false;
-  uint32_t step = ctl->step;
-  int p1pos = t1 - L.base[ll1];
-  int bx = b / (g.side[1] * g.side[2]);
-  int by = (b / g.side[2]) % g.side[1];
-  int bz = b % g.side[2];
-  float wf[3] = {(float)g.width[0], (float)g.width[1], (float)g.width[2]};
   for (int ll2 = ll1; ll2 < nl; ll2++) {
     if (!tab->rxnmol2[ll1 * kMaxList + ll2]) continue;
-    uint64_t pairbits = (uint64_t)tab->pairindex[ll1 * kMaxList + ll2] << 59;
-    // same box (pass 0): for ll1==ll2 only entries before m1
-    {
-      int j0 = gstart(bn, ll2 * nbox + b);
-      int jend = (ll1 == ll2) ? u : gstart(bn, ll2 * nbox + b + 1);
-      for (int k = j0; k < jend; k++) {
-        float4 o = bn.spk[k];
-        int i2 = __float_as_int(o.w);
-        if (!((mask >> i2) & 1u)) continue;
-        float dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
-        float d2 = dx * dx + dy * dy + dz * dz;
-        if (d2 > r2f) continue;
-        if (!loaded) {
-          p1[0] = m.px[t1];
-          p1[1] = m.py[t1];
-          p1[2] = m.pz[t1];
-          ser1 = m.serno[t1];
-          loaded = true;
-        }
-        uint64_t key = pairbits | ((uint64_t)p1pos << 32) | (uint64_t)(k - j0);
-        exact_pair(g, tab, ctl, m, edges, maxedges, key0, key1, step, t1, p1, i1, ser1, bn.sslot[k], i2, 0, key);
-      }
-    }
-    // neighbor boxes (pass 1): 3^3 stencil in add2indx3ZV order, half for ll1==ll2
-    for (int nb = 0; nb < 27; nb++) {
-      if (nb == 13) continue;
-      if (ll1 == ll2 && nb > 13) break;
-      int c0 = bx + nb / 9 - 1, c1 = by + (nb / 3) % 3 - 1, c2 = bz + nb % 3 - 1;
-      int wrap = 0;
-      float sh[3] = {0.f, 0.f, 0.f};
-      if (c0 < 0) {
+    const bool half = (ll1 == ll2);
+    const uint64_t key1base = ((uint64_t)tab->pairindex[ll1 * kMaxList + ll2] << 59) | ((uint64_t)lp1 << 32);
+    const int ncol = half ? 5 : 9;
+    for (int c = 0; c < ncol; c++) {
+      int dx = half ? (c < 3 ? -1 : 0) : c / 3 - 1;
+      int dy = half ? (c < 3 ? c - 1 : c - 4) : c % 3 - 1;
+      int dzhi = (half && c == 4) ? 0 : 1;
+      int nx = bx + dx, ny = by + dy, cw = 0;
+      float shx = 0.f, shy = 0.f;
+      if (nx < 0) {
         if (!g.wrapflag[0]) continue;
-        c0 = g.side[0] - 1;
-        wrap += 1;
-        sh[0] = -wf[0];
-      } else if (c0 >= g.side[0]) {
+        nx = g.side[0] - 1;
+        cw |= 1;
+        shx = -(float)g.width[0];
+      } else if (nx >= g.side[0]) {
         if (!g.wrapflag[0]) continue;
-        c0 = 0;
-        wrap += 2;
-        sh[0] = wf[0];
+        nx = 0;
+        cw |= 2;
+        shx = (float)g.width[0];
       }
-      if (c1 < 0) {
+      if (ny < 0) {
         if (!g.wrapflag[1]) continue;
-        c1 = g.side[1] - 1;
-        wrap += 1 << 2;
-        sh[1] = -wf[1];
-      } else if (c1 >= g.side[1]) {
+        ny = g.side[1] - 1;
+        cw |= 1 << 2;
+        shy = -(float)g.width[1];
+      } else if (ny >= g.side[1]) {
         if (!g.wrapflag[1]) continue;
-        c1 = 0;
-        wrap += 2 << 2;
-        sh[1] = wf[1];
+        ny = 0;
+        cw |= 2 << 2;
+        shy = (float)g.width[1];
       }
-      if (c2 < 0) {
-        if (!g.wrapflag[2]) continue;
-        c2 = g.side[2] - 1;
-        wrap += 1 << 4;
-        sh[2] = -wf[2];
-      } else if (c2 >= g.side[2]) {
-        if (!g.wrapflag[2]) continue;
-        c2 = 0;
-        wrap += 2 << 4;
-        sh[2] = wf[2];
-      }
-      int b2 = (c0 * g.side[1] + c1) * g.side[2] + c2;
-      int j0 = gstart(bn, ll2 * nbox + b2);
-      int j1 = gstart(bn, ll2 * nbox + b2 + 1);
-      for (int k = j0; k < j1; k++) {
-        if (k == u) continue;  // a box that is its own periodic neighbor
-        float4 o = bn.spk[k];
-        int i2 = __float_as_int(o.w);
-        if (!((mask >> i2) & 1u)) continue;
-        float dx = (o.x + sh[0]) - me.x, dy = (o.y + sh[1]) - me.y, dz = (o.z + sh[2]) - me.z;
-        float d2 = dx * dx + dy * dy + dz * dz;
-        if (d2 > r2f) continue;
-        if (!loaded) {
-          p1[0] = m.px[t1];
-          p1[1] = m.py[t1];
-          p1[2] = m.pz[t1];
-          ser1 = m.serno[t1];
-          loaded = true;
+      const int colb = ll2 * nbox + (nx * g.side[1] + ny) * sz;
+      // z segments: unwrapped run [zlo, zhi] plus wrapped single boxes
+      int zlo = bz - 1, zhi = bz + dzhi;
+      for (int seg = 0; seg < 3; seg++) {
+        int za, zb2, zw = 0, dzfix = -2;
+        float shz = 0.f;
+        if (seg == 0) {  // wrapped box below
+          if (zlo >= 0) continue;
+          if (!g.wrapflag[2]) continue;
+          za = zb2 = sz - 1;
+          zw = 1 << 4;
+          shz = -(float)g.width[2];
+          dzfix = -1;
+        } else if (seg == 1) {  // unwrapped run
+          za = zlo < 0 ? 0 : zlo;
+          zb2 = zhi >= sz ? sz - 1 : zhi;
+          if (za > zb2) continue;
+        } else {  // wrapped box above
+          if (zhi < sz) continue;
+          if (!g.wrapflag[2]) continue;
+          za = zb2 = 0;
+          zw = 2 << 4;
+          shz = (float)g.width[2];
+          dzfix = 1;
         }
-        uint64_t key = (1ull << 63) | pairbits | ((uint64_t)p1pos << 32) | ((uint64_t)nb << 27) | (uint64_t)(k - j0);
-        exact_pair(g, tab, ctl, m, edges, maxedges, key0, key1, step, t1, p1, i1, ser1, bn.sslot[k], i2, wrap, key);
+        const int k0 = bn.bstart[colb + za];
+        const int k1 = bn.bstart[colb + zb2 + 1];
+        const int wrap = cw | zw;
+        for (int k = k0; k < k1; k++) {
+          float4 o = bn.spk[k];
+          int i2 = __float_as_int(o.w);
+          float dxf = (o.x + shx) - me.x, dyf = (o.y + shy) - me.y, dzf = (o.z + shz) - me.z;
+          float d2 = dxf * dxf + dyf * dyf + dzf * dzf;
+          if (d2 > r2f || !((mask >> i2) & 1u) || k == u) continue;
+          int dz = dzfix > -2 ? dzfix : (b.box[k] % sz) - bz;
+          int nbi = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+          int lp2 = b.lpos[k];
+          if (nbi == 13 && half && lp2 >= lp1) continue;
+          if (!loaded) {
+            p1[0] = b.px[u];
+            p1[1] = b.py[u];
+            p1[2] = b.pz[u];
+            ser1 = b.serno[u];
+            loaded = true;
+          }
+          uint64_t key = nbi == 13 ? (key1base | (uint64_t)lp2)
+                                   : ((1ull << 63) | key1base | ((uint64_t)nbi << 27) | (uint64_t)lp2);
+          exact_pair(g, tab, ctl, b, edges, maxedges, key0, key1, step, u, p1, i1, ser1, k, i2,
+                     nbi == 13 ? 0 : wrap, key);
+        }
       }
     }
   }
@@ -535,11 +573,11 @@ __device__ void wg_bitonic(KP key, VP val, int n) {
       for (int i = threadIdx.x; i < n; i += blockDim.x) {
         int ixj = i ^ j;
         if (ixj > i) {
-          uint64_t a = key[i], c = key[ixj];
+          uint64_t x = key[i], c = key[ixj];
           bool up = ((i & k) == 0);
-          if ((a > c) == up) {
+          if ((x > c) == up) {
             key[i] = c;
-            key[ixj] = a;
+            key[ixj] = x;
             int tv = val[i];
             val[i] = val[ixj];
             val[ixj] = tv;
@@ -551,7 +589,7 @@ __device__ void wg_bitonic(KP key, VP val, int n) {
   }
 }
 
-__device__ int pow2ceil(int n) {
+__host__ __device__ inline int pow2ceil(int n) {
   int p = 1;
   while (p < n) p <<= 1;
   return p;
@@ -604,7 +642,11 @@ struct ResolveArgs {
   Geometry g;
   const DevTables* tab;
   Control* ctl;
-  Mols m;
+  Recs a;  // source buffer (first-order reactants)
+  Recs b;  // destination buffer (binned molecules, products)
+  int* inv;
+  Binned bn;
+  int nscan;
   Edge* edges;
   int maxedges;
   uint64_t* bestkey;
@@ -614,11 +656,11 @@ struct ResolveArgs {
   UniEvent* uniev;
   int maxevents;
   LogRow* log;
-  uint64_t* sk;  // global sort scratch, capacity sortcap
+  uint64_t* sk;
   int* sv;
   int sortcap;
   int* killslot;
-  struct Prod* prods;
+  Prod* prods;
   uint32_t key0, key1;
 };
 
@@ -632,12 +674,13 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   const int NT = blockDim.x;
   Control* ctl = a.ctl;
   const DevTables* tab = a.tab;
-  Mols m = a.m;
+  Recs B = a.b;
   int E = ctl->nedges;
   if (E > a.maxedges) E = a.maxedges;
   int U = ctl->nuni;
   if (U > a.maxevents) U = a.maxevents;
   uint32_t step = ctl->step;
+  int nbinned = a.bn.bstart[a.nscan - 1];
 
   // ---- greedy matching: accept an edge when it is the smallest live key at
   // both of its molecules; this is the lexicographically-first maximal
@@ -683,14 +726,12 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     }
     __syncthreads();
   }
-  int B = s_nacc;
+  int B2 = s_nacc;
   __syncthreads();
 
-  // ---- order the events: first-order reactions in live order (slot order),
-  // then bimolecular ones in edge-key order
-  // the sort regions must fit the scratch arrays: events at [0,U) and
-  // [U,U+pow2(B)), kills at [0,pow2(U+2B))
-  if (U + pow2ceil(B) > a.sortcap || pow2ceil(U + 2 * B) > a.sortcap) {
+  // ---- order the events: first-order reactions in live-list order, then
+  // bimolecular ones in edge-key order
+  if (U + pow2ceil(B2) > a.sortcap || pow2ceil(U + 2 * B2) > a.sortcap) {
     if (tid == 0) atomicOr(&ctl->err, kErrSortOverflow);
     return;
   }
@@ -700,17 +741,16 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   }
   __syncthreads();
   wg_sort(a.sk, a.sv, U, lk, lv);
-  // event order array in sv[0..U) (uni), then accepted edges sorted into sv[U..U+B)
-  for (int i = tid; i < B; i += NT) {
+  for (int i = tid; i < B2; i += NT) {
     a.sk[U + i] = a.edges[a.accepted[i]].key;
     a.sv[U + i] = a.accepted[i];
   }
   __syncthreads();
-  wg_sort(a.sk + U, a.sv + U, B, lk, lv);
-  int NE = U + B;
+  wg_sort(a.sk + U, a.sv + U, B2, lk, lv);
+  int NE = U + B2;
 
-  // ---- product counts and serial numbers in creation order
-  // each thread handles a contiguous chunk of events so the scan is in order
+  // ---- product counts and serial numbers in creation order (each thread
+  // takes a contiguous chunk of events so the scan follows creation order)
   int per = (NE + NT - 1) / NT;
   int e0 = tid * per, e1 = min(NE, e0 + per);
   int myprod = 0;
@@ -720,38 +760,34 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   }
   int K;
   int pbase = wg_scan(myprod, s_scan, &K);
-  int D = U + 2 * B;  // reactants killed, in molsort's kill count
+  int D = U + 2 * B2;  // reactants killed = molsort's kill count
   uint64_t ser0 = ctl->serno_next;
 
-  // ---- build products (positions, species, lists) and kill reactants
-  Prod* prods = a.prods;
+  // ---- products (positions, species, lists) and reactant kills
   int pc = pbase;
   for (int e = e0; e < e1; e++) {
     double rxnpos[3];
     const DevRxn* rx;
     uint64_t sA, sB = 0;
-    int order;
-    int ridx;
+    int order, ridx;
     if (e < U) {
       UniEvent ev = a.uniev[a.sv[e]];
       rx = &tab->rxn1[ev.r];
       ridx = ev.r;
       order = 1;
-      int t = ev.slot;
-      rxnpos[0] = m.px[t];
-      rxnpos[1] = m.py[t];
-      rxnpos[2] = m.pz[t];
-      sA = m.serno[t];
-      a.killslot[e] = t;
-      a.killed[t] = 1;
+      rxnpos[0] = a.a.px[ev.rec];
+      rxnpos[1] = a.a.py[ev.rec];
+      rxnpos[2] = a.a.pz[ev.rec];
+      sA = a.a.serno[ev.rec];
+      a.killslot[e] = ev.slot;
     } else {
       Edge ed = a.edges[a.sv[e]];
       rx = &tab->rxn2[ed.r];
       ridx = ed.r;
       order = 2;
-      double p1[3] = {m.px[ed.s1], m.py[ed.s1], m.pz[ed.s1]};
-      double p2[3] = {m.px[ed.s2], m.py[ed.s2], m.pz[ed.s2]};
-      int id1 = m.ident[ed.s1], id2 = m.ident[ed.s2];
+      double p1[3] = {B.px[ed.s1], B.py[ed.s1], B.pz[ed.s1]};
+      double p2[3] = {B.px[ed.s2], B.py[ed.s2], B.pz[ed.s2]};
+      int id1 = B.ident[ed.s1], id2 = B.ident[ed.s2];
       if (ed.wrap && !rx->confspread) {  // morebireact: move the faster molecule next to its partner
         double v[3];
 #pragma unroll
@@ -783,10 +819,11 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       double x = (dc1 == 0 && dc2 == 0) ? 0.5 : dc2 / (dc1 + dc2);
 #pragma unroll
       for (int d = 0; d < 3; d++) rxnpos[d] = x * pA[d] + (1.0 - x) * pB[d];
-      sA = m.serno[swap ? ed.s2 : ed.s1];
-      sB = m.serno[swap ? ed.s1 : ed.s2];
-      a.killslot[U + 2 * (e - U)] = ed.s1;
-      a.killslot[U + 2 * (e - U) + 1] = ed.s2;
+      sA = B.serno[swap ? ed.s2 : ed.s1];
+      sB = B.serno[swap ? ed.s1 : ed.s2];
+      int k2 = U + 2 * (e - U);
+      a.killslot[k2] = a.L.base[tab->listof[id1]] + B.lpos[ed.s1];
+      a.killslot[k2 + 1] = a.L.base[tab->listof[id2]] + B.lpos[ed.s2];
     }
     double uvec[3] = {0, 0, 0};
     if (rx->rotate) smolrng_unitvec(a.key0, a.key1, sA, step, 3, uvec);
@@ -805,11 +842,9 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       pr.x = pp[0];
       pr.y = pp[1];
       pr.z = pp[2];
-      prods[pc] = pr;
+      a.prods[pc] = pr;
       pc++;
     }
-    // reaction log
-    int li = e;
     LogRow row;
     row.order = order;
     row.r = ridx;
@@ -818,14 +853,13 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     int pfirst = pc - rx->nprod;
     row.prd[0] = rx->nprod > 0 ? ser0 + (uint64_t)pfirst : 0;
     row.prd[1] = rx->nprod > 1 ? ser0 + (uint64_t)pfirst + 1 : 0;
-    a.log[li] = row;
+    a.log[e] = row;
   }
   __syncthreads();
 
-  // ---- molsort: sort the killed slots, fill holes from the list tails
-  // (swap-from-tail compaction), then append products in the dead-list
-  // order the reference produces: creation order reversed and rotated by
-  // the number of kills.
+  // ---- molsort: sort the killed list slots, fill holes from the list tails
+  // (swap-from-tail compaction), then append products in the dead-list order
+  // the reference produces: creation order reversed, rotated by the kills.
   for (int i = tid; i < D; i += NT) {
     a.sk[i] = (uint64_t)a.killslot[i];
     a.sv[i] = 0;
@@ -835,7 +869,6 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   if (tid < a.L.nlist) {
     int ll = tid;
     int lo = a.L.base[ll], hi = a.L.base[ll + 1];
-    // range of kills in this list (binary searches on the sorted slots)
     int l = 0, h = D;
     while (l < h) {
       int mid = (l + h) >> 1;
@@ -855,60 +888,65 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     s_nf[ll] = ctl->nl[ll] - (l - k0);
   }
   __syncthreads();
+  // bimolecular reactants stay in the destination buffer as dead records
+  for (int i = tid; i < B2; i += NT) {
+    Edge ed = a.edges[a.accepted[i]];
+    B.ident[ed.s1] = 0;
+    B.ident[ed.s2] = 0;
+    a.killed[ed.s1] = 0;
+    a.killed[ed.s2] = 0;
+  }
   for (int ll = 0; ll < a.L.nlist; ll++) {
     int k0 = s_k0[ll], k1 = s_k1[ll], nk = k1 - k0;
     int nlc = s_nl[ll], nf = s_nf[ll];
     int lo = a.L.base[ll];
     for (int q = tid; q < nk; q += NT) {
-      int t = lo + nf + q;  // tail position nf+q
-      // killed?  binary search among this list's kills
+      int t = lo + nf + q;  // tail slot
       int l = k0, h = k1;
       while (l < h) {
         int mid = (l + h) >> 1;
         if ((int)a.sk[mid] < t) l = mid + 1;
         else h = mid;
       }
-      bool dead = (l < k1 && (int)a.sk[l] == t);
-      if (dead) continue;
-      int above = k1 - l;  // kills at positions > t (t itself not killed)
+      if (l < k1 && (int)a.sk[l] == t) continue;  // killed
+      int above = k1 - l;                          // kills above t
       int rank = (nlc - 1 - (nf + q)) - above;
       int dst = (int)a.sk[k0 + rank];
-      m.px[dst] = m.px[t];
-      m.py[dst] = m.py[t];
-      m.pz[dst] = m.pz[t];
-      m.ident[dst] = m.ident[t];
-      m.serno[dst] = m.serno[t];
-      m.box[dst] = m.box[t];
+      int rec = a.inv[t];
+      B.lpos[rec] = dst - lo;
+      a.inv[dst] = rec;
     }
   }
   __syncthreads();
-  // clear kill marks (slots are reused)
-  for (int i = tid; i < D; i += NT) a.killed[a.killslot[i]] = 0;
   // product append order: Q[j] = creation index K-1-((j+D) mod K)
   if (tid < a.L.nlist) s_pl[tid] = 0;
   __syncthreads();
+  int overflow = 0;
   for (int j0 = 0; j0 < K; j0 += NT) {
     int j = j0 + tid;
     int c = -1, ll = -1;
     if (j < K) {
       c = K - 1 - (int)(((long)j + D) % K);
-      ll = prods[c].list;
+      ll = a.prods[c].list;
     }
     for (int l2 = 0; l2 < a.L.nlist; l2++) {
       int tot;
       int pre = wg_scan((ll == l2) ? 1 : 0, s_scan, &tot);
       if (ll == l2) {
-        int dst = a.L.base[l2] + s_nf[l2] + s_pl[l2] + pre;
-        if (dst < a.L.base[l2 + 1]) {
-          Prod pr = prods[c];
-          m.px[dst] = pr.x;
-          m.py[dst] = pr.y;
-          m.pz[dst] = pr.z;
-          m.ident[dst] = pr.ident;
-          m.serno[dst] = ser0 + (uint64_t)c;
-          m.box[dst] = pos2box(a.g, pr.x, pr.y, pr.z);
+        int slot = a.L.base[l2] + s_nf[l2] + s_pl[l2] + pre;
+        int rec = nbinned + c;
+        if (slot < a.L.base[l2 + 1] && rec < a.L.reccap) {
+          Prod pr = a.prods[c];
+          B.px[rec] = pr.x;
+          B.py[rec] = pr.y;
+          B.pz[rec] = pr.z;
+          B.ident[rec] = pr.ident;
+          B.serno[rec] = ser0 + (uint64_t)c;
+          B.lpos[rec] = slot - a.L.base[l2];
+          B.box[rec] = pos2box(a.g, pr.x, pr.y, pr.z);
+          a.inv[slot] = rec;
         } else {
-          atomicOr(&ctl->err, kErrCapacity);
+          overflow = 1;
         }
       }
       __syncthreads();
@@ -916,7 +954,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       __syncthreads();
     }
   }
-  __syncthreads();
+  if (overflow) atomicOr(&ctl->err, kErrCapacity);
   // ---- species counts, list lengths, counters
   for (int e = tid; e < NE; e += NT) {
     const DevRxn& rx = e < U ? tab->rxn1[a.uniev[a.sv[e]].r] : tab->rxn2[a.edges[a.sv[e]].r];
@@ -938,8 +976,11 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       ctl->pfx[ll] = run;
       if (tab->diffuselist[ll]) run += nlnew;
     }
+    int nrec = nbinned + K;
+    ctl->nrec = nrec < a.L.reccap ? nrec : a.L.reccap;
+    ctl->nbinned = nbinned;
     ctl->serno_next = ser0 + (uint64_t)K;
-    ctl->naccept = B;
+    ctl->naccept = B2;
     ctl->nlog = NE;
     if (E > ctl->maxedges_seen) ctl->maxedges_seen = E;
     ctl->rounds = rounds;
@@ -963,10 +1004,18 @@ static int dalloc(T** p, size_t n) {
   return 0;
 }
 
-static int pow2ceil_h(int n) {
-  int p = 1;
-  while (p < n) p <<= 1;
-  return p;
+static int alloc_recs(Recs* r, size_t n) {
+  int ok = 0;
+  ok |= dalloc(&r->px, n) | dalloc(&r->py, n) | dalloc(&r->pz, n);
+  ok |= dalloc(&r->ident, n) | dalloc(&r->serno, n) | dalloc(&r->lpos, n) | dalloc(&r->box, n) |
+        dalloc(&r->rank, n);
+  return ok;
+}
+
+static void free_recs(Recs* r) {
+  void* ptrs[] = {r->px, r->py, r->pz, r->ident, r->serno, r->lpos, r->box, r->rank};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
 }
 
 DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const Geometry& geo, const double* gtable,
@@ -987,58 +1036,66 @@ DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const 
   }
   for (int ll = cfg.nlist; ll <= kMaxList; ll++) s->lay.base[ll] = run;
   s->lay.captot = run;
-  int captot = run;
   s->nscan = cfg.nlist * geo.nbox + 1;
   s->nscanblk = (s->nscan + kScanTile - 1) / kScanTile;
-  int ok = 0;
   if (s->nscanblk > kScanBlock * 8) {
     snprintf(err, errlen, "too many virtual boxes for the scan (%d)", s->nscan);
     delete s;
     return nullptr;
   }
-  s->sortcap = pow2ceil_h(2 * cfg.maxevents + 16);
-  ok |= dalloc(&s->m.px, captot) | dalloc(&s->m.py, captot) | dalloc(&s->m.pz, captot);
-  ok |= dalloc(&s->m.ident, captot) | dalloc(&s->m.serno, captot) | dalloc(&s->m.box, captot) |
-        dalloc(&s->m.rank, captot);
+  s->sortcap = pow2ceil(2 * cfg.maxevents + 16);
+  s->lay.reccap = run + 2 * s->sortcap;
+  size_t rc = (size_t)s->lay.reccap;
+  int ok = 0;
+  ok |= alloc_recs(&s->buf[0], rc) | alloc_recs(&s->buf[1], rc);
+  ok |= dalloc(&s->inv, (size_t)s->lay.captot);
   ok |= dalloc(&s->bn.count, s->nscan) | dalloc(&s->bn.part, s->nscan) | dalloc(&s->bn.boff, s->nscanblk) |
-        dalloc(&s->bn.bstart, s->nscan);
-  ok |= dalloc(&s->bn.tmp, captot) | dalloc(&s->bn.sslot, captot) | dalloc(&s->bn.sbox, captot) |
-        dalloc(&s->bn.spk, captot);
+        dalloc(&s->bn.bstart, s->nscan) | dalloc(&s->bn.spk, rc);
   ok |= dalloc(&s->edges, cfg.maxedges) | dalloc(&s->estate, cfg.maxedges) | dalloc(&s->accepted, cfg.maxedges);
-  ok |= dalloc(&s->bestkey, captot) | dalloc(&s->killed, captot);
+  ok |= dalloc(&s->bestkey, rc) | dalloc(&s->killed, rc);
   ok |= dalloc(&s->uniev, cfg.maxevents) | dalloc(&s->log, s->sortcap);
   ok |= dalloc(&s->sortkey, s->sortcap) | dalloc(&s->sortval, s->sortcap) | dalloc(&s->killslot, s->sortcap);
   ok |= dalloc(&s->ctl, 1) | dalloc(&s->tab, 1) | dalloc(&s->gtab, ngauss);
   ok |= dalloc(&s->prods, (size_t)s->sortcap * kMaxProd);
   if (ok) {
-    snprintf(err, errlen, "device allocation failed (%d molecule slots)", captot);
+    snprintf(err, errlen, "device allocation failed (%d molecule records)", s->lay.reccap);
     engine_destroy(s);
     return nullptr;
   }
   s->inj = nullptr;
   s->injcap = 0;
-  s->ktime_n = 0;
+  s->cur = 0;
+  for (int i = 0; i < kMaxSpecies; i++) {
+    s->pc.reactmask[i] = tab.reactmask[i];
+    if (geo.prefilter_margin >= 0) {
+      double rr = sqrt(tab.maxbind2[i]) + geo.prefilter_margin;
+      s->pc.r2f[i] = (float)(rr * rr);
+    } else {
+      s->pc.r2f[i] = 3.0e38f;
+    }
+  }
   (void)hipMemset(s->bn.count, 0, sizeof(int) * s->nscan);
-  (void)hipMemset(s->bestkey, 0xFF, sizeof(uint64_t) * captot);
-  (void)hipMemset(s->killed, 0, sizeof(int) * captot);
+  (void)hipMemset(s->bestkey, 0xFF, sizeof(uint64_t) * rc);
+  (void)hipMemset(s->killed, 0, sizeof(int) * rc);
   (void)hipMemset(s->ctl, 0, sizeof(Control));
   (void)hipMemcpy(s->tab, &tab, sizeof(DevTables), hipMemcpyHostToDevice);
   (void)hipMemcpy(s->gtab, gtable, sizeof(double) * ngauss, hipMemcpyHostToDevice);
   (void)hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   for (int k = 0; k < 16; k++) (void)hipEventCreate(&s->ev[k]);
-  s->graph_ready = 0;
   return s;
 }
 
 void engine_destroy(DeviceState* s) {
   if (!s) return;
-  void* ptrs[] = {s->m.px, s->m.py, s->m.pz, s->m.ident, s->m.serno, s->m.box, s->m.rank, s->bn.count, s->bn.part,
-                  s->bn.boff, s->bn.bstart, s->bn.tmp, s->bn.sslot, s->bn.sbox, s->bn.spk,
-                  s->edges, s->estate, s->accepted, s->bestkey, s->killed, s->uniev, s->log, s->sortkey,
-                  s->sortval, s->killslot, s->ctl, s->tab, s->gtab, s->inj, s->prods};
+  free_recs(&s->buf[0]);
+  free_recs(&s->buf[1]);
+  void* ptrs[] = {s->inv, s->bn.count, s->bn.part, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges, s->estate,
+                  s->accepted, s->bestkey, s->killed, s->uniev, s->log, s->sortkey, s->sortval, s->killslot,
+                  s->ctl, s->tab, s->gtab, s->inj, s->prods};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  if (s->graph_ready) (void)hipGraphExecDestroy(s->graph);
+  for (int k = 0; k < 2; k++)
+    if (s->graph_ready[k]) (void)hipGraphExecDestroy(s->graph[k]);
   if (s->stream) {
     for (int k = 0; k < 16; k++) (void)hipEventDestroy(s->ev[k]);
     (void)hipStreamDestroy(s->stream);
@@ -1046,27 +1103,38 @@ void engine_destroy(DeviceState* s) {
   delete s;
 }
 
+// Upload list ll in live order.  Records are appended after the records of
+// the lists uploaded before it; the first step bins everything.
 int engine_upload_list(DeviceState* s, int ll, int n, const int* ident, const double* pos, const uint64_t* serno) {
   if (ll < 0 || ll >= s->lay.nlist) return -2;
   int cap = s->lay.base[ll + 1] - s->lay.base[ll];
   if (n > cap) return -3;
-  int b0 = s->lay.base[ll];
-  std::vector<double> x(n), y(n), z(n);
-  for (int k = 0; k < n; k++) {
-    x[k] = pos[3 * k];
-    y[k] = pos[3 * k + 1];
-    z[k] = pos[3 * k + 2];
-  }
-  if (n) {
-    HIPOK(hipMemcpy(s->m.px + b0, x.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(s->m.py + b0, y.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(s->m.pz + b0, z.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(s->m.ident + b0, ident, sizeof(int) * n, hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(s->m.serno + b0, serno, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
-  }
   Control c;
   HIPOK(hipMemcpy(&c, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
+  int r0 = c.nrec;
+  if (r0 + n > s->lay.reccap) return -3;
+  Recs& R = s->buf[s->cur];
+  if (n) {
+    std::vector<double> x(n), y(n), z(n);
+    std::vector<int> lp(n), inv(n), box(n, 0);
+    for (int k = 0; k < n; k++) {
+      x[k] = pos[3 * k];
+      y[k] = pos[3 * k + 1];
+      z[k] = pos[3 * k + 2];
+      lp[k] = k;
+      inv[k] = r0 + k;
+    }
+    HIPOK(hipMemcpy(R.px + r0, x.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(R.py + r0, y.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(R.pz + r0, z.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(R.ident + r0, ident, sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(R.serno + r0, serno, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(R.lpos + r0, lp.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(R.box + r0, box.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(s->inv + s->lay.base[ll], inv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  }
   c.nl[ll] = n;
+  c.nrec = r0 + n;
   int run = 0;
   for (int l2 = 0; l2 < s->lay.nlist; l2++) {
     c.pfx[l2] = run;
@@ -1086,13 +1154,17 @@ int engine_set_counters(DeviceState* s, uint64_t serno_next, uint32_t step, cons
   return 0;
 }
 
-static ResolveArgs resolve_args(DeviceState* s) {
+static ResolveArgs resolve_args(DeviceState* s, int src) {
   ResolveArgs a;
   a.L = s->lay;
   a.g = s->geo;
   a.tab = s->tab;
   a.ctl = s->ctl;
-  a.m = s->m;
+  a.a = s->buf[src];
+  a.b = s->buf[1 - src];
+  a.inv = s->inv;
+  a.bn = s->bn;
+  a.nscan = s->nscan;
   a.edges = s->edges;
   a.maxedges = s->cfg.maxedges;
   a.bestkey = s->bestkey;
@@ -1112,16 +1184,17 @@ static ResolveArgs resolve_args(DeviceState* s) {
   return a;
 }
 
-// Launch one step on s->stream; the number of live molecules for the grids is
-// an upper bound (capacity), the kernels read the exact counts on the device.
-// timed: record events between the kernels into ev[0..7].
-static void launch_step(DeviceState* s, const double* dinj, int timed) {
-  int captot = s->lay.captot;
-  int nb = (captot + kBlock - 1) / kBlock;
+// Launch one step from buffer src into 1-src on s->stream.  Grids cover the
+// record capacity; kernels read the exact counts from device memory.
+// timed: events between the kernels into ev[0..6].
+static void launch_step(DeviceState* s, int src, const double* dinj, int timed) {
+  int rc = s->lay.reccap;
+  int nb = (rc + kBlock - 1) / kBlock;
   hipStream_t st = s->stream;
   int nbox = s->geo.nbox;
+  Recs A = s->buf[src], B = s->buf[1 - src];
   if (timed) (void)hipEventRecord(s->ev[0], st);
-  hipLaunchKernelGGL(k_diffuse, dim3(nb), dim3(kBlock), 0, st, s->lay, s->geo, s->tab, s->ctl, s->m, s->bn.count,
+  hipLaunchKernelGGL(k_diffuse, dim3(nb), dim3(kBlock), 0, st, s->lay, s->geo, s->tab, s->ctl, A, s->bn.count,
                      s->gtab, s->ngauss - 1, dinj, s->key0, s->key1, s->uniev, s->cfg.maxevents, nbox);
   if (timed) (void)hipEventRecord(s->ev[1], st);
   hipLaunchKernelGGL(k_scan1, dim3(s->nscanblk), dim3(kScanBlock), 0, st, s->bn.count, s->bn.part, s->bn.boff,
@@ -1130,14 +1203,14 @@ static void launch_step(DeviceState* s, const double* dinj, int timed) {
   hipLaunchKernelGGL(k_scan3, dim3((s->nscan + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s->bn.part, s->bn.boff,
                      s->bn.bstart, s->nscan);
   if (timed) (void)hipEventRecord(s->ev[2], st);
-  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, st, s->lay, s->ctl, s->m, s->bn, nbox);
+  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, st, s->lay, s->tab, s->ctl, A, B, s->bn, s->inv, nbox,
+                     s->sortcap);
   if (timed) (void)hipEventRecord(s->ev[3], st);
-  hipLaunchKernelGGL(k_boxsort, dim3(nb), dim3(kBlock), 0, st, s->lay, s->ctl, s->m, s->bn, nbox, s->nscan);
   if (timed) (void)hipEventRecord(s->ev[4], st);
-  hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, s->lay, s->geo, s->tab, s->ctl, s->m, s->bn, nbox,
+  hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, s->lay, s->geo, s->tab, s->pc, s->ctl, B, s->bn, nbox,
                      s->nscan, s->edges, s->cfg.maxedges, s->key0, s->key1);
   if (timed) (void)hipEventRecord(s->ev[5], st);
-  hipLaunchKernelGGL(k_resolve, dim3(1), dim3(kResolveThreads), 0, st, resolve_args(s));
+  hipLaunchKernelGGL(k_resolve, dim3(1), dim3(kResolveThreads), 0, st, resolve_args(s, src));
   if (timed) (void)hipEventRecord(s->ev[6], st);
 }
 
@@ -1159,9 +1232,10 @@ int engine_step(DeviceState* s, const double* inject, long ninject) {
     HIPOK(hipMemcpy(s->inj, inject, sizeof(double) * 3 * ninject, hipMemcpyHostToDevice));
     dinj = s->inj;
   }
-  launch_step(s, dinj, 0);
+  launch_step(s, s->cur, dinj, 0);
   HIPOK(hipGetLastError());
   HIPOK(hipStreamSynchronize(s->stream));
+  s->cur ^= 1;
   Control c;
   HIPOK(hipMemcpy(&c, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
   return c.err;
@@ -1169,21 +1243,25 @@ int engine_step(DeviceState* s, const double* inject, long ninject) {
 
 int engine_run(DeviceState* s, long nsteps, int use_graph) {
   hipEvent_t e0 = s->ev[8], e1 = s->ev[9];
-  // The launch sequence reads every count from device memory, so one captured
-  // step can be replayed for all steps.
-  if (use_graph && !s->graph_ready) {
-    hipGraph_t g;
-    HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    launch_step(s, nullptr, 0);
-    HIPOK(hipStreamEndCapture(s->stream, &g));
-    HIPOK(hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0));
-    HIPOK(hipGraphDestroy(g));
-    s->graph_ready = 1;
+  // one captured graph per buffer parity: the launch sequence reads every
+  // count from device memory, so the graphs replay for all steps
+  if (use_graph) {
+    for (int k = 0; k < 2; k++) {
+      if (s->graph_ready[k]) continue;
+      hipGraph_t g;
+      HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      launch_step(s, k, nullptr, 0);
+      HIPOK(hipStreamEndCapture(s->stream, &g));
+      HIPOK(hipGraphInstantiate(&s->graph[k], g, nullptr, nullptr, 0));
+      HIPOK(hipGraphDestroy(g));
+      s->graph_ready[k] = 1;
+    }
   }
   HIPOK(hipEventRecord(e0, s->stream));
   for (long k = 0; k < nsteps; k++) {
-    if (use_graph) HIPOK(hipGraphLaunch(s->graph, s->stream));
-    else launch_step(s, nullptr, 0);
+    if (use_graph) HIPOK(hipGraphLaunch(s->graph[s->cur], s->stream));
+    else launch_step(s, s->cur, nullptr, 0);
+    s->cur ^= 1;
   }
   HIPOK(hipEventRecord(e1, s->stream));
   HIPOK(hipStreamSynchronize(s->stream));
@@ -1195,13 +1273,14 @@ int engine_run(DeviceState* s, long nsteps, int use_graph) {
   return c.err;
 }
 
-// Run nsteps with events between kernels; per-kernel average ms in out[]:
-// diffuse, scan, scatter, boxsort, pairs, resolve.
+// Steps with events between the kernels; per-kernel average ms in out[]:
+// diffuse, box scan, scatter, (unused), pairs, resolve.
 int engine_profile(DeviceState* s, long nsteps, double* out) {
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (long k = 0; k < nsteps; k++) {
-    launch_step(s, nullptr, 1);
+    launch_step(s, s->cur, nullptr, 1);
     HIPOK(hipStreamSynchronize(s->stream));
+    s->cur ^= 1;
     for (int j = 0; j < 6; j++) {
       float ms = 0;
       HIPOK(hipEventElapsedTime(&ms, s->ev[j], s->ev[j + 1]));
@@ -1226,21 +1305,34 @@ int engine_get_list(DeviceState* s, int ll, int* ident, double* pos, uint64_t* s
   if (ll < 0 || ll >= s->lay.nlist) return -2;
   int n = c.nl[ll];
   if (n > max) n = max;
-  int b0 = s->lay.base[ll];
-  if (n > 0) {
-    if (ident) HIPOK(hipMemcpy(ident, s->m.ident + b0, sizeof(int) * n, hipMemcpyDeviceToHost));
-    if (serno) HIPOK(hipMemcpy(serno, s->m.serno + b0, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
-    if (box) HIPOK(hipMemcpy(box, s->m.box + b0, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (n <= 0) return c.nl[ll];
+  Recs& R = s->buf[s->cur];
+  int rc = s->lay.reccap;
+  hipLaunchKernelGGL(k_rebuild_inv, dim3((rc + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream, s->lay, s->tab,
+                     s->ctl, R, s->inv);
+  HIPOK(hipStreamSynchronize(s->stream));
+  std::vector<int> inv(n);
+  HIPOK(hipMemcpy(inv.data(), s->inv + s->lay.base[ll], sizeof(int) * n, hipMemcpyDeviceToHost));
+  int nr = c.nrec > 0 ? c.nrec : 1;
+  std::vector<double> x(nr), y(nr), z(nr);
+  std::vector<int> id(nr), bx(nr);
+  std::vector<uint64_t> sr(nr);
+  HIPOK(hipMemcpy(x.data(), R.px, sizeof(double) * nr, hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(y.data(), R.py, sizeof(double) * nr, hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(z.data(), R.pz, sizeof(double) * nr, hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(id.data(), R.ident, sizeof(int) * nr, hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(bx.data(), R.box, sizeof(int) * nr, hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(sr.data(), R.serno, sizeof(uint64_t) * nr, hipMemcpyDeviceToHost));
+  for (int k = 0; k < n; k++) {
+    int r = inv[k];
+    if (r < 0 || r >= nr) return -3;
+    if (ident) ident[k] = id[r];
+    if (serno) serno[k] = sr[r];
+    if (box) box[k] = bx[r];
     if (pos) {
-      std::vector<double> x(n), y(n), z(n);
-      HIPOK(hipMemcpy(x.data(), s->m.px + b0, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPOK(hipMemcpy(y.data(), s->m.py + b0, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPOK(hipMemcpy(z.data(), s->m.pz + b0, sizeof(double) * n, hipMemcpyDeviceToHost));
-      for (int k = 0; k < n; k++) {
-        pos[3 * k] = x[k];
-        pos[3 * k + 1] = y[k];
-        pos[3 * k + 2] = z[k];
-      }
+      pos[3 * k] = x[r];
+      pos[3 * k + 1] = y[r];
+      pos[3 * k + 2] = z[r];
     }
   }
   return c.nl[ll];
@@ -1257,7 +1349,7 @@ int engine_get_log(DeviceState* s, LogRow* rows, int maxrows) {
 double engine_last_ms(DeviceState* s) { return s->last_ms; }
 
 int engine_device_bytes(DeviceState* s, double* out) {
-  *out = (double)s->lay.captot * (3 * 8 + 4 + 8 + 4 + 4 + 4 * 4 + 3 * 8 + 8 + 4) + (double)s->nscan * 12;
+  *out = 2.0 * s->lay.reccap * (3 * 8 + 4 + 8 + 4 + 4 + 4) + (double)s->lay.reccap * 16 + (double)s->nscan * 16;
   return 0;
 }
 
