@@ -88,6 +88,21 @@ def cpu_baseline(spec, steps):
     return molsteps / dt, dt
 
 
+def calib_copy(mib, reps=5):
+    """A plain device-to-device copy of known size: reads and writes `mib` MiB
+    per launch.  Profiling counters read on it give the scale factor between
+    counter bytes and true HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calib_copy_mib": mib, "launches": reps, "read_bytes": n * 4, "write_bytes": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,7 +113,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--n", type=int, default=500000, help="molecules per reactant species")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calib_copy_mib > 0:
+        calib_copy(args.calib_copy_mib)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)

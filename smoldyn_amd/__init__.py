@@ -14,7 +14,9 @@ from pathlib import Path
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
-LIBPATH = _HERE / "libsmoldyn_amd.so"
+# SMOLAMD_LIBRARY selects another build of the engine library (e.g. a tuning
+# variant); the default is the in-tree build.
+LIBPATH = Path(os.environ.get("SMOLAMD_LIBRARY", str(_HERE / "libsmoldyn_amd.so")))
 
 # enum ErrorCode (libsmoldyn.h:13-26)
 ECok, ECnotify, ECwarning, ECnonexist, ECall, ECmissing, ECbounds, ECsyntax, ECerror, ECmemory, ECbug, ECsame, \

@@ -535,27 +535,40 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Layout L, Geometry g, const De
         const int k0 = bn.bstart[colb + za];
         const int k1 = bn.bstart[colb + zb2 + 1];
         const int wrap = cw | zw;
-        for (int k = k0; k < k1; k++) {
-          float4 o = bn.spk[k];
-          int i2 = __float_as_int(o.w);
-          float dxf = (o.x + shx) - me.x, dyf = (o.y + shy) - me.y, dzf = (o.z + shz) - me.z;
-          float d2 = dxf * dxf + dyf * dyf + dzf * dzf;
-          if (d2 > r2f || !((mask >> i2) & 1u) || k == u) continue;
-          int dz = dzfix > -2 ? dzfix : (b.box[k] % sz) - bz;
-          int nbi = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-          int lp2 = b.lpos[k];
-          if (nbi == 13 && half && lp2 >= lp1) continue;
-          if (!loaded) {
-            p1[0] = b.px[u];
-            p1[1] = b.py[u];
-            p1[2] = b.pz[u];
-            ser1 = b.serno[u];
-            loaded = true;
+        const float ox = me.x - shx, oy = me.y - shy, oz = me.z - shz;
+        for (int kb = k0; kb < k1; kb += 32) {
+          // screening: bit j of hits <=> record kb+j is a candidate partner
+          const int kn = min(k1 - kb, 32);
+          uint32_t hits = 0;
+#pragma unroll 4
+          for (int j = 0; j < kn; j++) {
+            const float4 o = bn.spk[kb + j];
+            const float dxf = o.x - ox, dyf = o.y - oy, dzf = o.z - oz;
+            const float d2 = dxf * dxf + dyf * dyf + dzf * dzf;
+            const bool hit = (d2 <= r2f) & ((mask >> __float_as_int(o.w)) & 1u);
+            hits |= (uint32_t)hit << j;
           }
-          uint64_t key = nbi == 13 ? (key1base | (uint64_t)lp2)
-                                   : ((1ull << 63) | key1base | ((uint64_t)nbi << 27) | (uint64_t)lp2);
-          exact_pair(g, tab, ctl, b, edges, maxedges, key0, key1, step, u, p1, i1, ser1, k, i2,
-                     nbi == 13 ? 0 : wrap, key);
+          if (u >= kb && u < kb + 32) hits &= ~(1u << (u - kb));
+          while (hits) {
+            const int k = kb + __ffs(hits) - 1;
+            hits &= hits - 1;
+            const int i2 = __float_as_int(bn.spk[k].w);
+            const int dz = dzfix > -2 ? dzfix : (b.box[k] % sz) - bz;
+            const int nbi = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+            const int lp2 = b.lpos[k];
+            if (nbi == 13 && half && lp2 >= lp1) continue;
+            if (!loaded) {
+              p1[0] = b.px[u];
+              p1[1] = b.py[u];
+              p1[2] = b.pz[u];
+              ser1 = b.serno[u];
+              loaded = true;
+            }
+            const uint64_t key = nbi == 13 ? (key1base | (uint64_t)lp2)
+                                           : ((1ull << 63) | key1base | ((uint64_t)nbi << 27) | (uint64_t)lp2);
+            exact_pair(g, tab, ctl, b, edges, maxedges, key0, key1, step, u, p1, i1, ser1, k, i2,
+                       nbi == 13 ? 0 : wrap, key);
+          }
         }
       }
     }
