@@ -634,20 +634,62 @@ __device__ void wg_sort(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv) {
   }
 }
 
-// exclusive scan of one value per thread; returns the prefix, total in *tot
-__device__ int wg_scan(int v, int* s, int* tot) {
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-    int x = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
-    __syncthreads();
-    s[threadIdx.x] += x;
-    __syncthreads();
+// Sort n <= kRankSort (key,val) pairs in gk/gv in place by rank counting in
+// LDS: each key's destination is the number of keys before it (ties by
+// index).  Two barriers instead of the log^2 stages of a bitonic network.
+constexpr int kRankSort = 512;
+__device__ void wg_rank_sort(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    lk[i] = gk[i];
+    lv[i] = gv[i];
   }
-  int incl = s[threadIdx.x];
-  *tot = s[blockDim.x - 1];
   __syncthreads();
-  return incl - v;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint64_t k = lk[i];
+    int r = 0;
+    for (int j = 0; j < n; j++) {
+      const uint64_t x = lk[j];
+      r += (x < k) | ((x == k) & (j < i));
+    }
+    gk[r] = k;
+    gv[r] = lv[i];
+  }
+  __syncthreads();
+}
+
+__device__ void wg_sort_any(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv) {
+  if (n <= 1) return;
+  if (n <= kRankSort) wg_rank_sort(gk, gv, n, lk, lv);
+  else wg_sort(gk, gv, n, lk, lv);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int t = __shfl_up(v, off);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// exclusive scan of one value per thread (wave scans by shuffles, then the
+// wave totals through LDS); returns the prefix, total in *tot.  s holds at
+// least blockDim.x/64 ints.
+__device__ int wg_scan(int v, int* s, int* tot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int inc = wave_incl_scan(v);
+  if (lane == 63) s[w] = inc;
+  __syncthreads();
+  int off = 0, t = 0;
+  for (int i = 0; i < nw; i++) {
+    int x = s[i];
+    off += (i < w) ? x : 0;
+    t += x;
+  }
+  *tot = t;
+  __syncthreads();
+  return off + inc - v;
 }
 
 struct ResolveArgs {
@@ -680,7 +722,9 @@ struct ResolveArgs {
 __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   __shared__ uint64_t lk[kLdsSort];
   __shared__ int lv[kLdsSort];
-  __shared__ int s_scan[kResolveThreads];
+  __shared__ int s_scan[kResolveThreads / 64];
+  __shared__ int s_wcnt[(kResolveThreads / 64) * kMaxList];
+  __shared__ int s_dcount[kMaxSpecies];
   __shared__ int s_flag, s_nacc;
   __shared__ int s_nl[kMaxList], s_nf[kMaxList], s_k0[kMaxList], s_k1[kMaxList], s_pl[kMaxList];
   const int tid = threadIdx.x;
@@ -753,13 +797,13 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     a.sv[i] = i;
   }
   __syncthreads();
-  wg_sort(a.sk, a.sv, U, lk, lv);
+  wg_sort_any(a.sk, a.sv, U, lk, lv);
   for (int i = tid; i < B2; i += NT) {
     a.sk[U + i] = a.edges[a.accepted[i]].key;
     a.sv[U + i] = a.accepted[i];
   }
   __syncthreads();
-  wg_sort(a.sk + U, a.sv + U, B2, lk, lv);
+  wg_sort_any(a.sk + U, a.sv + U, B2, lk, lv);
   int NE = U + B2;
 
   // ---- product counts and serial numbers in creation order (each thread
@@ -878,7 +922,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     a.sv[i] = 0;
   }
   __syncthreads();
-  wg_sort(a.sk, a.sv, D, lk, lv);
+  wg_sort_any(a.sk, a.sv, D, lk, lv);
   if (tid < a.L.nlist) {
     int ll = tid;
     int lo = a.L.base[ll], hi = a.L.base[ll + 1];
@@ -935,6 +979,8 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   if (tid < a.L.nlist) s_pl[tid] = 0;
   __syncthreads();
   int overflow = 0;
+  const int lane = tid & 63, wv = tid >> 6, nwv = NT >> 6;
+  const uint64_t lanes_below = (1ull << lane) - 1;
   for (int j0 = 0; j0 < K; j0 += NT) {
     int j = j0 + tid;
     int c = -1, ll = -1;
@@ -942,40 +988,53 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       c = K - 1 - (int)(((long)j + D) % K);
       ll = a.prods[c].list;
     }
+    // rank of this product among the chunk's products of its list, in j order
+    int inwave = 0;
     for (int l2 = 0; l2 < a.L.nlist; l2++) {
-      int tot;
-      int pre = wg_scan((ll == l2) ? 1 : 0, s_scan, &tot);
-      if (ll == l2) {
-        int slot = a.L.base[l2] + s_nf[l2] + s_pl[l2] + pre;
-        int rec = nbinned + c;
-        if (slot < a.L.base[l2 + 1] && rec < a.L.reccap) {
-          Prod pr = a.prods[c];
-          B.px[rec] = pr.x;
-          B.py[rec] = pr.y;
-          B.pz[rec] = pr.z;
-          B.ident[rec] = pr.ident;
-          B.serno[rec] = ser0 + (uint64_t)c;
-          B.lpos[rec] = slot - a.L.base[l2];
-          B.box[rec] = pos2box(a.g, pr.x, pr.y, pr.z);
-          a.inv[slot] = rec;
-        } else {
-          overflow = 1;
-        }
-      }
-      __syncthreads();
-      if (tid == 0) s_pl[l2] += tot;
-      __syncthreads();
+      uint64_t m = __ballot(ll == l2);
+      if (lane == 0) s_wcnt[wv * kMaxList + l2] = __popcll(m);
+      if (ll == l2) inwave = __popcll(m & lanes_below);
     }
+    __syncthreads();
+    if (ll >= 0) {
+      int pre = inwave;
+      for (int w2 = 0; w2 < wv; w2++) pre += s_wcnt[w2 * kMaxList + ll];
+      int slot = a.L.base[ll] + s_nf[ll] + s_pl[ll] + pre;
+      int rec = nbinned + c;
+      if (slot < a.L.base[ll + 1] && rec < a.L.reccap) {
+        Prod pr = a.prods[c];
+        B.px[rec] = pr.x;
+        B.py[rec] = pr.y;
+        B.pz[rec] = pr.z;
+        B.ident[rec] = pr.ident;
+        B.serno[rec] = ser0 + (uint64_t)c;
+        B.lpos[rec] = slot - a.L.base[ll];
+        B.box[rec] = pos2box(a.g, pr.x, pr.y, pr.z);
+        a.inv[slot] = rec;
+      } else {
+        overflow = 1;
+      }
+    }
+    __syncthreads();
+    if (tid < a.L.nlist) {
+      int t = 0;
+      for (int w2 = 0; w2 < nwv; w2++) t += s_wcnt[w2 * kMaxList + tid];
+      s_pl[tid] += t;
+    }
+    __syncthreads();
   }
   if (overflow) atomicOr(&ctl->err, kErrCapacity);
-  // ---- species counts, list lengths, counters
+  // ---- species counts (accumulated in LDS), list lengths, counters
+  if (tid < kMaxSpecies) s_dcount[tid] = 0;
+  __syncthreads();
   for (int e = tid; e < NE; e += NT) {
     const DevRxn& rx = e < U ? tab->rxn1[a.uniev[a.sv[e]].r] : tab->rxn2[a.edges[a.sv[e]].r];
-    atomicSub(&ctl->count[rx.rct0], 1);
-    if (e >= U) atomicSub(&ctl->count[rx.rct1], 1);
-    for (int q = 0; q < rx.nprod; q++) atomicAdd(&ctl->count[rx.prd[q]], 1);
+    atomicSub(&s_dcount[rx.rct0], 1);
+    if (e >= U) atomicSub(&s_dcount[rx.rct1], 1);
+    for (int q = 0; q < rx.nprod; q++) atomicAdd(&s_dcount[rx.prd[q]], 1);
   }
   __syncthreads();
+  if (tid < kMaxSpecies && s_dcount[tid]) ctl->count[tid] += s_dcount[tid];
   if (tid == 0) {
     int run = 0;
     unsigned long long live0 = 0;
