@@ -638,7 +638,7 @@ __device__ void wg_sort(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv) {
 // LDS: each key's destination is the number of keys before it (ties by
 // index).  Two barriers instead of the log^2 stages of a bitonic network.
 constexpr int kRankSort = 512;
-__device__ void wg_rank_sort(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv) {
+__device__ void wg_rank_sort(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv, int* ok32, int* ov) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     lk[i] = gk[i];
     lv[i] = gv[i];
@@ -653,14 +653,36 @@ __device__ void wg_rank_sort(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv
     }
     gk[r] = k;
     gv[r] = lv[i];
+    if (ok32) ok32[r] = (int)k;
+    if (ov) ov[r] = lv[i];
   }
   __syncthreads();
 }
 
-__device__ void wg_sort_any(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv) {
-  if (n <= 1) return;
-  if (n <= kRankSort) wg_rank_sort(gk, gv, n, lk, lv);
-  else wg_sort(gk, gv, n, lk, lv);
+// Sort n (key,val) pairs of gk/gv in place; when ok32/ov are given (LDS,
+// capacity >= n) the sorted low key words / values are also left there.
+__device__ void wg_sort_any(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv, int* ok32 = nullptr,
+                            int* ov = nullptr) {
+  if (n <= kRankSort) {
+    if (n == 1 && (ok32 || ov)) {
+      if (threadIdx.x == 0) {
+        if (ok32) ok32[0] = (int)gk[0];
+        if (ov) ov[0] = gv[0];
+      }
+      __syncthreads();
+      return;
+    }
+    if (n >= 1) wg_rank_sort(gk, gv, n, lk, lv, ok32, ov);
+    return;
+  }
+  wg_sort(gk, gv, n, lk, lv);
+  if (ok32 || ov) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      if (ok32) ok32[i] = (int)gk[i];
+      if (ov) ov[i] = gv[i];
+    }
+    __syncthreads();
+  }
 }
 
 __device__ __forceinline__ int wave_incl_scan(int v) {
@@ -691,6 +713,16 @@ __device__ int wg_scan(int v, int* s, int* tot) {
   __syncthreads();
   return off + inc - v;
 }
+
+constexpr int kLdsEvents = 1024;  // sorted event indices kept in LDS up to this count
+constexpr int kLdsKills = 2048;   // sorted kill slots kept in LDS up to this count
+
+// Sorted kill slots, read from LDS when they fit there, else from global.
+struct KillView {
+  const int* lds;
+  const uint64_t* glb;
+  __device__ __forceinline__ int operator[](int i) const { return lds ? lds[i] : (int)glb[i]; }
+};
 
 struct ResolveArgs {
   Layout L;
@@ -725,6 +757,8 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   __shared__ int s_scan[kResolveThreads / 64];
   __shared__ int s_wcnt[(kResolveThreads / 64) * kMaxList];
   __shared__ int s_dcount[kMaxSpecies];
+  __shared__ int s_ev[kLdsEvents];
+  __shared__ int s_kill[kLdsKills];
   __shared__ int s_flag, s_nacc;
   __shared__ int s_nl[kMaxList], s_nf[kMaxList], s_k0[kMaxList], s_k1[kMaxList], s_pl[kMaxList];
   const int tid = threadIdx.x;
@@ -743,6 +777,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   // both of its molecules; this is the lexicographically-first maximal
   // matching, i.e. what the sequential loop of bireact accepts.
   if (tid == 0) s_nacc = 0;
+  if (tid < kMaxSpecies) s_dcount[tid] = 0;
   for (int e = tid; e < E; e += NT) a.estate[e] = 0;
   __syncthreads();
   int rounds = 0;
@@ -797,14 +832,16 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     a.sv[i] = i;
   }
   __syncthreads();
-  wg_sort_any(a.sk, a.sv, U, lk, lv);
+  const bool ev_lds = (U + B2) <= kLdsEvents;
+  wg_sort_any(a.sk, a.sv, U, lk, lv, nullptr, ev_lds ? s_ev : nullptr);
   for (int i = tid; i < B2; i += NT) {
     a.sk[U + i] = a.edges[a.accepted[i]].key;
     a.sv[U + i] = a.accepted[i];
   }
   __syncthreads();
-  wg_sort_any(a.sk + U, a.sv + U, B2, lk, lv);
+  wg_sort_any(a.sk + U, a.sv + U, B2, lk, lv, nullptr, ev_lds ? s_ev + U : nullptr);
   int NE = U + B2;
+  const int* evs = ev_lds ? s_ev : a.sv;  // event order: sorted event indices
 
   // ---- product counts and serial numbers in creation order (each thread
   // takes a contiguous chunk of events so the scan follows creation order)
@@ -812,7 +849,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   int e0 = tid * per, e1 = min(NE, e0 + per);
   int myprod = 0;
   for (int e = e0; e < e1; e++) {
-    const DevRxn& rx = e < U ? tab->rxn1[a.uniev[a.sv[e]].r] : tab->rxn2[a.edges[a.sv[e]].r];
+    const DevRxn& rx = e < U ? tab->rxn1[a.uniev[evs[e]].r] : tab->rxn2[a.edges[evs[e]].r];
     myprod += rx.nprod;
   }
   int K;
@@ -828,7 +865,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     uint64_t sA, sB = 0;
     int order, ridx;
     if (e < U) {
-      UniEvent ev = a.uniev[a.sv[e]];
+      UniEvent ev = a.uniev[evs[e]];
       rx = &tab->rxn1[ev.r];
       ridx = ev.r;
       order = 1;
@@ -838,7 +875,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       sA = a.a.serno[ev.rec];
       a.killslot[e] = ev.slot;
     } else {
-      Edge ed = a.edges[a.sv[e]];
+      Edge ed = a.edges[evs[e]];
       rx = &tab->rxn2[ed.r];
       ridx = ed.r;
       order = 2;
@@ -882,6 +919,9 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       a.killslot[k2] = a.L.base[tab->listof[id1]] + B.lpos[ed.s1];
       a.killslot[k2 + 1] = a.L.base[tab->listof[id2]] + B.lpos[ed.s2];
     }
+    atomicSub(&s_dcount[rx->rct0], 1);
+    if (order == 2) atomicSub(&s_dcount[rx->rct1], 1);
+    for (int q = 0; q < rx->nprod; q++) atomicAdd(&s_dcount[rx->prd[q]], 1);
     double uvec[3] = {0, 0, 0};
     if (rx->rotate) smolrng_unitvec(a.key0, a.key1, sA, step, 3, uvec);
     for (int q = 0; q < rx->nprod; q++) {
@@ -922,21 +962,23 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     a.sv[i] = 0;
   }
   __syncthreads();
-  wg_sort_any(a.sk, a.sv, D, lk, lv);
+  const bool kill_lds = D <= kLdsKills;
+  wg_sort_any(a.sk, a.sv, D, lk, lv, kill_lds ? s_kill : nullptr, nullptr);
+  const KillView kv{kill_lds ? s_kill : nullptr, a.sk};
   if (tid < a.L.nlist) {
     int ll = tid;
     int lo = a.L.base[ll], hi = a.L.base[ll + 1];
     int l = 0, h = D;
     while (l < h) {
       int mid = (l + h) >> 1;
-      if ((int)a.sk[mid] < lo) l = mid + 1;
+      if (kv[mid] < lo) l = mid + 1;
       else h = mid;
     }
     int k0 = l;
     h = D;
     while (l < h) {
       int mid = (l + h) >> 1;
-      if ((int)a.sk[mid] < hi) l = mid + 1;
+      if (kv[mid] < hi) l = mid + 1;
       else h = mid;
     }
     s_k0[ll] = k0;
@@ -962,13 +1004,13 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       int l = k0, h = k1;
       while (l < h) {
         int mid = (l + h) >> 1;
-        if ((int)a.sk[mid] < t) l = mid + 1;
+        if (kv[mid] < t) l = mid + 1;
         else h = mid;
       }
-      if (l < k1 && (int)a.sk[l] == t) continue;  // killed
+      if (l < k1 && kv[l] == t) continue;  // killed
       int above = k1 - l;                          // kills above t
       int rank = (nlc - 1 - (nf + q)) - above;
-      int dst = (int)a.sk[k0 + rank];
+      int dst = kv[k0 + rank];
       int rec = a.inv[t];
       B.lpos[rec] = dst - lo;
       a.inv[dst] = rec;
@@ -1024,15 +1066,8 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     __syncthreads();
   }
   if (overflow) atomicOr(&ctl->err, kErrCapacity);
-  // ---- species counts (accumulated in LDS), list lengths, counters
-  if (tid < kMaxSpecies) s_dcount[tid] = 0;
-  __syncthreads();
-  for (int e = tid; e < NE; e += NT) {
-    const DevRxn& rx = e < U ? tab->rxn1[a.uniev[a.sv[e]].r] : tab->rxn2[a.edges[a.sv[e]].r];
-    atomicSub(&s_dcount[rx.rct0], 1);
-    if (e >= U) atomicSub(&s_dcount[rx.rct1], 1);
-    for (int q = 0; q < rx.nprod; q++) atomicAdd(&s_dcount[rx.prd[q]], 1);
-  }
+  // ---- species counts (accumulated in LDS by the product loop), list
+  // lengths, counters
   __syncthreads();
   if (tid < kMaxSpecies && s_dcount[tid]) ctl->count[tid] += s_dcount[tid];
   if (tid == 0) {

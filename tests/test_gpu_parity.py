@@ -34,7 +34,14 @@ def _run_pair(spec, nsteps, check_every=1, inject=None):
         nev += len(lo)
         if (k + 1) % check_every == 0 or k == nsteps - 1:
             ss.compare_states(ss.oracle_state(osim), ss.device_state(dsim))
+            _compare_counts(spec, osim, dsim)
     return nev
+
+
+def _compare_counts(spec, osim, dsim):
+    """Species counts kept by the device's counters equal the oracle's."""
+    for name in spec["species"]:
+        assert dsim.count(name) == osim.count(osim.index(name)), name
 
 
 def test_c1_injected_displacements_one_step():
