@@ -37,8 +37,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 #   k_pairs    read packed position 16 + live position 4 + box 4 once (the
 #              neighboring boxes' records are re-read from L2, not HBM) = 24
 ALGO_BYTES = {"diffuse": 72, "scatter": 112, "pairs": 24}
-KERNEL_NAMES = ["diffuse", "scan", "scatter", "unused", "pairs", "resolve"]
-HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_scan1", "scatter": "k_scatter", "unused": "-",
+KERNEL_NAMES = ["diffuse", "scan", "scatter", "event_overhead", "pairs", "resolve"]
+HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_scan1+k_scan2+k_scan3", "scatter": "k_scatter",
                      "pairs": "k_pairs", "resolve": "k_resolve"}
 
 
@@ -147,8 +147,11 @@ def main():
     kt = sim.profile_steps(args.profile_steps) if args.profile_steps > 0 else np.zeros(6)
     nmol = sim.diagnostics()["nmol"]
     times = dict(zip(KERNEL_NAMES, [float(x) for x in kt]))
-    dom = max(ALGO_BYTES, key=lambda k: times[k])
-    achieved = ALGO_BYTES[dom] * nmol / (times[dom] * 1e-3) / 1e9 if times[dom] > 0 else 0.0
+    # each kernel's event interval includes one event overhead; the kernel's
+    # own duration is the interval minus that overhead
+    net = {k: max(v - times["event_overhead"], 0.0) for k, v in times.items() if k != "event_overhead"}
+    dom = max(ALGO_BYTES, key=lambda k: net[k])
+    achieved = ALGO_BYTES[dom] * nmol / (net[dom] * 1e-3) / 1e9 if net[dom] > 0 else 0.0
     step_bytes = sum(ALGO_BYTES.values()) * nmol
     counts = {s: sim.count(s) for s in ("A", "B", "C")}
     out = {
@@ -168,8 +171,10 @@ def main():
                                "a~1.0, b/a 1.2, ~4 molecules per virtual box, FP64",
                    "molecules": 2 * args.n, "graph": use_graph},
         "device_ms_per_step": dev_ms / args.steps,
-        "kernel_ms": times,
-        "roofline": {"bound": "hbm", "kernel": HIP_KERNEL_SYMBOL[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "kernel_ms_events": times,
+        "kernel_ms": net,
+        "roofline": {"bound": "hbm", "kernel": HIP_KERNEL_SYMBOL[dom], "kernel_ms": net[dom],
+                     "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "algo_bytes_per_molecule": ALGO_BYTES[dom], "molecules_per_launch": nmol,
                      "traffic": None},

@@ -138,8 +138,10 @@ int smolamdGetBoxes(simptr sim, int* side, double* boxmin, double* boxsize);
 /* Run nsteps time steps (no stopping at time_stop) with the step captured in
    a HIP graph when use_graph != 0; *ms receives the device time (HIP events). */
 enum ErrorCode smolamdRunSteps(simptr sim, long nsteps, int use_graph, double* ms);
-/* Per-kernel average device time (ms) over nsteps timed steps:
-   out[0..5] = diffuse, box scan, scatter, box sort, pair search, resolve. */
+/* Per-kernel average device time (ms) over nsteps timed steps, from HIP
+   events on the engine's stream: out[0..5] = diffuse, box scan, scatter,
+   event overhead (two back-to-back events with no kernel between them),
+   pair search, resolve.  Each kernel interval includes one event overhead. */
 enum ErrorCode smolamdProfileSteps(simptr sim, long nsteps, double* out);
 /* Engine diagnostics: out[0]=live molecules, [1]=edges max, [2]=matching rounds
    of last step, [3]=error bits, [4]=next serial, [5]=step index,
