@@ -33,8 +33,18 @@ enum SmolRngPurpose {
   SMOLRNG_PRODDIR = 3,   /* product unbinding direction, 2 attempts per block */
   SMOLRNG_BIRXN = 4,     /* bimolecular reaction probability < 1 */
   SMOLRNG_WALL = 5,      /* absorbing-wall coin flip */
-  SMOLRNG_SURFACE = 6,   /* surface-action draws */
-  SMOLRNG_PLACE = 7      /* initial uniform placement */
+  SMOLRNG_SURFACE = 6,   /* surface-action draws of one molecule, sub = pass<<8 | iteration */
+  SMOLRNG_PLACE = 7,     /* initial uniform placement */
+  SMOLRNG_RXNSURF = 8    /* surface-action draws of a reacting pair (rxnXsurface) */
+};
+
+/* Surface draw passes (bits 8-15 of the SMOLRNG_SURFACE sub-index). */
+enum SmolRngSurfPass {
+  SMOLRNG_SURF_DIFFUSED = 0,  /* collisions after diffusion, sub = iteration 1..100 */
+  SMOLRNG_SURF_REBORN = 1,    /* collisions of reaction products */
+  SMOLRNG_SURF_DESORBED = 2,  /* collisions of a desorbed molecule */
+  SMOLRNG_SURF_BOUND = 3,     /* surface-bound action (desorption) */
+  SMOLRNG_SURF_DESORBDIST = 4 /* desorption distance */
 };
 
 typedef struct { uint32_t v[4]; } smolrng4;
@@ -80,6 +90,13 @@ SMOLRNG_FN smolrng4 smolrng_mol(uint32_t k0, uint32_t k1, uint64_t serial, uint3
 /* Same as the reference's randCOD (genrand_real2): u32 * 2^-32, in [0,1). */
 SMOLRNG_FN double smolrng_u01(uint32_t v) { return (double)v * (1.0 / 4294967296.0); }
 
+/* Same as the reference's randCCD (genrand_real1): u32 / (2^32 - 1), in [0,1]. */
+SMOLRNG_FN double smolrng_u01cc(uint32_t v) { return (double)v * (1.0 / 4294967295.0); }
+/* Draw block for a pair of molecules at one step. */
+SMOLRNG_FN smolrng4 smolrng_pair(uint32_t k0, uint32_t k1, uint64_t ser1, uint64_t ser2, uint32_t step,
+                                 uint32_t purpose, uint32_t sub) {
+  return smolrng_philox(k0, k1, (uint32_t)ser1, (uint32_t)ser2, step, (purpose & 0xFFu) | (sub << 8));
+}
 /* Same as the reference's randOOD (genrand_real3): (u32 + 0.5) * 2^-32, in (0,1). */
 SMOLRNG_FN double smolrng_u01open(uint32_t v) { return ((double)v + 0.5) * (1.0 / 4294967296.0); }
 

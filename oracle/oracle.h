@@ -61,6 +61,30 @@ uint64_t osim_next_serno(osim* sim);
 const double* osim_gausstable(osim* sim, int flat);
 int osim_ngauss(osim* sim);
 
+/* surfaces (osurf.c).  Shapes: 0 rect, 1 tri, 2 sph, 5 disk; faces 0 front,
+   1 back, 2 none, 3 both; actions as the reference's SrfAction. */
+int osim_add_surface(osim* sim, const char* name);
+int osim_set_surface_action(osim* sim, int s, int ident, int ms, int face, int act, int newident);
+int osim_set_surface_rate(osim* sim, int s, int ident, int ms, int ms1, int ms2, int newident, double value,
+                          int which);
+int osim_add_panel(osim* sim, int s, int ps, const char* rectaxis, const double* params);
+int osim_set_panel_jump(osim* sim, int p1, int f1, int p2, int f2, int bidirectional);
+int osim_box_panels(osim* sim, int b, int* out, int maxn);
+/* geometry restatements, exported for the tests against the reference */
+double osurf_geo_spherenormal(const double* cent, const double* pt, int front, int dim, double* ans);
+int osurf_geo_ptinslab(const double* pt1, const double* pt2, const double* test, int dim);
+double osurf_geo_linexsphs(const double* pt1, const double* pt2, const double* cent, double rad, int dim,
+                           double* crss2ptr, double* nrdistptr, double* nrposptr);
+int osurf_geo_rectxaabb3(const double* r1, const double* r2, const double* r3, const double* r4,
+                         const double* bpt1, const double* bpt2);
+int osurf_geo_sphsxaabb3(const double* cent, double rad, const double* bpt1, const double* bpt2);
+int osurf_geo_diskxaabb3(const double* cent, double rad, const double* norm, const double* bpt1,
+                         const double* bpt2);
+double osurf_desorbdist_y(double step, double y, int revads);
+int osurf_geo_linexaabb2(const double* pt1, const double* pt2, const double* norm, const double* bpt1,
+                         const double* bpt2);
+int osurf_geo_circlexaabb2(const double* cent, double rad, const double* bpt1, const double* bpt2);
+
 /* rxnparam restatement, exported for the rate-table tests */
 double orxn_numrxnrate(double step, double a, double b);
 double orxn_numrxnrateprob(double step, double a, double b, double prob);

@@ -30,7 +30,8 @@
 
 uint32_t osim_randU32(osim* sim) { return osfmt_u32(&sim->sfmt); }
 double osim_randCOD(osim* sim) { return osfmt_u32(&sim->sfmt) * (1.0 / 4294967296.0); }
-static double o_randCCD(osim* sim) { return osfmt_u32(&sim->sfmt) * (1.0 / 4294967295.0); }
+double osim_randCCD(osim* sim) { return osfmt_u32(&sim->sfmt) * (1.0 / 4294967295.0); }
+static double o_randCCD(osim* sim) { return osim_randCCD(sim); }
 static double o_randOOD(osim* sim) { return (((double)osfmt_u32(&sim->sfmt)) + 0.5) * (1.0 / 4294967296.0); }
 
 /* ------------------------------------------------------------------------ */
@@ -1317,7 +1318,7 @@ static double o_wallcalcdist2(osim* sim, const double* p1, const double* p2, int
   return dist2;
 }
 
-int osurf_rxnXsurface(osim* sim, omol* m1, omol* m2);
+int osurf_rxnXsurface_j(osim* sim, omol* m1, omol* m2, int j);
 
 /* morebireact (smolreact.c:3109-3158), solution molecules */
 static int o_morebireact(osim* sim, int r, omol* mp1, omol* mp2, int ll1, int m1, int ll2, int et,
@@ -1390,7 +1391,7 @@ static int o_bireact(osim* sim, int neigh) {
               double dist2 = 0;
               for (d = 0; d < sim->dim; d++) dist2 += (mp1->pos[d] - mp2->pos[d]) * (mp1->pos[d] - mp2->pos[d]);
               if (dist2 <= rxn->bindrad2 && o_bicoin(sim, rxn, mp1, mp2, j) &&
-                  (mp1->mstate != OMS_SOLN || mp2->mstate != OMS_SOLN || !osurf_rxnXsurface(sim, mp1, mp2)) &&
+                  (mp1->mstate != OMS_SOLN || mp2->mstate != OMS_SOLN || !osurf_rxnXsurface_j(sim, mp1, mp2, j)) &&
                   mp1->ident != 0 && mp2->ident != 0) {
                 if (o_morebireact(sim, ss->table[i][j], mp1, mp2, ll1, m1, ll2, 1, NULL)) return 1;
                 if (mp1->ident == 0) {
@@ -1423,7 +1424,7 @@ static int o_bireact(osim* sim, int neigh) {
                   for (d = 0; d < sim->dim; d++)
                     dist2 += (mp1->pos[d] - mp2->pos[d]) * (mp1->pos[d] - mp2->pos[d]);
                   ok = dist2 <= rxn->bindrad2 && o_bicoin(sim, rxn, mp1, mp2, j) &&
-                       (mp1->mstate != OMS_SOLN || mp2->mstate != OMS_SOLN || !osurf_rxnXsurface(sim, mp1, mp2)) &&
+                       (mp1->mstate != OMS_SOLN || mp2->mstate != OMS_SOLN || !osurf_rxnXsurface_j(sim, mp1, mp2, j)) &&
                        mp1->ident != 0 && mp2->ident != 0;
                 }
                 if (ok) {

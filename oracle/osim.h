@@ -142,6 +142,7 @@ omol* osim_getnextmol(osim* sim);
 void osim_molkill(osim* sim, omol* m, int ll, int mi);
 void osim_molchangeident(osim* sim, omol* m, int ll, int mi, int i, int ms, int pnl);
 double osim_randCOD(osim* sim);
+double osim_randCCD(osim* sim);
 uint32_t osim_randU32(osim* sim);
 
 #endif

@@ -8,6 +8,7 @@
 void ref_randomize(long seed) { randomize(seed); }
 unsigned long ref_randULI(void) { return randULI(); }
 double ref_randCOD(void) { return randCOD(); }
+double ref_randCCD(void) { return randCCD(); }
 void ref_gausstable(double* a, int n, int shuffle) {
   randtableD(a, n, 1);
   if (shuffle) randshuffletableD(a, n);

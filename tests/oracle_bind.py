@@ -20,6 +20,10 @@ MS = {"solution": 0, "soln": 0, "front": 1, "back": 2, "up": 3, "down": 4, "bsol
 RP = {"none": 0, "irrev": 1, "confspread": 2, "bounce": 3, "pgem": 4, "pgemmax": 5, "pgemmaxw": 6,
       "ratio": 7, "unbindrad": 8, "pgem2": 9, "pgemmax2": 10, "ratio2": 11, "offset": 12, "fixed": 13}
 RNG_SFMT, RNG_COUNTER = 0, 1
+PS = {"rect": 0, "tri": 1, "sph": 2, "sphere": 2, "cyl": 3, "cylinder": 3, "hemi": 4, "hemisphere": 4, "disk": 5}
+FACE = {"front": 0, "f": 0, "back": 1, "b": 1, "none": 2, "both": 3}
+SA = {"reflect": 0, "transmit": 1, "trans": 1, "absorb": 2, "jump": 3, "port": 4, "mult": 5, "no": 6,
+      "adsorb": 8, "revdes": 9, "irrevdes": 10, "flip": 11}
 
 _lib = None
 
@@ -79,6 +83,19 @@ def lib():
             "orxn_bindingradius": (D, [D, D, D, D, I]),
             "orxn_bindingradiusprob": (D, [D, D, D, D, I, D, dp]),
             "orxn_unbindingradius": (D, [D, D, D, D]),
+            "osim_add_surface": (I, [P, C.c_char_p]),
+            "osim_set_surface_action": (I, [P, I, I, I, I, I, I]),
+            "osim_set_surface_rate": (I, [P, I, I, I, I, I, I, D, I]),
+            "osim_add_panel": (I, [P, I, I, C.c_char_p, dp]),
+            "osim_set_panel_jump": (I, [P, I, I, I, I, I]),
+            "osim_box_panels": (I, [P, I, ip, I]),
+            "osurf_geo_spherenormal": (D, [dp, dp, I, I, dp]),
+            "osurf_geo_ptinslab": (I, [dp, dp, dp, I]),
+            "osurf_geo_linexsphs": (D, [dp, dp, dp, D, I, dp, dp, dp]),
+            "osurf_geo_rectxaabb3": (I, [dp, dp, dp, dp, dp, dp]),
+            "osurf_geo_sphsxaabb3": (I, [dp, D, dp, dp]),
+            "osurf_geo_diskxaabb3": (I, [dp, D, dp, dp, dp]),
+            "osurf_desorbdist_y": (D, [D, D, I]),
             "omath_randtable": (None, [dp, I]),
             "omath_erfn": (D, [D]),
         }
@@ -254,6 +271,34 @@ class OSim:
         self.L.osim_rxnlog(self.p, rows.ctypes.data_as(C.POINTER(C.c_int64)), n)
         return rows[:n]
 
+    def add_surface(self, name):
+        s = self.L.osim_add_surface(self.p, name.encode())
+        assert s >= 0
+        return s
+
+    def surface_action(self, s, ident, ms, face, act, newident=-1):
+        er = self.L.osim_set_surface_action(self.p, s, ident, ms, face, act, newident)
+        assert er == 0, er
+
+    def surface_rate(self, s, ident, ms, ms1, ms2, value, which=1, newident=-5):
+        er = self.L.osim_set_surface_rate(self.p, s, ident, ms, ms1, ms2, newident, value, which)
+        assert er == 0, er
+
+    def add_panel(self, s, ps, params, axis=None):
+        prm = np.zeros(16)
+        prm[:len(params)] = params
+        p = self.L.osim_add_panel(self.p, s, ps, axis.encode() if axis else None, _dp(prm))
+        assert p >= 0, p
+        return p
+
+    def panel_jump(self, p1, f1, p2, f2, bidirectional):
+        assert self.L.osim_set_panel_jump(self.p, p1, f1, p2, f2, 1 if bidirectional else 0) == 0
+
+    def box_panels(self, b):
+        out = np.zeros(256, np.int32)
+        n = self.L.osim_box_panels(self.p, b, _ip(out), 256)
+        return out[:max(n, 0)].tolist() if n >= 0 else None
+
     def gausstable(self, flat=True):
         n = self.L.osim_ngauss(self.p)
         ptr = self.L.osim_gausstable(self.p, 1 if flat else 0)
@@ -280,6 +325,15 @@ def reflib():
         ("bindingradius", D, [D, D, D, D, I]),
         ("bindingradiusprob", D, [D, D, D, D, I, D, dp]),
         ("unbindingradius", D, [D, D, D, D]),
+        ("Geo_SphereNormal", D, [dp, dp, I, I, dp]),
+        ("Geo_PtInSlab", I, [dp, dp, dp, I]),
+        ("Geo_LineXSphs", D, [dp, dp, dp, D, I, dp, dp, dp]),
+        ("Geo_RectXaabb3", I, [dp, dp, dp, dp, dp, dp]),
+        ("Geo_SphsXaabb3", I, [dp, D, dp, dp]),
+        ("Geo_DiskXaabb3", I, [dp, D, dp, dp, dp]),
+        ("surfaceprob", D, [D, D, D, D, dp, I]),
+        ("desorbdist", D, [D, I]),
+        ("ref_randCCD", D, []),
     ]:
         f = getattr(L, name)
         f.restype = res

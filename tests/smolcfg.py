@@ -12,6 +12,14 @@ import re
 IGNORED = {"graphics", "graphic_iter", "graphic_delay", "color", "display_size", "text_display",
            "output_files", "cmd", "frame_thickness", "frame_color", "grid_thickness", "background_color",
            "tiff_name", "tiff_iter", "light", "quit_at_end", "end_file"}
+# display-only statements inside a surface block (surfreadstring, smolsurface.c)
+SURF_IGNORED = {"color", "thickness", "polygon", "shininess", "stipple", "max_panels", "end_surface"}
+# panel parameter counts by shape and dimension (surfreadstring "panel")
+PANEL_NPARAM = {("rect", 1): 1, ("rect", 2): 3, ("rect", 3): 5, ("tri", 1): 1, ("tri", 2): 4, ("tri", 3): 9,
+                ("sph", 1): 2, ("sph", 2): 4, ("sph", 3): 6, ("disk", 2): 5, ("disk", 3): 8}
+PANEL_SHAPE = {"rect": "rect", "r": "rect", "tri": "tri", "t": "tri", "sph": "sph", "sphere": "sph", "s": "sph",
+               "disk": "disk", "d": "disk", "cyl": "cyl", "hemi": "hemi"}
+STATES = {"soln": 0, "fsoln": 0, "front": 1, "back": 2, "up": 3, "down": 4, "bsoln": 5, "all": 6}
 
 
 class Unsupported(Exception):
@@ -110,8 +118,24 @@ def read_config(text, sim_factory, defines=None):
         sim.wall(d, side, t)
     times = [0.0, None, None]
     rxns = {}
+    surfaces = {}
+    panels = {}  # (surface, panel name) -> panel id
+    cur = [None, {}]  # current surface index, shape counters
     for words in pending:
         k = words[0]
+        if cur[0] is not None and k != "end_surface" and k not in ("start_surface",):
+            _surface_statement(sim, dim, cur, words, panels)
+            continue
+        if k == "start_surface":
+            name = words[1]
+            if name not in surfaces:
+                surfaces[name] = sim.add_surface(name)
+            cur[0] = surfaces[name]
+            cur[1] = cur[1] if cur[0] in panels.get("__counters__", {}) else {}
+            continue
+        if k == "end_surface":
+            cur[0] = None
+            continue
         if k == "random_seed":
             sim.seed(int(words[1]))
         elif k == "time_start":
@@ -170,3 +194,61 @@ def read_config(text, sim_factory, defines=None):
         else:
             raise Unsupported(k)
     return sim
+
+
+def _species_state(sim, tok):
+    """species[(state)] -> (ident, state); 'all' is ident -5 (molstring2index1)."""
+    ms = 0
+    if "(" in tok:
+        tok, st = tok[:-1].split("(")
+        ms = STATES[st]
+    return (-5 if tok == "all" else sim.index(tok)), ms
+
+
+def _surface_statement(sim, dim, cur, words, panels):
+    from oracle_bind import FACE, PS, SA
+    s = cur[0]
+    k = words[0]
+    if k in SURF_IGNORED:
+        return
+    if k == "action":
+        # new format: species face action; old format: face species action
+        a, b = words[1], words[2]
+        if b in FACE and b not in ("none",):
+            sp, face = a, b
+        else:
+            sp, face = b, a
+        ident, ms = _species_state(sim, sp)
+        newident = sim.index(words[4]) if len(words) > 4 else ident
+        sim.surface_action(s, ident, ms, FACE[face], SA[words[3]], newident)
+    elif k in ("rate", "rate_internal"):
+        ident, ms = _species_state(sim, words[1])
+        ms1, ms2 = STATES[words[2]], STATES[words[3]]
+        newident = sim.index(words[5]) if len(words) > 5 else -5
+        sim.surface_rate(s, ident, ms, ms1, ms2, float(words[4]), 1 if k == "rate" else 2, newident)
+    elif k == "panel":
+        shape = PANEL_SHAPE[words[1]]
+        toks = words[2:]
+        axis = None
+        if shape == "rect":
+            axis, toks = toks[0], toks[1:]
+        npar = PANEL_NPARAM.get((shape, dim))
+        if npar is None:
+            raise Unsupported("panel %s in %dD" % (shape, dim))
+        nums, name = [], None
+        for t in toks:
+            try:
+                nums.append(float(t))
+            except ValueError:
+                name = t
+        if shape in ("sph", "disk") and len(nums) < npar:  # drawing slices/stacks default
+            nums += [20.0] * (npar - len(nums))
+        cnt = cur[1].get(shape, 0)
+        cur[1][shape] = cnt + 1
+        pid = sim.add_panel(s, PS[shape], nums[:npar], axis)
+        panels[(s, name or "%s%d" % (shape, cnt))] = pid
+    elif k == "jump":
+        p1, f1, arrow, p2, f2 = words[1:6]
+        sim.panel_jump(panels[(s, p1)], FACE[f1], panels[(s, p2)], FACE[f2], arrow == "<->")
+    else:
+        raise Unsupported("surface statement " + k)

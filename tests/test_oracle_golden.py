@@ -6,7 +6,11 @@ they pin the whole CPU path: Gaussian-table construction and shuffle, molecule
 placement, diffusion, reflective and periodic walls, virtual-box grid and
 neighbor lists (including wrap codes), live-list and dead-list ordering,
 first- and second-order reactions with their greedy pair order, product
-placement and serial numbers."""
+placement and serial numbers.  surf1 (1D) and ReflectToNewSpecies (2D) pin the
+surface path: rectangle, triangle and sphere panels, reflect / transmit /
+absorb / jump actions, species change on reflection, box panel lists and the
+box walk along a trajectory (line2nextbox), and bimolecular reactions with
+surfaces present (rxnXsurface)."""
 import gzip
 from pathlib import Path
 
@@ -25,7 +29,7 @@ def run_model(name):
     return code, sim.listmols()
 
 
-@pytest.mark.parametrize("name", ["bounce2", "molecule", "lotvolt"])
+@pytest.mark.parametrize("name", ["bounce2", "molecule", "lotvolt", "surf1", "ReflectToNewSpecies"])
 def test_s95_output_identical(name):
     code, out = run_model(name)
     assert code == 1

@@ -161,3 +161,77 @@ def test_rxnparam_golden_fixture():
     for row in data["unbindingradius"]:
         pgem, dt, difc, a, want = row
         assert L.orxn_unbindingradius(pgem, dt, difc, a) == want
+
+
+# ---------------------------------------------------------------------------
+# surface geometry: the oracle's restatements against the reference's own
+# Geometry.c / SurfaceParam.c compiled into oracle/_ref
+
+
+@pytest.fixture(scope="module")
+def refsurf():
+    if REF is None:
+        pytest.skip("reference checker library not built (no reference tree)")
+    return REF
+
+
+def test_geo_linexsphs_and_spherenormal_bit_exact(refsurf):
+    L = ob.lib()
+    rng = np.random.default_rng(5)
+    for dim in (1, 2, 3):
+        for _ in range(400):
+            p1, p2, c = rng.normal(size=dim) * 5, rng.normal(size=dim) * 5, rng.normal(size=dim)
+            rad = abs(rng.normal()) * 4 + 0.1
+            o = [np.zeros(1) for _ in range(3)]
+            r = [np.zeros(1) for _ in range(3)]
+            xo = L.osurf_geo_linexsphs(_dp(p1), _dp(p2), _dp(c), rad, dim, *[_dp(a) for a in o])
+            xr = refsurf.Geo_LineXSphs(_dp(p1), _dp(p2), _dp(c), rad, dim, *[_dp(a) for a in r])
+            assert np.array_equal(np.array([xo] + [a[0] for a in o]).view(np.uint64),
+                                  np.array([xr] + [a[0] for a in r]).view(np.uint64)) or \
+                (np.isnan(xo) and np.isnan(xr))
+            no, nr = np.zeros(3), np.zeros(3)
+            fo = L.osurf_geo_spherenormal(_dp(c), _dp(p1), 1, dim, _dp(no))
+            fr = refsurf.Geo_SphereNormal(_dp(c), _dp(p1), 1, dim, _dp(nr))
+            assert fo == fr and np.array_equal(no, nr)
+
+
+def test_geo_box_intersection_tests_match(refsurf):
+    """Panel-in-box tests decide each box's panel list (panelinbox)."""
+    L = ob.lib()
+    rng = np.random.default_rng(6)
+    for _ in range(3000):
+        b1 = rng.uniform(-10, 10, 3)
+        b2 = b1 + rng.uniform(0.1, 5, 3)
+        r1, r2, r3 = rng.uniform(-12, 12, (3, 3))
+        assert L.osurf_geo_rectxaabb3(_dp(r1), _dp(r2), _dp(r3), _dp(r1), _dp(b1), _dp(b2)) == \
+            refsurf.Geo_RectXaabb3(_dp(r1), _dp(r2), _dp(r3), _dp(r1), _dp(b1), _dp(b2))
+        c = rng.uniform(-12, 12, 3)
+        rad = rng.uniform(0.1, 8)
+        assert L.osurf_geo_sphsxaabb3(_dp(c), rad, _dp(b1), _dp(b2)) == \
+            refsurf.Geo_SphsXaabb3(_dp(c), rad, _dp(b1), _dp(b2))
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        assert L.osurf_geo_diskxaabb3(_dp(c), rad, _dp(n), _dp(b1), _dp(b2)) == \
+            refsurf.Geo_DiskXaabb3(_dp(c), rad, _dp(n), _dp(b1), _dp(b2))
+        assert L.osurf_geo_ptinslab(_dp(r1), _dp(r2), _dp(c), 3) == refsurf.Geo_PtInSlab(_dp(r1), _dp(r2), _dp(c), 3)
+
+
+def test_desorption_distance_matches_reference(refsurf):
+    """desorbdist (SurfaceParam.c:202-220) for irreversible desorption (algo
+    SPAirrDes=10) and reversible adsorption (SPArevAds=8), same uniform."""
+    L = ob.lib()
+    for seed in range(1, 200):
+        for algo, rev in ((10, 0), (8, 1)):
+            refsurf.ref_randomize(seed)
+            xr = refsurf.desorbdist(0.37, algo)
+            refsurf.ref_randomize(seed)
+            y = refsurf.ref_randCCD()
+            assert L.osurf_desorbdist_y(0.37, y, rev) == xr
+
+
+def test_irreversible_desorption_probability_closed_form(refsurf):
+    """surfaceprob IrrDes (SurfaceParam.c:159-163) is k1/k2 (1 - exp(-k2 dt)),
+    which the oracle's rate setup uses for desorption."""
+    for k, dt in ((0.1, 0.002), (3.0, 0.01), (50.0, 0.001)):
+        p = refsurf.surfaceprob(k, 0.0, dt, 1.0, None, 10)
+        assert p == 1.0 - np.exp(-k * dt)
