@@ -95,8 +95,28 @@ enum ErrorCode smolAddSolutionMolecules(simptr sim, const char* species, int num
                                         double* highposition);                /* libsmoldyn.h:114 */
 int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state); /* libsmoldyn.h:117 */
 
-/* ---------------- boundaries (libsmoldyn.h:122) ---------------- */
+/* ---------------- boundaries and surfaces (libsmoldyn.h:122-136) ---------------- */
 enum ErrorCode smolSetBoundaryType(simptr sim, int dimension, int highside, char type); /* :122 */
+enum ErrorCode smolAddSurface(simptr sim, const char* surface);               /* libsmoldyn.h:123 */
+int smolGetSurfaceIndex(simptr sim, const char* surface);                     /* libsmoldyn.h:124 */
+int smolGetSurfaceIndexNT(simptr sim, const char* surface);                   /* libsmoldyn.h:125 */
+enum ErrorCode smolSetSurfaceAction(simptr sim, const char* surface, enum PanelFace face, const char* species,
+                                    enum MolecState state, enum SrfAction action,
+                                    const char* newspecies);                  /* libsmoldyn.h:128 */
+enum ErrorCode smolSetSurfaceRate(simptr sim, const char* surface, const char* species, enum MolecState state,
+                                  enum MolecState state1, enum MolecState state2, double rate,
+                                  const char* newspecies, int isinternal);    /* libsmoldyn.h:129 */
+/* Panel shapes on the device path: PSrect, PSsph, PSdisk (3D). */
+enum ErrorCode smolAddPanel(simptr sim, const char* surface, enum PanelShape panelshape, const char* panel,
+                            const char* axisstring, double* params);          /* libsmoldyn.h:130 */
+int smolGetPanelIndex(simptr sim, const char* surface, enum PanelShape* panelshapeptr,
+                      const char* panel);                                     /* libsmoldyn.h:131 */
+int smolGetPanelIndexNT(simptr sim, const char* surface, enum PanelShape* panelshapeptr,
+                        const char* panel);                                   /* libsmoldyn.h:132 */
+enum ErrorCode smolSetPanelJump(simptr sim, const char* surface, const char* panel1, enum PanelFace face1,
+                                const char* panel2, enum PanelFace face2,
+                                int isbidirectional);                         /* libsmoldyn.h:134 */
+enum ErrorCode smolSetSurfaceSimParams(simptr sim, const char* parameter, double value); /* :136 */
 
 /* ---------------- reactions (libsmoldyn.h:152-158) ---------------- */
 enum ErrorCode smolAddReaction(simptr sim, const char* reaction, const char* reactant1, enum MolecState rstate1,

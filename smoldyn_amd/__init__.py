@@ -81,6 +81,16 @@ def lib():
         "smolAddSolutionMolecules": (I, [P, cp, I, dp, dp]),
         "smolGetMoleculeCount": (I, [P, cp, I]),
         "smolSetBoundaryType": (I, [P, I, I, C.c_char]),
+        "smolAddSurface": (I, [P, cp]),
+        "smolGetSurfaceIndex": (I, [P, cp]),
+        "smolGetSurfaceIndexNT": (I, [P, cp]),
+        "smolSetSurfaceAction": (I, [P, cp, I, cp, I, I, cp]),
+        "smolSetSurfaceRate": (I, [P, cp, cp, I, I, I, D, cp, I]),
+        "smolAddPanel": (I, [P, cp, I, cp, cp, dp]),
+        "smolGetPanelIndex": (I, [P, cp, ip, cp]),
+        "smolGetPanelIndexNT": (I, [P, cp, ip, cp]),
+        "smolSetPanelJump": (I, [P, cp, cp, I, cp, I, I]),
+        "smolSetSurfaceSimParams": (I, [P, cp, D]),
         "smolAddReaction": (I, [P, cp, cp, I, cp, I, I, C.POINTER(cp), ip, D]),
         "smolGetReactionIndex": (I, [P, ip, cp]),
         "smolGetReactionIndexNT": (I, [P, ip, cp]),
@@ -146,6 +156,35 @@ class Simulation:
 
     def __del__(self):
         self.close()
+
+    # ---- surfaces (libsmoldyn.h:123-136)
+    FACES = {"front": 0, "back": 1, "both": 3}
+    SHAPES = {"rect": 0, "sph": 2, "sphere": 2, "disk": 5}
+    ACTIONS = {"reflect": 0, "transmit": 1, "absorb": 2, "jump": 3, "port": 4, "mult": 5}
+    STATES = {"soln": 0, "fsoln": 0, "front": 1, "back": 2, "up": 3, "down": 4, "bsoln": 5, "all": 6}
+
+    def add_surface(self, name):
+        _check(self.L.smolAddSurface(self.p, name.encode()), "smolAddSurface")
+
+    def surface_action(self, surface, face, species, action, state="soln"):
+        _check(self.L.smolSetSurfaceAction(self.p, surface.encode(), self.FACES[face], species.encode(),
+                                           self.STATES[state], self.ACTIONS[action], None), "smolSetSurfaceAction")
+
+    def surface_rate(self, surface, species, state, state1, state2, rate, newspecies=None, internal=False):
+        _check(self.L.smolSetSurfaceRate(self.p, surface.encode(), species.encode(), self.STATES[state],
+                                         self.STATES[state1], self.STATES[state2], rate,
+                                         newspecies.encode() if newspecies else None, 1 if internal else 0),
+               "smolSetSurfaceRate")
+
+    def add_panel(self, surface, shape, params, axis=None, name=None):
+        prm = np.zeros(16)
+        prm[:len(params)] = params
+        _check(self.L.smolAddPanel(self.p, surface.encode(), self.SHAPES[shape], name.encode() if name else None,
+                                   axis.encode() if axis else None, _dp(prm)), "smolAddPanel")
+
+    def panel_jump(self, surface, panel1, face1, panel2, face2, bidirectional=True):
+        _check(self.L.smolSetPanelJump(self.p, surface.encode(), panel1.encode(), self.FACES[face1], panel2.encode(),
+                                       self.FACES[face2], 1 if bidirectional else 0), "smolSetPanelJump")
 
     def boundary(self, d, high, t):
         _check(self.L.smolSetBoundaryType(self.p, d, high, t.encode()), "smolSetBoundaryType")

@@ -21,6 +21,7 @@
 #include "../../include/smoldyn_amd_rng.h"
 #include "engine.h"
 #include "rxnparam.h"
+#include "surfaces_host.h"
 
 using namespace smolamd;
 
@@ -89,6 +90,7 @@ struct smolamd_sim {
   DevTables tab;
   std::vector<double> gtable;
   int counts[kMaxSpecies] = {0};
+  SurfaceSet surf;
 };
 
 namespace {
@@ -563,7 +565,8 @@ ErrorCode do_update(simptr sim, const char* fn) {
   // neighborZV type 7 reads the first dim entries of the per-wall periodic
   // flags as per-dimension flags (Zn.c:263, smolboxes.c:772-776)
   LCHECK(sim->accur >= 9, fn, ECerror, "accuracy below 9 selects neighbor lists the device path does not build");
-  for (int d = 0; d < 3; d++) g.wrapflag[d] = (sim->wtype[d] == 'p');
+  // no neighbor wrapping once surfaces exist (smolboxes.c:772-776)
+  for (int d = 0; d < 3; d++) g.wrapflag[d] = (sim->wtype[d] == 'p' && !sim->surf.enabled());
 
   // reactions (rxnsupdate, smolreact.c:2384-2401)
   for (int order = 1; order <= 2; order++)
@@ -648,17 +651,18 @@ ErrorCode do_update(simptr sim, const char* fn) {
     }
   }
   {
-    bool transparent = false;
     double ext = 0;
-    for (int w = 0; w < 6; w++) transparent |= sim->wtype[w] == 't';
     for (int d = 0; d < 3; d++) {
       ext = fmax(ext, fabs(sim->low[d]));
       ext = fmax(ext, fabs(sim->high[d]));
       ext = fmax(ext, sim->high[d] - sim->low[d]);
     }
-    // FP32 rounding of coordinates of magnitude <= 2*ext perturbs a distance by
-    // less than 5 * 2^-23 * 2*ext; the margin is three times that
-    g.prefilter_margin = transparent ? -1.0 : 32.0 * ext * ldexp(1.0, -23);
+    // Molecules with a coordinate beyond 2*ext bypass the FP32 screen (the
+    // engine flags them).  For the others, FP32 rounding of the coordinates
+    // (<= 2*ext), of the periodic shift (<= ext) and of their difference
+    // perturbs a distance by less than 6 * ext * 2^-23; the margin is 32 * ext
+    // * 2^-23.
+    g.prefilter_margin = 32.0 * ext * ldexp(1.0, -23);
   }
   // rxnsupdatelists order 2 (smolreact.c:2352-2377) and the (ll1,ll2) loop rank
   for (size_t r = 0; r < sim->rxn[2].size(); r++) {
@@ -721,10 +725,39 @@ ErrorCode do_update(simptr sim, const char* fn) {
   }
   cfg.maxedges = (int)(captot / 2 > 65536 ? captot / 2 : 65536);
   cfg.maxevents = (int)(captot / 16 > 16384 ? captot / 16 : 16384);
+  // surfaces (surfupdate, smolsurface.c:3797-3893, and the box panel lists)
+  SurfaceTables st;
+  HostSurf hs;
+  memset(&hs, 0, sizeof(hs));
+  if (sim->surf.enabled()) {
+    std::vector<double> dsol(nsp);
+    std::vector<int> lsol(nsp);
+    bool mob[kMaxList] = {false};
+    for (int i = 0; i < nsp; i++) {
+      dsol[i] = sim->difc[i][MSsoln];
+      lsol[i] = lookup[i][MSsoln];
+    }
+    for (int i = 1; i < nsp; i++)
+      for (int ms = 0; ms < kMS; ms++)
+        if (ismobile(sim, i, ms) && lookup[i][ms] >= 0) mob[lookup[i][ms]] = true;
+    char msg[400] = "";
+    int er = sim->surf.build(g, nsp, dsol.data(), sim->dt, lsol.data(), mob, nlist, st, msg, sizeof(msg));
+    LCHECK(er == 0, fn, ECerror, "%s", msg);
+    hs.on = 1;
+    hs.npnl = (int)st.pnl.size();
+    hs.pnl = st.pnl.data();
+    hs.nbox = g.nbox;
+    hs.bpstart = st.bpstart.data();
+    hs.bplist = st.bplist.data();
+    hs.nsrf = (int)sim->surf.srf.size();
+    hs.act = st.act.data();
+    for (int ll = 0; ll < kMaxList; ll++) hs.listdiffuse[ll] = st.listdiffuse[ll];
+    hs.epsilon = sim->surf.epsilon;
+  }
   if (sim->dev) engine_destroy(sim->dev);
   char err[256] = "";
-  sim->dev = engine_create(cfg, t, g, sim->gtable.data(), 4096, (uint32_t)sim->key, (uint32_t)(sim->key >> 32), err,
-                           sizeof(err));
+  sim->dev = engine_create(cfg, t, g, sim->gtable.data(), 4096, (uint32_t)sim->key, (uint32_t)(sim->key >> 32), &hs,
+                           err, sizeof(err));
   LCHECK(sim->dev, fn, ECmemory, "device engine: %s", err);
   for (int l = 0; l < nlist; l++) {
     int er = engine_upload_list(sim->dev, l, (int)ident[l].size(), ident[l].data(), pos[l].data(), ser[l].data());
@@ -1176,6 +1209,167 @@ enum ErrorCode smolSetBoundaryType(simptr sim, int dimension, int highside, char
       if (highside >= 0 && side != highside) continue;
       sim->wtype[2 * d + side] = type;
     }
+  }
+  invalidate(sim);
+  return ECok;
+}
+
+// ---------------------------------------------------------------------------
+// surfaces (libsmoldyn.cpp:1240-1480)
+
+enum ErrorCode smolAddSurface(simptr sim, const char* surface) {
+  const char* fn = "smolAddSurface";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(surface && surface[0], fn, ECmissing, "missing surface name");
+  LCHECK(strcmp(surface, "all"), fn, ECall, "surface cannot be 'all'");
+  LCHECK(sim->surf.find_surface(surface) < 0, fn, ECerror, "surface is already in system");
+  LCHECK((int)sim->surf.srf.size() < kMaxSurf, fn, ECmemory, "too many surfaces");
+  sim->surf.add_surface(surface);
+  invalidate(sim);
+  return ECok;
+}
+
+int smolGetSurfaceIndexNT(simptr sim, const char* surface) {
+  if (!sim || !surface) return (int)ECmissing;
+  if (!strcmp(surface, "all")) return (int)ECall;
+  int s = sim->surf.find_surface(surface);
+  return s < 0 ? (int)ECnonexist : s;
+}
+
+int smolGetSurfaceIndex(simptr sim, const char* surface) {
+  int s = smolGetSurfaceIndexNT(sim, surface);
+  if (s < 0) smolSetError("smolGetSurfaceIndex", (ErrorCode)s, "surface not found", "");
+  return s;
+}
+
+// the reference passes no new species to surfsetaction here (libsmoldyn.cpp:1322-1330)
+enum ErrorCode smolSetSurfaceAction(simptr sim, const char* surface, enum PanelFace face, const char* species,
+                                    enum MolecState state, enum SrfAction action, const char* newspecies) {
+  const char* fn = "smolSetSurfaceAction";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  int s = smolGetSurfaceIndexNT(sim, surface);
+  LCHECK(s >= 0 || s == (int)ECall, fn, ECnonexist, "surface not found");
+  LCHECK(face == PFfront || face == PFback || face == PFboth, fn, ECbounds, "invalid face");
+  int i = smolGetSpeciesIndexNT(sim, species);
+  LCHECK(i > 0 || i == (int)ECall, fn, ECnonexist, "species not found");
+  if (i == (int)ECall) i = -5;
+  LCHECK((state >= 0 && state < kMS) || state == MSall, fn, ECbounds, "invalid state");
+  LCHECK(action >= 0 && action <= SAmult, fn, ECbounds, "invalid action");
+  if (newspecies && newspecies[0]) LCHECK(species_index(sim, newspecies) > 0, fn, ECnonexist, "unrecognized new species name");
+  int nsp = (int)sim->spname.size();
+  for (int k = 0; k < (int)sim->surf.srf.size(); k++)
+    if (s < 0 || k == s) sim->surf.set_action(k, i, nsp, state, face, action);
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolSetSurfaceRate(simptr sim, const char* surface, const char* species, enum MolecState state,
+                                  enum MolecState state1, enum MolecState state2, double rate, const char* newspecies,
+                                  int isinternal) {
+  const char* fn = "smolSetSurfaceRate";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  int s = smolGetSurfaceIndexNT(sim, surface);
+  LCHECK(s >= 0 || s == (int)ECall, fn, ECnonexist, "surface not found");
+  int i = smolGetSpeciesIndexNT(sim, species);
+  LCHECK(i > 0 || i == (int)ECall, fn, ECnonexist, "species not found");
+  if (i == (int)ECall) i = -5;
+  LCHECK(state >= 0 && state < kMS, fn, ECbounds, "invalid state");
+  LCHECK(state1 >= 0 && state1 < kMS1, fn, ECbounds, "invalid state1");
+  LCHECK(state == MSsoln || state1 == MSsoln || state1 == MSbsoln || state1 == state, fn, ECsyntax,
+         "nonsensical state combination");
+  LCHECK(state2 >= 0 && state2 < kMS1, fn, ECbounds, "invalid state2");
+  LCHECK(state1 != state2, fn, ECsyntax, "cannot set rate for state1 = state2");
+  int i2 = -5;
+  if (newspecies && newspecies[0]) {
+    i2 = species_index(sim, newspecies);
+    LCHECK(i2 > 0, fn, ECerror, "invalid newspecies");
+  }
+  LCHECK(rate >= 0, fn, ECbounds, "rate needs to be non-negative");
+  LCHECK(!(isinternal && rate > 1), fn, ECbounds, "internal rate needs to be <= 1");
+  int nsp = (int)sim->spname.size();
+  for (int k = 0; k < (int)sim->surf.srf.size(); k++)
+    if (s < 0 || k == s) {
+      int er = sim->surf.set_rate(k, i, nsp, state, state1, state2, i2, rate, isinternal ? 2 : 1);
+      LCHECK(!er, fn, ECerror, "error in surfsetrate");
+    }
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolAddPanel(simptr sim, const char* surface, enum PanelShape panelshape, const char* panel,
+                            const char* axisstring, double* params) {
+  const char* fn = "smolAddPanel";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  int s = smolGetSurfaceIndexNT(sim, surface);
+  LCHECK(s >= 0, fn, ECnonexist, "surface not found");
+  LCHECK(panelshape >= 0 && panelshape < PSall, fn, ECnonexist, "invalid panel shape");
+  if (panelshape == PSrect) LCHECK(axisstring, fn, ECmissing, "missing axisstring");
+  LCHECK(params, fn, ECmissing, "missing params");
+  LCHECK(panelshape == PSrect || panelshape == PSsph || panelshape == PSdisk, fn, ECerror,
+         "panel shape %d is not implemented on the device path (rectangles, spheres and disks are)", panelshape);
+  int er = sim->surf.add_panel(s, panelshape, panel, axisstring, params);
+  LCHECK(er != -3, fn, ECsyntax, "cannot parse axisstring");
+  LCHECK(er != -4, fn, ECbounds, "drawing slices and stacks need to be positive");
+  LCHECK(er != -7, fn, ECbounds, "radius needs to be positive");
+  LCHECK(er != -8, fn, ECbounds, "normal vector has zero length");
+  LCHECK(er != -9, fn, ECerror, "panel name was used before for a different panel shape");
+  LCHECK(er >= 0, fn, ECbug, "bug in smolAddPanel");
+  invalidate(sim);
+  return ECok;
+}
+
+int smolGetPanelIndexNT(simptr sim, const char* surface, enum PanelShape* panelshapeptr, const char* panel) {
+  if (!sim || !panel) return (int)ECmissing;
+  int s = smolGetSurfaceIndexNT(sim, surface);
+  if (s < 0) return s;
+  if (!strcmp(panel, "all")) return (int)ECall;
+  int id = sim->surf.find_panel(s, panel);
+  if (id < 0) return (int)ECnonexist;
+  const HostPanel& p = sim->surf.pnl[id];
+  if (panelshapeptr) *panelshapeptr = (enum PanelShape)p.ps;
+  const std::vector<int>& v = sim->surf.srf[s]->panels[p.ps];
+  for (size_t k = 0; k < v.size(); k++)
+    if (v[k] == id) return (int)k;
+  return (int)ECbug;
+}
+
+int smolGetPanelIndex(simptr sim, const char* surface, enum PanelShape* panelshapeptr, const char* panel) {
+  int p = smolGetPanelIndexNT(sim, surface, panelshapeptr, panel);
+  if (p < 0) smolSetError("smolGetPanelIndex", (ErrorCode)p, "panel not found", "");
+  return p;
+}
+
+enum ErrorCode smolSetPanelJump(simptr sim, const char* surface, const char* panel1, enum PanelFace face1,
+                                const char* panel2, enum PanelFace face2, int isbidirectional) {
+  const char* fn = "smolSetPanelJump";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  int s = smolGetSurfaceIndexNT(sim, surface);
+  LCHECK(s >= 0, fn, ECnonexist, "surface not found");
+  LCHECK(panel1 && panel2, fn, ECmissing, "missing panel name");
+  int p1 = sim->surf.find_panel(s, panel1), p2 = sim->surf.find_panel(s, panel2);
+  LCHECK(p1 >= 0 && p2 >= 0, fn, ECnonexist, "panel not found");
+  LCHECK(face1 == PFfront || face1 == PFback, fn, ECbounds, "face1 must be front or back");
+  LCHECK(face2 == PFfront || face2 == PFback, fn, ECbounds, "face2 must be front or back");
+  LCHECK(sim->surf.pnl[p1].ps == sim->surf.pnl[p2].ps, fn, ECerror, "jumping panels must have the same shape");
+  sim->surf.set_jump(p1, face1, p2, face2, isbidirectional);
+  invalidate(sim);
+  return ECok;
+}
+
+enum ErrorCode smolSetSurfaceSimParams(simptr sim, const char* parameter, double value) {
+  const char* fn = "smolSetSurfaceSimParams";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(parameter, fn, ECmissing, "missing parameter name");
+  if (!strcmp(parameter, "epsilon")) {
+    LCHECK(value > 0, fn, ECbounds, "epsilon needs to be positive");
+    sim->surf.epsilon = value;
+  } else if (!strcmp(parameter, "margin")) {
+    LCHECK(value >= 0, fn, ECbounds, "margin needs to be non-negative");
+    sim->surf.margin = value;
+  } else if (!strcmp(parameter, "neighbordist")) {
+    sim->surf.neighdist = value;
+  } else {
+    LCHECK(0, fn, ECsyntax, "parameter name not recognized");
   }
   invalidate(sim);
   return ECok;

@@ -67,8 +67,8 @@ struct Geometry {
   int nbox;
   // FP32 pre-filter of the pair search: a pair is tested exactly (FP64) when
   // its FP32 distance is within binding radius + prefilter_margin; the margin
-  // bounds the FP32 rounding of coordinates up to the system extent.  A
-  // negative margin turns the pre-filter off (transparent walls).
+  // bounds the FP32 rounding of coordinates up to twice the system extent
+  // (molecules beyond that bypass the pre-filter).
   double prefilter_margin;
 };
 
@@ -97,6 +97,46 @@ enum ErrBits : int {
   kErrEventOverflow = 2,
   kErrCapacity = 4,
   kErrSortOverflow = 8,
+  kErrSurface = 16,  // a surface action the device path does not run (absorption, adsorption, list change)
+};
+
+// ---------------------------------------------------------------------------
+// surfaces (solution-phase molecules; panels: 3D rectangle, sphere, disk)
+
+constexpr int kMaxSurf = 16;
+enum DevPanelShape : int { kPsRect = 0, kPsSph = 2, kPsDisk = 5 };
+enum DevSrfAction : int { kSaReflect = 0, kSaTrans = 1, kSaAbsorb = 2, kSaJump = 3, kSaMult = 5, kSaAdsorb = 8 };
+
+struct DevPanel {
+  int ps;           // kPsRect, kPsSph or kPsDisk
+  int srf;          // surface index
+  int jumpp[2];     // jump destination panel per face (-1 none)
+  int jumpf[2];     // destination face
+  double point[3][3];  // rect: corners 0..2; sph/disk: center, then radius in point[1][0]
+  double front[3];     // rect: (+-1, normal axis, edge axis); sph: (+-1); disk: unit normal
+};
+
+// Surface actions of solution-phase molecules, per surface, species and face
+// (0 front, 1 back).  A rate-based action (kSaMult) draws one uniform u and
+// takes the first state ms with u < cumprob[ms]; outcome[ms] is the action
+// that state means on that face and newspec[ms] the species after it.
+struct DevSrfAct {
+  int act;
+  int newspec_fixed;   // species after a fixed action
+  double cumprob[6];
+  int outcome[6];
+  int newspec[6];
+};
+
+struct DevSurf {
+  int on;                   // surfaces defined: walls are not applied, collisions are checked
+  int npnl;
+  int listdiffuse[kMaxList];  // lists whose molecules are checked after diffusion
+  double epsilon;
+  const DevPanel* pnl;
+  const int* bpstart;       // panels of box b: bplist[bpstart[b] .. bpstart[b+1])
+  const int* bplist;
+  const DevSrfAct* act;     // [surface][species][face]
 };
 
 struct Edge {
@@ -129,8 +169,23 @@ struct EngineConfig {
   double expected_per_box;  // molecules per virtual box per list, for the pair-search tile
 };
 
+// Host description of the surfaces handed to engine_create (arrays copied).
+struct HostSurf {
+  int on;
+  int npnl;
+  const DevPanel* pnl;
+  int nbox;
+  const int* bpstart;  // nbox + 1 entries
+  const int* bplist;
+  int nsrf;
+  const DevSrfAct* act;  // nsrf * kMaxSpecies * 2
+  int listdiffuse[kMaxList];
+  double epsilon;
+};
+
 DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const Geometry& geo,
-                           const double* gtable, int ngauss, uint32_t key0, uint32_t key1, char* err, int errlen);
+                           const double* gtable, int ngauss, uint32_t key0, uint32_t key1, const HostSurf* surf,
+                           char* err, int errlen);
 void engine_destroy(DeviceState* s);
 // Upload molecules of list ll in live order (host arrays), replacing its content.
 int engine_upload_list(DeviceState* s, int ll, int n, const int* ident, const double* pos, const uint64_t* serno);

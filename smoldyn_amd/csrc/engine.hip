@@ -122,6 +122,12 @@ struct DeviceState {
   Prod* prods;
   Control* ctl;
   PairConst pc;
+  DevSurf sf;
+  DevPanel* d_pnl;
+  int* d_bpstart;
+  int* d_bplist;
+  DevSrfAct* d_act;
+  double screen_lim;
   double* inj;
   long injcap;
   hipStream_t stream;
@@ -146,6 +152,10 @@ __device__ __forceinline__ int pos2box(const Geometry& g, double x, double y, do
   }
   return b;
 }
+
+}  // namespace smolamd
+#include "surface_dev.h"
+namespace smolamd {
 
 // checkwalls for one molecule, walls in the reference's order 0..5
 __device__ __forceinline__ void apply_walls(const Geometry& g, double* p) {
@@ -194,11 +204,13 @@ __device__ __forceinline__ int wave_hist_rank(int* bcount, int key) {
   return base + (lane - leader);
 }
 
+template <bool kSurf>
 __global__ __launch_bounds__(kBlock) void k_diffuse(Layout L, Geometry g, const DevTables* __restrict__ tab,
                                                     Control* ctl, Recs a, int* __restrict__ bcount,
                                                     const double* __restrict__ gtab, int gmask,
                                                     const double* __restrict__ inj, uint32_t key0, uint32_t key1,
-                                                    UniEvent* __restrict__ uniev, int maxevents, int nbox) {
+                                                    UniEvent* __restrict__ uniev, int maxevents, int nbox,
+                                                    DevSurf sf) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   int hkey = -1;  // histogram key (list*nbox + box), -1 when not binned
   if (r < ctl->nrec) {
@@ -212,6 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(Layout L, Geometry g, const 
       uint32_t step = ctl->step;
       double pos[3] = {a.px[r], a.py[r], a.pz[r]};
       if (tab->diffuselist[ll]) {
+        const double posx[3] = {pos[0], pos[1], pos[2]};
         double ds = tab->difstep[i];
         double gg[3];
         if (inj) {
@@ -228,7 +241,21 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(Layout L, Geometry g, const 
         pos[0] += ds * gg[0];
         pos[1] += ds * gg[1];
         pos[2] += ds * gg[2];
-        apply_walls(g, pos);
+        if (!kSurf) {
+          apply_walls(g, pos);
+        } else if (sf.listdiffuse[ll]) {
+          // with surfaces the walls are not applied (smolsim.cpp:2618-2630)
+          int i2 = i;
+          int st = srf_check(sf, g, tab->difc, pos, posx, &i2, ser, step, SMOLRNG_SURF_DIFFUSED, key0, key1);
+          if (st != kSrfOk) atomicOr(&ctl->err, kErrSurface);
+          if (i2 != i) {
+            if (tab->listof[i2] != ll) atomicOr(&ctl->err, kErrSurface);
+            a.ident[r] = i2;
+            atomicSub(&ctl->count[i], 1);
+            atomicAdd(&ctl->count[i2], 1);
+            i = i2;
+          }
+        }
         a.px[r] = pos[0];
         a.py[r] = pos[1];
         a.pz[r] = pos[2];
@@ -346,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ part, 
 // and the kills of one step never exceed the window.
 __global__ __launch_bounds__(kBlock) void k_scatter(Layout L, const DevTables* __restrict__ tab,
                                                     const Control* __restrict__ ctl, Recs a, Recs b, Binned bn,
-                                                    int* __restrict__ inv, int nbox, int window) {
+                                                    int* __restrict__ inv, int nbox, int window, double lim) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= ctl->nrec) return;
   int bx = a.box[r];
@@ -363,7 +390,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Layout L, const DevTables* _
   b.serno[dst] = a.serno[r];
   b.lpos[dst] = lp;
   b.box[dst] = bx;
-  bn.spk[dst] = make_float4((float)x, (float)y, (float)z, __int_as_float(i));
+  // bit 16 of the payload word marks a coordinate beyond the screen's bound:
+  // such a molecule bypasses the FP32 screen
+  int far = !(fabs(x) <= lim && fabs(y) <= lim && fabs(z) <= lim);
+  bn.spk[dst] = make_float4((float)x, (float)y, (float)z, __int_as_float(i | (far << 16)));
   if (lp >= ctl->nl[ll] - window) inv[L.base[ll] + lp] = dst;
 }
 
@@ -397,8 +427,11 @@ __device__ __forceinline__ void emit_edge(Control* ctl, Edge* edges, int maxedge
   }
 }
 
+template <bool kSurf>
 __device__ __forceinline__ int first_reaction(const DevTables* tab, int i1, int i2, double dist2, uint32_t key0,
-                                              uint32_t key1, uint64_t ser1, uint64_t ser2, uint32_t step) {
+                                              uint32_t key1, uint64_t ser1, uint64_t ser2, uint32_t step,
+                                              const DevSurf& sf, const Geometry& g, const double* p1,
+                                              const double* p2, int wrap) {
   int set = i1 * kMaxSpecies + i2;
   int n = tab->nbi[set];
   for (int j = 0; j < n; j++) {
@@ -409,6 +442,9 @@ __device__ __forceinline__ int first_reaction(const DevTables* tab, int i1, int 
       smolrng4 rr = smolrng_philox(key0, key1, (uint32_t)ser1, (uint32_t)ser2, step, SMOLRNG_BIRXN | ((uint32_t)j << 8));
       if (!(smolrng_u01(rr.v[0]) < rx.prob)) continue;
     }
+    // rxnXsurface (smolreact.c:3242,3289; not for wrapped pairs)
+    if (kSurf && !wrap && srf_rxnblocked(sf, g, tab->difc, p1, p2, i1, i2, ser1, ser2, step, j, key0, key1))
+      continue;
     if (!rx.permit_soln) continue;
     return r;
   }
@@ -418,7 +454,9 @@ __device__ __forceinline__ int first_reaction(const DevTables* tab, int i1, int 
 // Exact FP64 test of one candidate (the reference's dist2 in its operation
 // order, smolreact.c:3238-3240, and wallcalcdist2, smolwall.c:100-116) and
 // the edge for the first reaction that accepts it.
-__device__ __forceinline__ void exact_pair(const Geometry& g, const DevTables* tab, Control* ctl, const Recs& b,
+template <bool kSurf>
+__device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf, const DevTables* tab, Control* ctl,
+                                           const Recs& b,
                                            Edge* edges, int maxedges, uint32_t key0, uint32_t key1, uint32_t step,
                                            int t1, const double* p1, int i1, uint64_t ser1, int t2, int i2, int wrap,
                                            uint64_t key) {
@@ -438,7 +476,7 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevTables* t
 #pragma unroll
     for (int d = 0; d < 3; d++) dist2 += (p1[d] - p2[d]) * (p1[d] - p2[d]);
   }
-  int r = first_reaction(tab, i1, i2, dist2, key0, key1, ser1, b.serno[t2], step);
+  int r = first_reaction<kSurf>(tab, i1, i2, dist2, key0, key1, ser1, b.serno[t2], step, sf, g, p1, p2, wrap);
   if (r >= 0) emit_edge(ctl, edges, maxedges, key, t1, t2, r, wrap);
 }
 
@@ -451,17 +489,19 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevTables* t
 // (0,0) is the molecule's own box, bireact's same-box pass.  Candidates pass
 // an FP32 screen, then the reference's FP64 test; only then is their stencil
 // index looked up for the edge key.
+template <bool kSurf>
 __global__ __launch_bounds__(kBlock) void k_pairs(Layout L, Geometry g, const DevTables* __restrict__ tab,
                                                   PairConst pc, Control* ctl, Recs b, Binned bn, int nbox,
                                                   int nscan, Edge* __restrict__ edges, int maxedges, uint32_t key0,
-                                                  uint32_t key1) {
+                                                  uint32_t key1, DevSurf sf) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= bn.bstart[nscan - 1]) return;  // records binned
   const float4 me = bn.spk[u];
-  const int i1 = __float_as_int(me.w);
+  const int w1 = __float_as_int(me.w);
+  const int i1 = w1 & 0xFFFF;
   const uint32_t mask = pc.reactmask[i1];
   if (!mask) return;
-  const float r2f = pc.r2f[i1];
+  const float r2f = (w1 >> 16) ? 3.0e38f : pc.r2f[i1];
   const int ll1 = tab->listof[i1];
   const int nl = tab->nlist;
   const int bb = b.box[u];
@@ -545,14 +585,15 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Layout L, Geometry g, const De
             const float4 o = bn.spk[kb + j];
             const float dxf = o.x - ox, dyf = o.y - oy, dzf = o.z - oz;
             const float d2 = dxf * dxf + dyf * dyf + dzf * dzf;
-            const bool hit = (d2 <= r2f) & ((mask >> __float_as_int(o.w)) & 1u);
+            const int w2 = __float_as_int(o.w);
+            const bool hit = ((d2 <= r2f) | (w2 >> 16)) & ((mask >> (w2 & 31)) & 1u);
             hits |= (uint32_t)hit << j;
           }
           if (u >= kb && u < kb + 32) hits &= ~(1u << (u - kb));
           while (hits) {
             const int k = kb + __ffs(hits) - 1;
             hits &= hits - 1;
-            const int i2 = __float_as_int(bn.spk[k].w);
+            const int i2 = __float_as_int(bn.spk[k].w) & 0xFFFF;
             const int dz = dzfix > -2 ? dzfix : (b.box[k] % sz) - bz;
             const int nbi = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
             const int lp2 = b.lpos[k];
@@ -566,7 +607,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Layout L, Geometry g, const De
             }
             const uint64_t key = nbi == 13 ? (key1base | (uint64_t)lp2)
                                            : ((1ull << 63) | key1base | ((uint64_t)nbi << 27) | (uint64_t)lp2);
-            exact_pair(g, tab, ctl, b, edges, maxedges, key0, key1, step, u, p1, i1, ser1, k, i2,
+            exact_pair<kSurf>(g, sf, tab, ctl, b, edges, maxedges, key0, key1, step, u, p1, i1, ser1, k, i2,
                        nbi == 13 ? 0 : wrap, key);
           }
         }
@@ -727,6 +768,7 @@ struct KillView {
 struct ResolveArgs {
   Layout L;
   Geometry g;
+  DevSurf sf;
   const DevTables* tab;
   Control* ctl;
   Recs a;  // source buffer (first-order reactants)
@@ -921,7 +963,6 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     }
     atomicSub(&s_dcount[rx->rct0], 1);
     if (order == 2) atomicSub(&s_dcount[rx->rct1], 1);
-    for (int q = 0; q < rx->nprod; q++) atomicAdd(&s_dcount[rx->prd[q]], 1);
     double uvec[3] = {0, 0, 0};
     if (rx->rotate) smolrng_unitvec(a.key0, a.key1, sA, step, 3, uvec);
     for (int q = 0; q < rx->nprod; q++) {
@@ -935,7 +976,17 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
         pp[1] = rxnpos[1] + v1[1];
         pp[2] = rxnpos[2] + v1[2];
       }
-      apply_walls(a.g, pp);  // checkwalls for reborn molecules
+      if (!a.sf.on) {
+        apply_walls(a.g, pp);  // checkwalls for reborn molecules
+      } else {
+        // checksurfaces for reborn molecules: segment reaction position -> product
+        int i2 = pr.ident;
+        int st = srf_check(a.sf, a.g, tab->difc, pp, rxnpos, &i2, ser0 + (uint64_t)pc, step, SMOLRNG_SURF_REBORN,
+                           a.key0, a.key1);
+        if (st != kSrfOk || tab->listof[i2] != pr.list) atomicOr(&ctl->err, kErrSurface);
+        pr.ident = i2;
+      }
+      atomicAdd(&s_dcount[pr.ident], 1);
       pr.x = pp[0];
       pr.y = pp[1];
       pr.z = pp[2];
@@ -1126,7 +1177,7 @@ static void free_recs(Recs* r) {
 }
 
 DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const Geometry& geo, const double* gtable,
-                           int ngauss, uint32_t key0, uint32_t key1, char* err, int errlen) {
+                           int ngauss, uint32_t key0, uint32_t key1, const HostSurf* surf, char* err, int errlen) {
   DeviceState* s = new DeviceState();
   memset((void*)s, 0, sizeof(DeviceState));
   s->cfg = cfg;
@@ -1164,6 +1215,11 @@ DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const 
   ok |= dalloc(&s->sortkey, s->sortcap) | dalloc(&s->sortval, s->sortcap) | dalloc(&s->killslot, s->sortcap);
   ok |= dalloc(&s->ctl, 1) | dalloc(&s->tab, 1) | dalloc(&s->gtab, ngauss);
   ok |= dalloc(&s->prods, (size_t)s->sortcap * kMaxProd);
+  if (surf && surf->on) {
+    ok |= dalloc(&s->d_pnl, surf->npnl > 0 ? surf->npnl : 1) | dalloc(&s->d_bpstart, (size_t)surf->nbox + 1) |
+          dalloc(&s->d_bplist, surf->bpstart[surf->nbox] > 0 ? surf->bpstart[surf->nbox] : 1) |
+          dalloc(&s->d_act, (size_t)surf->nsrf * kMaxSpecies * 2);
+  }
   if (ok) {
     snprintf(err, errlen, "device allocation failed (%d molecule records)", s->lay.reccap);
     engine_destroy(s);
@@ -1174,18 +1230,40 @@ DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const 
   s->cur = 0;
   for (int i = 0; i < kMaxSpecies; i++) {
     s->pc.reactmask[i] = tab.reactmask[i];
-    if (geo.prefilter_margin >= 0) {
-      double rr = sqrt(tab.maxbind2[i]) + geo.prefilter_margin;
-      s->pc.r2f[i] = (float)(rr * rr);
-    } else {
-      s->pc.r2f[i] = 3.0e38f;
-    }
+    double rr = sqrt(tab.maxbind2[i]) + geo.prefilter_margin;
+    s->pc.r2f[i] = (float)(rr * rr);
   }
   (void)hipMemset(s->bn.count, 0, sizeof(int) * s->nscan);
   (void)hipMemset(s->bestkey, 0xFF, sizeof(uint64_t) * rc);
   (void)hipMemset(s->killed, 0, sizeof(int) * rc);
   (void)hipMemset(s->ctl, 0, sizeof(Control));
   (void)hipMemcpy(s->tab, &tab, sizeof(DevTables), hipMemcpyHostToDevice);
+  memset(&s->sf, 0, sizeof(DevSurf));
+  if (surf && surf->on) {
+    if (surf->npnl > 0) (void)hipMemcpy(s->d_pnl, surf->pnl, sizeof(DevPanel) * surf->npnl, hipMemcpyHostToDevice);
+    (void)hipMemcpy(s->d_bpstart, surf->bpstart, sizeof(int) * (surf->nbox + 1), hipMemcpyHostToDevice);
+    if (surf->bpstart[surf->nbox] > 0)
+      (void)hipMemcpy(s->d_bplist, surf->bplist, sizeof(int) * surf->bpstart[surf->nbox], hipMemcpyHostToDevice);
+    (void)hipMemcpy(s->d_act, surf->act, sizeof(DevSrfAct) * surf->nsrf * kMaxSpecies * 2, hipMemcpyHostToDevice);
+    s->sf.on = 1;
+    s->sf.npnl = surf->npnl;
+    for (int ll = 0; ll < kMaxList; ll++) s->sf.listdiffuse[ll] = surf->listdiffuse[ll];
+    s->sf.epsilon = surf->epsilon;
+    s->sf.pnl = s->d_pnl;
+    s->sf.bpstart = s->d_bpstart;
+    s->sf.bplist = s->d_bplist;
+    s->sf.act = s->d_act;
+  }
+  {
+    // bound on coordinates for the FP32 pair screen (see k_scatter)
+    double ext = 0;
+    for (int d = 0; d < 3; d++) {
+      ext = fmax(ext, fabs(geo.low[d]));
+      ext = fmax(ext, fabs(geo.high[d]));
+      ext = fmax(ext, geo.high[d] - geo.low[d]);
+    }
+    s->screen_lim = 2.0 * ext;
+  }
   (void)hipMemcpy(s->gtab, gtable, sizeof(double) * ngauss, hipMemcpyHostToDevice);
   (void)hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   for (int k = 0; k < 16; k++) (void)hipEventCreate(&s->ev[k]);
@@ -1198,7 +1276,7 @@ void engine_destroy(DeviceState* s) {
   free_recs(&s->buf[1]);
   void* ptrs[] = {s->inv, s->bn.count, s->bn.part, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges, s->estate,
                   s->accepted, s->bestkey, s->killed, s->uniev, s->log, s->sortkey, s->sortval, s->killslot,
-                  s->ctl, s->tab, s->gtab, s->inj, s->prods};
+                  s->ctl, s->tab, s->gtab, s->inj, s->prods, s->d_pnl, s->d_bpstart, s->d_bplist, s->d_act};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; k++)
@@ -1265,6 +1343,7 @@ static ResolveArgs resolve_args(DeviceState* s, int src) {
   ResolveArgs a;
   a.L = s->lay;
   a.g = s->geo;
+  a.sf = s->sf;
   a.tab = s->tab;
   a.ctl = s->ctl;
   a.a = s->buf[src];
@@ -1301,8 +1380,10 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   int nbox = s->geo.nbox;
   Recs A = s->buf[src], B = s->buf[1 - src];
   if (timed) (void)hipEventRecord(s->ev[0], st);
-  hipLaunchKernelGGL(k_diffuse, dim3(nb), dim3(kBlock), 0, st, s->lay, s->geo, s->tab, s->ctl, A, s->bn.count,
-                     s->gtab, s->ngauss - 1, dinj, s->key0, s->key1, s->uniev, s->cfg.maxevents, nbox);
+  auto kdiff = s->sf.on ? k_diffuse<true> : k_diffuse<false>;
+  auto kpair = s->sf.on ? k_pairs<true> : k_pairs<false>;
+  hipLaunchKernelGGL(kdiff, dim3(nb), dim3(kBlock), 0, st, s->lay, s->geo, s->tab, s->ctl, A, s->bn.count,
+                     s->gtab, s->ngauss - 1, dinj, s->key0, s->key1, s->uniev, s->cfg.maxevents, nbox, s->sf);
   if (timed) (void)hipEventRecord(s->ev[1], st);
   hipLaunchKernelGGL(k_scan1, dim3(s->nscanblk), dim3(kScanBlock), 0, st, s->bn.count, s->bn.part, s->bn.boff,
                      s->nscan);
@@ -1311,11 +1392,11 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
                      s->bn.bstart, s->nscan);
   if (timed) (void)hipEventRecord(s->ev[2], st);
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, st, s->lay, s->tab, s->ctl, A, B, s->bn, s->inv, nbox,
-                     s->sortcap);
+                     s->sortcap, s->screen_lim);
   if (timed) (void)hipEventRecord(s->ev[3], st);
   if (timed) (void)hipEventRecord(s->ev[4], st);
-  hipLaunchKernelGGL(k_pairs, dim3(nb), dim3(kBlock), 0, st, s->lay, s->geo, s->tab, s->pc, s->ctl, B, s->bn, nbox,
-                     s->nscan, s->edges, s->cfg.maxedges, s->key0, s->key1);
+  hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->lay, s->geo, s->tab, s->pc, s->ctl, B, s->bn, nbox,
+                     s->nscan, s->edges, s->cfg.maxedges, s->key0, s->key1, s->sf);
   if (timed) (void)hipEventRecord(s->ev[5], st);
   hipLaunchKernelGGL(k_resolve, dim3(1), dim3(kResolveThreads), 0, st, resolve_args(s, src));
   if (timed) (void)hipEventRecord(s->ev[6], st);

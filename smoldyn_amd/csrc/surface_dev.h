@@ -1,0 +1,415 @@
+// Device restatement of the reference's molecule-surface path for
+// solution-phase molecules in 3D (source/Smoldyn/smolsurface.c,
+// smolboxes.c:846-892 line2nextbox, source/libSteve/Geometry.c).  Panels:
+// rectangle, sphere, disk.  Actions: reflect, transmit, jump, and rate-based
+// actions whose outcomes are those; absorption and adsorption are reported to
+// the caller (the kernels raise kErrSurface for them).
+//
+// The arithmetic follows the reference operation by operation (the library
+// is compiled with -ffp-contract=off), so collisions give the same FP64
+// positions as the CPU path.  Included by engine.hip only.
+#ifndef SMOLDYN_AMD_SURFACE_DEV_H
+#define SMOLDYN_AMD_SURFACE_DEV_H
+
+namespace smolamd {
+
+constexpr double kVeryClose = 1.0e-12;                // smoldyn.h:60
+constexpr double kDblEpsilon = 2.220446049250313e-16;  // DBL_EPSILON
+enum { kFaceFront = 0, kFaceBack = 1, kFaceNone = 2 };
+
+// panelside (smolsurface.c:3930-3963)
+__device__ __forceinline__ int srf_panelside(const DevPanel& pn, const double* pt, double* distp, int strict) {
+  double dist = 0;
+  if (pn.ps == kPsRect) {
+    int a = (int)pn.front[1];
+    dist = pn.front[0] * (pt[a] - pn.point[0][a]);
+  } else if (pn.ps == kPsDisk) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) dist += (pt[d] - pn.point[0][d]) * pn.front[d];
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; d++) dist += (pt[d] - pn.point[0][d]) * (pt[d] - pn.point[0][d]);
+    dist = pn.front[0] * (sqrt(dist) - pn.point[1][0]);
+  }
+  if (distp) *distp = dist;
+  if (dist > 0) return kFaceFront;
+  if (!strict || dist < 0) return kFaceBack;
+  return kFaceNone;
+}
+
+// Geo_SphereNormal (Geometry.c:199-213)
+__device__ __forceinline__ void srf_spherenormal(const double* cent, const double* pt, int front, double* ans) {
+  double dist = 0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    ans[d] = front * (pt[d] - cent[d]);
+    dist += ans[d] * ans[d];
+  }
+  if (dist > 0) {
+    dist = sqrt(dist);
+#pragma unroll
+    for (int d = 0; d < 3; d++) ans[d] /= dist;
+  } else {
+    ans[0] = 1;
+    ans[1] = 0;
+    ans[2] = 0;
+  }
+}
+
+// Geo_LineXSphs (Geometry.c:1200-1221)
+__device__ __forceinline__ double srf_linexsphs(const double* pt1, const double* pt2, const double* cent, double rad,
+                                                double* cross2p, double* nrdistp, double* nrposp) {
+  double a = 0, b = 0, c = 0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    a += (pt2[d] - pt1[d]) * (pt2[d] - pt1[d]);
+    b += (cent[d] - pt1[d]) * (pt2[d] - pt1[d]);
+    c += (pt1[d] - cent[d]) * (pt1[d] - cent[d]);
+  }
+  double nrdist = sqrt(c - b * b / a);
+  *nrdistp = nrdist;
+  *nrposp = b / a;
+  b *= -2;
+  c -= rad * rad;
+  double radical = b * b - 4 * a * c;
+  if (nrdist <= rad && radical < 0) radical = 0;
+  radical = sqrt(radical);
+  *cross2p = (-b + radical) / (2 * a);
+  return (-b - radical) / (2 * a);
+}
+
+// lineXpanel (smolsurface.c:3999-4101), 3D rectangle / sphere / disk
+__device__ __forceinline__ int srf_linexpanel(const DevPanel& pn, const double* pt1, const double* pt2,
+                                              double* crsspt, int* face1p, int* face2p, double* crossp,
+                                              double* cross2p) {
+  double dist1, dist2, cross = -1, cross2 = -1;
+  int face1 = srf_panelside(pn, pt1, &dist1, 0);
+  int face2 = srf_panelside(pn, pt2, &dist2, 0);
+  int intsct;
+  if (pn.ps == kPsRect) {
+    if (face1 == face2) return 0;
+    cross = dist1 / (dist1 - dist2);
+#pragma unroll
+    for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
+    int d = (int)pn.front[2];
+    intsct = ((pn.point[0][d] <= crsspt[d] && crsspt[d] <= pn.point[1][d]) ||
+              (pn.point[1][d] <= crsspt[d] && crsspt[d] <= pn.point[0][d]));
+    d = (d + 1) % 3;
+    if (d == (int)pn.front[1]) d = (d + 1) % 3;
+    intsct = intsct && ((pn.point[1][d] <= crsspt[d] && crsspt[d] <= pn.point[2][d]) ||
+                        (pn.point[2][d] <= crsspt[d] && crsspt[d] <= pn.point[1][d]));
+  } else if (pn.ps == kPsSph) {
+    int facein = pn.front[0] > 0 ? kFaceBack : kFaceFront;
+    if (face1 == facein && face2 == facein) return 0;
+    double nrdist, nrpos;
+    cross = srf_linexsphs(pt1, pt2, pn.point[0], pn.point[1][0], &cross2, &nrdist, &nrpos);
+    if (face1 == face2 && (!(nrdist <= pn.point[1][0]) || !(nrpos >= 0) || !(nrpos <= 1))) return 0;
+    if (face1 == facein) cross = cross2;
+#pragma unroll
+    for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
+    intsct = 1;
+  } else {
+    if (face1 == face2) return 0;
+    cross = dist1 / (dist1 - dist2);
+#pragma unroll
+    for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
+    double dot = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) dot += (crsspt[d] - pn.point[0][d]) * (crsspt[d] - pn.point[0][d]);
+    intsct = (dot <= pn.point[1][0] * pn.point[1][0]);
+  }
+  *face1p = face1;
+  if (face2p) *face2p = face2;
+  *crossp = cross;
+  if (cross2p) *cross2p = cross2;
+  return intsct;
+}
+
+// getpanelnormal (smolsurface.c:4430-4456)
+__device__ __forceinline__ void srf_getpanelnormal(const DevPanel& pn, const double* pt, double* norm) {
+  if (pn.ps == kPsRect) {
+    norm[0] = norm[1] = norm[2] = 0;
+    norm[(int)pn.front[1]] = pn.front[0];
+  } else if (pn.ps == kPsDisk) {
+    norm[0] = pn.front[0];
+    norm[1] = pn.front[1];
+    norm[2] = pn.front[2];
+  } else {
+    srf_spherenormal(pn.point[0], pt, (int)pn.front[0], norm);
+  }
+}
+
+// fixpt2panel (smolsurface.c:4460-4474): onto the panel, then nudged by
+// doubling multiples of DBL_EPSILON until on the requested side
+__device__ __forceinline__ void srf_fixpt2panel(const DevPanel& pn, double* pt, int face, double epsilon) {
+  double dist, norm[3];
+  int faceat = srf_panelside(pn, pt, &dist, 1);
+  if ((faceat == face || face == kFaceNone) && fabs(dist) <= epsilon) return;
+  srf_getpanelnormal(pn, pt, norm);
+#pragma unroll
+  for (int d = 0; d < 3; d++) pt[d] -= dist * norm[d];
+  if (face == kFaceNone) return;
+  int sign = (face == kFaceFront) ? 1 : -1;
+  // each pass doubles the step; 1100 doublings exceed any finite coordinate
+  for (double factor = 1.0; face != srf_panelside(pn, pt, nullptr, 1) && factor < 1e300; factor *= 2.0) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) pt[d] += sign * factor * kDblEpsilon * norm[d];
+  }
+}
+
+// surfacereflect (smolsurface.c:4821-4869), solution-phase molecule
+__device__ __forceinline__ void srf_reflect(const DevPanel& pn, double* pos, const double* crsspt) {
+  if (pn.ps == kPsRect) {
+    int axis = (int)pn.front[1];
+    pos[axis] -= 2.0 * (pos[axis] - crsspt[axis]);
+  } else if (pn.ps == kPsDisk) {
+    double dot = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) dot += (pos[d] - crsspt[d]) * pn.front[d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) pos[d] -= 2.0 * pn.front[d] * dot;
+  } else {
+    double norm[3], dot = 0;
+    srf_spherenormal(pn.point[0], crsspt, 1, norm);
+#pragma unroll
+    for (int d = 0; d < 3; d++) dot += (pos[d] - crsspt[d]) * norm[d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) pos[d] -= 2.0 * norm[d] * dot;
+  }
+}
+
+// line2nextbox (smolboxes.c:846-892): the next virtual box the segment
+// pt1->pt2 enters after box b, or -1
+__device__ __forceinline__ int srf_line2nextbox(const Geometry& g, const double* pt1, const double* pt2, int b) {
+  if (pos2box(g, pt2[0], pt2[1], pt2[2]) == b) return -1;
+  int z1[3];
+  z1[2] = b % g.side[2];
+  z1[1] = (b / g.side[2]) % g.side[1];
+  z1[0] = b / (g.side[2] * g.side[1]);
+  const int zb[3] = {z1[0], z1[1], z1[2]};
+  double crsmin = 1.01;
+  int d2 = 0, boxside2 = 0, flag = 0;
+  for (int d = 0; d < 3; d++) {
+    if (pt2[d] != pt1[d]) {
+      int boxside = (pt2[d] > pt1[d]) ? 1 : 0;
+      int sum = zb[d] + boxside;
+      if (sum > 0 && sum < g.side[d]) {
+        double edge = g.bmin[d] + (double)sum * g.bsize[d];
+        double crs = (edge - pt1[d]) / (pt2[d] - pt1[d]);
+        if (crs < crsmin) {
+          crsmin = crs;
+          d2 = d;
+          boxside2 = boxside;
+          flag = 0;
+        } else if (crs == crsmin) {
+          if (boxside == 0 && boxside2 == 0 && (flag == 0 || flag == 2)) flag = 2;
+          else flag = 1;
+        }
+      }
+    }
+  }
+  if (flag) {
+    for (int d = 0; d < 3; d++) {
+      if (pt2[d] != pt1[d]) {
+        int boxside = (pt2[d] > pt1[d]) ? 1 : 0;
+        int sum = zb[d] + boxside;
+        if (sum > 0 && sum < g.side[d]) {
+          double edge = g.bmin[d] + (double)sum * g.bsize[d];
+          double crs = (edge - pt1[d]) / (pt2[d] - pt1[d]);
+          if (crs == crsmin && (boxside == 1 || flag == 2)) z1[d] += boxside ? 1 : -1;
+        }
+      }
+    }
+  } else {
+    if (crsmin == 1.01) return -1;
+    z1[d2] += boxside2 ? 1 : -1;
+  }
+  return (z1[0] * g.side[1] + z1[1]) * g.side[2] + z1[2];
+}
+
+// surfaction (smolsurface.c:4330-4384) for a solution-phase molecule at face
+// 0/1: returns the action and sets the species after it.  u01 draws the
+// uniform of a rate-based action.
+template <typename Draw>
+__device__ __forceinline__ int srf_action(const DevSurf& sf, int srf, int ident, int face, int* i2, Draw u01) {
+  const DevSrfAct& a = sf.act[(srf * kMaxSpecies + ident) * 2 + face];
+  if (a.act != kSaMult) {
+    *i2 = a.newspec_fixed;
+    return a.act;
+  }
+  double r = u01();
+  int ms2 = 5;
+  for (int ms = 0; ms < 6; ms++)
+    if (r < a.cumprob[ms]) {
+      ms2 = ms;
+      break;
+    }
+  *i2 = a.newspec[ms2];
+  return a.outcome[ms2];
+}
+
+// surfacejump (smolsurface.c:4873-4926) for rect, sphere and disk panels;
+// returns 0 when the panel has no jump destination for that face
+__device__ __forceinline__ int srf_jump(const DevSurf& sf, const DevPanel& pn, double* pos, double* crsspt,
+                                        int face) {
+  int p2 = pn.jumpp[face], face2 = pn.jumpf[face];
+  if (p2 < 0 || !(face2 == kFaceFront || face2 == kFaceBack)) return 0;
+  const DevPanel& q = sf.pnl[p2];
+  double delta[3];
+  int dir;
+  if (pn.ps == kPsRect) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) delta[d] = 0.5 * (q.point[0][d] + q.point[2][d]) - 0.5 * (pn.point[0][d] + pn.point[2][d]);
+    dir = (pn.front[0] == q.front[0]) ? 1 : -1;
+  } else if (pn.ps == kPsSph) {
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      delta[d] = (crsspt[d] - pn.point[0][d]) * q.point[1][0] / pn.point[1][0] + q.point[0][d] - crsspt[d];
+    dir = (pn.front[0] == q.front[0]) ? 1 : -1;
+  } else {
+    double dot = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) delta[d] = q.point[0][d] - pn.point[0][d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) dot += pn.front[d] * q.front[d];
+    dir = dot > 0 ? 1 : -1;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    crsspt[d] += delta[d];
+    pos[d] += delta[d];
+  }
+  srf_fixpt2panel(q, crsspt, face2, 0);
+  dir *= (face != face2) ? 1 : -1;
+  if (dir == -1) srf_reflect(q, pos, crsspt);
+  return 1;
+}
+
+enum SrfStatus : int { kSrfOk = 0, kSrfAbsorbed = 1, kSrfUnsupported = 2 };
+
+// checksurfaces (smolsurface.c:5118-5180) for one solution-phase molecule:
+// segment posx -> pos, up to 100 interactions.  ident may change (species
+// change on an action); difc gives the motion correction of molchangeident
+// (smolmolec.c:2882-2885).  Draws: Philox keyed by (serial, step,
+// SMOLRNG_SURFACE, pass<<8 | iteration).
+__device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, const double* difc, double* pos,
+                                         const double* posx, int* ident, uint64_t serial, uint32_t step, int pass,
+                                         uint32_t key0, uint32_t key1) {
+  double via[3] = {posx[0], posx[1], posx[2]};
+  for (int it = 1;; it++) {
+    if (it > 100) {
+      pos[0] = posx[0];
+      pos[1] = posx[1];
+      pos[2] = posx[2];
+      return kSrfOk;
+    }
+    double crossmin = 2, crossmin2 = 2, crssptmin[3] = {0, 0, 0};
+    int pmin = -1, facemin = kFaceFront;
+    for (int b = pos2box(g, via[0], via[1], via[2]); b >= 0; b = srf_line2nextbox(g, via, pos, b)) {
+      for (int k = sf.bpstart[b]; k < sf.bpstart[b + 1]; k++) {
+        int p = sf.bplist[k];
+        double crsspt[3], cross;
+        int face;
+        if (srf_linexpanel(sf.pnl[p], via, pos, crsspt, &face, nullptr, &cross, nullptr) && cross <= crossmin2) {
+          if (cross <= crossmin) {
+            crossmin2 = crossmin;
+            crossmin = cross;
+            pmin = p;
+            crssptmin[0] = crsspt[0];
+            crssptmin[1] = crsspt[1];
+            crssptmin[2] = crsspt[2];
+            facemin = face;
+          } else {
+            crossmin2 = cross;
+          }
+        }
+      }
+    }
+    if (!(crossmin < 2)) return kSrfOk;
+    if (crossmin2 != crossmin && crossmin2 - crossmin < kVeryClose) {
+      pos[0] = via[0];
+      pos[1] = via[1];
+      pos[2] = via[2];
+      return kSrfOk;
+    }
+    // dosurfinteract (smolsurface.c:4930-5051), solution-phase branches
+    const DevPanel& pn = sf.pnl[pmin];
+    int i = *ident, i2;
+    auto draw = [&]() {
+      smolrng4 r = smolrng_mol(key0, key1, serial, step, SMOLRNG_SURFACE, ((uint32_t)pass << 8) | (uint32_t)it);
+      return smolrng_u01(r.v[0]);
+    };
+    int act = srf_action(sf, pn.srf, i, facemin, &i2, draw);
+    if (act == kSaTrans) {
+      srf_fixpt2panel(pn, crssptmin, facemin == kFaceFront ? kFaceBack : kFaceFront, sf.epsilon);
+    } else if (act == kSaReflect) {
+      srf_reflect(pn, pos, crssptmin);
+      srf_fixpt2panel(pn, crssptmin, facemin, sf.epsilon);
+    } else if (act == kSaJump) {
+      if (!srf_jump(sf, pn, pos, crssptmin, facemin))
+        srf_fixpt2panel(pn, crssptmin, facemin == kFaceFront ? kFaceBack : kFaceFront, sf.epsilon);
+    } else if (act == kSaAbsorb) {
+      return kSrfAbsorbed;
+    } else {
+      return kSrfUnsupported;
+    }
+    if (i2 != i) {  // molchangeident with the crossing point
+      if (i2 == 0) return kSrfAbsorbed;
+      if (difc[i] > 0) {
+        double ratio = sqrt(difc[i2] / difc[i]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) pos[d] = crssptmin[d] + ratio * (pos[d] - crssptmin[d]);
+      }
+      *ident = i2;
+    }
+    via[0] = crssptmin[0];
+    via[1] = crssptmin[1];
+    via[2] = crssptmin[2];
+  }
+}
+
+// rxnXsurface (smolsurface.c:4388-4424) for two solution-phase molecules: 1
+// when a panel between them blocks the reaction.  Rate-based actions draw
+// Philox keyed by (serial pair, step, SMOLRNG_RXNSURF, j<<8 | draw index).
+__device__ __forceinline__ int srf_rxnblocked(const DevSurf& sf, const Geometry& g, const double* difc, const double* pos1,
+                                              const double* pos2, int i1, int i2, uint64_t ser1, uint64_t ser2,
+                                              uint32_t step, int j, uint32_t key0, uint32_t key1) {
+  double dc1 = difc[i1], dc2 = difc[i2];
+  if (dc1 == 0 && dc2 == 0) dc1 = dc2 = 1;
+  double rxnpt = dc1 / (dc1 + dc2);
+  uint32_t c = 0;
+  auto draw = [&]() {
+    smolrng4 r = smolrng_pair(key0, key1, ser1, ser2, step, SMOLRNG_RXNSURF, ((uint32_t)j << 8) | c);
+    c++;
+    return smolrng_u01(r.v[0]);
+  };
+  auto trans = [&](int srf, int face, int ident) {
+    int tmp;
+    return srf_action(sf, srf, ident, face, &tmp, draw) == kSaTrans;
+  };
+  int result = 0;
+  for (int b = pos2box(g, pos1[0], pos1[1], pos1[2]); b >= 0 && result == 0; b = srf_line2nextbox(g, pos1, pos2, b)) {
+    for (int k = sf.bpstart[b]; k < sf.bpstart[b + 1] && result == 0; k++) {
+      const DevPanel& pn = sf.pnl[sf.bplist[k]];
+      double crsspt[3], cross, cross2;
+      int face1, face2;
+      if (srf_linexpanel(pn, pos1, pos2, crsspt, &face1, &face2, &cross, &cross2)) {
+        if (face1 != face2) {
+          if (rxnpt < cross || (rxnpt == cross && face1 == kFaceBack)) result = !trans(pn.srf, face2, i2);
+          else result = !trans(pn.srf, face1, i1);
+        } else {
+          int facein = (face1 == kFaceFront) ? kFaceBack : kFaceFront;
+          if (rxnpt < cross || (rxnpt == cross && face1 == kFaceBack))
+            result = !(trans(pn.srf, face2, i2) && trans(pn.srf, facein, i2));
+          else if (rxnpt < cross2 || (rxnpt == cross2 && face1 == kFaceFront))
+            result = !(trans(pn.srf, face1, i1) && trans(pn.srf, face2, i2));
+          else result = !(trans(pn.srf, face1, i1) && trans(pn.srf, facein, i1));
+        }
+      }
+    }
+  }
+  return result;
+}
+
+}  // namespace smolamd
+#endif

@@ -16,7 +16,7 @@ ORACLE_DIR = ROOT / "oracle"
 LIB = ORACLE_DIR / "liboracle.so"
 REFLIB = ORACLE_DIR / "_ref" / "libsteve_ref.so"
 
-MS = {"solution": 0, "soln": 0, "front": 1, "back": 2, "up": 3, "down": 4, "bsoln": 5, "all": 6}
+MS = {"solution": 0, "soln": 0, "fsoln": 0, "front": 1, "back": 2, "up": 3, "down": 4, "bsoln": 5, "all": 6}
 RP = {"none": 0, "irrev": 1, "confspread": 2, "bounce": 3, "pgem": 4, "pgemmax": 5, "pgemmaxw": 6,
       "ratio": 7, "unbindrad": 8, "pgem2": 9, "pgemmax2": 10, "ratio2": 11, "offset": 12, "fixed": 13}
 RNG_SFMT, RNG_COUNTER = 0, 1

@@ -4,7 +4,11 @@ A spec is a dict: low/high bounds, walls ('r','p','t' per dimension), dt,
 seed, species {name: difc}, optional lists {list: [species]}, reactions
 [(name, [reactants], [products], rate)], product placements
 {name: (kind, param)}, molecules [(species, n)] placed uniformly from the
-counter stream, or explicit positions [(species, array)].
+counter stream, or explicit positions [(species, array)], and optional
+surfaces [{name, actions [(face, species, action)], rates [(species, state,
+state1, state2, value, internal)], panels [(shape, axis, params, name)],
+jumps [(panel1, face1, panel2, face2, bidirectional)]}] set up the way the
+libsmoldyn calls set them (smolSetSurfaceAction passes no species change).
 """
 import numpy as np
 
@@ -35,9 +39,23 @@ def build_oracle(spec):
         rx[name] = sim.reaction(name, [idx[r] for r in rct], [idx[p] for p in prd], rate)
     for name, (kind, param) in spec.get("placement", {}).items():
         sim.revparam(rx[name], kind, param)
+    for srf in spec.get("surfaces", []):
+        s = sim.add_surface(srf["name"])
+        for face, sp, act in srf.get("actions", []):
+            sim.surface_action(s, -5 if sp == "all" else idx[sp], 0, ob.FACE[face], ob.SA[act], -1)
+        for rt in srf.get("rates", []):
+            sp, st, st1, st2, value, internal = rt[:6]
+            newid = idx[rt[6]] if len(rt) > 6 else -5
+            sim.surface_rate(s, -5 if sp == "all" else idx[sp], ob.MS[st], ob.MS[st1], ob.MS[st2], value,
+                             2 if internal else 1, newid)
+        pid = {}
+        for k, (shape, axis, params, pname) in enumerate(srf.get("panels", [])):
+            pid[pname] = sim.add_panel(s, ob.PS[shape], params, axis)
+        for p1, f1, p2, f2, bidir in srf.get("jumps", []):
+            sim.panel_jump(pid[p1], ob.FACE[f1], pid[p2], ob.FACE[f2], bidir)
     for sp, n in spec.get("molecules", []):
         if isinstance(n, int):
-            sim.add_mols(n, idx[sp], spec["low"], spec["high"])
+            sim.add_mols(n, idx[sp], spec.get("mol_low", spec["low"]), spec.get("mol_high", spec["high"]))
         else:
             sim.add_mols_at(idx[sp], n)
     sim.update()
@@ -68,13 +86,59 @@ def build_device(spec):
         sim.reaction(name, rct, prd, rate)
     for name, (kind, param) in spec.get("placement", {}).items():
         sim.product_placement(name, kind, param)
+    for srf in spec.get("surfaces", []):
+        sim.add_surface(srf["name"])
+        for face, sp, act in srf.get("actions", []):
+            sim.surface_action(srf["name"], face, sp, act)
+        for rt in srf.get("rates", []):
+            sp, st, st1, st2, value, internal = rt[:6]
+            sim.surface_rate(srf["name"], sp, st, st1, st2, value, rt[6] if len(rt) > 6 else None, internal)
+        for shape, axis, params, pname in srf.get("panels", []):
+            sim.add_panel(srf["name"], shape, params, axis, pname)
+        for p1, f1, p2, f2, bidir in srf.get("jumps", []):
+            sim.panel_jump(srf["name"], p1, f1, p2, f2, bidir)
     for sp, n in spec.get("molecules", []):
         if isinstance(n, int):
-            sim.add_molecules(sp, n)
+            sim.add_molecules(sp, n, spec.get("mol_low"), spec.get("mol_high"))
         else:
             sim.add_molecules_at(sp, n)
     sim.update()
     return sim
+
+
+def c3_spec(n=2000, side=100.0, seed=31, kback=5.0, ndisk=8, rdisk=None, bindrad_rate=0.2):
+    """Surfaces workload, scaled: A and B inside a reflecting sphere (radius
+    0.4*side, centre of the cube), a membrane plane through the centre that
+    reflects B and transmits A, disk patches on the membrane where A crosses
+    with probability 0.5 and turns into species D (rate_internal with a new
+    species), reversible A+B<->C with unbinding placement, D=2, dt=0.002 ..."""
+    c = side / 2
+    r = 0.4 * side
+    h = r / 3 ** 0.5 * 0.95
+    rng = np.random.default_rng(seed)
+    rdisk = rdisk if rdisk is not None else side / 40
+    panels = [("rect", "+2", [0.0, 0.0, c, side, side], "membrane")]
+    disks = []
+    for k in range(ndisk):
+        ang = rng.uniform(0, 2 * np.pi)
+        rad = rng.uniform(0, r * 0.7)
+        disks.append(("disk", None, [c + rad * np.cos(ang), c + rad * np.sin(ang), c, rdisk, 0, 0, 1, 20], "d%d" % k))
+    return dict(low=[0.0] * 3, high=[side] * 3, walls="rrr", dt=0.002, seed=seed,
+                species={"A": 2.0, "B": 2.0, "C": 1.0, "D": 2.0},
+                reactions=[("fwd", ["A", "B"], ["C"], bindrad_rate * 100.0), ("back", ["C"], ["A", "B"], kback)],
+                placement={"back": ("ratio", 1.5)},
+                surfaces=[
+                    {"name": "ball", "actions": [("both", "all", "reflect")],
+                     "panels": [("sph", None, [c, c, c, r, 20, 20], "s0")]},
+                    {"name": "membrane", "actions": [("both", "B", "reflect"), ("both", "C", "reflect")],
+                     "panels": panels},
+                    {"name": "patches", "actions": [("both", "B", "reflect")],
+                     "rates": [("A", "soln", "fsoln", "bsoln", 0.5, True, "D"),
+                               ("A", "soln", "bsoln", "fsoln", 0.5, True, "D")],
+                     "panels": disks},
+                ],
+                mol_low=[c - h] * 3, mol_high=[c + h] * 3,
+                molecules=[("A", n), ("B", n)])
 
 
 def oracle_state(osim):

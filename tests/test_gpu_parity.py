@@ -128,3 +128,48 @@ def test_c2_full_size_invariants_over_many_steps():
     assert np.array_equal((idx[:, 0] * side[1] + idx[:, 1]) * side[2] + idx[:, 2], st["box"])
     d = dsim.diagnostics()
     assert d["err"] == 0
+
+
+# ---------------------------------------------------------------------------
+# surfaces
+
+
+def test_surfaces_sphere_membrane_patches_bit_exact():
+    """Collisions with a reflecting sphere, a membrane that reflects B and C
+    and transmits A, and disk patches where A crosses with probability 0.5
+    and becomes D; reversible A+B<->C with products placed across surfaces;
+    pairs separated by a reflecting panel do not react."""
+    spec = ss.c3_spec(n=3000, side=60.0, seed=41, kback=20.0)
+    nev = _run_pair(spec, 300, check_every=25)
+    assert nev > 20
+
+
+def test_surfaces_injected_large_steps():
+    """Large injected steps make molecules cross several boxes and several
+    panels per step (line2nextbox walk, repeated interactions)."""
+    spec = ss.c3_spec(n=1500, side=40.0, seed=43, kback=20.0)
+    spec["dt"] = 0.5
+    nev = _run_pair(spec, 40, check_every=5, inject=True)
+    assert nev >= 0
+
+
+def test_surfaces_jump_panels():
+    """Jump panels: molecules crossing one wall of the box reappear at the
+    opposite one (the surface form of periodic boundaries)."""
+    side = 50.0
+    spec = dict(low=[0.0] * 3, high=[side] * 3, walls="ttt", dt=0.01, seed=47,
+                species={"A": 3.0, "B": 3.0, "C": 1.0},
+                reactions=[("fwd", ["A", "B"], ["C"], 50.0)],
+                surfaces=[{"name": "walls", "actions": [("both", "all", "jump")],
+                           "panels": [("rect", "+0", [0, 0, 0, side, side], "lo0"),
+                                      ("rect", "-0", [side, 0, 0, side, side], "hi0"),
+                                      ("rect", "+1", [0, 0, 0, side, side], "lo1"),
+                                      ("rect", "-1", [0, side, 0, side, side], "hi1"),
+                                      ("rect", "+2", [0, 0, 0, side, side], "lo2"),
+                                      ("rect", "-2", [0, 0, side, side, side], "hi2")],
+                           "jumps": [("lo0", "front", "hi0", "front", True),
+                                     ("lo1", "front", "hi1", "front", True),
+                                     ("lo2", "front", "hi2", "front", True)]}],
+                molecules=[("A", 2000), ("B", 2000)])
+    nev = _run_pair(spec, 200, check_every=20)
+    assert nev > 10
