@@ -51,6 +51,7 @@ enum PanelShape { PSrect, PStri, PSsph, PScyl, PShemi, PSdisk, PSall, PSnone };
 enum SrfAction { SAreflect, SAtrans, SAabsorb, SAjump, SAport, SAmult, SAno, SAnone, SAadsorb, SArevdes, SAirrevdes, SAflip };
 
 typedef struct smolamd_sim* simptr;
+typedef struct smolamd_ensemble* ensembleptr;
 
 /* ---------------- miscellaneous and errors (libsmoldyn.h:34-45) ---------------- */
 double smolGetVersion(void);                                                   /* libsmoldyn.h:34 */
@@ -167,6 +168,21 @@ enum ErrorCode smolamdProfileSteps(simptr sim, long nsteps, double* out);
    of last step, [3]=error bits, [4]=next serial, [5]=step index,
    [6]=molecule-steps run on the device so far. */
 enum ErrorCode smolamdGetDiagnostics(simptr sim, double* out);
+
+/* Ensembles: no counterpart in the reference, which runs replicates as
+   separate processes.  n simulations defined with the calls above (not yet
+   stepped) run together: every time step is one set of device launches for
+   all of them.  Each member gives exactly the results it gives alone.
+   Members need the same species and time step and no surfaces; the
+   simulations stay owned by the caller and are not stepped themselves. */
+ensembleptr smolamdEnsembleNew(simptr* sims, int n);
+void smolamdEnsembleFree(ensembleptr e);
+int smolamdEnsembleSize(ensembleptr e);
+enum ErrorCode smolamdEnsembleRunSteps(ensembleptr e, long nsteps, int use_graph, double* ms);
+int smolamdEnsembleGetMoleculeCount(ensembleptr e, int member, const char* species);
+int smolamdEnsembleGetMolecules(ensembleptr e, int member, int list, int* species, double* pos,
+                                unsigned long long* serno, int* box, int maxmol);
+enum ErrorCode smolamdEnsembleGetDiagnostics(ensembleptr e, int member, double* out);
 
 /* Reaction-parameter solver of the host setup (rxnparam.c of the reference). */
 double smolamdBindingRadius(double rate, double dt, double difc, double b, int rel);

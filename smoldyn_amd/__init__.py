@@ -106,6 +106,13 @@ def lib():
         "smolamdRunSteps": (I, [P, L64, I, dp]),
         "smolamdProfileSteps": (I, [P, L64, dp]),
         "smolamdGetDiagnostics": (I, [P, dp]),
+        "smolamdEnsembleNew": (P, [C.POINTER(P), I]),
+        "smolamdEnsembleFree": (None, [P]),
+        "smolamdEnsembleSize": (I, [P]),
+        "smolamdEnsembleRunSteps": (I, [P, L64, I, dp]),
+        "smolamdEnsembleGetMoleculeCount": (I, [P, I, cp]),
+        "smolamdEnsembleGetMolecules": (I, [P, I, I, ip, dp, C.POINTER(C.c_ulonglong), ip, I]),
+        "smolamdEnsembleGetDiagnostics": (I, [P, I, dp]),
         "smolamdBindingRadius": (D, [D, D, D, D, I]),
         "smolamdNumRxnRate": (D, [D, D, D]),
         "smolamdUnbindingRadius": (D, [D, D, D, D]),
@@ -310,5 +317,58 @@ class Simulation:
     def diagnostics(self):
         out = np.zeros(8)
         _check(self.L.smolamdGetDiagnostics(self.p, _dp(out)), "smolamdGetDiagnostics")
+        return dict(nmol=int(out[0]), maxedges=int(out[1]), rounds=int(out[2]), err=int(out[3]),
+                    serno_next=int(out[4]), step=int(out[5]), molsteps=int(out[6]))
+
+
+class Ensemble:
+    """Independent simulations stepped together on the device
+    (smolamdEnsembleNew): every step is one set of launches for all members,
+    and each member gives exactly the results it gives alone.  Members are
+    Simulation objects defined but not yet updated or stepped; they must
+    stay alive as long as the ensemble."""
+
+    def __init__(self, sims):
+        self.L = lib()
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.p for s in self.sims])
+        self.p = self.L.smolamdEnsembleNew(arr, len(self.sims))
+        if not self.p:
+            code, fn, msg = last_error()
+            raise SmoldynError("%s: %s (%d)" % (fn, msg, code))
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            self.L.smolamdEnsembleFree(self.p)
+            self.p = None
+
+    def __len__(self):
+        return len(self.sims)
+
+    def run_steps(self, n, graph=True):
+        ms = C.c_double(0)
+        _check(self.L.smolamdEnsembleRunSteps(self.p, n, 1 if graph else 0, C.byref(ms)), "smolamdEnsembleRunSteps")
+        return ms.value
+
+    def count(self, member, species=None):
+        n = self.L.smolamdEnsembleGetMoleculeCount(self.p, member, species.encode() if species else None)
+        _check(n if n < 0 else 0, "smolamdEnsembleGetMoleculeCount")
+        return n
+
+    def molecules(self, member, ll=0):
+        n = self.L.smolamdEnsembleGetMolecules(self.p, member, ll, None, None, None, None, 0)
+        _check(n if n < 0 else 0, "smolamdEnsembleGetMolecules")
+        ident = np.zeros(n, np.int32)
+        pos = np.zeros((n, 3), np.float64)
+        ser = np.zeros(n, np.uint64)
+        box = np.zeros(n, np.int32)
+        self.L.smolamdEnsembleGetMolecules(self.p, member, ll, ident.ctypes.data_as(C.POINTER(C.c_int)), _dp(pos),
+                                           ser.ctypes.data_as(C.POINTER(C.c_ulonglong)),
+                                           box.ctypes.data_as(C.POINTER(C.c_int)), n)
+        return dict(ident=ident, pos=pos, serno=ser, box=box)
+
+    def diagnostics(self, member):
+        out = np.zeros(8)
+        _check(self.L.smolamdEnsembleGetDiagnostics(self.p, member, _dp(out)), "smolamdEnsembleGetDiagnostics")
         return dict(nmol=int(out[0]), maxedges=int(out[1]), rounds=int(out[2]), err=int(out[3]),
                     serno_next=int(out[4]), step=int(out[5]), molsteps=int(out[6]))

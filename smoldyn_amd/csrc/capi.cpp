@@ -91,6 +91,15 @@ struct smolamd_sim {
   std::vector<double> gtable;
   int counts[kMaxSpecies] = {0};
   SurfaceSet surf;
+  // device inputs of the last update, kept for ensembles
+  struct Prepared {
+    EngineConfig cfg;
+    DevTables tab;
+    Geometry geo;
+    std::vector<std::vector<int>> ident;
+    std::vector<std::vector<double>> pos;
+    std::vector<std::vector<uint64_t>> ser;
+  } prep;
 };
 
 namespace {
@@ -478,8 +487,11 @@ void gauss_table(std::vector<double>& a, int n) {
   for (int i = n / 2; i < n; i++) a[i] = -a[n - i - 1];
 }
 
-ErrorCode do_update(simptr sim, const char* fn) {
-  if (sim->ok) return ECok;
+// Setup of simupdate (smolsim.cpp:2367-2473) for the device path.  With
+// for_batch the device inputs are kept in sim->prep and no engine is
+// created (the simulation becomes one member of an ensemble).
+ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
+  if (sim->ok && !for_batch) return ECok;
   LCHECK(!sim->started, fn, ECerror, "the simulation definition changed after stepping started");
   LCHECK(sim->dim == 3, fn, ECerror, "the device engine runs 3-D systems");
   int nsp = (int)sim->spname.size();
@@ -723,8 +735,13 @@ ErrorCode do_update(simptr sim, const char* fn) {
     for (int l = 0; l < nlist; l++) maxl = (long)ident[l].size() > maxl ? (long)ident[l].size() : maxl;
     cfg.expected_per_box = (double)maxl / (double)g.nbox;
   }
-  cfg.maxedges = (int)(captot / 2 > 65536 ? captot / 2 : 65536);
-  cfg.maxevents = (int)(captot / 16 > 16384 ? captot / 16 : 16384);
+  if (!for_batch) {
+    cfg.maxedges = (int)(captot / 2 > 65536 ? captot / 2 : 65536);
+    cfg.maxevents = (int)(captot / 16 > 16384 ? captot / 16 : 16384);
+  } else {  // one of many: smaller event buffers
+    cfg.maxedges = (int)(captot / 2 > 4096 ? captot / 2 : 4096);
+    cfg.maxevents = (int)(captot / 16 > 1024 ? captot / 16 : 1024);
+  }
   // surfaces (surfupdate, smolsurface.c:3797-3893, and the box panel lists)
   SurfaceTables st;
   HostSurf hs;
@@ -753,6 +770,16 @@ ErrorCode do_update(simptr sim, const char* fn) {
     hs.act = st.act.data();
     for (int ll = 0; ll < kMaxList; ll++) hs.listdiffuse[ll] = st.listdiffuse[ll];
     hs.epsilon = sim->surf.epsilon;
+  }
+  if (for_batch) {
+    LCHECK(!sim->surf.enabled(), fn, ECerror, "ensemble members cannot have surfaces");
+    sim->prep.cfg = cfg;
+    sim->prep.tab = t;
+    sim->prep.geo = g;
+    sim->prep.ident = ident;
+    sim->prep.pos = pos;
+    sim->prep.ser = ser;
+    return ECok;
   }
   if (sim->dev) engine_destroy(sim->dev);
   char err[256] = "";
@@ -1632,3 +1659,135 @@ double smolamdUnbindingRadius(double pgem, double dt, double difc, double a) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ensembles: many independent simulations stepped together on one device
+// (no counterpart in the reference, which runs replicates as separate
+// processes)
+
+struct smolamd_ensemble {
+  std::vector<simptr> sims;
+  DeviceState* dev = nullptr;
+  int nlist0 = 0;
+  std::vector<std::string> spname;
+};
+
+extern "C" ensembleptr smolamdEnsembleNew(simptr* sims, int n) {
+  const char* fn = "smolamdEnsembleNew";
+  if (!sims || n < 1) {
+    set_error(fn, ECmissing, "no simulations");
+    return nullptr;
+  }
+  std::vector<EngineConfig> cfg(n);
+  std::vector<DevTables> tab(n);
+  std::vector<Geometry> geo(n);
+  std::vector<uint32_t> k0(n), k1(n);
+  for (int k = 0; k < n; k++) {
+    simptr sim = sims[k];
+    if (!sim) {
+      set_error(fn, ECmissing, "missing simulation %d", k);
+      return nullptr;
+    }
+    if (do_update(sim, fn, true) != ECok) return nullptr;
+    if (sim->spname != sims[0]->spname || sim->dt != sims[0]->dt) {
+      set_error(fn, ECerror, "ensemble members need the same species and time step (member %d differs)", k);
+      return nullptr;
+    }
+    cfg[k] = sim->prep.cfg;
+    tab[k] = sim->prep.tab;
+    geo[k] = sim->prep.geo;
+    k0[k] = (uint32_t)sim->key;
+    k1[k] = (uint32_t)(sim->key >> 32);
+  }
+  char err[256] = "";
+  DeviceState* dev = engine_create_batch(n, cfg.data(), tab.data(), geo.data(), k0.data(), k1.data(),
+                                         sims[0]->gtable.data(), 4096, nullptr, err, sizeof(err));
+  if (!dev) {
+    set_error(fn, ECmemory, "device engine: %s", err);
+    return nullptr;
+  }
+  for (int k = 0; k < n; k++) {
+    simptr sim = sims[k];
+    for (int l = 0; l < sim->prep.cfg.nlist; l++) {
+      int er = engine_upload_list_inst(dev, k, l, (int)sim->prep.ident[l].size(), sim->prep.ident[l].data(),
+                                       sim->prep.pos[l].data(), sim->prep.ser[l].data());
+      if (er) {
+        engine_destroy(dev);
+        set_error(fn, ECmemory, "upload of member %d list %d failed", k, l);
+        return nullptr;
+      }
+    }
+  }
+  for (int k = 0; k < n; k++) engine_set_counters_inst(dev, k, sims[k]->serno, sims[k]->stepindex, sims[k]->counts);
+  ensembleptr e = new smolamd_ensemble();
+  e->sims.assign(sims, sims + n);
+  e->dev = dev;
+  e->spname = sims[0]->spname;
+  return e;
+}
+
+extern "C" void smolamdEnsembleFree(ensembleptr e) {
+  if (!e) return;
+  engine_destroy(e->dev);
+  delete e;
+}
+
+extern "C" int smolamdEnsembleSize(ensembleptr e) { return e ? (int)e->sims.size() : 0; }
+
+extern "C" enum ErrorCode smolamdEnsembleRunSteps(ensembleptr e, long nsteps, int use_graph, double* ms) {
+  const char* fn = "smolamdEnsembleRunSteps";
+  LCHECK(e && e->dev, fn, ECmissing, "missing ensemble");
+  int bits = engine_run(e->dev, nsteps, use_graph);
+  if (bits) return engine_error(fn, bits);
+  if (ms) *ms = engine_last_ms(e->dev);
+  for (simptr sim : e->sims) {
+    sim->time += sim->dt * (double)nsteps;
+    sim->stepindex += (uint32_t)nsteps;
+  }
+  return ECok;
+}
+
+extern "C" int smolamdEnsembleGetMoleculeCount(ensembleptr e, int member, const char* species) {
+  const char* fn = "smolamdEnsembleGetMoleculeCount";
+  if (!e || member < 0 || member >= (int)e->sims.size()) {
+    set_error(fn, ECbounds, "member out of range");
+    return (int)ECbounds;
+  }
+  Control c;
+  if (engine_get_control(e->dev, member, &c)) {
+    set_error(fn, ECerror, "HIP runtime error");
+    return (int)ECerror;
+  }
+  if (!species || !strcmp(species, "all")) {
+    int s = 0;
+    for (int i = 1; i < kMaxSpecies; i++) s += c.count[i];
+    return s;
+  }
+  for (size_t i = 1; i < e->spname.size(); i++)
+    if (e->spname[i] == species) return c.count[i];
+  set_error(fn, ECnonexist, "species not found");
+  return (int)ECnonexist;
+}
+
+extern "C" int smolamdEnsembleGetMolecules(ensembleptr e, int member, int list, int* species, double* pos,
+                                           unsigned long long* serno, int* box, int maxmol) {
+  if (!e || member < 0 || member >= (int)e->sims.size()) return (int)ECbounds;
+  return engine_get_list_inst(e->dev, member, list, species, pos, (uint64_t*)serno, box, maxmol);
+}
+
+extern "C" enum ErrorCode smolamdEnsembleGetDiagnostics(ensembleptr e, int member, double* out) {
+  const char* fn = "smolamdEnsembleGetDiagnostics";
+  LCHECK(e && member >= 0 && member < (int)e->sims.size(), fn, ECbounds, "member out of range");
+  Control c;
+  LCHECK(engine_get_control(e->dev, member, &c) == 0, fn, ECerror, "HIP runtime error");
+  double n = 0;
+  for (int l = 0; l < e->sims[member]->prep.cfg.nlist; l++) n += c.nl[l];
+  out[0] = n;
+  out[1] = c.maxedges_seen;
+  out[2] = c.rounds;
+  out[3] = c.err;
+  out[4] = (double)c.serno_next;
+  out[5] = c.step;
+  out[6] = (double)c.molsteps;
+  return ECok;
+}

@@ -88,8 +88,7 @@ struct Control {
   int maxedges_seen;
   int rounds;              // matching rounds of the last step
   unsigned long long molsteps;  // sum over steps of the live molecules at step start
-  int nrec;                // molecule records in the current buffer (live, dead-marked, new)
-  int nbinned;             // records binned in the last step
+  int nbinned;             // records binned in the last step (all simulations of a batch)
 };
 
 enum ErrBits : int {
@@ -186,6 +185,19 @@ struct HostSurf {
 DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const Geometry& geo,
                            const double* gtable, int ngauss, uint32_t key0, uint32_t key1, const HostSurf* surf,
                            char* err, int errlen);
+// A batch of n independent simulations stepped together (ensembles): one
+// set of launches per step covers all of them, k_resolve runs one workgroup
+// per simulation.  Surfaces are supported for n == 1 only.
+DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables* tab, const Geometry* geo,
+                                 const uint32_t* key0, const uint32_t* key1, const double* gtable, int ngauss,
+                                 const HostSurf* surf, char* err, int errlen);
+int engine_num_instances(DeviceState* s);
+int engine_upload_list_inst(DeviceState* s, int inst, int ll, int n, const int* ident, const double* pos,
+                            const uint64_t* serno);
+int engine_set_counters_inst(DeviceState* s, int inst, uint64_t serno_next, uint32_t step, const int* counts);
+int engine_get_control(DeviceState* s, int inst, Control* out);
+int engine_get_list_inst(DeviceState* s, int inst, int ll, int* ident, double* pos, uint64_t* serno, int* box,
+                         int max);
 void engine_destroy(DeviceState* s);
 // Upload molecules of list ll in live order (host arrays), replacing its content.
 int engine_upload_list(DeviceState* s, int ll, int n, const int* ident, const double* pos, const uint64_t* serno);
@@ -202,8 +214,5 @@ int engine_sync_control(DeviceState* s, Control* out);
 int engine_get_list(DeviceState* s, int ll, int* ident, double* pos, uint64_t* serno, int* box, int max);
 int engine_get_log(DeviceState* s, LogRow* rows, int maxrows);
 double engine_last_ms(DeviceState* s);
-// Timing of the dominant kernels of the last engine_run (average ms per launch).
-int engine_kernel_times(DeviceState* s, double* out, int nmax);
-int engine_device_bytes(DeviceState* s, double* out);
 
 }  // namespace smolamd

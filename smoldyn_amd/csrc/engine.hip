@@ -66,6 +66,7 @@ struct Layout {
 // One buffer of molecule records.
 struct Recs {
   double *px, *py, *pz;
+  int* inst;      // simulation instance (ensembles only; null for one simulation)
   int* ident;     // species, 0 for a record whose molecule died
   uint64_t* serno;
   int* lpos;      // position in the reference's live list of its list
@@ -93,48 +94,84 @@ struct PairConst {
   float r2f[kMaxSpecies];           // (largest binding radius + FP32 margin)^2
 };
 
-struct DeviceState {
-  EngineConfig cfg;
-  Layout lay;
-  Geometry geo;
-  DevTables* tab;
-  DevTables htab;
-  double* gtab;
-  int ngauss;
+// One simulation of the batch, as the kernels see it (device memory).
+struct Inst {
+  Layout L;        // list slots in the batch-wide slot numbering
+  Geometry g;
+  DevSurf sf;
+  const DevTables* tab;
+  Control* ctl;
+  PairConst pc;
   uint32_t key0, key1;
-  Recs buf[2];
-  int cur;  // buffer holding the current state
-  int* inv; // list slot -> record of the current buffer
-  Binned bn;
-  int nscan;
-  int nscanblk;
+  int boxbase;     // first histogram key (list-box) of this simulation
+  double screen_lim;  // coordinates beyond this bypass the FP32 pair screen
+  UniEvent* uniev;
+  int maxevents;
   Edge* edges;
-  uint64_t* bestkey;
-  int* killed;
+  int maxedges;
   int* estate;
   int* accepted;
-  UniEvent* uniev;
   LogRow* log;
-  uint64_t* sortkey;
-  int* sortval;
+  uint64_t* sk;
+  int* sv;
   int sortcap;
   int* killslot;
   Prod* prods;
-  Control* ctl;
-  PairConst pc;
-  DevSurf sf;
-  DevPanel* d_pnl;
-  int* d_bpstart;
-  int* d_bplist;
-  DevSrfAct* d_act;
-  double screen_lim;
-  double* inj;
-  long injcap;
-  hipStream_t stream;
-  hipGraphExec_t graph[2];
-  int graph_ready[2];
-  hipEvent_t ev[16];
-  double last_ms;
+};
+
+// Batch-wide counters.  The record count of the current buffer is kept per
+// step parity: k_resolve of step s allocates product records with nprod[s&1]
+// and raises nrec[s&1]; kernels of step s read nrec[(s+1)&1].
+struct Glob {
+  int nrec[2];
+  int nprod[2];
+};
+
+struct DeviceState {
+  int ninst = 0;
+  bool multi = false;                // more than one simulation: records carry their instance
+  std::vector<EngineConfig> cfg;     // per simulation
+  std::vector<DevTables> htab;       // host copies of the simulation tables
+  std::vector<Inst> hinst;           // host copy of the instance views
+  Inst* dinst = nullptr;
+  Glob* glob = nullptr;
+  int reccap = 0;                    // records per buffer
+  int captot = 0;                    // list slots of all simulations
+  double* gtab = nullptr;
+  int ngauss = 0;
+  Recs buf[2] = {};
+  int cur = 0;                       // buffer holding the current state
+  int* inv = nullptr;                // list slot -> record of the current buffer
+  Binned bn = {};
+  int nscan = 0, nscanblk = 0;
+  uint64_t* bestkey = nullptr;
+  int* killed = nullptr;
+  // per-simulation arrays (the instance views point into them)
+  Control* ctl = nullptr;
+  DevTables* tab = nullptr;
+  UniEvent* uniev = nullptr;
+  Edge* edges = nullptr;
+  int* estate = nullptr;
+  int* accepted = nullptr;
+  LogRow* log = nullptr;
+  uint64_t* sortkey = nullptr;
+  int* sortval = nullptr;
+  int* killslot = nullptr;
+  Prod* prods = nullptr;
+  // surfaces (single simulation)
+  bool surf_on = false;
+  DevPanel* d_pnl = nullptr;
+  int* d_bpstart = nullptr;
+  int* d_bplist = nullptr;
+  DevSrfAct* d_act = nullptr;
+  double* inj = nullptr;
+  long injcap = 0;
+  int hnrec = 0;                     // records uploaded so far
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph[2] = {};
+  int graph_ready[2] = {0, 0};
+  hipEvent_t ev[16] = {};
+  double last_ms = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -204,20 +241,28 @@ __device__ __forceinline__ int wave_hist_rank(int* bcount, int key) {
   return base + (lane - leader);
 }
 
-template <bool kSurf>
-__global__ __launch_bounds__(kBlock) void k_diffuse(Layout L, Geometry g, const DevTables* __restrict__ tab,
-                                                    Control* ctl, Recs a, int* __restrict__ bcount,
+// Record count of the current buffer (see Glob), read by the kernels of a
+// step before k_resolve.
+__device__ __forceinline__ int cur_nrec(const Inst* insts, const Glob* gl) {
+  return gl->nrec[(insts[0].ctl->step + 1) & 1];
+}
+
+template <bool kSurf, bool kMulti>
+__global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
+                                                    Recs a, int* __restrict__ bcount,
                                                     const double* __restrict__ gtab, int gmask,
-                                                    const double* __restrict__ inj, uint32_t key0, uint32_t key1,
-                                                    UniEvent* __restrict__ uniev, int maxevents, int nbox,
-                                                    DevSurf sf) {
+                                                    const double* __restrict__ inj) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  int hkey = -1;  // histogram key (list*nbox + box), -1 when not binned
-  if (r < ctl->nrec) {
+  int hkey = -1;  // histogram key (simulation's first key + list*nbox + box), -1 when not binned
+  if (r < cur_nrec(insts, gl)) {
     int i = a.ident[r];
     if (i == 0) {
       a.box[r] = -1;
     } else {
+      const Inst& I = insts[kMulti ? a.inst[r] : 0];
+      const DevTables* tab = I.tab;
+      Control* ctl = I.ctl;
+      const Geometry& g = I.g;
       int ll = tab->listof[i];
       int lp = a.lpos[r];
       uint64_t ser = a.serno[r];
@@ -233,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(Layout L, Geometry g, const 
           gg[1] = inj[3 * k + 1];
           gg[2] = inj[3 * k + 2];
         } else {
-          smolrng4 rn = smolrng_mol(key0, key1, ser, step, SMOLRNG_DIFFUSE, 0);
+          smolrng4 rn = smolrng_mol(I.key0, I.key1, ser, step, SMOLRNG_DIFFUSE, 0);
           gg[0] = gtab[rn.v[0] & (uint32_t)gmask];
           gg[1] = gtab[rn.v[1] & (uint32_t)gmask];
           gg[2] = gtab[rn.v[2] & (uint32_t)gmask];
@@ -243,10 +288,10 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(Layout L, Geometry g, const 
         pos[2] += ds * gg[2];
         if (!kSurf) {
           apply_walls(g, pos);
-        } else if (sf.listdiffuse[ll]) {
+        } else if (I.sf.listdiffuse[ll]) {
           // with surfaces the walls are not applied (smolsim.cpp:2618-2630)
           int i2 = i;
-          int st = srf_check(sf, g, tab->difc, pos, posx, &i2, ser, step, SMOLRNG_SURF_DIFFUSED, key0, key1);
+          int st = srf_check(I.sf, g, tab->difc, pos, posx, &i2, ser, step, SMOLRNG_SURF_DIFFUSED, I.key0, I.key1);
           if (st != kSrfOk) atomicOr(&ctl->err, kErrSurface);
           if (i2 != i) {
             if (tab->listof[i2] != ll) atomicOr(&ctl->err, kErrSurface);
@@ -261,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(Layout L, Geometry g, const 
         a.pz[r] = pos[2];
       }
       int b = pos2box(g, pos[0], pos[1], pos[2]);
-      hkey = ll * nbox + b;
+      hkey = I.boxbase + ll * g.nbox + b;
       // unireact: coin flips in table order, first success reacts; a
       // molecule that reacts takes no part in the pair search, so it is not
       // binned
@@ -275,14 +320,14 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(Layout L, Geometry g, const 
           if (!d.permit_soln) continue;
           if (blk != j / 4) {
             blk = j / 4;
-            rb = smolrng_mol(key0, key1, ser, step, SMOLRNG_UNIRXN, (uint32_t)blk);
+            rb = smolrng_mol(I.key0, I.key1, ser, step, SMOLRNG_UNIRXN, (uint32_t)blk);
           }
           if (smolrng_u01(rb.v[j & 3]) < d.prob) {
             int e = atomicAdd(&ctl->nuni, 1);
-            if (e < maxevents) {
-              uniev[e].rec = r;
-              uniev[e].slot = L.base[ll] + lp;
-              uniev[e].r = rx;
+            if (e < I.maxevents) {
+              I.uniev[e].rec = r;
+              I.uniev[e].slot = I.L.base[ll] + lp;
+              I.uniev[e].r = rx;
             } else {
               atomicOr(&ctl->err, kErrEventOverflow);
             }
@@ -371,41 +416,46 @@ __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ part, 
 // The inverse map (list slot -> record) is written only for the last
 // `window` slots of each list: compaction in k_resolve reads tail slots only,
 // and the kills of one step never exceed the window.
-__global__ __launch_bounds__(kBlock) void k_scatter(Layout L, const DevTables* __restrict__ tab,
-                                                    const Control* __restrict__ ctl, Recs a, Recs b, Binned bn,
-                                                    int* __restrict__ inv, int nbox, int window, double lim) {
+template <bool kMulti>
+__global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
+                                                    Recs a, Recs b, Binned bn, int* __restrict__ inv) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= ctl->nrec) return;
+  if (r >= cur_nrec(insts, gl)) return;
   int bx = a.box[r];
   if (bx < 0) return;
+  int in = kMulti ? a.inst[r] : 0;
+  const Inst& I = insts[in];
   int i = a.ident[r];
-  int ll = tab->listof[i];
-  int dst = bn.bstart[ll * nbox + bx] + a.rank[r];
+  int ll = I.tab->listof[i];
+  int dst = bn.bstart[I.boxbase + ll * I.g.nbox + bx] + a.rank[r];
   double x = a.px[r], y = a.py[r], z = a.pz[r];
   int lp = a.lpos[r];
   b.px[dst] = x;
   b.py[dst] = y;
   b.pz[dst] = z;
+  if (kMulti) b.inst[dst] = in;
   b.ident[dst] = i;
   b.serno[dst] = a.serno[r];
   b.lpos[dst] = lp;
   b.box[dst] = bx;
   // bit 16 of the payload word marks a coordinate beyond the screen's bound:
   // such a molecule bypasses the FP32 screen
+  const double lim = I.screen_lim;
   int far = !(fabs(x) <= lim && fabs(y) <= lim && fabs(z) <= lim);
   bn.spk[dst] = make_float4((float)x, (float)y, (float)z, __int_as_float(i | (far << 16)));
-  if (lp >= ctl->nl[ll] - window) inv[L.base[ll] + lp] = dst;
+  if (lp >= I.ctl->nl[ll] - I.sortcap) inv[I.L.base[ll] + lp] = dst;
 }
 
 // Full inverse map of the current buffer, for reading the state back.
-__global__ __launch_bounds__(kBlock) void k_rebuild_inv(Layout L, const DevTables* __restrict__ tab,
-                                                        const Control* __restrict__ ctl, Recs a,
-                                                        int* __restrict__ inv) {
+template <bool kMulti>
+__global__ __launch_bounds__(kBlock) void k_rebuild_inv(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
+                                                        Recs a, int* __restrict__ inv) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= ctl->nrec) return;
+  if (r >= cur_nrec(insts, gl)) return;
   int i = a.ident[r];
   if (i == 0) return;
-  inv[L.base[tab->listof[i]] + a.lpos[r]] = r;
+  const Inst& I = insts[kMulti ? a.inst[r] : 0];
+  inv[I.L.base[I.tab->listof[i]] + a.lpos[r]] = r;
 }
 
 // ---------------------------------------------------------------------------
@@ -489,13 +539,16 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf,
 // (0,0) is the molecule's own box, bireact's same-box pass.  Candidates pass
 // an FP32 screen, then the reference's FP64 test; only then is their stencil
 // index looked up for the edge key.
-template <bool kSurf>
-__global__ __launch_bounds__(kBlock) void k_pairs(Layout L, Geometry g, const DevTables* __restrict__ tab,
-                                                  PairConst pc, Control* ctl, Recs b, Binned bn, int nbox,
-                                                  int nscan, Edge* __restrict__ edges, int maxedges, uint32_t key0,
-                                                  uint32_t key1, DevSurf sf) {
+template <bool kSurf, bool kMulti>
+__global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, Recs b, Binned bn, int nscan) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= bn.bstart[nscan - 1]) return;  // records binned
+  const Inst& I = insts[kMulti ? b.inst[u] : 0];
+  const Geometry& g = I.g;
+  const DevTables* tab = I.tab;
+  const PairConst& pc = I.pc;
+  Control* ctl = I.ctl;
+  const int nbox = g.nbox;
   const float4 me = bn.spk[u];
   const int w1 = __float_as_int(me.w);
   const int i1 = w1 & 0xFFFF;
@@ -547,7 +600,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Layout L, Geometry g, const De
         cw |= 2 << 2;
         shy = (float)g.width[1];
       }
-      const int colb = ll2 * nbox + (nx * g.side[1] + ny) * sz;
+      const int colb = I.boxbase + ll2 * nbox + (nx * g.side[1] + ny) * sz;
       // z segments: unwrapped run [zlo, zhi] plus wrapped single boxes
       int zlo = bz - 1, zhi = bz + dzhi;
       for (int seg = 0; seg < 3; seg++) {
@@ -607,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Layout L, Geometry g, const De
             }
             const uint64_t key = nbi == 13 ? (key1base | (uint64_t)lp2)
                                            : ((1ull << 63) | key1base | ((uint64_t)nbi << 27) | (uint64_t)lp2);
-            exact_pair<kSurf>(g, sf, tab, ctl, b, edges, maxedges, key0, key1, step, u, p1, i1, ser1, k, i2,
+            exact_pair<kSurf>(g, I.sf, tab, ctl, b, I.edges, I.maxedges, I.key0, I.key1, step, u, p1, i1, ser1, k, i2,
                        nbi == 13 ? 0 : wrap, key);
           }
         }
@@ -765,35 +818,13 @@ struct KillView {
   __device__ __forceinline__ int operator[](int i) const { return lds ? lds[i] : (int)glb[i]; }
 };
 
-struct ResolveArgs {
-  Layout L;
-  Geometry g;
-  DevSurf sf;
-  const DevTables* tab;
-  Control* ctl;
-  Recs a;  // source buffer (first-order reactants)
-  Recs b;  // destination buffer (binned molecules, products)
-  int* inv;
-  Binned bn;
-  int nscan;
-  Edge* edges;
-  int maxedges;
-  uint64_t* bestkey;
-  int* killed;
-  int* estate;
-  int* accepted;
-  UniEvent* uniev;
-  int maxevents;
-  LogRow* log;
-  uint64_t* sk;
-  int* sv;
-  int sortcap;
-  int* killslot;
-  Prod* prods;
-  uint32_t key0, key1;
-};
 
-__global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
+// One workgroup per simulation of the batch (blockIdx.x).
+__global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restrict__ insts, Glob* gl, Recs RA,
+                                                             Recs RB, int* __restrict__ inv, Binned bn, int nscan,
+                                                             uint64_t* __restrict__ bestkey, int* __restrict__ killed,
+                                                             int reccap) {
+  const Inst& I = insts[blockIdx.x];
   __shared__ uint64_t lk[kLdsSort];
   __shared__ int lv[kLdsSort];
   __shared__ int s_scan[kResolveThreads / 64];
@@ -801,62 +832,62 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   __shared__ int s_dcount[kMaxSpecies];
   __shared__ int s_ev[kLdsEvents];
   __shared__ int s_kill[kLdsKills];
-  __shared__ int s_flag, s_nacc;
+  __shared__ int s_flag, s_nacc, s_prodoff;
   __shared__ int s_nl[kMaxList], s_nf[kMaxList], s_k0[kMaxList], s_k1[kMaxList], s_pl[kMaxList];
   const int tid = threadIdx.x;
   const int NT = blockDim.x;
-  Control* ctl = a.ctl;
-  const DevTables* tab = a.tab;
-  Recs B = a.b;
+  Control* ctl = I.ctl;
+  const DevTables* tab = I.tab;
+  Recs B = RB;
   int E = ctl->nedges;
-  if (E > a.maxedges) E = a.maxedges;
+  if (E > I.maxedges) E = I.maxedges;
   int U = ctl->nuni;
-  if (U > a.maxevents) U = a.maxevents;
+  if (U > I.maxevents) U = I.maxevents;
   uint32_t step = ctl->step;
-  int nbinned = a.bn.bstart[a.nscan - 1];
+  int nbinned = bn.bstart[nscan - 1];
 
   // ---- greedy matching: accept an edge when it is the smallest live key at
   // both of its molecules; this is the lexicographically-first maximal
   // matching, i.e. what the sequential loop of bireact accepts.
   if (tid == 0) s_nacc = 0;
   if (tid < kMaxSpecies) s_dcount[tid] = 0;
-  for (int e = tid; e < E; e += NT) a.estate[e] = 0;
+  for (int e = tid; e < E; e += NT) I.estate[e] = 0;
   __syncthreads();
   int rounds = 0;
   for (;;) {
     if (tid == 0) s_flag = 0;
     __syncthreads();
     for (int e = tid; e < E; e += NT) {
-      if (a.estate[e] != 0) continue;
-      Edge ed = a.edges[e];
-      if (atomic_load_i32(&a.killed[ed.s1]) || atomic_load_i32(&a.killed[ed.s2])) {
-        a.estate[e] = 2;
+      if (I.estate[e] != 0) continue;
+      Edge ed = I.edges[e];
+      if (atomic_load_i32(&killed[ed.s1]) || atomic_load_i32(&killed[ed.s2])) {
+        I.estate[e] = 2;
         continue;
       }
-      atomicMin((unsigned long long*)&a.bestkey[ed.s1], (unsigned long long)ed.key);
-      atomicMin((unsigned long long*)&a.bestkey[ed.s2], (unsigned long long)ed.key);
+      atomicMin((unsigned long long*)&bestkey[ed.s1], (unsigned long long)ed.key);
+      atomicMin((unsigned long long*)&bestkey[ed.s2], (unsigned long long)ed.key);
       s_flag = 1;
     }
     __syncthreads();
     if (!s_flag) break;
     rounds++;
     for (int e = tid; e < E; e += NT) {
-      if (a.estate[e] != 0) continue;
-      Edge ed = a.edges[e];
-      if (atomic_load_u64(&a.bestkey[ed.s1]) == ed.key && atomic_load_u64(&a.bestkey[ed.s2]) == ed.key) {
-        a.estate[e] = 1;
-        atomicExch(&a.killed[ed.s1], 1);
-        atomicExch(&a.killed[ed.s2], 1);
+      if (I.estate[e] != 0) continue;
+      Edge ed = I.edges[e];
+      if (atomic_load_u64(&bestkey[ed.s1]) == ed.key && atomic_load_u64(&bestkey[ed.s2]) == ed.key) {
+        I.estate[e] = 1;
+        atomicExch(&killed[ed.s1], 1);
+        atomicExch(&killed[ed.s2], 1);
         int k = atomicAdd(&s_nacc, 1);
-        a.accepted[k] = e;
+        I.accepted[k] = e;
       }
     }
     __syncthreads();
     for (int e = tid; e < E; e += NT) {
-      if (a.estate[e] == 2) continue;
-      Edge ed = a.edges[e];
-      atomicExch((unsigned long long*)&a.bestkey[ed.s1], (unsigned long long)kNoKey);
-      atomicExch((unsigned long long*)&a.bestkey[ed.s2], (unsigned long long)kNoKey);
+      if (I.estate[e] == 2) continue;
+      Edge ed = I.edges[e];
+      atomicExch((unsigned long long*)&bestkey[ed.s1], (unsigned long long)kNoKey);
+      atomicExch((unsigned long long*)&bestkey[ed.s2], (unsigned long long)kNoKey);
     }
     __syncthreads();
   }
@@ -865,25 +896,25 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
 
   // ---- order the events: first-order reactions in live-list order, then
   // bimolecular ones in edge-key order
-  if (U + pow2ceil(B2) > a.sortcap || pow2ceil(U + 2 * B2) > a.sortcap) {
+  if (U + pow2ceil(B2) > I.sortcap || pow2ceil(U + 2 * B2) > I.sortcap) {
     if (tid == 0) atomicOr(&ctl->err, kErrSortOverflow);
     return;
   }
   for (int i = tid; i < U; i += NT) {
-    a.sk[i] = (uint64_t)a.uniev[i].slot;
-    a.sv[i] = i;
+    I.sk[i] = (uint64_t)I.uniev[i].slot;
+    I.sv[i] = i;
   }
   __syncthreads();
   const bool ev_lds = (U + B2) <= kLdsEvents;
-  wg_sort_any(a.sk, a.sv, U, lk, lv, nullptr, ev_lds ? s_ev : nullptr);
+  wg_sort_any(I.sk, I.sv, U, lk, lv, nullptr, ev_lds ? s_ev : nullptr);
   for (int i = tid; i < B2; i += NT) {
-    a.sk[U + i] = a.edges[a.accepted[i]].key;
-    a.sv[U + i] = a.accepted[i];
+    I.sk[U + i] = I.edges[I.accepted[i]].key;
+    I.sv[U + i] = I.accepted[i];
   }
   __syncthreads();
-  wg_sort_any(a.sk + U, a.sv + U, B2, lk, lv, nullptr, ev_lds ? s_ev + U : nullptr);
+  wg_sort_any(I.sk + U, I.sv + U, B2, lk, lv, nullptr, ev_lds ? s_ev + U : nullptr);
   int NE = U + B2;
-  const int* evs = ev_lds ? s_ev : a.sv;  // event order: sorted event indices
+  const int* evs = ev_lds ? s_ev : I.sv;  // event order: sorted event indices
 
   // ---- product counts and serial numbers in creation order (each thread
   // takes a contiguous chunk of events so the scan follows creation order)
@@ -891,11 +922,13 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   int e0 = tid * per, e1 = min(NE, e0 + per);
   int myprod = 0;
   for (int e = e0; e < e1; e++) {
-    const DevRxn& rx = e < U ? tab->rxn1[a.uniev[evs[e]].r] : tab->rxn2[a.edges[evs[e]].r];
+    const DevRxn& rx = e < U ? tab->rxn1[I.uniev[evs[e]].r] : tab->rxn2[I.edges[evs[e]].r];
     myprod += rx.nprod;
   }
   int K;
   int pbase = wg_scan(myprod, s_scan, &K);
+  // product records of this simulation: [nbinned + s_prodoff, + K)
+  if (tid == 0) s_prodoff = atomicAdd(&gl->nprod[step & 1], K);
   int D = U + 2 * B2;  // reactants killed = molsort's kill count
   uint64_t ser0 = ctl->serno_next;
 
@@ -907,17 +940,17 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     uint64_t sA, sB = 0;
     int order, ridx;
     if (e < U) {
-      UniEvent ev = a.uniev[evs[e]];
+      UniEvent ev = I.uniev[evs[e]];
       rx = &tab->rxn1[ev.r];
       ridx = ev.r;
       order = 1;
-      rxnpos[0] = a.a.px[ev.rec];
-      rxnpos[1] = a.a.py[ev.rec];
-      rxnpos[2] = a.a.pz[ev.rec];
-      sA = a.a.serno[ev.rec];
-      a.killslot[e] = ev.slot;
+      rxnpos[0] = RA.px[ev.rec];
+      rxnpos[1] = RA.py[ev.rec];
+      rxnpos[2] = RA.pz[ev.rec];
+      sA = RA.serno[ev.rec];
+      I.killslot[e] = ev.slot;
     } else {
-      Edge ed = a.edges[evs[e]];
+      Edge ed = I.edges[evs[e]];
       rx = &tab->rxn2[ed.r];
       ridx = ed.r;
       order = 2;
@@ -930,8 +963,8 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
         for (int d = 0; d < 3; d++) {
           int cc = (ed.wrap >> (2 * d)) & 3;
           if (cc == 0) v[d] = p2[d] - p1[d];
-          else if (cc == 1) v[d] = p2[d] - p1[d] - a.g.width[d];
-          else v[d] = p2[d] - p1[d] + a.g.width[d];
+          else if (cc == 1) v[d] = p2[d] - p1[d] - I.g.width[d];
+          else v[d] = p2[d] - p1[d] + I.g.width[d];
         }
         if (tab->difc[id1] < tab->difc[id2]) {
 #pragma unroll
@@ -958,13 +991,13 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       sA = B.serno[swap ? ed.s2 : ed.s1];
       sB = B.serno[swap ? ed.s1 : ed.s2];
       int k2 = U + 2 * (e - U);
-      a.killslot[k2] = a.L.base[tab->listof[id1]] + B.lpos[ed.s1];
-      a.killslot[k2 + 1] = a.L.base[tab->listof[id2]] + B.lpos[ed.s2];
+      I.killslot[k2] = I.L.base[tab->listof[id1]] + B.lpos[ed.s1];
+      I.killslot[k2 + 1] = I.L.base[tab->listof[id2]] + B.lpos[ed.s2];
     }
     atomicSub(&s_dcount[rx->rct0], 1);
     if (order == 2) atomicSub(&s_dcount[rx->rct1], 1);
     double uvec[3] = {0, 0, 0};
-    if (rx->rotate) smolrng_unitvec(a.key0, a.key1, sA, step, 3, uvec);
+    if (rx->rotate) smolrng_unitvec(I.key0, I.key1, sA, step, 3, uvec);
     for (int q = 0; q < rx->nprod; q++) {
       Prod pr;
       pr.ident = rx->prd[q];
@@ -976,13 +1009,13 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
         pp[1] = rxnpos[1] + v1[1];
         pp[2] = rxnpos[2] + v1[2];
       }
-      if (!a.sf.on) {
-        apply_walls(a.g, pp);  // checkwalls for reborn molecules
+      if (!I.sf.on) {
+        apply_walls(I.g, pp);  // checkwalls for reborn molecules
       } else {
         // checksurfaces for reborn molecules: segment reaction position -> product
         int i2 = pr.ident;
-        int st = srf_check(a.sf, a.g, tab->difc, pp, rxnpos, &i2, ser0 + (uint64_t)pc, step, SMOLRNG_SURF_REBORN,
-                           a.key0, a.key1);
+        int st = srf_check(I.sf, I.g, tab->difc, pp, rxnpos, &i2, ser0 + (uint64_t)pc, step, SMOLRNG_SURF_REBORN,
+                           I.key0, I.key1);
         if (st != kSrfOk || tab->listof[i2] != pr.list) atomicOr(&ctl->err, kErrSurface);
         pr.ident = i2;
       }
@@ -990,7 +1023,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       pr.x = pp[0];
       pr.y = pp[1];
       pr.z = pp[2];
-      a.prods[pc] = pr;
+      I.prods[pc] = pr;
       pc++;
     }
     LogRow row;
@@ -1001,7 +1034,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     int pfirst = pc - rx->nprod;
     row.prd[0] = rx->nprod > 0 ? ser0 + (uint64_t)pfirst : 0;
     row.prd[1] = rx->nprod > 1 ? ser0 + (uint64_t)pfirst + 1 : 0;
-    a.log[e] = row;
+    I.log[e] = row;
   }
   __syncthreads();
 
@@ -1009,16 +1042,16 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   // (swap-from-tail compaction), then append products in the dead-list order
   // the reference produces: creation order reversed, rotated by the kills.
   for (int i = tid; i < D; i += NT) {
-    a.sk[i] = (uint64_t)a.killslot[i];
-    a.sv[i] = 0;
+    I.sk[i] = (uint64_t)I.killslot[i];
+    I.sv[i] = 0;
   }
   __syncthreads();
   const bool kill_lds = D <= kLdsKills;
-  wg_sort_any(a.sk, a.sv, D, lk, lv, kill_lds ? s_kill : nullptr, nullptr);
-  const KillView kv{kill_lds ? s_kill : nullptr, a.sk};
-  if (tid < a.L.nlist) {
+  wg_sort_any(I.sk, I.sv, D, lk, lv, kill_lds ? s_kill : nullptr, nullptr);
+  const KillView kv{kill_lds ? s_kill : nullptr, I.sk};
+  if (tid < I.L.nlist) {
     int ll = tid;
-    int lo = a.L.base[ll], hi = a.L.base[ll + 1];
+    int lo = I.L.base[ll], hi = I.L.base[ll + 1];
     int l = 0, h = D;
     while (l < h) {
       int mid = (l + h) >> 1;
@@ -1040,16 +1073,16 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   __syncthreads();
   // bimolecular reactants stay in the destination buffer as dead records
   for (int i = tid; i < B2; i += NT) {
-    Edge ed = a.edges[a.accepted[i]];
+    Edge ed = I.edges[I.accepted[i]];
     B.ident[ed.s1] = 0;
     B.ident[ed.s2] = 0;
-    a.killed[ed.s1] = 0;
-    a.killed[ed.s2] = 0;
+    killed[ed.s1] = 0;
+    killed[ed.s2] = 0;
   }
-  for (int ll = 0; ll < a.L.nlist; ll++) {
+  for (int ll = 0; ll < I.L.nlist; ll++) {
     int k0 = s_k0[ll], k1 = s_k1[ll], nk = k1 - k0;
     int nlc = s_nl[ll], nf = s_nf[ll];
-    int lo = a.L.base[ll];
+    int lo = I.L.base[ll];
     for (int q = tid; q < nk; q += NT) {
       int t = lo + nf + q;  // tail slot
       int l = k0, h = k1;
@@ -1062,14 +1095,14 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
       int above = k1 - l;                          // kills above t
       int rank = (nlc - 1 - (nf + q)) - above;
       int dst = kv[k0 + rank];
-      int rec = a.inv[t];
+      int rec = inv[t];
       B.lpos[rec] = dst - lo;
-      a.inv[dst] = rec;
+      inv[dst] = rec;
     }
   }
   __syncthreads();
   // product append order: Q[j] = creation index K-1-((j+D) mod K)
-  if (tid < a.L.nlist) s_pl[tid] = 0;
+  if (tid < I.L.nlist) s_pl[tid] = 0;
   __syncthreads();
   int overflow = 0;
   const int lane = tid & 63, wv = tid >> 6, nwv = NT >> 6;
@@ -1079,11 +1112,11 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     int c = -1, ll = -1;
     if (j < K) {
       c = K - 1 - (int)(((long)j + D) % K);
-      ll = a.prods[c].list;
+      ll = I.prods[c].list;
     }
     // rank of this product among the chunk's products of its list, in j order
     int inwave = 0;
-    for (int l2 = 0; l2 < a.L.nlist; l2++) {
+    for (int l2 = 0; l2 < I.L.nlist; l2++) {
       uint64_t m = __ballot(ll == l2);
       if (lane == 0) s_wcnt[wv * kMaxList + l2] = __popcll(m);
       if (ll == l2) inwave = __popcll(m & lanes_below);
@@ -1092,24 +1125,26 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
     if (ll >= 0) {
       int pre = inwave;
       for (int w2 = 0; w2 < wv; w2++) pre += s_wcnt[w2 * kMaxList + ll];
-      int slot = a.L.base[ll] + s_nf[ll] + s_pl[ll] + pre;
-      int rec = nbinned + c;
-      if (slot < a.L.base[ll + 1] && rec < a.L.reccap) {
-        Prod pr = a.prods[c];
+      int slot = I.L.base[ll] + s_nf[ll] + s_pl[ll] + pre;
+      int rec = nbinned + s_prodoff + c;
+      if (slot < I.L.base[ll + 1] && rec < reccap) {
+        Prod pr = I.prods[c];
         B.px[rec] = pr.x;
         B.py[rec] = pr.y;
         B.pz[rec] = pr.z;
         B.ident[rec] = pr.ident;
+        if (B.inst) B.inst[rec] = blockIdx.x;  // batch member (multi-simulation records)
         B.serno[rec] = ser0 + (uint64_t)c;
-        B.lpos[rec] = slot - a.L.base[ll];
-        B.box[rec] = pos2box(a.g, pr.x, pr.y, pr.z);
-        a.inv[slot] = rec;
+        B.lpos[rec] = slot - I.L.base[ll];
+        B.box[rec] = pos2box(I.g, pr.x, pr.y, pr.z);
+        inv[slot] = rec;
       } else {
         overflow = 1;
+        if (rec < reccap) B.ident[rec] = 0;  // no molecule: the next step skips the record
       }
     }
     __syncthreads();
-    if (tid < a.L.nlist) {
+    if (tid < I.L.nlist) {
       int t = 0;
       for (int w2 = 0; w2 < nwv; w2++) t += s_wcnt[w2 * kMaxList + tid];
       s_pl[tid] += t;
@@ -1124,18 +1159,22 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(ResolveArgs a) {
   if (tid == 0) {
     int run = 0;
     unsigned long long live0 = 0;
-    for (int ll = 0; ll < a.L.nlist; ll++) live0 += (unsigned long long)s_nl[ll];
+    for (int ll = 0; ll < I.L.nlist; ll++) live0 += (unsigned long long)s_nl[ll];
     ctl->molsteps += live0;
-    for (int ll = 0; ll < a.L.nlist; ll++) {
+    for (int ll = 0; ll < I.L.nlist; ll++) {
       int nlnew = s_nf[ll] + s_pl[ll];
-      int cap = a.L.base[ll + 1] - a.L.base[ll];
+      int cap = I.L.base[ll + 1] - I.L.base[ll];
       if (nlnew > cap) nlnew = cap;
       ctl->nl[ll] = nlnew;
       ctl->pfx[ll] = run;
       if (tab->diffuselist[ll]) run += nlnew;
     }
-    int nrec = nbinned + K;
-    ctl->nrec = nrec < a.L.reccap ? nrec : a.L.reccap;
+    int top = nbinned + s_prodoff + K;
+    atomicMax(&gl->nrec[step & 1], top < reccap ? top : reccap);
+    if (blockIdx.x == 0) {  // counters of the next step
+      gl->nrec[(step + 1) & 1] = 0;
+      gl->nprod[(step + 1) & 1] = 0;
+    }
     ctl->nbinned = nbinned;
     ctl->serno_next = ser0 + (uint64_t)K;
     ctl->naccept = B2;
@@ -1162,146 +1201,218 @@ static int dalloc(T** p, size_t n) {
   return 0;
 }
 
-static int alloc_recs(Recs* r, size_t n) {
+static int alloc_recs(Recs* r, size_t n, bool multi) {
   int ok = 0;
   ok |= dalloc(&r->px, n) | dalloc(&r->py, n) | dalloc(&r->pz, n);
   ok |= dalloc(&r->ident, n) | dalloc(&r->serno, n) | dalloc(&r->lpos, n) | dalloc(&r->box, n) |
         dalloc(&r->rank, n);
+  if (multi) ok |= dalloc(&r->inst, n);
   return ok;
 }
 
 static void free_recs(Recs* r) {
-  void* ptrs[] = {r->px, r->py, r->pz, r->ident, r->serno, r->lpos, r->box, r->rank};
+  void* ptrs[] = {r->px, r->py, r->pz, r->inst, r->ident, r->serno, r->lpos, r->box, r->rank};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
 
-DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const Geometry& geo, const double* gtable,
-                           int ngauss, uint32_t key0, uint32_t key1, const HostSurf* surf, char* err, int errlen) {
+static int upload_instances(DeviceState* s) {
+  HIPOK(hipMemcpy(s->dinst, s->hinst.data(), sizeof(Inst) * s->ninst, hipMemcpyHostToDevice));
+  return 0;
+}
+
+DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables* tab, const Geometry* geo,
+                                 const uint32_t* key0, const uint32_t* key1, const double* gtable, int ngauss,
+                                 const HostSurf* surf, char* err, int errlen) {
   DeviceState* s = new DeviceState();
-  memset((void*)s, 0, sizeof(DeviceState));
-  s->cfg = cfg;
-  s->geo = geo;
-  s->htab = tab;
-  s->key0 = key0;
-  s->key1 = key1;
+  s->ninst = n;
+  s->multi = n > 1;
   s->ngauss = ngauss;
-  s->lay.nlist = cfg.nlist;
-  int run = 0;
-  for (int ll = 0; ll < cfg.nlist; ll++) {
-    s->lay.base[ll] = run;
-    run += cfg.cap[ll];
+  s->cfg.assign(cfg, cfg + n);
+  s->htab.assign(tab, tab + n);
+  s->hinst.assign(n, Inst());
+  if (s->multi && surf && surf->on) {
+    snprintf(err, errlen, "surfaces are only supported for a single simulation");
+    delete s;
+    return nullptr;
   }
-  for (int ll = cfg.nlist; ll <= kMaxList; ll++) s->lay.base[ll] = run;
-  s->lay.captot = run;
-  s->nscan = cfg.nlist * geo.nbox + 1;
+  // slot, key and per-simulation array layout
+  int run = 0, keys = 0;
+  size_t nev = 0, ned = 0, nsort = 0;
+  std::vector<size_t> evoff(n), edoff(n), sortoff(n);
+  long prodroom = 0;
+  for (int k = 0; k < n; k++) {
+    Inst& I = s->hinst[k];
+    memset(&I, 0, sizeof(Inst));
+    I.L.nlist = cfg[k].nlist;
+    for (int ll = 0; ll < cfg[k].nlist; ll++) {
+      I.L.base[ll] = run;
+      run += cfg[k].cap[ll];
+    }
+    for (int ll = cfg[k].nlist; ll <= kMaxList; ll++) I.L.base[ll] = run;
+    I.g = geo[k];
+    I.key0 = key0[k];
+    I.key1 = key1[k];
+    I.boxbase = keys;
+    keys += cfg[k].nlist * geo[k].nbox;
+    I.maxevents = cfg[k].maxevents;
+    I.maxedges = cfg[k].maxedges;
+    I.sortcap = pow2ceil(2 * cfg[k].maxevents + 16);
+    evoff[k] = nev;
+    edoff[k] = ned;
+    sortoff[k] = nsort;
+    nev += (size_t)I.maxevents;
+    ned += (size_t)I.maxedges;
+    nsort += (size_t)I.sortcap;
+    // product records one step can create: the single-simulation engine
+    // keeps the generous bound of the sort space; batch members use their
+    // event capacity (overflow sets kErrCapacity)
+    prodroom += n == 1 ? 2L * I.sortcap : 2L * cfg[k].maxevents + 16;
+    // pair-screen constants: largest binding radius + FP32 margin
+    for (int i = 0; i < kMaxSpecies; i++) {
+      I.pc.reactmask[i] = tab[k].reactmask[i];
+      double rr = sqrt(tab[k].maxbind2[i]) + geo[k].prefilter_margin;
+      I.pc.r2f[i] = (float)(rr * rr);
+    }
+    double ext = 0;
+    for (int d = 0; d < 3; d++) {
+      ext = fmax(ext, fabs(geo[k].low[d]));
+      ext = fmax(ext, fabs(geo[k].high[d]));
+      ext = fmax(ext, geo[k].high[d] - geo[k].low[d]);
+    }
+    I.screen_lim = 2.0 * ext;
+  }
+  s->captot = run;
+  s->nscan = keys + 1;
   s->nscanblk = (s->nscan + kScanTile - 1) / kScanTile;
   if (s->nscanblk > kScanBlock * 8) {
     snprintf(err, errlen, "too many virtual boxes for the scan (%d)", s->nscan);
     delete s;
     return nullptr;
   }
-  s->sortcap = pow2ceil(2 * cfg.maxevents + 16);
-  s->lay.reccap = run + 2 * s->sortcap;
-  size_t rc = (size_t)s->lay.reccap;
+  if ((long)run + prodroom >= (1L << 31)) {
+    snprintf(err, errlen, "too many molecule records");
+    delete s;
+    return nullptr;
+  }
+  s->reccap = run + (int)prodroom;
+  size_t rc = (size_t)s->reccap;
   int ok = 0;
-  ok |= alloc_recs(&s->buf[0], rc) | alloc_recs(&s->buf[1], rc);
-  ok |= dalloc(&s->inv, (size_t)s->lay.captot);
+  ok |= alloc_recs(&s->buf[0], rc, s->multi) | alloc_recs(&s->buf[1], rc, s->multi);
+  ok |= dalloc(&s->inv, (size_t)s->captot);
   ok |= dalloc(&s->bn.count, s->nscan) | dalloc(&s->bn.part, s->nscan) | dalloc(&s->bn.boff, s->nscanblk) |
         dalloc(&s->bn.bstart, s->nscan) | dalloc(&s->bn.spk, rc);
-  ok |= dalloc(&s->edges, cfg.maxedges) | dalloc(&s->estate, cfg.maxedges) | dalloc(&s->accepted, cfg.maxedges);
   ok |= dalloc(&s->bestkey, rc) | dalloc(&s->killed, rc);
-  ok |= dalloc(&s->uniev, cfg.maxevents) | dalloc(&s->log, s->sortcap);
-  ok |= dalloc(&s->sortkey, s->sortcap) | dalloc(&s->sortval, s->sortcap) | dalloc(&s->killslot, s->sortcap);
-  ok |= dalloc(&s->ctl, 1) | dalloc(&s->tab, 1) | dalloc(&s->gtab, ngauss);
-  ok |= dalloc(&s->prods, (size_t)s->sortcap * kMaxProd);
+  ok |= dalloc(&s->edges, ned) | dalloc(&s->estate, ned) | dalloc(&s->accepted, ned);
+  ok |= dalloc(&s->uniev, nev) | dalloc(&s->log, nsort);
+  ok |= dalloc(&s->sortkey, nsort) | dalloc(&s->sortval, nsort) | dalloc(&s->killslot, nsort);
+  ok |= dalloc(&s->prods, nsort * kMaxProd);
+  ok |= dalloc(&s->ctl, (size_t)n) | dalloc(&s->tab, (size_t)n) | dalloc(&s->gtab, ngauss);
+  ok |= dalloc(&s->dinst, (size_t)n) | dalloc(&s->glob, 1);
   if (surf && surf->on) {
     ok |= dalloc(&s->d_pnl, surf->npnl > 0 ? surf->npnl : 1) | dalloc(&s->d_bpstart, (size_t)surf->nbox + 1) |
           dalloc(&s->d_bplist, surf->bpstart[surf->nbox] > 0 ? surf->bpstart[surf->nbox] : 1) |
           dalloc(&s->d_act, (size_t)surf->nsrf * kMaxSpecies * 2);
   }
   if (ok) {
-    snprintf(err, errlen, "device allocation failed (%d molecule records)", s->lay.reccap);
+    snprintf(err, errlen, "device allocation failed (%d molecule records)", s->reccap);
     engine_destroy(s);
     return nullptr;
   }
-  s->inj = nullptr;
-  s->injcap = 0;
-  s->cur = 0;
-  for (int i = 0; i < kMaxSpecies; i++) {
-    s->pc.reactmask[i] = tab.reactmask[i];
-    double rr = sqrt(tab.maxbind2[i]) + geo.prefilter_margin;
-    s->pc.r2f[i] = (float)(rr * rr);
+  for (int k = 0; k < n; k++) {
+    Inst& I = s->hinst[k];
+    I.tab = s->tab + k;
+    I.ctl = s->ctl + k;
+    I.uniev = s->uniev + evoff[k];
+    I.edges = s->edges + edoff[k];
+    I.estate = s->estate + edoff[k];
+    I.accepted = s->accepted + edoff[k];
+    I.log = s->log + sortoff[k];
+    I.sk = s->sortkey + sortoff[k];
+    I.sv = s->sortval + sortoff[k];
+    I.killslot = s->killslot + sortoff[k];
+    I.prods = s->prods + sortoff[k] * kMaxProd;
   }
-  (void)hipMemset(s->bn.count, 0, sizeof(int) * s->nscan);
-  (void)hipMemset(s->bestkey, 0xFF, sizeof(uint64_t) * rc);
-  (void)hipMemset(s->killed, 0, sizeof(int) * rc);
-  (void)hipMemset(s->ctl, 0, sizeof(Control));
-  (void)hipMemcpy(s->tab, &tab, sizeof(DevTables), hipMemcpyHostToDevice);
-  memset(&s->sf, 0, sizeof(DevSurf));
   if (surf && surf->on) {
+    Inst& I = s->hinst[0];
     if (surf->npnl > 0) (void)hipMemcpy(s->d_pnl, surf->pnl, sizeof(DevPanel) * surf->npnl, hipMemcpyHostToDevice);
     (void)hipMemcpy(s->d_bpstart, surf->bpstart, sizeof(int) * (surf->nbox + 1), hipMemcpyHostToDevice);
     if (surf->bpstart[surf->nbox] > 0)
       (void)hipMemcpy(s->d_bplist, surf->bplist, sizeof(int) * surf->bpstart[surf->nbox], hipMemcpyHostToDevice);
     (void)hipMemcpy(s->d_act, surf->act, sizeof(DevSrfAct) * surf->nsrf * kMaxSpecies * 2, hipMemcpyHostToDevice);
-    s->sf.on = 1;
-    s->sf.npnl = surf->npnl;
-    for (int ll = 0; ll < kMaxList; ll++) s->sf.listdiffuse[ll] = surf->listdiffuse[ll];
-    s->sf.epsilon = surf->epsilon;
-    s->sf.pnl = s->d_pnl;
-    s->sf.bpstart = s->d_bpstart;
-    s->sf.bplist = s->d_bplist;
-    s->sf.act = s->d_act;
+    I.sf.on = 1;
+    I.sf.npnl = surf->npnl;
+    for (int ll = 0; ll < kMaxList; ll++) I.sf.listdiffuse[ll] = surf->listdiffuse[ll];
+    I.sf.epsilon = surf->epsilon;
+    I.sf.pnl = s->d_pnl;
+    I.sf.bpstart = s->d_bpstart;
+    I.sf.bplist = s->d_bplist;
+    I.sf.act = s->d_act;
+    s->surf_on = true;
   }
-  {
-    // bound on coordinates for the FP32 pair screen (see k_scatter)
-    double ext = 0;
-    for (int d = 0; d < 3; d++) {
-      ext = fmax(ext, fabs(geo.low[d]));
-      ext = fmax(ext, fabs(geo.high[d]));
-      ext = fmax(ext, geo.high[d] - geo.low[d]);
-    }
-    s->screen_lim = 2.0 * ext;
-  }
+  (void)hipMemset(s->bn.count, 0, sizeof(int) * s->nscan);
+  (void)hipMemset(s->bestkey, 0xFF, sizeof(uint64_t) * rc);
+  (void)hipMemset(s->killed, 0, sizeof(int) * rc);
+  (void)hipMemset(s->ctl, 0, sizeof(Control) * n);
+  (void)hipMemset(s->glob, 0, sizeof(Glob));
+  (void)hipMemcpy(s->tab, tab, sizeof(DevTables) * n, hipMemcpyHostToDevice);
   (void)hipMemcpy(s->gtab, gtable, sizeof(double) * ngauss, hipMemcpyHostToDevice);
-  (void)hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (upload_instances(s)) {
+    snprintf(err, errlen, "device copy failed");
+    engine_destroy(s);
+    return nullptr;
+  }
+  if (hipStreamCreate(&s->stream) != hipSuccess) {
+    snprintf(err, errlen, "stream creation failed");
+    engine_destroy(s);
+    return nullptr;
+  }
   for (int k = 0; k < 16; k++) (void)hipEventCreate(&s->ev[k]);
   return s;
+}
+
+DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const Geometry& geo, const double* gtable,
+                           int ngauss, uint32_t key0, uint32_t key1, const HostSurf* surf, char* err, int errlen) {
+  return engine_create_batch(1, &cfg, &tab, &geo, &key0, &key1, gtable, ngauss, surf, err, errlen);
 }
 
 void engine_destroy(DeviceState* s) {
   if (!s) return;
   free_recs(&s->buf[0]);
   free_recs(&s->buf[1]);
-  void* ptrs[] = {s->inv, s->bn.count, s->bn.part, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges, s->estate,
-                  s->accepted, s->bestkey, s->killed, s->uniev, s->log, s->sortkey, s->sortval, s->killslot,
-                  s->ctl, s->tab, s->gtab, s->inj, s->prods, s->d_pnl, s->d_bpstart, s->d_bplist, s->d_act};
+  void* ptrs[] = {s->inv,     s->bn.count, s->bn.part, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges,  s->estate,
+                  s->accepted, s->bestkey, s->killed,  s->uniev,   s->log,       s->sortkey, s->sortval, s->killslot,
+                  s->ctl,     s->tab,     s->gtab,    s->inj,     s->prods,     s->d_pnl,   s->d_bpstart,
+                  s->d_bplist, s->d_act,  s->dinst,   s->glob};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; k++)
     if (s->graph_ready[k]) (void)hipGraphExecDestroy(s->graph[k]);
   if (s->stream) {
-    for (int k = 0; k < 16; k++) (void)hipEventDestroy(s->ev[k]);
+    for (int k = 0; k < 16; k++)
+      if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
     (void)hipStreamDestroy(s->stream);
   }
   delete s;
 }
 
-// Upload list ll in live order.  Records are appended after the records of
-// the lists uploaded before it; the first step bins everything.
-int engine_upload_list(DeviceState* s, int ll, int n, const int* ident, const double* pos, const uint64_t* serno) {
-  if (ll < 0 || ll >= s->lay.nlist) return -2;
-  int cap = s->lay.base[ll + 1] - s->lay.base[ll];
+// Upload list ll of simulation inst in live order.  Records are appended
+// after the records uploaded before; the first step bins everything.
+int engine_upload_list_inst(DeviceState* s, int inst, int ll, int n, const int* ident, const double* pos,
+                            const uint64_t* serno) {
+  if (inst < 0 || inst >= s->ninst) return -2;
+  const Inst& I = s->hinst[inst];
+  if (ll < 0 || ll >= I.L.nlist) return -2;
+  int cap = I.L.base[ll + 1] - I.L.base[ll];
   if (n > cap) return -3;
   Control c;
-  HIPOK(hipMemcpy(&c, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
-  int r0 = c.nrec;
-  if (r0 + n > s->lay.reccap) return -3;
+  HIPOK(hipMemcpy(&c, s->ctl + inst, sizeof(Control), hipMemcpyDeviceToHost));
+  int r0 = s->hnrec;
+  if (r0 + n > s->reccap) return -3;
   Recs& R = s->buf[s->cur];
   if (n) {
     std::vector<double> x(n), y(n), z(n);
-    std::vector<int> lp(n), inv(n), box(n, 0);
+    std::vector<int> lp(n), inv(n), box(n, 0), in(n, inst);
     for (int k = 0; k < n; k++) {
       x[k] = pos[3 * k];
       y[k] = pos[3 * k + 1];
@@ -1316,74 +1427,57 @@ int engine_upload_list(DeviceState* s, int ll, int n, const int* ident, const do
     HIPOK(hipMemcpy(R.serno + r0, serno, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(R.lpos + r0, lp.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(R.box + r0, box.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(s->inv + s->lay.base[ll], inv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    if (s->multi) HIPOK(hipMemcpy(R.inst + r0, in.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(s->inv + I.L.base[ll], inv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   }
+  s->hnrec = r0 + n;
   c.nl[ll] = n;
-  c.nrec = r0 + n;
   int run = 0;
-  for (int l2 = 0; l2 < s->lay.nlist; l2++) {
+  for (int l2 = 0; l2 < I.L.nlist; l2++) {
     c.pfx[l2] = run;
-    if (s->htab.diffuselist[l2]) run += c.nl[l2];
+    if (s->htab[inst].diffuselist[l2]) run += c.nl[l2];
   }
-  HIPOK(hipMemcpy(s->ctl, &c, sizeof(Control), hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(s->ctl + inst, &c, sizeof(Control), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int engine_upload_list(DeviceState* s, int ll, int n, const int* ident, const double* pos, const uint64_t* serno) {
+  return engine_upload_list_inst(s, 0, ll, n, ident, pos, serno);
+}
+
+// Counters of simulation inst; also sets the record count of the uploaded
+// state (every simulation of a batch starts at the same step).
+int engine_set_counters_inst(DeviceState* s, int inst, uint64_t serno_next, uint32_t step, const int* counts) {
+  Control c;
+  HIPOK(hipMemcpy(&c, s->ctl + inst, sizeof(Control), hipMemcpyDeviceToHost));
+  c.serno_next = serno_next;
+  c.step = step;
+  for (int i = 0; i < kMaxSpecies; i++) c.count[i] = counts ? counts[i] : 0;
+  HIPOK(hipMemcpy(s->ctl + inst, &c, sizeof(Control), hipMemcpyHostToDevice));
+  Glob g;
+  memset(&g, 0, sizeof(g));
+  g.nrec[(step + 1) & 1] = s->hnrec;
+  HIPOK(hipMemcpy(s->glob, &g, sizeof(Glob), hipMemcpyHostToDevice));
   return 0;
 }
 
 int engine_set_counters(DeviceState* s, uint64_t serno_next, uint32_t step, const int* counts) {
-  Control c;
-  HIPOK(hipMemcpy(&c, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
-  c.serno_next = serno_next;
-  c.step = step;
-  for (int i = 0; i < kMaxSpecies; i++) c.count[i] = counts ? counts[i] : 0;
-  HIPOK(hipMemcpy(s->ctl, &c, sizeof(Control), hipMemcpyHostToDevice));
-  return 0;
-}
-
-static ResolveArgs resolve_args(DeviceState* s, int src) {
-  ResolveArgs a;
-  a.L = s->lay;
-  a.g = s->geo;
-  a.sf = s->sf;
-  a.tab = s->tab;
-  a.ctl = s->ctl;
-  a.a = s->buf[src];
-  a.b = s->buf[1 - src];
-  a.inv = s->inv;
-  a.bn = s->bn;
-  a.nscan = s->nscan;
-  a.edges = s->edges;
-  a.maxedges = s->cfg.maxedges;
-  a.bestkey = s->bestkey;
-  a.killed = s->killed;
-  a.estate = s->estate;
-  a.accepted = s->accepted;
-  a.uniev = s->uniev;
-  a.maxevents = s->cfg.maxevents;
-  a.log = s->log;
-  a.sk = s->sortkey;
-  a.sv = s->sortval;
-  a.sortcap = s->sortcap;
-  a.killslot = s->killslot;
-  a.prods = s->prods;
-  a.key0 = s->key0;
-  a.key1 = s->key1;
-  return a;
+  return engine_set_counters_inst(s, 0, serno_next, step, counts);
 }
 
 // Launch one step from buffer src into 1-src on s->stream.  Grids cover the
 // record capacity; kernels read the exact counts from device memory.
 // timed: events between the kernels into ev[0..6].
 static void launch_step(DeviceState* s, int src, const double* dinj, int timed) {
-  int rc = s->lay.reccap;
-  int nb = (rc + kBlock - 1) / kBlock;
+  int nb = (s->reccap + kBlock - 1) / kBlock;
   hipStream_t st = s->stream;
-  int nbox = s->geo.nbox;
   Recs A = s->buf[src], B = s->buf[1 - src];
+  auto kdiff = s->multi ? k_diffuse<false, true> : (s->surf_on ? k_diffuse<true, false> : k_diffuse<false, false>);
+  auto kscat = s->multi ? k_scatter<true> : k_scatter<false>;
+  auto kpair = s->multi ? k_pairs<false, true> : (s->surf_on ? k_pairs<true, false> : k_pairs<false, false>);
   if (timed) (void)hipEventRecord(s->ev[0], st);
-  auto kdiff = s->sf.on ? k_diffuse<true> : k_diffuse<false>;
-  auto kpair = s->sf.on ? k_pairs<true> : k_pairs<false>;
-  hipLaunchKernelGGL(kdiff, dim3(nb), dim3(kBlock), 0, st, s->lay, s->geo, s->tab, s->ctl, A, s->bn.count,
-                     s->gtab, s->ngauss - 1, dinj, s->key0, s->key1, s->uniev, s->cfg.maxevents, nbox, s->sf);
+  hipLaunchKernelGGL(kdiff, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, s->bn.count, s->gtab,
+                     s->ngauss - 1, dinj);
   if (timed) (void)hipEventRecord(s->ev[1], st);
   hipLaunchKernelGGL(k_scan1, dim3(s->nscanblk), dim3(kScanBlock), 0, st, s->bn.count, s->bn.part, s->bn.boff,
                      s->nscan);
@@ -1391,25 +1485,34 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   hipLaunchKernelGGL(k_scan3, dim3((s->nscan + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s->bn.part, s->bn.boff,
                      s->bn.bstart, s->nscan);
   if (timed) (void)hipEventRecord(s->ev[2], st);
-  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, st, s->lay, s->tab, s->ctl, A, B, s->bn, s->inv, nbox,
-                     s->sortcap, s->screen_lim);
+  hipLaunchKernelGGL(kscat, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, B, s->bn, s->inv);
   if (timed) (void)hipEventRecord(s->ev[3], st);
   if (timed) (void)hipEventRecord(s->ev[4], st);
-  hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->lay, s->geo, s->tab, s->pc, s->ctl, B, s->bn, nbox,
-                     s->nscan, s->edges, s->cfg.maxedges, s->key0, s->key1, s->sf);
+  hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->dinst, B, s->bn, s->nscan);
   if (timed) (void)hipEventRecord(s->ev[5], st);
-  hipLaunchKernelGGL(k_resolve, dim3(1), dim3(kResolveThreads), 0, st, resolve_args(s, src));
+  hipLaunchKernelGGL(k_resolve, dim3(s->ninst), dim3(kResolveThreads), 0, st, s->dinst, s->glob, A, B, s->inv, s->bn,
+                     s->nscan, s->bestkey, s->killed, s->reccap);
   if (timed) (void)hipEventRecord(s->ev[6], st);
+}
+
+// OR of the error bits of all simulations
+static int read_errors(DeviceState* s) {
+  std::vector<Control> c(s->ninst);
+  if (hipMemcpy(c.data(), s->ctl, sizeof(Control) * s->ninst, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  int e = 0;
+  for (auto& x : c) e |= x.err;
+  return e;
 }
 
 int engine_step(DeviceState* s, const double* inject, long ninject) {
   const double* dinj = nullptr;
   if (inject) {
+    if (s->multi) return -1;
     Control c0;
     HIPOK(hipMemcpy(&c0, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
     long need = 0;
-    for (int ll = 0; ll < s->lay.nlist; ll++)
-      if (s->htab.diffuselist[ll]) need += c0.nl[ll];
+    for (int ll = 0; ll < s->hinst[0].L.nlist; ll++)
+      if (s->htab[0].diffuselist[ll]) need += c0.nl[ll];
     if (ninject < need) return -2;
     if (ninject * 3 > s->injcap) {
       if (s->inj) HIPOK(hipFree(s->inj));
@@ -1424,9 +1527,7 @@ int engine_step(DeviceState* s, const double* inject, long ninject) {
   HIPOK(hipGetLastError());
   HIPOK(hipStreamSynchronize(s->stream));
   s->cur ^= 1;
-  Control c;
-  HIPOK(hipMemcpy(&c, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
-  return c.err;
+  return read_errors(s);
 }
 
 int engine_run(DeviceState* s, long nsteps, int use_graph) {
@@ -1456,13 +1557,11 @@ int engine_run(DeviceState* s, long nsteps, int use_graph) {
   float ms = 0;
   HIPOK(hipEventElapsedTime(&ms, e0, e1));
   s->last_ms = ms;
-  Control c;
-  HIPOK(hipMemcpy(&c, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
-  return c.err;
+  return read_errors(s);
 }
 
 // Steps with events between the kernels; per-kernel average ms in out[]:
-// diffuse, box scan, scatter, (unused), pairs, resolve.
+// diffuse, box scan, scatter, event overhead, pairs, resolve.
 int engine_profile(DeviceState* s, long nsteps, double* out) {
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (long k = 0; k < nsteps; k++) {
@@ -1476,32 +1575,45 @@ int engine_profile(DeviceState* s, long nsteps, double* out) {
     }
   }
   for (int j = 0; j < 6; j++) out[j] = acc[j] / (double)(nsteps > 0 ? nsteps : 1);
-  Control c;
-  HIPOK(hipMemcpy(&c, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
-  return c.err;
+  return read_errors(s);
 }
 
-int engine_sync_control(DeviceState* s, Control* out) {
+int engine_get_control(DeviceState* s, int inst, Control* out) {
+  if (inst < 0 || inst >= s->ninst) return -2;
   HIPOK(hipStreamSynchronize(s->stream));
-  HIPOK(hipMemcpy(out, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(out, s->ctl + inst, sizeof(Control), hipMemcpyDeviceToHost));
   return 0;
 }
 
-int engine_get_list(DeviceState* s, int ll, int* ident, double* pos, uint64_t* serno, int* box, int max) {
+int engine_sync_control(DeviceState* s, Control* out) { return engine_get_control(s, 0, out); }
+
+static int current_nrec(DeviceState* s) {
   Control c;
-  if (engine_sync_control(s, &c)) return -1;
-  if (ll < 0 || ll >= s->lay.nlist) return -2;
+  Glob g;
+  if (hipMemcpy(&c, s->ctl, sizeof(Control), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  if (hipMemcpy(&g, s->glob, sizeof(Glob), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return g.nrec[(c.step + 1) & 1];
+}
+
+int engine_get_list_inst(DeviceState* s, int inst, int ll, int* ident, double* pos, uint64_t* serno, int* box,
+                         int max) {
+  Control c;
+  if (engine_get_control(s, inst, &c)) return -1;
+  const Inst& I = s->hinst[inst];
+  if (ll < 0 || ll >= I.L.nlist) return -2;
   int n = c.nl[ll];
   if (n > max) n = max;
   if (n <= 0) return c.nl[ll];
   Recs& R = s->buf[s->cur];
-  int rc = s->lay.reccap;
-  hipLaunchKernelGGL(k_rebuild_inv, dim3((rc + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream, s->lay, s->tab,
-                     s->ctl, R, s->inv);
+  int nb = (s->reccap + kBlock - 1) / kBlock;
+  auto kinv = s->multi ? k_rebuild_inv<true> : k_rebuild_inv<false>;
+  hipLaunchKernelGGL(kinv, dim3(nb), dim3(kBlock), 0, s->stream, s->dinst, s->glob, R, s->inv);
   HIPOK(hipStreamSynchronize(s->stream));
   std::vector<int> inv(n);
-  HIPOK(hipMemcpy(inv.data(), s->inv + s->lay.base[ll], sizeof(int) * n, hipMemcpyDeviceToHost));
-  int nr = c.nrec > 0 ? c.nrec : 1;
+  HIPOK(hipMemcpy(inv.data(), s->inv + I.L.base[ll], sizeof(int) * n, hipMemcpyDeviceToHost));
+  int nr = current_nrec(s);
+  if (nr <= 0) nr = 1;
+  // gather the records of this list only
   std::vector<double> x(nr), y(nr), z(nr);
   std::vector<int> id(nr), bx(nr);
   std::vector<uint64_t> sr(nr);
@@ -1526,19 +1638,20 @@ int engine_get_list(DeviceState* s, int ll, int* ident, double* pos, uint64_t* s
   return c.nl[ll];
 }
 
+int engine_get_list(DeviceState* s, int ll, int* ident, double* pos, uint64_t* serno, int* box, int max) {
+  return engine_get_list_inst(s, 0, ll, ident, pos, serno, box, max);
+}
+
 int engine_get_log(DeviceState* s, LogRow* rows, int maxrows) {
   Control c;
   if (engine_sync_control(s, &c)) return -1;
   int n = c.nlog < maxrows ? c.nlog : maxrows;
-  if (n > 0 && rows) HIPOK(hipMemcpy(rows, s->log, sizeof(LogRow) * n, hipMemcpyDeviceToHost));
+  if (n > 0 && rows) HIPOK(hipMemcpy(rows, s->hinst[0].log, sizeof(LogRow) * n, hipMemcpyDeviceToHost));
   return c.nlog;
 }
 
 double engine_last_ms(DeviceState* s) { return s->last_ms; }
 
-int engine_device_bytes(DeviceState* s, double* out) {
-  *out = 2.0 * s->lay.reccap * (3 * 8 + 4 + 8 + 4 + 4 + 4) + (double)s->lay.reccap * 16 + (double)s->nscan * 16;
-  return 0;
-}
+int engine_num_instances(DeviceState* s) { return s->ninst; }
 
 }  // namespace smolamd

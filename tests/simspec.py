@@ -62,7 +62,7 @@ def build_oracle(spec):
     return sim, rx
 
 
-def build_device(spec):
+def build_device(spec, update=True):
     import smoldyn_amd as sa
     sim = sa.Simulation(spec["low"], spec["high"])
     for d, w in enumerate(spec["walls"]):
@@ -102,7 +102,8 @@ def build_device(spec):
             sim.add_molecules(sp, n, spec.get("mol_low"), spec.get("mol_high"))
         else:
             sim.add_molecules_at(sp, n)
-    sim.update()
+    if update:
+        sim.update()
     return sim
 
 
@@ -139,6 +140,33 @@ def c3_spec(n=2000, side=100.0, seed=31, kback=5.0, ndisk=8, rdisk=None, bindrad
                 ],
                 mol_low=[c - h] * 3, mol_high=[c + h] * 3,
                 molecules=[("A", n), ("B", n)])
+
+
+def c4_member(k, seed=500, n_lo=1000, n_hi=100000, side_lo=50.0, side_hi=400.0, dt=0.01, scale=1.0):
+    """Member k of the batched parameter sweep: A+B<->C in a reflective cube,
+    N log-uniform in [n_lo, n_hi] (split evenly between A and B), D
+    log-uniform in [0.1, 10] (A; B at half, C at a quarter), binding radius uniform in
+    [0.2, 2.0] (forward rate from the reaction-parameter solver), side
+    uniform in [side_lo, side_hi], unbinding radius 1.5x the binding radius,
+    back rate 1.  scale multiplies N (for smaller tests)."""
+    import smoldyn_amd as sa
+    rng = np.random.default_rng(seed * 100003 + k)
+    n = int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))) * scale)
+    n = max(n, 20)
+    d = float(np.exp(rng.uniform(np.log(0.1), np.log(10.0))))
+    a = float(rng.uniform(0.2, 2.0))
+    side = float(rng.uniform(side_lo, side_hi))
+    sa.lib()
+    # forward rate whose binding radius is a with unbinding radius 1.5 a
+    # (numrxnrate(step, a, b) = rate * dt, rxnparam.c:558-583)
+    dsum = 1.5 * d
+    step = (2 * dsum * dt) ** 0.5
+    rate = sa.lib().smolamdNumRxnRate(step, a, 1.5 * a) / dt
+    return dict(low=[0.0] * 3, high=[side] * 3, walls="rrr", dt=dt, seed=seed + k,
+                species={"A": d, "B": d / 2, "C": d / 4},
+                reactions=[("fwd", ["A", "B"], ["C"], rate), ("back", ["C"], ["A", "B"], 1.0)],
+                placement={"back": ("ratio", 1.5)},
+                molecules=[("A", n // 2), ("B", n - n // 2)])
 
 
 def oracle_state(osim):

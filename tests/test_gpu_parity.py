@@ -173,3 +173,53 @@ def test_surfaces_jump_panels():
                 molecules=[("A", 2000), ("B", 2000)])
     nev = _run_pair(spec, 200, check_every=20)
     assert nev > 10
+
+
+# ---------------------------------------------------------------------------
+# ensembles
+
+
+def _same_lists(a, b, what):
+    assert len(a["ident"]) == len(b["ident"]), what
+    assert np.array_equal(a["serno"], b["serno"]), what
+    assert np.array_equal(a["ident"], b["ident"]), what
+    assert np.array_equal(a["pos"].view(np.uint64), b["pos"].view(np.uint64)), what
+    assert np.array_equal(a["box"], b["box"]), what
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_ensemble_members_equal_single_runs(graph):
+    """Members of uneven size, diffusion coefficient, binding radius and box
+    size stepped together give exactly their single-simulation results."""
+    import smoldyn_amd as sa
+    specs = [ss.c4_member(k, seed=700, scale=0.03) for k in range(7)]
+    members = [ss.build_device(sp, update=False) for sp in specs]
+    ens = sa.Ensemble(members)
+    singles = [ss.build_device(sp) for sp in specs]
+    for block in range(3):
+        ens.run_steps(40, graph=graph)
+        for s in singles:
+            s.run_steps(40, graph=graph)
+        for k, s in enumerate(singles):
+            _same_lists(ens.molecules(k), s.molecules(0), "member %d after %d steps" % (k, 40 * (block + 1)))
+            for sp in ("A", "B", "C"):
+                assert ens.count(k, sp) == s.count(sp)
+    assert sum(ens.diagnostics(k)["molsteps"] for k in range(len(specs))) == \
+        sum(s.diagnostics()["molsteps"] for s in singles)
+
+
+def test_ensemble_against_oracle():
+    """One ensemble member checked step by step against the oracle."""
+    import smoldyn_amd as sa
+    specs = [ss.c4_member(k, seed=900, scale=0.02) for k in range(3)]
+    members = [ss.build_device(sp, update=False) for sp in specs]
+    ens = sa.Ensemble(members)
+    osim, _ = ss.build_oracle(specs[1])
+    for step in range(60):
+        osim.step()
+        ens.run_steps(1, graph=False)
+        if step % 20 == 19:
+            o = ss.oracle_state(osim)[0]
+            d = ens.molecules(1)
+            _same_lists(dict(ident=o["ident"], serno=o["serno"], pos=o["pos"], box=o["box"]), d,
+                        "member 1 step %d" % step)
