@@ -27,6 +27,11 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+
+def log(msg):
+    """Progress line on stderr (the JSON result goes to stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
 # Algorithmic HBM bytes per molecule per launch of each kernel (molecule
 # records in box order, FP64 positions):
 #   k_diffuse  read pos 24 + species 4 + live position 4 + serial 8,
@@ -71,6 +76,175 @@ def build_device(spec):
     return sim
 
 
+def c4_spec(k, seed=500, dt=0.01):
+    """Member k of the batched parameter sweep: A+B<->C in a reflective cube
+    of side uniform in [50, 400], N log-uniform in [1e3, 1e5] (A and B), D_A
+    log-uniform in [0.1, 10] (B at half, C at a quarter), binding radius
+    uniform in [0.2, 2.0] with unbinding radius 1.5x (forward rate from the
+    reaction-parameter solver), back rate 1, dt 0.01."""
+    import smoldyn_amd as sa
+    rng = np.random.default_rng(seed * 100003 + k)
+    n = int(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
+    d = float(np.exp(rng.uniform(np.log(0.1), np.log(10.0))))
+    a = float(rng.uniform(0.2, 2.0))
+    side = float(rng.uniform(50.0, 400.0))
+    dsum = 1.5 * d
+    rate = sa.lib().smolamdNumRxnRate((2 * dsum * dt) ** 0.5, a, 1.5 * a) / dt
+    return dict(low=[0.0] * 3, high=[side] * 3, walls="rrr", dt=dt, seed=seed + k,
+                species={"A": d, "B": d / 2, "C": d / 4},
+                reactions=[("fwd", ["A", "B"], ["C"], rate), ("back", ["C"], ["A", "B"], 1.0)],
+                placement={"back": ("ratio", 1.5)},
+                molecules=[("A", n // 2), ("B", n - n // 2)])
+
+
+def build_member(spec, molperbox=None, update=False):
+    import smoldyn_amd as sa
+    sim = sa.Simulation(spec["low"], spec["high"])
+    for d, w in enumerate(spec["walls"]):
+        sim.boundary(d, 0, w)
+        sim.boundary(d, 1, w)
+    sim.times(0.0, 1e30, spec["dt"])
+    sim.seed(spec["seed"])
+    for name, d in spec["species"].items():
+        sim.species(name)
+        sim.mobility(name, d)
+    if molperbox:
+        sim.partitions("molperbox", molperbox)
+    for name, rct, prd, rate in spec["reactions"]:
+        sim.reaction(name, rct, prd, rate)
+    for name, (kind, param) in spec["placement"].items():
+        sim.product_placement(name, kind, param)
+    for sp, n in spec["molecules"]:
+        sim.add_molecules(sp, n)
+    if update:
+        sim.update()
+    return sim
+
+
+def ensemble_leg(members, steps, warmup, tune=True, sim_steps=2000):
+    """Batched sweep (one set of launches per step for all members) against
+    the plain-GPU form (the same members, each its own engine, run one after
+    another), both with HIP-graph stepping.  Throughput in simulations per
+    hour for sim_steps-step simulations, from a timed window of `steps`."""
+    import smoldyn_amd as sa
+    from smoldyn_amd.autotune import size_class, tune_box_size
+    specs = [c4_spec(k) for k in range(members)]
+    counts = [sum(n for _, n in sp["molecules"]) for sp in specs]
+    best, timing = ({}, {})
+    if tune:
+        best, timing = tune_box_size(lambda k, m: build_member(specs[k], m), range(members), counts)
+    log("autotune: %s" % best)
+    mpb = [best.get(size_class(c)) for c in counts]
+    sims = [build_member(specs[k], mpb[k]) for k in range(members)]
+    t0 = time.perf_counter()
+    ens = sa.Ensemble(sims)
+    setup = time.perf_counter() - t0
+    ens.run_steps(warmup)
+    m0 = sum(ens.diagnostics(k)["molsteps"] for k in range(members))
+    t0 = time.perf_counter()
+    ens.run_steps(steps)
+    t_batch = time.perf_counter() - t0
+    molsteps = sum(ens.diagnostics(k)["molsteps"] for k in range(members)) - m0
+    err = max(ens.diagnostics(k)["err"] for k in range(members))
+    del ens
+    del sims
+    log("batched %.2f ms/step; plain-GPU members next" % (t_batch * 1e3 / steps))
+    t_plain = 0.0
+    for k in range(members):
+        sim = build_member(specs[k], mpb[k], update=True)
+        sim.run_steps(warmup)
+        t0 = time.perf_counter()
+        sim.run_steps(steps)
+        t_plain += time.perf_counter() - t0
+        del sim
+    scale = sim_steps / steps
+    sph = members * 3600.0 / (t_batch * scale)
+    sph_plain = members * 3600.0 / (t_plain * scale)
+    return {"members": members, "molecules": int(sum(counts)), "timed_steps": steps, "sim_steps": sim_steps,
+            "sims_per_hour": sph, "plain_gpu_sims_per_hour": sph_plain, "speedup_vs_plain_gpu": sph / sph_plain,
+            "batched_ms_per_step": t_batch * 1e3 / steps, "plain_gpu_ms_per_member_step": t_plain * 1e3 / steps / members,
+            "molecule_steps_per_s": molsteps / t_batch, "setup_s": setup, "err": err,
+            "molperbox_by_size_class": {str(c): v for c, v in best.items()},
+            "autotune_ms_per_step": {str(c): {str(m): t for m, t in tv.items()} for c, tv in timing.items()}}
+
+
+def c5_spec(k, seed=700, dt=0.01, n=10000, side=100.0):
+    """Member k of the learned-surrogate sweep: A+B<->C in a reflective cube of
+    side 100 with n/2 A + n/2 B, D = 1/1/0.5, dt 0.01 (rms step of the pair
+    0.2); rms step / binding radius log-uniform in [0.05, 5], unbinding /
+    binding radius ratio uniform in [1, 3]; the target forward rate is the
+    reaction-parameter solver's rate at those radii, and the setup inverts it
+    back into a binding radius (the surrogate's job).  Back rate 1."""
+    import smoldyn_amd as sa
+    rng = np.random.default_rng(seed * 100003 + k)
+    step = (2 * 2.0 * dt) ** 0.5
+    while True:
+        soa = float(np.exp(rng.uniform(np.log(0.05), np.log(5.0))))
+        ratio = float(rng.uniform(1.0, 3.0))
+        a = step / soa
+        rate = sa.lib().smolamdNumRxnRate(step, a, ratio * a) / dt
+        if rate > 0:
+            break
+    return dict(low=[0.0] * 3, high=[side] * 3, walls="rrr", dt=dt, seed=seed + k,
+                species={"A": 1.0, "B": 1.0, "C": 0.5},
+                reactions=[("fwd", ["A", "B"], ["C"], rate), ("back", ["C"], ["A", "B"], 1.0)],
+                placement={"back": ("ratio", ratio)},
+                molecules=[("A", n // 2), ("B", n - n // 2)], target_a=a, ratio=ratio, step=step)
+
+
+def sweep_leg(members, steps, warmup, sim_steps=2000):
+    """Learned-surrogate sweep: `members` parameter sets batched in one
+    ensemble whose binding radii come from the surrogate (judged by the
+    reaction-parameter solver, bisection as fallback).  Reports the batched
+    throughput and what the surrogate gains on the sweep's inversions."""
+    import smoldyn_amd as sa
+    from smoldyn_amd import surrogate as sg
+    specs = [c5_spec(k) for k in range(members)]
+    model = sg.RateSurrogate.load()
+    ratedt = np.array([sp["reactions"][0][3] * sp["dt"] for sp in specs])
+    step = np.array([sp["step"] for sp in specs])
+    ratio = np.array([sp["ratio"] for sp in specs])
+    model.invert(ratedt[:64], step[:64], ratio[:64])  # device initialization
+    a_s, ok, ts = model.invert(ratedt, step, ratio)
+    t0 = time.perf_counter()
+    a_n = sg.invert_numerical(ratedt, step, ratio)
+    t_bisect = time.perf_counter() - t0
+    err_s = sg.rate_error(ratedt, step, ratio, a_s)
+    err_n = sg.rate_error(ratedt, step, ratio, a_n)
+    log("sweep inversions: surrogate %.2f ms, bisection %.2f ms" % (ts["total_s"] * 1e3, t_bisect * 1e3))
+    model.install(1e-3)
+    try:
+        t0 = time.perf_counter()
+        sims = [build_member(sp) for sp in specs]
+        ens = sa.Ensemble(sims)
+        setup = time.perf_counter() - t0
+        acc, rej = sg.RateSurrogate.stats()
+    finally:
+        sg.RateSurrogate.uninstall()
+    log("sweep ensemble built in %.1f s" % setup)
+    ens.run_steps(warmup)
+    m0 = sum(ens.diagnostics(k)["molsteps"] for k in range(members))
+    t0 = time.perf_counter()
+    ens.run_steps(steps)
+    t_batch = time.perf_counter() - t0
+    molsteps = sum(ens.diagnostics(k)["molsteps"] for k in range(members)) - m0
+    err = max(ens.diagnostics(k)["err"] for k in range(members))
+    del ens, sims
+    return {"members": members, "molecules": int(sum(sum(n for _, n in sp["molecules"]) for sp in specs)),
+            "timed_steps": steps, "batched_ms_per_step": t_batch * 1e3 / steps,
+            "sims_per_hour": members * 3600.0 / (t_batch * sim_steps / steps), "sim_steps": sim_steps,
+            "molecule_steps_per_s": molsteps / t_batch, "setup_s": setup, "err": err,
+            "setup_surrogate_accepted": acc, "setup_surrogate_rejected": rej,
+            "inversion": {"queries": members, "surrogate_ms": ts["total_s"] * 1e3,
+                          "surrogate_network_kernel_ms": ts["network_kernel_ms"],
+                          "surrogate_accepted_fraction": float(ok.mean()), "bisection_ms": t_bisect * 1e3,
+                          "speedup": t_bisect / ts["total_s"],
+                          "median_rate_error_surrogate": float(np.median(err_s)),
+                          "median_rate_error_bisection": float(np.median(err_n)),
+                          "max_rate_error_surrogate": float(err_s.max()),
+                          "max_rate_error_bisection": float(err_n.max())}}
+
+
 def cpu_baseline(spec, steps):
     """The CPU oracle (a restatement of the reference CPU path, 1 thread) on the
     same workload: setup excluded, `steps` steps timed."""
@@ -113,6 +287,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--n", type=int, default=500000, help="molecules per reactant species")
+    ap.add_argument("--ensemble-members", type=int, default=256,
+                    help="members of the batched parameter sweep (0: skip the ensemble leg)")
+    ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--sweep-members", type=int, default=4096,
+                    help="members of the learned-surrogate sweep, 1e4 molecules each (0: skip)")
+    ap.add_argument("--sweep-steps", type=int, default=100)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -126,6 +307,7 @@ def main():
 
     import smoldyn_amd as sa
     spec = c2_spec(n_per_species=args.n)
+    log("setting up the 1M-molecule workload")
     t_setup = time.perf_counter()
     sim = build_device(spec)
     setup_s = time.perf_counter() - t_setup
@@ -135,6 +317,7 @@ def main():
     d0 = sim.diagnostics()
     # timed region: all steps captured as HIP-graph replays on one stream; the
     # device accumulates the live molecule count at the start of every step
+    log("timing %d steps" % args.steps)
     t0 = time.perf_counter()
     dev_ms = sim.run_steps(args.steps, graph=use_graph)
     wall = time.perf_counter() - t0
@@ -183,7 +366,17 @@ def main():
         "final_counts": counts,
         "setup_s": setup_s,
     }
+    if args.ensemble_members > 0:
+        del sim
+        log("batched ensemble leg, %d members" % args.ensemble_members)
+        out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps, 20, tune=not args.no_autotune)
+    if args.sweep_members > 0:
+        if "sim" in locals():
+            del sim
+        log("learned-surrogate sweep leg, %d members" % args.sweep_members)
+        out["surrogate_sweep"] = sweep_leg(args.sweep_members, args.sweep_steps, 10)
     if not args.no_cpu:
+        log("CPU oracle baseline, %d steps" % args.cpu_steps)
         cv, cdt = cpu_baseline(spec, args.cpu_steps)
         out["cpu_baseline"] = {"value": cv, "unit": "molecule-steps/s", "cores": 1, "kind": "port",
                                "sample": "same 1M-molecule workload, %d steps after setup (%.1f s), CPU oracle "

@@ -22,6 +22,7 @@
 #include "engine.h"
 #include "rxnparam.h"
 #include "surfaces_host.h"
+#include "surrogate_host.h"
 
 using namespace smolamd;
 
@@ -319,6 +320,8 @@ int set_rate(simptr sim, int order, int r) {
     else if (dsum <= 0) return 4;
     else if (rparamt == RPunbindrad)
       rx.bindrad2 = rxnparam::bindingradiusprob(rate3, dtt, dsum, rparam, 0, rx.chi, &rx.prob);
+    else if (rparamt == RPratio && surrogate::enabled() && rx.chi < 0 && rx.prob == 1)
+      rx.bindrad2 = surrogate::binding_radius(rate3, dtt, dsum, rparam, nullptr);
     else if (rparamt == RPratio)
       rx.bindrad2 = rxnparam::bindingradiusprob(rate3, dtt, dsum, rparam, 1, rx.chi, &rx.prob);
     else if (rparamt == RPpgem)
@@ -738,9 +741,9 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   if (!for_batch) {
     cfg.maxedges = (int)(captot / 2 > 65536 ? captot / 2 : 65536);
     cfg.maxevents = (int)(captot / 16 > 16384 ? captot / 16 : 16384);
-  } else {  // one of many: smaller event buffers
-    cfg.maxedges = (int)(captot / 2 > 4096 ? captot / 2 : 4096);
-    cfg.maxevents = (int)(captot / 16 > 1024 ? captot / 16 : 1024);
+  } else {  // one of many: event buffers in proportion to the member's size
+    cfg.maxedges = (int)(captot > 4096 ? captot : 4096);
+    cfg.maxevents = (int)(captot / 4 > 2048 ? captot / 4 : 2048);
   }
   // surfaces (surfupdate, smolsurface.c:3797-3893, and the box panel lists)
   SurfaceTables st;
@@ -805,7 +808,10 @@ ErrorCode engine_error(const char* fn, int bits) {
     set_error(fn, ECerror, "HIP runtime error");
     return ECerror;
   }
-  set_error(fn, ECerror, "device engine overflow (bits 0x%x: 1 edges, 2 events, 4 molecule capacity, 8 sort)", bits);
+  set_error(fn, ECerror,
+            "device engine error (bits 0x%x: 1 candidate pairs, 2 events, 4 molecule capacity, 8 sort space, "
+            "16 surface action, 32 stepping halted)",
+            bits);
   return ECerror;
 }
 

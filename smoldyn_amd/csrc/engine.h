@@ -97,6 +97,7 @@ enum ErrBits : int {
   kErrCapacity = 4,
   kErrSortOverflow = 8,
   kErrSurface = 16,  // a surface action the device path does not run (absorption, adsorption, list change)
+  kErrHalted = 32,   // stepping stopped after an error (later steps did nothing)
 };
 
 // ---------------------------------------------------------------------------

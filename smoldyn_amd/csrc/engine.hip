@@ -125,6 +125,8 @@ struct Inst {
 struct Glob {
   int nrec[2];
   int nprod[2];
+  int ninst;  // simulations in the batch
+  int halt;   // set on an inconsistency: every later step does nothing
 };
 
 struct DeviceState {
@@ -254,12 +256,15 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
                                                     const double* __restrict__ inj) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   int hkey = -1;  // histogram key (simulation's first key + list*nbox + box), -1 when not binned
+  if (gl->halt) return;  // uniform over the grid: no wave reaches the histogram below
   if (r < cur_nrec(insts, gl)) {
     int i = a.ident[r];
-    if (i == 0) {
+    int in = kMulti ? a.inst[r] : 0;
+    if (i <= 0 || i >= kMaxSpecies || in < 0 || in >= gl->ninst) {
+      if (i != 0) atomicOr((int*)&gl->halt, 1);
       a.box[r] = -1;
     } else {
-      const Inst& I = insts[kMulti ? a.inst[r] : 0];
+      const Inst& I = insts[in];
       const DevTables* tab = I.tab;
       Control* ctl = I.ctl;
       const Geometry& g = I.g;
@@ -420,7 +425,7 @@ template <bool kMulti>
 __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
                                                     Recs a, Recs b, Binned bn, int* __restrict__ inv) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= cur_nrec(insts, gl)) return;
+  if (gl->halt || r >= cur_nrec(insts, gl)) return;
   int bx = a.box[r];
   if (bx < 0) return;
   int in = kMulti ? a.inst[r] : 0;
@@ -451,11 +456,14 @@ template <bool kMulti>
 __global__ __launch_bounds__(kBlock) void k_rebuild_inv(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
                                                         Recs a, int* __restrict__ inv) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= cur_nrec(insts, gl)) return;
+  if (gl->halt || r >= cur_nrec(insts, gl)) return;
   int i = a.ident[r];
-  if (i == 0) return;
-  const Inst& I = insts[kMulti ? a.inst[r] : 0];
-  inv[I.L.base[I.tab->listof[i]] + a.lpos[r]] = r;
+  int in = kMulti ? a.inst[r] : 0;
+  if (i <= 0 || i >= kMaxSpecies || in < 0 || in >= gl->ninst) return;
+  const Inst& I = insts[in];
+  int ll = I.tab->listof[i];
+  int lp = a.lpos[r];
+  if (lp >= 0 && lp < I.L.base[ll + 1] - I.L.base[ll]) inv[I.L.base[ll] + lp] = r;
 }
 
 // ---------------------------------------------------------------------------
@@ -540,9 +548,10 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf,
 // an FP32 screen, then the reference's FP64 test; only then is their stencil
 // index looked up for the edge key.
 template <bool kSurf, bool kMulti>
-__global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, Recs b, Binned bn, int nscan) {
+__global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, const Glob* __restrict__ gl, Recs b,
+                                                  Binned bn, int nscan) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= bn.bstart[nscan - 1]) return;  // records binned
+  if (gl->halt || u >= bn.bstart[nscan - 1]) return;  // records binned
   const Inst& I = insts[kMulti ? b.inst[u] : 0];
   const Geometry& g = I.g;
   const DevTables* tab = I.tab;
@@ -825,6 +834,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
                                                              uint64_t* __restrict__ bestkey, int* __restrict__ killed,
                                                              int reccap) {
   const Inst& I = insts[blockIdx.x];
+  if (gl->halt) return;  // a simulation that starts after another one halted skips its step
   __shared__ uint64_t lk[kLdsSort];
   __shared__ int lv[kLdsSort];
   __shared__ int s_scan[kResolveThreads / 64];
@@ -897,8 +907,21 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   // ---- order the events: first-order reactions in live-list order, then
   // bimolecular ones in edge-key order
   if (U + pow2ceil(B2) > I.sortcap || pow2ceil(U + 2 * B2) > I.sortcap) {
-    if (tid == 0) atomicOr(&ctl->err, kErrSortOverflow);
-    return;
+    // more events than the sort space: flag the error, halt the later steps,
+    // and finish this step without its reactions so the bookkeeping below
+    // (record count, step counter) stays consistent
+    if (tid == 0) {
+      atomicOr(&ctl->err, kErrSortOverflow);
+      atomicOr(&gl->halt, 1);
+    }
+    for (int i = tid; i < B2; i += NT) {
+      Edge ed = I.edges[I.accepted[i]];
+      killed[ed.s1] = 0;
+      killed[ed.s2] = 0;
+    }
+    U = 0;
+    B2 = 0;
+    __syncthreads();
   }
   for (int i = tid; i < U; i += NT) {
     I.sk[i] = (uint64_t)I.uniev[i].slot;
@@ -1354,7 +1377,12 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
   (void)hipMemset(s->bestkey, 0xFF, sizeof(uint64_t) * rc);
   (void)hipMemset(s->killed, 0, sizeof(int) * rc);
   (void)hipMemset(s->ctl, 0, sizeof(Control) * n);
-  (void)hipMemset(s->glob, 0, sizeof(Glob));
+  {
+    Glob g0;
+    memset(&g0, 0, sizeof(g0));
+    g0.ninst = n;
+    (void)hipMemcpy(s->glob, &g0, sizeof(Glob), hipMemcpyHostToDevice);
+  }
   (void)hipMemcpy(s->tab, tab, sizeof(DevTables) * n, hipMemcpyHostToDevice);
   (void)hipMemcpy(s->gtab, gtable, sizeof(double) * ngauss, hipMemcpyHostToDevice);
   if (upload_instances(s)) {
@@ -1457,6 +1485,7 @@ int engine_set_counters_inst(DeviceState* s, int inst, uint64_t serno_next, uint
   Glob g;
   memset(&g, 0, sizeof(g));
   g.nrec[(step + 1) & 1] = s->hnrec;
+  g.ninst = s->ninst;
   HIPOK(hipMemcpy(s->glob, &g, sizeof(Glob), hipMemcpyHostToDevice));
   return 0;
 }
@@ -1488,7 +1517,7 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   hipLaunchKernelGGL(kscat, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, B, s->bn, s->inv);
   if (timed) (void)hipEventRecord(s->ev[3], st);
   if (timed) (void)hipEventRecord(s->ev[4], st);
-  hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->dinst, B, s->bn, s->nscan);
+  hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, B, s->bn, s->nscan);
   if (timed) (void)hipEventRecord(s->ev[5], st);
   hipLaunchKernelGGL(k_resolve, dim3(s->ninst), dim3(kResolveThreads), 0, st, s->dinst, s->glob, A, B, s->inv, s->bn,
                      s->nscan, s->bestkey, s->killed, s->reccap);
@@ -1498,9 +1527,12 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
 // OR of the error bits of all simulations
 static int read_errors(DeviceState* s) {
   std::vector<Control> c(s->ninst);
+  Glob g;
   if (hipMemcpy(c.data(), s->ctl, sizeof(Control) * s->ninst, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  if (hipMemcpy(&g, s->glob, sizeof(Glob), hipMemcpyDeviceToHost) != hipSuccess) return -1;
   int e = 0;
   for (auto& x : c) e |= x.err;
+  if (g.halt) e |= kErrHalted;
   return e;
 }
 

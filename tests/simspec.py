@@ -15,13 +15,19 @@ import numpy as np
 import oracle_bind as ob
 
 
-def build_oracle(spec):
+def build_oracle(spec, sfmt=False):
+    """sfmt: draw from the reference's own random stream (SFMT-19937 in
+    program order) instead of the counter stream the device uses."""
     sim = ob.OSim(3, spec["low"], spec["high"])
     for d, w in enumerate(spec["walls"]):
         sim.wall(d, 0, w)
         sim.wall(d, 1, w)
     sim.times(0.0, spec.get("tmax", 1e9), spec["dt"])
-    sim.rngmode(ob.RNG_COUNTER, spec["seed"])
+    if sfmt:
+        sim.rngmode(ob.RNG_SFMT, 0)
+        sim.seed(spec["seed"])
+    else:
+        sim.rngmode(ob.RNG_COUNTER, spec["seed"])
     idx = {}
     for name, d in spec["species"].items():
         idx[name] = sim.species(name)

@@ -223,3 +223,20 @@ def test_ensemble_against_oracle():
             d = ens.molecules(1)
             _same_lists(dict(ident=o["ident"], serno=o["serno"], pos=o["pos"], box=o["box"]), d,
                         "member 1 step %d" % step)
+
+
+def test_ensemble_event_overflow_reports_error():
+    """A member with far more reacting pairs in one step than its event space
+    (dense, large binding radius) makes the run return an error code; the
+    other steps do nothing and the process keeps running (regression test)."""
+    import smoldyn_amd as sa
+    spec = dict(low=[0.0] * 3, high=[12.0] * 3, walls="rrr", dt=0.01, seed=77,
+                species={"A": 1.0, "B": 1.0, "C": 0.5},
+                reactions=[("fwd", ["A", "B"], ["C"], 2000.0)],
+                molecules=[("A", 6000), ("B", 6000)])
+    small = ss.c4_member(0, seed=701, scale=0.03)
+    members = [ss.build_device(small, update=False), ss.build_device(spec, update=False)]
+    ens = sa.Ensemble(members)
+    with pytest.raises(sa.SmoldynError):
+        ens.run_steps(5)
+    assert ens.diagnostics(1)["err"] != 0
