@@ -245,6 +245,20 @@ def sweep_leg(members, steps, warmup, sim_steps=2000):
                           "max_rate_error_bisection": float(err_n.max())}}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by their reading on a plain copy),
+    or None when no summary covers it."""
+    for path in sorted((ROOT / "profiles").glob("*.md")):
+        for line in path.read_text().splitlines():
+            line = line.strip()
+            if line.startswith("TRAFFIC " + kernel + " "):
+                rec = json.loads(line.split(" ", 2)[2])
+                rec["source"] = "profiles/" + path.name
+                return rec
+    return None
+
+
 def cpu_baseline(spec, steps):
     """The CPU oracle (a restatement of the reference CPU path, 1 thread) on the
     same workload: setup excluded, `steps` steps timed."""
@@ -337,6 +351,7 @@ def main():
     achieved = ALGO_BYTES[dom] * nmol / (net[dom] * 1e-3) / 1e9 if net[dom] > 0 else 0.0
     step_bytes = sum(ALGO_BYTES.values()) * nmol
     counts = {s: sim.count(s) for s in ("A", "B", "C")}
+    tr = measured_traffic(HIP_KERNEL_SYMBOL[dom])
     out = {
         "metric": "molecule-steps/s at 1M molecules",
         "value": value,
@@ -360,7 +375,9 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "algo_bytes_per_molecule": ALGO_BYTES[dom], "molecules_per_launch": nmol,
-                     "traffic": None},
+                     "traffic": tr["bytes_per_launch"] if tr else None,
+                     "traffic_note": ("rocprofv3 FETCH_SIZE+WRITE_SIZE per launch at %d molecules, scaled by a "
+                                      "plain-copy calibration (%s)" % (tr["molecules"], tr["source"])) if tr else None},
         "step_algo_bytes": step_bytes,
         "step_frac_of_hbm": step_bytes / (dev_ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "final_counts": counts,
