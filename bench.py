@@ -55,7 +55,7 @@ def c2_spec(n_per_species=500000, side=1000.0, seed=2026, kback=1.0):
                 molecules=[("A", n_per_species), ("B", n_per_species)])
 
 
-def build_device(spec):
+def build_device(spec, precision=64):
     import smoldyn_amd as sa
     sim = sa.Simulation(spec["low"], spec["high"])
     for d, w in enumerate(spec["walls"]):
@@ -63,6 +63,7 @@ def build_device(spec):
         sim.boundary(d, 1, w)
     sim.times(0.0, 1e30, spec["dt"])
     sim.seed(spec["seed"])
+    sim.precision(precision)
     for name, d in spec["species"].items():
         sim.species(name)
         sim.mobility(name, d)
@@ -245,6 +246,63 @@ def sweep_leg(members, steps, warmup, sim_steps=2000):
                           "max_rate_error_bisection": float(err_n.max())}}
 
 
+def precision_leg(spec, steps, warmup, check_steps=100):
+    """FP32 box-offset records on the same workload: throughput and per-kernel
+    times, then the error against FP64 records from the same start: the max
+    position difference after one step with the same injected deviates
+    (bound: two roundings of an offset, 2 h 2^-24 for box size h), and the
+    reaction events of `check_steps` steps with the same Philox streams."""
+    sim = build_device(spec, 32)
+    sim.run_steps(warmup)
+    d0 = sim.diagnostics()
+    t0 = time.perf_counter()
+    sim.run_steps(steps)
+    wall = time.perf_counter() - t0
+    d1 = sim.diagnostics()
+    if d1["err"]:
+        raise SystemExit("device engine error bits 0x%x (FP32 records)" % d1["err"])
+    kt = sim.profile_steps(100)
+    times = dict(zip(KERNEL_NAMES, [float(x) for x in kt]))
+    net = {k: max(v - times["event_overhead"], 0.0) for k, v in times.items() if k != "event_overhead"}
+    value = (d1["molsteps"] - d0["molsteps"]) / wall
+    del sim
+    s64, s32 = build_device(spec, 64), build_device(spec, 32)
+    _, _, _, bsize = s64.boxes()
+    h = float(np.max(bsize))
+    n = sum(len(s64.molecules(ll)["ident"]) for ll in range(s64.nlists()))
+    g = np.random.default_rng(5).standard_normal((n, 3))
+    s64.inject_step(g)
+    s32.inject_step(g)
+    err1 = 0.0
+    for ll in range(s64.nlists()):
+        a, b = s64.molecules(ll), s32.molecules(ll)
+        if len(a["ident"]) and np.array_equal(a["serno"], b["serno"]):
+            err1 = max(err1, float(np.abs(a["pos"] - b["pos"]).max()))
+        elif len(a["ident"]):
+            err1 = float("inf")
+    same, diverged = 0, False
+    tot = np.zeros((2, 2))
+    for _ in range(check_steps):
+        s64.run_time_step()
+        s32.run_time_step()
+        l64, l32 = s64.reaction_log(), s32.reaction_log()
+        if not diverged and sorted(map(tuple, l64[:, 2:].tolist())) == sorted(map(tuple, l32[:, 2:].tolist())):
+            same += 1
+        else:
+            diverged = True
+        for j, lg in enumerate((l64, l32)):
+            tot[j, 0] += np.sum(lg[:, 0] == 2)
+            tot[j, 1] += np.sum(lg[:, 0] == 1)
+    rate_err = [float(abs(tot[1, c] - tot[0, c]) / tot[0, c]) if tot[0, c] else None for c in range(2)]
+    return {"value": value, "unit": "molecule-steps/s", "ms_per_step": wall * 1e3 / steps, "kernel_ms": net,
+            "speedup_vs_fp64": None, "box_size": h,
+            "max_position_error_one_step": err1, "position_error_bound_one_step": 2 * h * 2.0 ** -24,
+            "identical_event_steps": same, "checked_steps": check_steps,
+            "events_fp64": {"forward": int(tot[0, 0]), "reverse": int(tot[0, 1])},
+            "events_fp32": {"forward": int(tot[1, 0]), "reverse": int(tot[1, 1])},
+            "rate_rel_diff": {"forward": rate_err[0], "reverse": rate_err[1]}}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE runs scaled by their reading on a plain copy),
@@ -305,6 +363,8 @@ def main():
                     help="members of the batched parameter sweep (0: skip the ensemble leg)")
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--precision-steps", type=int, default=2000,
+                    help="timed steps of the FP32 box-offset leg (0: skip)")
     ap.add_argument("--sweep-members", type=int, default=4096,
                     help="members of the learned-surrogate sweep, 1e4 molecules each (0: skip)")
     ap.add_argument("--sweep-steps", type=int, default=100)
@@ -383,6 +443,10 @@ def main():
         "final_counts": counts,
         "setup_s": setup_s,
     }
+    if args.precision_steps > 0:
+        log("FP32 box-offset records, %d steps" % args.precision_steps)
+        out["fp32_offsets"] = precision_leg(spec, args.precision_steps, args.warmup)
+        out["fp32_offsets"]["speedup_vs_fp64"] = out["fp32_offsets"]["value"] / value
     if args.ensemble_members > 0:
         del sim
         log("batched ensemble leg, %d members" % args.ensemble_members)

@@ -141,6 +141,13 @@ enum ErrorCode smolamdAddMoleculesAt(simptr sim, const char* species, int number
    order (lists in order, live order within a list).  Like swapping
    sim->diffusefn in the reference (smolsim.cpp:325). */
 enum ErrorCode smolamdInjectDisplacements(simptr sim, const double* gauss, long nmol);
+/* Record precision of the device path, before smolUpdateSim: 64 (FP64
+   positions, bit-exact with the reference's arithmetic) or 32 (each position
+   stored as FP32 offsets from the FP64 origin of its virtual box; diffusion,
+   walls, surfaces, distance tests and product placement stay FP64).  No
+   counterpart in the reference, which stores FP64 positions
+   (smoldyn.h:280-293). */
+enum ErrorCode smolamdSetPrecision(simptr sim, int bits);
 /* Molecules of list `list` in live order; returns the count (or <0). Any of
    the output pointers may be NULL. pos is n*3 doubles. box is the virtual box
    each molecule was assigned to in the last step. */

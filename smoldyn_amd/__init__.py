@@ -98,6 +98,7 @@ def lib():
         "smolSetReactionProducts": (I, [P, cp, I, D, cp, dp]),
         "smolamdAddMoleculesAt": (I, [P, cp, I, dp]),
         "smolamdInjectDisplacements": (I, [P, dp, L64]),
+        "smolamdSetPrecision": (I, [P, I]),
         "smolamdGetMolecules": (I, [P, I, ip, dp, C.POINTER(C.c_ulonglong), ip, I]),
         "smolamdGetListCount": (I, [P]),
         "smolamdGetReactionLog": (I, [P, C.POINTER(C.c_longlong), I]),
@@ -201,6 +202,11 @@ class Simulation:
 
     def seed(self, s):
         _check(self.L.smolSetRandomSeed(self.p, s), "smolSetRandomSeed")
+
+    def precision(self, bits):
+        """Record precision: 64 (FP64 positions) or 32 (FP32 offsets from the
+        FP64 origin of each molecule's virtual box, FP64 arithmetic)."""
+        _check(self.L.smolamdSetPrecision(self.p, bits), "smolamdSetPrecision")
 
     def partitions(self, method, value):
         _check(self.L.smolSetPartitions(self.p, method.encode(), value), "smolSetPartitions")

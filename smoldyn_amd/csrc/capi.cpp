@@ -75,6 +75,7 @@ struct smolamd_sim {
   uint64_t key = 0;
   double mpbox = -1, boxsize = -1, accur = 10;
   int maxmol = -1;
+  int precision = 64;  // record precision: 64 (FP64 positions) or 32 (FP32 box offsets)
   std::vector<std::string> spname{"empty"};
   std::vector<std::array<double, kMS>> difc{{0, 0, 0, 0, 0}};
   std::vector<std::array<int, kMS>> listlookup{{-1, -1, -1, -1, -1}};
@@ -718,6 +719,7 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   long ntot = (long)sim->pending.size();
   EngineConfig cfg;
   memset(&cfg, 0, sizeof(cfg));
+  cfg.f32 = sim->precision == 32;
   cfg.nlist = nlist;
   // a list can only hold molecules placed at setup or created by reactions;
   // lists that receive neither keep a token capacity
@@ -1029,6 +1031,20 @@ enum ErrorCode smolSetTimeStep(simptr sim, double timestep) {
   LCHECK(timestep > 0, fn, ECbounds, "timestep must be >0");
   LCHECK(!sim->started, fn, ECerror, "the time step is fixed once stepping started");
   sim->dt = timestep;
+  invalidate(sim);
+  return ECok;
+}
+
+// Record precision of the device path (no counterpart in the reference):
+// 64 keeps FP64 positions; 32 stores each position as FP32 offsets from the
+// FP64 origin of its virtual box, with all arithmetic (diffusion, walls,
+// surfaces, distance tests, product placement) still in FP64.
+enum ErrorCode smolamdSetPrecision(simptr sim, int bits) {
+  const char* fn = "smolamdSetPrecision";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(bits == 32 || bits == 64, fn, ECbounds, "precision must be 32 or 64");
+  LCHECK(!sim->started, fn, ECerror, "the precision is fixed once stepping started");
+  sim->precision = bits;
   invalidate(sim);
   return ECok;
 }

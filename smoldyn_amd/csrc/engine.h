@@ -150,6 +150,7 @@ struct UniEvent {
   int rec;   // record index in the source buffer of the step
   int slot;  // list base + live position
   int r;
+  int box;   // virtual box the record's position is stored relative to (FP32-offset records)
 };
 
 // One reaction event in creation order (for the reaction log)
@@ -167,6 +168,9 @@ struct EngineConfig {
   int maxedges;
   int maxevents;
   double expected_per_box;  // molecules per virtual box per list, for the pair-search tile
+  // record precision: 0 = FP64 positions; 1 = FP32 offsets from the FP64
+  // origin of the record's virtual box (all arithmetic stays FP64)
+  int f32;
 };
 
 // Host description of the surfaces handed to engine_create (arrays copied).

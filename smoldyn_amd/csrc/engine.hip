@@ -66,6 +66,7 @@ struct Layout {
 // One buffer of molecule records.
 struct Recs {
   double *px, *py, *pz;
+  float *ox, *oy, *oz;  // FP32-offset records: position - origin of `box` (null in FP64 mode)
   int* inst;      // simulation instance (ensembles only; null for one simulation)
   int* ident;     // species, 0 for a record whose molecule died
   uint64_t* serno;
@@ -132,6 +133,7 @@ struct Glob {
 struct DeviceState {
   int ninst = 0;
   bool multi = false;                // more than one simulation: records carry their instance
+  bool f32 = false;                  // FP32-offset records
   std::vector<EngineConfig> cfg;     // per simulation
   std::vector<DevTables> htab;       // host copies of the simulation tables
   std::vector<Inst> hinst;           // host copy of the instance views
@@ -192,6 +194,47 @@ __device__ __forceinline__ int pos2box(const Geometry& g, double x, double y, do
   return b;
 }
 
+// FP64 origin of virtual box b (the corner pos2box measures from); host and
+// device evaluate the same expression, so FP32-offset records decode alike.
+__host__ __device__ __forceinline__ void box_origin(const Geometry& g, int b, double* o) {
+  const int iz = b % g.side[2], t = b / g.side[2];
+  const int iy = t % g.side[1], ix = t / g.side[1];
+  o[0] = g.bmin[0] + (double)ix * g.bsize[0];
+  o[1] = g.bmin[1] + (double)iy * g.bsize[1];
+  o[2] = g.bmin[2] + (double)iz * g.bsize[2];
+}
+
+// Record position: FP64 arrays, or FP32 offsets from the origin of box `bx`.
+template <bool kF32>
+__device__ __forceinline__ void load_pos(const Recs& a, const Geometry& g, int r, int bx, double* p) {
+  if (kF32) {
+    double o[3];
+    box_origin(g, bx, o);
+    p[0] = o[0] + (double)a.ox[r];
+    p[1] = o[1] + (double)a.oy[r];
+    p[2] = o[2] + (double)a.oz[r];
+  } else {
+    p[0] = a.px[r];
+    p[1] = a.py[r];
+    p[2] = a.pz[r];
+  }
+}
+
+template <bool kF32>
+__device__ __forceinline__ void store_pos(const Recs& a, const Geometry& g, int r, int bx, const double* p) {
+  if (kF32) {
+    double o[3];
+    box_origin(g, bx, o);
+    a.ox[r] = (float)(p[0] - o[0]);
+    a.oy[r] = (float)(p[1] - o[1]);
+    a.oz[r] = (float)(p[2] - o[2]);
+  } else {
+    a.px[r] = p[0];
+    a.py[r] = p[1];
+    a.pz[r] = p[2];
+  }
+}
+
 }  // namespace smolamd
 #include "surface_dev.h"
 namespace smolamd {
@@ -249,7 +292,7 @@ __device__ __forceinline__ int cur_nrec(const Inst* insts, const Glob* gl) {
   return gl->nrec[(insts[0].ctl->step + 1) & 1];
 }
 
-template <bool kSurf, bool kMulti>
+template <bool kSurf, bool kMulti, bool kF32>
 __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
                                                     Recs a, int* __restrict__ bcount,
                                                     const double* __restrict__ gtab, int gmask,
@@ -272,8 +315,11 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       int lp = a.lpos[r];
       uint64_t ser = a.serno[r];
       uint32_t step = ctl->step;
-      double pos[3] = {a.px[r], a.py[r], a.pz[r]};
-      if (tab->diffuselist[ll]) {
+      const int bold = kF32 ? a.box[r] : 0;
+      double pos[3];
+      load_pos<kF32>(a, g, r, bold, pos);
+      const bool moved = tab->diffuselist[ll];
+      if (moved) {
         const double posx[3] = {pos[0], pos[1], pos[2]};
         double ds = tab->difstep[i];
         double gg[3];
@@ -306,11 +352,10 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
             i = i2;
           }
         }
-        a.px[r] = pos[0];
-        a.py[r] = pos[1];
-        a.pz[r] = pos[2];
       }
       int b = pos2box(g, pos[0], pos[1], pos[2]);
+      // FP32-offset records are stored relative to the new box
+      if (moved || (kF32 && b != bold)) store_pos<kF32>(a, g, r, b, pos);
       hkey = I.boxbase + ll * g.nbox + b;
       // unireact: coin flips in table order, first success reacts; a
       // molecule that reacts takes no part in the pair search, so it is not
@@ -333,6 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
               I.uniev[e].rec = r;
               I.uniev[e].slot = I.L.base[ll] + lp;
               I.uniev[e].r = rx;
+              I.uniev[e].box = b;
             } else {
               atomicOr(&ctl->err, kErrEventOverflow);
             }
@@ -421,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ part, 
 // The inverse map (list slot -> record) is written only for the last
 // `window` slots of each list: compaction in k_resolve reads tail slots only,
 // and the kills of one step never exceed the window.
-template <bool kMulti>
+template <bool kMulti, bool kF32>
 __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
                                                     Recs a, Recs b, Binned bn, int* __restrict__ inv) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -433,11 +479,26 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ ins
   int i = a.ident[r];
   int ll = I.tab->listof[i];
   int dst = bn.bstart[I.boxbase + ll * I.g.nbox + bx] + a.rank[r];
-  double x = a.px[r], y = a.py[r], z = a.pz[r];
+  double x, y, z;
+  if (kF32) {
+    const float fx = a.ox[r], fy = a.oy[r], fz = a.oz[r];
+    b.ox[dst] = fx;
+    b.oy[dst] = fy;
+    b.oz[dst] = fz;
+    double o[3];
+    box_origin(I.g, bx, o);
+    x = o[0] + (double)fx;
+    y = o[1] + (double)fy;
+    z = o[2] + (double)fz;
+  } else {
+    x = a.px[r];
+    y = a.py[r];
+    z = a.pz[r];
+    b.px[dst] = x;
+    b.py[dst] = y;
+    b.pz[dst] = z;
+  }
   int lp = a.lpos[r];
-  b.px[dst] = x;
-  b.py[dst] = y;
-  b.pz[dst] = z;
   if (kMulti) b.inst[dst] = in;
   b.ident[dst] = i;
   b.serno[dst] = a.serno[r];
@@ -512,13 +573,14 @@ __device__ __forceinline__ int first_reaction(const DevTables* tab, int i1, int 
 // Exact FP64 test of one candidate (the reference's dist2 in its operation
 // order, smolreact.c:3238-3240, and wallcalcdist2, smolwall.c:100-116) and
 // the edge for the first reaction that accepts it.
-template <bool kSurf>
+template <bool kSurf, bool kF32>
 __device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf, const DevTables* tab, Control* ctl,
                                            const Recs& b,
                                            Edge* edges, int maxedges, uint32_t key0, uint32_t key1, uint32_t step,
                                            int t1, const double* p1, int i1, uint64_t ser1, int t2, int i2, int wrap,
                                            uint64_t key) {
-  double p2[3] = {b.px[t2], b.py[t2], b.pz[t2]};
+  double p2[3];
+  load_pos<kF32>(b, g, t2, kF32 ? b.box[t2] : 0, p2);
   double dist2 = 0;
   if (wrap) {
 #pragma unroll
@@ -547,7 +609,7 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf,
 // (0,0) is the molecule's own box, bireact's same-box pass.  Candidates pass
 // an FP32 screen, then the reference's FP64 test; only then is their stencil
 // index looked up for the edge key.
-template <bool kSurf, bool kMulti>
+template <bool kSurf, bool kMulti, bool kF32>
 __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, const Glob* __restrict__ gl, Recs b,
                                                   Binned bn, int nscan) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -661,15 +723,13 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
             const int lp2 = b.lpos[k];
             if (nbi == 13 && half && lp2 >= lp1) continue;
             if (!loaded) {
-              p1[0] = b.px[u];
-              p1[1] = b.py[u];
-              p1[2] = b.pz[u];
+              load_pos<kF32>(b, g, u, bb, p1);
               ser1 = b.serno[u];
               loaded = true;
             }
             const uint64_t key = nbi == 13 ? (key1base | (uint64_t)lp2)
                                            : ((1ull << 63) | key1base | ((uint64_t)nbi << 27) | (uint64_t)lp2);
-            exact_pair<kSurf>(g, I.sf, tab, ctl, b, I.edges, I.maxedges, I.key0, I.key1, step, u, p1, i1, ser1, k, i2,
+            exact_pair<kSurf, kF32>(g, I.sf, tab, ctl, b, I.edges, I.maxedges, I.key0, I.key1, step, u, p1, i1, ser1, k, i2,
                        nbi == 13 ? 0 : wrap, key);
           }
         }
@@ -829,6 +889,7 @@ struct KillView {
 
 
 // One workgroup per simulation of the batch (blockIdx.x).
+template <bool kF32>
 __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restrict__ insts, Glob* gl, Recs RA,
                                                              Recs RB, int* __restrict__ inv, Binned bn, int nscan,
                                                              uint64_t* __restrict__ bestkey, int* __restrict__ killed,
@@ -967,9 +1028,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       rx = &tab->rxn1[ev.r];
       ridx = ev.r;
       order = 1;
-      rxnpos[0] = RA.px[ev.rec];
-      rxnpos[1] = RA.py[ev.rec];
-      rxnpos[2] = RA.pz[ev.rec];
+      load_pos<kF32>(RA, I.g, ev.rec, ev.box, rxnpos);
       sA = RA.serno[ev.rec];
       I.killslot[e] = ev.slot;
     } else {
@@ -977,8 +1036,9 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       rx = &tab->rxn2[ed.r];
       ridx = ed.r;
       order = 2;
-      double p1[3] = {B.px[ed.s1], B.py[ed.s1], B.pz[ed.s1]};
-      double p2[3] = {B.px[ed.s2], B.py[ed.s2], B.pz[ed.s2]};
+      double p1[3], p2[3];
+      load_pos<kF32>(B, I.g, ed.s1, kF32 ? B.box[ed.s1] : 0, p1);
+      load_pos<kF32>(B, I.g, ed.s2, kF32 ? B.box[ed.s2] : 0, p2);
       int id1 = B.ident[ed.s1], id2 = B.ident[ed.s2];
       if (ed.wrap && !rx->confspread) {  // morebireact: move the faster molecule next to its partner
         double v[3];
@@ -1152,14 +1212,14 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       int rec = nbinned + s_prodoff + c;
       if (slot < I.L.base[ll + 1] && rec < reccap) {
         Prod pr = I.prods[c];
-        B.px[rec] = pr.x;
-        B.py[rec] = pr.y;
-        B.pz[rec] = pr.z;
+        const int pbx = pos2box(I.g, pr.x, pr.y, pr.z);
+        const double pp[3] = {pr.x, pr.y, pr.z};
+        store_pos<kF32>(B, I.g, rec, pbx, pp);
         B.ident[rec] = pr.ident;
         if (B.inst) B.inst[rec] = blockIdx.x;  // batch member (multi-simulation records)
         B.serno[rec] = ser0 + (uint64_t)c;
         B.lpos[rec] = slot - I.L.base[ll];
-        B.box[rec] = pos2box(I.g, pr.x, pr.y, pr.z);
+        B.box[rec] = pbx;
         inv[slot] = rec;
       } else {
         overflow = 1;
@@ -1224,9 +1284,10 @@ static int dalloc(T** p, size_t n) {
   return 0;
 }
 
-static int alloc_recs(Recs* r, size_t n, bool multi) {
+static int alloc_recs(Recs* r, size_t n, bool multi, bool f32) {
   int ok = 0;
-  ok |= dalloc(&r->px, n) | dalloc(&r->py, n) | dalloc(&r->pz, n);
+  if (f32) ok |= dalloc(&r->ox, n) | dalloc(&r->oy, n) | dalloc(&r->oz, n);
+  else ok |= dalloc(&r->px, n) | dalloc(&r->py, n) | dalloc(&r->pz, n);
   ok |= dalloc(&r->ident, n) | dalloc(&r->serno, n) | dalloc(&r->lpos, n) | dalloc(&r->box, n) |
         dalloc(&r->rank, n);
   if (multi) ok |= dalloc(&r->inst, n);
@@ -1234,7 +1295,7 @@ static int alloc_recs(Recs* r, size_t n, bool multi) {
 }
 
 static void free_recs(Recs* r) {
-  void* ptrs[] = {r->px, r->py, r->pz, r->inst, r->ident, r->serno, r->lpos, r->box, r->rank};
+  void* ptrs[] = {r->px, r->py, r->pz, r->ox, r->oy, r->oz, r->inst, r->ident, r->serno, r->lpos, r->box, r->rank};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1250,7 +1311,14 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
   DeviceState* s = new DeviceState();
   s->ninst = n;
   s->multi = n > 1;
+  s->f32 = cfg[0].f32 != 0;
   s->ngauss = ngauss;
+  for (int k = 1; k < n; k++)
+    if ((cfg[k].f32 != 0) != s->f32) {
+      snprintf(err, errlen, "ensemble members must use the same record precision");
+      delete s;
+      return nullptr;
+    }
   s->cfg.assign(cfg, cfg + n);
   s->htab.assign(tab, tab + n);
   s->hinst.assign(n, Inst());
@@ -1321,7 +1389,7 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
   s->reccap = run + (int)prodroom;
   size_t rc = (size_t)s->reccap;
   int ok = 0;
-  ok |= alloc_recs(&s->buf[0], rc, s->multi) | alloc_recs(&s->buf[1], rc, s->multi);
+  ok |= alloc_recs(&s->buf[0], rc, s->multi, s->f32) | alloc_recs(&s->buf[1], rc, s->multi, s->f32);
   ok |= dalloc(&s->inv, (size_t)s->captot);
   ok |= dalloc(&s->bn.count, s->nscan) | dalloc(&s->bn.part, s->nscan) | dalloc(&s->bn.boff, s->nscanblk) |
         dalloc(&s->bn.bstart, s->nscan) | dalloc(&s->bn.spk, rc);
@@ -1448,9 +1516,33 @@ int engine_upload_list_inst(DeviceState* s, int inst, int ll, int n, const int* 
       lp[k] = k;
       inv[k] = r0 + k;
     }
-    HIPOK(hipMemcpy(R.px + r0, x.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(R.py + r0, y.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(R.pz + r0, z.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    if (s->f32) {
+      // FP32 offsets from the origin of each molecule's virtual box
+      std::vector<float> fx(n), fy(n), fz(n);
+      const Geometry& g = I.g;
+      for (int k = 0; k < n; k++) {
+        int b = 0;
+        for (int d = 0; d < 3; d++) {
+          int indx = (int)((pos[3 * k + d] - g.bmin[d]) / g.bsize[d]);
+          if (indx < 0) indx = 0;
+          else if (indx >= g.side[d]) indx = g.side[d] - 1;
+          b = g.side[d] * b + indx;
+        }
+        double o[3];
+        box_origin(g, b, o);
+        box[k] = b;
+        fx[k] = (float)(x[k] - o[0]);
+        fy[k] = (float)(y[k] - o[1]);
+        fz[k] = (float)(z[k] - o[2]);
+      }
+      HIPOK(hipMemcpy(R.ox + r0, fx.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+      HIPOK(hipMemcpy(R.oy + r0, fy.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+      HIPOK(hipMemcpy(R.oz + r0, fz.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    } else {
+      HIPOK(hipMemcpy(R.px + r0, x.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      HIPOK(hipMemcpy(R.py + r0, y.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      HIPOK(hipMemcpy(R.pz + r0, z.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
     HIPOK(hipMemcpy(R.ident + r0, ident, sizeof(int) * n, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(R.serno + r0, serno, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(R.lpos + r0, lp.data(), sizeof(int) * n, hipMemcpyHostToDevice));
@@ -1501,9 +1593,16 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   int nb = (s->reccap + kBlock - 1) / kBlock;
   hipStream_t st = s->stream;
   Recs A = s->buf[src], B = s->buf[1 - src];
-  auto kdiff = s->multi ? k_diffuse<false, true> : (s->surf_on ? k_diffuse<true, false> : k_diffuse<false, false>);
-  auto kscat = s->multi ? k_scatter<true> : k_scatter<false>;
-  auto kpair = s->multi ? k_pairs<false, true> : (s->surf_on ? k_pairs<true, false> : k_pairs<false, false>);
+  const bool f = s->f32;
+  auto kdiff = s->multi ? (f ? k_diffuse<false, true, true> : k_diffuse<false, true, false>)
+               : s->surf_on ? (f ? k_diffuse<true, false, true> : k_diffuse<true, false, false>)
+                            : (f ? k_diffuse<false, false, true> : k_diffuse<false, false, false>);
+  auto kscat = s->multi ? (f ? k_scatter<true, true> : k_scatter<true, false>)
+                        : (f ? k_scatter<false, true> : k_scatter<false, false>);
+  auto kpair = s->multi ? (f ? k_pairs<false, true, true> : k_pairs<false, true, false>)
+               : s->surf_on ? (f ? k_pairs<true, false, true> : k_pairs<true, false, false>)
+                            : (f ? k_pairs<false, false, true> : k_pairs<false, false, false>);
+  auto kres = f ? k_resolve<true> : k_resolve<false>;
   if (timed) (void)hipEventRecord(s->ev[0], st);
   hipLaunchKernelGGL(kdiff, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, s->bn.count, s->gtab,
                      s->ngauss - 1, dinj);
@@ -1519,7 +1618,7 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   if (timed) (void)hipEventRecord(s->ev[4], st);
   hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, B, s->bn, s->nscan);
   if (timed) (void)hipEventRecord(s->ev[5], st);
-  hipLaunchKernelGGL(k_resolve, dim3(s->ninst), dim3(kResolveThreads), 0, st, s->dinst, s->glob, A, B, s->inv, s->bn,
+  hipLaunchKernelGGL(kres, dim3(s->ninst), dim3(kResolveThreads), 0, st, s->dinst, s->glob, A, B, s->inv, s->bn,
                      s->nscan, s->bestkey, s->killed, s->reccap);
   if (timed) (void)hipEventRecord(s->ev[6], st);
 }
@@ -1649,11 +1748,26 @@ int engine_get_list_inst(DeviceState* s, int inst, int ll, int* ident, double* p
   std::vector<double> x(nr), y(nr), z(nr);
   std::vector<int> id(nr), bx(nr);
   std::vector<uint64_t> sr(nr);
-  HIPOK(hipMemcpy(x.data(), R.px, sizeof(double) * nr, hipMemcpyDeviceToHost));
-  HIPOK(hipMemcpy(y.data(), R.py, sizeof(double) * nr, hipMemcpyDeviceToHost));
-  HIPOK(hipMemcpy(z.data(), R.pz, sizeof(double) * nr, hipMemcpyDeviceToHost));
   HIPOK(hipMemcpy(id.data(), R.ident, sizeof(int) * nr, hipMemcpyDeviceToHost));
   HIPOK(hipMemcpy(bx.data(), R.box, sizeof(int) * nr, hipMemcpyDeviceToHost));
+  if (s->f32) {
+    std::vector<float> fx(nr), fy(nr), fz(nr);
+    HIPOK(hipMemcpy(fx.data(), R.ox, sizeof(float) * nr, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(fy.data(), R.oy, sizeof(float) * nr, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(fz.data(), R.oz, sizeof(float) * nr, hipMemcpyDeviceToHost));
+    for (int r = 0; r < nr; r++) {
+      if (bx[r] < 0 || bx[r] >= I.g.nbox) continue;  // not a live record
+      double o[3];
+      box_origin(I.g, bx[r], o);
+      x[r] = o[0] + (double)fx[r];
+      y[r] = o[1] + (double)fy[r];
+      z[r] = o[2] + (double)fz[r];
+    }
+  } else {
+    HIPOK(hipMemcpy(x.data(), R.px, sizeof(double) * nr, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(y.data(), R.py, sizeof(double) * nr, hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(z.data(), R.pz, sizeof(double) * nr, hipMemcpyDeviceToHost));
+  }
   HIPOK(hipMemcpy(sr.data(), R.serno, sizeof(uint64_t) * nr, hipMemcpyDeviceToHost));
   for (int k = 0; k < n; k++) {
     int r = inv[k];

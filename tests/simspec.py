@@ -76,6 +76,8 @@ def build_device(spec, update=True):
         sim.boundary(d, 1, w)
     sim.times(0.0, spec.get("tmax", 1e9), spec["dt"])
     sim.seed(spec["seed"])
+    if "precision" in spec:
+        sim.precision(spec["precision"])
     for lname in spec.get("lists", {}):
         sim.add_list(lname)
     for name, d in spec["species"].items():
