@@ -43,7 +43,7 @@ def log(msg):
 #              neighboring boxes' records are re-read from L2, not HBM) = 24
 ALGO_BYTES = {"diffuse": 72, "scatter": 112, "pairs": 24}
 KERNEL_NAMES = ["diffuse", "scan", "scatter", "event_overhead", "pairs", "resolve"]
-HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_scan1+k_scan2+k_scan3", "scatter": "k_scatter",
+HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_scan1+k_scan23", "scatter": "k_scatter",
                      "pairs": "k_pairs", "resolve": "k_resolve"}
 
 
@@ -96,6 +96,87 @@ def c4_spec(k, seed=500, dt=0.01):
                 reactions=[("fwd", ["A", "B"], ["C"], rate), ("back", ["C"], ["A", "B"], 1.0)],
                 placement={"back": ("ratio", 1.5)},
                 molecules=[("A", n // 2), ("B", n - n // 2)])
+
+
+def c3_surfaces_spec(n_per_species=250000, side=500.0, radius=200.0, ndisk=64, rdisk=10.0, seed=31):
+    """Surfaces workload: A and B uniform in a reflecting sphere of radius 200
+    centred in a cube of side 500; a membrane plane through the centre
+    (z = 250) that transmits A and reflects B and C; 64 disks of radius 10 at
+    random non-overlapping centres on it, where A crossing either face turns
+    into D with probability 0.2 (rate_internal); A+B->C with rate 21.42267826
+    (binding radius 0.5), D_A = D_B = 2, D_C = 1, D_D = 2, dt 0.002."""
+    rng = np.random.default_rng(seed)
+    c = side / 2
+    centres = []
+    while len(centres) < ndisk:
+        ang, rad = rng.uniform(0, 2 * np.pi), radius * np.sqrt(rng.uniform(0, 0.8))
+        p = (c + rad * np.cos(ang), c + rad * np.sin(ang))
+        if all((p[0] - q[0]) ** 2 + (p[1] - q[1]) ** 2 > (2 * rdisk) ** 2 for q in centres):
+            centres.append(p)
+    disks = [("disk", [x, y, c, rdisk, 0, 0, 1, 20], "d%d" % k) for k, (x, y) in enumerate(centres)]
+
+    def in_sphere(n):
+        v = rng.standard_normal((n, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        return c + v * (radius * 0.999 * rng.uniform(0, 1, n) ** (1 / 3))[:, None]
+
+    return dict(low=[0.0] * 3, high=[side] * 3, walls="rrr", dt=0.002, seed=seed,
+                species={"A": 2.0, "B": 2.0, "C": 1.0, "D": 2.0},
+                reactions=[("fwd", ["A", "B"], ["C"], 21.42267826)], placement={},
+                surfaces=[("ball", [("both", "all", "reflect")], [], [("sph", None, [c, c, c, radius, 20, 20], "s0")]),
+                          ("membrane", [("both", "B", "reflect"), ("both", "C", "reflect")], [],
+                           [("rect", "+2", [0.0, 0.0, c, side, side], "plane")]),
+                          ("patches", [("both", "B", "reflect"), ("both", "C", "reflect")],
+                           [("A", "soln", "fsoln", "bsoln", 0.2, True, "D"),
+                            ("A", "soln", "bsoln", "fsoln", 0.2, True, "D")],
+                           [(sh, None, prm, nm) for sh, prm, nm in disks])],
+                molecules_at=[("A", in_sphere(n_per_species)), ("B", in_sphere(n_per_species))])
+
+
+def surfaces_leg(steps, warmup):
+    """The surfaces workload on the device: throughput and kernel times."""
+    import smoldyn_amd as sa
+    spec = c3_surfaces_spec()
+    sim = sa.Simulation(spec["low"], spec["high"])
+    for d, w in enumerate(spec["walls"]):
+        sim.boundary(d, 0, w)
+        sim.boundary(d, 1, w)
+    sim.times(0.0, 1e30, spec["dt"])
+    sim.seed(spec["seed"])
+    for name, d in spec["species"].items():
+        sim.species(name)
+        sim.mobility(name, d)
+    for name, rct, prd, rate in spec["reactions"]:
+        sim.reaction(name, rct, prd, rate)
+    for name, actions, rates, panels in spec["surfaces"]:
+        sim.add_surface(name)
+        for face, sp, act in actions:
+            sim.surface_action(name, face, sp, act)
+        for sp, st, st1, st2, value, internal, newsp in rates:
+            sim.surface_rate(name, sp, st, st1, st2, value, newsp, internal)
+        for shape, axis, prm, pname in panels:
+            sim.add_panel(name, shape, prm, axis, pname)
+    for sp, pos in spec["molecules_at"]:
+        sim.add_molecules_at(sp, pos)
+    t0 = time.perf_counter()
+    sim.update()
+    setup = time.perf_counter() - t0
+    sim.run_steps(warmup)
+    d0 = sim.diagnostics()
+    t0 = time.perf_counter()
+    sim.run_steps(steps)
+    wall = time.perf_counter() - t0
+    d1 = sim.diagnostics()
+    if d1["err"]:
+        raise SystemExit("device engine error bits 0x%x (surfaces workload)" % d1["err"])
+    kt = sim.profile_steps(100)
+    times = dict(zip(KERNEL_NAMES, [float(x) for x in kt]))
+    net = {k: max(v - times["event_overhead"], 0.0) for k, v in times.items() if k != "event_overhead"}
+    return {"workload": "500k molecules (250k A + 250k B) in a reflecting sphere r=200, transmitting membrane "
+                        "with 64 disk patches (A->D with probability 0.2 per crossing), A+B->C a=0.5, dt 0.002",
+            "value": (d1["molsteps"] - d0["molsteps"]) / wall, "unit": "molecule-steps/s",
+            "ms_per_step": wall * 1e3 / steps, "timed_steps": steps, "kernel_ms": net, "setup_s": setup,
+            "counts": {s: sim.count(s) for s in spec["species"]}}
 
 
 def build_member(spec, molperbox=None, update=False):
@@ -363,6 +444,8 @@ def main():
                     help="members of the batched parameter sweep (0: skip the ensemble leg)")
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--surface-steps", type=int, default=1000,
+                    help="timed steps of the surfaces workload (0: skip)")
     ap.add_argument("--precision-steps", type=int, default=2000,
                     help="timed steps of the FP32 box-offset leg (0: skip)")
     ap.add_argument("--sweep-members", type=int, default=4096,
@@ -443,6 +526,9 @@ def main():
         "final_counts": counts,
         "setup_s": setup_s,
     }
+    if args.surface_steps > 0:
+        log("surfaces workload, %d steps" % args.surface_steps)
+        out["surfaces"] = surfaces_leg(args.surface_steps, 20)
     if args.precision_steps > 0:
         log("FP32 box-offset records, %d steps" % args.precision_steps)
         out["fp32_offsets"] = precision_leg(spec, args.precision_steps, args.warmup)

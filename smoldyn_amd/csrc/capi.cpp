@@ -720,6 +720,10 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   EngineConfig cfg;
   memset(&cfg, 0, sizeof(cfg));
   cfg.f32 = sim->precision == 32;
+  {  // tuning override of the pair search's z sub-bins per box (1, 2, 4 or 8)
+    const char* e = getenv("SMOLAMD_ZSUB_MAX");
+    cfg.zsub_max = e ? atoi(e) : 0;
+  }
   cfg.nlist = nlist;
   // a list can only hold molecules placed at setup or created by reactions;
   // lists that receive neither keep a token capacity

@@ -171,6 +171,8 @@ struct EngineConfig {
   // record precision: 0 = FP64 positions; 1 = FP32 offsets from the FP64
   // origin of the record's virtual box (all arithmetic stays FP64)
   int f32;
+  // largest z sub-bin count per virtual box for the pair search (0: automatic)
+  int zsub_max;
 };
 
 // Host description of the surfaces handed to engine_create (arrays copied).

@@ -9,13 +9,16 @@
 // so the live position is kept as a record attribute plus an inverse map
 // (list slot -> record) instead of as the storage order.
 //
-// One reference time step (smolsim.cpp:2607-2690 simulatetimestep, no
-// surfaces) runs as seven launches and no host synchronisation:
+// One reference time step (smolsim.cpp:2607-2690 simulatetimestep) runs as
+// six launches and no host synchronisation:
 //
-//   k_diffuse   diffuse (smolmolec.c:3115-3168), checkwalls (smolwall.c:368-424),
-//               pos2box (smolboxes.c:67-79), unireact coins (smolreact.c:2997-3054),
-//               box histogram; one thread per record of the source buffer
-//   k_scan1/2/3 exclusive scan of the histogram (box lists of reassignmolecs)
+//   k_diffuse   diffuse (smolmolec.c:3115-3168), checkwalls (smolwall.c:368-424)
+//               or checksurfaces, pos2box (smolboxes.c:67-79), unireact coins
+//               (smolreact.c:2997-3054), histogram of (list, box, z sub-bin);
+//               one thread per record of the source buffer
+//   k_scan1, k_scan23  exclusive scan of the histogram (box lists of
+//               reassignmolecs, each box's records ordered by z sub-bin);
+//               k_scan2 + k_scan3 instead of k_scan23 for large scans
 //   k_scatter   counting-sort scatter into the destination buffer + inverse map
 //               + packed FP32 payload for the pair search
 //   k_pairs     bireact candidates (smolreact.c:3201-3296): same box and half
@@ -50,6 +53,8 @@ namespace smolamd {
   } while (0)
 
 constexpr int kBlock = 256;
+constexpr int kRankBits = 27;  // arrival rank within a histogram bucket; the z sub-bin sits above it
+constexpr int kRankMask = (1 << kRankBits) - 1;
 constexpr int kScanPer = 4;
 constexpr int kScanBlock = 1024;
 constexpr int kScanTile = kScanBlock * kScanPer;  // 4096 entries per scan block
@@ -76,10 +81,10 @@ struct Recs {
 };
 
 struct Binned {
-  int* count;   // [nlist*nbox+1], zeroed by k_scan1 after reading
+  int* count;   // [nlist*nbox*zsub+1] per simulation, zeroed by k_scan1 after reading
   int* part;    // block-local exclusive prefix
-  int* boff;    // exclusive prefix of block sums
-  int* bstart;  // first destination record of (list, box); bstart[nscan-1] = records binned
+  int* boff;    // block sums of the 4096-entry tiles
+  int* bstart;  // first destination record of (list, box, z sub-bin); bstart[nscan-1] = records binned
   float4* spk;  // FP32 position + species bits of the destination records
 };
 
@@ -104,7 +109,12 @@ struct Inst {
   Control* ctl;
   PairConst pc;
   uint32_t key0, key1;
-  int boxbase;     // first histogram key (list-box) of this simulation
+  int keybase;     // first histogram key of this simulation
+  // z sub-bins per virtual box (1, 2, 4 or 8): records are binned by (list,
+  // box, z sub-bin), so a box's records are ordered by z slab; a sub-bin is at
+  // least as tall as the largest screening radius, so partners within it lie
+  // in the same or an adjacent sub-bin of a neighbor column
+  int zsub;
   double screen_lim;  // coordinates beyond this bypass the FP32 pair screen
   UniEvent* uniev;
   int maxevents;
@@ -298,7 +308,8 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
                                                     const double* __restrict__ gtab, int gmask,
                                                     const double* __restrict__ inj) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  int hkey = -1;  // histogram key (simulation's first key + list*nbox + box), -1 when not binned
+  int hkey = -1;  // histogram key (simulation's first key + (list*nbox + box)*zsub + sub-bin), -1 when not binned
+  int zsbits = 0;  // z sub-bin, kept above the rank for k_scatter
   if (gl->halt) return;  // uniform over the grid: no wave reaches the histogram below
   if (r < cur_nrec(insts, gl)) {
     int i = a.ident[r];
@@ -356,7 +367,20 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       int b = pos2box(g, pos[0], pos[1], pos[2]);
       // FP32-offset records are stored relative to the new box
       if (moved || (kF32 && b != bold)) store_pos<kF32>(a, g, r, b, pos);
-      hkey = I.boxbase + ll * g.nbox + b;
+      // z sub-bin: floor of the box-relative z coordinate times zsub (exact
+      // for a power of two), clamped like the box index
+      int zs = 0;
+      if (I.zsub > 1) {
+        const double zrel = (pos[2] - g.bmin[2]) / g.bsize[2];
+        int iz = (int)zrel;
+        if (iz < 0) iz = 0;
+        else if (iz >= g.side[2]) iz = g.side[2] - 1;
+        zs = (int)((zrel - (double)iz) * (double)I.zsub);
+        if (zs < 0) zs = 0;
+        else if (zs >= I.zsub) zs = I.zsub - 1;
+      }
+      hkey = I.keybase + (ll * g.nbox + b) * I.zsub + zs;
+      zsbits = zs << kRankBits;
       // unireact: coin flips in table order, first success reacts; a
       // molecule that reacts takes no part in the pair search, so it is not
       // binned
@@ -391,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
     }
   }
   int rank = wave_hist_rank(bcount, hkey);
-  if (hkey >= 0) a.rank[r] = rank;
+  if (hkey >= 0) a.rank[r] = rank | zsbits;
 }
 
 // ---------------------------------------------------------------------------
@@ -463,6 +487,33 @@ __global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ part, 
   if (i < n) bstart[i] = part[i] + boff[i / kScanTile];
 }
 
+// Tiles up to which the box starts come from k_scan23 alone (each workgroup
+// sums the block sums before its tile); larger scans use k_scan2 + k_scan3.
+constexpr int kScanFusedTiles = 512;
+
+// Box starts of one 4096-entry tile per workgroup: the tile's offset is the
+// sum of the block sums of the tiles before it (reduced here, in a fixed
+// order), added to the tile-local prefix.  Replaces a separate scan of the
+// block sums.
+__global__ __launch_bounds__(kScanBlock) void k_scan23(const int* __restrict__ part, const int* __restrict__ bsum,
+                                                       int* __restrict__ bstart, int n) {
+  __shared__ int s_red[kScanBlock / 64];
+  const int t = blockIdx.x;
+  int sum = 0;
+  for (int i = threadIdx.x; i < t; i += blockDim.x) sum += bsum[i];
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  int off = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) off += s_red[w];
+  const int base = t * kScanTile + threadIdx.x * kScanPer;
+#pragma unroll
+  for (int k = 0; k < kScanPer; k++) {
+    const int i = base + k;
+    if (i < n) bstart[i] = part[i] + off;
+  }
+}
+
 // counting-sort scatter of the binned records into the destination buffer
 // The inverse map (list slot -> record) is written only for the last
 // `window` slots of each list: compaction in k_resolve reads tail slots only,
@@ -478,7 +529,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ ins
   const Inst& I = insts[in];
   int i = a.ident[r];
   int ll = I.tab->listof[i];
-  int dst = bn.bstart[I.boxbase + ll * I.g.nbox + bx] + a.rank[r];
+  const int rk = a.rank[r];
+  const int zs = rk >> kRankBits;
+  int dst = bn.bstart[I.keybase + (ll * I.g.nbox + bx) * I.zsub + zs] + (rk & kRankMask);
   double x, y, z;
   if (kF32) {
     const float fx = a.ox[r], fy = a.oy[r], fz = a.oz[r];
@@ -508,7 +561,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ ins
   // such a molecule bypasses the FP32 screen
   const double lim = I.screen_lim;
   int far = !(fabs(x) <= lim && fabs(y) <= lim && fabs(z) <= lim);
-  bn.spk[dst] = make_float4((float)x, (float)y, (float)z, __int_as_float(i | (far << 16)));
+  bn.spk[dst] = make_float4((float)x, (float)y, (float)z, __int_as_float(i | (far << 16) | (zs << 17)));
   if (lp >= I.ctl->nl[ll] - I.sortcap) inv[I.L.base[ll] + lp] = dst;
 }
 
@@ -625,7 +678,9 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
   const int i1 = w1 & 0xFFFF;
   const uint32_t mask = pc.reactmask[i1];
   if (!mask) return;
-  const float r2f = (w1 >> 16) ? 3.0e38f : pc.r2f[i1];
+  const float r2f = ((w1 >> 16) & 1) ? 3.0e38f : pc.r2f[i1];
+  const int zsub = I.zsub;
+  const int zs1 = (w1 >> 17) & 7;
   const int ll1 = tab->listof[i1];
   const int nl = tab->nlist;
   const int bb = b.box[u];
@@ -646,7 +701,6 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
     for (int c = 0; c < ncol; c++) {
       int dx = half ? (c < 3 ? -1 : 0) : c / 3 - 1;
       int dy = half ? (c < 3 ? c - 1 : c - 4) : c % 3 - 1;
-      int dzhi = (half && c == 4) ? 0 : 1;
       int nx = bx + dx, ny = by + dy, cw = 0;
       float shx = 0.f, shy = 0.f;
       if (nx < 0) {
@@ -671,25 +725,28 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
         cw |= 2 << 2;
         shy = (float)g.width[1];
       }
-      const int colb = I.boxbase + ll2 * nbox + (nx * g.side[1] + ny) * sz;
-      // z segments: unwrapped run [zlo, zhi] plus wrapped single boxes
-      int zlo = bz - 1, zhi = bz + dzhi;
+      const int colb = I.keybase + (ll2 * nbox + (nx * g.side[1] + ny) * sz) * zsub;
+      // z segments in sub-bins: unwrapped run [zlo, zhi] plus wrapped single
+      // sub-bins; column (0,0) of the half stencil stops at its own box
+      const int nzs = sz * zsub;
+      const int gz1 = bz * zsub + zs1;
+      const int zlo = gz1 - 1, zhi = (half && c == 4) ? min(gz1 + 1, (bz + 1) * zsub - 1) : gz1 + 1;
       for (int seg = 0; seg < 3; seg++) {
         int za, zb2, zw = 0, dzfix = -2;
         float shz = 0.f;
         if (seg == 0) {  // wrapped box below
           if (zlo >= 0) continue;
           if (!g.wrapflag[2]) continue;
-          za = zb2 = sz - 1;
+          za = zb2 = nzs - 1;
           zw = 1 << 4;
           shz = -(float)g.width[2];
           dzfix = -1;
         } else if (seg == 1) {  // unwrapped run
           za = zlo < 0 ? 0 : zlo;
-          zb2 = zhi >= sz ? sz - 1 : zhi;
+          zb2 = zhi >= nzs ? nzs - 1 : zhi;
           if (za > zb2) continue;
         } else {  // wrapped box above
-          if (zhi < sz) continue;
+          if (zhi < nzs) continue;
           if (!g.wrapflag[2]) continue;
           za = zb2 = 0;
           zw = 2 << 4;
@@ -710,7 +767,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
             const float dxf = o.x - ox, dyf = o.y - oy, dzf = o.z - oz;
             const float d2 = dxf * dxf + dyf * dyf + dzf * dzf;
             const int w2 = __float_as_int(o.w);
-            const bool hit = ((d2 <= r2f) | (w2 >> 16)) & ((mask >> (w2 & 31)) & 1u);
+            const bool hit = ((d2 <= r2f) | ((w2 >> 16) & 1)) & ((mask >> (w2 & 31)) & 1u);
             hits |= (uint32_t)hit << j;
           }
           if (u >= kb && u < kb + 32) hits &= ~(1u << (u - kb));
@@ -1344,8 +1401,27 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
     I.g = geo[k];
     I.key0 = key0[k];
     I.key1 = key1[k];
-    I.boxbase = keys;
-    keys += cfg[k].nlist * geo[k].nbox;
+    // z sub-bins: the largest power of two whose slab height is at least the
+    // largest screening radius (binding radius + FP32 margin)
+    {
+      double rmax = 0;
+      for (int i = 0; i < kMaxSpecies; i++)
+        if (tab[k].reactmask[i]) rmax = fmax(rmax, sqrt(tab[k].maxbind2[i]) + geo[k].prefilter_margin);
+      // at most 4, and not below about one molecule per sub-bin
+      int zs = 1;
+      while (zs < 4 && 2 * zs <= cfg[k].expected_per_box + 0.5 && rmax > 0 &&
+             geo[k].bsize[2] / (2 * zs) >= rmax * (1 + 1e-9))
+        zs *= 2;
+      if (rmax <= 0) zs = 1;
+      I.zsub = cfg[k].zsub_max > 0 && zs > cfg[k].zsub_max ? cfg[k].zsub_max : zs;
+    }
+    I.keybase = keys;
+    if ((long)keys + (long)cfg[k].nlist * geo[k].nbox * I.zsub >= (1L << 30)) {
+      snprintf(err, errlen, "too many virtual boxes for the scan");
+      delete s;
+      return nullptr;
+    }
+    keys += cfg[k].nlist * geo[k].nbox * I.zsub;
     I.maxevents = cfg[k].maxevents;
     I.maxedges = cfg[k].maxedges;
     I.sortcap = pow2ceil(2 * cfg[k].maxevents + 16);
@@ -1374,6 +1450,16 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
     I.screen_lim = 2.0 * ext;
   }
   s->captot = run;
+  if ((keys + 1 + kScanTile - 1) / kScanTile > kScanBlock * 8) {
+    // too many keys for the scan with z sub-bins: one sub-bin per box
+    keys = 0;
+    for (int k = 0; k < n; k++) {
+      Inst& I = s->hinst[k];
+      I.zsub = 1;
+      I.keybase = keys;
+      keys += cfg[k].nlist * geo[k].nbox;
+    }
+  }
   s->nscan = keys + 1;
   s->nscanblk = (s->nscan + kScanTile - 1) / kScanTile;
   if (s->nscanblk > kScanBlock * 8) {
@@ -1609,9 +1695,14 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   if (timed) (void)hipEventRecord(s->ev[1], st);
   hipLaunchKernelGGL(k_scan1, dim3(s->nscanblk), dim3(kScanBlock), 0, st, s->bn.count, s->bn.part, s->bn.boff,
                      s->nscan);
-  hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kScanBlock), 0, st, s->bn.boff, s->nscanblk);
-  hipLaunchKernelGGL(k_scan3, dim3((s->nscan + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s->bn.part, s->bn.boff,
-                     s->bn.bstart, s->nscan);
+  if (s->nscanblk <= kScanFusedTiles) {
+    hipLaunchKernelGGL(k_scan23, dim3(s->nscanblk), dim3(kScanBlock), 0, st, s->bn.part, s->bn.boff, s->bn.bstart,
+                       s->nscan);
+  } else {
+    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kScanBlock), 0, st, s->bn.boff, s->nscanblk);
+    hipLaunchKernelGGL(k_scan3, dim3((s->nscan + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s->bn.part, s->bn.boff,
+                       s->bn.bstart, s->nscan);
+  }
   if (timed) (void)hipEventRecord(s->ev[2], st);
   hipLaunchKernelGGL(kscat, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, B, s->bn, s->inv);
   if (timed) (void)hipEventRecord(s->ev[3], st);
