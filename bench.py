@@ -44,7 +44,7 @@ def log(msg):
 ALGO_BYTES = {"diffuse": 72, "scatter": 112, "pairs": 24}
 KERNEL_NAMES = ["diffuse", "scan", "scatter", "event_overhead", "pairs", "resolve"]
 HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_scan1+k_scan23", "scatter": "k_scatter",
-                     "pairs": "k_pairs", "resolve": "k_resolve"}
+                     "pairs": "k_pairs+k_exact", "resolve": "k_resolve"}
 
 
 def c2_spec(n_per_species=500000, side=1000.0, seed=2026, kback=1.0):

@@ -53,6 +53,7 @@ namespace smolamd {
   } while (0)
 
 constexpr int kBlock = 256;
+constexpr int kExactBlocks = 512;  // grid of k_exact (grid-stride over the candidates)
 constexpr int kRankBits = 27;  // arrival rank within a histogram bucket; the z sub-bin sits above it
 constexpr int kRankMask = (1 << kRankBits) - 1;
 constexpr int kScanPer = 4;
@@ -138,6 +139,7 @@ struct Glob {
   int nprod[2];
   int ninst;  // simulations in the batch
   int halt;   // set on an inconsistency: every later step does nothing
+  int ncand;  // screened candidate pairs of the step (k_pairs -> k_exact), reset by k_resolve
 };
 
 struct DeviceState {
@@ -181,6 +183,8 @@ struct DeviceState {
   double* inj = nullptr;
   long injcap = 0;
   int hnrec = 0;                     // records uploaded so far
+  int4* cand = nullptr;              // screened candidate pairs (k_pairs -> k_exact)
+  int candcap = 0;
   hipStream_t stream = nullptr;
   hipGraphExec_t graph[2] = {};
   int graph_ready[2] = {0, 0};
@@ -659,12 +663,13 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf,
 // two contiguous ranges (two when z wraps).  Half stencil (same list): columns
 // (-1,-1),(-1,0),(-1,1),(0,-1) with dz -1..1 and column (0,0) with dz -1..0;
 // full stencil (two lists): all nine columns, dz -1..1.  dz = 0 in column
-// (0,0) is the molecule's own box, bireact's same-box pass.  Candidates pass
-// an FP32 screen, then the reference's FP64 test; only then is their stencil
-// index looked up for the edge key.
-template <bool kSurf, bool kMulti, bool kF32>
-__global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, const Glob* __restrict__ gl, Recs b,
-                                                  Binned bn, int nscan) {
+// (0,0) is the molecule's own box, bireact's same-box pass.  Records that
+// pass the FP32 screen are handed to k_exact, which runs the reference's FP64
+// test and keys the edge; the screen itself stays small so many waves hide
+// the latency of its loads.
+template <bool kMulti>
+__global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, Glob* __restrict__ gl, Recs b,
+                                                  Binned bn, int nscan, int4* __restrict__ cand, int candcap) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (gl->halt || u >= bn.bstart[nscan - 1]) return;  // records binned
   const Inst& I = insts[kMulti ? b.inst[u] : 0];
@@ -688,15 +693,9 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
   const int bx = bb / (g.side[1] * sz);
   const int by = (bb / sz) % g.side[1];
   const int bz = bb % sz;
-  const int lp1 = b.lpos[u];
-  const uint32_t step = ctl->step;
-  double p1[3];
-  uint64_t ser1 = 0;
-  bool loaded = false;
   for (int ll2 = ll1; ll2 < nl; ll2++) {
     if (!tab->rxnmol2[ll1 * kMaxList + ll2]) continue;
     const bool half = (ll1 == ll2);
-    const uint64_t key1base = ((uint64_t)tab->pairindex[ll1 * kMaxList + ll2] << 59) | ((uint64_t)lp1 << 32);
     const int ncol = half ? 5 : 9;
     for (int c = 0; c < ncol; c++) {
       int dx = half ? (c < 3 ? -1 : 0) : c / 3 - 1;
@@ -771,27 +770,56 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
             hits |= (uint32_t)hit << j;
           }
           if (u >= kb && u < kb + 32) hits &= ~(1u << (u - kb));
+          // candidates go to k_exact: record pair and the stencil position
+          // (column offsets, fixed dz of a wrapped sub-bin, wrap code, list)
+          const int code = (dx + 1) | ((dy + 1) << 2) | ((dzfix + 2) << 4) | (wrap << 6) | (ll2 << 12) |
+                           ((int)half << 15);
           while (hits) {
             const int k = kb + __ffs(hits) - 1;
             hits &= hits - 1;
-            const int i2 = __float_as_int(bn.spk[k].w) & 0xFFFF;
-            const int dz = dzfix > -2 ? dzfix : (b.box[k] % sz) - bz;
-            const int nbi = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-            const int lp2 = b.lpos[k];
-            if (nbi == 13 && half && lp2 >= lp1) continue;
-            if (!loaded) {
-              load_pos<kF32>(b, g, u, bb, p1);
-              ser1 = b.serno[u];
-              loaded = true;
-            }
-            const uint64_t key = nbi == 13 ? (key1base | (uint64_t)lp2)
-                                           : ((1ull << 63) | key1base | ((uint64_t)nbi << 27) | (uint64_t)lp2);
-            exact_pair<kSurf, kF32>(g, I.sf, tab, ctl, b, I.edges, I.maxedges, I.key0, I.key1, step, u, p1, i1, ser1, k, i2,
-                       nbi == 13 ? 0 : wrap, key);
+            const int ci = atomicAdd(&gl->ncand, 1);
+            if (ci < candcap) cand[ci] = make_int4(u, k, code, 0);
+            else atomicOr(&ctl->err, kErrEdgeOverflow);
           }
         }
       }
     }
+  }
+}
+
+// Exact test of the screened candidates (one thread per candidate, grid-
+// stride): stencil index and edge key of molecule 1's loop position, then the
+// reference's FP64 distance test and reaction choice (exact_pair).
+template <bool kSurf, bool kMulti, bool kF32>
+__global__ __launch_bounds__(kBlock) void k_exact(const Inst* __restrict__ insts, const Glob* __restrict__ gl, Recs b,
+                                                  Binned bn, const int4* __restrict__ cand, int candcap) {
+  if (gl->halt) return;
+  const int n = min(gl->ncand, candcap);
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const int4 cd = cand[c];
+    const int u = cd.x, k = cd.y, code = cd.z;
+    const int dx = (code & 3) - 1, dy = ((code >> 2) & 3) - 1, dzfix = ((code >> 4) & 3) - 2;
+    const int wrap = (code >> 6) & 63, ll2 = (code >> 12) & 7;
+    const bool half = (code >> 15) & 1;
+    const Inst& I = insts[kMulti ? b.inst[u] : 0];
+    const Geometry& g = I.g;
+    const DevTables* tab = I.tab;
+    const int sz = g.side[2];
+    const int bb = b.box[u];
+    const int dz = dzfix > -2 ? dzfix : (b.box[k] % sz) - bb % sz;
+    const int nbi = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+    const int lp1 = b.lpos[u], lp2 = b.lpos[k];
+    if (nbi == 13 && half && lp2 >= lp1) continue;
+    const int i1 = __float_as_int(bn.spk[u].w) & 0xFFFF;
+    const int i2 = __float_as_int(bn.spk[k].w) & 0xFFFF;
+    const int ll1 = tab->listof[i1];
+    const uint64_t key1base = ((uint64_t)tab->pairindex[ll1 * kMaxList + ll2] << 59) | ((uint64_t)lp1 << 32);
+    const uint64_t key = nbi == 13 ? (key1base | (uint64_t)lp2)
+                                   : ((1ull << 63) | key1base | ((uint64_t)nbi << 27) | (uint64_t)lp2);
+    double p1[3];
+    load_pos<kF32>(b, g, u, bb, p1);
+    exact_pair<kSurf, kF32>(g, I.sf, tab, I.ctl, b, I.edges, I.maxedges, I.key0, I.key1, I.ctl->step, u, p1, i1,
+                            b.serno[u], k, i2, nbi == 13 ? 0 : wrap, key);
   }
 }
 
@@ -1314,6 +1342,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     if (blockIdx.x == 0) {  // counters of the next step
       gl->nrec[(step + 1) & 1] = 0;
       gl->nprod[(step + 1) & 1] = 0;
+      gl->ncand = 0;
     }
     ctl->nbinned = nbinned;
     ctl->serno_next = ser0 + (uint64_t)K;
@@ -1481,6 +1510,10 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
         dalloc(&s->bn.bstart, s->nscan) | dalloc(&s->bn.spk, rc);
   ok |= dalloc(&s->bestkey, rc) | dalloc(&s->killed, rc);
   ok |= dalloc(&s->edges, ned) | dalloc(&s->estate, ned) | dalloc(&s->accepted, ned);
+  // screened candidates: the FP32 screen passes few pairs beyond the ones
+  // within the binding radius, so the edge capacity bounds them with room
+  s->candcap = ned * 2 > (size_t)(1 << 28) ? (1 << 28) : (int)(ned * 2);
+  ok |= dalloc(&s->cand, (size_t)s->candcap);
   ok |= dalloc(&s->uniev, nev) | dalloc(&s->log, nsort);
   ok |= dalloc(&s->sortkey, nsort) | dalloc(&s->sortval, nsort) | dalloc(&s->killslot, nsort);
   ok |= dalloc(&s->prods, nsort * kMaxProd);
@@ -1562,7 +1595,7 @@ void engine_destroy(DeviceState* s) {
   if (!s) return;
   free_recs(&s->buf[0]);
   free_recs(&s->buf[1]);
-  void* ptrs[] = {s->inv,     s->bn.count, s->bn.part, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges,  s->estate,
+  void* ptrs[] = {s->inv,     s->bn.count, s->bn.part, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges,  s->cand,  s->estate,
                   s->accepted, s->bestkey, s->killed,  s->uniev,   s->log,       s->sortkey, s->sortval, s->killslot,
                   s->ctl,     s->tab,     s->gtab,    s->inj,     s->prods,     s->d_pnl,   s->d_bpstart,
                   s->d_bplist, s->d_act,  s->dinst,   s->glob};
@@ -1685,9 +1718,10 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
                             : (f ? k_diffuse<false, false, true> : k_diffuse<false, false, false>);
   auto kscat = s->multi ? (f ? k_scatter<true, true> : k_scatter<true, false>)
                         : (f ? k_scatter<false, true> : k_scatter<false, false>);
-  auto kpair = s->multi ? (f ? k_pairs<false, true, true> : k_pairs<false, true, false>)
-               : s->surf_on ? (f ? k_pairs<true, false, true> : k_pairs<true, false, false>)
-                            : (f ? k_pairs<false, false, true> : k_pairs<false, false, false>);
+  auto kpair = s->multi ? k_pairs<true> : k_pairs<false>;
+  auto kexact = s->multi ? (f ? k_exact<false, true, true> : k_exact<false, true, false>)
+                : s->surf_on ? (f ? k_exact<true, false, true> : k_exact<true, false, false>)
+                             : (f ? k_exact<false, false, true> : k_exact<false, false, false>);
   auto kres = f ? k_resolve<true> : k_resolve<false>;
   if (timed) (void)hipEventRecord(s->ev[0], st);
   hipLaunchKernelGGL(kdiff, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, s->bn.count, s->gtab,
@@ -1707,7 +1741,10 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   hipLaunchKernelGGL(kscat, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, B, s->bn, s->inv);
   if (timed) (void)hipEventRecord(s->ev[3], st);
   if (timed) (void)hipEventRecord(s->ev[4], st);
-  hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, B, s->bn, s->nscan);
+  hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, B, s->bn, s->nscan, s->cand,
+                     s->candcap);
+  hipLaunchKernelGGL(kexact, dim3(kExactBlocks), dim3(kBlock), 0, st, s->dinst, s->glob, B, s->bn, s->cand,
+                     s->candcap);
   if (timed) (void)hipEventRecord(s->ev[5], st);
   hipLaunchKernelGGL(kres, dim3(s->ninst), dim3(kResolveThreads), 0, st, s->dinst, s->glob, A, B, s->inv, s->bn,
                      s->nscan, s->bestkey, s->killed, s->reccap);
