@@ -907,6 +907,40 @@ __device__ void wg_rank_sort(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv
   __syncthreads();
 }
 
+// Rank sort, in LDS, of two independent groups [0, n0) and [n0, n0+n1) of
+// (key, val) pairs already in lk/lv: each group is ordered by key (ties by
+// index) and its sorted values and low key words go to ov / ok32 (either may
+// be null), group 1 after group 0.  Up to 64 threads share the comparisons of
+// one element (a power of two, so their lanes are aligned within a wave).
+template <typename K>
+__device__ void wg_rank_sort2(const K* lk, const int* lv, int n0, int n1, int* ov, int* ok32) {
+  const int n = n0 + n1;
+  int tpe = 1;
+  while (tpe < 64 && n * tpe * 2 <= (int)blockDim.x) tpe <<= 1;
+  const int total = n * tpe;
+  for (int base = 0; base < total; base += blockDim.x) {  // same trip count for every thread
+    const int t = base + threadIdx.x;
+    const int i = t / tpe, q = t & (tpe - 1);
+    int r = 0;
+    K k = 0;
+    if (i < n) {
+      k = lk[i];
+      const int lo = i < n0 ? 0 : n0, hi = i < n0 ? n0 : n;
+      for (int j = lo + q; j < hi; j += tpe) {
+        const K x = lk[j];
+        r += (x < k) | ((x == k) & (j < i));
+      }
+    }
+    for (int o = tpe >> 1; o > 0; o >>= 1) r += __shfl_down(r, o, tpe);
+    if (i < n && q == 0) {
+      const int dst = (i < n0 ? 0 : n0) + r;
+      if (ov) ov[dst] = lv[i];
+      if (ok32) ok32[dst] = (int)k;
+    }
+  }
+  __syncthreads();
+}
+
 // Sort n (key,val) pairs of gk/gv in place; when ok32/ov are given (LDS,
 // capacity >= n) the sorted low key words / values are also left there.
 __device__ void wg_sort_any(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv, int* ok32 = nullptr,
@@ -1069,19 +1103,33 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     B2 = 0;
     __syncthreads();
   }
-  for (int i = tid; i < U; i += NT) {
-    I.sk[i] = (uint64_t)I.uniev[i].slot;
-    I.sv[i] = i;
-  }
-  __syncthreads();
   const bool ev_lds = (U + B2) <= kLdsEvents;
-  wg_sort_any(I.sk, I.sv, U, lk, lv, nullptr, ev_lds ? s_ev : nullptr);
-  for (int i = tid; i < B2; i += NT) {
-    I.sk[U + i] = I.edges[I.accepted[i]].key;
-    I.sv[U + i] = I.accepted[i];
+  if (U + B2 <= kRankSort) {  // both groups sorted at once, keys gathered straight into LDS
+    for (int i = tid; i < U; i += NT) {
+      lk[i] = (uint64_t)I.uniev[i].slot;
+      lv[i] = i;
+    }
+    for (int i = tid; i < B2; i += NT) {
+      const int a = I.accepted[i];
+      lk[U + i] = I.edges[a].key;
+      lv[U + i] = a;
+    }
+    __syncthreads();
+    wg_rank_sort2<uint64_t>(lk, lv, U, B2, s_ev, nullptr);
+  } else {
+    for (int i = tid; i < U; i += NT) {
+      I.sk[i] = (uint64_t)I.uniev[i].slot;
+      I.sv[i] = i;
+    }
+    __syncthreads();
+    wg_sort_any(I.sk, I.sv, U, lk, lv, nullptr, ev_lds ? s_ev : nullptr);
+    for (int i = tid; i < B2; i += NT) {
+      I.sk[U + i] = I.edges[I.accepted[i]].key;
+      I.sv[U + i] = I.accepted[i];
+    }
+    __syncthreads();
+    wg_sort_any(I.sk + U, I.sv + U, B2, lk, lv, nullptr, ev_lds ? s_ev + U : nullptr);
   }
-  __syncthreads();
-  wg_sort_any(I.sk + U, I.sv + U, B2, lk, lv, nullptr, ev_lds ? s_ev + U : nullptr);
   int NE = U + B2;
   const int* evs = ev_lds ? s_ev : I.sv;  // event order: sorted event indices
 
@@ -1099,6 +1147,9 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   // product records of this simulation: [nbinned + s_prodoff, + K)
   if (tid == 0) s_prodoff = atomicAdd(&gl->nprod[step & 1], K);
   int D = U + 2 * B2;  // reactants killed = molsort's kill count
+  // few kills: their list slots go straight to LDS for the rank sort
+  const bool kill_small = D <= kRankSort;
+  int* const kslot = kill_small ? lv : nullptr;
   uint64_t ser0 = ctl->serno_next;
 
   // ---- products (positions, species, lists) and reactant kills
@@ -1115,7 +1166,8 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       order = 1;
       load_pos<kF32>(RA, I.g, ev.rec, ev.box, rxnpos);
       sA = RA.serno[ev.rec];
-      I.killslot[e] = ev.slot;
+      if (kill_small) kslot[e] = ev.slot;
+      else I.killslot[e] = ev.slot;
     } else {
       Edge ed = I.edges[evs[e]];
       rx = &tab->rxn2[ed.r];
@@ -1159,8 +1211,15 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       sA = B.serno[swap ? ed.s2 : ed.s1];
       sB = B.serno[swap ? ed.s1 : ed.s2];
       int k2 = U + 2 * (e - U);
-      I.killslot[k2] = I.L.base[tab->listof[id1]] + B.lpos[ed.s1];
-      I.killslot[k2 + 1] = I.L.base[tab->listof[id2]] + B.lpos[ed.s2];
+      const int ks1 = I.L.base[tab->listof[id1]] + B.lpos[ed.s1];
+      const int ks2 = I.L.base[tab->listof[id2]] + B.lpos[ed.s2];
+      if (kill_small) {
+        kslot[k2] = ks1;
+        kslot[k2 + 1] = ks2;
+      } else {
+        I.killslot[k2] = ks1;
+        I.killslot[k2 + 1] = ks2;
+      }
     }
     atomicSub(&s_dcount[rx->rct0], 1);
     if (order == 2) atomicSub(&s_dcount[rx->rct1], 1);
@@ -1209,13 +1268,17 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   // ---- molsort: sort the killed list slots, fill holes from the list tails
   // (swap-from-tail compaction), then append products in the dead-list order
   // the reference produces: creation order reversed, rotated by the kills.
-  for (int i = tid; i < D; i += NT) {
-    I.sk[i] = (uint64_t)I.killslot[i];
-    I.sv[i] = 0;
-  }
-  __syncthreads();
   const bool kill_lds = D <= kLdsKills;
-  wg_sort_any(I.sk, I.sv, D, lk, lv, kill_lds ? s_kill : nullptr, nullptr);
+  if (kill_small) {
+    wg_rank_sort2<int>(lv, lv, D, 0, nullptr, s_kill);
+  } else {
+    for (int i = tid; i < D; i += NT) {
+      I.sk[i] = (uint64_t)I.killslot[i];
+      I.sv[i] = 0;
+    }
+    __syncthreads();
+    wg_sort_any(I.sk, I.sv, D, lk, lv, kill_lds ? s_kill : nullptr, nullptr);
+  }
   const KillView kv{kill_lds ? s_kill : nullptr, I.sk};
   if (tid < I.L.nlist) {
     int ll = tid;
