@@ -756,19 +756,34 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   HostSurf hs;
   memset(&hs, 0, sizeof(hs));
   if (sim->surf.enabled()) {
-    std::vector<double> dsol(nsp);
-    std::vector<int> lsol(nsp);
     bool mob[kMaxList] = {false};
-    for (int i = 0; i < nsp; i++) {
-      dsol[i] = sim->difc[i][MSsoln];
-      lsol[i] = lookup[i][MSsoln];
-    }
     for (int i = 1; i < nsp; i++)
       for (int ms = 0; ms < kMS; ms++)
         if (ismobile(sim, i, ms) && lookup[i][ms] >= 0) mob[lookup[i][ms]] = true;
+    std::vector<std::array<double, kMS>> dms(sim->difc.begin(), sim->difc.begin() + nsp);
+    std::vector<std::array<int, kMS>> lms(lookup.begin(), lookup.begin() + nsp);
     char msg[400] = "";
-    int er = sim->surf.build(g, nsp, dsol.data(), sim->dt, lsol.data(), mob, nlist, st, msg, sizeof(msg));
+    int er = sim->surf.build(g, nsp, reinterpret_cast<const double(*)[kMS]>(dms.data()), sim->dt,
+                             reinterpret_cast<const int(*)[kMS]>(lms.data()), mob, nlist, st, msg, sizeof(msg));
     LCHECK(er == 0, fn, ECerror, "%s", msg);
+    for (int ll = 0; ll < nlist; ll++)
+      LCHECK(!(st.listbound[ll] && mob[ll]), fn, ECerror,
+             "a molecule list holds both diffusing and surface-bound states; the device path needs them apart");
+    // reactions of surface-bound molecules are not on the device path
+    for (int order = 1; order <= 2; order++)
+      for (auto& rx : sim->rxn[order])
+        for (int k = 0; k < (order == 1 ? kMS : kMS1 * kMS1); k++) {
+          const int ms1 = order == 1 ? k : k / kMS1, ms2 = order == 1 ? 0 : k % kMS1;
+          const bool soln1 = ms1 == MSsoln || ms1 == MSbsoln, soln2 = ms2 == MSsoln || ms2 == MSbsoln;
+          LCHECK(!rx.permit[k] || (soln1 && soln2), fn, ECerror,
+                 "reactions of surface-bound molecules are not implemented on the device path");
+        }
+    hs.actb = st.actb.data();
+    for (int ll = 0; ll < kMaxList; ll++) hs.listbound[ll] = st.listbound[ll];
+    for (int i = 0; i < kMaxSpecies; i++) {
+      hs.blist[i][0] = st.blist[i][0];
+      hs.blist[i][1] = st.blist[i][1];
+    }
     hs.on = 1;
     hs.npnl = (int)st.pnl.size();
     hs.pnl = st.pnl.data();
@@ -1225,7 +1240,7 @@ int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state)
     set_error(fn, ECmissing, "missing sim");
     return (int)ECmissing;
   }
-  if (state != MSsoln && state != MSall) return 0;  // the device path holds solution molecules only
+  if (state != MSsoln && state != MSall && state != MSfront && state != MSback) return 0;
   int counts[kMaxSpecies] = {0};
   if (sim->dev && sim->ok) {
     Control c;
@@ -1233,7 +1248,14 @@ int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state)
       set_error(fn, ECerror, "HIP runtime error");
       return (int)ECerror;
     }
-    memcpy(counts, c.count, sizeof(counts));
+    // molcount by state: all states, solution = all minus the bound ones
+    for (int i = 0; i < kMaxSpecies; i++)
+      counts[i] = state == MSall    ? c.count[i]
+                  : state == MSfront ? c.countf[i]
+                  : state == MSback  ? c.countk[i]
+                                     : c.count[i] - c.countf[i] - c.countk[i];
+  } else if (state == MSfront || state == MSback) {
+    return 0;
   } else {
     for (auto& pm : sim->pending) counts[pm.ident]++;
   }

@@ -84,7 +84,10 @@ struct Control {
   int naccept;
   int nlog;
   int err;                 // bit flags, see ErrBits
-  int count[kMaxSpecies];  // molecules per species
+  int count[kMaxSpecies];  // molecules per species (all states)
+  int countf[kMaxSpecies];  // of which bound in the front state
+  int countk[kMaxSpecies];  // of which bound in the back state
+  int nmove;               // list changes queued by k_diffuse for the first molsort (surfaces)
   int maxedges_seen;
   int rounds;              // matching rounds of the last step
   unsigned long long molsteps;  // sum over steps of the live molecules at step start
@@ -96,7 +99,7 @@ enum ErrBits : int {
   kErrEventOverflow = 2,
   kErrCapacity = 4,
   kErrSortOverflow = 8,
-  kErrSurface = 16,  // a surface action the device path does not run (absorption, adsorption, list change)
+  kErrSurface = 16,  // a surface action the device path does not run (absorption, adsorption of a product, ...)
   kErrHalted = 32,   // stepping stopped after an error (later steps did nothing)
 };
 
@@ -105,7 +108,10 @@ enum ErrBits : int {
 
 constexpr int kMaxSurf = 16;
 enum DevPanelShape : int { kPsRect = 0, kPsSph = 2, kPsDisk = 5 };
-enum DevSrfAction : int { kSaReflect = 0, kSaTrans = 1, kSaAbsorb = 2, kSaJump = 3, kSaMult = 5, kSaAdsorb = 8 };
+enum DevSrfAction : int {
+  kSaReflect = 0, kSaTrans = 1, kSaAbsorb = 2, kSaJump = 3, kSaMult = 5, kSaNo = 6, kSaAdsorb = 8, kSaRevdes = 9,
+  kSaIrrevdes = 10, kSaFlip = 11
+};
 
 struct DevPanel {
   int ps;           // kPsRect, kPsSph or kPsDisk
@@ -137,6 +143,26 @@ struct DevSurf {
   const int* bpstart;       // panels of box b: bplist[bpstart[b] .. bpstart[b+1])
   const int* bplist;
   const DevSrfAct* act;     // [surface][species][face]
+  const DevSrfAct* actb;    // surface-bound molecules: [surface][species][front/back state], no face
+  int listbound[kMaxList];  // lists holding surface-bound states (checksurfacebound)
+  int blist[kMaxSpecies][2];  // list of each species in the front / back state (-1: none)
+};
+
+// A list change queued by k_diffuse and applied by the first molsort of the
+// step (smolmolec.c:2943-3023): an adsorbed molecule moving to the list of
+// its bound state, or a desorbing bound molecule whose list slot dies while
+// the record is appended, as a new solution molecule with the same serial,
+// to the end of its solution list.
+enum MoveKind : int { kMoveList = 1, kMoveDesorb = 2 };
+struct MoveEv {
+  int rec;   // record in the source buffer of the step
+  int dst;   // record in the destination buffer (k_scatter), -1 when not binned (a first-order reactant)
+  int slot;  // list base + live position before the change
+  int list;  // list after the change
+  int kind;
+  int obox;  // box of the molecule in its old list's box lists
+  int nbox;  // box it is added to in the new list's box lists
+  int ov;    // working order value (k_relist)
 };
 
 struct Edge {
@@ -184,8 +210,11 @@ struct HostSurf {
   const int* bpstart;  // nbox + 1 entries
   const int* bplist;
   int nsrf;
-  const DevSrfAct* act;  // nsrf * kMaxSpecies * 2
+  const DevSrfAct* act;   // nsrf * kMaxSpecies * 2
+  const DevSrfAct* actb;  // nsrf * kMaxSpecies * 2
   int listdiffuse[kMaxList];
+  int listbound[kMaxList];
+  int blist[kMaxSpecies][2];
   double epsilon;
 };
 

@@ -79,6 +79,10 @@ struct Recs {
   int* lpos;      // position in the reference's live list of its list
   int* box;       // virtual box (-1: not binned this step)
   int* rank;      // arrival rank within the box
+  int* ov;        // surfaces only: order value of the record within its box list (k_relist)
+  int* sb;        // surfaces only: 0 in solution, else (panel << 3) | pending << 2 | bound state (1 front,
+                  // 2 back); pending: adsorbed as a reaction product, still in its solution list until the
+                  // next step's first molsort
 };
 
 struct Binned {
@@ -93,6 +97,8 @@ struct Prod {
   int ident;
   int list;
   double x, y, z;
+  int sb;  // surfaces: a product adsorbed by the reborn check (pending its list change), else 0
+  int pad;
 };
 
 // Per-species screening constants of the pair search, passed by value.
@@ -127,6 +133,7 @@ struct Inst {
   uint64_t* sk;
   int* sv;
   int sortcap;
+  MoveEv* moves;  // list changes of the step (surfaces), capacity maxevents
   int* killslot;
   Prod* prods;
 };
@@ -180,6 +187,8 @@ struct DeviceState {
   int* d_bpstart = nullptr;
   int* d_bplist = nullptr;
   DevSrfAct* d_act = nullptr;
+  DevSrfAct* d_actb = nullptr;
+  MoveEv* moves = nullptr;
   double* inj = nullptr;
   long injcap = 0;
   int hnrec = 0;                     // records uploaded so far
@@ -207,6 +216,9 @@ __device__ __forceinline__ int pos2box(const Geometry& g, double x, double y, do
   }
   return b;
 }
+
+// Index of the bound state of a surface binding word: 0 front, 1 back.
+__device__ __forceinline__ int bstate(int sb) { return (sb & 1) ? 0 : 1; }
 
 // FP64 origin of virtual box b (the corner pos2box measures from); host and
 // device evaluate the same expression, so FP32-offset records decode alike.
@@ -310,7 +322,7 @@ template <bool kSurf, bool kMulti, bool kF32>
 __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
                                                     Recs a, int* __restrict__ bcount,
                                                     const double* __restrict__ gtab, int gmask,
-                                                    const double* __restrict__ inj) {
+                                                    const double* __restrict__ inj, int* __restrict__ inv) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   int hkey = -1;  // histogram key (simulation's first key + (list*nbox + box)*zsub + sub-bin), -1 when not binned
   int zsbits = 0;  // z sub-bin, kept above the rank for k_scatter
@@ -327,14 +339,23 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       Control* ctl = I.ctl;
       const Geometry& g = I.g;
       int ll = tab->listof[i];
+      int sb = 0;  // surface binding (kSurf): bound molecules live in the list of their state
+      bool pend = false;  // bound as a product last step, still in its solution list
+      if (kSurf) {
+        sb = a.sb[r];
+        pend = (sb & 4) != 0;
+        if (sb && !pend) ll = I.sf.blist[i][bstate(sb)];
+      }
       int lp = a.lpos[r];
       uint64_t ser = a.serno[r];
       uint32_t step = ctl->step;
+      int mkind = 0, newll = ll;  // list change queued for the first molsort (surfaces)
+      int obox = -1, rbox = -1;    // desorption: box in the old list, box the reference files it in
       const int bold = kF32 ? a.box[r] : 0;
       double pos[3];
       load_pos<kF32>(a, g, r, bold, pos);
       const bool moved = tab->diffuselist[ll];
-      if (moved) {
+      if (moved && !pend) {
         const double posx[3] = {pos[0], pos[1], pos[2]};
         double ds = tab->difstep[i];
         double gg[3];
@@ -356,21 +377,88 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
           apply_walls(g, pos);
         } else if (I.sf.listdiffuse[ll]) {
           // with surfaces the walls are not applied (smolsim.cpp:2618-2630)
-          int i2 = i;
-          int st = srf_check(I.sf, g, tab->difc, pos, posx, &i2, ser, step, SMOLRNG_SURF_DIFFUSED, I.key0, I.key1);
-          if (st != kSrfOk) atomicOr(&ctl->err, kErrSurface);
+          int i2 = i, bnd = 0;
+          int st = srf_check(I.sf, g, tab->difc, pos, posx, &i2, ser, step, SMOLRNG_SURF_DIFFUSED, I.key0, I.key1,
+                             &bnd);
+          if (st == kSrfAdsorbed && I.sf.blist[i2][bstate(bnd)] >= 0) {  // bound: to the list of its state
+            sb = bnd;
+            a.sb[r] = bnd;
+            newll = I.sf.blist[i2][bstate(bnd)];
+            mkind = kMoveList;
+            atomicAdd((bnd & 3) == 1 ? &ctl->countf[i2] : &ctl->countk[i2], 1);
+          } else if (st == kSrfAdsorbed) {
+            atomicOr(&ctl->err, kErrSurface);
+            atomicOr((int*)&gl->halt, 1);
+          } else if (st != kSrfOk) {
+            atomicOr(&ctl->err, kErrSurface);
+          } else if (i2 != i && tab->listof[i2] != ll) {
+            atomicOr(&ctl->err, kErrSurface);
+          }
           if (i2 != i) {
-            if (tab->listof[i2] != ll) atomicOr(&ctl->err, kErrSurface);
             a.ident[r] = i2;
             atomicSub(&ctl->count[i], 1);
             atomicAdd(&ctl->count[i2], 1);
             i = i2;
           }
         }
+      } else if (kSurf && sb && !pend && I.sf.listbound[ll]) {
+        // checksurfacebound (smolsurface.c:5184-5199): desorption
+        int i2 = i;
+        obox = pos2box(g, pos[0], pos[1], pos[2]);
+        const int st = srf_bound(I.sf, g, tab->difc, tab->difstep, pos, &i2, sb, ser, step, I.key0, I.key1, &rbox);
+        if (st == kSrfDesorbed) {
+          atomicSub((sb & 3) == 1 ? &ctl->countf[i] : &ctl->countk[i], 1);
+          if (i2 != i) {
+            a.ident[r] = i2;
+            atomicSub(&ctl->count[i], 1);
+            atomicAdd(&ctl->count[i2], 1);
+            i = i2;
+          }
+          sb = 0;
+          a.sb[r] = 0;
+          newll = tab->listof[i];
+          mkind = kMoveDesorb;
+        } else if (st != kSrfOk) {
+          atomicOr(&ctl->err, kErrSurface);
+        }
+      }
+      if (kSurf && pend) {  // the bound product joins the list of its state now (zero displacement: bound
+                            // molecules do not diffuse, so its segment crosses nothing)
+        sb &= ~4;
+        a.sb[r] = sb;
+        newll = I.sf.blist[i][bstate(sb)];
+        mkind = kMoveList;
+      }
+      int mev = -1;
+      if (kSurf && mkind) {  // queue the list change; the first molsort gives the record its new slot
+        const int e = atomicAdd(&ctl->nmove, 1);
+        if (e < I.maxevents) {
+          MoveEv mv;
+          mv.rec = r;
+          mv.dst = -1;
+          mv.slot = I.L.base[ll] + lp;
+          mv.list = newll;
+          mv.kind = mkind;
+          mv.obox = obox;
+          mv.nbox = -1;
+          mv.ov = lp;
+          I.moves[e] = mv;
+          mev = e;
+          a.lpos[r] = -(e + 2);
+        } else {  // a lost list change would corrupt the lists: stop
+          atomicOr(&ctl->err, kErrEventOverflow);
+          atomicOr((int*)&gl->halt, 1);
+        }
+        ll = newll;
       }
       int b = pos2box(g, pos[0], pos[1], pos[2]);
+      if (kSurf && mkind == kMoveDesorb) b = rbox;  // the reference's (possibly stale) box for this step
+      if (kSurf && mev >= 0) {  // boxes of the list change: a moving molecule keeps its box
+        if (mkind != kMoveDesorb) I.moves[mev].obox = b;
+        I.moves[mev].nbox = b;
+      }
       // FP32-offset records are stored relative to the new box
-      if (moved || (kF32 && b != bold)) store_pos<kF32>(a, g, r, b, pos);
+      if (moved || mkind == kMoveDesorb || (kF32 && b != bold)) store_pos<kF32>(a, g, r, b, pos);
       // z sub-bin: floor of the box-relative z coordinate times zsub (exact
       // for a power of two), clamped like the box index
       int zs = 0;
@@ -388,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       // unireact: coin flips in table order, first success reacts; a
       // molecule that reacts takes no part in the pair search, so it is not
       // binned
-      int nu = tab->nuni[i];
+      int nu = sb ? 0 : tab->nuni[i];  // surface-bound molecules do not react
       if (nu > 0) {
         smolrng4 rb;
         int blk = -1;
@@ -404,13 +492,16 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
             int e = atomicAdd(&ctl->nuni, 1);
             if (e < I.maxevents) {
               I.uniev[e].rec = r;
-              I.uniev[e].slot = I.L.base[ll] + lp;
+              I.uniev[e].slot = mkind ? -1 : I.L.base[ll] + lp;  // -1: appended by this step's first molsort
               I.uniev[e].r = rx;
               I.uniev[e].box = b;
             } else {
               atomicOr(&ctl->err, kErrEventOverflow);
             }
             hkey = -1;
+            // the first molsort may move this molecule's slot: it reads the
+            // record of an unbinned slot from the inverse map
+            if (kSurf && !mkind && lp >= ctl->nl[ll] - I.sortcap) inv[I.L.base[ll] + lp] = -(r + 2);
             break;
           }
         }
@@ -522,7 +613,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan23(const int* __restrict__ p
 // The inverse map (list slot -> record) is written only for the last
 // `window` slots of each list: compaction in k_resolve reads tail slots only,
 // and the kills of one step never exceed the window.
-template <bool kMulti, bool kF32>
+template <bool kMulti, bool kF32, bool kSurf>
 __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
                                                     Recs a, Recs b, Binned bn, int* __restrict__ inv) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -533,6 +624,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ ins
   const Inst& I = insts[in];
   int i = a.ident[r];
   int ll = I.tab->listof[i];
+  const int sb = kSurf ? a.sb[r] : 0;
+  if (kSurf && sb) ll = I.sf.blist[i][bstate(sb)];
   const int rk = a.rank[r];
   const int zs = rk >> kRankBits;
   int dst = bn.bstart[I.keybase + (ll * I.g.nbox + bx) * I.zsub + zs] + (rk & kRankMask);
@@ -565,7 +658,17 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ ins
   // such a molecule bypasses the FP32 screen
   const double lim = I.screen_lim;
   int far = !(fabs(x) <= lim && fabs(y) <= lim && fabs(z) <= lim);
-  bn.spk[dst] = make_float4((float)x, (float)y, (float)z, __int_as_float(i | (far << 16) | (zs << 17)));
+  // bit 20: a surface-bound molecule, which takes no part in the pair search
+  bn.spk[dst] = make_float4((float)x, (float)y, (float)z,
+                            __int_as_float(i | (far << 16) | (zs << 17) | ((sb != 0) << 20)));
+  if (kSurf) {
+    b.sb[dst] = sb;
+    b.ov[dst] = lp < 0 ? -1 : lp;  // box lists are in live order until the first molsort changes them
+    if (lp < 0) {  // list change of this step: the first molsort gives the slot
+      I.moves[-lp - 2].dst = dst;
+      return;
+    }
+  }
   if (lp >= I.ctl->nl[ll] - I.sortcap) inv[I.L.base[ll] + lp] = dst;
 }
 
@@ -580,6 +683,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild_inv(const Inst* __restrict__
   if (i <= 0 || i >= kMaxSpecies || in < 0 || in >= gl->ninst) return;
   const Inst& I = insts[in];
   int ll = I.tab->listof[i];
+  if (a.sb && a.sb[r] && !(a.sb[r] & 4)) ll = I.sf.blist[i][bstate(a.sb[r])];
   int lp = a.lpos[r];
   if (lp >= 0 && lp < I.L.base[ll + 1] - I.L.base[ll]) inv[I.L.base[ll] + lp] = r;
 }
@@ -667,7 +771,7 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf,
 // pass the FP32 screen are handed to k_exact, which runs the reference's FP64
 // test and keys the edge; the screen itself stays small so many waves hide
 // the latency of its loads.
-template <bool kMulti>
+template <bool kMulti, bool kSurf>
 __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, Glob* __restrict__ gl, Recs b,
                                                   Binned bn, int nscan, int4* __restrict__ cand, int candcap) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -683,6 +787,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
   const int i1 = w1 & 0xFFFF;
   const uint32_t mask = pc.reactmask[i1];
   if (!mask) return;
+  if (kSurf && ((w1 >> 20) & 1)) return;  // surface-bound: no reactions
   const float r2f = ((w1 >> 16) & 1) ? 3.0e38f : pc.r2f[i1];
   const int zsub = I.zsub;
   const int zs1 = (w1 >> 17) & 7;
@@ -766,7 +871,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
             const float dxf = o.x - ox, dyf = o.y - oy, dzf = o.z - oz;
             const float d2 = dxf * dxf + dyf * dyf + dzf * dzf;
             const int w2 = __float_as_int(o.w);
-            const bool hit = ((d2 <= r2f) | ((w2 >> 16) & 1)) & ((mask >> (w2 & 31)) & 1u);
+            bool hit = ((d2 <= r2f) | ((w2 >> 16) & 1)) & ((mask >> (w2 & 31)) & 1u);
+            if (kSurf) hit = hit & !((w2 >> 20) & 1);
             hits |= (uint32_t)hit << j;
           }
           if (u >= kb && u < kb + 32) hits &= ~(1u << (u - kb));
@@ -808,8 +914,12 @@ __global__ __launch_bounds__(kBlock) void k_exact(const Inst* __restrict__ insts
     const int bb = b.box[u];
     const int dz = dzfix > -2 ? dzfix : (b.box[k] % sz) - bb % sz;
     const int nbi = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-    const int lp1 = b.lpos[u], lp2 = b.lpos[k];
-    if (nbi == 13 && half && lp2 >= lp1) continue;
+    // molecule 1's live position orders the outer loop; the partners come in
+    // box-list order (= live order unless this step's first molsort changed
+    // the box list: surfaces)
+    const int lp1 = b.lpos[u];
+    const int o1 = kSurf ? b.ov[u] : lp1, lp2 = kSurf ? b.ov[k] : b.lpos[k];
+    if (nbi == 13 && half && lp2 >= o1) continue;
     const int i1 = __float_as_int(bn.spk[u].w) & 0xFFFF;
     const int i2 = __float_as_int(bn.spk[k].w) & 0xFFFF;
     const int ll1 = tab->listof[i1];
@@ -821,6 +931,12 @@ __global__ __launch_bounds__(kBlock) void k_exact(const Inst* __restrict__ insts
     exact_pair<kSurf, kF32>(g, I.sf, tab, I.ctl, b, I.edges, I.maxedges, I.key0, I.key1, I.ctl->step, u, p1, i1,
                             b.serno[u], k, i2, nbi == 13 ? 0 : wrap, key);
   }
+}
+
+// List slot of a first-order reactant at the time of unireact: its live
+// position after the step's first molsort (the record of the source buffer).
+__device__ __forceinline__ int uni_slot(const Inst& I, const Recs& RA, int rec) {
+  return I.L.base[I.tab->listof[RA.ident[rec]]] + RA.lpos[rec];
 }
 
 // ---------------------------------------------------------------------------
@@ -1007,6 +1123,249 @@ struct KillView {
 };
 
 
+// First molsort of a step with surfaces (smolmolec.c:2943-3023, called after
+// checksurfaces and checksurfacebound, smolsim.cpp:2618-2636): applies the
+// list changes k_diffuse queued, in the reference's order, to the live
+// positions (lpos) and the inverse map, before the pair search keys them.
+// Per list in order: the removed slots (molecules that moved out or died)
+// are filled from the top of the list as molsort's swap-from-tail scan does;
+// molecules moved in by earlier lists sit after the list and are rotated by
+// one per removal (the scan's second swap); removed molecules are met in the
+// scan's order and a moving one is appended to its new list then.  Last, the
+// desorbed molecules (the reference's resurrected ones, one serial number
+// each taken from the counter) are appended in reverse creation order (the
+// dead list's order after as many kills as desorptions).  One workgroup per
+// simulation; the placement itself is sequential, in LDS, on few events.
+constexpr int kRelistMax = 512;
+constexpr int kAppendOv = 1 << 26;  // order values of box-list appends: above every live position
+__global__ __launch_bounds__(kResolveThreads) void k_relist(const Inst* __restrict__ insts, Glob* gl, Recs RA,
+                                                            Recs RB, int* __restrict__ inv, int reccap, Binned bn) {
+  const Inst& I = insts[blockIdx.x];
+  if (gl->halt) return;
+  Control* ctl = I.ctl;
+  const int M0 = ctl->nmove;
+  if (M0 == 0) return;  // uniform over the workgroup
+  __shared__ int s_key[kRelistMax], s_val[kRelistMax], s_slot[kRelistMax], s_ev[kRelistMax], s_top[kRelistMax];
+  __shared__ int s_app[kRelistMax], s_appl[kRelistMax];
+  __shared__ int s_pref[2 * kRelistMax], s_plist[2 * kRelistMax], s_ppos[2 * kRelistMax];
+  __shared__ int s_lo[kMaxList], s_cnt[kMaxList], s_nl[kMaxList], s_np, s_nop;
+  // box-list operations in the scan's order: key = ((list*nbox + box) << 11) | sequence, value = event | kind<<30
+  __shared__ uint64_t s_opk[2 * kRelistMax];
+  __shared__ int s_opv[2 * kRelistMax], s_ops[2 * kRelistMax], s_ord[2 * kRelistMax];
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int M = M0 < I.maxevents ? M0 : I.maxevents;
+  if (M > kRelistMax) {
+    if (tid == 0) {
+      atomicOr(&ctl->err, kErrSortOverflow);
+      atomicOr(&gl->halt, 1);
+    }
+    return;
+  }
+  const int nlist = I.L.nlist;
+  for (int e = tid; e < M; e += NT) {
+    s_key[e] = I.moves[e].slot;
+    s_val[e] = e;
+  }
+  __syncthreads();
+  wg_rank_sort2<int>(s_key, s_val, M, 0, s_ev, s_slot);  // events by old slot
+  if (tid < nlist) {  // each list's removed slots: [s_lo, s_lo + s_cnt) of the sorted events
+    const int lo = I.L.base[tid], hi = I.L.base[tid + 1];
+    int l = 0, h = M;
+    while (l < h) {
+      const int mid = (l + h) >> 1;
+      if (s_slot[mid] < lo) l = mid + 1;
+      else h = mid;
+    }
+    const int k0 = l;
+    h = M;
+    while (l < h) {
+      const int mid = (l + h) >> 1;
+      if (s_slot[mid] < hi) l = mid + 1;
+      else h = mid;
+    }
+    s_lo[tid] = k0;
+    s_cnt[tid] = l - k0;
+    s_nl[tid] = ctl->nl[tid];
+  }
+  __syncthreads();
+  // records at the top r slots of each list (r = its removals): the kept
+  // ones fill the holes
+  for (int ll = 0; ll < nlist; ll++) {
+    const int lo = s_lo[ll], r = s_cnt[ll], b = s_nl[ll] - r;
+    for (int k = tid; k < r; k += NT) s_top[lo + k] = inv[I.L.base[ll] + b + k];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int np = 0, napp = 0, ndes = 0, nop = 0;
+    const uint64_t nbx = (uint64_t)I.g.nbox;
+    auto boxop = [&](int list, int box, int e, int add) {  // boxremovemol / boxaddmol of the scan
+      s_opk[nop] = (((uint64_t)list * nbx + (uint64_t)box) << 11) | (uint64_t)nop;
+      s_opv[nop] = e | (add << 30);
+      nop++;
+    };
+    auto ref = [&](int e) { return I.moves[e].dst >= 0 ? I.moves[e].dst : -(I.moves[e].rec + 2); };
+    auto place = [&](int rf, int list, int pos) {
+      s_pref[np] = rf;
+      s_plist[np] = list;
+      s_ppos[np] = pos;
+      np++;
+    };
+    for (int ll = 0; ll < nlist; ll++) {
+      const int lo = s_lo[ll], r = s_cnt[ll], n0 = s_nl[ll], b = n0 - r, base = I.L.base[ll];
+      auto meet = [&](int e) {  // a removed molecule met by the scan
+        const MoveEv mv = I.moves[e];
+        boxop(ll, mv.obox, e, 0);
+        if (mv.kind != kMoveList) return;  // a desorbed molecule: its slot dies here
+        boxop(mv.list, mv.nbox, e, 1);
+        if (mv.list > ll) {
+          s_app[napp] = e;
+          s_appl[napp] = mv.list;
+          napp++;
+        } else {
+          place(ref(e), mv.list, s_nl[mv.list]++);
+        }
+      };
+      int nlow = 0;
+      while (nlow < r && s_slot[lo + nlow] - base < b) nlow++;
+      int tp = n0, rtop = lo + r - 1;
+      for (int j = 0; j < nlow; j++) {
+        meet(s_ev[lo + j]);
+        for (;;) {
+          if (tp <= b) break;  // only on inconsistent input; the hole stays unfilled
+          tp--;
+          if (rtop >= lo + nlow && s_slot[rtop] - base == tp) {
+            meet(s_ev[rtop]);
+            rtop--;
+            continue;
+          }
+          place(s_top[lo + (tp - b)], ll, s_slot[lo + j] - base);
+          break;
+        }
+      }
+      if (tp > b) {  // the rest of the top slots are all removed: met from b, then downward
+        if (nlow < r) meet(s_ev[lo + nlow]);
+        for (int q = tp - 1; q > b; q--)
+          if (nlow + (q - b) < r) meet(s_ev[lo + nlow + (q - b)]);
+      }
+      // molecules appended to this list by earlier lists, rotated by r
+      int a = 0;
+      for (int k = 0; k < napp; k++) a += s_appl[k] == ll;
+      if (a > 0) {
+        int idx = 0;
+        for (int k = 0; k < napp; k++) {
+          if (s_appl[k] != ll) continue;
+          const int fin = (idx + r) % a;  // the item at append index idx ends at b + fin
+          place(ref(s_app[k]), ll, b + fin);
+          idx++;
+        }
+      }
+      s_nl[ll] = b + a;
+    }
+    // desorbed molecules: appended in reverse creation order (creation order =
+    // checksurfacebound's, i.e. by old slot)
+    for (int k = M - 1; k >= 0; k--) {
+      const int e = s_ev[k];
+      if (I.moves[e].kind != kMoveDesorb) continue;
+      place(ref(e), I.moves[e].list, s_nl[I.moves[e].list]++);
+      boxop(I.moves[e].list, I.moves[e].nbox, e, 1);
+      ndes++;
+    }
+    s_np = np;
+    s_nop = nop;
+    for (int ll = 0; ll < nlist; ll++) ctl->nl[ll] = s_nl[ll];
+    ctl->serno_next += (uint64_t)ndes;
+    ctl->nmove = 0;
+  }
+  __syncthreads();
+  for (int k = tid; k < s_np; k += NT) {
+    const int rf = s_pref[k], list = s_plist[k], pos = s_ppos[k];
+    const bool ok = (rf >= 0 ? rf < reccap : (rf <= -2 && -rf - 2 < reccap)) && list >= 0 && list < nlist &&
+                    pos >= 0 && pos < I.L.base[list + 1] - I.L.base[list];
+    if (!ok) {  // inconsistent list state: stop rather than write out of range
+      atomicOr(&ctl->err, kErrCapacity);
+      atomicOr(&gl->halt, 1);
+      continue;
+    }
+    if (rf >= 0) RB.lpos[rf] = pos;
+    else RA.lpos[-rf - 2] = pos;
+    inv[I.L.base[list] + pos] = rf;
+  }
+  // box-list order after the scan: bireact walks box lists, which reassign
+  // filled in live order (order value = old live position) and which the scan
+  // then changed: a removal moves the box list's last molecule into the
+  // removed one's place (boxremovemol, smolboxes.c:167-175; it takes that
+  // order value), an addition goes last (boxaddmol).  Operations grouped by
+  // (list, box) in the scan's order; one thread per group.
+  const int nop = s_nop;
+  for (int k = tid; k < nop; k += NT) s_ops[k] = k;
+  __syncthreads();
+  wg_rank_sort2<uint64_t>(s_opk, s_ops, nop, 0, s_ord, nullptr);
+  const int* ord = s_ord;  // op indices by (list, box, sequence)
+  for (int k = tid; k < nop; k += NT) {
+    const uint64_t grp = s_opk[ord[k]] >> 11;
+    if (k > 0 && (s_opk[ord[k - 1]] >> 11) == grp) continue;  // not the first op of its group
+    const int L = (int)(grp / (uint64_t)I.g.nbox), X = (int)(grp % (uint64_t)I.g.nbox);
+    int kend = k;
+    while (kend < nop && (s_opk[ord[kend]] >> 11) == grp) kend++;
+    const int key0 = I.keybase + (L * I.g.nbox + X) * I.zsub;
+    const int r0 = bn.bstart[key0], r1 = bn.bstart[key0 + I.zsub];
+    // first-order reactants of this step in this box (unbinned members), and
+    // unbinned molecules added to it
+    constexpr int kLoc = 32;
+    int lov[kLoc], nloc = 0;
+    const int nu = min(ctl->nuni, I.maxevents);
+    for (int u = 0; u < nu; u++) {
+      const UniEvent ev = I.uniev[u];
+      if (ev.slot < I.L.base[L] || ev.slot >= I.L.base[L + 1] || ev.box != X) continue;
+      if (nloc < kLoc) lov[nloc++] = ev.slot - I.L.base[L];
+      else atomicOr(&ctl->err, kErrCapacity);
+    }
+    for (int q = k; q < kend; q++) {
+      const int v = s_opv[ord[q]], e = v & 0x3FFFFFFF, add = v >> 30;
+      const int seq = (int)(s_opk[ord[q]] & 2047);
+      if (add) {
+        const int dst = I.moves[e].dst;
+        if (dst >= 0) RB.ov[dst] = kAppendOv + seq;
+        else if (nloc < kLoc) lov[nloc++] = kAppendOv + seq;
+        else atomicOr(&ctl->err, kErrCapacity);
+        continue;
+      }
+      // removal of e: the member with the largest order value takes its value
+      const int ovx = I.moves[e].ov;
+      int best = -1, bkind = 0, bidx = -1;
+      for (int rr = r0; rr < r1; rr++) {
+        const int o = RB.ov[rr];
+        if (o > best) {
+          best = o;
+          bkind = 1;
+          bidx = rr;
+        }
+      }
+      for (int t = 0; t < nloc; t++)
+        if (lov[t] > best) {
+          best = lov[t];
+          bkind = 2;
+          bidx = t;
+        }
+      for (int q2 = q + 1; q2 < kend; q2++) {  // molecules still to leave this box list
+        const int v2 = s_opv[ord[q2]];
+        if (v2 >> 30) continue;
+        const int o = I.moves[v2 & 0x3FFFFFFF].ov;
+        if (o > best) {
+          best = o;
+          bkind = 3;
+          bidx = v2 & 0x3FFFFFFF;
+        }
+      }
+      if (best > ovx) {
+        if (bkind == 1) RB.ov[bidx] = ovx;
+        else if (bkind == 2) lov[bidx] = ovx;
+        else I.moves[bidx].ov = ovx;
+      }
+    }
+  }
+}
+
 // One workgroup per simulation of the batch (blockIdx.x).
 template <bool kF32>
 __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restrict__ insts, Glob* gl, Recs RA,
@@ -1106,7 +1465,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   const bool ev_lds = (U + B2) <= kLdsEvents;
   if (U + B2 <= kRankSort) {  // both groups sorted at once, keys gathered straight into LDS
     for (int i = tid; i < U; i += NT) {
-      lk[i] = (uint64_t)I.uniev[i].slot;
+      lk[i] = (uint64_t)uni_slot(I, RA, I.uniev[i].rec);
       lv[i] = i;
     }
     for (int i = tid; i < B2; i += NT) {
@@ -1118,7 +1477,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     wg_rank_sort2<uint64_t>(lk, lv, U, B2, s_ev, nullptr);
   } else {
     for (int i = tid; i < U; i += NT) {
-      I.sk[i] = (uint64_t)I.uniev[i].slot;
+      I.sk[i] = (uint64_t)uni_slot(I, RA, I.uniev[i].rec);
       I.sv[i] = i;
     }
     __syncthreads();
@@ -1166,8 +1525,9 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       order = 1;
       load_pos<kF32>(RA, I.g, ev.rec, ev.box, rxnpos);
       sA = RA.serno[ev.rec];
-      if (kill_small) kslot[e] = ev.slot;
-      else I.killslot[e] = ev.slot;
+      const int us = uni_slot(I, RA, ev.rec);
+      if (kill_small) kslot[e] = us;
+      else I.killslot[e] = us;
     } else {
       Edge ed = I.edges[evs[e]];
       rx = &tab->rxn2[ed.r];
@@ -1229,6 +1589,8 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       Prod pr;
       pr.ident = rx->prd[q];
       pr.list = rx->prdlist[q];
+      pr.sb = 0;
+      pr.pad = 0;
       double pp[3] = {rxnpos[0], rxnpos[1], rxnpos[2]};
       if (rx->prdpos[q] != 0) {
         double v1[3] = {rx->prdpos[q] * uvec[0], rx->prdpos[q] * uvec[1], rx->prdpos[q] * uvec[2]};
@@ -1240,10 +1602,16 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
         apply_walls(I.g, pp);  // checkwalls for reborn molecules
       } else {
         // checksurfaces for reborn molecules: segment reaction position -> product
-        int i2 = pr.ident;
+        int i2 = pr.ident, bnd = 0;
         int st = srf_check(I.sf, I.g, tab->difc, pp, rxnpos, &i2, ser0 + (uint64_t)pc, step, SMOLRNG_SURF_REBORN,
-                           I.key0, I.key1);
-        if (st != kSrfOk || tab->listof[i2] != pr.list) atomicOr(&ctl->err, kErrSurface);
+                           I.key0, I.key1, &bnd);
+        if (st == kSrfAdsorbed && I.sf.blist[i2][bstate(bnd)] >= 0) {
+          // bound, but it stays in its solution list until the next step's first molsort
+          pr.sb = bnd | 4;
+          atomicAdd((bnd & 3) == 1 ? &ctl->countf[i2] : &ctl->countk[i2], 1);
+        } else if (st != kSrfOk || tab->listof[i2] != pr.list) {
+          atomicOr(&ctl->err, kErrSurface);
+        }
         pr.ident = i2;
       }
       atomicAdd(&s_dcount[pr.ident], 1);
@@ -1364,6 +1732,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
         const double pp[3] = {pr.x, pr.y, pr.z};
         store_pos<kF32>(B, I.g, rec, pbx, pp);
         B.ident[rec] = pr.ident;
+        if (B.sb) B.sb[rec] = pr.sb;
         if (B.inst) B.inst[rec] = blockIdx.x;  // batch member (multi-simulation records)
         B.serno[rec] = ser0 + (uint64_t)c;
         B.lpos[rec] = slot - I.L.base[ll];
@@ -1433,8 +1802,9 @@ static int dalloc(T** p, size_t n) {
   return 0;
 }
 
-static int alloc_recs(Recs* r, size_t n, bool multi, bool f32) {
+static int alloc_recs(Recs* r, size_t n, bool multi, bool f32, bool surf) {
   int ok = 0;
+  if (surf) ok |= dalloc(&r->sb, n) | dalloc(&r->ov, n);
   if (f32) ok |= dalloc(&r->ox, n) | dalloc(&r->oy, n) | dalloc(&r->oz, n);
   else ok |= dalloc(&r->px, n) | dalloc(&r->py, n) | dalloc(&r->pz, n);
   ok |= dalloc(&r->ident, n) | dalloc(&r->serno, n) | dalloc(&r->lpos, n) | dalloc(&r->box, n) |
@@ -1444,7 +1814,8 @@ static int alloc_recs(Recs* r, size_t n, bool multi, bool f32) {
 }
 
 static void free_recs(Recs* r) {
-  void* ptrs[] = {r->px, r->py, r->pz, r->ox, r->oy, r->oz, r->inst, r->ident, r->serno, r->lpos, r->box, r->rank};
+  void* ptrs[] = {r->px, r->py, r->pz, r->ox, r->oy, r->oz, r->inst, r->ident, r->serno, r->lpos, r->box, r->rank,
+                  r->sb, r->ov};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1567,7 +1938,8 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
   s->reccap = run + (int)prodroom;
   size_t rc = (size_t)s->reccap;
   int ok = 0;
-  ok |= alloc_recs(&s->buf[0], rc, s->multi, s->f32) | alloc_recs(&s->buf[1], rc, s->multi, s->f32);
+  ok |= alloc_recs(&s->buf[0], rc, s->multi, s->f32, surf && surf->on) |
+        alloc_recs(&s->buf[1], rc, s->multi, s->f32, surf && surf->on);
   ok |= dalloc(&s->inv, (size_t)s->captot);
   ok |= dalloc(&s->bn.count, s->nscan) | dalloc(&s->bn.part, s->nscan) | dalloc(&s->bn.boff, s->nscanblk) |
         dalloc(&s->bn.bstart, s->nscan) | dalloc(&s->bn.spk, rc);
@@ -1585,7 +1957,8 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
   if (surf && surf->on) {
     ok |= dalloc(&s->d_pnl, surf->npnl > 0 ? surf->npnl : 1) | dalloc(&s->d_bpstart, (size_t)surf->nbox + 1) |
           dalloc(&s->d_bplist, surf->bpstart[surf->nbox] > 0 ? surf->bpstart[surf->nbox] : 1) |
-          dalloc(&s->d_act, (size_t)surf->nsrf * kMaxSpecies * 2);
+          dalloc(&s->d_act, (size_t)surf->nsrf * kMaxSpecies * 2) |
+          dalloc(&s->d_actb, (size_t)surf->nsrf * kMaxSpecies * 2) | dalloc(&s->moves, nev);
   }
   if (ok) {
     snprintf(err, errlen, "device allocation failed (%d molecule records)", s->reccap);
@@ -1613,7 +1986,15 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
     if (surf->bpstart[surf->nbox] > 0)
       (void)hipMemcpy(s->d_bplist, surf->bplist, sizeof(int) * surf->bpstart[surf->nbox], hipMemcpyHostToDevice);
     (void)hipMemcpy(s->d_act, surf->act, sizeof(DevSrfAct) * surf->nsrf * kMaxSpecies * 2, hipMemcpyHostToDevice);
+    (void)hipMemcpy(s->d_actb, surf->actb, sizeof(DevSrfAct) * surf->nsrf * kMaxSpecies * 2, hipMemcpyHostToDevice);
     I.sf.on = 1;
+    for (int ll = 0; ll < kMaxList; ll++) I.sf.listbound[ll] = surf->listbound[ll];
+    for (int i = 0; i < kMaxSpecies; i++) {
+      I.sf.blist[i][0] = surf->blist[i][0];
+      I.sf.blist[i][1] = surf->blist[i][1];
+    }
+    I.sf.actb = s->d_actb;
+    I.moves = s->moves;
     I.sf.npnl = surf->npnl;
     for (int ll = 0; ll < kMaxList; ll++) I.sf.listdiffuse[ll] = surf->listdiffuse[ll];
     I.sf.epsilon = surf->epsilon;
@@ -1661,7 +2042,7 @@ void engine_destroy(DeviceState* s) {
   void* ptrs[] = {s->inv,     s->bn.count, s->bn.part, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges,  s->cand,  s->estate,
                   s->accepted, s->bestkey, s->killed,  s->uniev,   s->log,       s->sortkey, s->sortval, s->killslot,
                   s->ctl,     s->tab,     s->gtab,    s->inj,     s->prods,     s->d_pnl,   s->d_bpstart,
-                  s->d_bplist, s->d_act,  s->dinst,   s->glob};
+                  s->d_bplist, s->d_act,  s->d_actb,  s->moves,   s->dinst,   s->glob};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; k++)
@@ -1729,6 +2110,10 @@ int engine_upload_list_inst(DeviceState* s, int inst, int ll, int n, const int* 
     HIPOK(hipMemcpy(R.serno + r0, serno, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(R.lpos + r0, lp.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(R.box + r0, box.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    if (R.sb) {  // uploaded molecules are in solution
+      std::vector<int> zero(n, 0);
+      HIPOK(hipMemcpy(R.sb + r0, zero.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    }
     if (s->multi) HIPOK(hipMemcpy(R.inst + r0, in.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(s->inv + I.L.base[ll], inv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   }
@@ -1779,16 +2164,17 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   auto kdiff = s->multi ? (f ? k_diffuse<false, true, true> : k_diffuse<false, true, false>)
                : s->surf_on ? (f ? k_diffuse<true, false, true> : k_diffuse<true, false, false>)
                             : (f ? k_diffuse<false, false, true> : k_diffuse<false, false, false>);
-  auto kscat = s->multi ? (f ? k_scatter<true, true> : k_scatter<true, false>)
-                        : (f ? k_scatter<false, true> : k_scatter<false, false>);
-  auto kpair = s->multi ? k_pairs<true> : k_pairs<false>;
+  auto kscat = s->multi ? (f ? k_scatter<true, true, false> : k_scatter<true, false, false>)
+               : s->surf_on ? (f ? k_scatter<false, true, true> : k_scatter<false, false, true>)
+                            : (f ? k_scatter<false, true, false> : k_scatter<false, false, false>);
+  auto kpair = s->multi ? k_pairs<true, false> : (s->surf_on ? k_pairs<false, true> : k_pairs<false, false>);
   auto kexact = s->multi ? (f ? k_exact<false, true, true> : k_exact<false, true, false>)
                 : s->surf_on ? (f ? k_exact<true, false, true> : k_exact<true, false, false>)
                              : (f ? k_exact<false, false, true> : k_exact<false, false, false>);
   auto kres = f ? k_resolve<true> : k_resolve<false>;
   if (timed) (void)hipEventRecord(s->ev[0], st);
   hipLaunchKernelGGL(kdiff, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, s->bn.count, s->gtab,
-                     s->ngauss - 1, dinj);
+                     s->ngauss - 1, dinj, s->inv);
   if (timed) (void)hipEventRecord(s->ev[1], st);
   hipLaunchKernelGGL(k_scan1, dim3(s->nscanblk), dim3(kScanBlock), 0, st, s->bn.count, s->bn.part, s->bn.boff,
                      s->nscan);
@@ -1802,6 +2188,9 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   }
   if (timed) (void)hipEventRecord(s->ev[2], st);
   hipLaunchKernelGGL(kscat, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, B, s->bn, s->inv);
+  if (s->surf_on)  // first molsort: list changes from adsorption and desorption
+    hipLaunchKernelGGL(k_relist, dim3(s->ninst), dim3(kResolveThreads), 0, st, s->dinst, s->glob, A, B, s->inv,
+                       s->reccap, s->bn);
   if (timed) (void)hipEventRecord(s->ev[3], st);
   if (timed) (void)hipEventRecord(s->ev[4], st);
   hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, B, s->bn, s->nscan, s->cand,

@@ -231,10 +231,12 @@ __device__ __forceinline__ int srf_line2nextbox(const Geometry& g, const double*
 // 0/1: returns the action and sets the species after it.  u01 draws the
 // uniform of a rate-based action.
 template <typename Draw>
-__device__ __forceinline__ int srf_action(const DevSurf& sf, int srf, int ident, int face, int* i2, Draw u01) {
+__device__ __forceinline__ int srf_action(const DevSurf& sf, int srf, int ident, int face, int* i2, Draw u01,
+                                          int* ms2p = nullptr) {
   const DevSrfAct& a = sf.act[(srf * kMaxSpecies + ident) * 2 + face];
   if (a.act != kSaMult) {
     *i2 = a.newspec_fixed;
+    if (ms2p) *ms2p = 0;
     return a.act;
   }
   double r = u01();
@@ -245,7 +247,58 @@ __device__ __forceinline__ int srf_action(const DevSurf& sf, int srf, int ident,
       break;
     }
   *i2 = a.newspec[ms2];
+  if (ms2p) *ms2p = ms2;
   return a.outcome[ms2];
+}
+
+// ptinpanel (smolsurface.c:4276-4323) for 3D rectangles, spheres and disks
+__device__ __forceinline__ bool srf_ptinpanel(const DevPanel& pn, const double* pt) {
+  if (pn.ps == kPsRect) {
+    int d = (int)pn.front[2];
+    bool in = (pn.point[0][d] <= pt[d] && pt[d] <= pn.point[1][d]) || (pn.point[1][d] <= pt[d] && pt[d] <= pn.point[0][d]);
+    d = (d + 1) % 3;
+    if (d == (int)pn.front[1]) d = (d + 1) % 3;
+    return in && ((pn.point[1][d] <= pt[d] && pt[d] <= pn.point[2][d]) ||
+                  (pn.point[2][d] <= pt[d] && pt[d] <= pn.point[1][d]));
+  }
+  if (pn.ps == kPsSph) return true;
+  double len2 = 0, dot = 0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) len2 += (pt[d] - pn.point[0][d]) * (pt[d] - pn.point[0][d]);
+  if (len2 <= pn.point[1][0] * pn.point[1][0]) return true;
+#pragma unroll
+  for (int d = 0; d < 3; d++) dot += (pt[d] - pn.point[0][d]) * pn.front[d];
+  len2 -= dot * dot;
+  return len2 <= pn.point[1][0] * pn.point[1][0];
+}
+
+// panelnormal (smolsurface.c:4400-4428): outward unit normal of the given face
+__device__ __forceinline__ void srf_panelnormal(const DevPanel& pn, const double* pos, int face, double* norm) {
+  if (face != kFaceBack) face = kFaceFront;
+  if (pn.ps == kPsRect) {
+    norm[0] = norm[1] = norm[2] = 0;
+    norm[(int)pn.front[1]] =
+        ((face == kFaceFront && pn.front[0] == 1) || (face == kFaceBack && pn.front[0] == -1)) ? 1.0 : -1.0;
+  } else if (pn.ps == kPsDisk) {
+    const double sgn = face == kFaceFront ? 1.0 : -1.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) norm[d] = sgn * pn.front[d];
+  } else {
+    srf_spherenormal(pn.point[0], pos,
+                     ((face == kFaceFront && pn.front[0] == 1) || (face == kFaceBack && pn.front[0] == -1)) ? 1 : -1,
+                     norm);
+  }
+}
+
+// desorbdist (SurfaceParam.c:202-220): rms step times a rational fit of the
+// inverse distribution of desorption distances at the uniform y, for
+// irreversible desorption or for reversible adsorption
+__device__ __forceinline__ double srf_desorbdist(double step, double y, bool revads) {
+  const double y2 = y * y;
+  double x;
+  if (!revads) x = (0.571825 * y - 0.552246 * y2) / (1.0 - 1.53908 * y + 0.546424 * y2);
+  else x = (0.729614 * y - 0.70252 * y2) / (1.0 - 1.47494 * y + 0.484371 * y2);
+  return step * x;
 }
 
 // surfacejump (smolsurface.c:4873-4926) for rect, sphere and disk panels;
@@ -285,7 +338,7 @@ __device__ __forceinline__ int srf_jump(const DevSurf& sf, const DevPanel& pn, d
   return 1;
 }
 
-enum SrfStatus : int { kSrfOk = 0, kSrfAbsorbed = 1, kSrfUnsupported = 2 };
+enum SrfStatus : int { kSrfOk = 0, kSrfAbsorbed = 1, kSrfUnsupported = 2, kSrfAdsorbed = 3, kSrfDesorbed = 4 };
 
 // checksurfaces (smolsurface.c:5118-5180) for one solution-phase molecule:
 // segment posx -> pos, up to 100 interactions.  ident may change (species
@@ -294,7 +347,7 @@ enum SrfStatus : int { kSrfOk = 0, kSrfAbsorbed = 1, kSrfUnsupported = 2 };
 // SMOLRNG_SURFACE, pass<<8 | iteration).
 __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, const double* difc, double* pos,
                                          const double* posx, int* ident, uint64_t serial, uint32_t step, int pass,
-                                         uint32_t key0, uint32_t key1) {
+                                         uint32_t key0, uint32_t key1, int* bound = nullptr) {
   double via[3] = {posx[0], posx[1], posx[2]};
   for (int it = 1;; it++) {
     if (it > 100) {
@@ -339,7 +392,19 @@ __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, c
       smolrng4 r = smolrng_mol(key0, key1, serial, step, SMOLRNG_SURFACE, ((uint32_t)pass << 8) | (uint32_t)it);
       return smolrng_u01(r.v[0]);
     };
-    int act = srf_action(sf, pn.srf, i, facemin, &i2, draw);
+    int ms2;
+    int act = srf_action(sf, pn.srf, i, facemin, &i2, draw, &ms2);
+    if (act == kSaAdsorb) {
+      // adsorption (dosurfinteract, smolsurface.c:5005-5020): onto the panel
+      // at the crossing point, on the face of the bound state
+      if (!bound || !srf_ptinpanel(pn, crssptmin)) return kSrfUnsupported;
+#pragma unroll
+      for (int d = 0; d < 3; d++) pos[d] = crssptmin[d];
+      srf_fixpt2panel(pn, pos, ms2 == 1 ? kFaceFront : kFaceBack, sf.epsilon);
+      *ident = i2;
+      *bound = (pmin << 3) | ms2;
+      return kSrfAdsorbed;
+    }
     if (act == kSaTrans) {
       srf_fixpt2panel(pn, crssptmin, facemin == kFaceFront ? kFaceBack : kFaceFront, sf.epsilon);
     } else if (act == kSaReflect) {
@@ -366,6 +431,53 @@ __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, c
     via[1] = crssptmin[1];
     via[2] = crssptmin[2];
   }
+}
+
+// checksurfacebound (smolsurface.c:5184-5199) for one surface-bound
+// molecule: the rate-based action of its state with no face (dosurfinteract,
+// smolsurface.c:4930-5051).  On desorption the molecule is placed off the
+// panel by the desorption distance along the normal of the face it leaves
+// to, checked against the surfaces from the panel (pass DESORBED), and
+// becomes a solution-phase molecule: returns kSrfDesorbed with pos and
+// ident updated, and *boxp = the box the reference files it in (that of the
+// position before the surface check).  bound = (panel << 3) | state.
+__device__ __forceinline__ int srf_bound(const DevSurf& sf, const Geometry& g, const double* difc,
+                                         const double* difstep, double* pos, int* ident, int bound, uint64_t serial,
+                                         uint32_t step, uint32_t key0, uint32_t key1, int* boxp) {
+  const int p = bound >> 3, ms = bound & 3;
+  const DevPanel& pn = sf.pnl[p];
+  const int i = *ident;
+  const DevSrfAct& a = sf.actb[(pn.srf * kMaxSpecies + i) * 2 + (ms - 1)];
+  if (a.act != kSaMult) return kSrfOk;
+  const double r =
+      smolrng_u01(smolrng_mol(key0, key1, serial, step, SMOLRNG_SURFACE, (uint32_t)SMOLRNG_SURF_BOUND << 8).v[0]);
+  int ms2 = 5;
+  for (int m = 0; m < 6; m++)
+    if (r < a.cumprob[m]) {
+      ms2 = m;
+      break;
+    }
+  const int o = a.outcome[ms2], i2 = a.newspec[ms2];
+  if (o == kSaNo) return kSrfOk;
+  if (o != kSaRevdes && o != kSaIrrevdes) return kSrfUnsupported;
+  const int face = ms2 == 0 ? kFaceFront : kFaceBack;
+  double crsspt[3] = {pos[0], pos[1], pos[2]};
+  srf_fixpt2panel(pn, crsspt, face, sf.epsilon);
+  const double y = smolrng_u01cc(
+      smolrng_mol(key0, key1, serial, step, SMOLRNG_SURFACE, (uint32_t)SMOLRNG_SURF_DESORBDIST << 8).v[0]);
+  const double x = srf_desorbdist(difstep[i2], y, o == kSaRevdes);
+  double norm[3];
+  srf_panelnormal(pn, crsspt, face, norm);
+#pragma unroll
+  for (int d = 0; d < 3; d++) pos[d] = crsspt[d] + x * norm[d];
+  // the reference files the new molecule in the box of this position, before
+  // its surface check (smolsurface.c:5036)
+  *boxp = pos2box(g, pos[0], pos[1], pos[2]);
+  int i3 = i2;
+  const int st = srf_check(sf, g, difc, pos, crsspt, &i3, serial, step, SMOLRNG_SURF_DESORBED, key0, key1, nullptr);
+  if (st != kSrfOk) return kSrfUnsupported;
+  *ident = i3;
+  return kSrfDesorbed;
 }
 
 // rxnXsurface (smolsurface.c:4388-4424) for two solution-phase molecules: 1

@@ -17,7 +17,8 @@ namespace {
 enum { MSsoln_ = 0, MSbsoln_ = 5, MSnone_ = 7, MSall_ = 6 };
 enum { PFfront_ = 0, PFback_ = 1, PFnone_ = 2, PFboth_ = 3 };
 enum { SAreflect_ = 0, SAtrans_ = 1, SAabsorb_ = 2, SAjump_ = 3, SAport_ = 4, SAmult_ = 5, SAno_ = 6,
-       SAadsorb_ = 8 };
+       SAnone_ = 7, SAadsorb_ = 8, SArevdes_ = 9, SAirrevdes_ = 10, SAflip_ = 11 };
+enum { MSfront_ = 1, MSback_ = 2 };
 constexpr double kVeryLarge = 1.0e+20;  // smoldyn.h:61
 
 const char* shape_name(int ps) {
@@ -129,6 +130,70 @@ void tristate2index(int ms, int ms1, int ms2, int* ms3p, int* facep, int* ms4p) 
   *ms3p = ms3;
   *facep = face;
   *ms4p = ms4;
+}
+
+// srfsamestate (smolsurface.c:640-658): whether ms2 is the state a molecule
+// in ms1 at face1 stays in; *ms3p = that staying state
+bool same_state(int ms1, int face1, int ms2, int* ms3p) {
+  bool same;
+  if (face1 == PFfront_ && ms2 == MSsoln_) same = true;
+  else if (face1 == PFback_ && ms2 == MSbsoln_) same = true;
+  else if (face1 == PFnone_ && ms1 == ms2) same = true;
+  else same = false;
+  if (ms3p) {
+    if (face1 == PFfront_) *ms3p = MSsoln_;
+    else if (face1 == PFback_) *ms3p = MSbsoln_;
+    else *ms3p = (ms1 == MSsoln_) ? MSnone_ : ms1;
+  }
+  return same;
+}
+
+// srfreverseaction (smolsurface.c:661-710)
+void reverse_action(int ms1, int face1, int ms2, int* ms3p, int* face2p, int* ms4p, int* actp) {
+  int ms3, face2, ms4, act;
+  if (ms1 == MSsoln_ && face1 == PFnone_) {
+    ms3 = ms4 = MSnone_;
+    face2 = PFnone_;
+    act = SAnone_;
+  } else if (ms1 == MSsoln_) {
+    if (ms2 == MSsoln_ || ms2 == MSbsoln_) {
+      ms3 = MSsoln_;
+      face2 = (ms2 == MSsoln_) ? PFfront_ : PFback_;
+      act = (face1 == face2) ? SAreflect_ : SAtrans_;
+    } else {
+      ms3 = ms2;
+      face2 = PFnone_;
+      act = SAadsorb_;
+    }
+    ms4 = (face1 == PFfront_) ? MSsoln_ : MSbsoln_;
+  } else if (face1 == PFnone_) {
+    if (ms2 == MSsoln_ || ms2 == MSbsoln_) {
+      ms3 = MSsoln_;
+      face2 = (ms2 == MSsoln_) ? PFfront_ : PFback_;
+      act = SArevdes_;
+    } else {
+      ms3 = ms2;
+      face2 = PFnone_;
+      act = SAflip_;
+    }
+    ms4 = ms1;
+  } else {
+    if (ms2 == MSsoln_ || ms2 == MSbsoln_) {
+      ms3 = ms1;
+      face2 = (ms2 == MSsoln_) ? PFfront_ : PFback_;
+      ms4 = (face1 == PFfront_) ? MSsoln_ : MSbsoln_;
+      act = (face1 == face2) ? SAreflect_ : SAtrans_;
+    } else {
+      ms3 = ms2;
+      face2 = PFboth_;
+      ms4 = ms1;
+      act = SAno_;
+    }
+  }
+  if (ms3p) *ms3p = ms3;
+  if (face2p) *face2p = face2;
+  if (ms4p) *ms4p = ms4;
+  if (actp) *actp = act;
 }
 
 std::unique_ptr<SrfDetail> new_detail(int species) {  // surfaceactionalloc
@@ -311,11 +376,9 @@ int SurfaceSet::set_jump(int p1, int f1, int p2, int f2, int bidirectional) {
   return 0;
 }
 
-int SurfaceSet::build(const Geometry& g, int nspecies, const double* difc_soln, double dt, const int* listof_soln,
-                      const bool* mobile_list, int nlist, SurfaceTables& out, char* msg, int msglen) const {
-  (void)difc_soln;
-  (void)dt;
-  (void)listof_soln;
+int SurfaceSet::build(const Geometry& g, int nspecies, const double (*difc_ms)[kSrfMS], double dt,
+                      const int (*lookup_ms)[kSrfMS], const bool* mobile_list, int nlist, SurfaceTables& out,
+                      char* msg, int msglen) const {
   int nsrf = (int)srf.size();
   if (nsrf > kMaxSurf) {
     snprintf(msg, msglen, "more than %d surfaces", kMaxSurf);
@@ -351,9 +414,106 @@ int SurfaceSet::build(const Geometry& g, int nspecies, const double* difc_soln, 
     out.pnl.push_back(d);
   }
 
-  // actions of solution-phase molecules at the front and back faces, with
-  // the probabilities of rate-based actions (surfupdateparams)
+  // probabilities of the rate-based actions (srfcalcprob, smolsurface.c:
+  // 3070-3146, for the cases restated here: given probabilities, and
+  // desorption or state flips by rate without a reverse rate), their
+  // normalization (smolsurface.c:3758-3772) and cumulative sums, and the
+  // reverse probabilities that tell reversible from irreversible desorption
+  struct Probs {
+    double prob[kSrfMS1], cum[kSrfMS1], rev[kSrfMS1];
+  };
+  std::vector<Probs> pr((size_t)nsrf * kMaxSpecies * kSrfMS * 3);
+  auto P = [&](int s, int i, int ms, int f) -> Probs& { return pr[(((size_t)s * kMaxSpecies + i) * kSrfMS + ms) * 3 + f]; };
+  for (int s = 0; s < nsrf; s++) {
+    const HostSurface& S = *srf[s];
+    for (int i = 1; i < nspecies; i++)
+      for (int ms = 0; ms < kSrfMS; ms++)
+        for (int f = 0; f < 3; f++) {
+          const SrfDetail* det = S.det[i][ms][f].get();
+          if (S.action[i][ms][f] != SAmult_ || !det) continue;
+          Probs& q = P(s, i, ms, f);
+          for (int ms2 = 0; ms2 < kSrfMS1; ms2++) {
+            double p = 0;
+            if (ms == MSsoln_ && f == PFnone_) p = 0;
+            else if (det->datasrc[ms2] == 2) p = det->prob[ms2];
+            else if (det->datasrc[ms2] == 1 && det->rate[ms2] != 0) {
+              double rate = det->rate[ms2];
+              if (rate < 0 && rate != -1) {
+                snprintf(msg, msglen, "surface %s: negative rate", S.name.c_str());
+                return 3;
+              }
+              // reverse rate (srfreverseaction, smolsurface.c:661-710)
+              int ms3, face2, ms4, actrev;
+              reverse_action(ms, f, ms2, &ms3, &face2, &ms4, &actrev);
+              int i2 = det->newspec[ms2];
+              double raterev = 0;
+              if (i2 && actrev != SAno_ && ms3 < kSrfMS && face2 < 3) {
+                if (srf[s]->action[i2][ms3][face2] == SAmult_) {
+                  const SrfDetail* b2 = srf[s]->det[i2][ms3][face2].get();
+                  if (b2 && b2->datasrc[ms4] == 1) raterev = b2->rate[ms4];
+                } else if (srf[s]->action[i2][ms3][face2] == actrev) {
+                  raterev = -1;
+                }
+              }
+              if (raterev < 0 && raterev != -1) raterev = 0;
+              if (ms != MSsoln_ && f == PFnone_ && ms2 == ms) {
+                p = 0;
+              } else if (ms != MSsoln_ && f == PFnone_ && raterev == 0) {
+                // desorption or a flip, irreversible (surfaceprob, SurfaceParam.c:159-163)
+                double sum = 0;
+                for (int ms5 = 0; ms5 < kSrfMS1; ms5++)
+                  if (ms5 != ms && det->datasrc[ms5] == 1 && det->rate[ms5] >= 0) sum += det->rate[ms5];
+                if (rate == -1) p = 1;
+                else p = rate / (sum > 0 ? sum : rate) * (1.0 - exp(-(sum > 0 ? sum : rate) * dt));
+              } else {
+                snprintf(msg, msglen,
+                         "surface %s: collision probabilities from rates use the reference's adsorption lookup "
+                         "tables, which the device path does not restate; give them as probabilities "
+                         "(rate_internal)",
+                         S.name.c_str());
+                return 3;
+              }
+            }
+            q.prob[ms2] = p;
+          }
+          // normalization: the probabilities of leaving the state are scaled
+          // down when they exceed 1, else the staying probability is the rest
+          double sum = 0;
+          for (int ms2 = 0; ms2 < kSrfMS1; ms2++)
+            if (!same_state(ms, f, ms2, nullptr)) sum += q.prob[ms2];
+          if (sum > 1.0) {
+            q.prob[ms] = 0;
+            for (int ms2 = 0; ms2 < kSrfMS1; ms2++) q.prob[ms2] /= sum;
+          } else {
+            int ms3;
+            same_state(ms, f, MSsoln_, &ms3);
+            if (ms3 < kSrfMS1) q.prob[ms3] = 1.0 - sum;
+          }
+          sum = 0;
+          for (int ms2 = 0; ms2 < kSrfMS1; ms2++) {
+            sum += q.prob[ms2];
+            q.cum[ms2] = sum;
+          }
+        }
+    for (int i = 1; i < nspecies; i++)
+      for (int ms = 0; ms < kSrfMS; ms++)
+        for (int f = 0; f < 3; f++) {
+          const SrfDetail* det = S.det[i][ms][f].get();
+          if (S.action[i][ms][f] != SAmult_ || !det) continue;
+          for (int ms2 = 0; ms2 < kSrfMS1; ms2++) {
+            int ms3, face2, ms4;
+            reverse_action(ms, f, ms2, &ms3, &face2, &ms4, nullptr);
+            int i2 = det->newspec[ms2];
+            if (face2 != PFboth_ && ms3 < kSrfMS && face2 < 3 && S.det[i][ms3][face2] && S.det[i2][ms3][face2])
+              P(s, i, ms, f).rev[ms2] = P(s, i2, ms3, face2).prob[ms4];
+          }
+        }
+  }
+
+  // device action tables: solution-phase molecules at the front and back
+  // faces; surface-bound molecules (front or back state) with no face
   out.act.assign((size_t)nsrf * kMaxSpecies * 2, DevSrfAct());
+  out.actb.assign((size_t)nsrf * kMaxSpecies * 2, DevSrfAct());
   for (int s = 0; s < nsrf; s++) {
     const HostSurface& S = *srf[s];
     for (int i = 1; i < nspecies; i++)
@@ -377,48 +537,82 @@ int SurfaceSet::build(const Geometry& g, int nspecies, const double* difc_soln, 
           snprintf(msg, msglen, "surface %s: rate-based action without rates", S.name.c_str());
           return 2;
         }
-        double prob[kSrfMS1];
+        const Probs& q = P(s, i, MSsoln_, f);
         for (int ms2 = 0; ms2 < kSrfMS1; ms2++) {
-          if (det->datasrc[ms2] == 2) prob[ms2] = det->prob[ms2];
-          else if (det->datasrc[ms2] == 1 && det->rate[ms2] != 0) {
-            snprintf(msg, msglen,
-                     "surface %s: collision probabilities from rates use the reference's adsorption lookup tables, "
-                     "which the device path does not restate; give them as probabilities (rate_internal)",
-                     S.name.c_str());
-            return 3;
-          } else prob[ms2] = 0;
-        }
-        // normalization (smolsurface.c:3758-3772)
-        double sum = 0;
-        for (int ms2 = 0; ms2 < kSrfMS1; ms2++) {
-          bool same = (f == PFfront_ && ms2 == MSsoln_) || (f == PFback_ && ms2 == MSbsoln_);
-          if (!same) sum += prob[ms2];
-        }
-        if (sum > 1.0) {
-          prob[MSsoln_] = 0;
-          for (int ms2 = 0; ms2 < kSrfMS1; ms2++) prob[ms2] /= sum;
-        } else {
-          prob[f == PFfront_ ? MSsoln_ : MSbsoln_] = 1.0 - sum;
-        }
-        sum = 0;
-        for (int ms2 = 0; ms2 < kSrfMS1; ms2++) {
-          sum += prob[ms2];
-          a.cumprob[ms2] = sum;
+          a.cumprob[ms2] = q.cum[ms2];
           a.newspec[ms2] = det->newspec[ms2];
           int o;
           if (det->newspec[ms2] == 0) o = SAabsorb_;
           else if (f == PFfront_) o = ms2 == MSsoln_ ? SAreflect_ : ms2 == MSbsoln_ ? SAtrans_ : SAadsorb_;
           else o = ms2 == MSsoln_ ? SAtrans_ : ms2 == MSbsoln_ ? SAreflect_ : SAadsorb_;
           a.outcome[ms2] = o;
-          if (prob[ms2] > 0 && (o == SAabsorb_ || o == SAadsorb_)) {
-            snprintf(msg, msglen,
-                     "surface %s: absorption and adsorption outcomes are not implemented on the device path",
+          bool possible = q.prob[ms2] > 0;
+          if (possible && o == SAabsorb_) {
+            snprintf(msg, msglen, "surface %s: absorption outcomes are not implemented on the device path",
                      S.name.c_str());
+            return 2;
+          }
+          if (possible && o == SAadsorb_) {
+            int i2 = det->newspec[ms2];
+            if (ms2 != MSfront_ && ms2 != MSback_) {
+              snprintf(msg, msglen, "surface %s: adsorption to the up/down states is not implemented",
+                       S.name.c_str());
+              return 2;
+            }
+            if (difc_ms[i2][ms2] > 0) {
+              snprintf(msg, msglen, "surface %s: diffusion of surface-bound molecules is not implemented",
+                       S.name.c_str());
+              return 2;
+            }
+          }
+        }
+      }
+    for (int i = 1; i < nspecies; i++)
+      for (int b = 0; b < 2; b++) {
+        const int ms = b == 0 ? MSfront_ : MSback_;
+        DevSrfAct& a = out.actb[((size_t)s * kMaxSpecies + i) * 2 + b];
+        memset(&a, 0, sizeof(a));
+        int act = S.action[i][ms][PFnone_];
+        a.act = act == SAmult_ ? SAmult_ : SAno_;
+        a.newspec_fixed = i;
+        if (act != SAmult_) {
+          if (act != SAno_) {
+            snprintf(msg, msglen, "surface %s: fixed actions of surface-bound molecules are not implemented",
+                     S.name.c_str());
+            return 2;
+          }
+          continue;
+        }
+        const SrfDetail* det = S.det[i][ms][PFnone_].get();
+        if (!det) continue;
+        const Probs& q = P(s, i, ms, PFnone_);
+        for (int ms2 = 0; ms2 < kSrfMS1; ms2++) {
+          a.cumprob[ms2] = q.cum[ms2];
+          a.newspec[ms2] = det->newspec[ms2];
+          int o;
+          if (det->newspec[ms2] == 0) o = SAabsorb_;
+          else if (ms2 == ms) o = SAno_;
+          else if (ms2 == MSsoln_ || ms2 == MSbsoln_) o = q.rev[ms2] > 0 ? SArevdes_ : SAirrevdes_;
+          else o = SAflip_;
+          a.outcome[ms2] = o;
+          if (q.prob[ms2] > 0 && (o == SAabsorb_ || o == SAflip_)) {
+            snprintf(msg, msglen, "surface %s: absorption and state flips of surface-bound molecules are not "
+                     "implemented on the device path", S.name.c_str());
             return 2;
           }
         }
       }
   }
+  // lists holding surface-bound states (srfmollist SMLsrfbound) and the
+  // list of each species in the front and back states
+  for (int ll = 0; ll < kMaxList; ll++) out.listbound[ll] = 0;
+  for (int i = 0; i < kMaxSpecies; i++) out.blist[i][0] = out.blist[i][1] = -1;
+  for (int i = 1; i < nspecies; i++)
+    for (int ms = 1; ms < kSrfMS; ms++) {
+      int ll = lookup_ms[i][ms];
+      if (ll >= 0 && ll < kMaxList) out.listbound[ll] = 1;
+      if (ms == MSfront_ || ms == MSback_) out.blist[i][ms - 1] = ll;
+    }
 
   // box panel lists (boxesupdateparams, smolboxes.c:652-671 with panelinbox
   // :95-152); edge boxes extend to +-1e20 as the reference's do

@@ -45,8 +45,11 @@ struct HostSurface {
 struct SurfaceTables {
   std::vector<DevPanel> pnl;
   std::vector<int> bpstart, bplist;
-  std::vector<DevSrfAct> act;
+  std::vector<DevSrfAct> act;   // solution-phase molecules, [surface][species][face]
+  std::vector<DevSrfAct> actb;  // surface-bound molecules, [surface][species][front/back state]
   int listdiffuse[kMaxList];
+  int listbound[kMaxList];
+  int blist[kMaxSpecies][2];
 };
 
 class SurfaceSet {
@@ -71,8 +74,11 @@ class SurfaceSet {
   // Device tables for a simulation with nspecies species (difc of the
   // solution state), time step dt and the given box grid.  Returns 0, or
   // fills msg and returns nonzero for a model the device path does not run.
-  int build(const Geometry& g, int nspecies, const double* difc_soln, double dt, const int* listof_soln,
-            const bool* mobile_list, int nlist, SurfaceTables& out, char* msg, int msglen) const;
+  // difc_ms / lookup_ms: diffusion coefficient and list of each species in
+  // each molecule state
+  int build(const Geometry& g, int nspecies, const double (*difc_ms)[kSrfMS], double dt,
+            const int (*lookup_ms)[kSrfMS], const bool* mobile_list, int nlist, SurfaceTables& out, char* msg,
+            int msglen) const;
 };
 
 }  // namespace smolamd

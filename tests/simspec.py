@@ -150,6 +150,20 @@ def c3_spec(n=2000, side=100.0, seed=31, kback=5.0, ndisk=8, rdisk=None, bindrad
                 molecules=[("A", n), ("B", n)])
 
 
+def c3_adsorb_spec(n=2000, side=100.0, seed=33, ndisk=8, p_ads=0.3, k_des=2.0, kback=5.0, rdisk=None):
+    """Surfaces workload with surface-bound molecules: c3_spec's sphere and
+    membrane, disk patches on the membrane where A crossing from either side
+    adsorbs with probability p_ads (to the front or back state) and bound A
+    desorbs at rate k_des; bound molecules sit in their own list."""
+    spec = c3_spec(n=n, side=side, seed=seed, kback=kback, ndisk=ndisk, rdisk=rdisk)
+    patches = spec["surfaces"][2]
+    patches["rates"] = [("A", "soln", "fsoln", "front", p_ads, True),
+                        ("A", "soln", "bsoln", "back", p_ads, True),
+                        ("A", "front", "front", "fsoln", k_des, False),
+                        ("A", "back", "back", "bsoln", k_des, False)]
+    return spec
+
+
 def c4_member(k, seed=500, n_lo=1000, n_hi=100000, side_lo=50.0, side_hi=400.0, dt=0.01, scale=1.0):
     """Member k of the batched parameter sweep: A+B<->C in a reflective cube,
     N log-uniform in [n_lo, n_hi] (split evenly between A and B), D

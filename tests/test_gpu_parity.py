@@ -40,8 +40,9 @@ def _run_pair(spec, nsteps, check_every=1, inject=None):
 
 def _compare_counts(spec, osim, dsim):
     """Species counts kept by the device's counters equal the oracle's."""
+    import smoldyn_amd as sa
     for name in spec["species"]:
-        assert dsim.count(name) == osim.count(osim.index(name)), name
+        assert dsim.count(name, sa.MSall) == osim.count(osim.index(name)), name
 
 
 def test_c1_injected_displacements_one_step():
@@ -142,6 +143,29 @@ def test_surfaces_sphere_membrane_patches_bit_exact():
     spec = ss.c3_spec(n=3000, side=60.0, seed=41, kback=20.0)
     nev = _run_pair(spec, 300, check_every=25)
     assert nev > 20
+
+
+def test_surfaces_adsorption_desorption_bit_exact():
+    """Adsorption of A onto disk patches from either side, desorption of the
+    bound molecules (reversible-adsorption desorption distance), the list
+    changes of the first molsort (bound molecules live in their own list) and
+    the serial numbers desorption consumes: identical to the oracle."""
+    spec = ss.c3_adsorb_spec(n=3000, side=60.0, seed=45, p_ads=0.5, k_des=5.0, kback=20.0, ndisk=4, rdisk=6.0)
+    nev = _run_pair(spec, 300, check_every=10)
+    assert nev > 20
+    dsim = ss.build_device(spec)
+    dsim.run_steps(200)
+    import smoldyn_amd as sa
+    assert dsim.count("A", sa.MSfront) + dsim.count("A", sa.MSback) > 0
+    assert dsim.count("A", sa.MSall) == dsim.count("A", sa.MSsoln) + dsim.count("A", sa.MSfront) + \
+        dsim.count("A", sa.MSback)
+
+
+def test_surfaces_adsorption_injected_steps():
+    """Large injected steps with adsorbing patches: several crossings per step."""
+    spec = ss.c3_adsorb_spec(n=1500, side=40.0, seed=46, p_ads=0.5, k_des=20.0, kback=20.0, ndisk=4, rdisk=4.0)
+    spec["dt"] = 0.2
+    _run_pair(spec, 40, check_every=4, inject=True)
 
 
 def test_surfaces_injected_large_steps():
