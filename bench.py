@@ -102,9 +102,10 @@ def c3_surfaces_spec(n_per_species=250000, side=500.0, radius=200.0, ndisk=64, r
     """Surfaces workload: A and B uniform in a reflecting sphere of radius 200
     centred in a cube of side 500; a membrane plane through the centre
     (z = 250) that transmits A and reflects B and C; 64 disks of radius 10 at
-    random non-overlapping centres on it, where A crossing either face turns
-    into D with probability 0.2 (rate_internal); A+B->C with rate 21.42267826
-    (binding radius 0.5), D_A = D_B = 2, D_C = 1, D_D = 2, dt 0.002."""
+    random non-overlapping centres on it, where A crossing either face
+    adsorbs with probability 0.2 (rate_internal) and bound A desorbs at rate
+    0.1; A+B->C with rate 21.42267826 (binding radius 0.5), D_A = D_B = 2,
+    D_C = 1, dt 0.002."""
     rng = np.random.default_rng(seed)
     c = side / 2
     centres = []
@@ -121,14 +122,16 @@ def c3_surfaces_spec(n_per_species=250000, side=500.0, radius=200.0, ndisk=64, r
         return c + v * (radius * 0.999 * rng.uniform(0, 1, n) ** (1 / 3))[:, None]
 
     return dict(low=[0.0] * 3, high=[side] * 3, walls="rrr", dt=0.002, seed=seed,
-                species={"A": 2.0, "B": 2.0, "C": 1.0, "D": 2.0},
+                species={"A": 2.0, "B": 2.0, "C": 1.0},
                 reactions=[("fwd", ["A", "B"], ["C"], 21.42267826)], placement={},
                 surfaces=[("ball", [("both", "all", "reflect")], [], [("sph", None, [c, c, c, radius, 20, 20], "s0")]),
                           ("membrane", [("both", "B", "reflect"), ("both", "C", "reflect")], [],
                            [("rect", "+2", [0.0, 0.0, c, side, side], "plane")]),
                           ("patches", [("both", "B", "reflect"), ("both", "C", "reflect")],
-                           [("A", "soln", "fsoln", "bsoln", 0.2, True, "D"),
-                            ("A", "soln", "bsoln", "fsoln", 0.2, True, "D")],
+                           [("A", "soln", "fsoln", "front", 0.2, True, None),
+                            ("A", "soln", "bsoln", "back", 0.2, True, None),
+                            ("A", "front", "front", "fsoln", 0.1, False, None),
+                            ("A", "back", "back", "bsoln", 0.1, False, None)],
                            [(sh, None, prm, nm) for sh, prm, nm in disks])],
                 molecules_at=[("A", in_sphere(n_per_species)), ("B", in_sphere(n_per_species))])
 
@@ -173,10 +176,12 @@ def surfaces_leg(steps, warmup):
     times = dict(zip(KERNEL_NAMES, [float(x) for x in kt]))
     net = {k: max(v - times["event_overhead"], 0.0) for k, v in times.items() if k != "event_overhead"}
     return {"workload": "500k molecules (250k A + 250k B) in a reflecting sphere r=200, transmitting membrane "
-                        "with 64 disk patches (A->D with probability 0.2 per crossing), A+B->C a=0.5, dt 0.002",
+                        "with 64 disk patches (A adsorbs with probability 0.2 per crossing, desorbs at rate 0.1), "
+                        "A+B->C a=0.5, dt 0.002",
             "value": (d1["molsteps"] - d0["molsteps"]) / wall, "unit": "molecule-steps/s",
             "ms_per_step": wall * 1e3 / steps, "timed_steps": steps, "kernel_ms": net, "setup_s": setup,
-            "counts": {s: sim.count(s) for s in spec["species"]}}
+            "counts": {s: sim.count(s, sa.MSall) for s in spec["species"]},
+            "bound_A": sim.count("A", sa.MSfront) + sim.count("A", sa.MSback)}
 
 
 def build_member(spec, molperbox=None, update=False):

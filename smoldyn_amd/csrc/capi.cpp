@@ -732,6 +732,11 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   for (int order = 1; order <= 2; order++)
     for (auto& rx : sim->rxn[order])
       for (int p = 0; p < rx.nprod; p++) reachable[lookup[rx.prd[p]][MSsoln]] = 1;
+  // with surfaces, molecules can adsorb into the lists of the bound states
+  if (sim->surf.enabled())
+    for (int i = 1; i < nsp; i++)
+      for (int ms = MSfront; ms <= MSback; ms++)
+        if (lookup[i][ms] >= 0) reachable[lookup[i][ms]] = 1;
   long capl = sim->maxmol > 0 ? sim->maxmol : ntot + (ntot / 4 > 4096 ? ntot / 4 : 4096);
   long captot = 0;
   for (int l = 0; l < nlist; l++) {
