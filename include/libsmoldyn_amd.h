@@ -96,6 +96,27 @@ enum ErrorCode smolAddSolutionMolecules(simptr sim, const char* species, int num
                                         double* highposition);                /* libsmoldyn.h:114 */
 int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state); /* libsmoldyn.h:117 */
 
+/* ---------------- run-time commands (libsmoldyn.h:85-95) ----------------
+ * Command timing codes b a @ i x & I j E e N n B A and their scheduling follow
+ * the reference (libSteve/SimCommand.c).  The device path runs the commands
+ * stop, molcount, molcountheader, molcountspecies and molcountspecieslist;
+ * adding any other command returns ECerror.  Observation commands copy the
+ * species counters on the device between steps; rows reach files and data
+ * tables when the run returns (or every 256 steps), in execution order. */
+enum ErrorCode smolSetOutputPath(simptr sim, const char* path);                 /* libsmoldyn.h:87 */
+enum ErrorCode smolAddOutputFile(simptr sim, char* filename, int suffix, int append); /* :88 */
+enum ErrorCode smolAddOutputData(simptr sim, char* dataname);                   /* libsmoldyn.h:89 */
+/* An existing file is not overwritten unless overwrite != 0 (the reference's
+   behavior as a Python module; its command-line build asks on the terminal). */
+enum ErrorCode smolOpenOutputFiles(simptr sim, int overwrite);                  /* libsmoldyn.h:90 */
+enum ErrorCode smolAddCommand(simptr sim, char type, double on, double off, double step, double multiplier,
+                              const char* commandstring);                      /* libsmoldyn.h:92 */
+enum ErrorCode smolAddCommandFromString(simptr sim, char* string);              /* libsmoldyn.h:93 */
+/* *array is allocated with calloc; the caller frees it with free(). */
+enum ErrorCode smolGetOutputData(simptr sim, char* dataname, int* nrow, int* ncol, double** array,
+                                 int erase);                                   /* libsmoldyn.h:94 */
+enum ErrorCode smolRunCommand(simptr sim, const char* commandstring);           /* libsmoldyn.h:95 */
+
 /* ---------------- boundaries and surfaces (libsmoldyn.h:122-136) ---------------- */
 enum ErrorCode smolSetBoundaryType(simptr sim, int dimension, int highside, char type); /* :122 */
 enum ErrorCode smolAddSurface(simptr sim, const char* surface);               /* libsmoldyn.h:123 */
@@ -163,8 +184,18 @@ int smolamdGetReactionLog(simptr sim, long long* rows, int maxrows);
 enum ErrorCode smolamdGetReactionParams(simptr sim, int order, int reaction, double* out);
 /* Virtual-box grid: side[3], min[3], size[3]; returns the box count. */
 int smolamdGetBoxes(simptr sim, int* side, double* boxmin, double* boxsize);
-/* Run nsteps time steps (no stopping at time_stop) with the step captured in
-   a HIP graph when use_graph != 0; *ms receives the device time (HIP events). */
+/* The command scheduler on its own, for checking its timing rules: a script
+   of lines "times tmin tmax dt", "cmd T on off step multiplier text",
+   "cmdstr line" (as smolAddCommandFromString), "dt v", "tmax v", "run"
+   (smolRunSim), "runstep" (smolRunTimeStep), "rununtil t" (smolRunSimUntil)
+   runs with no simulation attached; a command whose text starts with "stop"
+   stops the run.  Each command execution is an event (time, command-list
+   position, iteration); each run ends with an event of position
+   -1000 - (its return code).  Returns the number of events (or -1). */
+int smolamdCommandTimeline(const char* script, int maxev, double* ev_time, int* ev_cmd, long long* ev_iter);
+/* Run nsteps time steps (no stopping at time_stop and no run-time commands)
+   with the step captured in a HIP graph when use_graph != 0; *ms receives the
+   device time (HIP events). */
 enum ErrorCode smolamdRunSteps(simptr sim, long nsteps, int use_graph, double* ms);
 /* Per-kernel average device time (ms) over nsteps timed steps, from HIP
    events on the engine's stream: out[0..5] = diffuse, box scan, scatter,

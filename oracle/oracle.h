@@ -95,6 +95,9 @@ double orxn_unbindingradius(double pgem, double dt, double difc, double a);
 void omath_randtable(double* a, int n);
 double omath_erfn(double x);
 
+/* run-time command scheduler restatement (ocmd.c) driven by a script */
+int ocmd_timeline(const char* script, int maxev, double* et, int* ec, long long* ei);
+
 #ifdef __cplusplus
 }
 #endif

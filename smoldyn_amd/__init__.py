@@ -96,6 +96,15 @@ def lib():
         "smolGetReactionIndexNT": (I, [P, ip, cp]),
         "smolSetReactionRate": (I, [P, cp, D, I]),
         "smolSetReactionProducts": (I, [P, cp, I, D, cp, dp]),
+        "smolSetOutputPath": (I, [P, cp]),
+        "smolAddOutputFile": (I, [P, cp, I, I]),
+        "smolAddOutputData": (I, [P, cp]),
+        "smolOpenOutputFiles": (I, [P, I]),
+        "smolAddCommand": (I, [P, C.c_char, D, D, D, D, cp]),
+        "smolAddCommandFromString": (I, [P, cp]),
+        "smolGetOutputData": (I, [P, cp, ip, ip, C.POINTER(dp), I]),
+        "smolRunCommand": (I, [P, cp]),
+        "smolamdCommandTimeline": (I, [cp, I, dp, ip, C.POINTER(C.c_longlong)]),
         "smolamdAddMoleculesAt": (I, [P, cp, I, dp]),
         "smolamdInjectDisplacements": (I, [P, dp, L64]),
         "smolamdSetPrecision": (I, [P, I]),
@@ -136,6 +145,27 @@ def last_error():
     msg = C.create_string_buffer(512)
     code = lib().smolGetError(fn, msg, 1)
     return code, fn.value.decode(), msg.value.decode()
+
+
+_libc = C.CDLL(None)
+_libc.free.argtypes = [C.c_void_p]
+_libc.free.restype = None
+
+
+def command_timeline(script, maxev=200000):
+    """Run the library's command scheduler on a script (smolamdCommandTimeline):
+    returns (times, command positions, iterations) of every execution, with a
+    position of -1000 - code at the end of each run."""
+    L = lib()
+    t = np.zeros(maxev)
+    c = np.zeros(maxev, np.int32)
+    it = np.zeros(maxev, np.int64)
+    n = L.smolamdCommandTimeline(script.encode(), maxev, _dp(t), c.ctypes.data_as(C.POINTER(C.c_int)),
+                                 it.ctypes.data_as(C.POINTER(C.c_longlong)))
+    if n < 0:
+        raise SmoldynError("smolamdCommandTimeline: bad script")
+    n = min(n, maxev)
+    return t[:n], c[:n], it[:n]
 
 
 def _check(code, what):
@@ -244,6 +274,42 @@ class Simulation:
         st = (C.c_int * max(1, len(products)))(*([MSsoln] * len(products)))
         _check(self.L.smolAddReaction(self.p, name.encode(), r1, MSsoln, r2, MSsoln, len(products), arr, st, rate),
                "smolAddReaction")
+
+    # ---- run-time commands and output (libsmoldyn.h:85-95)
+    def set_output_path(self, path):
+        _check(self.L.smolSetOutputPath(self.p, str(path).encode()), "smolSetOutputPath")
+
+    def add_output_file(self, name, suffix=-1, append=False):
+        _check(self.L.smolAddOutputFile(self.p, name.encode(), suffix, 1 if append else 0), "smolAddOutputFile")
+
+    def add_output_data(self, name):
+        _check(self.L.smolAddOutputData(self.p, name.encode()), "smolAddOutputData")
+
+    def open_output_files(self, overwrite=False):
+        _check(self.L.smolOpenOutputFiles(self.p, 1 if overwrite else 0), "smolOpenOutputFiles")
+
+    def add_command(self, command, timing, on=0.0, off=0.0, step=0.0, multiplier=0.0):
+        _check(self.L.smolAddCommand(self.p, timing.encode()[:1], on, off, step, multiplier, command.encode()),
+               "smolAddCommand")
+
+    def add_command_str(self, line):
+        _check(self.L.smolAddCommandFromString(self.p, line.encode()), "smolAddCommandFromString")
+
+    def output_data(self, name, erase=True):
+        """Rows of data table `name` as an (nrow, ncol) array (smolGetOutputData)."""
+        nr, nc = C.c_int(0), C.c_int(0)
+        arr = C.POINTER(C.c_double)()
+        _check(self.L.smolGetOutputData(self.p, name.encode(), C.byref(nr), C.byref(nc), C.byref(arr),
+                                        1 if erase else 0), "smolGetOutputData")
+        try:
+            n = nr.value * nc.value
+            out = np.ctypeslib.as_array(arr, shape=(n,)).copy() if n > 0 else np.zeros(0)
+        finally:
+            _libc.free(C.cast(arr, C.c_void_p))
+        return out.reshape(nr.value, nc.value)
+
+    def run_command(self, command):
+        return _check(self.L.smolRunCommand(self.p, command.encode()), "smolRunCommand")
 
     def product_placement(self, name, kind, param=0.0):
         _check(self.L.smolSetReactionProducts(self.p, name.encode(), RP[kind], param, None, None),

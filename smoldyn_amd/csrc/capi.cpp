@@ -19,6 +19,7 @@
 
 #include "../../include/libsmoldyn_amd.h"
 #include "../../include/smoldyn_amd_rng.h"
+#include "commands.h"
 #include "engine.h"
 #include "rxnparam.h"
 #include "surfaces_host.h"
@@ -93,6 +94,8 @@ struct smolamd_sim {
   std::vector<double> gtable;
   int counts[kMaxSpecies] = {0};
   SurfaceSet surf;
+  cmd::Commands cmds;  // run-time commands, output files and data tables
+  int obsrows = 0;     // observation rows queued on the device since the last flush
   // device inputs of the last update, kept for ensembles
   struct Prepared {
     EngineConfig cfg;
@@ -858,39 +861,101 @@ int run_one(simptr sim, const double* inject, long ninject, const char* fn, Erro
   return 0;
 }
 
-// run until time reaches tmax or tbreak, as smolsimulate does, in batches of
-// device steps between host checks
-ErrorCode run_until(simptr sim, const char* fn) {
-  ErrorCode ec = do_update(sim, fn);
-  if (ec != ECok) return ec;
-  sim->started = true;
-  for (;;) {
-    // count the steps the reference would take: time += dt until a stop
-    long n = 0;
-    double tt = sim->time;
-    int code = 0;
-    while (n < 256) {
-      tt += sim->dt;
-      n++;
-      if (tt >= sim->tmax) {
-        code = 1;
-        break;
+// Write the queued command outputs: wait for the device, read the
+// observation rows, format them.  Returns the device error bits.
+int flush_outputs(simptr sim) {
+  int bits = sim->dev ? engine_finish(sim->dev) : 0;
+  std::vector<cmd::ObsRow> rows;
+  if (sim->obsrows > 0 && sim->dev) {
+    std::vector<int> buf((size_t)sim->obsrows * kObsStride);
+    if (engine_read_obs(sim->dev, sim->obsrows, buf.data()) == 0) {
+      rows.resize((size_t)sim->obsrows);
+      for (int r = 0; r < sim->obsrows; r++) {
+        const int* o = buf.data() + (size_t)r * kObsStride;
+        rows[(size_t)r].step = (uint32_t)o[0];
+        rows[(size_t)r].count.assign(o + 1, o + 1 + kMaxSpecies);
+        rows[(size_t)r].countf.assign(o + 1 + kMaxSpecies, o + 1 + 2 * kMaxSpecies);
+        rows[(size_t)r].countk.assign(o + 1 + 2 * kMaxSpecies, o + 1 + 3 * kMaxSpecies);
       }
-      if (tt >= sim->tbreak) {
-        code = 10;
-        break;
-      }
+    } else if (!bits) {
+      bits = -1;
     }
-    int bits = engine_run(sim->dev, n, 1);
-    if (bits) return engine_error(fn, bits);
-    sim->time = tt;
-    sim->stepindex += (uint32_t)n;
-    if (code == 1) {
-      set_error(fn, ECnotify, "Simulation complete");
-      return ECok;
-    }
-    if (code == 10) return ECok;
   }
+  cmd::flush(sim->cmds, rows, sim->spname);
+  sim->obsrows = 0;
+  return bits;
+}
+
+// smolsimulate on the device (smolsim.cpp:2747-2760): commands run between
+// steps; an observation enqueues a copy of the species counters on the
+// stream, so the steps are never waited for except every 256 steps (error
+// check) and when the observation table is full.  Returns the reference's
+// code (1 tmax, 7 stopped by a command, 10 tbreak) or -1 after a device error
+// (*ec set).
+int simulate_device(simptr sim, const char* fn, ErrorCode* ec) {
+  *ec = do_update(sim, fn);
+  if (*ec != ECok) return -1;
+  sim->started = true;
+  cmd::update(sim->cmds, sim->tmin, sim->tmax, sim->dt);
+  cmd::Clock k;
+  k.tmin = sim->tmin;
+  k.tmax = sim->tmax;
+  k.dt = sim->dt;
+  k.time = sim->time;
+  k.tbreak = sim->tbreak;
+  const double t0 = sim->time;
+  const uint32_t s0 = sim->stepindex;
+  int bits = 0;
+  long since = 0;
+  auto observe = [&](uint32_t* step) -> int {
+    if (sim->obsrows >= kObsRows) {
+      int b = flush_outputs(sim);
+      if (b && !bits) bits = b;
+    }
+    if (engine_observe(sim->dev, sim->obsrows)) return -1;
+    *step = sim->stepindex;
+    return sim->obsrows++;
+  };
+  auto ex = [&](cmd::Command& c, double time) -> int {
+    std::string err;
+    int code = cmd::run_command(sim->cmds, c, time, sim->spname, observe, &err);
+    if (code == cmd::kWarn) fprintf(stderr, "command '%s' error: %s\n", c.str.c_str(), err.c_str());
+    return code;
+  };
+  auto step = [&]() -> int {
+    if (bits) return -1;
+    if (engine_enqueue(sim->dev, 1)) {
+      bits = -1;
+      return -1;
+    }
+    sim->stepindex++;
+    if (++since >= 256) {
+      since = 0;
+      int b = flush_outputs(sim);
+      if (b) {
+        bits = b;
+        return -1;
+      }
+    }
+    return 0;
+  };
+  int code = cmd::simulate(sim->cmds, k, step, ex);
+  int b = flush_outputs(sim);
+  if (b && !bits) bits = b;
+  sim->time = k.time;
+  if (bits) {
+    // the device stops at the failing step: the clock follows its step counter
+    Control c;
+    if (engine_sync_control(sim->dev, &c) == 0) {
+      sim->stepindex = c.step;
+      double t = t0;
+      for (uint32_t q = s0; q != c.step; q++) t += sim->dt;
+      sim->time = t;
+    }
+    *ec = engine_error(fn, bits);
+    return -1;
+  }
+  return code;
 }
 
 }  // namespace
@@ -979,33 +1044,44 @@ enum ErrorCode smolUpdateSim(simptr sim) {
   return do_update(sim, fn);
 }
 
+// libsmoldyn.cpp:217-238: one step through smolsimulate with the break time
+// half a step ahead (the break time stays set, as in the reference)
 enum ErrorCode smolRunTimeStep(simptr sim) {
   const char* fn = "smolRunTimeStep";
   LCHECK(sim, fn, ECmissing, "missing sim");
+  sim->tbreak = sim->time + sim->dt / 2;
   ErrorCode ec;
-  int code = run_one(sim, nullptr, 0, fn, &ec);
+  int code = simulate_device(sim, fn, &ec);
   if (code < 0) return ec;
   LCHECK(code != 1, fn, ECnotify, "Simulation complete");
+  LCHECK(code != 7, fn, ECnotify, "Simulation stopped by a runtime command");
   return g_warncode;
 }
 
+// libsmoldyn.cpp:242-270: output files are opened (overwriting) first
 enum ErrorCode smolRunSim(simptr sim) {
   const char* fn = "smolRunSim";
   LCHECK(sim, fn, ECmissing, "missing sim");
-  ErrorCode ec = run_until(sim, fn);
-  if (ec < ECwarning) return ec;
+  char msg[kStrLen];
+  LCHECK(cmd::open_files(sim->cmds, true, msg, (int)sizeof(msg)) == 0, fn, ECerror,
+         "Cannot open output files for writing");
+  ErrorCode ec;
+  int code = simulate_device(sim, fn, &ec);
+  if (code < 0) return ec;
+  LCHECK(code != 1, fn, ECnotify, "Simulation complete");
+  LCHECK(code != 7, fn, ECnotify, "Simulation stopped by a runtime command");
   if (g_warncode == ECnotify) g_warncode = ECok;
   return g_warncode;
 }
 
+// libsmoldyn.cpp:274-284: the break time is the stop time afterwards
 enum ErrorCode smolRunSimUntil(simptr sim, double breaktime) {
   const char* fn = "smolRunSimUntil";
   LCHECK(sim, fn, ECmissing, "missing sim");
-  double keep = sim->tbreak;
+  const double stoptime = sim->tmax;
   sim->tbreak = breaktime;
-  ErrorCode ec = run_until(sim, fn);
-  sim->tbreak = keep;
-  if (ec < ECwarning) return ec;
+  smolRunSim(sim);
+  sim->tbreak = stoptime;
   return g_warncode;
 }
 
@@ -1024,6 +1100,7 @@ enum ErrorCode smolSetSimTimes(simptr sim, double timestart, double timestop, do
   sim->tmin = sim->time = timestart;
   sim->tmax = timestop;
   sim->dt = timestep;
+  cmd::set_condition(sim->cmds, 1, 0);
   invalidate(sim);
   return ECok;
 }
@@ -1055,6 +1132,7 @@ enum ErrorCode smolSetTimeStep(simptr sim, double timestep) {
   LCHECK(timestep > 0, fn, ECbounds, "timestep must be >0");
   LCHECK(!sim->started, fn, ECerror, "the time step is fixed once stepping started");
   sim->dt = timestep;
+  cmd::set_condition(sim->cmds, 1, 0);
   invalidate(sim);
   return ECok;
 }
@@ -1275,6 +1353,215 @@ int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state)
     return (int)ECnonexist;
   }
   return counts[i];
+}
+
+// ---- run-time commands and output (libsmoldyn.cpp:656-798)
+
+enum ErrorCode smolSetOutputPath(simptr sim, const char* path) {
+  const char* fn = "smolSetOutputPath";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(path, fn, ECmissing, "missing path");
+  sim->cmds.froot = path;
+  return ECok;
+}
+
+enum ErrorCode smolAddOutputFile(simptr sim, char* filename, int suffix, int append) {
+  const char* fn = "smolAddOutputFile";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(filename, fn, ECmissing, "missing filename");
+  LCHECK(!strchr(filename, ' '), fn, ECwarning, "only first word of filename is used");
+  cmd::add_files(sim->cmds, filename, append != 0);
+  if (suffix >= 0) LCHECK(!cmd::set_suffix(sim->cmds, filename, suffix), fn, ECbug, "scmdsetfsuffix bug");
+  return g_warncode;
+}
+
+enum ErrorCode smolAddOutputData(simptr sim, char* dataname) {
+  const char* fn = "smolAddOutputData";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(dataname, fn, ECmissing, "missing dataname");
+  LCHECK(!strchr(dataname, ' '), fn, ECwarning, "only first word of dataname is used");
+  cmd::add_data(sim->cmds, dataname);
+  return g_warncode;
+}
+
+enum ErrorCode smolOpenOutputFiles(simptr sim, int overwrite) {
+  const char* fn = "smolOpenOutputFiles";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  char msg[kStrLen];
+  LCHECK(cmd::open_files(sim->cmds, overwrite != 0, msg, (int)sizeof(msg)) == 0, fn, ECerror,
+         "Unable (or forbidden) to open output files for writing");
+  return ECok;
+}
+
+enum ErrorCode smolAddCommand(simptr sim, char type, double on, double off, double step, double multiplier,
+                              const char* commandstring) {
+  const char* fn = "smolAddCommand";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(cmd::supported(commandstring), fn, ECerror,
+         "command '%s' is not run by the device engine (stop, molcount, molcountheader, molcountspecies, "
+         "molcountspecieslist are)",
+         commandstring ? commandstring : "");
+  int er = cmd::add_command(sim->cmds, type, on, off, step, multiplier, commandstring);
+  LCHECK(er != 3, fn, ECsyntax, "missing command string");
+  return ECok;
+}
+
+enum ErrorCode smolAddCommandFromString(simptr sim, char* string) {
+  const char* fn = "smolAddCommandFromString";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(string, fn, ECmissing, "missing string");
+  cmd::Commands probe;
+  int er = cmd::add_command_string(probe, string);
+  if (er == 0)
+    LCHECK(cmd::supported(probe.list[0].c.str.c_str()), fn, ECerror,
+           "command '%s' is not run by the device engine (stop, molcount, molcountheader, molcountspecies, "
+           "molcountspecieslist are)",
+           probe.list[0].c.str.c_str());
+  er = cmd::add_command_string(sim->cmds, string);
+  LCHECK(er != 3, fn, ECsyntax, "cmd format: type [on off dt] string");
+  LCHECK(er != 5, fn, ECbounds, "cmd time step needs to be >0");
+  return ECok;
+}
+
+enum ErrorCode smolGetOutputData(simptr sim, char* dataname, int* nrow, int* ncol, double** array, int erase) {
+  const char* fn = "smolGetOutputData";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(dataname, fn, ECmissing, "missing dataname");
+  LCHECK(nrow && ncol && array, fn, ECmissing, "missing pointer for returned data");
+  LCHECK(!sim->cmds.data.empty(), fn, ECerror, "no data files in the sim");
+  int did = cmd::find_data(sim->cmds, dataname);
+  LCHECK(did >= 0, fn, ECerror, "no data file of the requested name");
+  cmd::Commands::Data& d = sim->cmds.data[(size_t)did];
+  const int nr = (int)d.rows.size(), nc = d.ncol;
+  double* out = (double*)calloc((size_t)(nr * nc > 0 ? nr * nc : 1), sizeof(double));
+  LCHECK(out, fn, ECmemory, "out of memory");
+  for (int i = 0; i < nr; i++)
+    for (size_t j = 0; j < d.rows[(size_t)i].size(); j++) out[(size_t)i * nc + j] = d.rows[(size_t)i][j];
+  *nrow = nr;
+  *ncol = nc;
+  *array = out;
+  if (erase) {
+    d.rows.clear();
+    d.ncol = 0;
+  }
+  return ECok;
+}
+
+enum ErrorCode smolRunCommand(simptr sim, const char* commandstring) {
+  const char* fn = "smolRunCommand";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(commandstring, fn, ECmissing, "missing command string");
+  if (sim->dev) flush_outputs(sim);
+  std::vector<cmd::ObsRow> rows;
+  auto observe = [&](uint32_t* step) -> int {
+    cmd::ObsRow r;
+    r.count.assign(kMaxSpecies, 0);
+    r.countf.assign(kMaxSpecies, 0);
+    r.countk.assign(kMaxSpecies, 0);
+    if (sim->dev && sim->ok) {
+      Control c;
+      if (engine_sync_control(sim->dev, &c)) return -1;
+      for (int i = 0; i < kMaxSpecies; i++) {
+        r.count[(size_t)i] = c.count[i];
+        r.countf[(size_t)i] = c.countf[i];
+        r.countk[(size_t)i] = c.countk[i];
+      }
+      r.step = c.step;
+    } else {
+      for (auto& pm : sim->pending) r.count[(size_t)pm.ident]++;
+      r.step = sim->stepindex;
+    }
+    *step = r.step;
+    rows.push_back(r);
+    return (int)rows.size() - 1;
+  };
+  cmd::Command c;
+  c.str = commandstring;
+  std::string err;
+  int code = cmd::run_command(sim->cmds, c, sim->time, sim->spname, observe, &err);
+  cmd::flush(sim->cmds, rows, sim->spname);
+  LCHECK(code == cmd::kOk, fn, ECwarning, "%s", err.empty() ? "command returned a control code" : err.c_str());
+  return ECok;
+}
+
+// The command scheduler driven by a script, with no simulation attached (see
+// the header for the script lines).  Each execution is an event (time,
+// command-list position, iteration); each run adds an event with position
+// -1000 - (its return code).  Returns the number of events, -1 on a bad line.
+int smolamdCommandTimeline(const char* script, int maxev, double* ev_time, int* ev_cmd, long long* ev_iter) {
+  if (!script) return -1;
+  cmd::Commands cs;
+  cmd::Clock k;
+  int nev = 0;
+  auto record = [&](double t, int c, long long it) {
+    if (nev < maxev) {
+      if (ev_time) ev_time[nev] = t;
+      if (ev_cmd) ev_cmd[nev] = c;
+      if (ev_iter) ev_iter[nev] = it;
+    }
+    nev++;
+  };
+  cmd::Exec ex = [&](cmd::Command& c, double time) -> int {
+    int pos = -1;
+    for (size_t i = 0; i < cs.list.size(); i++)
+      if (cs.list[i].c.str == c.str) {
+        pos = (int)i;
+        break;
+      }
+    record(time, pos, cs.iter - 1);
+    return c.str.compare(0, 4, "stop") == 0 ? cmd::kStop : cmd::kOk;
+  };
+  cmd::Step step = []() { return 0; };
+  auto run = [&]() {
+    cmd::update(cs, k.tmin, k.tmax, k.dt);
+    int code = cmd::simulate(cs, k, step, ex);
+    record(k.time, -1000 - code, cs.iter);
+  };
+  std::string text(script);
+  size_t a = 0;
+  while (a < text.size()) {
+    size_t b = text.find('\n', a);
+    if (b == std::string::npos) b = text.size();
+    std::string line = text.substr(a, b - a);
+    a = b + 1;
+    char word[32] = {0};
+    int nread = 0;
+    if (sscanf(line.c_str(), "%31s%n", word, &nread) != 1) continue;
+    const char* rest = line.c_str() + nread;
+    if (!strcmp(word, "times")) {
+      if (sscanf(rest, "%lf %lf %lf", &k.tmin, &k.tmax, &k.dt) != 3) return -1;
+      k.time = k.tmin;
+      cmd::set_condition(cs, 1, 0);
+    } else if (!strcmp(word, "cmd")) {
+      char t = 0;
+      double on, off, st, mu;
+      int n2 = 0;
+      if (sscanf(rest, " %c %lf %lf %lf %lf %n", &t, &on, &off, &st, &mu, &n2) != 5) return -1;
+      cmd::add_command(cs, t, on, off, st, mu, rest + n2);
+    } else if (!strcmp(word, "cmdstr")) {
+      cmd::add_command_string(cs, rest);
+    } else if (!strcmp(word, "dt")) {
+      if (sscanf(rest, "%lf", &k.dt) != 1) return -1;
+      cmd::set_condition(cs, 1, 0);
+    } else if (!strcmp(word, "tmax")) {
+      if (sscanf(rest, "%lf", &k.tmax) != 1) return -1;
+    } else if (!strcmp(word, "run")) {
+      run();
+    } else if (!strcmp(word, "runstep")) {
+      k.tbreak = k.time + k.dt / 2;
+      run();
+    } else if (!strcmp(word, "rununtil")) {
+      double t;
+      if (sscanf(rest, "%lf", &t) != 1) return -1;
+      const double stop = k.tmax;
+      k.tbreak = t;
+      run();
+      k.tbreak = stop;
+    } else {
+      return -1;
+    }
+  }
+  return nev;
 }
 
 enum ErrorCode smolSetBoundaryType(simptr sim, int dimension, int highside, char type) {

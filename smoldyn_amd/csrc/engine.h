@@ -247,6 +247,15 @@ int engine_run(DeviceState* s, long nsteps, int use_graph);
 // diffuse, box scan, scatter, box sort, pair search, resolve.
 int engine_profile(DeviceState* s, long nsteps, double* out);
 int engine_sync_control(DeviceState* s, Control* out);
+// Asynchronous stepping for the command loop: enqueue steps, enqueue a copy of
+// the species counters into observation row `row` (kObsStride ints: step,
+// count, countf, countk), then wait and read the error bits and the rows.
+constexpr int kObsRows = 4096;
+constexpr int kObsStride = 1 + 3 * kMaxSpecies;
+int engine_enqueue(DeviceState* s, long nsteps);
+int engine_observe(DeviceState* s, int row);
+int engine_finish(DeviceState* s);
+int engine_read_obs(DeviceState* s, int nrow, int* out);
 int engine_get_list(DeviceState* s, int ll, int* ident, double* pos, uint64_t* serno, int* box, int max);
 int engine_get_log(DeviceState* s, LogRow* rows, int maxrows);
 double engine_last_ms(DeviceState* s);

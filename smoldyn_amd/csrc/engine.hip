@@ -199,6 +199,7 @@ struct DeviceState {
   int graph_ready[2] = {0, 0};
   hipEvent_t ev[16] = {};
   double last_ms = 0;
+  int* obs = nullptr;                // observation rows (species counters between steps)
 };
 
 // ---------------------------------------------------------------------------
@@ -2042,7 +2043,7 @@ void engine_destroy(DeviceState* s) {
   void* ptrs[] = {s->inv,     s->bn.count, s->bn.part, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges,  s->cand,  s->estate,
                   s->accepted, s->bestkey, s->killed,  s->uniev,   s->log,       s->sortkey, s->sortval, s->killslot,
                   s->ctl,     s->tab,     s->gtab,    s->inj,     s->prods,     s->d_pnl,   s->d_bpstart,
-                  s->d_bplist, s->d_act,  s->d_actb,  s->moves,   s->dinst,   s->glob};
+                  s->d_bplist, s->d_act,  s->d_actb,  s->moves,   s->dinst,   s->glob,    s->obs};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; k++)
@@ -2269,6 +2270,57 @@ int engine_run(DeviceState* s, long nsteps, int use_graph) {
   HIPOK(hipEventElapsedTime(&ms, e0, e1));
   s->last_ms = ms;
   return read_errors(s);
+}
+
+// Species counters of simulation 0 into one observation row:
+// [step, count[kMaxSpecies], countf[kMaxSpecies], countk[kMaxSpecies]]
+__global__ __launch_bounds__(64) void k_observe(const Control* __restrict__ ctl, int* __restrict__ row) {
+  const int t = threadIdx.x;
+  if (t == 0) row[0] = (int)ctl->step;
+  if (t < kMaxSpecies) {
+    row[1 + t] = ctl->count[t];
+    row[1 + kMaxSpecies + t] = ctl->countf[t];
+    row[1 + 2 * kMaxSpecies + t] = ctl->countk[t];
+  }
+}
+
+int engine_enqueue(DeviceState* s, long nsteps) {
+  for (int k = 0; k < 2; k++) {
+    if (s->graph_ready[k]) continue;
+    hipGraph_t g;
+    HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    launch_step(s, k, nullptr, 0);
+    HIPOK(hipStreamEndCapture(s->stream, &g));
+    HIPOK(hipGraphInstantiate(&s->graph[k], g, nullptr, nullptr, 0));
+    HIPOK(hipGraphDestroy(g));
+    s->graph_ready[k] = 1;
+  }
+  for (long k = 0; k < nsteps; k++) {
+    HIPOK(hipGraphLaunch(s->graph[s->cur], s->stream));
+    s->cur ^= 1;
+  }
+  return 0;
+}
+
+int engine_observe(DeviceState* s, int row) {
+  if (row < 0 || row >= kObsRows) return -2;
+  if (!s->obs && dalloc(&s->obs, (size_t)kObsRows * kObsStride)) return -1;
+  hipLaunchKernelGGL(k_observe, dim3(1), dim3(64), 0, s->stream, s->ctl, s->obs + (size_t)row * kObsStride);
+  HIPOK(hipGetLastError());
+  return 0;
+}
+
+int engine_finish(DeviceState* s) {
+  HIPOK(hipStreamSynchronize(s->stream));
+  return read_errors(s);
+}
+
+int engine_read_obs(DeviceState* s, int nrow, int* out) {
+  if (nrow <= 0) return 0;
+  if (nrow > kObsRows || !s->obs) return -2;
+  HIPOK(hipStreamSynchronize(s->stream));
+  HIPOK(hipMemcpy(out, s->obs, sizeof(int) * (size_t)nrow * kObsStride, hipMemcpyDeviceToHost));
+  return 0;
 }
 
 // Steps with events between the kernels; per-kernel average ms in out[]:

@@ -42,6 +42,7 @@ def lib():
         dp = C.POINTER(C.c_double)
         ip = C.POINTER(C.c_int)
         sig = {
+            "ocmd_timeline": (I, [C.c_char_p, I, dp, ip, C.POINTER(C.c_longlong)]),
             "osim_new": (P, [I, dp, dp]),
             "osim_free": (None, [P]),
             "osim_set_wall": (I, [P, I, I, C.c_char]),
@@ -339,3 +340,17 @@ def reflib():
         f.restype = res
         f.argtypes = args
     return L
+
+
+def command_timeline(script, maxev=200000):
+    """The oracle's command scheduler on a script: (times, command positions,
+    iterations) of every execution, -1000 - code at the end of each run."""
+    L = lib()
+    t = np.zeros(maxev)
+    c = np.zeros(maxev, np.int32)
+    it = np.zeros(maxev, np.int64)
+    n = L.ocmd_timeline(script.encode(), maxev, t.ctypes.data_as(C.POINTER(C.c_double)),
+                        c.ctypes.data_as(C.POINTER(C.c_int)), it.ctypes.data_as(C.POINTER(C.c_longlong)))
+    assert n >= 0, "bad script"
+    n = min(n, maxev)
+    return t[:n], c[:n], it[:n]

@@ -1,0 +1,154 @@
+"""Run-time commands on the device: molcount rows recorded between steps
+without host syncs equal the oracle's species counts after the same steps,
+and the reference's own command tests (tests/test_data_in_array.py,
+tests/test_data_outputfile.py) restated on the device engine."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import simspec as ss
+import smoldyn_amd as sa
+
+pytestmark = pytest.mark.gpu
+
+
+def _steps_to(tmin, tmax, dt):
+    t, n = tmin, 0
+    while True:
+        t += dt
+        n += 1
+        if t >= tmax:
+            return n
+
+
+def _oracle_counts(spec, nsteps):
+    osim, _ = ss.build_oracle(spec)
+    names = list(spec["species"])
+    out = []
+    for k in range(nsteps + 1):
+        if k:
+            osim.step()
+        out.append([osim.count(osim.index(n)) for n in names])
+    return out
+
+
+def test_molcount_rows_equal_oracle_counts():
+    """Reversible A+B<->C (periodic, ratio 1.2 unbinding) at 10k molecules:
+    'N 3 molcount d' and 'E molcountspecieslist s A(all) C' through smolRunSim;
+    every row (time and counts) equals the oracle's state after that many
+    steps, the times bit for bit."""
+    dt = 0.005
+    spec = ss.c2_spec(n=5000, side=1000.0 * (10000 / 1e6) ** (1 / 3), kback=20.0, seed=13, tmax=40 * dt)
+    dsim = ss.build_device(spec, update=False)
+    dsim.add_output_data("d")
+    dsim.add_output_data("s")
+    dsim.add_command("molcount d", "N", step=3)
+    dsim.add_command("molcountspecieslist s A(all) C", "E")
+    assert dsim.run() == sa.ECok
+    d = dsim.output_data("d")
+    s = dsim.output_data("s")
+    t, c, it = ob.command_timeline("times 0 %r %r\ncmd N 0 0 3 0 a\ncmd E 0 0 0 0 b\nrun\n" % (spec["tmax"], dt))
+    nsteps = _steps_to(0.0, spec["tmax"], dt)
+    counts = _oracle_counts(spec, nsteps)
+    exp_d = [[t[j]] + counts[min(it[j], nsteps)] for j in range(len(t)) if c[j] == 0]
+    exp_s = [[t[j], counts[min(it[j], nsteps)][0], counts[min(it[j], nsteps)][2]] for j in range(len(t)) if c[j] == 1]
+    assert d.shape == (len(exp_d), 4) and s.shape == (len(exp_s), 3)
+    assert np.array_equal(d, np.array(exp_d, dtype=np.float64))
+    assert np.array_equal(s, np.array(exp_s, dtype=np.float64))
+    assert d[-1][1] < 5000  # reactions happened
+
+
+def test_single_steps_record_before_and_after():
+    """smolRunTimeStep runs the commands before and after its step: rows at
+    steps 0,1,1,2,2,3, each equal to the oracle's counts."""
+    dt = 0.01
+    spec = ss.c1_spec(n=3000, seed=17, tmax=1.0)
+    dsim = ss.build_device(spec, update=False)
+    dsim.add_output_data("d")
+    dsim.add_command("molcount d", "E")
+    for _ in range(3):
+        dsim.run_time_step()
+    d = dsim.output_data("d")
+    counts = _oracle_counts(spec, 3)
+    clock = [0.0, dt, dt + dt, dt + dt + dt]
+    steps = [0, 1, 1, 2, 2, 3]
+    assert np.array_equal(d, np.array([[clock[k]] + counts[k] for k in steps], dtype=np.float64))
+
+
+def test_stop_command_ends_the_run():
+    spec = ss.c1_spec(n=2000, seed=19, tmax=1.0)
+    dsim = ss.build_device(spec, update=False)
+    dsim.add_output_data("d")
+    dsim.add_command("molcount d", "E")
+    dsim.add_command("stop", "@", on=0.25)
+    assert dsim.run() == sa.ECok
+    d = dsim.output_data("d")
+    clock = [0.0]
+    while clock[-1] < 0.25:
+        clock.append(clock[-1] + 0.01)
+    k = len(clock) - 1
+    # rows at steps 0..k and once more in the end-of-run pass
+    assert list(d[:, 0]) == clock + [clock[-1]]
+    assert dsim.diagnostics()["step"] == k
+
+
+def test_reference_data_in_array_model():
+    """tests/test_data_in_array.py: R -> 0 at rate 0.1 for 400 R plus one B
+    at the centre in a reflective cube of side 100, 'E molcount mydata',
+    stop 10, dt 0.01.  1001 rows of 3 columns, first row [0, 400, 1]; the
+    last row at time 10 with R near 400 exp(-1) (the reference's SFMT draws
+    give 142; the counter stream gives another draw of the same law)."""
+    s = sa.Simulation([0, 0, 0], [100, 100, 100])
+    for d in range(3):
+        s.boundary(d, 0, "r")
+        s.boundary(d, 1, "r")
+    s.times(0.0, 10.0, 0.01)
+    s.seed(100)
+    for name in ("R", "B"):
+        s.species(name)
+        s.mobility(name, 1.0)
+    s.reaction("r1", ["R"], [], 0.1)
+    s.add_molecules("R", 400)
+    s.add_molecules("B", 1, [50, 50, 50], [50, 50, 50])
+    s.add_output_data("mydata")
+    s.add_command("molcount mydata", "E")
+    assert s.run() == sa.ECok
+    data = s.output_data("mydata", erase=False)
+    assert data.shape == (1001, 3)
+    assert list(data[0]) == [0.0, 400.0, 1.0]
+    assert abs(data[-1][0] - 10.0) < 1e-3 * 10
+    p = np.exp(-1.0)
+    sd = np.sqrt(400 * p * (1 - p))
+    assert abs(data[-1][1] - 400 * p) < 4 * sd, data[-1]
+    assert np.all(np.diff(data[:, 1]) <= 0) and np.all(data[:, 2] == 1)
+
+
+def test_reference_data_outputfile_model(tmp_path):
+    """tests/test_data_outputfile.py: C <-> A + B (kf 0.1, kb 100) in a
+    periodic cube of side 100 with boxes of 10, 1000 A and 1000 B, 'i 0 100 10
+    molcount box.dat', dt 0.1 to 100: an 11 x 4 file whose column means are
+    [50.036, 590.727, 590.727, 409.272] within 10 %."""
+    s = sa.Simulation([0, 0, 0], [100, 100, 100])
+    for d in range(3):
+        s.boundary(d, 0, "p")
+        s.boundary(d, 1, "p")
+    s.times(0.0, 100.0, 0.1)
+    s.seed(7)
+    s.partitions("boxsize", 10.0)
+    for name in ("A", "B", "C"):
+        s.species(name)
+        s.mobility(name, 1.0)
+    s.reaction("r1fwd", ["C"], ["A", "B"], 0.1)
+    s.reaction("r1rev", ["A", "B"], ["C"], 100.0)
+    s.add_molecules("A", 1000)
+    s.add_molecules("B", 1000)
+    s.set_output_path(str(tmp_path) + "/")
+    s.add_output_file("box.dat")
+    s.add_command("molcount box.dat", "i", on=0, off=100, step=10)
+    assert s.run() == sa.ECok
+    s.close()
+    data = np.loadtxt(tmp_path / "box.dat")
+    assert data.shape == (11, 4)
+    expected = [50.036, 590.727, 590.727, 409.272]
+    assert np.isclose(data.mean(axis=0), expected, atol=1e-1, rtol=1e-1).all(), data.mean(axis=0)
+    assert round(float(data[:, 0].mean()), 3) == 50.036
