@@ -332,6 +332,38 @@ def sweep_leg(members, steps, warmup, sim_steps=2000):
                           "max_rate_error_bisection": float(err_n.max())}}
 
 
+def commands_leg(spec, steps, every=10, warmup=20):
+    """The same workload through smolRunSim with a 'N every molcount' command
+    (species counts recorded on the device between steps, rows written when
+    the run returns): wall-clock ms per step against graph-replayed steps
+    with no commands, and the last row against smolGetMoleculeCount."""
+    import smoldyn_amd as sa
+    sim = build_device(spec)
+    sim.run_steps(warmup)
+    t = 0.0
+    for _ in range(warmup):
+        t += spec["dt"]
+    t_start = t
+    for _ in range(steps):
+        t += spec["dt"]
+    _check = sa._check
+    _check(sim.L.smolSetTimeStop(sim.p, t), "smolSetTimeStop")
+    sim.add_output_data("counts")
+    sim.add_command("molcount counts", "N", step=every)
+    t0 = time.perf_counter()
+    code = sim.run()
+    wall = time.perf_counter() - t0
+    rows = sim.output_data("counts")
+    last = [sim.count(name, sa.MSall) for name in spec["species"]]
+    t1 = time.perf_counter()
+    sim.run_steps(steps)
+    wall_plain = time.perf_counter() - t1
+    return {"steps": steps, "every": every, "rows": int(rows.shape[0]), "code": int(code),
+            "first_time": float(rows[0][0]) if len(rows) else None, "start_time": t_start,
+            "ms_per_step": wall / steps * 1e3, "ms_per_step_no_commands": wall_plain / steps * 1e3,
+            "last_row_equals_counts": bool(len(rows) and list(rows[-1][1:]) == [float(x) for x in last])}
+
+
 def precision_leg(spec, steps, warmup, check_steps=100):
     """FP32 box-offset records on the same workload: throughput and per-kernel
     times, then the error against FP64 records from the same start: the max
@@ -451,6 +483,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--surface-steps", type=int, default=1000,
                     help="timed steps of the surfaces workload (0: skip)")
+    ap.add_argument("--command-steps", type=int, default=2000,
+                    help="steps of the smolRunSim leg with a molcount command (0 skips it)")
     ap.add_argument("--precision-steps", type=int, default=2000,
                     help="timed steps of the FP32 box-offset leg (0: skip)")
     ap.add_argument("--sweep-members", type=int, default=4096,
@@ -534,6 +568,9 @@ def main():
     if args.surface_steps > 0:
         log("surfaces workload, %d steps" % args.surface_steps)
         out["surfaces"] = surfaces_leg(args.surface_steps, 20)
+    if args.command_steps > 0:
+        log("run-time commands: molcount every 10 steps through smolRunSim, %d steps" % args.command_steps)
+        out["commands"] = commands_leg(spec, args.command_steps)
     if args.precision_steps > 0:
         log("FP32 box-offset records, %d steps" % args.precision_steps)
         out["fp32_offsets"] = precision_leg(spec, args.precision_steps, args.warmup)

@@ -99,8 +99,8 @@ int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state)
 /* ---------------- run-time commands (libsmoldyn.h:85-95) ----------------
  * Command timing codes b a @ i x & I j E e N n B A and their scheduling follow
  * the reference (libSteve/SimCommand.c).  The device path runs the commands
- * stop, molcount, molcountheader, molcountspecies and molcountspecieslist;
- * adding any other command returns ECerror.  Observation commands copy the
+ * stop, molcount, molcountheader, molcountspecies, molcountspecieslist,
+ * listmols and listmols2; adding any other command returns ECerror.  Observation commands copy the
  * species counters on the device between steps; rows reach files and data
  * tables when the run returns (or every 256 steps), in execution order. */
 enum ErrorCode smolSetOutputPath(simptr sim, const char* path);                 /* libsmoldyn.h:87 */

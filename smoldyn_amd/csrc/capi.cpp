@@ -886,6 +886,47 @@ int flush_outputs(simptr sim) {
   return bits;
 }
 
+// Every molecule for listmols, in molscancmd's order (smolmolec.c:1189-1205):
+// lists in order, live order within a list.  Queued outputs are written first.
+int dump_molecules(simptr sim, std::vector<cmd::MolRow>* out) {
+  out->clear();
+  if (sim->dev && sim->ok) {
+    flush_outputs(sim);
+    const int nl = (int)sim->listname.size();
+    for (int ll = 0; ll < nl; ll++) {
+      int n = engine_get_list_inst(sim->dev, 0, ll, nullptr, nullptr, nullptr, nullptr, 0);
+      if (n < 0) return 1;
+      if (n == 0) continue;
+      std::vector<int> id((size_t)n), ms((size_t)n);
+      std::vector<double> pos((size_t)n * 3);
+      std::vector<uint64_t> ser((size_t)n);
+      if (engine_get_list_inst(sim->dev, 0, ll, id.data(), pos.data(), ser.data(), nullptr, n, ms.data()) < 0)
+        return 1;
+      for (int k = 0; k < n; k++) {
+        cmd::MolRow m;
+        m.ident = id[(size_t)k];
+        m.ms = ms[(size_t)k];
+        for (int d = 0; d < 3; d++) m.pos[d] = pos[(size_t)k * 3 + (size_t)d];
+        m.serno = ser[(size_t)k];
+        out->push_back(m);
+      }
+    }
+    return 0;
+  }
+  // before the first update: the molecules added so far, list by list
+  for (int ll = -1; ll < (int)sim->listname.size(); ll++)
+    for (auto& pm : sim->pending) {
+      const int l2 = sim->listlookup[(size_t)pm.ident][MSsoln];
+      if (l2 != ll) continue;
+      cmd::MolRow m;
+      m.ident = pm.ident;
+      for (int d = 0; d < 3; d++) m.pos[d] = pm.pos[d];
+      m.serno = pm.serno;
+      out->push_back(m);
+    }
+  return 0;
+}
+
 // smolsimulate on the device (smolsim.cpp:2747-2760): commands run between
 // steps; an observation enqueues a copy of the species counters on the
 // stream, so the steps are never waited for except every 256 steps (error
@@ -916,9 +957,14 @@ int simulate_device(simptr sim, const char* fn, ErrorCode* ec) {
     *step = sim->stepindex;
     return sim->obsrows++;
   };
+  auto dump = [&](std::vector<cmd::MolRow>* mols) -> int {
+    int b = flush_outputs(sim);
+    if (b && !bits) bits = b;
+    return bits ? 1 : dump_molecules(sim, mols);
+  };
   auto ex = [&](cmd::Command& c, double time) -> int {
     std::string err;
-    int code = cmd::run_command(sim->cmds, c, time, sim->spname, observe, &err);
+    int code = cmd::run_command(sim->cmds, c, time, sim->spname, observe, dump, &err);
     if (code == cmd::kWarn) fprintf(stderr, "command '%s' error: %s\n", c.str.c_str(), err.c_str());
     return code;
   };
@@ -1399,7 +1445,7 @@ enum ErrorCode smolAddCommand(simptr sim, char type, double on, double off, doub
   LCHECK(sim, fn, ECmissing, "missing sim");
   LCHECK(cmd::supported(commandstring), fn, ECerror,
          "command '%s' is not run by the device engine (stop, molcount, molcountheader, molcountspecies, "
-         "molcountspecieslist are)",
+         "molcountspecieslist, listmols, listmols2 are)",
          commandstring ? commandstring : "");
   int er = cmd::add_command(sim->cmds, type, on, off, step, multiplier, commandstring);
   LCHECK(er != 3, fn, ECsyntax, "missing command string");
@@ -1415,7 +1461,7 @@ enum ErrorCode smolAddCommandFromString(simptr sim, char* string) {
   if (er == 0)
     LCHECK(cmd::supported(probe.list[0].c.str.c_str()), fn, ECerror,
            "command '%s' is not run by the device engine (stop, molcount, molcountheader, molcountspecies, "
-           "molcountspecieslist are)",
+           "molcountspecieslist, listmols, listmols2 are)",
            probe.list[0].c.str.c_str());
   er = cmd::add_command_string(sim->cmds, string);
   LCHECK(er != 3, fn, ECsyntax, "cmd format: type [on off dt] string");
@@ -1475,10 +1521,15 @@ enum ErrorCode smolRunCommand(simptr sim, const char* commandstring) {
     rows.push_back(r);
     return (int)rows.size() - 1;
   };
+  auto dump = [&](std::vector<cmd::MolRow>* mols) -> int {
+    cmd::flush(sim->cmds, rows, sim->spname);
+    rows.clear();
+    return dump_molecules(sim, mols);
+  };
   cmd::Command c;
   c.str = commandstring;
   std::string err;
-  int code = cmd::run_command(sim->cmds, c, sim->time, sim->spname, observe, &err);
+  int code = cmd::run_command(sim->cmds, c, sim->time, sim->spname, observe, dump, &err);
   cmd::flush(sim->cmds, rows, sim->spname);
   LCHECK(code == cmd::kOk, fn, ECwarning, "%s", err.empty() ? "command returned a control code" : err.c_str());
   return ECok;

@@ -503,17 +503,81 @@ int parse_species(const std::vector<std::string>& spname, const char* word, int*
 bool supported(const char* str) {
   std::string w = first_word(str);
   return w.empty() || w == "stop" || w == "molcount" || w == "molcountheader" || w == "molcountspecies" ||
-         w == "molcountspecieslist";
+         w == "molcountspecieslist" || w == "listmols" || w == "listmols2";
+}
+
+namespace {
+
+// molserno2string (smolmolec.c:625-630)
+void write_serno(FILE* fp, uint64_t s) {
+  if (!fp) return;
+  if (s < 0xFFFFFFFFull) fprintf(fp, "%llu", (unsigned long long)s);
+  else fprintf(fp, "%llu.%llu", (unsigned long long)(s >> 32), (unsigned long long)(s & 0xFFFFFFFFull));
+}
+
+const char* ms_name(int ms) {  // molms2string (smolmolec.c:93-103)
+  static const char* names[] = {"solution", "front", "back", "up", "down", "bsoln", "all", "none", "some"};
+  return ms >= 0 && ms <= 8 ? names[ms] : "none";
+}
+
+}  // namespace
+
+// cmdlistmols (smolcmd.c:2044-2075): "name(state) x y z serno", data rows
+// ident, state, x, y, z, serno.  cmdlistmols2 (:2079-2110): the command's
+// execution count, ident and state as numbers first.
+static int listmols(Commands& c, const Command& cmd, bool two, const char* line2,
+                    const std::vector<std::string>& spname, const Dump& dump, std::string* err) {
+  FILE* fp = nullptr;
+  int did = -1;
+  if (target(c, line2, 3, &fp, &did) == -1) {
+    if (err) *err = "file or data name not recognized";
+    return kWarn;
+  }
+  std::vector<MolRow> mols;
+  if (dump(&mols)) {
+    if (err) *err = "reading the molecules from the device failed";
+    return kWarn;
+  }
+  for (const MolRow& m : mols) {
+    if (m.ident <= 0) continue;
+    if (two) {
+      if (fp) fprintf(fp, "%lld", (long long)cmd.invoke);
+      write_sep(c, fp);
+      if (fp) fprintf(fp, "%i", m.ident);
+      write_sep(c, fp);
+      if (fp) fprintf(fp, "%i", m.ms);
+      data_append(c, did, true, (double)cmd.invoke);
+      data_append(c, did, false, (double)m.ident);
+      data_append(c, did, false, (double)m.ms);
+    } else {
+      if (fp) fprintf(fp, "%s(%s)", m.ident < (int)spname.size() ? spname[(size_t)m.ident].c_str() : "?",
+                      ms_name(m.ms));
+      data_append(c, did, true, (double)m.ident);
+      data_append(c, did, false, (double)m.ms);
+    }
+    for (int d = 0; d < 3; d++) {
+      write_sep(c, fp);
+      write_num(c, fp, m.pos[d]);
+      data_append(c, did, false, m.pos[d]);
+    }
+    write_sep(c, fp);
+    write_serno(fp, m.serno);
+    data_append(c, did, false, (double)m.serno);
+    if (fp) fputc('\n', fp);
+  }
+  if (fp) fflush(fp);
+  return kOk;
 }
 
 // docommand (smolcmd.c:175-320) for the commands of the device path
 int run_command(Commands& c, const Command& cmd, double time, const std::vector<std::string>& spname,
-                const Observe& observe, std::string* err) {
+                const Observe& observe, const Dump& dump, std::string* err) {
   const char* line = cmd.str.c_str();
   std::string w = first_word(line);
   if (w.empty()) return kOk;
   const char* line2 = word_at(line, 2);
   if (w == "stop") return kStop;  // cmdstop (smolcmd.c:326-330)
+  if (w == "listmols" || w == "listmols2") return listmols(c, cmd, w == "listmols2", line2, spname, dump, err);
   Pending p;
   p.time = time;
   auto fail = [&](const char* m) {

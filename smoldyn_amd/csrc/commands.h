@@ -145,15 +145,26 @@ int flush(Commands& c, const std::vector<ObsRow>& rows, const std::vector<std::s
 int parse_species(const std::vector<std::string>& spname, const char* word, int* ms);
 
 // ---- commands of the device path: stop, molcount, molcountheader,
-// molcountspecies, molcountspecieslist
+// molcountspecies, molcountspecieslist, listmols, listmols2
 bool supported(const char* str);
 // Observe returns the observation row of the current state and the device
 // step counter the row will carry, or -1.
 using Observe = std::function<int(uint32_t* step)>;
-// Run one command: a control code, or an observation queued in c.pending.
-// kWarn with a message in err when the command cannot run.
+// One molecule of a listmols dump.
+struct MolRow {
+  int ident = 0, ms = 0;
+  double pos[3] = {0, 0, 0};
+  uint64_t serno = 0;
+};
+// Dump writes out every queued output, then returns every molecule in the
+// order the reference's molscancmd visits them (lists in order, live order
+// within a list); 0 on success.
+using Dump = std::function<int(std::vector<MolRow>* mols)>;
+// Run one command: a control code, or an observation queued in c.pending
+// (listmols output is written at once, after the queued outputs).  kWarn with
+// a message in err when the command cannot run.
 int run_command(Commands& c, const Command& cmd, double time, const std::vector<std::string>& spname,
-                const Observe& observe, std::string* err);
+                const Observe& observe, const Dump& dump, std::string* err);
 
 }  // namespace cmd
 }  // namespace smolamd

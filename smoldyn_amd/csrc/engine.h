@@ -232,8 +232,9 @@ int engine_upload_list_inst(DeviceState* s, int inst, int ll, int n, const int* 
                             const uint64_t* serno);
 int engine_set_counters_inst(DeviceState* s, int inst, uint64_t serno_next, uint32_t step, const int* counts);
 int engine_get_control(DeviceState* s, int inst, Control* out);
+// mstate (optional): MolecState of each molecule (0 solution, 1 front, 2 back)
 int engine_get_list_inst(DeviceState* s, int inst, int ll, int* ident, double* pos, uint64_t* serno, int* box,
-                         int max);
+                         int max, int* mstate = nullptr);
 void engine_destroy(DeviceState* s);
 // Upload molecules of list ll in live order (host arrays), replacing its content.
 int engine_upload_list(DeviceState* s, int ll, int n, const int* ident, const double* pos, const uint64_t* serno);

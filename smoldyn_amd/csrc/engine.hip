@@ -2359,7 +2359,7 @@ static int current_nrec(DeviceState* s) {
 }
 
 int engine_get_list_inst(DeviceState* s, int inst, int ll, int* ident, double* pos, uint64_t* serno, int* box,
-                         int max) {
+                         int max, int* mstate) {
   Control c;
   if (engine_get_control(s, inst, &c)) return -1;
   const Inst& I = s->hinst[inst];
@@ -2401,9 +2401,15 @@ int engine_get_list_inst(DeviceState* s, int inst, int ll, int* ident, double* p
     HIPOK(hipMemcpy(z.data(), R.pz, sizeof(double) * nr, hipMemcpyDeviceToHost));
   }
   HIPOK(hipMemcpy(sr.data(), R.serno, sizeof(uint64_t) * nr, hipMemcpyDeviceToHost));
+  std::vector<int> sb;
+  if (mstate && R.sb) {
+    sb.resize((size_t)nr);
+    HIPOK(hipMemcpy(sb.data(), R.sb, sizeof(int) * nr, hipMemcpyDeviceToHost));
+  }
   for (int k = 0; k < n; k++) {
     int r = inv[k];
     if (r < 0 || r >= nr) return -3;
+    if (mstate) mstate[k] = sb.empty() ? 0 : (sb[(size_t)r] & 3);  // a bound product keeps its state in its old list
     if (ident) ident[k] = id[r];
     if (serno) serno[k] = sr[r];
     if (box) box[k] = bx[r];

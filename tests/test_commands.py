@@ -266,7 +266,7 @@ def test_output_data_errors():
 def test_commands_outside_the_device_path_are_refused():
     s = _sim()
     with pytest.raises(sa.SmoldynError):
-        s.add_command("listmols out.txt", "E")
+        s.add_command("molpos A out.txt", "E")
     with pytest.raises(sa.SmoldynError):
         s.add_command_str("E pause")
     s.add_command("molcount", "E")
@@ -307,3 +307,26 @@ def test_file_and_data_in_one_command(tmp_path):
     s.close()
     assert list(d[0]) == [0.0, 7.0, 3.0, 0.0]
     assert (tmp_path / "f.txt").read_text() == "0 7 3 0\n"
+
+
+def test_listmols_formats(tmp_path):
+    """cmdlistmols: "name(state) x y z serial" per molecule in list order;
+    listmols2: execution count, ident, state, position, serial."""
+    s = sa.Simulation([0, 0, 0], [10, 10, 10])
+    s.times(0.0, 1.0, 0.1)
+    for name in ("A", "B"):
+        s.species(name)
+        s.mobility(name, 1.0)
+    s.add_molecules_at("A", [[1, 2, 3], [4.5, 5, 6]])
+    s.add_molecules_at("B", [[7, 8, 9.25]])
+    s.set_output_path(str(tmp_path) + "/")
+    s.add_output_file("lm.txt")
+    s.add_output_data("m")
+    s.open_output_files(overwrite=True)
+    s.run_command("listmols lm.txt")
+    s.run_command("listmols2 m")
+    d = s.output_data("m")
+    s.close()
+    assert (tmp_path / "lm.txt").read_text() == ("A(solution) 1 2 3 1\nA(solution) 4.5 5 6 2\n"
+                                                 "B(solution) 7 8 9.25 3\n")
+    assert d.tolist() == [[0, 1, 0, 1, 2, 3, 1], [0, 1, 0, 4.5, 5, 6, 2], [0, 2, 0, 7, 8, 9.25, 3]]

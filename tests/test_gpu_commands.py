@@ -152,3 +152,36 @@ def test_reference_data_outputfile_model(tmp_path):
     expected = [50.036, 590.727, 590.727, 409.272]
     assert np.isclose(data.mean(axis=0), expected, atol=1e-1, rtol=1e-1).all(), data.mean(axis=0)
     assert round(float(data[:, 0].mean()), 3) == 50.036
+
+
+def test_listmols_equals_oracle_listmols(tmp_path):
+    """'N 5 listmols lm.txt' (smolcmd.c cmdlistmols) on the device writes the
+    same text, byte for byte, as the oracle's listmols (whose format is pinned
+    by the reference's S95 regression outputs) after the same steps; the data
+    table of 'listmols2' at iteration 7 holds the same molecules."""
+    dt = 0.005
+    spec = ss.c2_spec(n=1500, side=1000.0 * (3000 / 1e6) ** (1 / 3), kback=20.0, seed=23, tmax=12 * dt)
+    dsim = ss.build_device(spec, update=False)
+    dsim.set_output_path(str(tmp_path) + "/")
+    dsim.add_output_file("lm.txt")
+    dsim.add_output_data("d")
+    dsim.add_command("listmols lm.txt", "N", step=5)
+    dsim.add_command("listmols2 d", "&", on=7)
+    assert dsim.run() == sa.ECok
+    d = dsim.output_data("d")
+    dsim.close()
+    osim, _ = ss.build_oracle(spec)
+    text, rows7 = "", None
+    for k in range(11):
+        if k:
+            osim.step()
+        if k % 5 == 0:
+            text += osim.listmols()
+        if k == 7:
+            rows7 = []
+            for ll in range(osim.nlist()):
+                L = osim.get_list(ll)
+                for j in range(len(L["ident"])):
+                    rows7.append([1, L["ident"][j], L["mstate"][j], *L["pos"][j], L["serno"][j]])
+    assert (tmp_path / "lm.txt").read_text() == text
+    assert np.array_equal(d, np.array(rows7, dtype=np.float64))
