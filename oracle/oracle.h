@@ -95,6 +95,9 @@ double orxn_unbindingradius(double pgem, double dt, double difc, double a);
 void omath_randtable(double* a, int n);
 double omath_erfn(double x);
 
+int osim_set_confspread(osim* sim, int r, double radius);
+int osim_set_prdpos(osim* sim, int order, int r, int prd, const double* pos);
+
 /* run-time command scheduler restatement (ocmd.c) driven by a script */
 int ocmd_timeline(const char* script, int maxev, double* et, int* ec, long long* ei);
 

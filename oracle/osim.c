@@ -370,6 +370,32 @@ int osim_set_revparam(osim* sim, int order, int r, int rparamt, double rparam) {
   return 0;
 }
 
+/* RxnSetValue "confspreadrad" (smolreact.c:1762-1765), the confspread_radius
+   statement (smolsim.cpp:1736-1755): a conformational-spread reaction whose
+   reactants react within the given radius */
+int osim_set_confspread(osim* sim, int r, double radius) {
+  orxn* rxn;
+  if (r < 0 || r >= sim->rxnss[2].totrxn || radius < 0) return 1;
+  rxn = sim->rxnss[2].rxn[r];
+  rxn->rparamt = ORP_CONFSPREAD;
+  rxn->bindrad2 = radius * radius;
+  sim->ok = 0;
+  return 0;
+}
+
+/* the position argument of RxnSetRevparam for offset and fixed placement
+   (smolreact.c:1870-1874) */
+int osim_set_prdpos(osim* sim, int order, int r, int prd, const double* pos) {
+  orxn* rxn;
+  int d;
+  if (order < 1 || order > 2 || r < 0 || r >= sim->rxnss[order].totrxn) return 1;
+  rxn = sim->rxnss[order].rxn[r];
+  if (prd < 0 || prd >= rxn->nprod || !pos) return 2;
+  for (d = 0; d < sim->dim; d++) rxn->prdpos[prd][d] = pos[d];
+  sim->ok = 0;
+  return 0;
+}
+
 int osim_set_rxn_prob(osim* sim, int order, int r, double prob) {
   if (order < 1 || order > 2 || r < 0 || r >= sim->rxnss[order].totrxn) return 1;
   sim->rxnss[order].rxn[r]->prob = prob;
@@ -1209,8 +1235,24 @@ static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, i
     mp->mstate = rxn->prdstate[prd];
     mp->list = sim->listlookup[mp->ident][mp->mstate];
     mp->box = rxnbox;
-    for (d = 0; d < dim; d++) mp->posx[d] = rxnpos[d];
     mp->pnl = mp->pnlx = -1;
+    if (rxn->rparamt == ORP_CONFSPREAD && order == 2) {
+      /* confspread (smolreact.c:2736-2751): the first two products take the
+         reactants' places and serial numbers, others the reaction position */
+      if (prd < 2) {
+        omol* from = prd == 0 ? m1 : m2;
+        mp->box = from->box;
+        mp->serno = from->serno;
+        for (d = 0; d < dim; d++) {
+          mp->posx[d] = mp->pos[d] = from->pos[d];
+          mp->posoffset[d] = from->posoffset[d];
+        }
+      } else {
+        for (d = 0; d < dim; d++) mp->posx[d] = mp->pos[d] = rxnpos[d];
+      }
+      continue;
+    }
+    for (d = 0; d < dim; d++) mp->posx[d] = rxnpos[d];
     for (d = 0; d < dim && rxn->prdpos[prd][d] == 0; d++)
       ;
     if (d != dim) {

@@ -311,9 +311,18 @@ class Simulation:
     def run_command(self, command):
         return _check(self.L.smolRunCommand(self.p, command.encode()), "smolRunCommand")
 
-    def product_placement(self, name, kind, param=0.0):
-        _check(self.L.smolSetReactionProducts(self.p, name.encode(), RP[kind], param, None, None),
-               "smolSetReactionProducts")
+    def product_placement(self, name, kind, param=0.0, product=None, position=None):
+        """smolSetReactionProducts; offset and fixed placement take one
+        product's position per call."""
+        pos = None if position is None else np.ascontiguousarray(position, np.float64)
+        _check(self.L.smolSetReactionProducts(self.p, name.encode(), RP[kind], param,
+                                              product.encode() if product else None,
+                                              None if pos is None else _dp(pos)), "smolSetReactionProducts")
+
+    def binding_radius(self, name, radius):
+        """smolSetReactionRate with isinternal 1: the binding radius (for
+        confspread reactions, the conformational-spread radius)."""
+        _check(self.L.smolSetReactionRate(self.p, name.encode(), radius, 1), "smolSetReactionRate")
 
     def reaction_index(self, name):
         order = C.c_int(-1)

@@ -350,7 +350,13 @@ int set_product(simptr sim, int order, int r) {
     if (rparamt == RPnone || rparamt == RPirrev || rparamt == RPconfspread) rx.unbindrad = 0;
     else return 1;
   } else if (rparamt == RPoffset || rparamt == RPfixed) {
-    return 12;  // not supported on the device path
+    // the offsets are the caller's; unbinding radius as rxnsetproduct computes it (smolreact.c:1451-1458)
+    double dist = 0;
+    if (nprod == 1)
+      for (int d = 0; d < dim; d++) dist = rx.prdpos[0][d];
+    else
+      for (int d = 0; d < dim; d++) dist += rx.prdpos[0][d] - rx.prdpos[1][d];
+    rx.unbindrad = sqrt(dist);
   } else if (nprod == 1) {
     if (rparamt == RPnone || rparamt == RPirrev || rparamt == RPconfspread) {
       rx.unbindrad = 0;
@@ -641,7 +647,7 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
       dr.bindrad2 = rx.bindrad2;
       dr.prob = rx.prob;
       dr.confspread = rx.rparamt == RPconfspread;
-      LCHECK(!dr.confspread, fn, ECerror, "conformational spread reactions are not implemented on the device path");
+      dr.fixed = rx.rparamt == RPfixed;
       dr.permit_soln = order == 1 ? rx.permit[MSsoln] : rx.permit[MSsoln * kMS1 + MSsoln];
       for (int p = 0; p < rx.nprod; p++) {
         dr.prd[p] = rx.prd[p];
@@ -649,8 +655,12 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
                "surface-bound products need the surface path");
         dr.prdlist[p] = lookup[rx.prd[p]][MSsoln];
         dr.prdpos[p] = rx.prdpos[p][0];
+        for (int d = 0; d < 3; d++) dr.prdvec[p][d] = rx.prdpos[p][d];
+        if (dr.fixed) continue;
         if (rx.prdpos[p][0] != 0) dr.rotate = 1;
-        LCHECK(rx.prdpos[p][1] == 0 && rx.prdpos[p][2] == 0, fn, ECerror, "off-axis product offsets");
+        LCHECK(rx.prdpos[p][1] == 0 && rx.prdpos[p][2] == 0, fn, ECerror,
+               "off-axis product offsets of offset placement (a random rotation matrix per event) are not "
+               "implemented on the device path; fixed placement and offsets along x are");
       }
     }
   // pair-search screening tables: which partner species react, and the
@@ -1891,22 +1901,50 @@ enum ErrorCode smolSetReactionRate(simptr sim, const char* reaction, double rate
   return ECok;
 }
 
+// libsmoldyn.cpp:1990-2020 with RxnSetRevparam (smolreact.c:1854-1880)
 enum ErrorCode smolSetReactionProducts(simptr sim, const char* reaction, enum RevParam method, double parameter,
                                        const char* product, double* position) {
   const char* fn = "smolSetReactionProducts";
-  (void)product;
-  (void)position;
   LCHECK(sim, fn, ECmissing, "missing sim");
   int order = -1;
   int r = smolGetReactionIndexNT(sim, &order, reaction);
   LCHECK(r >= 0, fn, ECnonexist, "reaction not found");
-  LCHECK(method != RPoffset && method != RPfixed && method != RPbounce && method != RPconfspread, fn, ECerror,
-         "this product placement is not implemented on the device path");
+  LCHECK(method != RPbounce, fn, ECerror, "bounce product placement is not implemented on the device path");
   Rxn& rx = sim->rxn[order][r];
+  int prd = -1;
+  if (product && product[0]) {
+    int i = species_index(sim, product);
+    LCHECK(i > 0, fn, ECnonexist, "product species not found");
+    for (int p = 0; p < rx.nprod && prd < 0; p++)
+      if (rx.prd[p] == i) prd = p;
+    LCHECK(prd >= 0, fn, ECerror, "listed product is not a product of the listed reaction");
+  }
+  int er = rx.rparamt != RPnone ? 1 : 0;
   rx.rparamt = method;
-  if (method != RPnone && method != RPirrev) rx.rparam = parameter;
+  if (method == RPnone || method == RPirrev || method == RPconfspread) {
+  } else if (method == RPpgem || method == RPpgemmax || method == RPpgemmaxw || method == RPpgem2 ||
+             method == RPpgemmax2) {
+    if (!(parameter > 0 && parameter <= 1)) er = 2;
+    rx.rparam = parameter;
+  } else if (method == RPratio || method == RPunbindrad || method == RPratio2) {
+    if (parameter < 0) er = 2;
+    rx.rparam = parameter;
+  } else if (method == RPoffset || method == RPfixed) {
+    er = 0;
+    if (prd < 0) er = 4;
+    else if (!position) er = 5;
+    else
+      for (int d = 0; d < sim->dim; d++) rx.prdpos[prd][d] = position[d];
+  } else {
+    er = 3;
+  }
   invalidate(sim);
-  return ECok;
+  LCHECK(er != 1, fn, ECwarning, "reaction product parameter was set before");
+  LCHECK(er != 2, fn, ECbounds, "reaction product parameter out of bounds");
+  LCHECK(er != 3, fn, ECnonexist, "invalid reaction product method");
+  LCHECK(er != 4, fn, ECmissing, "missing product name");
+  LCHECK(er != 5, fn, ECmissing, "missing product position");
+  return g_warncode;
 }
 
 // ---------------------------------------------------------------------------

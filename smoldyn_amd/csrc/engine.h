@@ -32,9 +32,11 @@ struct DevRxn {
   double bindrad2;
   double prob;
   double prdpos[kMaxProd];  // axial product offset (prdpos[prd][0])
+  double prdvec[kMaxProd][3];  // product offset vectors of fixed placement
   int rotate;               // 1 if any product offset is nonzero
   int permit_soln;          // permitted for solution reactant(s)
-  int confspread;
+  int confspread;           // products 0 and 1 take the reactants' places and serial numbers
+  int fixed;                // products at the reaction position plus prdvec, not rotated
 };
 
 struct DevTables {

@@ -99,6 +99,7 @@ struct Prod {
   double x, y, z;
   int sb;  // surfaces: a product adsorbed by the reborn check (pending its list change), else 0
   int pad;
+  uint64_t serno;  // a confspread product keeps its reactant's serial number; 0 for a new serial
 };
 
 // Per-species screening constants of the pair search, passed by value.
@@ -1516,6 +1517,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   int pc = pbase;
   for (int e = e0; e < e1; e++) {
     double rxnpos[3];
+    double pAv[3] = {0, 0, 0}, pBv[3] = {0, 0, 0};  // reactant places, in the reaction's reactant order
     const DevRxn* rx;
     uint64_t sA, sB = 0;
     int order, ridx;
@@ -1568,7 +1570,11 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       double dc1 = tab->difc[iA], dc2 = tab->difc[iB];
       double x = (dc1 == 0 && dc2 == 0) ? 0.5 : dc2 / (dc1 + dc2);
 #pragma unroll
-      for (int d = 0; d < 3; d++) rxnpos[d] = x * pA[d] + (1.0 - x) * pB[d];
+      for (int d = 0; d < 3; d++) {
+        rxnpos[d] = x * pA[d] + (1.0 - x) * pB[d];
+        pAv[d] = pA[d];
+        pBv[d] = pB[d];
+      }
       sA = B.serno[swap ? ed.s2 : ed.s1];
       sB = B.serno[swap ? ed.s1 : ed.s2];
       int k2 = U + 2 * (e - U);
@@ -1592,8 +1598,22 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       pr.list = rx->prdlist[q];
       pr.sb = 0;
       pr.pad = 0;
+      pr.serno = 0;
       double pp[3] = {rxnpos[0], rxnpos[1], rxnpos[2]};
-      if (rx->prdpos[q] != 0) {
+      if (rx->confspread && order == 2) {
+        // doreact (smolreact.c:2736-2751): products 0 and 1 replace the reactants
+        if (q < 2) {
+          const double* src = q == 0 ? pAv : pBv;
+          pp[0] = src[0];
+          pp[1] = src[1];
+          pp[2] = src[2];
+          pr.serno = q == 0 ? sA : sB;
+        }
+      } else if (rx->fixed) {  // fixed placement: offsets are not rotated (smolreact.c:2811-2812)
+        pp[0] = rxnpos[0] + rx->prdvec[q][0];
+        pp[1] = rxnpos[1] + rx->prdvec[q][1];
+        pp[2] = rxnpos[2] + rx->prdvec[q][2];
+      } else if (rx->prdpos[q] != 0) {
         double v1[3] = {rx->prdpos[q] * uvec[0], rx->prdpos[q] * uvec[1], rx->prdpos[q] * uvec[2]};
         pp[0] = rxnpos[0] + v1[0];
         pp[1] = rxnpos[1] + v1[1];
@@ -1604,7 +1624,10 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       } else {
         // checksurfaces for reborn molecules: segment reaction position -> product
         int i2 = pr.ident, bnd = 0;
-        int st = srf_check(I.sf, I.g, tab->difc, pp, rxnpos, &i2, ser0 + (uint64_t)pc, step, SMOLRNG_SURF_REBORN,
+        // the segment starts where the product was made (posx): the reaction position,
+        // or the product's own place for a confspread product
+        const double* px0 = pr.serno ? pp : rxnpos;
+        int st = srf_check(I.sf, I.g, tab->difc, pp, px0, &i2, ser0 + (uint64_t)pc, step, SMOLRNG_SURF_REBORN,
                            I.key0, I.key1, &bnd);
         if (st == kSrfAdsorbed && I.sf.blist[i2][bstate(bnd)] >= 0) {
           // bound, but it stays in its solution list until the next step's first molsort
@@ -1628,8 +1651,9 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     row.rct[0] = sA;
     row.rct[1] = sB;
     int pfirst = pc - rx->nprod;
-    row.prd[0] = rx->nprod > 0 ? ser0 + (uint64_t)pfirst : 0;
-    row.prd[1] = rx->nprod > 1 ? ser0 + (uint64_t)pfirst + 1 : 0;
+    const bool cs = rx->confspread && order == 2;
+    row.prd[0] = rx->nprod > 0 ? (cs ? sA : ser0 + (uint64_t)pfirst) : 0;
+    row.prd[1] = rx->nprod > 1 ? (cs ? sB : ser0 + (uint64_t)pfirst + 1) : 0;
     I.log[e] = row;
   }
   __syncthreads();
@@ -1735,7 +1759,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
         B.ident[rec] = pr.ident;
         if (B.sb) B.sb[rec] = pr.sb;
         if (B.inst) B.inst[rec] = blockIdx.x;  // batch member (multi-simulation records)
-        B.serno[rec] = ser0 + (uint64_t)c;
+        B.serno[rec] = pr.serno ? pr.serno : ser0 + (uint64_t)c;
         B.lpos[rec] = slot - I.L.base[ll];
         B.box[rec] = pbx;
         inv[slot] = rec;

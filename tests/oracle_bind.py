@@ -43,6 +43,8 @@ def lib():
         ip = C.POINTER(C.c_int)
         sig = {
             "ocmd_timeline": (I, [C.c_char_p, I, dp, ip, C.POINTER(C.c_longlong)]),
+            "osim_set_confspread": (I, [P, I, D]),
+            "osim_set_prdpos": (I, [P, I, I, I, dp]),
             "osim_new": (P, [I, dp, dp]),
             "osim_free": (None, [P]),
             "osim_set_wall": (I, [P, I, I, C.c_char]),
@@ -200,6 +202,13 @@ class OSim:
         ri = self.L.osim_add_reaction(self.p, name.encode(), order, _ip(r), _ip(rs), len(prd), _ip(p), _ip(ps), rate)
         assert ri >= 0
         return (order, ri)
+
+    def confspread(self, rxn, radius):
+        assert rxn[0] == 2 and self.L.osim_set_confspread(self.p, rxn[1], radius) == 0
+
+    def prdpos(self, rxn, prd, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        assert self.L.osim_set_prdpos(self.p, rxn[0], rxn[1], prd, _dp(pos)) == 0
 
     def revparam(self, rxn, kind, value):
         assert self.L.osim_set_revparam(self.p, rxn[0], rxn[1], RP[kind], value) == 0

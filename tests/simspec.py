@@ -43,8 +43,13 @@ def build_oracle(spec, sfmt=False):
     rx = {}
     for name, rct, prd, rate in spec.get("reactions", []):
         rx[name] = sim.reaction(name, [idx[r] for r in rct], [idx[p] for p in prd], rate)
-    for name, (kind, param) in spec.get("placement", {}).items():
-        sim.revparam(rx[name], kind, param)
+    for name, pl in spec.get("placement", {}).items():
+        sim.revparam(rx[name], pl[0], pl[1])
+        for prod, vec in (pl[2] if len(pl) > 2 else []):
+            rct, prd = next((r[1], r[2]) for r in spec["reactions"] if r[0] == name)
+            sim.prdpos(rx[name], prd.index(prod), vec)
+    for name, radius in spec.get("confspread", {}).items():
+        sim.confspread(rx[name], radius)
     for srf in spec.get("surfaces", []):
         s = sim.add_surface(srf["name"])
         for face, sp, act in srf.get("actions", []):
@@ -92,8 +97,15 @@ def build_device(spec, update=True):
         sim.partitions("boxsize", spec["boxsize"])
     for name, rct, prd, rate in spec.get("reactions", []):
         sim.reaction(name, rct, prd, rate)
-    for name, (kind, param) in spec.get("placement", {}).items():
-        sim.product_placement(name, kind, param)
+    for name, pl in spec.get("placement", {}).items():
+        if len(pl) > 2:  # offset / fixed: one call per product position (RxnSetRevparam)
+            for prod, vec in pl[2]:
+                sim.product_placement(name, pl[0], pl[1], prod, vec)
+        else:
+            sim.product_placement(name, pl[0], pl[1])
+    for name, radius in spec.get("confspread", {}).items():
+        sim.product_placement(name, "confspread")
+        sim.binding_radius(name, radius)
     for srf in spec.get("surfaces", []):
         sim.add_surface(srf["name"])
         for face, sp, act in srf.get("actions", []):

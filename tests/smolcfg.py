@@ -173,6 +173,8 @@ def read_config(text, sim_factory, defines=None):
             rct = [t for t in rest[:arrow] if t not in ("+", "0")]
             prd = [t for t in rest[arrow + 1:-1] if t not in ("+", "0")]
             rxns[name] = sim.reaction(name, [sim.index(t) for t in rct], [sim.index(t) for t in prd], rate)
+        elif k == "confspread_radius":
+            sim.confspread(rxns[words[1]], float(words[2]))
         elif k == "product_placement":
             sim.revparam(rxns[words[1]], words[2], float(words[3]) if len(words) > 3 else 0.0)
         elif k == "mol":

@@ -264,3 +264,30 @@ def test_ensemble_event_overflow_reports_error():
     with pytest.raises(sa.SmoldynError):
         ens.run_steps(5)
     assert ens.diagnostics(1)["err"] != 0
+
+
+def test_confspread_reactions():
+    """Conformational spread (the reference's allostery model in 3D, mobile):
+    red + blue -> green + blue within radius 2 with probability
+    1 - exp(-k dt), green -> red; the products keep the reactants' places and
+    serial numbers.  Events and lists bit-exact with the oracle."""
+    spec = dict(low=[0, 0, 0], high=[40, 40, 40], walls="ppp", dt=0.01, seed=41,
+                species={"red": 1.0, "green": 1.0, "blue": 0.5},
+                reactions=[("back", ["green"], ["red"], 2.0), ("fwd", ["red", "blue"], ["green", "blue"], 10.0)],
+                confspread={"fwd": 2.0},
+                molecules=[("red", 1500), ("blue", 1500)])
+    nev = _run_pair(spec, 60, check_every=10)
+    assert nev > 50
+
+
+def test_fixed_product_offsets():
+    """Fixed product placement (off-axis offsets, not rotated) for C -> A + B,
+    reversible with A + B -> C, in a reflective box: products at the reaction
+    position plus the given vectors, walls applied to them."""
+    spec = dict(low=[0, 0, 0], high=[30, 30, 30], walls="rrr", dt=0.01, seed=43,
+                species={"A": 1.0, "B": 1.0, "C": 0.5},
+                reactions=[("fwd", ["A", "B"], ["C"], 20.0), ("back", ["C"], ["A", "B"], 5.0)],
+                placement={"back": ("fixed", 0.0, [("A", [0.6, 0.3, -0.2]), ("B", [-0.4, -0.2, 0.1])])},
+                molecules=[("A", 1500), ("B", 1500)])
+    nev = _run_pair(spec, 60, check_every=10)
+    assert nev > 50

@@ -10,7 +10,10 @@ placement and serial numbers.  surf1 (1D) and ReflectToNewSpecies (2D) pin the
 surface path: rectangle, triangle and sphere panels, reflect / transmit /
 absorb / jump actions, species change on reflection, box panel lists and the
 box walk along a trajectory (line2nextbox), and bimolecular reactions with
-surfaces present (rxnXsurface)."""
+surfaces present (rxnXsurface).  allostery (2D, immobile molecules) pins
+conformational-spread reactions: the confspread radius, the reaction
+probability 1-exp(-k dt), and products that keep their reactants' places and
+serial numbers."""
 import gzip
 from pathlib import Path
 
@@ -29,7 +32,7 @@ def run_model(name):
     return code, sim.listmols()
 
 
-@pytest.mark.parametrize("name", ["bounce2", "molecule", "lotvolt", "surf1", "ReflectToNewSpecies"])
+@pytest.mark.parametrize("name", ["bounce2", "molecule", "lotvolt", "surf1", "ReflectToNewSpecies", "allostery"])
 def test_s95_output_identical(name):
     code, out = run_model(name)
     assert code == 1
