@@ -35,7 +35,9 @@ enum SmolRngPurpose {
   SMOLRNG_WALL = 5,      /* absorbing-wall coin flip */
   SMOLRNG_SURFACE = 6,   /* surface-action draws of one molecule, sub = pass<<8 | iteration */
   SMOLRNG_PLACE = 7,     /* initial uniform placement */
-  SMOLRNG_RXNSURF = 8    /* surface-action draws of a reacting pair (rxnXsurface) */
+  SMOLRNG_RXNSURF = 8,   /* surface-action draws of a reacting pair (rxnXsurface) */
+  SMOLRNG_ZERO = 9,      /* zeroth-order reaction count: c0 = reaction, c1 = chunk, sub = block */
+  SMOLRNG_ZEROPOS = 10   /* zeroth-order product position: c0 = reaction, c1 = event */
 };
 
 /* Surface draw passes (bits 8-15 of the SMOLRNG_SURFACE sub-index). */
@@ -78,6 +80,30 @@ SMOLRNG_FN smolrng4 smolrng_philox(uint32_t k0, uint32_t k1, uint32_t c0, uint32
   r.v[2] = c2;
   r.v[3] = c3;
   return r;
+}
+
+/* Number of events of a zeroth-order reaction with mean lambda per step is
+   the sum over chunks j < m = ceil(lambda / SMOLRNG_ZERO_CHUNK) of Poisson
+   draws of mean mu = lambda / m.  One chunk: multiply uniforms until the
+   product falls to g = exp(-mu) or below (the multiplication method of the
+   reference's poisrandD for small means); g is computed once on the host. */
+#define SMOLRNG_ZERO_CHUNK 8.0
+SMOLRNG_FN int smolrng_poisson_chunk(uint32_t k0, uint32_t k1, uint32_t rxn, uint32_t chunk, uint32_t step,
+                                     double g) {
+  uint32_t blk = 0;
+  int idx = 1, n = 0;
+  smolrng4 b = smolrng_philox(k0, k1, rxn, chunk, step, SMOLRNG_ZERO);
+  double t = (double)b.v[0] * (1.0 / 4294967296.0);
+  while (t > g) {
+    if (idx == 4) {
+      blk++;
+      b = smolrng_philox(k0, k1, rxn, chunk, step, SMOLRNG_ZERO | (blk << 8));
+      idx = 0;
+    }
+    t *= (double)b.v[idx++] * (1.0 / 4294967296.0);
+    n++;
+  }
+  return n;
 }
 
 /* Draw block for one molecule (serial number) at one step. */

@@ -316,7 +316,7 @@ int osim_add_reaction(osim* sim, const char* name, int order, const int* rct, co
   orxnss* ss;
   orxn* rxn;
   int k, r;
-  if (order < 1 || order > 2 || nprod > O_MAXPRD) return -1;
+  if (order < 0 || order > 2 || nprod > O_MAXPRD) return -1;
   ss = &sim->rxnss[order];
   ss->present = 1;
   r = ss->totrxn;
@@ -342,7 +342,8 @@ int osim_add_reaction(osim* sim, const char* name, int order, const int* rct, co
   rxn->rparamt = ORP_NONE;
   rxn->unbindrad = -1;
   /* table over the distinct permutations of the reactants (Zn_permute) */
-  if (order == 1) {
+  if (order == 0) {
+  } else if (order == 1) {
     int i = rct[0];
     ss->table[i][ss->nrxn[i]++] = r;
   } else {
@@ -816,6 +817,13 @@ static int o_rxnsetrate(osim* sim, int order, int r) {
   } else if (rxn->rparamt == ORP_CONFSPREAD) {
     if (rxn->rate < 0) return 1;
     rxn->prob = 1.0 - exp(-sim->dt * rxn->rate * rxn->multiplicity);
+  } else if (order == 0) {
+    /* expected events per step in the system volume (systemvolume, smolwall.c:50-57) */
+    double vol = 1.0;
+    int d;
+    if (rxn->rate < 0) return 1;
+    for (d = 0; d < sim->dim; d++) vol *= sim->wpos[2 * d + 1] - sim->wpos[2 * d];
+    rxn->prob = rxn->rate * rxn->multiplicity * sim->dt * vol;
   } else if (order == 1) {
     int ms, j, i;
     double sum, product, prob;
@@ -1083,13 +1091,14 @@ int osim_update(osim* sim) {
   o_molsort(sim, 0);
   for (order = 1; order <= 2; order++)
     if (sim->rxnss[order].present) o_rxnlists(sim, order);
-  for (order = 1; order <= 2; order++)
+  for (order = 0; order <= 2; order++)
     for (r = 0; r < sim->rxnss[order].totrxn; r++) {
       er = o_rxnsetrate(sim, order, r);
       if (er > 1) return 100 + er;
     }
-  for (order = 1; order <= 2; order++)
+  for (order = 0; order <= 2; order++)
     for (r = 0; r < sim->rxnss[order].totrxn; r++) {
+      if (order == 0 && sim->rxnss[0].rxn[r]->nprod > 1) return 213; /* one product per zeroth-order reaction */
       er = o_rxnsetproduct(sim, order, r);
       if (er) return 200 + er;
     }
@@ -1212,12 +1221,16 @@ static void o_logevent(osim* sim, int order, int r, omol* m1, omol* m2, omol** p
 
 /* doreact (smolreact.c:2662-2901), default product-placement branch, no
    surfaces.  m1 is the reactant whose identity is rctident[0]. */
-static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, int mi1, int ll2, int mi2) {
+static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, int mi1, int ll2, int mi2,
+                     const double* pos0) {
   orxn* rxn = sim->rxnss[order].rxn[r];
   int dim = sim->dim, d, prd, calc = 0, rxnbox;
   double rxnpos[O_DIMMAX], m3[9], v1[O_DIMMAX], u[O_DIMMAX];
   omol* prods[O_MAXPRD];
-  if (order == 1) {
+  if (order == 0) {
+    for (d = 0; d < dim; d++) rxnpos[d] = pos0[d];
+    rxnbox = sim->nbox ? osim_pos2box(sim, rxnpos) : -1;
+  } else if (order == 1) {
     for (d = 0; d < dim; d++) rxnpos[d] = m1->pos[d];
     rxnbox = m1->box;
   } else {
@@ -1308,6 +1321,40 @@ static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, i
   return 0;
 }
 
+/* zeroreact (smolreact.c:2905-2993), system-wide reactions: a Poisson number
+   of events of mean prob, each at a uniform position in the open system box
+   (systemrandpos, smolwall.c:42-46) creating the products there.  Restated
+   for the counter stream only: the count is the chunked multiplication
+   method of smolrng_poisson_chunk, position draws are keyed by reaction and
+   event. */
+static int o_zeroreact(osim* sim) {
+  orxnss* ss = &sim->rxnss[0];
+  int r, d;
+  if (!ss->present) return 0;
+  if (sim->rngmode != ORNG_COUNTER) return 2;
+  for (r = 0; r < ss->totrxn; r++) {
+    orxn* rxn = ss->rxn[r];
+    int nmol = 0, m, j, k;
+    double mu, g;
+    if (!(rxn->prob > 0)) continue;
+    m = (int)ceil(rxn->prob / SMOLRNG_ZERO_CHUNK);
+    mu = rxn->prob / m;
+    g = exp(-mu);
+    for (j = 0; j < m; j++) nmol += smolrng_poisson_chunk(sim->key0, sim->key1, (uint32_t)r, (uint32_t)j,
+                                                          sim->stepindex, g);
+    for (k = 0; k < nmol; k++) {
+      double pos[O_DIMMAX];
+      smolrng4 b = smolrng_philox(sim->key0, sim->key1, (uint32_t)r, (uint32_t)k, sim->stepindex, SMOLRNG_ZEROPOS);
+      for (d = 0; d < sim->dim; d++) {
+        double lo = sim->wpos[2 * d], hi = sim->wpos[2 * d + 1];
+        pos[d] = lo + (hi - lo) * smolrng_u01open(b.v[d]);
+      }
+      if (o_doreact(sim, 0, r, NULL, NULL, -1, -1, -1, -1, pos)) return 1;
+    }
+  }
+  return 0;
+}
+
 /* unireact (smolreact.c:2997-3054) */
 static int o_unireact(osim* sim) {
   orxnss* ss = &sim->rxnss[1];
@@ -1337,7 +1384,7 @@ static int o_unireact(osim* sim) {
         }
         if (!coin) continue;
         if (mp->ident == 0) continue;
-        if (o_doreact(sim, 1, r, mp, NULL, ll, m, -1, -1)) return 1;
+        if (o_doreact(sim, 1, r, mp, NULL, ll, m, -1, -1, NULL)) return 1;
         sim->eventcount[0]++;
         j = ss->nrxn[i];
       }
@@ -1397,8 +1444,8 @@ static int o_morebireact(osim* sim, int r, omol* mp1, omol* mp2, int ll1, int m1
     }
   }
   sim->eventcount[et]++;
-  if (!swap) return o_doreact(sim, 2, r, mA, mB, ll1, m1, ll2, -1);
-  return o_doreact(sim, 2, r, mA, mB, ll2, -1, ll1, m1);
+  if (!swap) return o_doreact(sim, 2, r, mA, mB, ll1, m1, ll2, -1, NULL);
+  return o_doreact(sim, 2, r, mA, mB, ll2, -1, ll1, m1, NULL);
 }
 
 static int o_bicoin(osim* sim, orxn* rxn, omol* a, omol* b, int j) {
@@ -1507,6 +1554,8 @@ int osim_step(osim* sim) {
   }
   o_reassign(sim, 1, 0);
   o_molsort(sim, 0);
+  er = o_zeroreact(sim);
+  if (er) return er == 2 ? 12 : 3;
   if (o_unireact(sim)) return 4;
   if (o_bireact(sim, 0)) return 5;
   if (o_bireact(sim, 1)) return 5;

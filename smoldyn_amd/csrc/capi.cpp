@@ -258,6 +258,12 @@ int set_rate(simptr sim, int order, int r) {
   } else if (rx.rparamt == RPconfspread) {
     if (rx.rate < 0) return 1;
     rx.prob = 1.0 - exp(-sim->dt * rx.rate * rx.multiplicity);
+  } else if (order == 0) {
+    // events per step in the system volume (systemvolume, smolwall.c:50-57)
+    if (rx.rate < 0) return 1;
+    double vol = 1.0;
+    for (int d = 0; d < sim->dim; d++) vol *= sim->high[d] - sim->low[d];
+    rx.prob = rx.rate * rx.multiplicity * sim->dt * vol;
   } else if (order == 1) {
     int ms;
     for (ms = 0; ms < kMS && !rx.permit[ms]; ms++) {
@@ -594,18 +600,22 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   for (int d = 0; d < 3; d++) g.wrapflag[d] = (sim->wtype[d] == 'p' && !sim->surf.enabled());
 
   // reactions (rxnsupdate, smolreact.c:2384-2401)
-  for (int order = 1; order <= 2; order++)
+  for (int order = 0; order <= 2; order++)
     for (size_t r = 0; r < sim->rxn[order].size(); r++) {
       int er = set_rate(sim, order, (int)r);
       LCHECK(er <= 1, fn, ECerror, "error setting rate for reaction %s (code %d)", sim->rxn[order][r].name.c_str(), er);
     }
-  for (int order = 1; order <= 2; order++)
+  LCHECK(sim->rxn[0].size() <= (size_t)kMaxRxn0, fn, ECbounds, "too many zeroth-order reactions");
+  for (auto& r0 : sim->rxn[0])
+    LCHECK(r0.nprod <= 1, fn, ECerror,
+           "zeroth-order reaction %s: more than one product is not implemented on the device path",
+           r0.name.c_str());
+  for (int order = 0; order <= 2; order++)
     for (size_t r = 0; r < sim->rxn[order].size(); r++) {
       int er = set_product(sim, order, (int)r);
       LCHECK(er == 0, fn, ECerror, "product placement of reaction %s is not supported here (code %d)",
              sim->rxn[order][r].name.c_str(), er);
     }
-  LCHECK(sim->rxn[0].empty(), fn, ECerror, "zeroth-order reactions are not implemented on the device path");
 
   // device tables
   DevTables& t = sim->tab;
@@ -636,6 +646,26 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
     }
   LCHECK(sim->rxn[1].size() <= (size_t)kMaxRxn && sim->rxn[2].size() <= (size_t)kMaxRxn, fn, ECbounds,
          "too many reactions");
+  t.nrxn0 = (int)sim->rxn[0].size();
+  for (int d = 0; d < 3; d++) {
+    t.zlo[d] = sim->low[d];
+    t.zhi[d] = sim->high[d];
+  }
+  for (int r = 0; r < t.nrxn0; r++) {
+    const Rxn& rx = sim->rxn[0][(size_t)r];
+    DevRxn& dr = t.rxn0[r];
+    dr.order = 0;
+    dr.nprod = rx.nprod;
+    for (int p = 0; p < rx.nprod; p++) {
+      dr.prd[p] = rx.prd[p];
+      LCHECK(rx.prdstate[p] == MSsoln || rx.prdstate[p] == MSbsoln, fn, ECerror,
+             "surface-bound products need the surface path");
+      dr.prdlist[p] = lookup[rx.prd[p]][MSsoln];
+    }
+    const double lam = rx.prob > 0 ? rx.prob : 0;
+    t.zm[r] = lam > 0 ? (int)ceil(lam / SMOLRNG_ZERO_CHUNK) : 0;
+    t.zg[r] = t.zm[r] ? exp(-lam / t.zm[r]) : 1.0;
+  }
   for (int order = 1; order <= 2; order++)
     for (size_t r = 0; r < sim->rxn[order].size(); r++) {
       Rxn& rx = sim->rxn[order][r];
@@ -742,7 +772,7 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   // lists that receive neither keep a token capacity
   std::vector<int> reachable(nlist, 0);
   for (int l = 0; l < nlist; l++) reachable[l] = !ident[l].empty();
-  for (int order = 1; order <= 2; order++)
+  for (int order = 0; order <= 2; order++)
     for (auto& rx : sim->rxn[order])
       for (int p = 0; p < rx.nprod; p++) reachable[lookup[rx.prd[p]][MSsoln]] = 1;
   // with surfaces, molecules can adsorb into the lists of the bound states

@@ -39,11 +39,21 @@ struct DevRxn {
   int fixed;                // products at the reaction position plus prdvec, not rotated
 };
 
+constexpr int kMaxRxn0 = 8;  // zeroth-order reactions
+
 struct DevTables {
   int nspecies;
   int nlist;
   int dim;
   int ngauss;
+  // zeroth-order reactions: events per step are Poisson of mean prob, drawn in
+  // zm chunks of mean prob/zm against zg = exp(-prob/zm); each event makes
+  // its product at a uniform position in the open system box [zlo, zhi]
+  int nrxn0;
+  DevRxn rxn0[kMaxRxn0];
+  double zg[kMaxRxn0];
+  int zm[kMaxRxn0];
+  double zlo[3], zhi[3];
   double difc[kMaxSpecies];
   double difstep[kMaxSpecies];
   int listof[kMaxSpecies];  // list of the solution state

@@ -1385,6 +1385,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   __shared__ int s_kill[kLdsKills];
   __shared__ int s_flag, s_nacc, s_prodoff;
   __shared__ int s_nl[kMaxList], s_nf[kMaxList], s_k0[kMaxList], s_k1[kMaxList], s_pl[kMaxList];
+  __shared__ int s_zpre[kMaxRxn0 + 1];  // zeroth-order events before each reaction
   const int tid = threadIdx.x;
   const int NT = blockDim.x;
   Control* ctl = I.ctl;
@@ -1445,9 +1446,27 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   int B2 = s_nacc;
   __syncthreads();
 
+  // ---- zeroth-order events (zeroreact, smolreact.c:2905-2993): the Poisson
+  // count of each reaction is a sum of chunk counts, chunks spread over the
+  // threads; the events come first in creation order
+  const int nz = tab->nrxn0;
+  if (tid <= kMaxRxn0) s_zpre[tid] = 0;
+  __syncthreads();
+  for (int r = 0; r < nz; r++) {
+    int c = 0;
+    for (int j = tid; j < tab->zm[r]; j += NT)
+      c += smolrng_poisson_chunk(I.key0, I.key1, (uint32_t)r, (uint32_t)j, step, tab->zg[r]);
+    if (c) atomicAdd(&s_zpre[r + 1], c);
+  }
+  __syncthreads();
+  if (tid == 0)
+    for (int r = 0; r < nz; r++) s_zpre[r + 1] += s_zpre[r];
+  __syncthreads();
+  int Z = s_zpre[nz];
+
   // ---- order the events: first-order reactions in live-list order, then
   // bimolecular ones in edge-key order
-  if (U + pow2ceil(B2) > I.sortcap || pow2ceil(U + 2 * B2) > I.sortcap) {
+  if (U + pow2ceil(B2) > I.sortcap || pow2ceil(U + 2 * B2) > I.sortcap || Z + U + B2 > I.sortcap) {
     // more events than the sort space: flag the error, halt the later steps,
     // and finish this step without its reactions so the bookkeeping below
     // (record count, step counter) stays consistent
@@ -1462,6 +1481,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     }
     U = 0;
     B2 = 0;
+    Z = 0;
     __syncthreads();
   }
   const bool ev_lds = (U + B2) <= kLdsEvents;
@@ -1491,8 +1511,14 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     __syncthreads();
     wg_sort_any(I.sk + U, I.sv + U, B2, lk, lv, nullptr, ev_lds ? s_ev + U : nullptr);
   }
-  int NE = U + B2;
+  int NE = Z + U + B2;
   const int* evs = ev_lds ? s_ev : I.sv;  // event order: sorted event indices
+  // reaction of zeroth-order event e (e < Z)
+  auto zrxn = [&](int e) {
+    int r = 0;
+    while (r + 1 < nz && s_zpre[r + 1] <= e) r++;
+    return r;
+  };
 
   // ---- product counts and serial numbers in creation order (each thread
   // takes a contiguous chunk of events so the scan follows creation order)
@@ -1500,7 +1526,10 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   int e0 = tid * per, e1 = min(NE, e0 + per);
   int myprod = 0;
   for (int e = e0; e < e1; e++) {
-    const DevRxn& rx = e < U ? tab->rxn1[I.uniev[evs[e]].r] : tab->rxn2[I.edges[evs[e]].r];
+    const int ee = e - Z;
+    const DevRxn& rx = ee < 0    ? tab->rxn0[zrxn(e)]
+                       : ee < U ? tab->rxn1[I.uniev[evs[ee]].r]
+                                : tab->rxn2[I.edges[evs[ee]].r];
     myprod += rx.nprod;
   }
   int K;
@@ -1519,20 +1548,29 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     double rxnpos[3];
     double pAv[3] = {0, 0, 0}, pBv[3] = {0, 0, 0};  // reactant places, in the reaction's reactant order
     const DevRxn* rx;
-    uint64_t sA, sB = 0;
+    uint64_t sA = 0, sB = 0;
     int order, ridx;
-    if (e < U) {
-      UniEvent ev = I.uniev[evs[e]];
+    const int ee = e - Z;
+    if (ee < 0) {  // zeroth order: the product at a uniform position in the open system box
+      ridx = zrxn(e);
+      rx = &tab->rxn0[ridx];
+      order = 0;
+      const smolrng4 b = smolrng_philox(I.key0, I.key1, (uint32_t)ridx, (uint32_t)(e - s_zpre[ridx]), step,
+                                        SMOLRNG_ZEROPOS);
+#pragma unroll
+      for (int d = 0; d < 3; d++) rxnpos[d] = tab->zlo[d] + (tab->zhi[d] - tab->zlo[d]) * smolrng_u01open(b.v[d]);
+    } else if (ee < U) {
+      UniEvent ev = I.uniev[evs[ee]];
       rx = &tab->rxn1[ev.r];
       ridx = ev.r;
       order = 1;
       load_pos<kF32>(RA, I.g, ev.rec, ev.box, rxnpos);
       sA = RA.serno[ev.rec];
       const int us = uni_slot(I, RA, ev.rec);
-      if (kill_small) kslot[e] = us;
-      else I.killslot[e] = us;
+      if (kill_small) kslot[ee] = us;
+      else I.killslot[ee] = us;
     } else {
-      Edge ed = I.edges[evs[e]];
+      Edge ed = I.edges[evs[ee]];
       rx = &tab->rxn2[ed.r];
       ridx = ed.r;
       order = 2;
@@ -1577,7 +1615,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       }
       sA = B.serno[swap ? ed.s2 : ed.s1];
       sB = B.serno[swap ? ed.s1 : ed.s2];
-      int k2 = U + 2 * (e - U);
+      int k2 = U + 2 * (ee - U);
       const int ks1 = I.L.base[tab->listof[id1]] + B.lpos[ed.s1];
       const int ks2 = I.L.base[tab->listof[id2]] + B.lpos[ed.s2];
       if (kill_small) {
@@ -1588,7 +1626,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
         I.killslot[k2 + 1] = ks2;
       }
     }
-    atomicSub(&s_dcount[rx->rct0], 1);
+    if (order >= 1) atomicSub(&s_dcount[rx->rct0], 1);
     if (order == 2) atomicSub(&s_dcount[rx->rct1], 1);
     double uvec[3] = {0, 0, 0};
     if (rx->rotate) smolrng_unitvec(I.key0, I.key1, sA, step, 3, uvec);

@@ -291,3 +291,20 @@ def test_fixed_product_offsets():
                 molecules=[("A", 1500), ("B", 1500)])
     nev = _run_pair(spec, 60, check_every=10)
     assert nev > 50
+
+
+def test_zeroth_order_production():
+    """Zeroth-order production (0 -> A at 12.8 events per step: two Poisson
+    chunks; 0 -> B at 0.64) with A + B <-> C and C -> 0 in a periodic box:
+    event counts, product positions and serial numbers bit-exact with the
+    oracle's counter stream."""
+    vol = 40.0 ** 3
+    spec = dict(low=[0, 0, 0], high=[40, 40, 40], walls="ppp", dt=0.01, seed=47,
+                species={"A": 1.0, "B": 1.0, "C": 0.5},
+                reactions=[("makeA", [], ["A"], 12.8 / (0.01 * vol)), ("makeB", [], ["B"], 0.64 / (0.01 * vol)),
+                           ("fwd", ["A", "B"], ["C"], 20.0), ("back", ["C"], ["A", "B"], 2.0),
+                           ("decay", ["C"], [], 0.5)],
+                placement={"back": ("ratio", 1.2)},
+                molecules=[("A", 1500), ("B", 1500)])
+    nev = _run_pair(spec, 80, check_every=10)
+    assert nev > 1000
