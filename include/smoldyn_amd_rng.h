@@ -37,7 +37,8 @@ enum SmolRngPurpose {
   SMOLRNG_PLACE = 7,     /* initial uniform placement */
   SMOLRNG_RXNSURF = 8,   /* surface-action draws of a reacting pair (rxnXsurface) */
   SMOLRNG_ZERO = 9,      /* zeroth-order reaction count: c0 = reaction, c1 = chunk, sub = block */
-  SMOLRNG_ZEROPOS = 10   /* zeroth-order product position: c0 = reaction, c1 = event */
+  SMOLRNG_ZEROPOS = 10,  /* zeroth-order product position: c0 = reaction, c1 = event */
+  SMOLRNG_PRODROT = 11   /* random rotation of product offset vectors, sub = attempt */
 };
 
 /* Surface draw passes (bits 8-15 of the SMOLRNG_SURFACE sub-index). */
@@ -166,6 +167,40 @@ SMOLRNG_FN void smolrng_unitvec(uint32_t k0, uint32_t k1, uint64_t serial, uint3
   u[0] = 1.0;
   u[1] = 0.0;
   if (dim == 3) u[2] = 0.0;
+}
+
+/* Uniformly distributed rotation (Haar measure) for product offset vectors
+   off the x axis: a unit quaternion by Marsaglia's method (two points drawn
+   in the unit disk, Ann. Math. Stat. 43:645, 1972) turned into a rotation
+   matrix m (row major).  Arithmetic and sqrt only, so host and device agree
+   to the bit.  The reference draws Euler angles of the same distribution
+   (DirCosMD with thetarandCCD, smolreact.c:2846-2848). */
+SMOLRNG_FN void smolrng_rotation(uint32_t k0, uint32_t k1, uint64_t serial, uint32_t step, double* m) {
+  uint32_t a;
+  double w = 1.0, x = 0.0, y = 0.0, z = 0.0;
+  for (a = 0; a < 64; a++) {
+    smolrng4 r = smolrng_mol(k0, k1, serial, step, SMOLRNG_PRODROT, a);
+    double x1 = smolrng_sym(r.v[0]), y1 = smolrng_sym(r.v[1]);
+    double x2 = smolrng_sym(r.v[2]), y2 = smolrng_sym(r.v[3]);
+    double s1 = x1 * x1 + y1 * y1, s2 = x2 * x2 + y2 * y2;
+    if (s1 < 1.0 && s2 > 0.0 && s2 < 1.0) {
+      double f = sqrt((1.0 - s1) / s2);
+      w = x1;
+      x = y1;
+      y = x2 * f;
+      z = y2 * f;
+      break;
+    }
+  }
+  m[0] = 1.0 - 2.0 * (y * y + z * z);
+  m[1] = 2.0 * (x * y - w * z);
+  m[2] = 2.0 * (x * z + w * y);
+  m[3] = 2.0 * (x * y + w * z);
+  m[4] = 1.0 - 2.0 * (x * x + z * z);
+  m[5] = 2.0 * (y * z - w * x);
+  m[6] = 2.0 * (x * z - w * y);
+  m[7] = 2.0 * (y * z + w * x);
+  m[8] = 1.0 - 2.0 * (x * x + y * y);
 }
 
 #endif

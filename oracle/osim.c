@@ -397,6 +397,11 @@ int osim_set_prdpos(osim* sim, int order, int r, int prd, const double* pos) {
   return 0;
 }
 
+/* the counter stream's random rotation, for its statistical tests */
+void osim_rng_rotation(uint32_t k0, uint32_t k1, uint64_t serial, uint32_t step, double* m) {
+  smolrng_rotation(k0, k1, serial, step, m);
+}
+
 int osim_set_rxn_prob(osim* sim, int order, int r, double prob) {
   if (order < 1 || order > 2 || r < 0 || r >= sim->rxnss[order].totrxn) return 1;
   sim->rxnss[order].rxn[r]->prob = prob;
@@ -1101,6 +1106,11 @@ int osim_update(osim* sim) {
       if (order == 0 && sim->rxnss[0].rxn[r]->nprod > 1) return 213; /* one product per zeroth-order reaction */
       er = o_rxnsetproduct(sim, order, r);
       if (er) return 200 + er;
+      if (order == 0 && sim->rxnss[0].rxn[r]->nprod == 1) {
+        int d;
+        for (d = 0; d < sim->dim; d++)
+          if (sim->rxnss[0].rxn[r]->prdpos[0][d] != 0) return 214; /* no reactant to key an offset draw */
+      }
     }
   if (sim->srfss) {
     er = osurf_update(sim);
@@ -1224,7 +1234,7 @@ static void o_logevent(osim* sim, int order, int r, omol* m1, omol* m2, omol** p
 static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, int mi1, int ll2, int mi2,
                      const double* pos0) {
   orxn* rxn = sim->rxnss[order].rxn[r];
-  int dim = sim->dim, d, prd, calc = 0, rxnbox;
+  int dim = sim->dim, d, prd, calc = 0, rxnbox, offaxis = 0;
   double rxnpos[O_DIMMAX], m3[9], v1[O_DIMMAX], u[O_DIMMAX];
   omol* prods[O_MAXPRD];
   if (order == 0) {
@@ -1272,13 +1282,28 @@ static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, i
       if (rxn->rparamt == ORP_FIXED) {
         for (d = 0; d < dim; d++) v1[d] = rxn->prdpos[prd][d];
       } else if (sim->rngmode == ORNG_COUNTER) {
+        /* counter stream: offsets along x turn with one random unit vector;
+           when any product's offset leaves the x axis, all offsets of the
+           event turn with one random rotation (smolrng_rotation, 3D only) */
         if (!calc) {
-          smolrng_unitvec(sim->key0, sim->key1, m1->serno, sim->stepindex, dim, u);
+          int p2, d2;
+          for (p2 = 0; p2 < rxn->nprod; p2++)
+            for (d2 = 1; d2 < dim; d2++)
+              if (rxn->prdpos[p2][d2] != 0) offaxis = 1;
+          if (offaxis && dim != 3) return 12;
+          if (offaxis) smolrng_rotation(sim->key0, sim->key1, m1->serno, sim->stepindex, m3);
+          else smolrng_unitvec(sim->key0, sim->key1, m1->serno, sim->stepindex, dim, u);
           calc = 1;
         }
-        for (d = 1; d < dim; d++)
-          if (rxn->prdpos[prd][d] != 0) return 12; /* only axial offsets are rotated in counter mode */
-        for (d = 0; d < dim; d++) v1[d] = rxn->prdpos[prd][0] * u[d];
+        if (offaxis) {
+          for (d = 0; d < 3; d++) {
+            int j;
+            v1[d] = 0;
+            for (j = 0; j < 3; j++) v1[d] += m3[3 * d + j] * rxn->prdpos[prd][j];
+          }
+        } else {
+          for (d = 0; d < dim; d++) v1[d] = rxn->prdpos[prd][0] * u[d];
+        }
       } else if (dim == 1) {
         if (!calc) {
           m3[0] = (osim_randU32(sim) & 1) ? 1 : -1;

@@ -661,6 +661,8 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
       LCHECK(rx.prdstate[p] == MSsoln || rx.prdstate[p] == MSbsoln, fn, ECerror,
              "surface-bound products need the surface path");
       dr.prdlist[p] = lookup[rx.prd[p]][MSsoln];
+      LCHECK(rx.prdpos[p][0] == 0 && rx.prdpos[p][1] == 0 && rx.prdpos[p][2] == 0, fn, ECerror,
+             "zeroth-order reaction %s: product offsets are not implemented on the device path", rx.name.c_str());
     }
     const double lam = rx.prob > 0 ? rx.prob : 0;
     t.zm[r] = lam > 0 ? (int)ceil(lam / SMOLRNG_ZERO_CHUNK) : 0;
@@ -687,10 +689,8 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
         dr.prdpos[p] = rx.prdpos[p][0];
         for (int d = 0; d < 3; d++) dr.prdvec[p][d] = rx.prdpos[p][d];
         if (dr.fixed) continue;
-        if (rx.prdpos[p][0] != 0) dr.rotate = 1;
-        LCHECK(rx.prdpos[p][1] == 0 && rx.prdpos[p][2] == 0, fn, ECerror,
-               "off-axis product offsets of offset placement (a random rotation matrix per event) are not "
-               "implemented on the device path; fixed placement and offsets along x are");
+        if (rx.prdpos[p][0] != 0 || rx.prdpos[p][1] != 0 || rx.prdpos[p][2] != 0) dr.rotate = 1;
+        if (rx.prdpos[p][1] != 0 || rx.prdpos[p][2] != 0) dr.offrot = 1;
       }
     }
   // pair-search screening tables: which partner species react, and the

@@ -37,6 +37,7 @@ struct DevRxn {
   int permit_soln;          // permitted for solution reactant(s)
   int confspread;           // products 0 and 1 take the reactants' places and serial numbers
   int fixed;                // products at the reaction position plus prdvec, not rotated
+  int offrot;               // offsets off the x axis: prdvec turned by one random rotation per event
 };
 
 constexpr int kMaxRxn0 = 8;  // zeroth-order reactions

@@ -1629,7 +1629,9 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     if (order >= 1) atomicSub(&s_dcount[rx->rct0], 1);
     if (order == 2) atomicSub(&s_dcount[rx->rct1], 1);
     double uvec[3] = {0, 0, 0};
-    if (rx->rotate) smolrng_unitvec(I.key0, I.key1, sA, step, 3, uvec);
+    double rot[9];
+    if (rx->offrot) smolrng_rotation(I.key0, I.key1, sA, step, rot);
+    else if (rx->rotate) smolrng_unitvec(I.key0, I.key1, sA, step, 3, uvec);
     for (int q = 0; q < rx->nprod; q++) {
       Prod pr;
       pr.ident = rx->prd[q];
@@ -1651,6 +1653,17 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
         pp[0] = rxnpos[0] + rx->prdvec[q][0];
         pp[1] = rxnpos[1] + rx->prdvec[q][1];
         pp[2] = rxnpos[2] + rx->prdvec[q][2];
+      } else if (rx->offrot) {  // offsets off the x axis: one random rotation of the event
+        double v1[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          v1[d] = 0;
+#pragma unroll
+          for (int j = 0; j < 3; j++) v1[d] += rot[3 * d + j] * rx->prdvec[q][j];
+        }
+        pp[0] = rxnpos[0] + v1[0];
+        pp[1] = rxnpos[1] + v1[1];
+        pp[2] = rxnpos[2] + v1[2];
       } else if (rx->prdpos[q] != 0) {
         double v1[3] = {rx->prdpos[q] * uvec[0], rx->prdpos[q] * uvec[1], rx->prdpos[q] * uvec[2]};
         pp[0] = rxnpos[0] + v1[0];

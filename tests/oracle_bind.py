@@ -45,6 +45,7 @@ def lib():
             "ocmd_timeline": (I, [C.c_char_p, I, dp, ip, C.POINTER(C.c_longlong)]),
             "osim_set_confspread": (I, [P, I, D]),
             "osim_set_prdpos": (I, [P, I, I, I, dp]),
+            "osim_rng_rotation": (None, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, dp]),
             "osim_new": (P, [I, dp, dp]),
             "osim_free": (None, [P]),
             "osim_set_wall": (I, [P, I, I, C.c_char]),

@@ -308,3 +308,16 @@ def test_zeroth_order_production():
                 molecules=[("A", 1500), ("B", 1500)])
     nev = _run_pair(spec, 80, check_every=10)
     assert nev > 1000
+
+
+def test_offset_placement_rotated_off_axis():
+    """Offset product placement with offsets off the x axis, C -> A + B
+    reversible with A + B -> C: each event turns both offset vectors by one
+    random rotation (the counter stream's quaternion draw)."""
+    spec = dict(low=[0, 0, 0], high=[30, 30, 30], walls="ppp", dt=0.01, seed=53,
+                species={"A": 1.0, "B": 1.0, "C": 0.5},
+                reactions=[("fwd", ["A", "B"], ["C"], 20.0), ("back", ["C"], ["A", "B"], 5.0)],
+                placement={"back": ("offset", 0.0, [("A", [0.5, 0.4, 0.0]), ("B", [-0.3, 0.0, 0.2])])},
+                molecules=[("A", 1500), ("B", 1500)])
+    nev = _run_pair(spec, 60, check_every=10)
+    assert nev > 50
