@@ -984,7 +984,32 @@ static int o_rxnsetproduct(osim* sim, int order, int r) {
     } else
       return 2;
   } else if (rparamt == ORP_BOUNCE) {
-    return 11; /* bounce placement is not restated */
+    /* nprod >= 2, bounce (smolreact.c:1470-1494): offsets from the reactants'
+       relative diffusion coefficients; the reflection method (rparam -2)
+       needs the pre-diffusion positions and is not restated */
+    double dc1, dc2, dsum;
+    int ms1 = rxn->rctstate[0], ms2 = rxn->rctstate[1];
+    if (rpar == -2) return 11;
+    if (ms1 == OMS_BSOLN || ms1 == OMS_NONE || ms1 == OMS_ALL || ms1 == OMS_SOME) ms1 = OMS_SOLN;
+    if (ms2 == OMS_BSOLN || ms2 == OMS_NONE || ms2 == OMS_ALL || ms2 == OMS_SOME) ms2 = OMS_SOLN;
+    dc1 = o_difcsum(sim, rxn->rct[0], ms1, 0, OMS_NONE);
+    dc2 = o_difcsum(sim, rxn->rct[1], ms2, 0, OMS_NONE);
+    dsum = dc1 + dc2;
+    if (dsum == 0) dc1 = dc2 = 0.5;
+    else {
+      dc1 /= dsum;
+      dc2 /= dsum;
+    }
+    if (rpar >= 0) {
+      rxn->unbindrad = rpar;
+      rxn->prdpos[0][0] = rpar * dc1;
+      rxn->prdpos[1][0] = rpar * dc2;
+    } else {
+      rxn->unbindrad = rpar;
+      rxn->prdpos[0][0] = sqrt(rxn->bindrad2) * dc1;
+      rxn->prdpos[1][0] = sqrt(rxn->bindrad2) * dc2;
+    }
+    for (prd = 2; prd < nprod; prd++) rxn->prdpos[prd][0] = 0;
   } else {
     int ms1 = rxn->prdstate[0], ms2 = rxn->prdstate[1], orderr, rr, rev;
     double dc1, dc2, dsum;
@@ -1236,6 +1261,7 @@ static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, i
   orxn* rxn = sim->rxnss[order].rxn[r];
   int dim = sim->dim, d, prd, calc = 0, rxnbox, offaxis = 0;
   double rxnpos[O_DIMMAX], m3[9], v1[O_DIMMAX], u[O_DIMMAX];
+  double bv[O_DIMMAX] = {0}, bdist = 0, bx = 0; /* bounce: reactant separation vector, length, scale */
   omol* prods[O_MAXPRD];
   if (order == 0) {
     for (d = 0; d < dim; d++) rxnpos[d] = pos0[d];
@@ -1259,6 +1285,44 @@ static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, i
     mp->list = sim->listlookup[mp->ident][mp->mstate];
     mp->box = rxnbox;
     mp->pnl = mp->pnlx = -1;
+    if (rxn->rparamt == ORP_BOUNCE && order == 2) {
+      /* bounce (smolreact.c:2753-2805), scaling methods: the products keep
+         the reactants' serial numbers and move apart along the line joining
+         them (rparam >= 0: to distance rparam; otherwise reflected off the
+         binding radius) */
+      if (prd == 0) {
+        bdist = 0;
+        for (d = 0; d < dim; d++) {
+          bv[d] = m2->pos[d] - m1->pos[d];
+          bdist += bv[d] * bv[d];
+        }
+        if (bdist == 0) {
+          bdist = sqrt(rxn->bindrad2);
+          bv[0] = bdist;
+        } else
+          bdist = sqrt(bdist);
+        if (rxn->rparam >= 0) bx = 1.0 / bdist - (1.0 / rxn->rparam);
+        else bx = 2.0 / bdist - 2.0 / (rxn->prdpos[0][0] + rxn->prdpos[1][0]);
+        for (d = 0; d < dim; d++) mp->pos[d] = m1->pos[d] - bx * rxn->prdpos[0][0] * bv[d];
+        mp->box = m1->box;
+        mp->serno = m1->serno;
+        for (d = 0; d < dim; d++) {
+          mp->posx[d] = m1->pos[d];
+          mp->posoffset[d] = m1->posoffset[d];
+        }
+      } else if (prd == 1) {
+        mp->box = m2->box;
+        mp->serno = m2->serno;
+        for (d = 0; d < dim; d++) mp->pos[d] = m2->pos[d] + bx * rxn->prdpos[1][0] * bv[d];
+        for (d = 0; d < dim; d++) {
+          mp->posx[d] = m2->pos[d];
+          mp->posoffset[d] = m2->posoffset[d];
+        }
+      } else {
+        for (d = 0; d < dim; d++) mp->posx[d] = mp->pos[d] = rxnpos[d];
+      }
+      continue;
+    }
     if (rxn->rparamt == ORP_CONFSPREAD && order == 2) {
       /* confspread (smolreact.c:2736-2751): the first two products take the
          reactants' places and serial numbers, others the reaction position */

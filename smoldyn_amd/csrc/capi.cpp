@@ -370,7 +370,24 @@ int set_product(simptr sim, int order, int r) {
     } else
       return 2;
   } else if (rparamt == RPbounce) {
-    return 11;
+    // nprod >= 2, bounce (smolreact.c:1470-1494); the reflection method
+    // (rparam -2) needs the pre-diffusion positions, which the device does not keep
+    if (rpar == -2) return 11;
+    int ms1 = rx.rctstate[0], ms2 = rx.rctstate[1];
+    if (ms1 == MSbsoln || ms1 == MSnone || ms1 == MSall || ms1 == MSsome) ms1 = MSsoln;
+    if (ms2 == MSbsoln || ms2 == MSnone || ms2 == MSall || ms2 == MSsome) ms2 = MSsoln;
+    double dc1 = difcsum(sim, rx.rct[0], ms1, 0, MSnone), dc2 = difcsum(sim, rx.rct[1], ms2, 0, MSnone);
+    const double dsum = dc1 + dc2;
+    if (dsum == 0) dc1 = dc2 = 0.5;
+    else {
+      dc1 /= dsum;
+      dc2 /= dsum;
+    }
+    const double len = rpar >= 0 ? rpar : sqrt(rx.bindrad2);
+    rx.unbindrad = rpar;
+    rx.prdpos[0][0] = len * dc1;
+    rx.prdpos[1][0] = len * dc2;
+    for (int p = 2; p < nprod; p++) rx.prdpos[p][0] = 0;
   } else {
     int ms1 = rx.prdstate[0] == MSbsoln ? MSsoln : rx.prdstate[0];
     int ms2 = rx.prdstate[1] == MSbsoln ? MSsoln : rx.prdstate[1];
@@ -680,6 +697,8 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
       dr.prob = rx.prob;
       dr.confspread = rx.rparamt == RPconfspread;
       dr.fixed = rx.rparamt == RPfixed;
+      dr.bounce = order == 2 && rx.rparamt == RPbounce;
+      dr.bparam = rx.rparam;
       dr.permit_soln = order == 1 ? rx.permit[MSsoln] : rx.permit[MSsoln * kMS1 + MSsoln];
       for (int p = 0; p < rx.nprod; p++) {
         dr.prd[p] = rx.prd[p];
@@ -688,7 +707,7 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
         dr.prdlist[p] = lookup[rx.prd[p]][MSsoln];
         dr.prdpos[p] = rx.prdpos[p][0];
         for (int d = 0; d < 3; d++) dr.prdvec[p][d] = rx.prdpos[p][d];
-        if (dr.fixed) continue;
+        if (dr.fixed || dr.bounce) continue;
         if (rx.prdpos[p][0] != 0 || rx.prdpos[p][1] != 0 || rx.prdpos[p][2] != 0) dr.rotate = 1;
         if (rx.prdpos[p][1] != 0 || rx.prdpos[p][2] != 0) dr.offrot = 1;
       }
@@ -1939,7 +1958,6 @@ enum ErrorCode smolSetReactionProducts(simptr sim, const char* reaction, enum Re
   int order = -1;
   int r = smolGetReactionIndexNT(sim, &order, reaction);
   LCHECK(r >= 0, fn, ECnonexist, "reaction not found");
-  LCHECK(method != RPbounce, fn, ECerror, "bounce product placement is not implemented on the device path");
   Rxn& rx = sim->rxn[order][r];
   int prd = -1;
   if (product && product[0]) {
@@ -1952,6 +1970,8 @@ enum ErrorCode smolSetReactionProducts(simptr sim, const char* reaction, enum Re
   int er = rx.rparamt != RPnone ? 1 : 0;
   rx.rparamt = method;
   if (method == RPnone || method == RPirrev || method == RPconfspread) {
+  } else if (method == RPbounce) {
+    rx.rparam = parameter;
   } else if (method == RPpgem || method == RPpgemmax || method == RPpgemmaxw || method == RPpgem2 ||
              method == RPpgemmax2) {
     if (!(parameter > 0 && parameter <= 1)) er = 2;

@@ -38,6 +38,8 @@ struct DevRxn {
   int confspread;           // products 0 and 1 take the reactants' places and serial numbers
   int fixed;                // products at the reaction position plus prdvec, not rotated
   int offrot;               // offsets off the x axis: prdvec turned by one random rotation per event
+  int bounce;               // products 0 and 1 keep the reactants' serials and move apart (scaling methods)
+  double bparam;            // bounce parameter: >= 0 final separation, < 0 reflection off the binding radius
 };
 
 constexpr int kMaxRxn0 = 8;  // zeroth-order reactions

@@ -1547,6 +1547,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   for (int e = e0; e < e1; e++) {
     double rxnpos[3];
     double pAv[3] = {0, 0, 0}, pBv[3] = {0, 0, 0};  // reactant places, in the reaction's reactant order
+    double bq[2][3] = {{0, 0, 0}, {0, 0, 0}};       // bounce: places of products 0 and 1
     const DevRxn* rx;
     uint64_t sA = 0, sB = 0;
     int order, ridx;
@@ -1615,6 +1616,29 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       }
       sA = B.serno[swap ? ed.s2 : ed.s1];
       sB = B.serno[swap ? ed.s1 : ed.s2];
+      if (rx->bounce) {
+        // doreact's bounce scaling methods (smolreact.c:2766-2779): move apart along
+        // the line joining the reactants
+        double bv[3], dist = 0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          bv[d] = pBv[d] - pAv[d];
+          dist += bv[d] * bv[d];
+        }
+        if (dist == 0) {
+          dist = sqrt(rx->bindrad2);
+          bv[0] = dist;
+        } else {
+          dist = sqrt(dist);
+        }
+        const double bx = rx->bparam >= 0 ? 1.0 / dist - (1.0 / rx->bparam)
+                                          : 2.0 / dist - 2.0 / (rx->prdpos[0] + rx->prdpos[1]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          bq[0][d] = pAv[d] - bx * rx->prdpos[0] * bv[d];
+          bq[1][d] = pBv[d] + bx * rx->prdpos[1] * bv[d];
+        }
+      }
       int k2 = U + 2 * (ee - U);
       const int ks1 = I.L.base[tab->listof[id1]] + B.lpos[ed.s1];
       const int ks2 = I.L.base[tab->listof[id2]] + B.lpos[ed.s2];
@@ -1640,7 +1664,16 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       pr.pad = 0;
       pr.serno = 0;
       double pp[3] = {rxnpos[0], rxnpos[1], rxnpos[2]};
-      if (rx->confspread && order == 2) {
+      const double* px0 = rxnpos;  // where the product was made (posx), start of its surface segment
+      if (rx->bounce && order == 2) {
+        if (q < 2) {  // the reactant moved: same serial, segment from its place
+          pp[0] = bq[q][0];
+          pp[1] = bq[q][1];
+          pp[2] = bq[q][2];
+          pr.serno = q == 0 ? sA : sB;
+          px0 = q == 0 ? pAv : pBv;
+        }
+      } else if (rx->confspread && order == 2) {
         // doreact (smolreact.c:2736-2751): products 0 and 1 replace the reactants
         if (q < 2) {
           const double* src = q == 0 ? pAv : pBv;
@@ -1648,6 +1681,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
           pp[1] = src[1];
           pp[2] = src[2];
           pr.serno = q == 0 ? sA : sB;
+          px0 = src;
         }
       } else if (rx->fixed) {  // fixed placement: offsets are not rotated (smolreact.c:2811-2812)
         pp[0] = rxnpos[0] + rx->prdvec[q][0];
@@ -1675,9 +1709,6 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       } else {
         // checksurfaces for reborn molecules: segment reaction position -> product
         int i2 = pr.ident, bnd = 0;
-        // the segment starts where the product was made (posx): the reaction position,
-        // or the product's own place for a confspread product
-        const double* px0 = pr.serno ? pp : rxnpos;
         int st = srf_check(I.sf, I.g, tab->difc, pp, px0, &i2, ser0 + (uint64_t)pc, step, SMOLRNG_SURF_REBORN,
                            I.key0, I.key1, &bnd);
         if (st == kSrfAdsorbed && I.sf.blist[i2][bstate(bnd)] >= 0) {
@@ -1702,7 +1733,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     row.rct[0] = sA;
     row.rct[1] = sB;
     int pfirst = pc - rx->nprod;
-    const bool cs = rx->confspread && order == 2;
+    const bool cs = (rx->confspread || rx->bounce) && order == 2;
     row.prd[0] = rx->nprod > 0 ? (cs ? sA : ser0 + (uint64_t)pfirst) : 0;
     row.prd[1] = rx->nprod > 1 ? (cs ? sB : ser0 + (uint64_t)pfirst + 1) : 0;
     I.log[e] = row;

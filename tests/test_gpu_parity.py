@@ -321,3 +321,17 @@ def test_offset_placement_rotated_off_axis():
                 molecules=[("A", 1500), ("B", 1500)])
     nev = _run_pair(spec, 60, check_every=10)
     assert nev > 50
+
+
+def test_bounce_placement():
+    """Bounce product placement (A + B -> A + B to separation 1.0, and
+    A + A -> A + A reflected off the binding radius) in a periodic box,
+    wrapped pairs included: the molecules keep their serial numbers and move
+    apart along the line joining them."""
+    spec = dict(low=[0, 0, 0], high=[25, 25, 25], walls="ppp", dt=0.01, seed=59,
+                species={"A": 1.0, "B": 0.5},
+                reactions=[("hitAB", ["A", "B"], ["A", "B"], 30.0), ("hitAA", ["A", "A"], ["A", "A"], 15.0)],
+                placement={"hitAB": ("bounce", 1.0), "hitAA": ("bounce", -1.0)},
+                molecules=[("A", 1500), ("B", 1500)])
+    nev = _run_pair(spec, 60, check_every=10)
+    assert nev > 100
