@@ -96,6 +96,24 @@ enum ErrorCode smolAddSolutionMolecules(simptr sim, const char* species, int num
                                         double* highposition);                /* libsmoldyn.h:114 */
 int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state); /* libsmoldyn.h:117 */
 
+/* ---------------- configuration files (libsmoldyn.h:57-60) ----------------
+ * The statements of the models the device path runs (dim, boundaries,
+ * low_wall/high_wall, random_seed, time_*, species, difc, molecule_lists,
+ * mol_list, molperbox, boxsize, accuracy, max_mol, mol, reaction (incl. <->),
+ * reaction_rate, binding_radius, reaction_probability, confspread_radius,
+ * product_placement, start_surface ... end_surface with action, rate,
+ * rate_internal, panel, jump, output_root, output_files, append_files,
+ * output_data, output_precision, output_format, output_file_number, cmd) plus
+ * define / ifdefine / ifundefine / else / endif / read_file / end_file;
+ * display statements are ignored; anything else fails with file:line in the
+ * error message.  Lists are sized when the simulation is updated: a model
+ * whose population grows past its initial count plus 25 % (at least 4096)
+ * needs max_mol, as the device does not reallocate during a run. */
+simptr smolPrepareSimFromFile(const char* filepath, const char* filename, const char* flags); /* :58 */
+enum ErrorCode smolLoadSimFromFile(const char* filepath, const char* filename, simptr* simpointer,
+                                   const char* flags);                         /* libsmoldyn.h:59 */
+enum ErrorCode smolReadConfigString(simptr sim, const char* statement, char* parameters); /* :60 */
+
 /* ---------------- run-time commands (libsmoldyn.h:85-95) ----------------
  * Command timing codes b a @ i x & I j E e N n B A and their scheduling follow
  * the reference (libSteve/SimCommand.c).  The device path runs the commands

@@ -96,6 +96,9 @@ def lib():
         "smolGetReactionIndexNT": (I, [P, ip, cp]),
         "smolSetReactionRate": (I, [P, cp, D, I]),
         "smolSetReactionProducts": (I, [P, cp, I, D, cp, dp]),
+        "smolPrepareSimFromFile": (P, [cp, cp, cp]),
+        "smolLoadSimFromFile": (I, [cp, cp, C.POINTER(P), cp]),
+        "smolReadConfigString": (I, [P, cp, cp]),
         "smolSetOutputPath": (I, [P, cp]),
         "smolAddOutputFile": (I, [P, cp, I, I]),
         "smolAddOutputData": (I, [P, cp]),
@@ -186,6 +189,31 @@ class Simulation:
         self.p = self.L.smolNewSim(self.dim, _dp(lo), _dp(hi))
         if not self.p:
             raise SmoldynError("smolNewSim: %s" % (last_error(),))
+
+    @classmethod
+    def from_file(cls, path, update=True):
+        """A simulation read from a configuration file: smolPrepareSimFromFile
+        (read and update) or smolLoadSimFromFile (read only)."""
+        import os.path
+        d, f = os.path.split(str(path))
+        self = cls.__new__(cls)
+        self.L = lib()
+        self.dim = 3
+        if update:
+            self.p = self.L.smolPrepareSimFromFile((d + "/" if d else "").encode(), f.encode(), b"")
+            if not self.p:
+                raise SmoldynError("smolPrepareSimFromFile: %s" % (last_error(),))
+        else:
+            ptr = C.c_void_p(None)
+            self.p = None
+            _check(self.L.smolLoadSimFromFile((d + "/" if d else "").encode(), f.encode(), C.byref(ptr), b""),
+                   "smolLoadSimFromFile")
+            self.p = ptr.value
+        return self
+
+    def read_config(self, statement, parameters=""):
+        """One configuration statement (smolReadConfigString)."""
+        _check(self.L.smolReadConfigString(self.p, statement.encode(), parameters.encode()), "smolReadConfigString")
 
     def close(self):
         if getattr(self, "p", None):

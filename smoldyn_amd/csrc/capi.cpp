@@ -20,6 +20,7 @@
 #include "../../include/libsmoldyn_amd.h"
 #include "../../include/smoldyn_amd_rng.h"
 #include "commands.h"
+#include "config_internal.h"
 #include "engine.h"
 #include "rxnparam.h"
 #include "surfaces_host.h"
@@ -1460,6 +1461,51 @@ int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state)
   return counts[i];
 }
 
+// ---- configuration files (libsmoldyn.cpp:306-375)
+
+simptr smolPrepareSimFromFile(const char* filepath, const char* filename, const char* flags) {
+  const char* fn = "smolPrepareSimFromFile";
+  (void)flags;
+  if (!filename) {
+    set_error(fn, ECmissing, "missing filename");
+    return nullptr;
+  }
+  simptr sim = nullptr;
+  std::string err;
+  if (config_load(filepath, filename, &sim, &err)) {
+    set_error(fn, ECerror, "%s", err.c_str());
+    return nullptr;
+  }
+  if (do_update(sim, fn) != ECok) {
+    smolFreeSim(sim);
+    set_error(fn, ECerror, "Failed to update simulation");
+    return nullptr;
+  }
+  return sim;
+}
+
+enum ErrorCode smolLoadSimFromFile(const char* filepath, const char* filename, simptr* simpointer, const char* flags) {
+  const char* fn = "smolLoadSimFromFile";
+  (void)flags;
+  LCHECK(filename, fn, ECmissing, "missing filename");
+  LCHECK(simpointer, fn, ECmissing, "missing simpointer");
+  std::string err;
+  LCHECK(config_load(filepath, filename, simpointer, &err) == 0, fn, ECerror, "%s", err.c_str());
+  return ECok;
+}
+
+enum ErrorCode smolReadConfigString(simptr sim, const char* statement, char* parameters) {
+  const char* fn = "smolReadConfigString";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(statement, fn, ECmissing, "missing statement");
+  std::string cursurf, err;
+  LCHECK(strcmp(statement, "start_surface") != 0, fn, ECerror,
+         "Error in configuration string: surface blocks need a configuration file");
+  LCHECK(config_statement(sim, statement, parameters ? parameters : "", &cursurf, &err) == 0, fn, ECerror,
+         "Error in configuration string: %s", err.c_str());
+  return ECok;
+}
+
 // ---- run-time commands and output (libsmoldyn.cpp:656-798)
 
 enum ErrorCode smolSetOutputPath(simptr sim, const char* path) {
@@ -2270,3 +2316,43 @@ extern "C" enum ErrorCode smolamdEnsembleGetDiagnostics(ensembleptr e, int membe
   out[6] = (double)c.molsteps;
   return ECok;
 }
+
+namespace smolamd {
+
+int sim_dim(simptr sim) { return sim->dim; }
+
+void sim_bounds(simptr sim, double* lo, double* hi) {
+  for (int d = 0; d < 3; d++) {
+    lo[d] = sim->low[d];
+    hi[d] = sim->high[d];
+  }
+}
+
+std::vector<std::string> sim_species_names(simptr sim) {
+  return std::vector<std::string>(sim->spname.begin() + 1, sim->spname.end());
+}
+
+std::string sim_product_name(simptr sim, const char* reaction, int prd) {
+  for (int o = 0; o < 3; o++)
+    for (auto& r : sim->rxn[o])
+      if (r.name == reaction) return prd >= 0 && prd < r.nprod ? sim->spname[(size_t)r.prd[prd]] : std::string();
+  return std::string();
+}
+
+void sim_set_accuracy(simptr sim, double accuracy) {
+  sim->accur = accuracy;
+  invalidate(sim);
+}
+
+void sim_set_output_precision(simptr sim, int precision) { sim->cmds.precision = precision; }
+
+int sim_set_output_format(simptr sim, const char* format) {
+  if (!strcmp(format, "ssv") || !strcmp(format, "SSV")) sim->cmds.outformat = 's';
+  else if (!strcmp(format, "csv") || !strcmp(format, "CSV")) sim->cmds.outformat = 'c';
+  else return 1;
+  return 0;
+}
+
+int sim_set_file_suffix(simptr sim, const char* name, int suffix) { return cmd::set_suffix(sim->cmds, name, suffix); }
+
+}  // namespace smolamd

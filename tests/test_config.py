@@ -1,0 +1,113 @@
+"""Configuration files through the C-ABI (smolLoadSimFromFile,
+smolReadConfigString), on the CPU: the reader builds the simulation with the
+libsmoldyn calls a caller would make, so what it builds is read back through
+the same API before any device exists."""
+import gzip
+from pathlib import Path
+
+import pytest
+
+import smoldyn_amd as sa
+
+S95 = Path(__file__).resolve().parent / "golden" / "s95"
+
+
+def _load(tmp_path, text):
+    f = tmp_path / "model.txt"
+    f.write_text(text)
+    return sa.Simulation.from_file(f, update=False)
+
+
+def test_reference_models_load():
+    """The reference's own regression models that the device runs read
+    without error; molecule counts, lists and commands come through."""
+    for name, counts in [("lotvolt", {"rabbit": 1000, "fox": 1000}),
+                         ("molecule", {"red": 50, "green": 50, "blue": 50, "yellow": 50, "magenta": 50,
+                                       "cyan": 50})]:
+        s = sa.Simulation.from_file(S95 / (name + ".txt"), update=False)
+        for sp, n in counts.items():
+            assert s.count(sp, sa.MSall) == n, (name, sp)
+        s.close()
+
+
+def test_statements_and_blocks(tmp_path):
+    text = """# a model with most statement kinds
+define SIDE 50
+dim 3
+boundaries x 0 SIDE p
+boundaries y 0 SIDE
+low_wall z 0 r
+high_wall z 50 r
+random_seed 7
+time_start 0
+time_stop 1   /* a block comment
+over two lines */ 
+time_step 0.01
+species A B C
+difc all 1
+difc C 0.5
+molecule_lists la lbc
+mol_list A la
+mol_list B lbc
+mol_list C lbc
+molperbox 4
+reaction fwd A + B -> C 20
+reaction back C -> A + B 2
+product_placement back pgemmaxw 0.2
+reaction pair A + C <-> B + B 5 1
+mol 100 A u u u
+mol 20 B 10-20 5 u
+ifdefine NOT_SET
+  mol 1000 C u u u
+else
+  mol 3 C 25 25 25
+endif
+graphics opengl
+color A red
+output_data counts
+output_files out.txt
+output_precision 6
+output_format csv
+cmd N 10 molcount counts
+cmd a listmols out.txt
+start_surface wall
+  action both all reflect
+  panel rect +2 0 0 25 50 50 r1
+  panel sph 25 25 25 5 20 20
+end_surface
+end_file
+mol 5 A u u u
+"""
+    s = _load(tmp_path, text)
+    assert [s.count(x, sa.MSall) for x in "ABC"] == [100, 20, 3]
+    s.read_config("mol", "4 C 1 2 3")
+    assert s.count("C", sa.MSall) == 7
+    s.run_command("molcount counts")
+    d = s.output_data("counts")
+    assert d.tolist() == [[0.0, 100.0, 20.0, 7.0]]
+    s.close()
+
+
+@pytest.mark.parametrize("stmt,where", [("difm A 1 0 0 0 1 0 0 0 1", "model.txt:6"),
+                                        ("reaction compartment=c r A -> B 1", "model.txt:6"),
+                                        ("start_surface s\n panel cyl 0 0 0 1 1 1 2 20 20\nend_surface",
+                                         "model.txt:7")])
+def test_unsupported_statements_name_their_line(tmp_path, stmt, where):
+    text = "dim 3\nboundaries 0 0 10\nboundaries 1 0 10\nboundaries 2 0 10\n%s\n" % stmt
+    text = "species A B\n" + text
+    with pytest.raises(sa.SmoldynError) as e:
+        _load(tmp_path, text)
+    assert where in str(e.value)
+
+
+def test_read_config_string():
+    s = sa.Simulation([0, 0, 0], [10, 10, 10])
+    s.read_config("species", "X Y")
+    s.read_config("difc", "X 2")
+    s.read_config("mol", "12 X u u u")
+    assert s.count("X", sa.MSall) == 12
+    with pytest.raises(sa.SmoldynError):
+        s.read_config("start_surface", "s")
+    with pytest.raises(sa.SmoldynError):
+        s.read_config("no_such_statement", "1")
+    s.close()
