@@ -100,6 +100,7 @@ struct Prod {
   int sb;  // surfaces: a product adsorbed by the reborn check (pending its list change), else 0
   int pad;
   uint64_t serno;  // a confspread product keeps its reactant's serial number; 0 for a new serial
+  double sx, sy, sz;  // surfaces: where the product was made (start of its reborn surface check)
 };
 
 // Per-species screening constants of the pair search, passed by value.
@@ -1369,7 +1370,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_relist(const Inst* __restri
 }
 
 // One workgroup per simulation of the batch (blockIdx.x).
-template <bool kF32>
+template <bool kF32, bool kSurf>
 __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restrict__ insts, Glob* gl, Recs RA,
                                                              Recs RB, int* __restrict__ inv, Binned bn, int nscan,
                                                              uint64_t* __restrict__ bestkey, int* __restrict__ killed,
@@ -1450,19 +1451,22 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   // count of each reaction is a sum of chunk counts, chunks spread over the
   // threads; the events come first in creation order
   const int nz = tab->nrxn0;
-  if (tid <= kMaxRxn0) s_zpre[tid] = 0;
-  __syncthreads();
-  for (int r = 0; r < nz; r++) {
-    int c = 0;
-    for (int j = tid; j < tab->zm[r]; j += NT)
-      c += smolrng_poisson_chunk(I.key0, I.key1, (uint32_t)r, (uint32_t)j, step, tab->zg[r]);
-    if (c) atomicAdd(&s_zpre[r + 1], c);
+  int Z = 0;
+  if (nz > 0) {  // uniform over the workgroup
+    if (tid <= kMaxRxn0) s_zpre[tid] = 0;
+    __syncthreads();
+    for (int r = 0; r < nz; r++) {
+      int c = 0;
+      for (int j = tid; j < tab->zm[r]; j += NT)
+        c += smolrng_poisson_chunk(I.key0, I.key1, (uint32_t)r, (uint32_t)j, step, tab->zg[r]);
+      if (c) atomicAdd(&s_zpre[r + 1], c);
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int r = 0; r < nz; r++) s_zpre[r + 1] += s_zpre[r];
+    __syncthreads();
+    Z = s_zpre[nz];
   }
-  __syncthreads();
-  if (tid == 0)
-    for (int r = 0; r < nz; r++) s_zpre[r + 1] += s_zpre[r];
-  __syncthreads();
-  int Z = s_zpre[nz];
 
   // ---- order the events: first-order reactions in live-list order, then
   // bimolecular ones in edge-key order
@@ -1704,23 +1708,14 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
         pp[1] = rxnpos[1] + v1[1];
         pp[2] = rxnpos[2] + v1[2];
       }
-      if (!I.sf.on) {
+      if (!kSurf) {
         apply_walls(I.g, pp);  // checkwalls for reborn molecules
-      } else {
-        // checksurfaces for reborn molecules: segment reaction position -> product
-        int i2 = pr.ident, bnd = 0;
-        int st = srf_check(I.sf, I.g, tab->difc, pp, px0, &i2, ser0 + (uint64_t)pc, step, SMOLRNG_SURF_REBORN,
-                           I.key0, I.key1, &bnd);
-        if (st == kSrfAdsorbed && I.sf.blist[i2][bstate(bnd)] >= 0) {
-          // bound, but it stays in its solution list until the next step's first molsort
-          pr.sb = bnd | 4;
-          atomicAdd((bnd & 3) == 1 ? &ctl->countf[i2] : &ctl->countk[i2], 1);
-        } else if (st != kSrfOk || tab->listof[i2] != pr.list) {
-          atomicOr(&ctl->err, kErrSurface);
-        }
-        pr.ident = i2;
+        atomicAdd(&s_dcount[pr.ident], 1);
+      } else {  // checksurfaces for reborn molecules runs below, one thread per product
+        pr.sx = px0[0];
+        pr.sy = px0[1];
+        pr.sz = px0[2];
       }
-      atomicAdd(&s_dcount[pr.ident], 1);
       pr.x = pp[0];
       pr.y = pp[1];
       pr.z = pp[2];
@@ -1739,6 +1734,32 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     I.log[e] = row;
   }
   __syncthreads();
+  if (kSurf) {
+    // checksurfaces for reborn molecules: segment from where each product was
+    // made to its place (draws keyed by the product's serial number)
+    for (int c = tid; c < K; c += NT) {
+      Prod pr = I.prods[c];
+      double pp[3] = {pr.x, pr.y, pr.z};
+      const double px0[3] = {pr.sx, pr.sy, pr.sz};
+      int i2 = pr.ident, bnd = 0;
+      const int st = srf_check(I.sf, I.g, tab->difc, pp, px0, &i2, ser0 + (uint64_t)c, step, SMOLRNG_SURF_REBORN,
+                               I.key0, I.key1, &bnd);
+      if (st == kSrfAdsorbed && I.sf.blist[i2][bstate(bnd)] >= 0) {
+        // bound, but it stays in its solution list until the next step's first molsort
+        pr.sb = bnd | 4;
+        atomicAdd((bnd & 3) == 1 ? &ctl->countf[i2] : &ctl->countk[i2], 1);
+      } else if (st != kSrfOk || tab->listof[i2] != pr.list) {
+        atomicOr(&ctl->err, kErrSurface);
+      }
+      pr.ident = i2;
+      pr.x = pp[0];
+      pr.y = pp[1];
+      pr.z = pp[2];
+      atomicAdd(&s_dcount[pr.ident], 1);
+      I.prods[c] = pr;
+    }
+    __syncthreads();
+  }
 
   // ---- molsort: sort the killed list slots, fill holes from the list tails
   // (swap-from-tail compaction), then append products in the dead-list order
@@ -2278,7 +2299,8 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   auto kexact = s->multi ? (f ? k_exact<false, true, true> : k_exact<false, true, false>)
                 : s->surf_on ? (f ? k_exact<true, false, true> : k_exact<true, false, false>)
                              : (f ? k_exact<false, false, true> : k_exact<false, false, false>);
-  auto kres = f ? k_resolve<true> : k_resolve<false>;
+  auto kres = s->surf_on ? (f ? k_resolve<true, true> : k_resolve<false, true>)
+                         : (f ? k_resolve<true, false> : k_resolve<false, false>);
   if (timed) (void)hipEventRecord(s->ev[0], st);
   hipLaunchKernelGGL(kdiff, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, s->bn.count, s->gtab,
                      s->ngauss - 1, dinj, s->inv);
