@@ -84,6 +84,22 @@ double osurf_desorbdist_y(double step, double y, int revads);
 int osurf_geo_linexaabb2(const double* pt1, const double* pt2, const double* norm, const double* bpt1,
                          const double* bpt2);
 int osurf_geo_circlexaabb2(const double* cent, double rad, const double* bpt1, const double* bpt2);
+double osurf_geo_linenormal(const double* pt1, const double* pt2, double* ans);
+double osurf_geo_trinormal(const double* pt1, const double* pt2, const double* pt3, double* ans);
+double osurf_geo_unitcross(const double* v1start, const double* v1end, const double* v2start, const double* v2end,
+                           double* ans);
+int osurf_geo_ptintriangle2(const double (*point)[3], const double* test);
+double osurf_geo_linenormal3d(const double* pt1, const double* pt2, const double* point, double* ans);
+double osurf_geo_linexcyls(const double* pt1, const double* pt2, const double* cp1, const double* cp2, double rad,
+                           double* crss2ptr, double* nrdistptr, double* nrposptr);
+int osurf_geo_linexaabb(const double* pt1, const double* pt2, const double* bpt1, const double* bpt2, int dim,
+                        int infline);
+int osurf_geo_cylisxaabb3(const double* pt1, const double* pt2, double rad, const double* bpt1, const double* bpt2);
+int osurf_geo_cylsxaabb3(const double* pt1, const double* pt2, double rad, const double* bpt1, const double* bpt2);
+int osurf_geo_hemisxaabb3(const double* cent, double rad, const double* outvect, const double* bpt1,
+                          const double* bpt2);
+int osurf_geo_trixaabb3(const double* pt1, const double* pt2, const double* pt3, const double* norm,
+                        const double* bpt1, const double* bpt2);
 
 /* rxnparam restatement, exported for the rate-table tests */
 double orxn_numrxnrate(double step, double a, double b);

@@ -6,12 +6,12 @@
  * source/Smoldyn/smolsurface.c, the box-panel lists of smolboxes.c, and the
  * geometry helpers of source/libSteve/Geometry.c and SurfaceParam.c.
  *
- * Panel shapes: rectangle (dims 1-3), triangle (dims 1-2), sphere (dims 1-3)
- * and disk (dim 3).  Actions: reflect, transmit, absorb, jump, and rate-based
- * actions (adsorption with a given probability or rate, desorption, state
- * changes).  Surface-bound diffusion (movemol2closepanel), cylinders,
- * hemispheres, 3D triangles, ports and emitters are not restated; setting them
- * up returns an error.
+ * Panel shapes: rectangle (dims 1-3), triangle (dims 1-3), sphere (dims 1-3),
+ * cylinder, hemisphere and disk (dim 3).  Actions: reflect, transmit, absorb,
+ * jump, and rate-based actions (adsorption with a given probability or rate,
+ * desorption, state changes).  Surface-bound diffusion (movemol2closepanel),
+ * 2D cylinders, hemispheres and disks, ports and emitters are not restated;
+ * setting them up returns an error.
  *
  * Random draws: in SFMT mode the reference's global stream in program order;
  * in counter mode Philox draws keyed by (molecule serial, step, SMOLRNG_SURFACE,
@@ -299,6 +299,299 @@ int osurf_geo_circlexaabb2(const double* cent, double rad, const double* bpt1, c
   return 0;
 }
 
+/* Geo_LineNormal (Geometry.c:58-71): unit normal (dy, -dx) of the line
+   pt1 -> pt2, scaled by the reciprocal of its length */
+double osurf_geo_linenormal(const double* pt1, const double* pt2, double* ans) {
+  double dx = pt2[0] - pt1[0], dy = pt2[1] - pt1[1], len = sqrt(dx * dx + dy * dy), leninv;
+  if (len > 0) {
+    leninv = 1.0 / len;
+    ans[0] = leninv * dy;
+    ans[1] = -leninv * dx;
+  } else {
+    ans[0] = 1;
+    ans[1] = 0;
+  }
+  return len;
+}
+
+/* Geo_TriNormal (Geometry.c:176-195): unit normal (pt2-pt1) x (pt3-pt2);
+   a triangle of area below 50 DBL_EPSILON takes the normal of its first edge
+   in the xy plane.  Returns the area. */
+double osurf_geo_trinormal(const double* pt1, const double* pt2, const double* pt3, double* ans) {
+  double e1[3], e2[3], len;
+  int d;
+  for (d = 0; d < 3; d++) {
+    e1[d] = pt2[d] - pt1[d];
+    e2[d] = pt3[d] - pt2[d];
+  }
+  ans[0] = e1[1] * e2[2] - e1[2] * e2[1];
+  ans[1] = -e1[0] * e2[2] + e1[2] * e2[0];
+  ans[2] = e1[0] * e2[1] - e1[1] * e2[0];
+  len = sqrt(ans[0] * ans[0] + ans[1] * ans[1] + ans[2] * ans[2]);
+  if (len > 100 * DBL_EPSILON) {
+    for (d = 0; d < 3; d++) ans[d] /= len;
+  } else {
+    osurf_geo_linenormal(pt1, pt2, ans);
+    ans[2] = 0;
+  }
+  return len / 2;
+}
+
+/* Geo_UnitCross (Geometry.c:219-247): unit vector of (v1end - v1start) x
+   (v2end - v2start), a NULL start meaning the origin; the zero vector when
+   the cross product's length is at most 100 DBL_EPSILON.  Returns the length. */
+double osurf_geo_unitcross(const double* v1start, const double* v1end, const double* v2start, const double* v2end,
+                           double* ans) {
+  double a[3], b[3], c[3], len;
+  int d;
+  for (d = 0; d < 3; d++) {
+    a[d] = v1start ? v1end[d] - v1start[d] : v1end[d];
+    b[d] = v2start ? v2end[d] - v2start[d] : v2end[d];
+  }
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+  len = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  if (len > 100 * DBL_EPSILON) {
+    for (d = 0; d < 3; d++) ans[d] = c[d] / len;
+    return len;
+  }
+  ans[0] = ans[1] = ans[2] = 0;
+  return 0;
+}
+
+/* Geo_PtInTriangle2 (Geometry.c:474-492): test is in the triangle (vertices
+   point[0..2], outward edge normals point[3..5]) when it is on no edge's
+   outer side */
+int osurf_geo_ptintriangle2(const double (*point)[3], const double* test) {
+  int k;
+  for (k = 0; k < 3; k++) {
+    const double* pt = point[k];
+    const double* en = point[k + 3];
+    double dot = (test[0] - pt[0]) * en[0] + (test[1] - pt[1]) * en[1] + (test[2] - pt[2]) * en[2];
+    if (dot > 0) return 0;
+  }
+  return 1;
+}
+
+/* Geo_TriXaabb3 (Geometry.c:1585-1605): 0 when the triangle's bounding box
+   misses the box bpt1-bpt2 or all 8 box corners lie strictly on one side of
+   the triangle's plane, else 1 */
+int osurf_geo_trixaabb3(const double* pt1, const double* pt2, const double* pt3, const double* norm,
+                        const double* bpt1, const double* bpt2) {
+  double cmp, o[8];
+  int d;
+  for (d = 0; d < 3; d++) {
+    if (pt1[d] < bpt1[d] && pt2[d] < bpt1[d] && pt3[d] < bpt1[d]) return 0;
+    if (pt1[d] > bpt2[d] && pt2[d] > bpt2[d] && pt3[d] > bpt2[d]) return 0;
+  }
+  cmp = norm[0] * pt1[0] + norm[1] * pt1[1] + norm[2] * pt1[2];
+  os_corners(norm, bpt1, bpt2, o);
+  if (os_all_lt(o, cmp)) return 0;
+  if (os_all_gt(o, cmp)) return 0;
+  return 1;
+}
+
+/* Geo_LineNormal3D (Geometry.c:103-160): unit vector from the line pt1-pt2
+   to point, perpendicular to the line, and the distance.  Degenerate cases:
+   pt1 == pt2 gives the unit vector to point (unit x when point == pt1); a
+   point on the line gives (line_y, -line_x, 0) normalized, or unit x for a
+   line along z, and distance 0. */
+double osurf_geo_linenormal3d(const double* pt1, const double* pt2, const double* point, double* ans) {
+  const double eps = 100 * DBL_EPSILON;
+  double line[3], dot;
+  int d;
+  for (d = 0; d < 3; d++) line[d] = pt2[d] - pt1[d];
+  dot = line[0] * line[0] + line[1] * line[1] + line[2] * line[2];
+  if (dot < eps) {
+    for (d = 0; d < 3; d++) ans[d] = point[d] - pt1[d];
+    dot = ans[0] * ans[0] + ans[1] * ans[1] + ans[2] * ans[2];
+    if (dot < eps) {
+      ans[0] = 1;
+      ans[1] = ans[2] = 0;
+      return 0;
+    }
+    dot = sqrt(dot);
+    for (d = 0; d < 3; d++) ans[d] /= dot;
+    return dot;
+  }
+  dot = sqrt(dot);
+  for (d = 0; d < 3; d++) line[d] /= dot;
+  for (d = 0; d < 3; d++) ans[d] = point[d] - pt1[d];
+  for (int pass = 0; pass < 2; pass++) { /* projection removed twice */
+    dot = ans[0] * line[0] + ans[1] * line[1] + ans[2] * line[2];
+    for (d = 0; d < 3; d++) ans[d] -= dot * line[d];
+  }
+  dot = ans[0] * ans[0] + ans[1] * ans[1] + ans[2] * ans[2];
+  if (dot < eps) {
+    if (line[0] == 0 && line[1] == 0) {
+      ans[0] = 1;
+      ans[1] = ans[2] = 0;
+      return 0;
+    }
+    ans[0] = line[1];
+    ans[1] = -line[0];
+    ans[2] = 0;
+    dot = sqrt(ans[0] * ans[0] + ans[1] * ans[1] + ans[2] * ans[2]);
+    for (d = 0; d < 3; d++) ans[d] /= dot;
+    return 0;
+  }
+  dot = sqrt(dot);
+  for (d = 0; d < 3; d++) ans[d] /= dot;
+  return dot;
+}
+
+/* Geo_LineXCyls (Geometry.c:1248-1258): crossings of the segment pt1-pt2
+   with the infinite cylinder of axis cp1-cp2 and radius rad, found as the
+   crossings of the segment's perpendicular offsets from the axis with a
+   circle about the origin */
+double osurf_geo_linexcyls(const double* pt1, const double* pt2, const double* cp1, const double* cp2, double rad,
+                           double* crss2ptr, double* nrdistptr, double* nrposptr) {
+  double r1, r2, n1[3], n2[3], zero[3] = {0, 0, 0};
+  int d;
+  r1 = osurf_geo_linenormal3d(cp1, cp2, pt1, n1);
+  r2 = osurf_geo_linenormal3d(cp1, cp2, pt2, n2);
+  for (d = 0; d < 3; d++) {
+    n1[d] *= r1;
+    n2[d] *= r2;
+  }
+  return osurf_geo_linexsphs(n1, n2, zero, rad, 3, crss2ptr, nrdistptr, nrposptr);
+}
+
+/* Geo_LineXaabb (Geometry.c:1561-1582): slab test of the segment (or, with
+   infline, the whole line) through pt1, pt2 against the box bpt1-bpt2 */
+int osurf_geo_linexaabb(const double* pt1, const double* pt2, const double* bpt1, const double* bpt2, int dim,
+                        int infline) {
+  double nearhi = -1.0, farlo = 2.0, denom, a1, a2;
+  int d;
+  for (d = 0; d < dim; d++) {
+    denom = pt2[d] - pt1[d];
+    if (denom == 0) {
+      if (pt1[d] < bpt1[d] || pt1[d] > bpt2[d]) return 0;
+    } else {
+      a1 = (bpt1[d] - pt1[d]) / denom;
+      a2 = (bpt2[d] - pt1[d]) / denom;
+      if (a1 < a2) {
+        if (a1 > nearhi) nearhi = a1;
+        if (a2 < farlo) farlo = a2;
+      } else {
+        if (a2 > nearhi) nearhi = a2;
+        if (a1 < farlo) farlo = a1;
+      }
+    }
+  }
+  if (infline) return nearhi <= farlo;
+  return nearhi <= farlo && nearhi <= 1 && farlo >= 0;
+}
+
+/* Geo_CylisXaabb3 (Geometry.c:1777-1916): does the infinite cylinder of axis
+   pt1-pt2 and radius rad meet the box?  The box corners are projected onto
+   the plane perpendicular to the axis (axes v1, v2); separating-axis tests
+   along the three projected box edges, then the corner distances to the
+   circle, the axis line against the box, and the 12 projected box edges
+   against the circle. */
+int osurf_geo_cylisxaabb3(const double* pt1, const double* pt2, double rad, const double* bpt1, const double* bpt2) {
+  /* projected box edges of test 5, as corner index pairs (corner k takes
+     bpt2 in x, y, z for bits 4, 2, 1) */
+  static const int edge[12][2] = {{0, 1}, {1, 3}, {3, 2}, {2, 0}, {0, 4}, {1, 5},
+                                  {2, 6}, {3, 7}, {4, 5}, {5, 7}, {7, 6}, {6, 4}};
+  static const int axis[3] = {1, 2, 4};
+  double v1[3], v2[3], v3[3], cent[2], ab[8][2], o1[8], o2[8], nrm, b[8], cmp, r2, d2, mn, mx, crss1, crss2;
+  int k, a;
+  osurf_geo_linenormal3d(pt1, pt2, pt1, v1);
+  v3[0] = pt2[0] - pt1[0];
+  v3[1] = pt2[1] - pt1[1];
+  v3[2] = pt2[2] - pt1[2];
+  nrm = sqrt(v3[0] * v3[0] + v3[1] * v3[1] + v3[2] * v3[2]);
+  if (!nrm) return 0;
+  for (k = 0; k < 3; k++) v3[k] /= nrm;
+  v2[0] = v3[1] * v1[2] - v3[2] * v1[1];
+  v2[1] = v3[2] * v1[0] - v3[0] * v1[2];
+  v2[2] = v3[0] * v1[1] - v3[1] * v1[0];
+  cent[0] = v1[0] * pt1[0] + v1[1] * pt1[1] + v1[2] * pt1[2];
+  cent[1] = v2[0] * pt1[0] + v2[1] * pt1[1] + v2[2] * pt1[2];
+  os_corners(v1, bpt1, bpt2, o1);
+  os_corners(v2, bpt1, bpt2, o2);
+  for (k = 0; k < 8; k++) {
+    ab[k][0] = o1[k];
+    ab[k][1] = o2[k];
+  }
+  for (a = 0; a < 3; a++) { /* test 1 */
+    double u[2];
+    u[0] = ab[axis[a]][0] - ab[0][0];
+    u[1] = ab[axis[a]][1] - ab[0][1];
+    nrm = sqrt(u[0] * u[0] + u[1] * u[1]);
+    if (nrm > 0) {
+      u[0] /= nrm;
+      u[1] /= nrm;
+      for (k = 0; k < 8; k++) b[k] = u[0] * ab[k][0] + u[1] * ab[k][1];
+      cmp = u[0] * cent[0] + u[1] * cent[1] - rad;
+      if (os_all_lt(b, cmp)) return 0;
+      cmp = u[0] * cent[0] + u[1] * cent[1] + rad;
+      if (os_all_gt(b, cmp)) return 0;
+    }
+  }
+  r2 = rad * rad; /* tests 2, 3 */
+  mn = mx = 0;
+  for (k = 0; k < 8; k++) {
+    d2 = (ab[k][0] - cent[0]) * (ab[k][0] - cent[0]) + (ab[k][1] - cent[1]) * (ab[k][1] - cent[1]);
+    if (k == 0) mn = mx = d2;
+    else if (d2 < mn) mn = d2;
+    else if (d2 > mx) mx = d2;
+  }
+  if (r2 > mx) return 0;
+  if (r2 >= mn) return 1;
+  if (osurf_geo_linexaabb(pt1, pt2, bpt1, bpt2, 3, 1)) return 1; /* test 4 */
+  for (k = 0; k < 12; k++) {                                    /* test 5 */
+    crss1 = osurf_geo_linexsphs(ab[edge[k][0]], ab[edge[k][1]], cent, rad, 2, &crss2, NULL, NULL);
+    if ((crss1 >= 0 && crss1 <= 1) || (crss2 >= 0 && crss2 <= 1)) return 1;
+  }
+  return 0;
+}
+
+/* Geo_CylsXaabb3 (Geometry.c:1964-1970) with Geo_Cyl2Rect (:1137-1162): the
+   infinite cylinder test, then the rectangle through the axis spanned by the
+   axis and the cylinder's diameter along v2 = v1 x axis */
+int osurf_geo_cylsxaabb3(const double* pt1, const double* pt2, double rad, const double* bpt1, const double* bpt2) {
+  double v1[3], v2[3], nrm, r[4][3];
+  int d;
+  if (!osurf_geo_cylisxaabb3(pt1, pt2, rad, bpt1, bpt2)) return 0;
+  osurf_geo_linenormal3d(pt1, pt2, pt1, v1);
+  v2[0] = v1[1] * (pt2[2] - pt1[2]) - v1[2] * (pt2[1] - pt1[1]);
+  v2[1] = v1[2] * (pt2[0] - pt1[0]) - v1[0] * (pt2[2] - pt1[2]);
+  v2[2] = v1[0] * (pt2[1] - pt1[1]) - v1[1] * (pt2[0] - pt1[0]);
+  nrm = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+  for (d = 0; d < 3; d++) v2[d] /= nrm;
+  for (d = 0; d < 3; d++) {
+    r[0][d] = pt1[d] + rad * (-v1[d] - v2[d]);
+    r[1][d] = pt1[d] + rad * (+v1[d] - v2[d]);
+    r[2][d] = pt1[d] + rad * (-v1[d] + v2[d]);
+    r[3][d] = pt2[d] + rad * (-v1[d] - v2[d]);
+  }
+  return osurf_geo_rectxaabb3(r[0], r[1], r[2], r[3], bpt1, bpt2);
+}
+
+/* Geo_HemisXaabb3 (Geometry.c:1956-1962) with Geo_Hemis2Rect (:1113-1135):
+   the sphere test, then the box of side 2 rad x 2 rad x rad behind the
+   hemisphere's opening (outvect points away from the dome) */
+int osurf_geo_hemisxaabb3(const double* cent, double rad, const double* outvect, const double* bpt1,
+                          const double* bpt2) {
+  double v1[3], v2[3], zero[3] = {0, 0, 0}, r[4][3];
+  int d;
+  if (!osurf_geo_sphsxaabb3(cent, rad, bpt1, bpt2)) return 0;
+  osurf_geo_linenormal3d(zero, outvect, zero, v1);
+  v2[0] = v1[1] * outvect[2] - v1[2] * outvect[1];
+  v2[1] = v1[2] * outvect[0] - v1[0] * outvect[2];
+  v2[2] = v1[0] * outvect[1] - v1[1] * outvect[0];
+  for (d = 0; d < 3; d++) {
+    r[0][d] = cent[d] + rad * (-v1[d] - v2[d]);
+    r[1][d] = cent[d] + rad * (+v1[d] - v2[d]);
+    r[2][d] = cent[d] + rad * (-v1[d] + v2[d]);
+    r[3][d] = cent[d] + rad * (-v1[d] - v2[d] - outvect[d]);
+  }
+  return osurf_geo_rectxaabb3(r[0], r[1], r[2], r[3], bpt1, bpt2);
+}
+
 /* surfaceprob, IrrDes branch (SurfaceParam.c:159-163) */
 static double os_surfaceprob_irrdes(double k1, double k2, double dt) {
   if (k1 == -1) return 1;
@@ -490,7 +783,8 @@ int osim_set_surface_rate(osim* sim, int s, int ident, int ms, int ms1, int ms2,
   return 0;
 }
 
-/* surfaddpanel (smolsurface.c:2432-2640) for rect, tri (dim 1-2), sph, disk
+/* surfaddpanel (smolsurface.c:2432-2640) for rect, tri, sph (all dims), cyl,
+   hemi, disk (dim 3)
    (dim 3).  rectaxis is the rectangle's "+0"/"-1"... string.  Returns the
    panel id or <0. */
 int osim_add_panel(osim* sim, int s, int ps, const char* rectaxis, const double* params) {
@@ -581,22 +875,25 @@ int osim_add_panel(osim* sim, int s, int ps, const char* rectaxis, const double*
       point[0][0] = params[0];
       front[0] = (double)1;
     } else if (dim == 2) {
-      double dx, dy, len;
       point[0][0] = params[0];
       point[0][1] = params[1];
       point[1][0] = params[2];
       point[1][1] = params[3];
-      /* Geo_LineNormal (Geometry.c): unit normal (dy,-dx)/len */
-      dx = point[1][0] - point[0][0];
-      dy = point[1][1] - point[0][1];
-      len = sqrt(dx * dx + dy * dy);
-      front[0] = dy / len;
-      front[1] = -dx / len;
+      osurf_geo_linenormal(point[0], point[1], front);
       point[2][0] = front[1];
       point[2][1] = -front[0];
       point[3][0] = -front[1];
       point[3][1] = front[0];
-    } else return -2;
+    } else {
+      /* vertices, the unit normal, then the outward unit normal of each edge
+         (smolsurface.c:2521-2534) */
+      for (pt = 0; pt < 3; pt++)
+        for (d = 0; d < 3; d++) point[pt][d] = params[3 * pt + d];
+      osurf_geo_trinormal(point[0], point[1], point[2], front);
+      osurf_geo_unitcross(point[0], point[1], NULL, front, point[3]);
+      osurf_geo_unitcross(point[1], point[2], NULL, front, point[4]);
+      osurf_geo_unitcross(point[2], point[0], NULL, front, point[5]);
+    }
   } else if (ps == OPS_SPH) {
     double sg;
     if (dim == 1) {
@@ -622,6 +919,39 @@ int osim_add_panel(osim* sim, int s, int ps, const char* rectaxis, const double*
       sg = params[3];
     }
     front[0] = (double)(sg > 0 ? 1 : sg < 0 ? -1 : 0);
+  } else if (ps == OPS_CYL) {
+    double lengthinv, len = 0;
+    if (dim != 3) return -2;
+    for (pt = 0; pt < 2; pt++)
+      for (d = 0; d < 3; d++) point[pt][d] = params[3 * pt + d];
+    if (point[0][0] == point[1][0] && point[0][1] == point[1][1] && point[0][2] == point[1][2]) return -5;
+    point[2][0] = fabs(params[6]);
+    point[2][1] = params[7];
+    if (point[2][1] <= 0) return -4;
+    point[2][2] = params[8];
+    if (point[2][2] <= 0) return -4;
+    front[2] = (double)(params[6] > 0 ? 1 : params[6] < 0 ? -1 : 0);
+    for (d = 0; d < 3; d++) len += (point[1][d] - point[0][d]) * (point[1][d] - point[0][d]); /* Geo_LineLength */
+    lengthinv = 1.0 / sqrt(len);
+    for (d = 0; d < 3; d++) {
+      point[3][d] = -(point[1][d] - point[0][d]) * lengthinv;
+      point[4][d] = (point[1][d] - point[0][d]) * lengthinv;
+    }
+  } else if (ps == OPS_HEMI) {
+    double len = 0;
+    if (dim != 3) return -2;
+    for (d = 0; d < 3; d++) point[0][d] = params[d];
+    point[1][0] = fabs(params[3]);
+    point[1][1] = params[7];
+    if (point[1][1] <= 0) return -4;
+    point[1][2] = params[8];
+    if (point[1][2] <= 0) return -4;
+    for (d = 0; d < 3; d++) point[2][d] = params[4 + d];
+    for (d = 0; d < 3; d++) len += point[2][d] * point[2][d]; /* normalizeVD (Rn.c:500-508) */
+    if (!len) return -6;
+    len = sqrt(len);
+    for (d = 0; d < 3; d++) point[2][d] /= len;
+    front[0] = (double)(params[3] > 0 ? 1 : params[3] < 0 ? -1 : 0);
   } else if (ps == OPS_DISK) {
     double len;
     if (dim != 3) return -2;
@@ -673,9 +1003,13 @@ static int os_panelside(const osim* sim, const double* pt, const opanel* pnl, do
     dist = pnl->front[0] * (pt[d] - pnl->point[0][d]);
   } else if (pnl->ps == OPS_TRI || pnl->ps == OPS_DISK) {
     for (d = 0; d < dim; d++) dist += (pt[d] - pnl->point[0][d]) * pnl->front[d];
-  } else if (pnl->ps == OPS_SPH) {
+  } else if (pnl->ps == OPS_SPH || pnl->ps == OPS_HEMI) {
     for (d = 0; d < dim; d++) dist += (pt[d] - pnl->point[0][d]) * (pt[d] - pnl->point[0][d]);
     dist = pnl->front[0] * (sqrt(dist) - pnl->point[1][0]);
+  } else if (pnl->ps == OPS_CYL) {
+    double cylnorm[3];
+    dist = osurf_geo_linenormal3d(pnl->point[0], pnl->point[1], pt, cylnorm);
+    dist = pnl->front[2] * (dist - pnl->point[2][0]);
   }
   if (dist > 0) face = OPF_FRONT;
   else if (!strict || dist < 0) face = OPF_BACK;
@@ -684,7 +1018,7 @@ static int os_panelside(const osim* sim, const double* pt, const opanel* pnl, do
   return face;
 }
 
-/* lineXpanel (smolsurface.c:3999-4101) for rect, tri (dim 1-2), sph, disk */
+/* lineXpanel (smolsurface.c:3999-4101) */
 static int os_lineXpanel(const osim* sim, const double* pt1, const double* pt2, const opanel* pnl, double* crsspt,
                          int* face1ptr, int* face2ptr, double* crossptr, double* cross2ptr) {
   int dim = sim->dim, d, intsct, face1, face2, facein;
@@ -716,6 +1050,7 @@ static int os_lineXpanel(const osim* sim, const double* pt1, const double* pt2, 
     cross = dist1 / (dist1 - dist2);
     for (d = 0; d < dim; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
     if (dim == 1) intsct = 1;
+    else if (dim == 3) intsct = osurf_geo_ptintriangle2(point, crsspt);
     else {
       double v1[2] = {point[1][0] - point[0][0], point[1][1] - point[0][1]};
       double v2[2] = {crsspt[0] - point[0][0], crsspt[1] - point[0][1]};
@@ -723,14 +1058,41 @@ static int os_lineXpanel(const osim* sim, const double* pt1, const double* pt2, 
       intsct = v1[0] * v2[0] + v1[1] * v2[1] >= 0;
       intsct = intsct && (v1[0] * v3[0] + v1[1] * v3[1] <= 0);
     }
-  } else if (pnl->ps == OPS_SPH) {
+  } else if (pnl->ps == OPS_SPH || pnl->ps == OPS_HEMI) {
     facein = front[0] > 0 ? OPF_BACK : OPF_FRONT;
     if (face1 == facein && face2 == facein) return 0;
     cross = osurf_geo_linexsphs(pt1, pt2, point[0], point[1][0], dim, &cross2, &nrdist, &nrpos);
     if (face1 == face2 && (!(nrdist <= point[1][0]) || !(nrpos >= 0) || !(nrpos <= 1))) return 0;
     if (face1 == facein) cross = cross2;
     for (d = 0; d < dim; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
-    intsct = 1;
+    if (pnl->ps == OPS_SPH) intsct = 1;
+    else { /* hemisphere: the crossing must be on the dome side of the opening */
+      dot = 0;
+      for (d = 0; d < dim; d++) dot += (crsspt[d] - point[0][d]) * point[2][d];
+      intsct = (dot <= 0);
+      if (!intsct && face1 == face2) {
+        cross = cross2;
+        face1 = (face1 == OPF_FRONT) ? OPF_BACK : OPF_FRONT;
+        for (d = 0; d < dim; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
+        dot = 0;
+        for (d = 0; d < dim; d++) dot += (crsspt[d] - point[0][d]) * point[2][d];
+        intsct = (dot <= 0);
+      }
+    }
+  } else if (pnl->ps == OPS_CYL) {
+    facein = (int)front[2] > 0 ? OPF_BACK : OPF_FRONT;
+    if (face1 == facein && face1 == face2) return 0;
+    cross = osurf_geo_linexcyls(pt1, pt2, point[0], point[1], point[2][0], &cross2, &nrdist, &nrpos);
+    if (face1 == face2 && (!(nrdist <= point[2][0]) || !(nrpos >= 0) || !(nrpos <= 1))) return 0;
+    if (face1 == facein) cross = cross2;
+    for (d = 0; d < dim; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
+    intsct = osurf_geo_ptinslab(point[0], point[1], crsspt, dim);
+    if (!intsct && face1 == face2) {
+      cross = cross2;
+      face1 = (face1 == OPF_FRONT) ? OPF_BACK : OPF_FRONT;
+      for (d = 0; d < dim; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
+      intsct = osurf_geo_ptinslab(point[0], point[1], crsspt, dim);
+    }
   } else if (pnl->ps == OPS_DISK) {
     if (face1 == face2) return 0;
     cross = dist1 / (dist1 - dist2);
@@ -754,8 +1116,12 @@ static void os_getpanelnormal(const osim* sim, const double* pt, const opanel* p
     norm[(int)pnl->front[1]] = pnl->front[0];
   } else if (pnl->ps == OPS_TRI || pnl->ps == OPS_DISK) {
     for (d = 0; d < dim; d++) norm[d] = pnl->front[d];
-  } else if (pnl->ps == OPS_SPH) {
+  } else if (pnl->ps == OPS_SPH || pnl->ps == OPS_HEMI) {
     osurf_geo_spherenormal(pnl->point[0], pt, (int)pnl->front[0], dim, norm);
+  } else if (pnl->ps == OPS_CYL) {
+    osurf_geo_linenormal3d(pnl->point[0], pnl->point[1], pt, norm);
+    if (pnl->front[2] == -1)
+      for (d = 0; d < dim; d++) norm[d] *= -1;
   } else {
     for (d = 0; d < dim; d++) norm[d] = 0;
     norm[0] = 1;
@@ -775,11 +1141,15 @@ static void os_panelnormal(const osim* sim, const opanel* pnl, const double* pos
       for (d = 0; d < dim; d++) norm[d] = pnl->front[d];
     else
       for (d = 0; d < dim; d++) norm[d] = -pnl->front[d];
-  } else if (pnl->ps == OPS_SPH) {
+  } else if (pnl->ps == OPS_SPH || pnl->ps == OPS_HEMI) {
     osurf_geo_spherenormal(
         pnl->point[0], pos,
         ((face == OPF_FRONT && pnl->front[0] == 1) || (face == OPF_BACK && pnl->front[0] == -1)) ? 1 : -1, dim,
         norm);
+  } else if (pnl->ps == OPS_CYL) {
+    osurf_geo_linenormal3d(pnl->point[0], pnl->point[1], pos, norm);
+    if ((face == OPF_BACK && pnl->front[2] == 1) || (face == OPF_FRONT && pnl->front[2] == -1))
+      for (d = 0; d < dim; d++) norm[d] *= -1;
   }
 }
 
@@ -812,9 +1182,16 @@ static int os_ptinpanel(const osim* sim, const double* pt, const opanel* pnl) {
     return in && ((point[1][d] <= pt[d] && pt[d] <= point[2][d]) || (point[2][d] <= pt[d] && pt[d] <= point[1][d]));
   } else if (pnl->ps == OPS_TRI) {
     if (dim == 1) return 1;
+    if (dim == 3) return osurf_geo_ptintriangle2(point, pt);
     return osurf_geo_ptinslab(point[0], point[1], pt, 2);
   } else if (pnl->ps == OPS_SPH) {
     return 1;
+  } else if (pnl->ps == OPS_HEMI) {
+    double dot = 0;
+    for (d = 0; d < dim; d++) dot += (pt[d] - point[0][d]) * point[2][d];
+    return dot <= 0;
+  } else if (pnl->ps == OPS_CYL) {
+    return osurf_geo_ptinslab(point[0], point[1], pt, dim);
   } else if (pnl->ps == OPS_DISK) {
     double len2 = 0, dot = 0;
     for (d = 0; d < dim; d++) len2 += (pt[d] - point[0][d]) * (pt[d] - point[0][d]);
@@ -838,8 +1215,9 @@ static void os_surfacereflect(const osim* sim, omol* mp, const opanel* pnl, cons
     dot = 0;
     for (d = 0; d < dim; d++) dot += (pos[d] - crsspt[d]) * pnl->front[d];
     for (d = 0; d < dim; d++) pos[d] -= 2.0 * pnl->front[d] * dot;
-  } else if (pnl->ps == OPS_SPH) {
-    osurf_geo_spherenormal(pnl->point[0], crsspt, 1, dim, norm);
+  } else if (pnl->ps == OPS_SPH || pnl->ps == OPS_HEMI || pnl->ps == OPS_CYL) {
+    if (pnl->ps == OPS_CYL) osurf_geo_linenormal3d(pnl->point[0], pnl->point[1], crsspt, norm);
+    else osurf_geo_spherenormal(pnl->point[0], crsspt, 1, dim, norm);
     dot = 0;
     for (d = 0; d < dim; d++) dot += (pos[d] - crsspt[d]) * norm[d];
     for (d = 0; d < dim; d++) pos[d] -= 2.0 * norm[d] * dot;
@@ -875,7 +1253,9 @@ static int os_surfacejump(osim* sim, omol* mp, const opanel* pnl, double* crsspt
     for (d = 0; d < dim; d++) delta[d] -= cent[d];
     dir = (pnl->front[0] == pnl2->front[0]) ? 1 : -1;
   } else if (pnl->ps == OPS_TRI) {
-    /* Geo_TriCenter: mean of the vertices (dim of them) */
+    /* Geo_TriCenter (Geometry.c:43-53): the vertex in 1D, 0.5 times the sum
+       of the ends in 2D, (1.0/3.0) times the sum of the vertices in 3D */
+    const double w = dim == 1 ? 1.0 : dim == 2 ? 0.5 : 1.0 / 3.0;
     for (d = 0; d < dim; d++) {
       int k;
       cent[d] = delta[d] = 0;
@@ -883,17 +1263,21 @@ static int os_surfacejump(osim* sim, omol* mp, const opanel* pnl, double* crsspt
         cent[d] += pnl->point[k][d];
         delta[d] += pnl2->point[k][d];
       }
-      cent[d] /= dim;
-      delta[d] /= dim;
+      cent[d] *= w;
+      delta[d] *= w;
     }
     for (d = 0; d < dim; d++) delta[d] -= cent[d];
     dot = 0;
     for (d = 0; d < dim; d++) dot += pnl->front[d] * pnl2->front[d];
     dir = dot > 0 ? 1 : -1;
-  } else if (pnl->ps == OPS_SPH) {
+  } else if (pnl->ps == OPS_SPH || pnl->ps == OPS_HEMI) {
     for (d = 0; d < dim; d++)
       delta[d] = (crsspt[d] - pnl->point[0][d]) * pnl2->point[1][0] / pnl->point[1][0] + pnl2->point[0][d] - crsspt[d];
     dir = (pnl->front[0] == pnl2->front[0]) ? 1 : -1;
+  } else if (pnl->ps == OPS_CYL) {
+    for (d = 0; d < dim; d++)
+      delta[d] = (crsspt[d] - pnl->point[0][d]) * pnl2->point[2][0] / pnl->point[2][0] + pnl2->point[0][d] - crsspt[d];
+    dir = (pnl->front[2] == pnl2->front[2]) ? 1 : -1;
   } else if (pnl->ps == OPS_DISK) {
     for (d = 0; d < dim; d++) delta[d] = pnl2->point[0][d] - pnl->point[0][d];
     dot = 0;
@@ -1477,7 +1861,7 @@ static int os_panelinbox(const osim* sim, const opanel* pnl, int b) {
     }
     if (pnl->ps == OPS_RECT)
       return osurf_geo_rectxaabb3(pnl->point[0], pnl->point[1], pnl->point[3], pnl->point[0], v1, v2);
-    return -1;
+    return osurf_geo_trixaabb3(pnl->point[0], pnl->point[1], pnl->point[2], pnl->front, v1, v2);
   } else if (pnl->ps == OPS_SPH) {
     const double(*point)[3] = (const double(*)[3])pnl->point;
     if (dim == 1) {
@@ -1488,6 +1872,10 @@ static int os_panelinbox(const osim* sim, const opanel* pnl, int b) {
     }
     if (dim == 2) return osurf_geo_circlexaabb2(point[0], point[1][0], v1, v2);
     return osurf_geo_sphsxaabb3(point[0], point[1][0], v1, v2);
+  } else if (pnl->ps == OPS_CYL) {
+    return osurf_geo_cylsxaabb3(pnl->point[0], pnl->point[1], pnl->point[2][0], v1, v2);
+  } else if (pnl->ps == OPS_HEMI) {
+    return osurf_geo_hemisxaabb3(pnl->point[0], pnl->point[1][0], pnl->point[2], v1, v2);
   } else if (pnl->ps == OPS_DISK) {
     return osurf_geo_diskxaabb3(pnl->point[0], pnl->point[1][0], pnl->front, v1, v2);
   }

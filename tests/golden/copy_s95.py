@@ -8,7 +8,7 @@ from pathlib import Path
 
 SRC = Path("/root/reference/examples/S95_regression")
 DST = Path(__file__).resolve().parent / "s95"
-MODELS = ["bounce2", "molecule", "lotvolt", "bounds1", "surf1", "ReflectToNewSpecies", "allostery"]
+MODELS = ["bounce2", "molecule", "lotvolt", "bounds1", "surf1", "ReflectToNewSpecies", "allostery", "surf3"]
 
 
 def main():

@@ -99,6 +99,16 @@ def lib():
             "osurf_geo_rectxaabb3": (I, [dp, dp, dp, dp, dp, dp]),
             "osurf_geo_sphsxaabb3": (I, [dp, D, dp, dp]),
             "osurf_geo_diskxaabb3": (I, [dp, D, dp, dp, dp]),
+            "osurf_geo_linenormal": (D, [dp, dp, dp]),
+            "osurf_geo_trinormal": (D, [dp, dp, dp, dp]),
+            "osurf_geo_unitcross": (D, [dp, dp, dp, dp, dp]),
+            "osurf_geo_ptintriangle2": (I, [dp, dp]),
+            "osurf_geo_trixaabb3": (I, [dp, dp, dp, dp, dp, dp]),
+            "osurf_geo_linenormal3d": (D, [dp, dp, dp, dp]),
+            "osurf_geo_linexcyls": (D, [dp, dp, dp, dp, D, dp, dp, dp]),
+            "osurf_geo_linexaabb": (I, [dp, dp, dp, dp, I, I]),
+            "osurf_geo_cylsxaabb3": (I, [dp, dp, D, dp, dp]),
+            "osurf_geo_hemisxaabb3": (I, [dp, D, dp, dp, dp]),
             "osurf_desorbdist_y": (D, [D, D, I]),
             "omath_randtable": (None, [dp, I]),
             "omath_erfn": (D, [D]),
@@ -342,6 +352,16 @@ def reflib():
         ("Geo_RectXaabb3", I, [dp, dp, dp, dp, dp, dp]),
         ("Geo_SphsXaabb3", I, [dp, D, dp, dp]),
         ("Geo_DiskXaabb3", I, [dp, D, dp, dp, dp]),
+        ("Geo_LineNormal", D, [dp, dp, dp]),
+        ("Geo_TriNormal", D, [dp, dp, dp, dp]),
+        ("Geo_UnitCross", D, [dp, dp, dp, dp, dp]),
+        ("Geo_PtInTriangle2", I, [C.POINTER(dp), dp]),
+        ("Geo_TriXaabb3", I, [dp, dp, dp, dp, dp, dp]),
+        ("Geo_LineNormal3D", D, [dp, dp, dp, dp]),
+        ("Geo_LineXCyls", D, [dp, dp, dp, dp, D, dp, dp, dp]),
+        ("Geo_LineXaabb", I, [dp, dp, dp, dp, I, I]),
+        ("Geo_CylsXaabb3", I, [dp, dp, D, dp, dp]),
+        ("Geo_HemisXaabb3", I, [dp, D, dp, dp, dp]),
         ("surfaceprob", D, [D, D, D, D, dp, I]),
         ("desorbdist", D, [D, I]),
         ("ref_randCCD", D, []),

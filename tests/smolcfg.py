@@ -16,7 +16,8 @@ IGNORED = {"graphics", "graphic_iter", "graphic_delay", "color", "display_size",
 SURF_IGNORED = {"color", "thickness", "polygon", "shininess", "stipple", "max_panels", "end_surface"}
 # panel parameter counts by shape and dimension (surfreadstring "panel")
 PANEL_NPARAM = {("rect", 1): 1, ("rect", 2): 3, ("rect", 3): 5, ("tri", 1): 1, ("tri", 2): 4, ("tri", 3): 9,
-                ("sph", 1): 2, ("sph", 2): 4, ("sph", 3): 6, ("disk", 2): 5, ("disk", 3): 8}
+                ("sph", 1): 2, ("sph", 2): 4, ("sph", 3): 6, ("cyl", 3): 9, ("hemi", 3): 9, ("disk", 2): 5,
+                ("disk", 3): 8}
 PANEL_SHAPE = {"rect": "rect", "r": "rect", "tri": "tri", "t": "tri", "sph": "sph", "sphere": "sph", "s": "sph",
                "disk": "disk", "d": "disk", "cyl": "cyl", "hemi": "hemi"}
 STATES = {"soln": 0, "fsoln": 0, "front": 1, "back": 2, "up": 3, "down": 4, "bsoln": 5, "all": 6}

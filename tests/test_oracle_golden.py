@@ -10,7 +10,10 @@ placement and serial numbers.  surf1 (1D) and ReflectToNewSpecies (2D) pin the
 surface path: rectangle, triangle and sphere panels, reflect / transmit /
 absorb / jump actions, species change on reflection, box panel lists and the
 box walk along a trajectory (line2nextbox), and bimolecular reactions with
-surfaces present (rxnXsurface).  allostery (2D, immobile molecules) pins
+surfaces present (rxnXsurface).  surf3 (3D) pins every 3D panel shape:
+rectangles, triangles, a sphere, cylinders, hemispheres and disks (their
+crossing tests, normals, reflections, box-panel tests), with reflect, absorb,
+transmit and jump actions.  allostery (2D, immobile molecules) pins
 conformational-spread reactions: the confspread radius, the reaction
 probability 1-exp(-k dt), and products that keep their reactants' places and
 serial numbers."""
@@ -32,7 +35,8 @@ def run_model(name):
     return code, sim.listmols()
 
 
-@pytest.mark.parametrize("name", ["bounce2", "molecule", "lotvolt", "surf1", "ReflectToNewSpecies", "allostery"])
+@pytest.mark.parametrize("name", ["bounce2", "molecule", "lotvolt", "surf1", "ReflectToNewSpecies", "allostery",
+                                  "surf3"])
 def test_s95_output_identical(name):
     code, out = run_model(name)
     assert code == 1

@@ -216,6 +216,101 @@ def test_geo_box_intersection_tests_match(refsurf):
         assert L.osurf_geo_ptinslab(_dp(r1), _dp(r2), _dp(c), 3) == refsurf.Geo_PtInSlab(_dp(r1), _dp(r2), _dp(c), 3)
 
 
+def _ptrs(rows):
+    """double** over the rows of a C-contiguous (n, 3) array"""
+    import ctypes as C
+    dp = C.POINTER(C.c_double)
+    return (dp * len(rows))(*[_dp(r) for r in rows])
+
+
+def test_geo_triangle_functions_bit_exact(refsurf):
+    """3D triangle panels: the normal (Geo_TriNormal, with its small-area
+    fallback to Geo_LineNormal), the outward edge normals (Geo_UnitCross),
+    the in-triangle test of crossing points (Geo_PtInTriangle2) and the box
+    test (Geo_TriXaabb3), on random and on grid-aligned triangles and boxes
+    whose faces, edges and corners coincide."""
+    L = ob.lib()
+    rng = np.random.default_rng(11)
+    for it in range(3000):
+        if it % 3 == 0:
+            t = rng.integers(-3, 4, (3, 3)).astype(float)
+            b1 = rng.integers(-3, 3, 3).astype(float)
+            b2 = b1 + rng.integers(1, 3, 3)
+        else:
+            t = rng.uniform(-10, 10, (3, 3))
+            if it % 7 == 0:
+                t[2] = t[0] + (t[1] - t[0]) * rng.uniform(-2, 2)  # collinear: zero area
+            b1 = rng.uniform(-10, 10, 3)
+            b2 = b1 + rng.uniform(0.1, 6, 3)
+        nl, nr = np.zeros(3), np.zeros(3)
+        ao = L.osurf_geo_linenormal(_dp(t[0]), _dp(t[1]), _dp(nl))
+        ar = refsurf.Geo_LineNormal(_dp(t[0]), _dp(t[1]), _dp(nr))
+        assert ao == ar and np.array_equal(nl[:2].view(np.uint64), nr[:2].view(np.uint64))
+        pts_o, pts_r = np.zeros((6, 3)), np.zeros((6, 3))
+        pts_o[:3] = pts_r[:3] = t
+        fo, fr = np.zeros(3), np.zeros(3)
+        assert L.osurf_geo_trinormal(_dp(t[0]), _dp(t[1]), _dp(t[2]), _dp(fo)) == \
+            refsurf.Geo_TriNormal(_dp(t[0]), _dp(t[1]), _dp(t[2]), _dp(fr))
+        assert np.array_equal(fo.view(np.uint64), fr.view(np.uint64))
+        for k in range(3):
+            uo = L.osurf_geo_unitcross(_dp(t[k]), _dp(t[(k + 1) % 3]), None, _dp(fo), _dp(pts_o[3 + k]))
+            ur = refsurf.Geo_UnitCross(_dp(t[k]), _dp(t[(k + 1) % 3]), None, _dp(fr), _dp(pts_r[3 + k]))
+            assert uo == ur
+        assert np.array_equal(pts_o.view(np.uint64), pts_r.view(np.uint64))
+        for _ in range(4):  # points on the plane, inside, outside and on edges
+            w = rng.uniform(-0.3, 1.3, 3) if rng.random() < 0.7 else np.array([0.5, 0.5, 0.0])
+            w /= w.sum() if abs(w.sum()) > 1e-3 else 1.0
+            test = w @ t
+            assert L.osurf_geo_ptintriangle2(_dp(pts_o), _dp(test)) == refsurf.Geo_PtInTriangle2(_ptrs(pts_r), _dp(test))
+        assert L.osurf_geo_trixaabb3(_dp(t[0]), _dp(t[1]), _dp(t[2]), _dp(fo), _dp(b1), _dp(b2)) == \
+            refsurf.Geo_TriXaabb3(_dp(t[0]), _dp(t[1]), _dp(t[2]), _dp(fr), _dp(b1), _dp(b2))
+
+
+def test_geo_cylinder_hemisphere_functions_bit_exact(refsurf):
+    """Cylinder and hemisphere panels: the axis normal (Geo_LineNormal3D,
+    including points on the axis and axes along z), segment-cylinder
+    crossings (Geo_LineXCyls), the slab test of lines against boxes
+    (Geo_LineXaabb) and the box tests (Geo_CylsXaabb3, Geo_HemisXaabb3)."""
+    L = ob.lib()
+    rng = np.random.default_rng(12)
+    for it in range(3000):
+        p1 = rng.uniform(-10, 10, 3)
+        p2 = p1 + rng.normal(size=3) * 6
+        if it % 5 == 0:
+            p2[:2] = p1[:2]  # axis along z
+        elif it % 5 == 1:
+            p2 = np.round(p2)
+            p1 = np.round(p1)
+        pt = rng.uniform(-10, 10, 3) if it % 4 else p1 + (p2 - p1) * rng.uniform(-1, 2)  # on the axis
+        no, nr = np.zeros(3), np.zeros(3)
+        assert L.osurf_geo_linenormal3d(_dp(p1), _dp(p2), _dp(pt), _dp(no)) == \
+            refsurf.Geo_LineNormal3D(_dp(p1), _dp(p2), _dp(pt), _dp(nr))
+        assert np.array_equal(no.view(np.uint64), nr.view(np.uint64))
+        q1, q2 = rng.uniform(-12, 12, (2, 3))
+        rad = rng.uniform(0.2, 6)
+        o = [np.zeros(1) for _ in range(3)]
+        r = [np.zeros(1) for _ in range(3)]
+        xo = L.osurf_geo_linexcyls(_dp(q1), _dp(q2), _dp(p1), _dp(p2), rad, *[_dp(a) for a in o])
+        xr = refsurf.Geo_LineXCyls(_dp(q1), _dp(q2), _dp(p1), _dp(p2), rad, *[_dp(a) for a in r])
+        vo = np.array([xo] + [a[0] for a in o])
+        vr = np.array([xr] + [a[0] for a in r])
+        assert np.array_equal(vo.view(np.uint64), vr.view(np.uint64)) or \
+            np.array_equal(np.isnan(vo), np.isnan(vr)) and np.array_equal(vo[~np.isnan(vo)], vr[~np.isnan(vr)])
+        b1 = rng.uniform(-10, 10, 3) if it % 3 else np.round(rng.uniform(-10, 10, 3))
+        b2 = b1 + (rng.uniform(0.1, 6, 3) if it % 3 else rng.integers(1, 4, 3))
+        for inf in (0, 1):
+            assert L.osurf_geo_linexaabb(_dp(q1), _dp(q2), _dp(b1), _dp(b2), 3, inf) == \
+                refsurf.Geo_LineXaabb(_dp(q1), _dp(q2), _dp(b1), _dp(b2), 3, inf)
+        assert L.osurf_geo_cylsxaabb3(_dp(p1), _dp(p2), rad, _dp(b1), _dp(b2)) == \
+            refsurf.Geo_CylsXaabb3(_dp(p1), _dp(p2), rad, _dp(b1), _dp(b2))
+        ov = rng.normal(size=3)
+        ov /= np.linalg.norm(ov)
+        if it % 6 == 0:
+            ov = np.array([0.0, 0.0, 1.0 if it % 12 else -1.0])
+        assert L.osurf_geo_hemisxaabb3(_dp(p1), rad, _dp(ov), _dp(b1), _dp(b2)) == \
+            refsurf.Geo_HemisXaabb3(_dp(p1), rad, _dp(ov), _dp(b1), _dp(b2))
+
+
 def test_desorption_distance_matches_reference(refsurf):
     """desorbdist (SurfaceParam.c:202-220) for irreversible desorption (algo
     SPAirrDes=10) and reversible adsorption (SPArevAds=8), same uniform."""
