@@ -225,7 +225,8 @@ class Simulation:
 
     # ---- surfaces (libsmoldyn.h:123-136)
     FACES = {"front": 0, "back": 1, "both": 3}
-    SHAPES = {"rect": 0, "sph": 2, "sphere": 2, "disk": 5}
+    SHAPES = {"rect": 0, "tri": 1, "triangle": 1, "sph": 2, "sphere": 2, "cyl": 3, "cylinder": 3, "hemi": 4,
+              "hemisphere": 4, "disk": 5}
     ACTIONS = {"reflect": 0, "transmit": 1, "absorb": 2, "jump": 3, "port": 4, "mult": 5}
     STATES = {"soln": 0, "fsoln": 0, "front": 1, "back": 2, "up": 3, "down": 4, "bsoln": 5, "all": 6}
 

@@ -1828,11 +1828,11 @@ enum ErrorCode smolAddPanel(simptr sim, const char* surface, enum PanelShape pan
   LCHECK(panelshape >= 0 && panelshape < PSall, fn, ECnonexist, "invalid panel shape");
   if (panelshape == PSrect) LCHECK(axisstring, fn, ECmissing, "missing axisstring");
   LCHECK(params, fn, ECmissing, "missing params");
-  LCHECK(panelshape == PSrect || panelshape == PSsph || panelshape == PSdisk, fn, ECerror,
-         "panel shape %d is not implemented on the device path (rectangles, spheres and disks are)", panelshape);
   int er = sim->surf.add_panel(s, panelshape, panel, axisstring, params);
   LCHECK(er != -3, fn, ECsyntax, "cannot parse axisstring");
   LCHECK(er != -4, fn, ECbounds, "drawing slices and stacks need to be positive");
+  LCHECK(er != -5, fn, ECbounds, "cylinder ends cannot be at the same location");
+  LCHECK(er != -6, fn, ECbounds, "hemisphere outward vector has zero length");
   LCHECK(er != -7, fn, ECbounds, "radius needs to be positive");
   LCHECK(er != -8, fn, ECbounds, "normal vector has zero length");
   LCHECK(er != -9, fn, ECerror, "panel name was used before for a different panel shape");

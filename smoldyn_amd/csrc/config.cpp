@@ -93,10 +93,12 @@ bool species_state(const std::string& tok, std::string* name, MolecState* ms) {
   return !name->empty();
 }
 
+// surfstring2face (smolsurface.c:86-93): any leading part of the word
 bool face_of(const std::string& s, PanelFace* f) {
-  if (s == "front") *f = PFfront;
-  else if (s == "back") *f = PFback;
-  else if (s == "both") *f = PFboth;
+  auto begins = [&](const char* word) { return !s.empty() && std::string(word).compare(0, s.size(), s) == 0; };
+  if (begins("front")) *f = PFfront;
+  else if (begins("back")) *f = PFback;
+  else if (begins("all") || begins("both")) *f = PFboth;
   else return false;
   return true;
 }
@@ -153,12 +155,27 @@ int surface_statement(simptr sim, const std::string& srf, const std::string& wor
   if (word == "action") {
     // new format: species face action [new species]; old: face species action
     if (w.size() < 3) return fail("action format: species face action [new_species]");
+    // the new format when its species is known and its face readable, else
+    // the old one (smolsurface.c surfreadstring "action")
+    auto known = [&](const std::string& t) {
+      std::string nm;
+      MolecState m;
+      return species_state(t, &nm, &m) && (nm == "all" || smolGetSpeciesIndexNT(sim, nm.c_str()) > 0);
+    };
     PanelFace f;
-    std::string sp = w[0], fc = w[1];
-    if (!face_of(fc, &f)) {
-      sp = w[1];
-      fc = w[0];
-      if (!face_of(fc, &f)) return fail("action: face not recognized");
+    std::string sp = w[0];
+    if (!face_of(w[1], &f) || !known(w[0])) {
+      PanelFace f0;
+      const bool face0 = face_of(w[0], &f0), known1 = known(w[1]);
+      if (face0 || known1) {
+        if (!face0) return fail("in action, face name needs to be 'front', 'back', or 'both'");
+        if (!known1) return fail("in action, molecule name not recognized");
+        f = f0;
+        sp = w[1];
+      } else {
+        if (!face_of(w[1], &f)) return fail("in action, face name needs to be 'front', 'back', or 'both'");
+        return fail("in action, molecule name not recognized");
+      }
     }
     std::string name;
     MolecState ms;
@@ -190,7 +207,10 @@ int surface_statement(simptr sim, const std::string& srf, const std::string& wor
     if (sh == "rect" || sh == "r") ps = PSrect, npar = 5;
     else if (sh == "sph" || sh == "sphere" || sh == "s") ps = PSsph, npar = 6;
     else if (sh == "disk" || sh == "d") ps = PSdisk, npar = 8;
-    else return fail("panel shape '" + sh + "' is not run on the device path (rect, sph, disk are)");
+    else if (sh == "tri" || sh == "triangle" || sh == "t") ps = PStri, npar = 9;
+    else if (sh == "cyl" || sh == "cylinder" || sh == "c") ps = PScyl, npar = 9;
+    else if (sh == "hemi" || sh == "hemisphere" || sh == "h") ps = PShemi, npar = 9;
+    else return fail("panel shape '" + sh + "' not recognized");
     size_t k = 1;
     std::string axis;
     if (ps == PSrect) {
@@ -205,7 +225,7 @@ int surface_statement(simptr sim, const std::string& srf, const std::string& wor
       if (num(w[k], &v)) prm.push_back(v);
       else pname = w[k];
     }
-    if (ps != PSrect && prm.size() + 2 == npar) prm.push_back(20);  // drawing slices/stacks default
+    if ((ps == PSsph || ps == PSdisk) && prm.size() + 2 == npar) prm.push_back(20);  // drawing slices/stacks default
     if (ps == PSsph && prm.size() + 1 == npar) prm.push_back(20);
     if (ps == PSdisk && prm.size() + 1 == npar) prm.push_back(20);
     if (prm.size() != npar) return fail("panel " + sh + ": wrong number of parameters");

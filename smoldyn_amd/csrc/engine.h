@@ -119,22 +119,28 @@ enum ErrBits : int {
 };
 
 // ---------------------------------------------------------------------------
-// surfaces (solution-phase molecules; panels: 3D rectangle, sphere, disk)
+// surfaces (solution-phase molecules; 3D panels of every shape)
 
 constexpr int kMaxSurf = 16;
-enum DevPanelShape : int { kPsRect = 0, kPsSph = 2, kPsDisk = 5 };
+enum DevPanelShape : int { kPsRect = 0, kPsTri = 1, kPsSph = 2, kPsCyl = 3, kPsHemi = 4, kPsDisk = 5 };
+constexpr int kPanelPoints = 6;
 enum DevSrfAction : int {
   kSaReflect = 0, kSaTrans = 1, kSaAbsorb = 2, kSaJump = 3, kSaMult = 5, kSaNo = 6, kSaAdsorb = 8, kSaRevdes = 9,
   kSaIrrevdes = 10, kSaFlip = 11
 };
 
 struct DevPanel {
-  int ps;           // kPsRect, kPsSph or kPsDisk
+  int ps;           // DevPanelShape
   int srf;          // surface index
   int jumpp[2];     // jump destination panel per face (-1 none)
   int jumpf[2];     // destination face
-  double point[3][3];  // rect: corners 0..2; sph/disk: center, then radius in point[1][0]
-  double front[3];     // rect: (+-1, normal axis, edge axis); sph: (+-1); disk: unit normal
+  // rect: corners 0..2; tri: vertices 0..2, outward unit edge normals 3..5;
+  // sph/hemi/disk: center, then radius in point[1][0]; hemi: unit vector out
+  // of the opening in point[2]; cyl: axis ends 0, 1, radius in point[2][0]
+  double point[kPanelPoints][3];
+  // rect: (+-1, normal axis, edge axis); tri/disk: unit normal; sph/hemi:
+  // (+-1); cyl: front[2] = +-1
+  double front[3];
 };
 
 // Surface actions of solution-phase molecules, per surface, species and face

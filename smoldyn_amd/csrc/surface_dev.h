@@ -1,7 +1,8 @@
 // Device restatement of the reference's molecule-surface path for
 // solution-phase molecules in 3D (source/Smoldyn/smolsurface.c,
 // smolboxes.c:846-892 line2nextbox, source/libSteve/Geometry.c).  Panels:
-// rectangle, sphere, disk.  Actions: reflect, transmit, jump, and rate-based
+// rectangle, triangle, sphere, cylinder, hemisphere, disk.  Actions: reflect,
+// transmit, jump, and rate-based
 // actions whose outcomes are those; absorption and adsorption are reported to
 // the caller (the kernels raise kErrSurface for them).
 //
@@ -17,15 +18,86 @@ constexpr double kVeryClose = 1.0e-12;                // smoldyn.h:60
 constexpr double kDblEpsilon = 2.220446049250313e-16;  // DBL_EPSILON
 enum { kFaceFront = 0, kFaceBack = 1, kFaceNone = 2 };
 
+// Geo_LineNormal3D (Geometry.c:103-160): unit vector from the line a-b to
+// point, perpendicular to the line; returns the distance.  Degenerate lines
+// give the unit vector to point (unit x when point == a); a point on the line
+// gives (line_y, -line_x, 0) normalized, or unit x for a line along z.
+__device__ __forceinline__ double srf_linenormal3(const double* a, const double* b, const double* point, double* ans) {
+  const double eps = 100 * kDblEpsilon;
+  double line[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  double dot = line[0] * line[0] + line[1] * line[1] + line[2] * line[2];
+  if (dot < eps) {
+    for (int d = 0; d < 3; d++) ans[d] = point[d] - a[d];
+    dot = ans[0] * ans[0] + ans[1] * ans[1] + ans[2] * ans[2];
+    if (dot < eps) {
+      ans[0] = 1;
+      ans[1] = ans[2] = 0;
+      return 0;
+    }
+    dot = sqrt(dot);
+    for (int d = 0; d < 3; d++) ans[d] /= dot;
+    return dot;
+  }
+  dot = sqrt(dot);
+  for (int d = 0; d < 3; d++) line[d] /= dot;
+  for (int d = 0; d < 3; d++) ans[d] = point[d] - a[d];
+  for (int pass = 0; pass < 2; pass++) {
+    dot = ans[0] * line[0] + ans[1] * line[1] + ans[2] * line[2];
+    for (int d = 0; d < 3; d++) ans[d] -= dot * line[d];
+  }
+  dot = ans[0] * ans[0] + ans[1] * ans[1] + ans[2] * ans[2];
+  if (dot < eps) {
+    if (line[0] == 0 && line[1] == 0) {
+      ans[0] = 1;
+      ans[1] = ans[2] = 0;
+      return 0;
+    }
+    ans[0] = line[1];
+    ans[1] = -line[0];
+    ans[2] = 0;
+    dot = sqrt(ans[0] * ans[0] + ans[1] * ans[1] + ans[2] * ans[2]);
+    for (int d = 0; d < 3; d++) ans[d] /= dot;
+    return 0;
+  }
+  dot = sqrt(dot);
+  for (int d = 0; d < 3; d++) ans[d] /= dot;
+  return dot;
+}
+
+// Geo_PtInSlab (Geometry.c:495-505): test between the planes through a and
+// b perpendicular to a-b
+__device__ __forceinline__ bool srf_ptinslab(const double* a, const double* b, const double* t) {
+  double dot = 0;
+  for (int d = 0; d < 3; d++) dot += (t[d] - a[d]) * (b[d] - a[d]);
+  if (dot < 0) return false;
+  dot = 0;
+  for (int d = 0; d < 3; d++) dot += (t[d] - b[d]) * (a[d] - b[d]);
+  return !(dot < 0);
+}
+
+// Geo_PtInTriangle2 (Geometry.c:474-492): on no edge's outer side
+__device__ __forceinline__ bool srf_ptintriangle(const DevPanel& pn, const double* t) {
+  for (int k = 0; k < 3; k++) {
+    const double* p = pn.point[k];
+    const double* e = pn.point[k + 3];
+    if ((t[0] - p[0]) * e[0] + (t[1] - p[1]) * e[1] + (t[2] - p[2]) * e[2] > 0) return false;
+  }
+  return true;
+}
+
 // panelside (smolsurface.c:3930-3963)
 __device__ __forceinline__ int srf_panelside(const DevPanel& pn, const double* pt, double* distp, int strict) {
   double dist = 0;
   if (pn.ps == kPsRect) {
     int a = (int)pn.front[1];
     dist = pn.front[0] * (pt[a] - pn.point[0][a]);
-  } else if (pn.ps == kPsDisk) {
+  } else if (pn.ps == kPsDisk || pn.ps == kPsTri) {
 #pragma unroll
     for (int d = 0; d < 3; d++) dist += (pt[d] - pn.point[0][d]) * pn.front[d];
+  } else if (pn.ps == kPsCyl) {
+    double cylnorm[3];
+    dist = srf_linenormal3(pn.point[0], pn.point[1], pt, cylnorm);
+    dist = pn.front[2] * (dist - pn.point[2][0]);
   } else {
 #pragma unroll
     for (int d = 0; d < 3; d++) dist += (pt[d] - pn.point[0][d]) * (pt[d] - pn.point[0][d]);
@@ -78,7 +150,7 @@ __device__ __forceinline__ double srf_linexsphs(const double* pt1, const double*
   return (-b - radical) / (2 * a);
 }
 
-// lineXpanel (smolsurface.c:3999-4101), 3D rectangle / sphere / disk
+// lineXpanel (smolsurface.c:3999-4101), 3D panels
 __device__ __forceinline__ int srf_linexpanel(const DevPanel& pn, const double* pt1, const double* pt2,
                                               double* crsspt, int* face1p, int* face2p, double* crossp,
                                               double* cross2p) {
@@ -98,7 +170,7 @@ __device__ __forceinline__ int srf_linexpanel(const DevPanel& pn, const double* 
     if (d == (int)pn.front[1]) d = (d + 1) % 3;
     intsct = intsct && ((pn.point[1][d] <= crsspt[d] && crsspt[d] <= pn.point[2][d]) ||
                         (pn.point[2][d] <= crsspt[d] && crsspt[d] <= pn.point[1][d]));
-  } else if (pn.ps == kPsSph) {
+  } else if (pn.ps == kPsSph || pn.ps == kPsHemi) {
     int facein = pn.front[0] > 0 ? kFaceBack : kFaceFront;
     if (face1 == facein && face2 == facein) return 0;
     double nrdist, nrpos;
@@ -108,6 +180,47 @@ __device__ __forceinline__ int srf_linexpanel(const DevPanel& pn, const double* 
 #pragma unroll
     for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
     intsct = 1;
+    if (pn.ps == kPsHemi) {  // the crossing must be on the dome side of the opening
+      double dot = 0;
+      for (int d = 0; d < 3; d++) dot += (crsspt[d] - pn.point[0][d]) * pn.point[2][d];
+      intsct = (dot <= 0);
+      if (!intsct && face1 == face2) {
+        cross = cross2;
+        face1 = (face1 == kFaceFront) ? kFaceBack : kFaceFront;
+        for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
+        dot = 0;
+        for (int d = 0; d < 3; d++) dot += (crsspt[d] - pn.point[0][d]) * pn.point[2][d];
+        intsct = (dot <= 0);
+      }
+    }
+  } else if (pn.ps == kPsCyl) {
+    // Geo_LineXCyls (Geometry.c:1248-1258): the segment's perpendicular
+    // offsets from the axis against a circle about the origin
+    int facein = (int)pn.front[2] > 0 ? kFaceBack : kFaceFront;
+    if (face1 == facein && face1 == face2) return 0;
+    double n1[3], n2[3], zero[3] = {0, 0, 0}, nrdist, nrpos;
+    const double r1 = srf_linenormal3(pn.point[0], pn.point[1], pt1, n1);
+    const double r2 = srf_linenormal3(pn.point[0], pn.point[1], pt2, n2);
+    for (int d = 0; d < 3; d++) {
+      n1[d] *= r1;
+      n2[d] *= r2;
+    }
+    cross = srf_linexsphs(n1, n2, zero, pn.point[2][0], &cross2, &nrdist, &nrpos);
+    if (face1 == face2 && (!(nrdist <= pn.point[2][0]) || !(nrpos >= 0) || !(nrpos <= 1))) return 0;
+    if (face1 == facein) cross = cross2;
+    for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
+    intsct = srf_ptinslab(pn.point[0], pn.point[1], crsspt);
+    if (!intsct && face1 == face2) {
+      cross = cross2;
+      face1 = (face1 == kFaceFront) ? kFaceBack : kFaceFront;
+      for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
+      intsct = srf_ptinslab(pn.point[0], pn.point[1], crsspt);
+    }
+  } else if (pn.ps == kPsTri) {
+    if (face1 == face2) return 0;
+    cross = dist1 / (dist1 - dist2);
+    for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
+    intsct = srf_ptintriangle(pn, crsspt);
   } else {
     if (face1 == face2) return 0;
     cross = dist1 / (dist1 - dist2);
@@ -130,10 +243,14 @@ __device__ __forceinline__ void srf_getpanelnormal(const DevPanel& pn, const dou
   if (pn.ps == kPsRect) {
     norm[0] = norm[1] = norm[2] = 0;
     norm[(int)pn.front[1]] = pn.front[0];
-  } else if (pn.ps == kPsDisk) {
+  } else if (pn.ps == kPsDisk || pn.ps == kPsTri) {
     norm[0] = pn.front[0];
     norm[1] = pn.front[1];
     norm[2] = pn.front[2];
+  } else if (pn.ps == kPsCyl) {
+    srf_linenormal3(pn.point[0], pn.point[1], pt, norm);
+    if (pn.front[2] == -1)
+      for (int d = 0; d < 3; d++) norm[d] *= -1;
   } else {
     srf_spherenormal(pn.point[0], pt, (int)pn.front[0], norm);
   }
@@ -162,7 +279,7 @@ __device__ __forceinline__ void srf_reflect(const DevPanel& pn, double* pos, con
   if (pn.ps == kPsRect) {
     int axis = (int)pn.front[1];
     pos[axis] -= 2.0 * (pos[axis] - crsspt[axis]);
-  } else if (pn.ps == kPsDisk) {
+  } else if (pn.ps == kPsDisk || pn.ps == kPsTri) {
     double dot = 0;
 #pragma unroll
     for (int d = 0; d < 3; d++) dot += (pos[d] - crsspt[d]) * pn.front[d];
@@ -170,7 +287,8 @@ __device__ __forceinline__ void srf_reflect(const DevPanel& pn, double* pos, con
     for (int d = 0; d < 3; d++) pos[d] -= 2.0 * pn.front[d] * dot;
   } else {
     double norm[3], dot = 0;
-    srf_spherenormal(pn.point[0], crsspt, 1, norm);
+    if (pn.ps == kPsCyl) srf_linenormal3(pn.point[0], pn.point[1], crsspt, norm);
+    else srf_spherenormal(pn.point[0], crsspt, 1, norm);
 #pragma unroll
     for (int d = 0; d < 3; d++) dot += (pos[d] - crsspt[d]) * norm[d];
 #pragma unroll
@@ -251,7 +369,7 @@ __device__ __forceinline__ int srf_action(const DevSurf& sf, int srf, int ident,
   return a.outcome[ms2];
 }
 
-// ptinpanel (smolsurface.c:4276-4323) for 3D rectangles, spheres and disks
+// ptinpanel (smolsurface.c:4276-4323) for 3D panels
 __device__ __forceinline__ bool srf_ptinpanel(const DevPanel& pn, const double* pt) {
   if (pn.ps == kPsRect) {
     int d = (int)pn.front[2];
@@ -262,6 +380,13 @@ __device__ __forceinline__ bool srf_ptinpanel(const DevPanel& pn, const double* 
                   (pn.point[2][d] <= pt[d] && pt[d] <= pn.point[1][d]));
   }
   if (pn.ps == kPsSph) return true;
+  if (pn.ps == kPsTri) return srf_ptintriangle(pn, pt);
+  if (pn.ps == kPsCyl) return srf_ptinslab(pn.point[0], pn.point[1], pt);
+  if (pn.ps == kPsHemi) {
+    double dot = 0;
+    for (int d = 0; d < 3; d++) dot += (pt[d] - pn.point[0][d]) * pn.point[2][d];
+    return dot <= 0;
+  }
   double len2 = 0, dot = 0;
 #pragma unroll
   for (int d = 0; d < 3; d++) len2 += (pt[d] - pn.point[0][d]) * (pt[d] - pn.point[0][d]);
@@ -279,10 +404,14 @@ __device__ __forceinline__ void srf_panelnormal(const DevPanel& pn, const double
     norm[0] = norm[1] = norm[2] = 0;
     norm[(int)pn.front[1]] =
         ((face == kFaceFront && pn.front[0] == 1) || (face == kFaceBack && pn.front[0] == -1)) ? 1.0 : -1.0;
-  } else if (pn.ps == kPsDisk) {
+  } else if (pn.ps == kPsDisk || pn.ps == kPsTri) {
     const double sgn = face == kFaceFront ? 1.0 : -1.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) norm[d] = sgn * pn.front[d];
+  } else if (pn.ps == kPsCyl) {
+    srf_linenormal3(pn.point[0], pn.point[1], pos, norm);
+    if ((face == kFaceBack && pn.front[2] == 1) || (face == kFaceFront && pn.front[2] == -1))
+      for (int d = 0; d < 3; d++) norm[d] *= -1;
   } else {
     srf_spherenormal(pn.point[0], pos,
                      ((face == kFaceFront && pn.front[0] == 1) || (face == kFaceBack && pn.front[0] == -1)) ? 1 : -1,
@@ -301,7 +430,7 @@ __device__ __forceinline__ double srf_desorbdist(double step, double y, bool rev
   return step * x;
 }
 
-// surfacejump (smolsurface.c:4873-4926) for rect, sphere and disk panels;
+// surfacejump (smolsurface.c:4873-4926) for 3D panels;
 // returns 0 when the panel has no jump destination for that face
 __device__ __forceinline__ int srf_jump(const DevSurf& sf, const DevPanel& pn, double* pos, double* crsspt,
                                         int face) {
@@ -314,11 +443,23 @@ __device__ __forceinline__ int srf_jump(const DevSurf& sf, const DevPanel& pn, d
 #pragma unroll
     for (int d = 0; d < 3; d++) delta[d] = 0.5 * (q.point[0][d] + q.point[2][d]) - 0.5 * (pn.point[0][d] + pn.point[2][d]);
     dir = (pn.front[0] == q.front[0]) ? 1 : -1;
-  } else if (pn.ps == kPsSph) {
+  } else if (pn.ps == kPsSph || pn.ps == kPsHemi) {
 #pragma unroll
     for (int d = 0; d < 3; d++)
       delta[d] = (crsspt[d] - pn.point[0][d]) * q.point[1][0] / pn.point[1][0] + q.point[0][d] - crsspt[d];
     dir = (pn.front[0] == q.front[0]) ? 1 : -1;
+  } else if (pn.ps == kPsCyl) {
+    for (int d = 0; d < 3; d++)
+      delta[d] = (crsspt[d] - pn.point[0][d]) * q.point[2][0] / pn.point[2][0] + q.point[0][d] - crsspt[d];
+    dir = (pn.front[2] == q.front[2]) ? 1 : -1;
+  } else if (pn.ps == kPsTri) {
+    // Geo_TriCenter (Geometry.c:43-53): (1.0/3.0) times the vertex sum
+    double dot = 0;
+    for (int d = 0; d < 3; d++)
+      delta[d] = (1.0 / 3.0) * (q.point[0][d] + q.point[1][d] + q.point[2][d]) -
+                 (1.0 / 3.0) * (pn.point[0][d] + pn.point[1][d] + pn.point[2][d]);
+    for (int d = 0; d < 3; d++) dot += pn.front[d] * q.front[d];
+    dir = dot > 0 ? 1 : -1;
   } else {
     double dot = 0;
 #pragma unroll

@@ -6,6 +6,7 @@
 // order (surfaces, then shapes, then panels).
 #include "surfaces_host.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -97,6 +98,196 @@ bool sphere_x_box(const double* c, double rad, const double* b1, const double* b
     else if (d2 > mx) mx = d2;
   }
   return !(r2 > mx);
+}
+
+// Geo_LineNormal3D (Geometry.c:103-160): unit vector from the line a-b to
+// point, perpendicular to the line; returns the distance.  Degenerate lines
+// give the unit vector to point (unit x when point == a); a point on the line
+// gives (line_y, -line_x, 0) normalized, or unit x for a line along z.
+double line_normal3(const double* a, const double* b, const double* point, double* ans) {
+  const double eps = 100 * DBL_EPSILON;
+  double line[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  double dot = line[0] * line[0] + line[1] * line[1] + line[2] * line[2];
+  if (dot < eps) {
+    for (int d = 0; d < 3; d++) ans[d] = point[d] - a[d];
+    dot = ans[0] * ans[0] + ans[1] * ans[1] + ans[2] * ans[2];
+    if (dot < eps) {
+      ans[0] = 1;
+      ans[1] = ans[2] = 0;
+      return 0;
+    }
+    dot = sqrt(dot);
+    for (int d = 0; d < 3; d++) ans[d] /= dot;
+    return dot;
+  }
+  dot = sqrt(dot);
+  for (int d = 0; d < 3; d++) line[d] /= dot;
+  for (int d = 0; d < 3; d++) ans[d] = point[d] - a[d];
+  for (int pass = 0; pass < 2; pass++) {
+    dot = ans[0] * line[0] + ans[1] * line[1] + ans[2] * line[2];
+    for (int d = 0; d < 3; d++) ans[d] -= dot * line[d];
+  }
+  dot = ans[0] * ans[0] + ans[1] * ans[1] + ans[2] * ans[2];
+  if (dot < eps) {
+    if (line[0] == 0 && line[1] == 0) {
+      ans[0] = 1;
+      ans[1] = ans[2] = 0;
+      return 0;
+    }
+    ans[0] = line[1];
+    ans[1] = -line[0];
+    ans[2] = 0;
+    dot = sqrt(ans[0] * ans[0] + ans[1] * ans[1] + ans[2] * ans[2]);
+    for (int d = 0; d < 3; d++) ans[d] /= dot;
+    return 0;
+  }
+  dot = sqrt(dot);
+  for (int d = 0; d < 3; d++) ans[d] /= dot;
+  return dot;
+}
+
+// Geo_TriXaabb3 (Geometry.c:1585-1605): bounding-box overlap, then the box
+// corners not all strictly on one side of the triangle's plane
+bool tri_x_box(const double* p1, const double* p2, const double* p3, const double* n, const double* b1,
+               const double* b2) {
+  for (int d = 0; d < 3; d++) {
+    if (p1[d] < b1[d] && p2[d] < b1[d] && p3[d] < b1[d]) return false;
+    if (p1[d] > b2[d] && p2[d] > b2[d] && p3[d] > b2[d]) return false;
+  }
+  double cmp = n[0] * p1[0] + n[1] * p1[1] + n[2] * p1[2], o[8];
+  corners(n, b1, b2, o);
+  return !all_below(o, cmp) && !all_above(o, cmp);
+}
+
+// Geo_LineXSphs (Geometry.c:1200-1221) in 2D without the nearest-point outputs
+double line_x_circle(const double* p1, const double* p2, const double* c, double rad, double* cross2) {
+  double a = 0, b = 0, cc = 0;
+  for (int d = 0; d < 2; d++) {
+    a += (p2[d] - p1[d]) * (p2[d] - p1[d]);
+    b += (c[d] - p1[d]) * (p2[d] - p1[d]);
+    cc += (p1[d] - c[d]) * (p1[d] - c[d]);
+  }
+  const double nrdist = sqrt(cc - b * b / a);
+  b *= -2;
+  cc -= rad * rad;
+  double radical = b * b - 4 * a * cc;
+  if (nrdist <= rad && radical < 0) radical = 0;
+  radical = sqrt(radical);
+  *cross2 = (-b + radical) / (2 * a);
+  return (-b - radical) / (2 * a);
+}
+
+// Geo_LineXaabb (Geometry.c:1561-1582) for the infinite line through p1, p2
+bool line_x_box(const double* p1, const double* p2, const double* b1, const double* b2) {
+  double nearhi = -1.0, farlo = 2.0;
+  for (int d = 0; d < 3; d++) {
+    const double denom = p2[d] - p1[d];
+    if (denom == 0) {
+      if (p1[d] < b1[d] || p1[d] > b2[d]) return false;
+    } else {
+      const double a1 = (b1[d] - p1[d]) / denom, a2 = (b2[d] - p1[d]) / denom;
+      if (a1 < a2) {
+        if (a1 > nearhi) nearhi = a1;
+        if (a2 < farlo) farlo = a2;
+      } else {
+        if (a2 > nearhi) nearhi = a2;
+        if (a1 < farlo) farlo = a1;
+      }
+    }
+  }
+  return nearhi <= farlo;
+}
+
+// Geo_CylisXaabb3 (Geometry.c:1777-1916): infinite cylinder of axis p1-p2
+// against a box, in the plane perpendicular to the axis (axes v1, v2):
+// separating axes along the projected box edges, corner distances, the axis
+// line through the box, and the projected box edges against the circle
+bool cyl_inf_x_box(const double* p1, const double* p2, double rad, const double* b1, const double* b2) {
+  static const int edge[12][2] = {{0, 1}, {1, 3}, {3, 2}, {2, 0}, {0, 4}, {1, 5},
+                                  {2, 6}, {3, 7}, {4, 5}, {5, 7}, {7, 6}, {6, 4}};
+  static const int axis[3] = {1, 2, 4};
+  double v1[3], v2[3], v3[3], cent[2], ab[8][2], o1[8], o2[8], b[8];
+  line_normal3(p1, p2, p1, v1);
+  for (int d = 0; d < 3; d++) v3[d] = p2[d] - p1[d];
+  double nrm = sqrt(v3[0] * v3[0] + v3[1] * v3[1] + v3[2] * v3[2]);
+  if (!nrm) return false;
+  for (int d = 0; d < 3; d++) v3[d] /= nrm;
+  v2[0] = v3[1] * v1[2] - v3[2] * v1[1];
+  v2[1] = v3[2] * v1[0] - v3[0] * v1[2];
+  v2[2] = v3[0] * v1[1] - v3[1] * v1[0];
+  cent[0] = v1[0] * p1[0] + v1[1] * p1[1] + v1[2] * p1[2];
+  cent[1] = v2[0] * p1[0] + v2[1] * p1[1] + v2[2] * p1[2];
+  corners(v1, b1, b2, o1);
+  corners(v2, b1, b2, o2);
+  for (int k = 0; k < 8; k++) {
+    ab[k][0] = o1[k];
+    ab[k][1] = o2[k];
+  }
+  for (int a = 0; a < 3; a++) {
+    double u[2] = {ab[axis[a]][0] - ab[0][0], ab[axis[a]][1] - ab[0][1]};
+    nrm = sqrt(u[0] * u[0] + u[1] * u[1]);
+    if (nrm > 0) {
+      u[0] /= nrm;
+      u[1] /= nrm;
+      for (int k = 0; k < 8; k++) b[k] = u[0] * ab[k][0] + u[1] * ab[k][1];
+      if (all_below(b, u[0] * cent[0] + u[1] * cent[1] - rad)) return false;
+      if (all_above(b, u[0] * cent[0] + u[1] * cent[1] + rad)) return false;
+    }
+  }
+  double mn = 0, mx = 0;
+  for (int k = 0; k < 8; k++) {
+    const double d2 = (ab[k][0] - cent[0]) * (ab[k][0] - cent[0]) + (ab[k][1] - cent[1]) * (ab[k][1] - cent[1]);
+    if (k == 0) mn = mx = d2;
+    else if (d2 < mn) mn = d2;
+    else if (d2 > mx) mx = d2;
+  }
+  const double r2 = rad * rad;
+  if (r2 > mx) return false;
+  if (r2 >= mn) return true;
+  if (line_x_box(p1, p2, b1, b2)) return true;
+  for (int k = 0; k < 12; k++) {
+    double c2;
+    const double c1 = line_x_circle(ab[edge[k][0]], ab[edge[k][1]], cent, rad, &c2);
+    if ((c1 >= 0 && c1 <= 1) || (c2 >= 0 && c2 <= 1)) return true;
+  }
+  return false;
+}
+
+// Geo_CylsXaabb3 (Geometry.c:1964-1970) with Geo_Cyl2Rect (:1137-1162)
+bool cyl_x_box(const double* p1, const double* p2, double rad, const double* b1, const double* b2) {
+  if (!cyl_inf_x_box(p1, p2, rad, b1, b2)) return false;
+  double v1[3], v2[3], r[4][3];
+  line_normal3(p1, p2, p1, v1);
+  v2[0] = v1[1] * (p2[2] - p1[2]) - v1[2] * (p2[1] - p1[1]);
+  v2[1] = v1[2] * (p2[0] - p1[0]) - v1[0] * (p2[2] - p1[2]);
+  v2[2] = v1[0] * (p2[1] - p1[1]) - v1[1] * (p2[0] - p1[0]);
+  const double nrm = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+  for (int d = 0; d < 3; d++) v2[d] /= nrm;
+  for (int d = 0; d < 3; d++) {
+    r[0][d] = p1[d] + rad * (-v1[d] - v2[d]);
+    r[1][d] = p1[d] + rad * (+v1[d] - v2[d]);
+    r[2][d] = p1[d] + rad * (-v1[d] + v2[d]);
+    r[3][d] = p2[d] + rad * (-v1[d] - v2[d]);
+  }
+  return rect_x_box(r[0], r[1], r[2], r[3], b1, b2);
+}
+
+// Geo_HemisXaabb3 (Geometry.c:1956-1962) with Geo_Hemis2Rect (:1113-1135)
+bool hemi_x_box(const double* c, double rad, const double* out, const double* b1, const double* b2) {
+  if (!sphere_x_box(c, rad, b1, b2)) return false;
+  const double zero[3] = {0, 0, 0};
+  double v1[3], v2[3], r[4][3];
+  line_normal3(zero, out, zero, v1);
+  v2[0] = v1[1] * out[2] - v1[2] * out[1];
+  v2[1] = v1[2] * out[0] - v1[0] * out[2];
+  v2[2] = v1[0] * out[1] - v1[1] * out[0];
+  for (int d = 0; d < 3; d++) {
+    r[0][d] = c[d] + rad * (-v1[d] - v2[d]);
+    r[1][d] = c[d] + rad * (+v1[d] - v2[d]);
+    r[2][d] = c[d] + rad * (-v1[d] + v2[d]);
+    r[3][d] = c[d] + rad * (-v1[d] - v2[d] - out[d]);
+  }
+  return rect_x_box(r[0], r[1], r[2], r[3], b1, b2);
 }
 
 bool disk_x_box(const double* c, double rad, const double* n, const double* b1, const double* b2) {
@@ -273,8 +464,8 @@ int SurfaceSet::set_rate(int s, int ident, int nspecies, int ms, int ms1, int ms
   return 0;
 }
 
-// surfaddpanel for 3D rectangles, spheres and disks; a panel of an existing
-// name replaces it
+// surfaddpanel (smolsurface.c:2432-2640) for 3D panels of every shape; a
+// panel of an existing name replaces it
 int SurfaceSet::add_panel(int s, int ps, const char* name, const char* axis, const double* params) {
   HostPanel p;
   p.ps = ps;
@@ -313,6 +504,71 @@ int SurfaceSet::add_panel(int s, int ps, const char* name, const char* axis, con
     point[1][1] = params[4];
     point[1][2] = params[5];
     if (point[1][1] <= 0 || point[1][2] <= 0) return -4;
+    front[0] = params[3] > 0 ? 1 : (params[3] < 0 ? -1 : 0);
+  } else if (ps == 1) {
+    // vertices, unit normal (Geo_TriNormal), outward unit edge normals
+    // (Geo_UnitCross of each edge with the normal), smolsurface.c:2521-2534
+    for (int k = 0; k < 3; k++)
+      for (int d = 0; d < 3; d++) point[k][d] = params[3 * k + d];
+    double e1[3], e2[3];
+    for (int d = 0; d < 3; d++) {
+      e1[d] = point[1][d] - point[0][d];
+      e2[d] = point[2][d] - point[1][d];
+    }
+    front[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    front[1] = -e1[0] * e2[2] + e1[2] * e2[0];
+    front[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    double area = sqrt(front[0] * front[0] + front[1] * front[1] + front[2] * front[2]);
+    if (area > 100 * DBL_EPSILON) {
+      for (int d = 0; d < 3; d++) front[d] /= area;
+    } else {  // Geo_LineNormal of the first edge, in the xy plane
+      const double len = sqrt(e1[0] * e1[0] + e1[1] * e1[1]);
+      if (len > 0) {
+        const double leninv = 1.0 / len;
+        front[0] = leninv * e1[1];
+        front[1] = -leninv * e1[0];
+      } else {
+        front[0] = 1;
+        front[1] = 0;
+      }
+      front[2] = 0;
+    }
+    for (int k = 0; k < 3; k++) {
+      const double* a = point[k];
+      const double* b = point[(k + 1) % 3];
+      const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+      const double c[3] = {u[1] * front[2] - u[2] * front[1], u[2] * front[0] - u[0] * front[2],
+                           u[0] * front[1] - u[1] * front[0]};
+      const double len = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+      for (int d = 0; d < 3; d++) point[3 + k][d] = len > 100 * DBL_EPSILON ? c[d] / len : 0;
+    }
+  } else if (ps == 3) {
+    for (int k = 0; k < 2; k++)
+      for (int d = 0; d < 3; d++) point[k][d] = params[3 * k + d];
+    if (point[0][0] == point[1][0] && point[0][1] == point[1][1] && point[0][2] == point[1][2]) return -5;
+    point[2][0] = fabs(params[6]);
+    point[2][1] = params[7];
+    point[2][2] = params[8];
+    if (point[2][1] <= 0 || point[2][2] <= 0) return -4;
+    front[2] = params[6] > 0 ? 1 : (params[6] < 0 ? -1 : 0);
+    double len = 0;
+    for (int d = 0; d < 3; d++) len += (point[1][d] - point[0][d]) * (point[1][d] - point[0][d]);
+    const double lengthinv = 1.0 / sqrt(len);
+    for (int d = 0; d < 3; d++) {
+      point[3][d] = -(point[1][d] - point[0][d]) * lengthinv;
+      point[4][d] = (point[1][d] - point[0][d]) * lengthinv;
+    }
+  } else if (ps == 4) {
+    for (int d = 0; d < 3; d++) point[0][d] = params[d];
+    point[1][0] = fabs(params[3]);
+    point[1][1] = params[7];
+    point[1][2] = params[8];
+    if (point[1][1] <= 0 || point[1][2] <= 0) return -4;
+    double len = 0;  // normalizeVD (Rn.c:500-508)
+    for (int d = 0; d < 3; d++) len += params[4 + d] * params[4 + d];
+    if (!len) return -6;
+    len = sqrt(len);
+    for (int d = 0; d < 3; d++) point[2][d] = params[4 + d] / len;
     front[0] = params[3] > 0 ? 1 : (params[3] < 0 ? -1 : 0);
   } else if (ps == 5) {
     point[0][0] = params[0];
@@ -408,7 +664,7 @@ int SurfaceSet::build(const Geometry& g, int nspecies, const double (*difc_ms)[k
       d.jumpp[k] = p.jumpp[k];
       d.jumpf[k] = p.jumpf[k];
     }
-    for (int a = 0; a < 3; a++)
+    for (int a = 0; a < kPanelPoints; a++)
       for (int b = 0; b < 3; b++) d.point[a][b] = p.point[a][b];
     for (int b = 0; b < 3; b++) d.front[b] = p.front[b];
     out.pnl.push_back(d);
@@ -635,7 +891,10 @@ int SurfaceSet::build(const Geometry& g, int nspecies, const double (*difc_ms)[k
           const HostPanel& p = pnl[id];
           bool in;
           if (ps == 0) in = rect_x_box(p.point[0], p.point[1], p.point[3], p.point[0], v1, v2);
+          else if (ps == 1) in = tri_x_box(p.point[0], p.point[1], p.point[2], p.front, v1, v2);
           else if (ps == 2) in = sphere_x_box(p.point[0], p.point[1][0], v1, v2);
+          else if (ps == 3) in = cyl_x_box(p.point[0], p.point[1], p.point[2][0], v1, v2);
+          else if (ps == 4) in = hemi_x_box(p.point[0], p.point[1][0], p.point[2], v1, v2);
           else in = disk_x_box(p.point[0], p.point[1][0], p.front, v1, v2);
           if (in) {
             out.bplist.push_back(id);

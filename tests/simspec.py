@@ -162,6 +162,48 @@ def c3_spec(n=2000, side=100.0, seed=31, kback=5.0, ndisk=8, rdisk=None, bindrad
                 molecules=[("A", n), ("B", n)])
 
 
+def c3_shapes_spec(n=2000, side=60.0, seed=35, kback=20.0, p_cross=0.5):
+    """Every 3D panel shape at once: reflecting walls built from 6 rectangles;
+    a tetrahedron of 4 triangles that reflects B and C while A crosses with
+    probability p_cross and becomes D; a cylinder with hemisphere caps that
+    reflects everything but A, which crosses with probability p_cross; a
+    sphere inside the cylinder; a pair of triangles whose front faces jump
+    to each other.  Reversible A+B<->C with unbinding placement, so bound
+    pairs are tested for panels between them."""
+    s = side
+    v = [(8.0, 8.0, 8.0), (32.0, 8.0, 8.0), (20.0, 32.0, 8.0), (20.0, 16.0, 32.0)]
+    faces = [(0, 1, 2), (0, 3, 1), (1, 3, 2), (2, 3, 0)]
+    tet = [("tri", None, [c for k in f for c in v[k]], "t%d" % j) for j, f in enumerate(faces)]
+    cx, cy, z0, z1, rc = 40.0, 40.0, 15.0, 45.0, 6.0
+    tube = [("cyl", None, [cx, cy, z0, cx, cy, z1, rc, 20, 20], "c0"),
+            ("hemi", None, [cx, cy, z0, rc, 0, 0, 1, 20, 20], "h0"),
+            ("hemi", None, [cx, cy, z1, rc, 0, 0, -1, 20, 20], "h1")]
+    walls = [("rect", "+0", [0, 0, 0, s, s], "w0"), ("rect", "-0", [s, 0, 0, s, s], "w1"),
+             ("rect", "+1", [0, 0, 0, s, s], "w2"), ("rect", "-1", [0, s, 0, s, s], "w3"),
+             ("rect", "+2", [0, 0, 0, s, s], "w4"), ("rect", "-2", [0, 0, s, s, s], "w5")]
+    jumps = [("tri", None, [10, 40, 40, 10, 50, 40, 10, 45, 50], "j0"),
+             ("tri", None, [50, 10, 10, 50, 20, 10, 50, 15, 20], "j1")]
+    return dict(low=[0.0] * 3, high=[s] * 3, walls="rrr", dt=0.002, seed=seed,
+                species={"A": 2.0, "B": 2.0, "C": 1.0, "D": 2.0},
+                reactions=[("fwd", ["A", "B"], ["C"], 20.0), ("back", ["C"], ["A", "B"], kback)],
+                placement={"back": ("ratio", 1.5)},
+                surfaces=[
+                    {"name": "walls", "actions": [("both", "all", "reflect")], "panels": walls},
+                    {"name": "tet", "actions": [("both", "B", "reflect"), ("both", "C", "reflect")],
+                     "rates": [("A", "soln", "fsoln", "bsoln", p_cross, True, "D"),
+                               ("A", "soln", "bsoln", "fsoln", p_cross, True, "D")],
+                     "panels": tet},
+                    {"name": "tube", "actions": [("both", "B", "reflect"), ("both", "C", "reflect"),
+                                                 ("both", "D", "reflect")],
+                     "rates": [("A", "soln", "fsoln", "bsoln", p_cross, True),
+                               ("A", "soln", "bsoln", "fsoln", p_cross, True)],
+                     "panels": tube + [("sph", None, [cx, cy, 30.0, 3.0, 20, 20], "s0")]},
+                    {"name": "jumper", "actions": [("front", "all", "jump"), ("back", "all", "reflect")],
+                     "panels": jumps, "jumps": [("j0", "front", "j1", "front", True)]},
+                ],
+                molecules=[("A", n), ("B", n)])
+
+
 def c3_adsorb_spec(n=2000, side=100.0, seed=33, ndisk=8, p_ads=0.3, k_des=2.0, kback=5.0, rdisk=None):
     """Surfaces workload with surface-bound molecules: c3_spec's sphere and
     membrane, disk patches on the membrane where A crossing from either side

@@ -90,7 +90,7 @@ mol 5 A u u u
 
 @pytest.mark.parametrize("stmt,where", [("difm A 1 0 0 0 1 0 0 0 1", "model.txt:6"),
                                         ("reaction compartment=c r A -> B 1", "model.txt:6"),
-                                        ("start_surface s\n panel cyl 0 0 0 1 1 1 2 20 20\nend_surface",
+                                        ("start_surface s\n panel cyl 0 0 0 0 0 0 2 20 20\nend_surface",
                                          "model.txt:7")])
 def test_unsupported_statements_name_their_line(tmp_path, stmt, where):
     text = "dim 3\nboundaries 0 0 10\nboundaries 1 0 10\nboundaries 2 0 10\n%s\n" % stmt

@@ -199,6 +199,24 @@ def test_surfaces_jump_panels():
     assert nev > 10
 
 
+def test_surfaces_all_shapes_bit_exact():
+    """Triangles (a tetrahedron, a pair of jump triangles), a cylinder with
+    hemisphere caps, a sphere and rectangle walls; reflect, jump and
+    rate-based crossings with a species change; reversible A+B<->C with
+    pairs tested across these panels: identical to the oracle."""
+    spec = ss.c3_shapes_spec(n=3000, seed=35)
+    nev = _run_pair(spec, 300, check_every=25)
+    assert nev > 20
+
+
+def test_surfaces_all_shapes_injected_large_steps():
+    """Large injected steps across the same shapes: several crossings of
+    triangles, cylinders and hemispheres per step."""
+    spec = ss.c3_shapes_spec(n=1500, seed=36)
+    spec["dt"] = 0.5
+    _run_pair(spec, 40, check_every=4, inject=True)
+
+
 # ---------------------------------------------------------------------------
 # ensembles
 
