@@ -146,7 +146,8 @@ enum ErrorCode smolSetSurfaceAction(simptr sim, const char* surface, enum PanelF
 enum ErrorCode smolSetSurfaceRate(simptr sim, const char* surface, const char* species, enum MolecState state,
                                   enum MolecState state1, enum MolecState state2, double rate,
                                   const char* newspecies, int isinternal);    /* libsmoldyn.h:129 */
-/* Panel shapes on the device path: PSrect, PSsph, PSdisk (3D). */
+/* Panel shapes on the device path: every 3D shape (PSrect, PStri, PSsph, PScyl,
+   PShemi, PSdisk). */
 enum ErrorCode smolAddPanel(simptr sim, const char* surface, enum PanelShape panelshape, const char* panel,
                             const char* axisstring, double* params);          /* libsmoldyn.h:130 */
 int smolGetPanelIndex(simptr sim, const char* surface, enum PanelShape* panelshapeptr,
