@@ -185,6 +185,7 @@ struct DeviceState {
   Prod* prods = nullptr;
   // surfaces (single simulation)
   bool surf_on = false;
+  bool surf_curved = false;  // a triangle, cylinder or hemisphere panel
   DevPanel* d_pnl = nullptr;
   int* d_bpstart = nullptr;
   int* d_bplist = nullptr;
@@ -321,7 +322,7 @@ __device__ __forceinline__ int cur_nrec(const Inst* insts, const Glob* gl) {
   return gl->nrec[(insts[0].ctl->step + 1) & 1];
 }
 
-template <bool kSurf, bool kMulti, bool kF32>
+template <int kSurf, bool kMulti, bool kF32>
 __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
                                                     Recs a, int* __restrict__ bcount,
                                                     const double* __restrict__ gtab, int gmask,
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
         } else if (I.sf.listdiffuse[ll]) {
           // with surfaces the walls are not applied (smolsim.cpp:2618-2630)
           int i2 = i, bnd = 0;
-          int st = srf_check(I.sf, g, tab->difc, pos, posx, &i2, ser, step, SMOLRNG_SURF_DIFFUSED, I.key0, I.key1,
+          int st = srf_check<(kSurf > 1)>(I.sf, g, tab->difc, pos, posx, &i2, ser, step, SMOLRNG_SURF_DIFFUSED, I.key0, I.key1,
                              &bnd);
           if (st == kSrfAdsorbed && I.sf.blist[i2][bstate(bnd)] >= 0) {  // bound: to the list of its state
             sb = bnd;
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
         // checksurfacebound (smolsurface.c:5184-5199): desorption
         int i2 = i;
         obox = pos2box(g, pos[0], pos[1], pos[2]);
-        const int st = srf_bound(I.sf, g, tab->difc, tab->difstep, pos, &i2, sb, ser, step, I.key0, I.key1, &rbox);
+        const int st = srf_bound<(kSurf > 1)>(I.sf, g, tab->difc, tab->difstep, pos, &i2, sb, ser, step, I.key0, I.key1, &rbox);
         if (st == kSrfDesorbed) {
           atomicSub((sb & 3) == 1 ? &ctl->countf[i] : &ctl->countk[i], 1);
           if (i2 != i) {
@@ -616,7 +617,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan23(const int* __restrict__ p
 // The inverse map (list slot -> record) is written only for the last
 // `window` slots of each list: compaction in k_resolve reads tail slots only,
 // and the kills of one step never exceed the window.
-template <bool kMulti, bool kF32, bool kSurf>
+template <bool kMulti, bool kF32, int kSurf>
 __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
                                                     Recs a, Recs b, Binned bn, int* __restrict__ inv) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -710,7 +711,7 @@ __device__ __forceinline__ void emit_edge(Control* ctl, Edge* edges, int maxedge
   }
 }
 
-template <bool kSurf>
+template <int kSurf>
 __device__ __forceinline__ int first_reaction(const DevTables* tab, int i1, int i2, double dist2, uint32_t key0,
                                               uint32_t key1, uint64_t ser1, uint64_t ser2, uint32_t step,
                                               const DevSurf& sf, const Geometry& g, const double* p1,
@@ -726,7 +727,7 @@ __device__ __forceinline__ int first_reaction(const DevTables* tab, int i1, int 
       if (!(smolrng_u01(rr.v[0]) < rx.prob)) continue;
     }
     // rxnXsurface (smolreact.c:3242,3289; not for wrapped pairs)
-    if (kSurf && !wrap && srf_rxnblocked(sf, g, tab->difc, p1, p2, i1, i2, ser1, ser2, step, j, key0, key1))
+    if (kSurf && !wrap && srf_rxnblocked<(kSurf > 1)>(sf, g, tab->difc, p1, p2, i1, i2, ser1, ser2, step, j, key0, key1))
       continue;
     if (!rx.permit_soln) continue;
     return r;
@@ -737,7 +738,7 @@ __device__ __forceinline__ int first_reaction(const DevTables* tab, int i1, int 
 // Exact FP64 test of one candidate (the reference's dist2 in its operation
 // order, smolreact.c:3238-3240, and wallcalcdist2, smolwall.c:100-116) and
 // the edge for the first reaction that accepts it.
-template <bool kSurf, bool kF32>
+template <int kSurf, bool kF32>
 __device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf, const DevTables* tab, Control* ctl,
                                            const Recs& b,
                                            Edge* edges, int maxedges, uint32_t key0, uint32_t key1, uint32_t step,
@@ -774,7 +775,7 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf,
 // pass the FP32 screen are handed to k_exact, which runs the reference's FP64
 // test and keys the edge; the screen itself stays small so many waves hide
 // the latency of its loads.
-template <bool kMulti, bool kSurf>
+template <bool kMulti, int kSurf>
 __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, Glob* __restrict__ gl, Recs b,
                                                   Binned bn, int nscan, int4* __restrict__ cand, int candcap) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -899,7 +900,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
 // Exact test of the screened candidates (one thread per candidate, grid-
 // stride): stencil index and edge key of molecule 1's loop position, then the
 // reference's FP64 distance test and reaction choice (exact_pair).
-template <bool kSurf, bool kMulti, bool kF32>
+template <int kSurf, bool kMulti, bool kF32>
 __global__ __launch_bounds__(kBlock) void k_exact(const Inst* __restrict__ insts, const Glob* __restrict__ gl, Recs b,
                                                   Binned bn, const int4* __restrict__ cand, int candcap) {
   if (gl->halt) return;
@@ -1370,7 +1371,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_relist(const Inst* __restri
 }
 
 // One workgroup per simulation of the batch (blockIdx.x).
-template <bool kF32, bool kSurf>
+template <bool kF32, int kSurf>
 __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restrict__ insts, Glob* gl, Recs RA,
                                                              Recs RB, int* __restrict__ inv, Binned bn, int nscan,
                                                              uint64_t* __restrict__ bestkey, int* __restrict__ killed,
@@ -1742,7 +1743,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       double pp[3] = {pr.x, pr.y, pr.z};
       const double px0[3] = {pr.sx, pr.sy, pr.sz};
       int i2 = pr.ident, bnd = 0;
-      const int st = srf_check(I.sf, I.g, tab->difc, pp, px0, &i2, ser0 + (uint64_t)c, step, SMOLRNG_SURF_REBORN,
+      const int st = srf_check<(kSurf > 1)>(I.sf, I.g, tab->difc, pp, px0, &i2, ser0 + (uint64_t)c, step, SMOLRNG_SURF_REBORN,
                                I.key0, I.key1, &bnd);
       if (st == kSrfAdsorbed && I.sf.blist[i2][bstate(bnd)] >= 0) {
         // bound, but it stays in its solution list until the next step's first molsort
@@ -2131,6 +2132,9 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
     I.sf.bplist = s->d_bplist;
     I.sf.act = s->d_act;
     s->surf_on = true;
+    s->surf_curved = false;
+    for (int k = 0; k < surf->npnl; k++)
+      if (surf->pnl[k].ps == kPsTri || surf->pnl[k].ps == kPsCyl || surf->pnl[k].ps == kPsHemi) s->surf_curved = true;
   }
   (void)hipMemset(s->bn.count, 0, sizeof(int) * s->nscan);
   (void)hipMemset(s->bestkey, 0xFF, sizeof(uint64_t) * rc);
@@ -2289,18 +2293,24 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   hipStream_t st = s->stream;
   Recs A = s->buf[src], B = s->buf[1 - src];
   const bool f = s->f32;
-  auto kdiff = s->multi ? (f ? k_diffuse<false, true, true> : k_diffuse<false, true, false>)
-               : s->surf_on ? (f ? k_diffuse<true, false, true> : k_diffuse<true, false, false>)
-                            : (f ? k_diffuse<false, false, true> : k_diffuse<false, false, false>);
+  // surface variants: 2 when the model has triangles, cylinders or
+  // hemispheres, 1 for rectangles, spheres and disks only
+  const int sv = s->surf_on ? (s->surf_curved ? 2 : 1) : 0;
+  auto kdiff = s->multi ? (f ? k_diffuse<0, true, true> : k_diffuse<0, true, false>)
+               : sv == 2 ? (f ? k_diffuse<2, false, true> : k_diffuse<2, false, false>)
+               : sv == 1 ? (f ? k_diffuse<1, false, true> : k_diffuse<1, false, false>)
+                         : (f ? k_diffuse<0, false, true> : k_diffuse<0, false, false>);
   auto kscat = s->multi ? (f ? k_scatter<true, true, false> : k_scatter<true, false, false>)
                : s->surf_on ? (f ? k_scatter<false, true, true> : k_scatter<false, false, true>)
                             : (f ? k_scatter<false, true, false> : k_scatter<false, false, false>);
-  auto kpair = s->multi ? k_pairs<true, false> : (s->surf_on ? k_pairs<false, true> : k_pairs<false, false>);
-  auto kexact = s->multi ? (f ? k_exact<false, true, true> : k_exact<false, true, false>)
-                : s->surf_on ? (f ? k_exact<true, false, true> : k_exact<true, false, false>)
-                             : (f ? k_exact<false, false, true> : k_exact<false, false, false>);
-  auto kres = s->surf_on ? (f ? k_resolve<true, true> : k_resolve<false, true>)
-                         : (f ? k_resolve<true, false> : k_resolve<false, false>);
+  auto kpair = s->multi ? k_pairs<true, 0> : (s->surf_on ? k_pairs<false, 1> : k_pairs<false, 0>);
+  auto kexact = s->multi ? (f ? k_exact<0, true, true> : k_exact<0, true, false>)
+                : sv == 2 ? (f ? k_exact<2, false, true> : k_exact<2, false, false>)
+                : sv == 1 ? (f ? k_exact<1, false, true> : k_exact<1, false, false>)
+                          : (f ? k_exact<0, false, true> : k_exact<0, false, false>);
+  auto kres = sv == 2 ? (f ? k_resolve<true, 2> : k_resolve<false, 2>)
+              : sv == 1 ? (f ? k_resolve<true, 1> : k_resolve<false, 1>)
+                        : (f ? k_resolve<true, 0> : k_resolve<false, 0>);
   if (timed) (void)hipEventRecord(s->ev[0], st);
   hipLaunchKernelGGL(kdiff, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, s->bn.count, s->gtab,
                      s->ngauss - 1, dinj, s->inv);

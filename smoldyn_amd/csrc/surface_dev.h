@@ -86,15 +86,16 @@ __device__ __forceinline__ bool srf_ptintriangle(const DevPanel& pn, const doubl
 }
 
 // panelside (smolsurface.c:3930-3963)
+template <bool kCv>
 __device__ __forceinline__ int srf_panelside(const DevPanel& pn, const double* pt, double* distp, int strict) {
   double dist = 0;
   if (pn.ps == kPsRect) {
     int a = (int)pn.front[1];
     dist = pn.front[0] * (pt[a] - pn.point[0][a]);
-  } else if (pn.ps == kPsDisk || pn.ps == kPsTri) {
+  } else if (pn.ps == kPsDisk || (kCv && pn.ps == kPsTri)) {
 #pragma unroll
     for (int d = 0; d < 3; d++) dist += (pt[d] - pn.point[0][d]) * pn.front[d];
-  } else if (pn.ps == kPsCyl) {
+  } else if (kCv && pn.ps == kPsCyl) {
     double cylnorm[3];
     dist = srf_linenormal3(pn.point[0], pn.point[1], pt, cylnorm);
     dist = pn.front[2] * (dist - pn.point[2][0]);
@@ -151,12 +152,13 @@ __device__ __forceinline__ double srf_linexsphs(const double* pt1, const double*
 }
 
 // lineXpanel (smolsurface.c:3999-4101), 3D panels
+template <bool kCv>
 __device__ __forceinline__ int srf_linexpanel(const DevPanel& pn, const double* pt1, const double* pt2,
                                               double* crsspt, int* face1p, int* face2p, double* crossp,
                                               double* cross2p) {
   double dist1, dist2, cross = -1, cross2 = -1;
-  int face1 = srf_panelside(pn, pt1, &dist1, 0);
-  int face2 = srf_panelside(pn, pt2, &dist2, 0);
+  int face1 = srf_panelside<kCv>(pn, pt1, &dist1, 0);
+  int face2 = srf_panelside<kCv>(pn, pt2, &dist2, 0);
   int intsct;
   if (pn.ps == kPsRect) {
     if (face1 == face2) return 0;
@@ -170,7 +172,7 @@ __device__ __forceinline__ int srf_linexpanel(const DevPanel& pn, const double* 
     if (d == (int)pn.front[1]) d = (d + 1) % 3;
     intsct = intsct && ((pn.point[1][d] <= crsspt[d] && crsspt[d] <= pn.point[2][d]) ||
                         (pn.point[2][d] <= crsspt[d] && crsspt[d] <= pn.point[1][d]));
-  } else if (pn.ps == kPsSph || pn.ps == kPsHemi) {
+  } else if (pn.ps == kPsSph || (kCv && pn.ps == kPsHemi)) {
     int facein = pn.front[0] > 0 ? kFaceBack : kFaceFront;
     if (face1 == facein && face2 == facein) return 0;
     double nrdist, nrpos;
@@ -180,7 +182,7 @@ __device__ __forceinline__ int srf_linexpanel(const DevPanel& pn, const double* 
 #pragma unroll
     for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
     intsct = 1;
-    if (pn.ps == kPsHemi) {  // the crossing must be on the dome side of the opening
+    if (kCv && pn.ps == kPsHemi) {  // the crossing must be on the dome side of the opening
       double dot = 0;
       for (int d = 0; d < 3; d++) dot += (crsspt[d] - pn.point[0][d]) * pn.point[2][d];
       intsct = (dot <= 0);
@@ -193,7 +195,7 @@ __device__ __forceinline__ int srf_linexpanel(const DevPanel& pn, const double* 
         intsct = (dot <= 0);
       }
     }
-  } else if (pn.ps == kPsCyl) {
+  } else if (kCv && pn.ps == kPsCyl) {
     // Geo_LineXCyls (Geometry.c:1248-1258): the segment's perpendicular
     // offsets from the axis against a circle about the origin
     int facein = (int)pn.front[2] > 0 ? kFaceBack : kFaceFront;
@@ -216,7 +218,7 @@ __device__ __forceinline__ int srf_linexpanel(const DevPanel& pn, const double* 
       for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
       intsct = srf_ptinslab(pn.point[0], pn.point[1], crsspt);
     }
-  } else if (pn.ps == kPsTri) {
+  } else if (kCv && pn.ps == kPsTri) {
     if (face1 == face2) return 0;
     cross = dist1 / (dist1 - dist2);
     for (int d = 0; d < 3; d++) crsspt[d] = pt1[d] + cross * (pt2[d] - pt1[d]);
@@ -239,15 +241,16 @@ __device__ __forceinline__ int srf_linexpanel(const DevPanel& pn, const double* 
 }
 
 // getpanelnormal (smolsurface.c:4430-4456)
+template <bool kCv>
 __device__ __forceinline__ void srf_getpanelnormal(const DevPanel& pn, const double* pt, double* norm) {
   if (pn.ps == kPsRect) {
     norm[0] = norm[1] = norm[2] = 0;
     norm[(int)pn.front[1]] = pn.front[0];
-  } else if (pn.ps == kPsDisk || pn.ps == kPsTri) {
+  } else if (pn.ps == kPsDisk || (kCv && pn.ps == kPsTri)) {
     norm[0] = pn.front[0];
     norm[1] = pn.front[1];
     norm[2] = pn.front[2];
-  } else if (pn.ps == kPsCyl) {
+  } else if (kCv && pn.ps == kPsCyl) {
     srf_linenormal3(pn.point[0], pn.point[1], pt, norm);
     if (pn.front[2] == -1)
       for (int d = 0; d < 3; d++) norm[d] *= -1;
@@ -258,28 +261,30 @@ __device__ __forceinline__ void srf_getpanelnormal(const DevPanel& pn, const dou
 
 // fixpt2panel (smolsurface.c:4460-4474): onto the panel, then nudged by
 // doubling multiples of DBL_EPSILON until on the requested side
+template <bool kCv>
 __device__ __forceinline__ void srf_fixpt2panel(const DevPanel& pn, double* pt, int face, double epsilon) {
   double dist, norm[3];
-  int faceat = srf_panelside(pn, pt, &dist, 1);
+  int faceat = srf_panelside<kCv>(pn, pt, &dist, 1);
   if ((faceat == face || face == kFaceNone) && fabs(dist) <= epsilon) return;
-  srf_getpanelnormal(pn, pt, norm);
+  srf_getpanelnormal<kCv>(pn, pt, norm);
 #pragma unroll
   for (int d = 0; d < 3; d++) pt[d] -= dist * norm[d];
   if (face == kFaceNone) return;
   int sign = (face == kFaceFront) ? 1 : -1;
   // each pass doubles the step; 1100 doublings exceed any finite coordinate
-  for (double factor = 1.0; face != srf_panelside(pn, pt, nullptr, 1) && factor < 1e300; factor *= 2.0) {
+  for (double factor = 1.0; face != srf_panelside<kCv>(pn, pt, nullptr, 1) && factor < 1e300; factor *= 2.0) {
 #pragma unroll
     for (int d = 0; d < 3; d++) pt[d] += sign * factor * kDblEpsilon * norm[d];
   }
 }
 
 // surfacereflect (smolsurface.c:4821-4869), solution-phase molecule
+template <bool kCv>
 __device__ __forceinline__ void srf_reflect(const DevPanel& pn, double* pos, const double* crsspt) {
   if (pn.ps == kPsRect) {
     int axis = (int)pn.front[1];
     pos[axis] -= 2.0 * (pos[axis] - crsspt[axis]);
-  } else if (pn.ps == kPsDisk || pn.ps == kPsTri) {
+  } else if (pn.ps == kPsDisk || (kCv && pn.ps == kPsTri)) {
     double dot = 0;
 #pragma unroll
     for (int d = 0; d < 3; d++) dot += (pos[d] - crsspt[d]) * pn.front[d];
@@ -287,7 +292,7 @@ __device__ __forceinline__ void srf_reflect(const DevPanel& pn, double* pos, con
     for (int d = 0; d < 3; d++) pos[d] -= 2.0 * pn.front[d] * dot;
   } else {
     double norm[3], dot = 0;
-    if (pn.ps == kPsCyl) srf_linenormal3(pn.point[0], pn.point[1], crsspt, norm);
+    if (kCv && pn.ps == kPsCyl) srf_linenormal3(pn.point[0], pn.point[1], crsspt, norm);
     else srf_spherenormal(pn.point[0], crsspt, 1, norm);
 #pragma unroll
     for (int d = 0; d < 3; d++) dot += (pos[d] - crsspt[d]) * norm[d];
@@ -370,6 +375,7 @@ __device__ __forceinline__ int srf_action(const DevSurf& sf, int srf, int ident,
 }
 
 // ptinpanel (smolsurface.c:4276-4323) for 3D panels
+template <bool kCv>
 __device__ __forceinline__ bool srf_ptinpanel(const DevPanel& pn, const double* pt) {
   if (pn.ps == kPsRect) {
     int d = (int)pn.front[2];
@@ -380,9 +386,9 @@ __device__ __forceinline__ bool srf_ptinpanel(const DevPanel& pn, const double* 
                   (pn.point[2][d] <= pt[d] && pt[d] <= pn.point[1][d]));
   }
   if (pn.ps == kPsSph) return true;
-  if (pn.ps == kPsTri) return srf_ptintriangle(pn, pt);
-  if (pn.ps == kPsCyl) return srf_ptinslab(pn.point[0], pn.point[1], pt);
-  if (pn.ps == kPsHemi) {
+  if (kCv && pn.ps == kPsTri) return srf_ptintriangle(pn, pt);
+  if (kCv && pn.ps == kPsCyl) return srf_ptinslab(pn.point[0], pn.point[1], pt);
+  if (kCv && pn.ps == kPsHemi) {
     double dot = 0;
     for (int d = 0; d < 3; d++) dot += (pt[d] - pn.point[0][d]) * pn.point[2][d];
     return dot <= 0;
@@ -398,17 +404,18 @@ __device__ __forceinline__ bool srf_ptinpanel(const DevPanel& pn, const double* 
 }
 
 // panelnormal (smolsurface.c:4400-4428): outward unit normal of the given face
+template <bool kCv>
 __device__ __forceinline__ void srf_panelnormal(const DevPanel& pn, const double* pos, int face, double* norm) {
   if (face != kFaceBack) face = kFaceFront;
   if (pn.ps == kPsRect) {
     norm[0] = norm[1] = norm[2] = 0;
     norm[(int)pn.front[1]] =
         ((face == kFaceFront && pn.front[0] == 1) || (face == kFaceBack && pn.front[0] == -1)) ? 1.0 : -1.0;
-  } else if (pn.ps == kPsDisk || pn.ps == kPsTri) {
+  } else if (pn.ps == kPsDisk || (kCv && pn.ps == kPsTri)) {
     const double sgn = face == kFaceFront ? 1.0 : -1.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) norm[d] = sgn * pn.front[d];
-  } else if (pn.ps == kPsCyl) {
+  } else if (kCv && pn.ps == kPsCyl) {
     srf_linenormal3(pn.point[0], pn.point[1], pos, norm);
     if ((face == kFaceBack && pn.front[2] == 1) || (face == kFaceFront && pn.front[2] == -1))
       for (int d = 0; d < 3; d++) norm[d] *= -1;
@@ -432,6 +439,7 @@ __device__ __forceinline__ double srf_desorbdist(double step, double y, bool rev
 
 // surfacejump (smolsurface.c:4873-4926) for 3D panels;
 // returns 0 when the panel has no jump destination for that face
+template <bool kCv>
 __device__ __forceinline__ int srf_jump(const DevSurf& sf, const DevPanel& pn, double* pos, double* crsspt,
                                         int face) {
   int p2 = pn.jumpp[face], face2 = pn.jumpf[face];
@@ -443,16 +451,16 @@ __device__ __forceinline__ int srf_jump(const DevSurf& sf, const DevPanel& pn, d
 #pragma unroll
     for (int d = 0; d < 3; d++) delta[d] = 0.5 * (q.point[0][d] + q.point[2][d]) - 0.5 * (pn.point[0][d] + pn.point[2][d]);
     dir = (pn.front[0] == q.front[0]) ? 1 : -1;
-  } else if (pn.ps == kPsSph || pn.ps == kPsHemi) {
+  } else if (pn.ps == kPsSph || (kCv && pn.ps == kPsHemi)) {
 #pragma unroll
     for (int d = 0; d < 3; d++)
       delta[d] = (crsspt[d] - pn.point[0][d]) * q.point[1][0] / pn.point[1][0] + q.point[0][d] - crsspt[d];
     dir = (pn.front[0] == q.front[0]) ? 1 : -1;
-  } else if (pn.ps == kPsCyl) {
+  } else if (kCv && pn.ps == kPsCyl) {
     for (int d = 0; d < 3; d++)
       delta[d] = (crsspt[d] - pn.point[0][d]) * q.point[2][0] / pn.point[2][0] + q.point[0][d] - crsspt[d];
     dir = (pn.front[2] == q.front[2]) ? 1 : -1;
-  } else if (pn.ps == kPsTri) {
+  } else if (kCv && pn.ps == kPsTri) {
     // Geo_TriCenter (Geometry.c:43-53): (1.0/3.0) times the vertex sum
     double dot = 0;
     for (int d = 0; d < 3; d++)
@@ -473,9 +481,9 @@ __device__ __forceinline__ int srf_jump(const DevSurf& sf, const DevPanel& pn, d
     crsspt[d] += delta[d];
     pos[d] += delta[d];
   }
-  srf_fixpt2panel(q, crsspt, face2, 0);
+  srf_fixpt2panel<kCv>(q, crsspt, face2, 0);
   dir *= (face != face2) ? 1 : -1;
-  if (dir == -1) srf_reflect(q, pos, crsspt);
+  if (dir == -1) srf_reflect<kCv>(q, pos, crsspt);
   return 1;
 }
 
@@ -486,6 +494,7 @@ enum SrfStatus : int { kSrfOk = 0, kSrfAbsorbed = 1, kSrfUnsupported = 2, kSrfAd
 // change on an action); difc gives the motion correction of molchangeident
 // (smolmolec.c:2882-2885).  Draws: Philox keyed by (serial, step,
 // SMOLRNG_SURFACE, pass<<8 | iteration).
+template <bool kCv>
 __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, const double* difc, double* pos,
                                          const double* posx, int* ident, uint64_t serial, uint32_t step, int pass,
                                          uint32_t key0, uint32_t key1, int* bound = nullptr) {
@@ -504,7 +513,7 @@ __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, c
         int p = sf.bplist[k];
         double crsspt[3], cross;
         int face;
-        if (srf_linexpanel(sf.pnl[p], via, pos, crsspt, &face, nullptr, &cross, nullptr) && cross <= crossmin2) {
+        if (srf_linexpanel<kCv>(sf.pnl[p], via, pos, crsspt, &face, nullptr, &cross, nullptr) && cross <= crossmin2) {
           if (cross <= crossmin) {
             crossmin2 = crossmin;
             crossmin = cross;
@@ -538,22 +547,22 @@ __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, c
     if (act == kSaAdsorb) {
       // adsorption (dosurfinteract, smolsurface.c:5005-5020): onto the panel
       // at the crossing point, on the face of the bound state
-      if (!bound || !srf_ptinpanel(pn, crssptmin)) return kSrfUnsupported;
+      if (!bound || !srf_ptinpanel<kCv>(pn, crssptmin)) return kSrfUnsupported;
 #pragma unroll
       for (int d = 0; d < 3; d++) pos[d] = crssptmin[d];
-      srf_fixpt2panel(pn, pos, ms2 == 1 ? kFaceFront : kFaceBack, sf.epsilon);
+      srf_fixpt2panel<kCv>(pn, pos, ms2 == 1 ? kFaceFront : kFaceBack, sf.epsilon);
       *ident = i2;
       *bound = (pmin << 3) | ms2;
       return kSrfAdsorbed;
     }
     if (act == kSaTrans) {
-      srf_fixpt2panel(pn, crssptmin, facemin == kFaceFront ? kFaceBack : kFaceFront, sf.epsilon);
+      srf_fixpt2panel<kCv>(pn, crssptmin, facemin == kFaceFront ? kFaceBack : kFaceFront, sf.epsilon);
     } else if (act == kSaReflect) {
-      srf_reflect(pn, pos, crssptmin);
-      srf_fixpt2panel(pn, crssptmin, facemin, sf.epsilon);
+      srf_reflect<kCv>(pn, pos, crssptmin);
+      srf_fixpt2panel<kCv>(pn, crssptmin, facemin, sf.epsilon);
     } else if (act == kSaJump) {
-      if (!srf_jump(sf, pn, pos, crssptmin, facemin))
-        srf_fixpt2panel(pn, crssptmin, facemin == kFaceFront ? kFaceBack : kFaceFront, sf.epsilon);
+      if (!srf_jump<kCv>(sf, pn, pos, crssptmin, facemin))
+        srf_fixpt2panel<kCv>(pn, crssptmin, facemin == kFaceFront ? kFaceBack : kFaceFront, sf.epsilon);
     } else if (act == kSaAbsorb) {
       return kSrfAbsorbed;
     } else {
@@ -582,6 +591,7 @@ __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, c
 // becomes a solution-phase molecule: returns kSrfDesorbed with pos and
 // ident updated, and *boxp = the box the reference files it in (that of the
 // position before the surface check).  bound = (panel << 3) | state.
+template <bool kCv>
 __device__ __forceinline__ int srf_bound(const DevSurf& sf, const Geometry& g, const double* difc,
                                          const double* difstep, double* pos, int* ident, int bound, uint64_t serial,
                                          uint32_t step, uint32_t key0, uint32_t key1, int* boxp) {
@@ -603,19 +613,19 @@ __device__ __forceinline__ int srf_bound(const DevSurf& sf, const Geometry& g, c
   if (o != kSaRevdes && o != kSaIrrevdes) return kSrfUnsupported;
   const int face = ms2 == 0 ? kFaceFront : kFaceBack;
   double crsspt[3] = {pos[0], pos[1], pos[2]};
-  srf_fixpt2panel(pn, crsspt, face, sf.epsilon);
+  srf_fixpt2panel<kCv>(pn, crsspt, face, sf.epsilon);
   const double y = smolrng_u01cc(
       smolrng_mol(key0, key1, serial, step, SMOLRNG_SURFACE, (uint32_t)SMOLRNG_SURF_DESORBDIST << 8).v[0]);
   const double x = srf_desorbdist(difstep[i2], y, o == kSaRevdes);
   double norm[3];
-  srf_panelnormal(pn, crsspt, face, norm);
+  srf_panelnormal<kCv>(pn, crsspt, face, norm);
 #pragma unroll
   for (int d = 0; d < 3; d++) pos[d] = crsspt[d] + x * norm[d];
   // the reference files the new molecule in the box of this position, before
   // its surface check (smolsurface.c:5036)
   *boxp = pos2box(g, pos[0], pos[1], pos[2]);
   int i3 = i2;
-  const int st = srf_check(sf, g, difc, pos, crsspt, &i3, serial, step, SMOLRNG_SURF_DESORBED, key0, key1, nullptr);
+  const int st = srf_check<kCv>(sf, g, difc, pos, crsspt, &i3, serial, step, SMOLRNG_SURF_DESORBED, key0, key1, nullptr);
   if (st != kSrfOk) return kSrfUnsupported;
   *ident = i3;
   return kSrfDesorbed;
@@ -624,6 +634,7 @@ __device__ __forceinline__ int srf_bound(const DevSurf& sf, const Geometry& g, c
 // rxnXsurface (smolsurface.c:4388-4424) for two solution-phase molecules: 1
 // when a panel between them blocks the reaction.  Rate-based actions draw
 // Philox keyed by (serial pair, step, SMOLRNG_RXNSURF, j<<8 | draw index).
+template <bool kCv>
 __device__ __forceinline__ int srf_rxnblocked(const DevSurf& sf, const Geometry& g, const double* difc, const double* pos1,
                                               const double* pos2, int i1, int i2, uint64_t ser1, uint64_t ser2,
                                               uint32_t step, int j, uint32_t key0, uint32_t key1) {
@@ -646,7 +657,7 @@ __device__ __forceinline__ int srf_rxnblocked(const DevSurf& sf, const Geometry&
       const DevPanel& pn = sf.pnl[sf.bplist[k]];
       double crsspt[3], cross, cross2;
       int face1, face2;
-      if (srf_linexpanel(pn, pos1, pos2, crsspt, &face1, &face2, &cross, &cross2)) {
+      if (srf_linexpanel<kCv>(pn, pos1, pos2, crsspt, &face1, &face2, &cross, &cross2)) {
         if (face1 != face2) {
           if (rxnpt < cross || (rxnpt == cross && face1 == kFaceBack)) result = !trans(pn.srf, face2, i2);
           else result = !trans(pn.srf, face1, i1);
