@@ -217,6 +217,25 @@ def test_surfaces_all_shapes_injected_large_steps():
     _run_pair(spec, 40, check_every=4, inject=True)
 
 
+def test_surfaces_all_shapes_adsorption_bit_exact():
+    """Adsorption onto triangles, a cylinder and hemispheres (placement on
+    the panel at the crossing point on the face of the bound state, the
+    in-panel test of each shape) and desorption back off them along each
+    shape's normal: identical to the oracle."""
+    spec = ss.c3_shapes_spec(n=3000, seed=38)
+    ads = [("A", "soln", "fsoln", "front", 0.5, True), ("A", "soln", "bsoln", "back", 0.5, True),
+           ("A", "front", "front", "fsoln", 5.0, False), ("A", "back", "back", "bsoln", 5.0, False)]
+    spec["surfaces"][1]["rates"] = ads
+    spec["surfaces"][2]["rates"] = ads
+    spec["dt"] = 0.01
+    nev = _run_pair(spec, 200, check_every=10)
+    assert nev > 10
+    dsim = ss.build_device(spec)
+    dsim.run_steps(100)
+    import smoldyn_amd as sa
+    assert dsim.count("A", sa.MSfront) + dsim.count("A", sa.MSback) > 0
+
+
 # ---------------------------------------------------------------------------
 # ensembles
 
