@@ -111,3 +111,22 @@ def test_read_config_string():
     with pytest.raises(sa.SmoldynError):
         s.read_config("no_such_statement", "1")
     s.close()
+
+
+@pytest.mark.parametrize("stmt,ok", [("action both all reflect", True), ("action all both reflect", True),
+                                     ("action front A transmit", True), ("action A f transmit", True),
+                                     ("action b A reflect", True), ("action both nosuch reflect", False),
+                                     ("action A sideways reflect", False)])
+def test_surface_action_formats(tmp_path, stmt, ok):
+    """surfreadstring's action statement: the new format (species face
+    action) when its species is known and its face readable, else the old
+    one (face species action); faces may be shortened (f, b) and 'all'
+    means both faces."""
+    text = ("species A\ndim 3\nboundaries 0 0 10\nboundaries 1 0 10\nboundaries 2 0 10\n"
+            "start_surface s\n%s\npanel tri 1 1 1 9 1 1 5 9 1\nend_surface\n" % stmt)
+    if ok:
+        _load(tmp_path, text).close()
+    else:
+        with pytest.raises(sa.SmoldynError) as e:
+            _load(tmp_path, text)
+        assert "model.txt:7" in str(e.value)
