@@ -783,6 +783,11 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   EngineConfig cfg;
   memset(&cfg, 0, sizeof(cfg));
   cfg.f32 = sim->precision == 32;
+  // FP32 offsets round a stored position by up to 2^-24 of the box size,
+  // far more than fixpt2panel's nudges off a panel (multiples of DBL_EPSILON):
+  // a molecule stored next to a reflective panel could land on its far side
+  LCHECK(!(cfg.f32 && sim->surf.enabled()), fn, ECerror,
+         "FP32 box-offset records cannot be combined with surfaces (use precision 64)");
   {  // tuning override of the pair search's z sub-bins per box (1, 2, 4 or 8)
     const char* e = getenv("SMOLAMD_ZSUB_MAX");
     cfg.zsub_max = e ? atoi(e) : 0;

@@ -5,6 +5,7 @@
 // file order (molecule placement follows random_seed as in the reference).
 // Display statements are accepted and ignored; any other statement is an
 // error that names it and its line.
+#include <ctype.h>
 #include <errno.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -66,9 +67,18 @@ bool listed(const char* const* list, size_t n, const std::string& w) {
   return false;
 }
 
-// molstring2ms (smolmolec.c): state names
+// strbegin(shortword, word, 0) (libSteve/string2.c:81-90): s is a non-empty
+// leading part of word, compared without case
+bool strbegin_ci(const std::string& s, const char* word) {
+  size_t i = 0;
+  for (; i < s.size(); i++)
+    if (word[i] == '\0' || tolower((unsigned char)s[i]) != tolower((unsigned char)word[i])) return false;
+  return i > 0;
+}
+
+// molstring2ms (smolmolec.c:75-89): state names
 bool state_of(const std::string& s, MolecState* ms) {
-  if (s == "soln" || s == "solution" || s == "fsoln") *ms = MSsoln;
+  if (s == "soln" || s == "solution" || s == "fsoln" || s == "aq") *ms = MSsoln;
   else if (s == "front") *ms = MSfront;
   else if (s == "back") *ms = MSback;
   else if (s == "up") *ms = MSup;
@@ -93,25 +103,29 @@ bool species_state(const std::string& tok, std::string* name, MolecState* ms) {
   return !name->empty();
 }
 
-// surfstring2face (smolsurface.c:86-93): any leading part of the word
+// surfstring2face (smolsurface.c:86-93): any leading part of the word, any case
 bool face_of(const std::string& s, PanelFace* f) {
-  auto begins = [&](const char* word) { return !s.empty() && std::string(word).compare(0, s.size(), s) == 0; };
-  if (begins("front")) *f = PFfront;
-  else if (begins("back")) *f = PFback;
-  else if (begins("all") || begins("both")) *f = PFboth;
+  if (strbegin_ci(s, "front")) *f = PFfront;
+  else if (strbegin_ci(s, "back")) *f = PFback;
+  else if (strbegin_ci(s, "all") || strbegin_ci(s, "both")) *f = PFboth;
   else return false;
   return true;
 }
 
-// surfstring2act (smolsurface.c)
+// surfstring2act (smolsurface.c:108-125), tested in the reference's order
 bool action_of(const std::string& s, SrfAction* a) {
-  if (s == "reflect" || s == "r") *a = SAreflect;
-  else if (s == "transmit" || s == "t") *a = SAtrans;
-  else if (s == "absorb" || s == "a") *a = SAabsorb;
-  else if (s == "jump" || s == "j") *a = SAjump;
+  if (strbegin_ci(s, "reflect")) *a = SAreflect;
+  else if (strbegin_ci(s, "transmit")) *a = SAtrans;
+  else if (strbegin_ci(s, "absorb")) *a = SAabsorb;
+  else if (strbegin_ci(s, "jump")) *a = SAjump;
+  else if (s == "periodic") *a = SAjump;
   else if (s == "port") *a = SAport;
-  else if (s == "multiple" || s == "mult" || s == "m") *a = SAmult;
-  else if (s == "no") *a = SAno;
+  else if (strbegin_ci(s, "multiple")) *a = SAmult;
+  else if (strbegin_ci(s, "no")) *a = SAno;
+  else if (strbegin_ci(s, "adsorb")) *a = SAadsorb;
+  else if (strbegin_ci(s, "revdes")) *a = SArevdes;
+  else if (strbegin_ci(s, "irrevdes")) *a = SAirrevdes;
+  else if (strbegin_ci(s, "flip")) *a = SAflip;
   else return false;
   return true;
 }
@@ -204,12 +218,13 @@ int surface_statement(simptr sim, const std::string& srf, const std::string& wor
     const std::string sh = w[0];
     PanelShape ps;
     size_t npar;
-    if (sh == "rect" || sh == "r") ps = PSrect, npar = 5;
-    else if (sh == "sph" || sh == "sphere" || sh == "s") ps = PSsph, npar = 6;
-    else if (sh == "disk" || sh == "d") ps = PSdisk, npar = 8;
-    else if (sh == "tri" || sh == "triangle" || sh == "t") ps = PStri, npar = 9;
-    else if (sh == "cyl" || sh == "cylinder" || sh == "c") ps = PScyl, npar = 9;
-    else if (sh == "hemi" || sh == "hemisphere" || sh == "h") ps = PShemi, npar = 9;
+    // surfstring2ps (smolsurface.c:143-155): leading parts, any case
+    if (strbegin_ci(sh, "rectangle")) ps = PSrect, npar = 5;
+    else if (strbegin_ci(sh, "triangle")) ps = PStri, npar = 9;
+    else if (strbegin_ci(sh, "sphere")) ps = PSsph, npar = 6;
+    else if (strbegin_ci(sh, "cylinder")) ps = PScyl, npar = 9;
+    else if (strbegin_ci(sh, "hemisphere")) ps = PShemi, npar = 9;
+    else if (strbegin_ci(sh, "disk")) ps = PSdisk, npar = 8;
     else return fail("panel shape '" + sh + "' not recognized");
     size_t k = 1;
     std::string axis;

@@ -390,13 +390,12 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
             newll = I.sf.blist[i2][bstate(bnd)];
             mkind = kMoveList;
             atomicAdd((bnd & 3) == 1 ? &ctl->countf[i2] : &ctl->countk[i2], 1);
-          } else if (st == kSrfAdsorbed) {
+          } else if (st != kSrfOk || (i2 != i && tab->listof[i2] != ll)) {
+            // an outcome the device path does not run (absorption, adsorption
+            // into a state without a list, a species change into another
+            // list): report it and stop stepping
             atomicOr(&ctl->err, kErrSurface);
             atomicOr((int*)&gl->halt, 1);
-          } else if (st != kSrfOk) {
-            atomicOr(&ctl->err, kErrSurface);
-          } else if (i2 != i && tab->listof[i2] != ll) {
-            atomicOr(&ctl->err, kErrSurface);
           }
           if (i2 != i) {
             a.ident[r] = i2;
@@ -424,6 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
           mkind = kMoveDesorb;
         } else if (st != kSrfOk) {
           atomicOr(&ctl->err, kErrSurface);
+          atomicOr((int*)&gl->halt, 1);
         }
       }
       if (kSurf && pend) {  // the bound product joins the list of its state now (zero displacement: bound
@@ -1751,6 +1751,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
         atomicAdd((bnd & 3) == 1 ? &ctl->countf[i2] : &ctl->countk[i2], 1);
       } else if (st != kSrfOk || tab->listof[i2] != pr.list) {
         atomicOr(&ctl->err, kErrSurface);
+        atomicOr(&gl->halt, 1);
       }
       pr.ident = i2;
       pr.x = pp[0];

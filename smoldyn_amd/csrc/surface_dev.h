@@ -493,12 +493,16 @@ enum SrfStatus : int { kSrfOk = 0, kSrfAbsorbed = 1, kSrfUnsupported = 2, kSrfAd
 // segment posx -> pos, up to 100 interactions.  ident may change (species
 // change on an action); difc gives the motion correction of molchangeident
 // (smolmolec.c:2882-2885).  Draws: Philox keyed by (serial, step,
-// SMOLRNG_SURFACE, pass<<8 | iteration).
+// SMOLRNG_SURFACE, pass<<8 | iteration).  For a desorbed molecule (pass
+// SMOLRNG_SURF_DESORBED) this is checksurfaces1mol with crossminimum 0
+// (smolsurface.c:5041,5055-5112): crossings at a fraction within VERYCLOSE of
+// the segment's start are ignored.
 template <bool kCv>
 __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, const double* difc, double* pos,
                                          const double* posx, int* ident, uint64_t serial, uint32_t step, int pass,
                                          uint32_t key0, uint32_t key1, int* bound = nullptr) {
   double via[3] = {posx[0], posx[1], posx[2]};
+  const bool usemin = (pass == SMOLRNG_SURF_DESORBED);
   for (int it = 1;; it++) {
     if (it > 100) {
       pos[0] = posx[0];
@@ -513,7 +517,8 @@ __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, c
         int p = sf.bplist[k];
         double crsspt[3], cross;
         int face;
-        if (srf_linexpanel<kCv>(sf.pnl[p], via, pos, crsspt, &face, nullptr, &cross, nullptr) && cross <= crossmin2) {
+        if (srf_linexpanel<kCv>(sf.pnl[p], via, pos, crsspt, &face, nullptr, &cross, nullptr) && cross <= crossmin2 &&
+            (!usemin || cross - 0.0 > kVeryClose)) {
           if (cross <= crossmin) {
             crossmin2 = crossmin;
             crossmin = cross;

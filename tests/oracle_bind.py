@@ -326,10 +326,24 @@ class OSim:
         return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
 
 
+REFERENCE_TREE = Path("/root/reference/source/libSteve")
+_reflib_cache = []
+
+
+def reflib_available():
+    """True where the compiled reference checker can be used: it was built
+    and the reference tree it was built from is present (never on a machine
+    that only received this repository)."""
+    return REFLIB.exists() and REFERENCE_TREE.is_dir()
+
+
 def reflib():
-    """The compiled reference libSteve (checker only); None when absent."""
-    if not REFLIB.exists():
+    """The compiled reference libSteve (checker only), loaded on first use;
+    None where it is unavailable."""
+    if not reflib_available():
         return None
+    if _reflib_cache:
+        return _reflib_cache[0]
     L = C.CDLL(str(REFLIB))
     D, I = C.c_double, C.c_int
     dp = C.POINTER(C.c_double)
@@ -369,6 +383,7 @@ def reflib():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
+    _reflib_cache.append(L)
     return L
 
 

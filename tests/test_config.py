@@ -130,3 +130,27 @@ def test_surface_action_formats(tmp_path, stmt, ok):
         with pytest.raises(sa.SmoldynError) as e:
             _load(tmp_path, text)
         assert "model.txt:7" in str(e.value)
+
+
+@pytest.mark.parametrize("stmt,ok", [("panel rectangle +0 0 0 0 10 10", True), ("panel Rect +0 0 0 0 10 10", True),
+                                     ("panel TRIANG 1 1 1 9 1 1 5 9 1", True),
+                                     ("panel sph 5 5 5 2 10 10", True), ("panel Sphere 5 5 5 2", True),
+                                     ("panel cyli 2 5 5 8 5 5 1 10 10", True),
+                                     ("panel hemis 5 5 5 2 0 0 1 10 10", True), ("panel Di 5 5 5 2 0 0 1", True),
+                                     ("panel rectangles +0 0 0 0 10 10", False), ("panel all 5 5 5 2", False),
+                                     ("action Front A Reflect", True), ("action A BACK trans", True),
+                                     ("action A fr mult", True), ("action A both periodic", True),
+                                     ("action A both per", False)])
+def test_surface_names_take_leading_parts_any_case(tmp_path, stmt, ok):
+    """Shape, face and action names are read as the reference's strbegin
+    does (libSteve/string2.c:81-90, surfstring2ps / surfstring2face /
+    surfstring2act in smolsurface.c:86-155): any non-empty leading part of
+    the word, in any case; 'periodic' (a jump) only spelled out."""
+    text = ("species A\ndim 3\nboundaries 0 0 10\nboundaries 1 0 10\nboundaries 2 0 10\n"
+            "start_surface s\n%s\npanel tri 1 1 1 9 1 1 5 9 1\nend_surface\n" % stmt)
+    if ok:
+        _load(tmp_path, text).close()
+    else:
+        with pytest.raises(sa.SmoldynError) as e:
+            _load(tmp_path, text)
+        assert "model.txt:7" in str(e.value)
