@@ -1657,6 +1657,11 @@ int osim_step(osim* sim) {
   o_reassign(sim, 0, 1);
   sim->time += sim->dt;
   sim->stepindex++;
+  /* simdocommands (smolsim.cpp:2571-2581), called at the end of every step
+     (:2679), sorts the lists once more: molecules killed or moved by the
+     reborn wall or surface checks leave their lists before the next step's
+     diffusion (run-time commands themselves are restated in ocmd.c) */
+  o_molsort(sim, 0);
   if (sim->time >= sim->tmax) return 1;
   if (sim->time >= sim->tbreak) return 10;
   return 0;

@@ -130,16 +130,16 @@ bool action_of(const std::string& s, SrfAction* a) {
   return true;
 }
 
-// rxnstring2rp (smolreact.c:75-100)
+// rxnstring2rp (smolreact.c:79-106): one-letter codes and full names; "b"
+// is unbindrad, bounce is only spelled out
 bool placement_of(const std::string& s, RevParam* rp) {
   static const std::map<std::string, RevParam> m = {
-      {"none", RPnone},       {"irrev", RPirrev},     {"i", RPirrev},         {"confspread", RPconfspread},
-      {"a", RPconfspread},    {"bounce", RPbounce},   {"b", RPbounce},        {"pgem", RPpgem},
-      {"p", RPpgem},          {"pgemmax", RPpgemmax}, {"x", RPpgemmax},       {"pgemmaxw", RPpgemmaxw},
-      {"ratio", RPratio},     {"r", RPratio},         {"unbindrad", RPunbindrad}, {"b2", RPunbindrad},
-      {"pgem2", RPpgem2},     {"q", RPpgem2},         {"pgemmax2", RPpgemmax2}, {"y", RPpgemmax2},
-      {"ratio2", RPratio2},   {"s", RPratio2},        {"offset", RPoffset},   {"o", RPoffset},
-      {"fixed", RPfixed},     {"f", RPfixed}};
+      {"i", RPirrev},         {"a", RPconfspread},    {"p", RPpgem},          {"x", RPpgemmax},
+      {"r", RPratio},         {"b", RPunbindrad},     {"q", RPpgem2},         {"y", RPpgemmax2},
+      {"s", RPratio2},        {"o", RPoffset},        {"f", RPfixed},         {"irrev", RPirrev},
+      {"confspread", RPconfspread}, {"bounce", RPbounce}, {"pgem", RPpgem},   {"pgemmax", RPpgemmax},
+      {"ratio", RPratio},     {"unbindrad", RPunbindrad}, {"pgem2", RPpgem2}, {"pgemmax2", RPpgemmax2},
+      {"ratio2", RPratio2},   {"offset", RPoffset},   {"fixed", RPfixed}};
   auto it = m.find(s);
   if (it == m.end()) return false;
   *rp = it->second;

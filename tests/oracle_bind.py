@@ -19,6 +19,8 @@ REFLIB = ORACLE_DIR / "_ref" / "libsteve_ref.so"
 MS = {"solution": 0, "soln": 0, "fsoln": 0, "front": 1, "back": 2, "up": 3, "down": 4, "bsoln": 5, "all": 6}
 RP = {"none": 0, "irrev": 1, "confspread": 2, "bounce": 3, "pgem": 4, "pgemmax": 5, "pgemmaxw": 6,
       "ratio": 7, "unbindrad": 8, "pgem2": 9, "pgemmax2": 10, "ratio2": 11, "offset": 12, "fixed": 13}
+# one-letter codes of the configuration language (rxnstring2rp, smolreact.c:79-106)
+RP.update({"i": 1, "a": 2, "p": 4, "x": 5, "r": 7, "b": 8, "q": 9, "y": 10, "s": 11, "o": 12, "f": 13})
 RNG_SFMT, RNG_COUNTER = 0, 1
 PS = {"rect": 0, "tri": 1, "sph": 2, "sphere": 2, "cyl": 3, "cylinder": 3, "hemi": 4, "hemisphere": 4, "disk": 5}
 FACE = {"front": 0, "f": 0, "back": 1, "b": 1, "none": 2, "both": 3}

@@ -13,7 +13,13 @@ box walk along a trajectory (line2nextbox), and bimolecular reactions with
 surfaces present (rxnXsurface).  surf3 (3D) pins every 3D panel shape:
 rectangles, triangles, a sphere, cylinders, hemispheres and disks (their
 crossing tests, normals, reflections, box-panel tests), with reflect, absorb,
-transmit and jump actions.  allostery (2D, immobile molecules) pins
+transmit and jump actions.  bounds1 (2D) pins absorbing walls: the kill of
+a molecule beyond the wall, the Brownian-bridge coin exp(-2 d_i d / step^2)
+for one that stayed inside, both wall passes of a step, and the list sort at
+the end of every step (simdocommands, smolsim.cpp:2571-2581) that removes
+molecules killed by the second pass before the next diffusion.  bounds2 (2D)
+pins pgemmax product placement (unbindingradius of the geminate probability),
+a circle that reflects and jump rectangles on the box sides.  allostery (2D, immobile molecules) pins
 conformational-spread reactions: the confspread radius, the reaction
 probability 1-exp(-k dt), and products that keep their reactants' places and
 serial numbers."""
@@ -36,17 +42,9 @@ def run_model(name):
 
 
 @pytest.mark.parametrize("name", ["bounce2", "molecule", "lotvolt", "surf1", "ReflectToNewSpecies", "allostery",
-                                  "surf3"])
+                                  "surf3", "bounds1", "bounds2"])
 def test_s95_output_identical(name):
     code, out = run_model(name)
     assert code == 1
     gold = gzip.open(S95 / (name + "_275_out.txt.gz")).read().decode()
-    assert out == gold
-
-
-@pytest.mark.xfail(reason="absorbing-wall path not reproduced; the device engine rejects absorbing walls",
-                   strict=True)
-def test_s95_bounds1_absorbing_walls():
-    code, out = run_model("bounds1")
-    gold = gzip.open(S95 / "bounds1_275_out.txt.gz").read().decode()
     assert out == gold
