@@ -2344,6 +2344,16 @@ std::string sim_product_name(simptr sim, const char* reaction, int prd) {
   return std::string();
 }
 
+int sim_reaction_shape(simptr sim, const char* reaction, int* nprod) {
+  for (int o = 0; o < 3; o++)
+    for (auto& r : sim->rxn[o])
+      if (r.name == reaction) {
+        if (nprod) *nprod = r.nprod;
+        return o;
+      }
+  return -1;
+}
+
 void sim_set_accuracy(simptr sim, double accuracy) {
   sim->accur = accuracy;
   invalidate(sim);

@@ -448,6 +448,11 @@ int config_statement(simptr sim, const std::string& word, const std::string& res
     if (w.size() < 2) return fail("product_placement format: rname type parameters");
     RevParam rp;
     if (!placement_of(w[1], &rp)) return fail("unrecognized product placement '" + w[1] + "'");
+    int nprod = 0;
+    const int order = sim_reaction_shape(sim, w[0].c_str(), &nprod);
+    if (order < 0) return fail("unrecognized reaction name");
+    if (rp == RPbounce && (order != 2 || nprod < 2))
+      return fail("bounce can only be used with two reactants and at least two products");
     double prm = 0;
     if (rp == RPoffset || rp == RPfixed) {
       // the product by species name or as product_N (1-based), then its offset vector

@@ -15,6 +15,8 @@ void sim_bounds(simptr sim, double* lo, double* hi);
 std::vector<std::string> sim_species_names(simptr sim);
 // species of product prd of a reaction ("" when there is none)
 std::string sim_product_name(simptr sim, const char* reaction, int prd);
+// order of a reaction (-1 when there is none) and its product count
+int sim_reaction_shape(simptr sim, const char* reaction, int* nprod);
 // accuracy (smolsim.cpp "accuracy" statement)
 void sim_set_accuracy(simptr sim, double accuracy);
 // output_precision, output_format (ssv / csv), output_file_number

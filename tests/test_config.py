@@ -53,7 +53,7 @@ mol_list C lbc
 molperbox 4
 reaction fwd A + B -> C 20
 reaction back C -> A + B 2
-product_placement back pgemmaxw 0.2
+product_placement back x 0.2
 reaction pair A + C <-> B + B 5 1
 mol 100 A u u u
 mol 20 B 10-20 5 u
@@ -154,3 +154,21 @@ def test_surface_names_take_leading_parts_any_case(tmp_path, stmt, ok):
         with pytest.raises(sa.SmoldynError) as e:
             _load(tmp_path, text)
         assert "model.txt:7" in str(e.value)
+
+
+@pytest.mark.parametrize("code,ok", [("x 0.2", True), ("pgemmax 0.2", True), ("b 1.5", True), ("unbindrad 1.5", True),
+                                     ("r 1.2", True), ("p 0.3", True), ("bounce", False), ("pgemmaxw 0.2", False),
+                                     ("b2 1.5", False), ("none", False)])
+def test_product_placement_codes(tmp_path, code, ok):
+    """Placement names as rxnstring2rp reads them (smolreact.c:79-106): one
+    letter codes where 'b' is unbindrad; bounce (only for bimolecular
+    reactions) and pgemmaxw are not accepted for this first-order reaction,
+    the latter because the reader does not know the name."""
+    text = ("species A B C\ndim 3\nboundaries 0 0 10\nboundaries 1 0 10\nboundaries 2 0 10\ntime_step 0.01\n"
+            "difc all 1\nreaction fwd A + B -> C 20\nreaction back C -> A + B 2\nproduct_placement back %s\n" % code)
+    if ok:
+        _load(tmp_path, text).close()
+    else:
+        with pytest.raises(sa.SmoldynError) as e:
+            _load(tmp_path, text)
+        assert "model.txt:10" in str(e.value)
