@@ -384,6 +384,15 @@ int osim_set_confspread(osim* sim, int r, double radius) {
   return 0;
 }
 
+/* RxnSetValue "bindrad" (smolreact.c:1767-1769): a binding radius given
+   directly (the reaction's rate left undefined, so rxnsetrate keeps it) */
+int osim_set_bindrad(osim* sim, int r, double radius) {
+  if (r < 0 || r >= sim->rxnss[2].totrxn || radius < 0) return 1;
+  sim->rxnss[2].rxn[r]->bindrad2 = radius * radius;
+  sim->ok = 0;
+  return 0;
+}
+
 /* the position argument of RxnSetRevparam for offset and fixed placement
    (smolreact.c:1870-1874) */
 int osim_set_prdpos(osim* sim, int order, int r, int prd, const double* pos) {

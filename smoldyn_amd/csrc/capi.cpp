@@ -902,10 +902,16 @@ ErrorCode engine_error(const char* fn, int bits) {
     set_error(fn, ECerror, "HIP runtime error");
     return ECerror;
   }
-  set_error(fn, ECerror,
-            "device engine error (bits 0x%x: 1 candidate pairs, 2 events, 4 molecule capacity, 8 sort space, "
-            "16 surface action, 32 stepping halted)",
-            bits);
+  static const char* const names[] = {"candidate pair space exceeded", "event space exceeded",
+                                       "molecule list capacity exceeded", "sort space exceeded",
+                                       "surface outcome not run on the device", "stepping halted"};
+  std::string msg;
+  for (int k = 0; k < 6; k++)
+    if (bits & (1 << k)) {
+      if (!msg.empty()) msg += ", ";
+      msg += names[k];
+    }
+  set_error(fn, ECerror, "device engine error: %s (bits 0x%x)", msg.c_str(), bits);
   return ECerror;
 }
 

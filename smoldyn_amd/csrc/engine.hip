@@ -80,9 +80,7 @@ struct Recs {
   int* box;       // virtual box (-1: not binned this step)
   int* rank;      // arrival rank within the box
   int* ov;        // surfaces only: order value of the record within its box list (k_relist)
-  int* sb;        // surfaces only: 0 in solution, else (panel << 3) | pending << 2 | bound state (1 front,
-                  // 2 back); pending: adsorbed as a reaction product, still in its solution list until the
-                  // next step's first molsort
+  int* sb;        // surfaces only: 0 in solution, else (panel << 3) | bound state (1 front, 2 back)
 };
 
 struct Binned {
@@ -344,11 +342,9 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       const Geometry& g = I.g;
       int ll = tab->listof[i];
       int sb = 0;  // surface binding (kSurf): bound molecules live in the list of their state
-      bool pend = false;  // bound as a product last step, still in its solution list
       if (kSurf) {
         sb = a.sb[r];
-        pend = (sb & 4) != 0;
-        if (sb && !pend) ll = I.sf.blist[i][bstate(sb)];
+        if (sb) ll = I.sf.blist[i][bstate(sb)];
       }
       int lp = a.lpos[r];
       uint64_t ser = a.serno[r];
@@ -359,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       double pos[3];
       load_pos<kF32>(a, g, r, bold, pos);
       const bool moved = tab->diffuselist[ll];
-      if (moved && !pend) {
+      if (moved) {
         const double posx[3] = {pos[0], pos[1], pos[2]};
         double ds = tab->difstep[i];
         double gg[3];
@@ -404,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
             i = i2;
           }
         }
-      } else if (kSurf && sb && !pend && I.sf.listbound[ll]) {
+      } else if (kSurf && sb && I.sf.listbound[ll]) {
         // checksurfacebound (smolsurface.c:5184-5199): desorption
         int i2 = i;
         obox = pos2box(g, pos[0], pos[1], pos[2]);
@@ -425,13 +421,6 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
           atomicOr(&ctl->err, kErrSurface);
           atomicOr((int*)&gl->halt, 1);
         }
-      }
-      if (kSurf && pend) {  // the bound product joins the list of its state now (zero displacement: bound
-                            // molecules do not diffuse, so its segment crosses nothing)
-        sb &= ~4;
-        a.sb[r] = sb;
-        newll = I.sf.blist[i][bstate(sb)];
-        mkind = kMoveList;
       }
       int mev = -1;
       if (kSurf && mkind) {  // queue the list change; the first molsort gives the record its new slot
@@ -687,7 +676,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild_inv(const Inst* __restrict__
   if (i <= 0 || i >= kMaxSpecies || in < 0 || in >= gl->ninst) return;
   const Inst& I = insts[in];
   int ll = I.tab->listof[i];
-  if (a.sb && a.sb[r] && !(a.sb[r] & 4)) ll = I.sf.blist[i][bstate(a.sb[r])];
+  if (a.sb && a.sb[r]) ll = I.sf.blist[i][bstate(a.sb[r])];
   int lp = a.lpos[r];
   if (lp >= 0 && lp < I.L.base[ll + 1] - I.L.base[ll]) inv[I.L.base[ll] + lp] = r;
 }
@@ -1118,6 +1107,7 @@ __device__ int wg_scan(int v, int* s, int* tot) {
 
 constexpr int kLdsEvents = 1024;  // sorted event indices kept in LDS up to this count
 constexpr int kLdsKills = 2048;   // sorted kill slots kept in LDS up to this count
+constexpr int kMaxAds = 256;      // products adsorbed by their reborn surface check, per step
 
 // Sorted kill slots, read from LDS when they fit there, else from global.
 struct KillView {
@@ -1370,6 +1360,137 @@ __global__ __launch_bounds__(kResolveThreads) void k_relist(const Inst* __restri
   }
 }
 
+// The list sort that ends every step (simdocommands, smolsim.cpp:2571-2581,
+// a molsort of smolmolec.c:2943-3023) for products that their reborn surface
+// check adsorbed (molchangeident, smolmolec.c:2902-2906, put them in the list
+// of their bound state).  Those products sit among the products just
+// appended to the end of their solution lists, and nothing else changes list
+// in this sort.  Per list in order, the scan from the lowest such slot meets
+// each one, appends it to its bound list and fills its hole from the top of
+// the list (swap from tail); holes are met in ascending order, each one
+// followed by the removed molecules the swaps bring into it; when the
+// remaining top slots are all removed they are met from the lowest upward
+// and then downward from the top.  One thread runs the scan on the few
+// adsorbed products; the writes are spread over the workgroup.
+// ws_key/ws_val/ws_aux: LDS scratch of at least 2 * kMaxAds ints each.
+constexpr int kNoSlot = -1;
+__device__ void end_of_step_sort(const Inst& I, Control* ctl, Glob* gl, const Recs& B, int* __restrict__ inv,
+                                 int na, const int* s_aslot, const int* s_arec, const int* s_alist, const int* s_nf,
+                                 int* s_pl, int* ws_key, int* ws_val, int* ws_aux) {
+  const int tid = threadIdx.x, NT = blockDim.x;
+  if (na == 0) return;  // uniform: s_nads is read after a barrier
+  if (na > kMaxAds) {
+    if (tid == 0) {
+      atomicOr(&ctl->err, kErrSortOverflow);
+      atomicOr(&gl->halt, 1);
+    }
+    return;
+  }
+  int* s_sorted = ws_key;            // entry indices by slot
+  int* s_sslot = ws_key + kMaxAds;   // their slots
+  int* s_top = ws_val;               // records at the top slots of each list
+  int* s_tent = ws_val + kMaxAds;    // entry of a top slot that is removed, else -1
+  int* s_prec = ws_aux;              // placements: record, list, position
+  int* s_plist = ws_aux + 2 * kMaxAds;
+  int* s_ppos = ws_aux + 4 * kMaxAds;
+  __shared__ int s_key[kMaxAds], s_ord[kMaxAds], s_len[kMaxList], s_lo[kMaxList], s_cnt[kMaxList], s_np;
+  for (int a = tid; a < na; a += NT) {
+    s_key[a] = s_aslot[a];
+    s_ord[a] = a;
+  }
+  __syncthreads();
+  wg_rank_sort2<int>(s_key, s_ord, na, 0, s_sorted, s_sslot);
+  const int nlist = I.L.nlist;
+  if (tid < nlist) {
+    const int lo = I.L.base[tid], hi = I.L.base[tid + 1];
+    int l = 0, h = na;
+    while (l < h) {
+      const int mid = (l + h) >> 1;
+      if (s_sslot[mid] < lo) l = mid + 1;
+      else h = mid;
+    }
+    const int k0 = l;
+    h = na;
+    while (l < h) {
+      const int mid = (l + h) >> 1;
+      if (s_sslot[mid] < hi) l = mid + 1;
+      else h = mid;
+    }
+    s_lo[tid] = k0;
+    s_cnt[tid] = l - k0;
+    s_len[tid] = s_nf[tid] + s_pl[tid];
+  }
+  __syncthreads();
+  for (int ll = 0; ll < nlist; ll++) {  // top r slots of each list and which of them leave
+    const int lo = s_lo[ll], r = s_cnt[ll], b = s_len[ll] - r, base = I.L.base[ll];
+    for (int q = tid; q < r; q += NT) {
+      s_top[lo + q] = inv[base + b + q];
+      s_tent[lo + q] = kNoSlot;
+    }
+  }
+  __syncthreads();
+  for (int ll = 0; ll < nlist; ll++) {
+    const int lo = s_lo[ll], r = s_cnt[ll], b = s_len[ll] - r, base = I.L.base[ll];
+    for (int k = tid; k < r; k += NT) {
+      const int t = s_sslot[lo + k] - base;
+      if (t >= b) s_tent[lo + (t - b)] = s_sorted[lo + k];
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int np = 0;
+    auto place = [&](int rec, int list, int pos) {
+      s_prec[np] = rec;
+      s_plist[np] = list;
+      s_ppos[np] = pos;
+      np++;
+    };
+    auto meet = [&](int e) {  // append to the list of its bound state
+      const int L = s_alist[e];
+      place(s_arec[e], L, s_len[L]++);
+    };
+    for (int ll = 0; ll < nlist; ll++) {
+      const int lo = s_lo[ll], r = s_cnt[ll];
+      if (r == 0) continue;
+      const int n0 = s_len[ll], b = n0 - r, base = I.L.base[ll];
+      int nlow = 0;
+      while (nlow < r && s_sslot[lo + nlow] - base < b) nlow++;
+      int topl = n0;
+      for (int j = 0; j < nlow; j++) {
+        meet(s_sorted[lo + j]);
+        for (;;) {
+          topl--;
+          const int q = topl - b, e = s_tent[lo + q];
+          if (e != kNoSlot) {
+            meet(e);
+            continue;
+          }
+          place(s_top[lo + q], ll, s_sslot[lo + j] - base);
+          break;
+        }
+      }
+      if (topl > b) {  // the remaining top slots all leave: the lowest first, then from the top down
+        meet(s_tent[lo]);
+        for (int t = topl - 1; t > b; t--) meet(s_tent[lo + (t - b)]);
+      }
+      s_len[ll] = b;
+    }
+    s_np = np;
+    for (int ll = 0; ll < nlist; ll++) s_pl[ll] = s_len[ll] - s_nf[ll];
+  }
+  __syncthreads();
+  for (int k = tid; k < s_np; k += NT) {
+    const int rec = s_prec[k], list = s_plist[k], pos = s_ppos[k];
+    if (pos >= I.L.base[list + 1] - I.L.base[list]) {
+      atomicOr(&ctl->err, kErrCapacity);
+      atomicOr(&gl->halt, 1);
+      continue;
+    }
+    B.lpos[rec] = pos;
+    inv[I.L.base[list] + pos] = rec;
+  }
+}
+
 // One workgroup per simulation of the batch (blockIdx.x).
 template <bool kF32, int kSurf>
 __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restrict__ insts, Glob* gl, Recs RA,
@@ -1388,6 +1509,9 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   __shared__ int s_flag, s_nacc, s_prodoff;
   __shared__ int s_nl[kMaxList], s_nf[kMaxList], s_k0[kMaxList], s_k1[kMaxList], s_pl[kMaxList];
   __shared__ int s_zpre[kMaxRxn0 + 1];  // zeroth-order events before each reaction
+  // products adsorbed by their reborn surface check: (slot, record, bound list)
+  __shared__ int s_nads;
+  __shared__ int s_aslot[kSurf ? kMaxAds : 1], s_arec[kSurf ? kMaxAds : 1], s_alist[kSurf ? kMaxAds : 1];
   const int tid = threadIdx.x;
   const int NT = blockDim.x;
   Control* ctl = I.ctl;
@@ -1403,7 +1527,10 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   // ---- greedy matching: accept an edge when it is the smallest live key at
   // both of its molecules; this is the lexicographically-first maximal
   // matching, i.e. what the sequential loop of bireact accepts.
-  if (tid == 0) s_nacc = 0;
+  if (tid == 0) {
+    s_nacc = 0;
+    s_nads = 0;
+  }
   if (tid < kMaxSpecies) s_dcount[tid] = 0;
   for (int e = tid; e < E; e += NT) I.estate[e] = 0;
   __syncthreads();
@@ -1746,7 +1873,8 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       const int st = srf_check<(kSurf > 1)>(I.sf, I.g, tab->difc, pp, px0, &i2, ser0 + (uint64_t)c, step, SMOLRNG_SURF_REBORN,
                                I.key0, I.key1, &bnd);
       if (st == kSrfAdsorbed && I.sf.blist[i2][bstate(bnd)] >= 0) {
-        // bound, but it stays in its solution list until the next step's first molsort
+        // bound; it leaves its solution list in the list sort that ends the
+        // step (below); bit 2 marks it until then
         pr.sb = bnd | 4;
         atomicAdd((bnd & 3) == 1 ? &ctl->countf[i2] : &ctl->countk[i2], 1);
       } else if (st != kSrfOk || tab->listof[i2] != pr.list) {
@@ -1862,7 +1990,15 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
         const double pp[3] = {pr.x, pr.y, pr.z};
         store_pos<kF32>(B, I.g, rec, pbx, pp);
         B.ident[rec] = pr.ident;
-        if (B.sb) B.sb[rec] = pr.sb;
+        if (B.sb) B.sb[rec] = pr.sb & ~4;
+        if (kSurf && (pr.sb & 4)) {
+          const int a = atomicAdd(&s_nads, 1);
+          if (a < kMaxAds) {
+            s_aslot[a] = slot;
+            s_arec[a] = rec;
+            s_alist[a] = I.sf.blist[pr.ident][bstate(pr.sb)];
+          }
+        }
         if (B.inst) B.inst[rec] = blockIdx.x;  // batch member (multi-simulation records)
         B.serno[rec] = pr.serno ? pr.serno : ser0 + (uint64_t)c;
         B.lpos[rec] = slot - I.L.base[ll];
@@ -1882,6 +2018,10 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     __syncthreads();
   }
   if (overflow) atomicOr(&ctl->err, kErrCapacity);
+  if (kSurf) {
+    __syncthreads();
+    end_of_step_sort(I, ctl, gl, B, inv, s_nads, s_aslot, s_arec, s_alist, s_nf, s_pl, lv, s_ev, s_kill);
+  }
   // ---- species counts (accumulated in LDS by the product loop), list
   // lengths, counters
   __syncthreads();
@@ -2548,7 +2688,7 @@ int engine_get_list_inst(DeviceState* s, int inst, int ll, int* ident, double* p
   for (int k = 0; k < n; k++) {
     int r = inv[k];
     if (r < 0 || r >= nr) return -3;
-    if (mstate) mstate[k] = sb.empty() ? 0 : (sb[(size_t)r] & 3);  // a bound product keeps its state in its old list
+    if (mstate) mstate[k] = sb.empty() ? 0 : (sb[(size_t)r] & 3);
     if (ident) ident[k] = id[r];
     if (serno) serno[k] = sr[r];
     if (box) box[k] = bx[r];

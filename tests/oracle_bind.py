@@ -46,6 +46,7 @@ def lib():
         sig = {
             "ocmd_timeline": (I, [C.c_char_p, I, dp, ip, C.POINTER(C.c_longlong)]),
             "osim_set_confspread": (I, [P, I, D]),
+            "osim_set_bindrad": (I, [P, I, D]),
             "osim_set_prdpos": (I, [P, I, I, I, dp]),
             "osim_rng_rotation": (None, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, dp]),
             "osim_new": (P, [I, dp, dp]),
@@ -218,6 +219,9 @@ class OSim:
 
     def confspread(self, rxn, radius):
         assert rxn[0] == 2 and self.L.osim_set_confspread(self.p, rxn[1], radius) == 0
+
+    def bindrad(self, rxn, radius):
+        assert rxn[0] == 2 and self.L.osim_set_bindrad(self.p, rxn[1], radius) == 0
 
     def prdpos(self, rxn, prd, pos):
         pos = np.ascontiguousarray(pos, dtype=np.float64)

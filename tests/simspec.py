@@ -50,6 +50,8 @@ def build_oracle(spec, sfmt=False):
             sim.prdpos(rx[name], prd.index(prod), vec)
     for name, radius in spec.get("confspread", {}).items():
         sim.confspread(rx[name], radius)
+    for name, radius in spec.get("bindrad", {}).items():
+        sim.bindrad(rx[name], radius)
     for srf in spec.get("surfaces", []):
         s = sim.add_surface(srf["name"])
         for face, sp, act in srf.get("actions", []):
@@ -105,6 +107,8 @@ def build_device(spec, update=True):
             sim.product_placement(name, pl[0], pl[1])
     for name, radius in spec.get("confspread", {}).items():
         sim.product_placement(name, "confspread")
+        sim.binding_radius(name, radius)
+    for name, radius in spec.get("bindrad", {}).items():
         sim.binding_radius(name, radius)
     for srf in spec.get("surfaces", []):
         sim.add_surface(srf["name"])
@@ -243,6 +247,30 @@ def c4_member(k, seed=500, n_lo=1000, n_hi=100000, side_lo=50.0, side_hi=400.0, 
                 reactions=[("fwd", ["A", "B"], ["C"], rate), ("back", ["C"], ["A", "B"], 1.0)],
                 placement={"back": ("ratio", 1.5)},
                 molecules=[("A", n // 2), ("B", n - n // 2)])
+
+
+def c5_member(k, seed=700, dt=0.01, n=10000, side=100.0):
+    """Member k of the learned-surrogate sweep: A+B<->C in a reflective cube of
+    side 100 with n/2 A + n/2 B, D = 1/1/0.5, dt 0.01 (rms step of the pair
+    0.2); rms step / binding radius log-uniform in [0.05, 5], unbinding /
+    binding radius ratio uniform in [1, 3]; the target forward rate is the
+    reaction-parameter solver's rate at those radii, and the setup inverts it
+    back into a binding radius (the surrogate's job).  Back rate 1."""
+    import smoldyn_amd as sa
+    rng = np.random.default_rng(seed * 100003 + k)
+    step = (2 * 2.0 * dt) ** 0.5
+    while True:
+        soa = float(np.exp(rng.uniform(np.log(0.05), np.log(5.0))))
+        ratio = float(rng.uniform(1.0, 3.0))
+        a = step / soa
+        rate = sa.lib().smolamdNumRxnRate(step, a, ratio * a) / dt
+        if rate > 0:
+            break
+    return dict(low=[0.0] * 3, high=[side] * 3, walls="rrr", dt=dt, seed=seed + k,
+                species={"A": 1.0, "B": 1.0, "C": 0.5},
+                reactions=[("fwd", ["A", "B"], ["C"], rate), ("back", ["C"], ["A", "B"], 1.0)],
+                placement={"back": ("ratio", ratio)},
+                molecules=[("A", n // 2), ("B", n - n // 2)], target_a=a, ratio=ratio, step=step)
 
 
 def oracle_state(osim):

@@ -372,3 +372,80 @@ def test_bounce_placement():
                 molecules=[("A", 1500), ("B", 1500)])
     nev = _run_pair(spec, 60, check_every=10)
     assert nev > 100
+
+
+@pytest.mark.parametrize("kind,param", [("pgem", 0.3), ("pgemmax", 0.2), ("unbindrad", 1.4)])
+def test_geminate_and_unbinding_placements(kind, param):
+    """C -> A + B reversible with A + B -> C, the unbinding separation from
+    the geminate-rebinding probability (pgem: forward rate scaled by
+    1 - pgem; pgemmax: only when the unbinding radius comes out positive;
+    rxnsetrate smolreact.c:1383-1404, rxnsetproduct :1553-1570) or given
+    directly (unbindrad): products and serial numbers bit-exact with the
+    oracle, whose placements are pinned by closed forms and the reference's
+    bounds2 output."""
+    spec = dict(low=[0, 0, 0], high=[30, 30, 30], walls="ppp", dt=0.01, seed=67,
+                species={"A": 1.0, "B": 0.7, "C": 0.5},
+                reactions=[("fwd", ["A", "B"], ["C"], 25.0), ("back", ["C"], ["A", "B"], 8.0)],
+                placement={"back": (kind, param)},
+                molecules=[("A", 2000), ("B", 2000)])
+    nev = _run_pair(spec, 60, check_every=10)
+    assert nev > 100
+
+
+def _desorb_offset_spec(n=2000, seed=61, delta=5e-14):
+    """Molecules of A above a plane P (z = 50) that adsorbs A from its front
+    (probability 0.7) and from which A desorbs at rate 50; a second plane Q
+    lies delta above P, transmits A from its front and reflects A from its
+    back.  A desorbed molecule starts between the planes, so its path meets Q
+    at a fraction (delta - ulp)/x of its length: below VERYCLOSE (1e-12) for
+    most desorption distances x."""
+    rng = np.random.default_rng(seed)
+    pos = np.column_stack([rng.uniform(5, 95, (n, 2)), rng.uniform(50.02, 50.3, n)])
+    return dict(low=[0, 0, 0], high=[100, 100, 100], walls="rrr", dt=0.01, seed=seed,
+                species={"A": 1.0},
+                surfaces=[{"name": "p", "actions": [],
+                           "rates": [("A", "soln", "fsoln", "front", 0.7, True),
+                                     ("A", "front", "front", "fsoln", 50.0, False)],
+                           "panels": [("rect", "+2", [0.0, 0.0, 50.0, 100.0, 100.0], "p0")]},
+                          {"name": "q", "actions": [("front", "A", "transmit"), ("back", "A", "reflect")],
+                           "panels": [("rect", "+2", [0.0, 0.0, 50.0 + delta, 100.0, 100.0], "q0")]}],
+                molecules=[("A", pos)])
+
+
+def test_desorbed_molecule_ignores_crossings_at_its_start():
+    """checksurfaces1mol with crossminimum 0 (smolsurface.c:5041,5087): the
+    surface check of a desorbed molecule skips crossings within VERYCLOSE of
+    its start, so a desorbing molecule passes the plane Q that lies a few ulps
+    above the panel it leaves.  Counting the crossing instead reflects it back
+    onto P (about a hundred times more molecules stay bound after 200 steps,
+    which the oracle run with the filter removed shows); the device must match
+    the oracle, filter included, bit for bit."""
+    spec = _desorb_offset_spec()
+    _run_pair(spec, 200, check_every=20)
+
+
+def test_surface_outcome_not_run_halts_and_reports():
+    """A rate-based crossing that changes A into B, which lives in another
+    molecule list, is an outcome the device does not run: the step reports
+    the surface error and stepping stops (the step counter stays at the
+    failing step) instead of continuing with a half-processed molecule."""
+    import smoldyn_amd as sa
+    rng = np.random.default_rng(71)
+    pos = np.column_stack([rng.uniform(5, 45, (500, 2)), rng.uniform(25.001, 25.02, 500)])
+    spec = dict(low=[0, 0, 0], high=[50, 50, 50], walls="rrr", dt=0.01, seed=71,
+                species={"A": 1.0, "B": 1.0},
+                lists={"la": ["A"], "lb": ["B"]},
+                surfaces=[{"name": "m", "actions": [("both", "B", "reflect")],
+                           "rates": [("A", "soln", "fsoln", "bsoln", 1.0, True, "B"),
+                                     ("A", "soln", "bsoln", "fsoln", 1.0, True, "B")],
+                           "panels": [("rect", "+2", [0.0, 0.0, 25.0, 50.0, 50.0], "m0")]}],
+                molecules=[("A", pos)])
+    dsim = ss.build_device(spec)
+    with pytest.raises(sa.SmoldynError) as e:
+        dsim.run_steps(50)
+    assert "surface outcome not run on the device" in str(e.value)
+    d = dsim.diagnostics()
+    assert d["err"] & 16 and d["step"] < 50
+    with pytest.raises(sa.SmoldynError):
+        dsim.run_steps(5)
+    assert dsim.diagnostics()["step"] == d["step"]
