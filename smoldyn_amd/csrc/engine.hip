@@ -402,10 +402,11 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
         }
       } else if (kSurf && sb && I.sf.listbound[ll]) {
         // checksurfacebound (smolsurface.c:5184-5199): desorption
-        int i2 = i;
+        int i2 = i, rb = 0;
         obox = pos2box(g, pos[0], pos[1], pos[2]);
-        const int st = srf_bound<(kSurf > 1)>(I.sf, g, tab->difc, tab->difstep, pos, &i2, sb, ser, step, I.key0, I.key1, &rbox);
-        if (st == kSrfDesorbed) {
+        const int st = srf_bound<(kSurf > 1)>(I.sf, g, tab->difc, tab->difstep, pos, &i2, sb, ser, step, I.key0, I.key1,
+                                              &rbox, &rb);
+        if (st == kSrfDesorbed && (!rb || I.sf.blist[i2][bstate(rb)] >= 0)) {
           atomicSub((sb & 3) == 1 ? &ctl->countf[i] : &ctl->countk[i], 1);
           if (i2 != i) {
             a.ident[r] = i2;
@@ -413,9 +414,12 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
             atomicAdd(&ctl->count[i2], 1);
             i = i2;
           }
-          sb = 0;
-          a.sb[r] = 0;
-          newll = tab->listof[i];
+          // the resurrected molecule joins the list of its state: solution,
+          // or the bound state its surface check adsorbed it to
+          sb = rb;
+          a.sb[r] = rb;
+          newll = rb ? I.sf.blist[i][bstate(rb)] : tab->listof[i];
+          if (rb) atomicAdd((rb & 3) == 1 ? &ctl->countf[i] : &ctl->countk[i], 1);
           mkind = kMoveDesorb;
         } else if (st != kSrfOk) {
           atomicOr(&ctl->err, kErrSurface);

@@ -595,11 +595,13 @@ __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, c
 // to, checked against the surfaces from the panel (pass DESORBED), and
 // becomes a solution-phase molecule: returns kSrfDesorbed with pos and
 // ident updated, and *boxp = the box the reference files it in (that of the
-// position before the surface check).  bound = (panel << 3) | state.
+// position before the surface check).  bound = (panel << 3) | state.  The
+// check may adsorb the new molecule again (dosurfinteract's adsorb branch for
+// the resurrected molecule): then *rebound = its (panel << 3) | state, else 0.
 template <bool kCv>
 __device__ __forceinline__ int srf_bound(const DevSurf& sf, const Geometry& g, const double* difc,
                                          const double* difstep, double* pos, int* ident, int bound, uint64_t serial,
-                                         uint32_t step, uint32_t key0, uint32_t key1, int* boxp) {
+                                         uint32_t step, uint32_t key0, uint32_t key1, int* boxp, int* rebound) {
   const int p = bound >> 3, ms = bound & 3;
   const DevPanel& pn = sf.pnl[p];
   const int i = *ident;
@@ -629,10 +631,11 @@ __device__ __forceinline__ int srf_bound(const DevSurf& sf, const Geometry& g, c
   // the reference files the new molecule in the box of this position, before
   // its surface check (smolsurface.c:5036)
   *boxp = pos2box(g, pos[0], pos[1], pos[2]);
-  int i3 = i2;
-  const int st = srf_check<kCv>(sf, g, difc, pos, crsspt, &i3, serial, step, SMOLRNG_SURF_DESORBED, key0, key1, nullptr);
-  if (st != kSrfOk) return kSrfUnsupported;
+  int i3 = i2, bnd = 0;
+  const int st = srf_check<kCv>(sf, g, difc, pos, crsspt, &i3, serial, step, SMOLRNG_SURF_DESORBED, key0, key1, &bnd);
+  if (st != kSrfOk && st != kSrfAdsorbed) return kSrfUnsupported;
   *ident = i3;
+  *rebound = st == kSrfAdsorbed ? bnd : 0;
   return kSrfDesorbed;
 }
 

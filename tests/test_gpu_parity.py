@@ -426,15 +426,15 @@ def test_desorbed_molecule_ignores_crossings_at_its_start():
 
 def test_surface_outcome_not_run_halts_and_reports():
     """A rate-based crossing that changes A into B, which lives in another
-    molecule list, is an outcome the device does not run: the step reports
+    molecule list (B does not diffuse, so it sits in the default list of
+    immobile states), is an outcome the device does not run: the step reports
     the surface error and stepping stops (the step counter stays at the
     failing step) instead of continuing with a half-processed molecule."""
     import smoldyn_amd as sa
     rng = np.random.default_rng(71)
     pos = np.column_stack([rng.uniform(5, 45, (500, 2)), rng.uniform(25.001, 25.02, 500)])
     spec = dict(low=[0, 0, 0], high=[50, 50, 50], walls="rrr", dt=0.01, seed=71,
-                species={"A": 1.0, "B": 1.0},
-                lists={"la": ["A"], "lb": ["B"]},
+                species={"A": 1.0, "B": 0.0},
                 surfaces=[{"name": "m", "actions": [("both", "B", "reflect")],
                            "rates": [("A", "soln", "fsoln", "bsoln", 1.0, True, "B"),
                                      ("A", "soln", "bsoln", "fsoln", 1.0, True, "B")],
