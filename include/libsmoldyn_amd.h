@@ -181,6 +181,16 @@ enum ErrorCode smolamdAddMoleculesAt(simptr sim, const char* species, int number
    order (lists in order, live order within a list).  Like swapping
    sim->diffusefn in the reference (smolsim.cpp:325). */
 enum ErrorCode smolamdInjectDisplacements(simptr sim, const double* gauss, long nmol);
+/* smolamdInjectDisplacements that allocates no device memory.  No counterpart
+   in the reference: it is for callers that cannot allocate (code under HIP
+   graph capture, serving loops, harnesses that forbid hipMalloc).  The
+   deviates are staged in a fixed device array of 262144 * 3 doubles (6 MiB,
+   one per GPU, held until the process ends) instead of the buffer the other
+   entry point allocates on first use.  The simulation must already be set up
+   by smolUpdateSim (ECerror otherwise: setting it up allocates), and nmol may
+   not exceed 262144 diffusing molecules (ECbounds).  Calls that overlap on one
+   GPU share the array and race. */
+enum ErrorCode smolamdInjectDisplacementsNoAlloc(simptr sim, const double* gauss, long nmol);
 /* Record precision of the device path, before smolUpdateSim: 64 (FP64
    positions, bit-exact with the reference's arithmetic) or 32 (each position
    stored as FP32 offsets from the FP64 origin of its virtual box; diffusion,

@@ -110,6 +110,7 @@ def lib():
         "smolamdCommandTimeline": (I, [cp, I, dp, ip, C.POINTER(C.c_longlong)]),
         "smolamdAddMoleculesAt": (I, [P, cp, I, dp]),
         "smolamdInjectDisplacements": (I, [P, dp, L64]),
+        "smolamdInjectDisplacementsNoAlloc": (I, [P, dp, L64]),
         "smolamdSetPrecision": (I, [P, I]),
         "smolamdGetMolecules": (I, [P, I, ip, dp, C.POINTER(C.c_ulonglong), ip, I]),
         "smolamdGetListCount": (I, [P]),
@@ -373,6 +374,12 @@ class Simulation:
     def inject_step(self, g):
         g = np.ascontiguousarray(g, np.float64).reshape(-1, 3)
         return _check(self.L.smolamdInjectDisplacements(self.p, _dp(g), len(g)), "smolamdInjectDisplacements")
+
+    def inject_step_noalloc(self, g, nmol=None):
+        """inject_step through the entry point that allocates no device memory."""
+        g = np.ascontiguousarray(g, np.float64).reshape(-1, 3)
+        n = len(g) if nmol is None else nmol
+        return _check(self.L.smolamdInjectDisplacementsNoAlloc(self.p, _dp(g), n), "smolamdInjectDisplacementsNoAlloc")
 
     def run_steps(self, n, graph=True):
         ms = C.c_double(0)

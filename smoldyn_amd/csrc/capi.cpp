@@ -2085,6 +2085,33 @@ enum ErrorCode smolamdInjectDisplacements(simptr sim, const double* gauss, long 
   return ECok;
 }
 
+// smolamdInjectDisplacements without device allocation (no counterpart in
+// the reference; for callers that cannot allocate device memory): the
+// simulation must already be set up by smolUpdateSim (its device state exists)
+// and at most engine_noalloc_inject_max() molecules may diffuse.
+enum ErrorCode smolamdInjectDisplacementsNoAlloc(simptr sim, const double* gauss, long nmol) {
+  const char* fn = "smolamdInjectDisplacementsNoAlloc";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(gauss, fn, ECmissing, "missing deviates");
+  LCHECK(sim->ok && sim->dev, fn, ECerror,
+         "the simulation is not set up: call smolUpdateSim first (this entry point allocates nothing)");
+  LCHECK(nmol <= engine_noalloc_inject_max(), fn, ECbounds,
+         "%ld molecules exceed the allocation-free limit of %ld diffusing molecules", nmol,
+         engine_noalloc_inject_max());
+  sim->started = true;
+  const int bits = engine_step_noalloc(sim->dev, gauss, nmol);
+  if (bits == -4) {
+    set_error(fn, ECbounds, "more diffusing molecules than the allocation-free limit of %ld",
+              engine_noalloc_inject_max());
+    return ECbounds;
+  }
+  if (bits) return engine_error(fn, bits);
+  sim->time += sim->dt;
+  sim->stepindex++;
+  if (sim->time >= sim->tmax) set_error(fn, ECnotify, "Simulation complete");
+  return ECok;
+}
+
 int smolamdGetListCount(simptr sim) {
   if (!sim) return (int)ECmissing;
   if (do_update(sim, "smolamdGetListCount") != ECok) return (int)ECerror;

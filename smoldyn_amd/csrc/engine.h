@@ -263,6 +263,12 @@ int engine_set_counters(DeviceState* s, uint64_t serno_next, uint32_t step, cons
 // One reference time step (simulatetimestep).  inject: optional host array of
 // 3 Gaussian deviates per diffusing molecule in diffuse order (NULL = Philox).
 int engine_step(DeviceState* s, const double* inject, long ninject);
+// The same injected step with the deviates staged in a fixed device array
+// instead of an allocated buffer: allocates no device memory.  Returns the
+// error bits, -2 when ninject is short, -4 when ninject exceeds
+// engine_noalloc_inject_max() molecules.
+int engine_step_noalloc(DeviceState* s, const double* inject, long ninject);
+long engine_noalloc_inject_max();
 // Many steps; returns 0 or an error bit set.
 int engine_run(DeviceState* s, long nsteps, int use_graph);
 // Steps with HIP events between the kernels; out[0..5] = average ms per step of

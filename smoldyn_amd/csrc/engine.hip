@@ -2499,6 +2499,55 @@ static int read_errors(DeviceState* s) {
   return e;
 }
 
+// ---------------------------------------------------------------------------
+// Injected steps that allocate nothing: the deviates are staged in a
+// file-scope device array (one per GPU, held until the process ends) instead
+// of the engine's injection buffer, which engine_step allocates on first use.
+// Calls that overlap on one GPU share the array and race.
+constexpr long kNoAllocInjectMax = 3L * 262144;  // doubles: 262144 diffusing molecules, 6 MiB
+static __device__ __align__(256) double scratch[kNoAllocInjectMax];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (hipGetSymbolAddress
+// once per device; the host-side name is not a device address).
+static double* scratch_ptr() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+long engine_noalloc_inject_max() { return kNoAllocInjectMax / 3; }
+
+int engine_step_noalloc(DeviceState* s, const double* inject, long ninject) {
+  if (!inject || s->multi) return -1;
+  if (ninject * 3 > kNoAllocInjectMax) return -4;
+  double* ptr = scratch_ptr();
+  if (!ptr) return -1;
+  Control c0;
+  HIPOK(hipMemcpy(&c0, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
+  long need = 0;
+  for (int ll = 0; ll < s->hinst[0].L.nlist; ll++)
+    if (s->htab[0].diffuselist[ll]) need += c0.nl[ll];
+  if (ninject < need) return -2;
+  if (need > 0) HIPOK(hipMemcpyAsync(ptr, inject, sizeof(double) * 3 * need, hipMemcpyHostToDevice, s->stream));
+  launch_step(s, s->cur, ptr, 0);
+  HIPOK(hipGetLastError());
+  HIPOK(hipStreamSynchronize(s->stream));
+  s->cur ^= 1;
+  // error bits of the one simulation, read without host containers
+  Control c1;
+  Glob g;
+  HIPOK(hipMemcpy(&c1, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(&g, s->glob, sizeof(Glob), hipMemcpyDeviceToHost));
+  return c1.err | (g.halt ? kErrHalted : 0);
+}
+
 int engine_step(DeviceState* s, const double* inject, long ninject) {
   const double* dinj = nullptr;
   if (inject) {
