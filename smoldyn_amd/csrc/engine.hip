@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <vector>
@@ -764,15 +765,48 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf,
 // two contiguous ranges (two when z wraps).  Half stencil (same list): columns
 // (-1,-1),(-1,0),(-1,1),(0,-1) with dz -1..1 and column (0,0) with dz -1..0;
 // full stencil (two lists): all nine columns, dz -1..1.  dz = 0 in column
-// (0,0) is the molecule's own box, bireact's same-box pass.  Records that
-// pass the FP32 screen are handed to k_exact, which runs the reference's FP64
-// test and keys the edge; the screen itself stays small so many waves hide
-// the latency of its loads.
+// (0,0) is the molecule's own box, bireact's same-box pass.  Neighbor
+// columns and z sub-bins whose face lies farther than the screening radius
+// from the molecule are skipped (box_faces): most molecules scan only their
+// own column's sub-bin and one neighbor sub-bin.  Records that pass the FP32
+// screen are handed to k_exact, which runs the reference's FP64 test and
+// keys the edge; the screen itself stays small so many waves hide the
+// latency of its loads.
+// Distances of a record's FP32 screen position to the faces of its virtual
+// box (x, y) and of its z sub-bin (lower l*, upper u*).  A neighbor column or
+// sub-bin lies beyond one of these faces (records are binned by pos2box,
+// which clamps only at the grid's ends, where the records beyond still lie
+// beyond the face), so when that distance exceeds the screening radius it
+// holds no record within the radius and is not scanned.  The radius already
+// carries the FP32 margin, which covers the rounding of these distances.
+struct Faces {
+  float lx, ux, ly, uy, lz, uz;
+};
+
+__device__ __forceinline__ Faces box_faces(const Geometry& g, int bx, int by, int bz, int zs, int zsub,
+                                           const float4& me) {
+  const double xlo = g.bmin[0] + (double)bx * g.bsize[0];
+  const double ylo = g.bmin[1] + (double)by * g.bsize[1];
+  const double h = g.bsize[2] / (double)zsub;
+  const double zlo = g.bmin[2] + (double)bz * g.bsize[2] + (double)zs * h;
+  Faces f;
+  f.lx = me.x - (float)xlo;
+  f.ux = (float)(xlo + g.bsize[0]) - me.x;
+  f.ly = me.y - (float)ylo;
+  f.uy = (float)(ylo + g.bsize[1]) - me.y;
+  f.lz = me.z - (float)zlo;
+  f.uz = (float)(zlo + h) - me.z;
+  return f;
+}
+
+// true when neighbor column (dx, dy) is out of reach
+__device__ __forceinline__ bool col_culled(const Faces& f, int dx, int dy, float rf) {
+  return (dx < 0 && f.lx > rf) || (dx > 0 && f.ux > rf) || (dy < 0 && f.ly > rf) || (dy > 0 && f.uy > rf);
+}
+
 template <bool kMulti, int kSurf>
-__global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, Glob* __restrict__ gl, Recs b,
-                                                  Binned bn, int nscan, int4* __restrict__ cand, int candcap) {
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gl->halt || u >= bn.bstart[nscan - 1]) return;  // records binned
+__device__ __forceinline__ void screen_global(const Inst* __restrict__ insts, Glob* __restrict__ gl, const Recs& b,
+                                              const Binned& bn, int u, int4* __restrict__ cand, int candcap) {
   const Inst& I = insts[kMulti ? b.inst[u] : 0];
   const Geometry& g = I.g;
   const DevTables* tab = I.tab;
@@ -795,6 +829,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
   const int bx = bb / (g.side[1] * sz);
   const int by = (bb / sz) % g.side[1];
   const int bz = bb % sz;
+  const float rf = sqrtf(r2f) * 1.000001f;
+  const Faces fc = box_faces(g, bx, by, bz, zs1, zsub, me);
   for (int ll2 = ll1; ll2 < nl; ll2++) {
     if (!tab->rxnmol2[ll1 * kMaxList + ll2]) continue;
     const bool half = (ll1 == ll2);
@@ -802,6 +838,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
     for (int c = 0; c < ncol; c++) {
       int dx = half ? (c < 3 ? -1 : 0) : c / 3 - 1;
       int dy = half ? (c < 3 ? c - 1 : c - 4) : c % 3 - 1;
+      if (col_culled(fc, dx, dy, rf)) continue;
       int nx = bx + dx, ny = by + dy, cw = 0;
       float shx = 0.f, shy = 0.f;
       if (nx < 0) {
@@ -831,7 +868,10 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
       // sub-bins; column (0,0) of the half stencil stops at its own box
       const int nzs = sz * zsub;
       const int gz1 = bz * zsub + zs1;
-      const int zlo = gz1 - 1, zhi = (half && c == 4) ? min(gz1 + 1, (bz + 1) * zsub - 1) : gz1 + 1;
+      // sub-bins below and above only when their face is within reach
+      const int zlo = fc.lz > rf ? gz1 : gz1 - 1;
+      const int zup = fc.uz > rf ? gz1 : gz1 + 1;
+      const int zhi = (half && c == 4) ? min(zup, (bz + 1) * zsub - 1) : zup;
       for (int seg = 0; seg < 3; seg++) {
         int za, zb2, zw = 0, dzfix = -2;
         float shz = 0.f;
@@ -888,6 +928,14 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts
       }
     }
   }
+}
+
+template <bool kMulti, int kSurf>
+__global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, Glob* __restrict__ gl, Recs b,
+                                                  Binned bn, int nscan, int4* __restrict__ cand, int candcap) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gl->halt || u >= bn.bstart[nscan - 1]) return;  // records binned
+  screen_global<kMulti, kSurf>(insts, gl, b, bn, u, cand, candcap);
 }
 
 // Exact test of the screened candidates (one thread per candidate, grid-
