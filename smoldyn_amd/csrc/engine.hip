@@ -16,9 +16,9 @@
 //               or checksurfaces, pos2box (smolboxes.c:67-79), unireact coins
 //               (smolreact.c:2997-3054), histogram of (list, box, z sub-bin);
 //               one thread per record of the source buffer
-//   k_scan1, k_scan23  exclusive scan of the histogram (box lists of
+//   k_hsum, k_hscan  exclusive scan of the histogram (box lists of
 //               reassignmolecs, each box's records ordered by z sub-bin);
-//               k_scan2 + k_scan3 instead of k_scan23 for large scans
+//               k_scan2 scans the tile sums in between for large scans
 //   k_scatter   counting-sort scatter into the destination buffer + inverse map
 //               + packed FP32 payload for the pair search
 //   k_pairs     bireact candidates (smolreact.c:3201-3296): same box and half
@@ -85,9 +85,8 @@ struct Recs {
 };
 
 struct Binned {
-  int* count;   // [nlist*nbox*zsub+1] per simulation, zeroed by k_scan1 after reading
-  int* part;    // block-local exclusive prefix
-  int* boff;    // block sums of the 4096-entry tiles
+  int* count;   // [nlist*nbox*zsub+1] per simulation, zeroed by k_hscan after reading
+  int* boff;    // sums of the 4096-key tiles
   int* bstart;  // first destination record of (list, box, z sub-bin); bstart[nscan-1] = records binned
   float4* spk;  // FP32 position + species bits of the destination records
 };
@@ -512,38 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
 }
 
 // ---------------------------------------------------------------------------
-// histogram scan: part[i] = exclusive prefix within a 4096-entry tile
-
-__global__ __launch_bounds__(kScanBlock) void k_scan1(int* __restrict__ count, int* __restrict__ part,
-                                                      int* __restrict__ bsum, int n) {
-  __shared__ int s[kScanBlock];
-  int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-  int v[kScanPer];
-  int sum = 0;
-#pragma unroll
-  for (int k = 0; k < kScanPer; k++) {
-    int i = base + k;
-    v[k] = (i < n) ? count[i] : 0;
-    if (i < n) count[i] = 0;
-    sum += v[k];
-  }
-  s[threadIdx.x] = sum;
-  __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    int x = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0;
-    __syncthreads();
-    s[threadIdx.x] += x;
-    __syncthreads();
-  }
-  int run = s[threadIdx.x] - sum;
-#pragma unroll
-  for (int k = 0; k < kScanPer; k++) {
-    int i = base + k;
-    if (i < n) part[i] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == kScanBlock - 1) bsum[blockIdx.x] = s[threadIdx.x];
-}
+// histogram scan
 
 // exclusive scan of block sums, one workgroup (up to 1024*8 tiles)
 __global__ __launch_bounds__(kScanBlock) void k_scan2(int* __restrict__ bsum, int nblk) {
@@ -574,36 +542,108 @@ __global__ __launch_bounds__(kScanBlock) void k_scan2(int* __restrict__ bsum, in
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan3(const int* __restrict__ part, const int* __restrict__ boff,
-                                                  int* __restrict__ bstart, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) bstart[i] = part[i] + boff[i / kScanTile];
+// ---------------------------------------------------------------------------
+// Histogram scan in two passes over 4096-key tiles (256 threads x 16 keys):
+// k_hsum writes each tile's sum; k_hscan adds the sums of the tiles before
+// its own (or reads them already scanned by k_scan2 when there are many
+// tiles), scans its tile with wave shuffles and writes the box starts,
+// zeroing the counts for the next step.
+constexpr int kHThreads = 256;
+constexpr int kHPer = 16;
+constexpr int kHTile = kHThreads * kHPer;  // keys per tile
+constexpr int kHFusedTiles = 1024;         // up to this many tiles k_hscan sums the tile sums itself
+
+__device__ __forceinline__ void load16(const int* __restrict__ p, int i0, int n, int* v) {
+  if (i0 + kHPer <= n) {
+    const int4* q = reinterpret_cast<const int4*>(p + i0);
+#pragma unroll
+    for (int k = 0; k < kHPer / 4; k++) {
+      const int4 w = q[k];
+      v[4 * k] = w.x;
+      v[4 * k + 1] = w.y;
+      v[4 * k + 2] = w.z;
+      v[4 * k + 3] = w.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kHPer; k++) v[k] = (i0 + k < n) ? p[i0 + k] : 0;
+  }
 }
 
-// Tiles up to which the box starts come from k_scan23 alone (each workgroup
-// sums the block sums before its tile); larger scans use k_scan2 + k_scan3.
-constexpr int kScanFusedTiles = 512;
-
-// Box starts of one 4096-entry tile per workgroup: the tile's offset is the
-// sum of the block sums of the tiles before it (reduced here, in a fixed
-// order), added to the tile-local prefix.  Replaces a separate scan of the
-// block sums.
-__global__ __launch_bounds__(kScanBlock) void k_scan23(const int* __restrict__ part, const int* __restrict__ bsum,
-                                                       int* __restrict__ bstart, int n) {
-  __shared__ int s_red[kScanBlock / 64];
-  const int t = blockIdx.x;
-  int sum = 0;
-  for (int i = threadIdx.x; i < t; i += blockDim.x) sum += bsum[i];
-  for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o);
-  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = sum;
+// block-wide sum of one value per thread (kHThreads threads)
+__device__ __forceinline__ int block_sum(int x, int* s_w) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = x;
   __syncthreads();
-  int off = 0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); w++) off += s_red[w];
-  const int base = t * kScanTile + threadIdx.x * kScanPer;
+  int t = 0;
 #pragma unroll
-  for (int k = 0; k < kScanPer; k++) {
-    const int i = base + k;
-    if (i < n) bstart[i] = part[i] + off;
+  for (int w = 0; w < kHThreads / 64; w++) t += s_w[w];
+  return t;
+}
+
+__global__ __launch_bounds__(kHThreads) void k_hsum(const int* __restrict__ count, int* __restrict__ bsum, int n) {
+  __shared__ int s_w[kHThreads / 64];
+  int v[kHPer];
+  load16(count, blockIdx.x * kHTile + threadIdx.x * kHPer, n, v);
+  int t = 0;
+#pragma unroll
+  for (int k = 0; k < kHPer; k++) t += v[k];
+  t = block_sum(t, s_w);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = t;
+}
+
+// scanned != 0: bsum holds each tile's exclusive offset (k_scan2)
+__global__ __launch_bounds__(kHThreads) void k_hscan(int* __restrict__ count, const int* __restrict__ bsum,
+                                                     int* __restrict__ bstart, int n, int scanned) {
+  __shared__ int s_w[kHThreads / 64];
+  __shared__ int s_x[kHThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i0 = blockIdx.x * kHTile + tid * kHPer;
+  int v[kHPer];
+  load16(count, i0, n, v);
+  // the tile's offset
+  int off;
+  if (scanned) {
+    off = bsum[blockIdx.x];
+  } else {
+    int part = 0;
+    for (int k = tid; k < (int)blockIdx.x; k += kHThreads) part += bsum[k];
+    off = block_sum(part, s_w);
+  }
+  // thread-local exclusive prefix, then the thread's offset in the tile
+  int t = 0;
+#pragma unroll
+  for (int k = 0; k < kHPer; k++) {
+    const int c = v[k];
+    v[k] = t;
+    t += c;
+  }
+  int inc = t;  // inclusive scan of the thread totals within the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) s_x[wv] = inc;
+  __syncthreads();
+  int wbase = 0;
+  for (int w = 0; w < wv; w++) wbase += s_x[w];
+  const int base = off + wbase + inc - t;
+  if (i0 + kHPer <= n) {
+    int4* q = reinterpret_cast<int4*>(bstart + i0);
+    int4* z = reinterpret_cast<int4*>(count + i0);
+#pragma unroll
+    for (int k = 0; k < kHPer / 4; k++) {
+      q[k] = make_int4(base + v[4 * k], base + v[4 * k + 1], base + v[4 * k + 2], base + v[4 * k + 3]);
+      z[k] = make_int4(0, 0, 0, 0);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kHPer; k++)
+      if (i0 + k < n) {
+        bstart[i0 + k] = base + v[k];
+        count[i0 + k] = 0;
+      }
   }
 }
 
@@ -2263,7 +2303,7 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
   ok |= alloc_recs(&s->buf[0], rc, s->multi, s->f32, surf && surf->on) |
         alloc_recs(&s->buf[1], rc, s->multi, s->f32, surf && surf->on);
   ok |= dalloc(&s->inv, (size_t)s->captot);
-  ok |= dalloc(&s->bn.count, s->nscan) | dalloc(&s->bn.part, s->nscan) | dalloc(&s->bn.boff, s->nscanblk) |
+  ok |= dalloc(&s->bn.count, s->nscan) | dalloc(&s->bn.boff, s->nscanblk) |
         dalloc(&s->bn.bstart, s->nscan) | dalloc(&s->bn.spk, rc);
   ok |= dalloc(&s->bestkey, rc) | dalloc(&s->killed, rc);
   ok |= dalloc(&s->edges, ned) | dalloc(&s->estate, ned) | dalloc(&s->accepted, ned);
@@ -2364,7 +2404,7 @@ void engine_destroy(DeviceState* s) {
   if (!s) return;
   free_recs(&s->buf[0]);
   free_recs(&s->buf[1]);
-  void* ptrs[] = {s->inv,     s->bn.count, s->bn.part, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges,  s->cand,  s->estate,
+  void* ptrs[] = {s->inv,     s->bn.count, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges,  s->cand,  s->estate,
                   s->accepted, s->bestkey, s->killed,  s->uniev,   s->log,       s->sortkey, s->sortval, s->killslot,
                   s->ctl,     s->tab,     s->gtab,    s->inj,     s->prods,     s->d_pnl,   s->d_bpstart,
                   s->d_bplist, s->d_act,  s->d_actb,  s->moves,   s->dinst,   s->glob,    s->obs};
@@ -2508,15 +2548,13 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   hipLaunchKernelGGL(kdiff, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, s->bn.count, s->gtab,
                      s->ngauss - 1, dinj, s->inv);
   if (timed) (void)hipEventRecord(s->ev[1], st);
-  hipLaunchKernelGGL(k_scan1, dim3(s->nscanblk), dim3(kScanBlock), 0, st, s->bn.count, s->bn.part, s->bn.boff,
-                     s->nscan);
-  if (s->nscanblk <= kScanFusedTiles) {
-    hipLaunchKernelGGL(k_scan23, dim3(s->nscanblk), dim3(kScanBlock), 0, st, s->bn.part, s->bn.boff, s->bn.bstart,
-                       s->nscan);
-  } else {
-    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kScanBlock), 0, st, s->bn.boff, s->nscanblk);
-    hipLaunchKernelGGL(k_scan3, dim3((s->nscan + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s->bn.part, s->bn.boff,
-                       s->bn.bstart, s->nscan);
+  {
+    const int ntile = (s->nscan + kHTile - 1) / kHTile;
+    hipLaunchKernelGGL(k_hsum, dim3(ntile), dim3(kHThreads), 0, st, s->bn.count, s->bn.boff, s->nscan);
+    const int scanned = ntile > kHFusedTiles;
+    if (scanned) hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kScanBlock), 0, st, s->bn.boff, ntile);
+    hipLaunchKernelGGL(k_hscan, dim3(ntile), dim3(kHThreads), 0, st, s->bn.count, s->bn.boff, s->bn.bstart, s->nscan,
+                       scanned);
   }
   if (timed) (void)hipEventRecord(s->ev[2], st);
   hipLaunchKernelGGL(kscat, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, B, s->bn, s->inv);
