@@ -195,6 +195,7 @@ struct DeviceState {
   int hnrec = 0;                     // records uploaded so far
   int4* cand = nullptr;              // screened candidate pairs (k_pairs -> k_exact)
   int candcap = 0;
+  bool half1 = false;                // every simulation reacts within one list only (half stencil)
   hipStream_t stream = nullptr;
   hipGraphExec_t graph[2] = {};
   int graph_ready[2] = {0, 0};
@@ -799,19 +800,6 @@ __device__ __forceinline__ void exact_pair(const Geometry& g, const DevSurf& sf,
   if (r >= 0) emit_edge(ctl, edges, maxedges, key, t1, t2, r, wrap);
 }
 
-// Pair search, one thread per binned record (box order), molecule m1.
-// The stencil is walked as z-runs: in box order the boxes (x', y', z-1..z+1)
-// of one neighbor column are consecutive records, so each column is one or
-// two contiguous ranges (two when z wraps).  Half stencil (same list): columns
-// (-1,-1),(-1,0),(-1,1),(0,-1) with dz -1..1 and column (0,0) with dz -1..0;
-// full stencil (two lists): all nine columns, dz -1..1.  dz = 0 in column
-// (0,0) is the molecule's own box, bireact's same-box pass.  Neighbor
-// columns and z sub-bins whose face lies farther than the screening radius
-// from the molecule are skipped (box_faces): most molecules scan only their
-// own column's sub-bin and one neighbor sub-bin.  Records that pass the FP32
-// screen are handed to k_exact, which runs the reference's FP64 test and
-// keys the edge; the screen itself stays small so many waves hide the
-// latency of its loads.
 // Distances of a record's FP32 screen position to the faces of its virtual
 // box (x, y) and of its z sub-bin (lower l*, upper u*).  A neighbor column or
 // sub-bin lies beyond one of these faces (records are binned by pos2box,
@@ -844,7 +832,23 @@ __device__ __forceinline__ bool col_culled(const Faces& f, int dx, int dy, float
   return (dx < 0 && f.lx > rf) || (dx > 0 && f.ux > rf) || (dy < 0 && f.ly > rf) || (dy > 0 && f.uy > rf);
 }
 
-template <bool kMulti, int kSurf>
+// Pair search, one thread per binned record (box order), molecule m1.
+// The stencil is walked as z-runs: in box order the boxes (x', y', z-1..z+1)
+// of one neighbor column are consecutive records, so each column is one or
+// two contiguous ranges (two when z wraps).  Half stencil (same list): columns
+// (-1,-1),(-1,0),(-1,1),(0,-1) with dz -1..1 and column (0,0) with dz -1..0;
+// full stencil (two lists): all nine columns, dz -1..1.  dz = 0 in column
+// (0,0) is the molecule's own box, bireact's same-box pass.  Neighbor
+// columns and z sub-bins whose face lies farther than the screening radius
+// from the molecule are skipped (box_faces): most molecules scan only their
+// own column's sub-bin and one neighbor sub-bin.  Records that pass the FP32
+// screen are handed to k_exact, which runs the reference's FP64 test and
+// keys the edge; the screen itself stays small so many waves hide the
+// latency of its loads.
+// kHalf1: the simulation's only reacting list pair is one list with itself,
+// so every molecule that reacts is searched with the half stencil of its own
+// list (the column loop unrolls with constant offsets).
+template <bool kMulti, int kSurf, bool kHalf1>
 __device__ __forceinline__ void screen_global(const Inst* __restrict__ insts, Glob* __restrict__ gl, const Recs& b,
                                               const Binned& bn, int u, int4* __restrict__ cand, int candcap) {
   const Inst& I = insts[kMulti ? b.inst[u] : 0];
@@ -871,111 +875,119 @@ __device__ __forceinline__ void screen_global(const Inst* __restrict__ insts, Gl
   const int bz = bb % sz;
   const float rf = sqrtf(r2f) * 1.000001f;
   const Faces fc = box_faces(g, bx, by, bz, zs1, zsub, me);
-  for (int ll2 = ll1; ll2 < nl; ll2++) {
-    if (!tab->rxnmol2[ll1 * kMaxList + ll2]) continue;
-    const bool half = (ll1 == ll2);
-    const int ncol = half ? 5 : 9;
-    for (int c = 0; c < ncol; c++) {
-      int dx = half ? (c < 3 ? -1 : 0) : c / 3 - 1;
-      int dy = half ? (c < 3 ? c - 1 : c - 4) : c % 3 - 1;
-      if (col_culled(fc, dx, dy, rf)) continue;
-      int nx = bx + dx, ny = by + dy, cw = 0;
-      float shx = 0.f, shy = 0.f;
-      if (nx < 0) {
-        if (!g.wrapflag[0]) continue;
-        nx = g.side[0] - 1;
-        cw |= 1;
-        shx = -(float)g.width[0];
-      } else if (nx >= g.side[0]) {
-        if (!g.wrapflag[0]) continue;
-        nx = 0;
-        cw |= 2;
-        shx = (float)g.width[0];
+  auto column = [&](const int c, const int dx, const int dy, const int ll2, const bool half) {
+    if (col_culled(fc, dx, dy, rf)) return;
+    int nx = bx + dx, ny = by + dy, cw = 0;
+    float shx = 0.f, shy = 0.f;
+    if (nx < 0) {
+      if (!g.wrapflag[0]) return;
+      nx = g.side[0] - 1;
+      cw |= 1;
+      shx = -(float)g.width[0];
+    } else if (nx >= g.side[0]) {
+      if (!g.wrapflag[0]) return;
+      nx = 0;
+      cw |= 2;
+      shx = (float)g.width[0];
+    }
+    if (ny < 0) {
+      if (!g.wrapflag[1]) return;
+      ny = g.side[1] - 1;
+      cw |= 1 << 2;
+      shy = -(float)g.width[1];
+    } else if (ny >= g.side[1]) {
+      if (!g.wrapflag[1]) return;
+      ny = 0;
+      cw |= 2 << 2;
+      shy = (float)g.width[1];
+    }
+    const int colb = I.keybase + (ll2 * nbox + (nx * g.side[1] + ny) * sz) * zsub;
+    // z segments in sub-bins: unwrapped run [zlo, zhi] plus wrapped single
+    // sub-bins; column (0,0) of the half stencil stops at its own box
+    const int nzs = sz * zsub;
+    const int gz1 = bz * zsub + zs1;
+    // sub-bins below and above only when their face is within reach
+    const int zlo = fc.lz > rf ? gz1 : gz1 - 1;
+    const int zup = fc.uz > rf ? gz1 : gz1 + 1;
+    const int zhi = (half && c == 4) ? min(zup, (bz + 1) * zsub - 1) : zup;
+    for (int seg = 0; seg < 3; seg++) {
+      int za, zb2, zw = 0, dzfix = -2;
+      float shz = 0.f;
+      if (seg == 0) {  // wrapped box below
+        if (zlo >= 0) continue;
+        if (!g.wrapflag[2]) continue;
+        za = zb2 = nzs - 1;
+        zw = 1 << 4;
+        shz = -(float)g.width[2];
+        dzfix = -1;
+      } else if (seg == 1) {  // unwrapped run
+        za = zlo < 0 ? 0 : zlo;
+        zb2 = zhi >= nzs ? nzs - 1 : zhi;
+        if (za > zb2) continue;
+      } else {  // wrapped box above
+        if (zhi < nzs) continue;
+        if (!g.wrapflag[2]) continue;
+        za = zb2 = 0;
+        zw = 2 << 4;
+        shz = (float)g.width[2];
+        dzfix = 1;
       }
-      if (ny < 0) {
-        if (!g.wrapflag[1]) continue;
-        ny = g.side[1] - 1;
-        cw |= 1 << 2;
-        shy = -(float)g.width[1];
-      } else if (ny >= g.side[1]) {
-        if (!g.wrapflag[1]) continue;
-        ny = 0;
-        cw |= 2 << 2;
-        shy = (float)g.width[1];
-      }
-      const int colb = I.keybase + (ll2 * nbox + (nx * g.side[1] + ny) * sz) * zsub;
-      // z segments in sub-bins: unwrapped run [zlo, zhi] plus wrapped single
-      // sub-bins; column (0,0) of the half stencil stops at its own box
-      const int nzs = sz * zsub;
-      const int gz1 = bz * zsub + zs1;
-      // sub-bins below and above only when their face is within reach
-      const int zlo = fc.lz > rf ? gz1 : gz1 - 1;
-      const int zup = fc.uz > rf ? gz1 : gz1 + 1;
-      const int zhi = (half && c == 4) ? min(zup, (bz + 1) * zsub - 1) : zup;
-      for (int seg = 0; seg < 3; seg++) {
-        int za, zb2, zw = 0, dzfix = -2;
-        float shz = 0.f;
-        if (seg == 0) {  // wrapped box below
-          if (zlo >= 0) continue;
-          if (!g.wrapflag[2]) continue;
-          za = zb2 = nzs - 1;
-          zw = 1 << 4;
-          shz = -(float)g.width[2];
-          dzfix = -1;
-        } else if (seg == 1) {  // unwrapped run
-          za = zlo < 0 ? 0 : zlo;
-          zb2 = zhi >= nzs ? nzs - 1 : zhi;
-          if (za > zb2) continue;
-        } else {  // wrapped box above
-          if (zhi < nzs) continue;
-          if (!g.wrapflag[2]) continue;
-          za = zb2 = 0;
-          zw = 2 << 4;
-          shz = (float)g.width[2];
-          dzfix = 1;
-        }
-        const int k0 = bn.bstart[colb + za];
-        const int k1 = bn.bstart[colb + zb2 + 1];
-        const int wrap = cw | zw;
-        const float ox = me.x - shx, oy = me.y - shy, oz = me.z - shz;
-        for (int kb = k0; kb < k1; kb += 32) {
-          // screening: bit j of hits <=> record kb+j is a candidate partner
-          const int kn = min(k1 - kb, 32);
-          uint32_t hits = 0;
+      const int k0 = bn.bstart[colb + za];
+      const int k1 = bn.bstart[colb + zb2 + 1];
+      const int wrap = cw | zw;
+      const float ox = me.x - shx, oy = me.y - shy, oz = me.z - shz;
+      for (int kb = k0; kb < k1; kb += 32) {
+        // screening: bit j of hits <=> record kb+j is a candidate partner
+        const int kn = min(k1 - kb, 32);
+        uint32_t hits = 0;
 #pragma unroll 4
-          for (int j = 0; j < kn; j++) {
-            const float4 o = bn.spk[kb + j];
-            const float dxf = o.x - ox, dyf = o.y - oy, dzf = o.z - oz;
-            const float d2 = dxf * dxf + dyf * dyf + dzf * dzf;
-            const int w2 = __float_as_int(o.w);
-            bool hit = ((d2 <= r2f) | ((w2 >> 16) & 1)) & ((mask >> (w2 & 31)) & 1u);
-            if (kSurf) hit = hit & !((w2 >> 20) & 1);
-            hits |= (uint32_t)hit << j;
-          }
-          if (u >= kb && u < kb + 32) hits &= ~(1u << (u - kb));
-          // candidates go to k_exact: record pair and the stencil position
-          // (column offsets, fixed dz of a wrapped sub-bin, wrap code, list)
-          const int code = (dx + 1) | ((dy + 1) << 2) | ((dzfix + 2) << 4) | (wrap << 6) | (ll2 << 12) |
-                           ((int)half << 15);
-          while (hits) {
-            const int k = kb + __ffs(hits) - 1;
-            hits &= hits - 1;
-            const int ci = atomicAdd(&gl->ncand, 1);
-            if (ci < candcap) cand[ci] = make_int4(u, k, code, 0);
-            else atomicOr(&ctl->err, kErrEdgeOverflow);
-          }
+        for (int j = 0; j < kn; j++) {
+          const float4 o = bn.spk[kb + j];
+          const float dxf = o.x - ox, dyf = o.y - oy, dzf = o.z - oz;
+          const float d2 = dxf * dxf + dyf * dyf + dzf * dzf;
+          const int w2 = __float_as_int(o.w);
+          bool hit = ((d2 <= r2f) | ((w2 >> 16) & 1)) & ((mask >> (w2 & 31)) & 1u);
+          if (kSurf) hit = hit & !((w2 >> 20) & 1);
+          hits |= (uint32_t)hit << j;
+        }
+        if (u >= kb && u < kb + 32) hits &= ~(1u << (u - kb));
+        // candidates go to k_exact: record pair and the stencil position
+        // (column offsets, fixed dz of a wrapped sub-bin, wrap code, list)
+        const int code = (dx + 1) | ((dy + 1) << 2) | ((dzfix + 2) << 4) | (wrap << 6) | (ll2 << 12) |
+                         ((int)half << 15);
+        while (hits) {
+          const int k = kb + __ffs(hits) - 1;
+          hits &= hits - 1;
+          const int ci = atomicAdd(&gl->ncand, 1);
+          if (ci < candcap) cand[ci] = make_int4(u, k, code, 0);
+          else atomicOr(&ctl->err, kErrEdgeOverflow);
         }
       }
+    }
+  };
+  if (kHalf1) {  // the half stencil in the reference's column order
+    column(0, -1, -1, ll1, true);
+    column(1, -1, 0, ll1, true);
+    column(2, -1, 1, ll1, true);
+    column(3, 0, -1, ll1, true);
+    column(4, 0, 0, ll1, true);
+  } else {
+    for (int ll2 = ll1; ll2 < nl; ll2++) {
+      if (!tab->rxnmol2[ll1 * kMaxList + ll2]) continue;
+      const bool half = (ll1 == ll2);
+      const int ncol = half ? 5 : 9;
+      for (int c = 0; c < ncol; c++)
+        column(c, half ? (c < 3 ? -1 : 0) : c / 3 - 1, half ? (c < 3 ? c - 1 : c - 4) : c % 3 - 1, ll2, half);
     }
   }
 }
 
-template <bool kMulti, int kSurf>
+template <bool kMulti, int kSurf, bool kHalf1>
 __global__ __launch_bounds__(kBlock) void k_pairs(const Inst* __restrict__ insts, Glob* __restrict__ gl, Recs b,
                                                   Binned bn, int nscan, int4* __restrict__ cand, int candcap) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (gl->halt || u >= bn.bstart[nscan - 1]) return;  // records binned
-  screen_global<kMulti, kSurf>(insts, gl, b, bn, u, cand, candcap);
+  screen_global<kMulti, kSurf, kHalf1>(insts, gl, b, bn, u, cand, candcap);
 }
 
 // Exact test of the screened candidates (one thread per candidate, grid-
@@ -2285,6 +2297,19 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
       keys += cfg[k].nlist * geo[k].nbox;
     }
   }
+  // the pair search's half-stencil specialization: in every simulation the
+  // only reacting list pair is one list with itself
+  s->half1 = true;
+  for (int k = 0; k < n; k++) {
+    int ndiag = 0;
+    for (int l1 = 0; l1 < cfg[k].nlist; l1++)
+      for (int l2 = 0; l2 < cfg[k].nlist; l2++)
+        if (tab[k].rxnmol2[l1 * kMaxList + l2]) {
+          if (l1 != l2) s->half1 = false;
+          else ndiag++;
+        }
+    if (ndiag > 1) s->half1 = false;
+  }
   s->nscan = keys + 1;
   s->nscanblk = (s->nscan + kScanTile - 1) / kScanTile;
   if (s->nscanblk > kScanBlock * 8) {
@@ -2536,7 +2561,9 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   auto kscat = s->multi ? (f ? k_scatter<true, true, false> : k_scatter<true, false, false>)
                : s->surf_on ? (f ? k_scatter<false, true, true> : k_scatter<false, false, true>)
                             : (f ? k_scatter<false, true, false> : k_scatter<false, false, false>);
-  auto kpair = s->multi ? k_pairs<true, 0> : (s->surf_on ? k_pairs<false, 1> : k_pairs<false, 0>);
+  auto kpair = s->multi ? (s->half1 ? k_pairs<true, 0, true> : k_pairs<true, 0, false>)
+               : s->surf_on ? (s->half1 ? k_pairs<false, 1, true> : k_pairs<false, 1, false>)
+                            : (s->half1 ? k_pairs<false, 0, true> : k_pairs<false, 0, false>);
   auto kexact = s->multi ? (f ? k_exact<0, true, true> : k_exact<0, true, false>)
                 : sv == 2 ? (f ? k_exact<2, false, true> : k_exact<2, false, false>)
                 : sv == 1 ? (f ? k_exact<1, false, true> : k_exact<1, false, false>)
