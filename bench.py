@@ -32,19 +32,24 @@ def log(msg):
     """Progress line on stderr (the JSON result goes to stdout)."""
     print("bench: " + msg, file=sys.stderr, flush=True)
 
-# Algorithmic HBM bytes per molecule per launch of each kernel (molecule
-# records in box order, FP64 positions):
-#   k_diffuse  read pos 24 + species 4 + live position 4 + serial 8,
-#              write pos 24 + box 4 + rank 4                            = 72
-#   k_scatter  read box 4 + rank 4 + species 4 + pos 24 + serial 8 + live
-#              position 4 + box start 4; write pos 24 + species 4 + serial 8
-#              + live position 4 + box 4 + packed FP32 position 16      = 112
-#   k_pairs    read packed position 16 + live position 4 + box 4 once (the
-#              neighboring boxes' records are re-read from L2, not HBM) = 24
-ALGO_BYTES = {"diffuse": 72, "scatter": 112, "pairs": 24}
-KERNEL_NAMES = ["diffuse", "scan", "scatter", "event_overhead", "pairs", "resolve"]
-HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_scan1+k_scan23", "scatter": "k_scatter",
-                     "pairs": "k_pairs+k_exact", "resolve": "k_resolve"}
+# Algorithmic HBM bytes per molecule per launch, as the survey of the
+# reference fixes them for an FP64 structure-of-arrays step of three passes
+# (counted once per molecule, whatever the layout moves besides):
+#   k_diffuse        pass A: read pos 24 + species 2, write pos 24 + key 4  = 54
+#   k_hsum+k_hscan   cell histogram and scan, 16 B per cell / 4 per cell   =  4
+#   k_scatter        pass B: read key 4 + pos 24 + species 2 + rank 4,
+#                    write the 30 B payload reordered                      = 64
+#   k_pairs          pass C: read own 26 B + cell offsets 1, write result 4 = 31
+#   k_exact, k_resolve  work per candidate pair and per event, not per molecule
+# 153 B per molecule-step in all.  The layout's own bytes (design bytes: the
+# record also carries its serial number, live-list position and rank, and the
+# pair screen's packed FP32 copy) are reported beside them.
+ALGO_BYTES = {"diffuse": 54, "scan": 4, "scatter": 64, "pairs": 31, "exact": 0, "resolve": 0}
+DESIGN_BYTES = {"diffuse": 72, "scan": 16, "scatter": 112, "pairs": 24, "exact": 0, "resolve": 0}
+STEP_ALGO_BYTES = 153
+KERNEL_NAMES = ["diffuse", "scan", "scatter", "event_overhead", "pairs", "exact", "resolve"]
+HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_hsum+k_hscan", "scatter": "k_scatter", "pairs": "k_pairs",
+                     "exact": "k_exact", "resolve": "k_resolve"}
 
 
 def c2_spec(n_per_species=500000, side=1000.0, seed=2026, kback=1.0):
@@ -452,6 +457,44 @@ def cpu_baseline(spec, steps):
     return molsteps / dt, dt
 
 
+# The survey's own timings of the compiled reference (1 core, Xeon @2.70 GHz,
+# gcc 11.4 -O3) on two probe models, in molecule-steps/s, and the oracle's
+# throughput on the same models relative to them (bench.py --calibrate-cpu on
+# the same kind of host core): 1.06 on the C1-like model, 0.62 on the
+# reversible workload scaled to 100k molecules.
+SURVEY_REFERENCE_PROBES = {"c1_like": 0.99e6, "c2_scaled_100k": 0.52e6}
+ORACLE_OVER_REFERENCE = {"c1_like": 1.06, "c2_scaled_100k": 0.62}
+
+
+def calibrate_cpu():
+    """The CPU oracle on the survey's two probe models (the reference's
+    timings of them are SURVEY_REFERENCE_PROBES): C1-like, 10k A + 10k B in a
+    reflective cube of side 100, 1000 steps; and the reversible workload
+    scaled to 100k molecules (periodic cube of side 464.16), 100 steps.
+    Prints the oracle's molecule-steps/s and its ratio to the reference's."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import simspec as ss
+    probes = {"c1_like": (ss.c1_spec(n=10000, seed=11), 1000),
+              "c2_scaled_100k": (c2_spec(n_per_species=50000, side=464.16), 100)}
+    out = {}
+    for name, (spec, steps) in probes.items():
+        osim, _ = ss.build_oracle(dict(spec, tmax=1e30))
+        nmol = sum(len(x["ident"]) for x in ss.oracle_state(osim))
+        molsteps = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            molsteps += nmol
+            osim.step()
+            nmol = sum(osim.L.osim_list_count(osim.p, ll) for ll in range(osim.nlist()))
+        dt = time.perf_counter() - t0
+        v = molsteps / dt
+        out[name] = {"steps": steps, "seconds": dt, "oracle_molecule_steps_per_s": v,
+                     "reference_molecule_steps_per_s": SURVEY_REFERENCE_PROBES[name],
+                     "oracle_over_reference": v / SURVEY_REFERENCE_PROBES[name]}
+        log("%s: oracle %.3g molecule-steps/s over %d steps (%.1f s)" % (name, v, steps, dt))
+    print(json.dumps(out))
+
+
 def calib_copy(mib, reps=5):
     """A plain device-to-device copy of known size: reads and writes `mib` MiB
     per launch.  Profiling counters read on it give the scale factor between
@@ -473,6 +516,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--steady-steps", type=int, default=500,
+                    help="timed steps of the steady-state window, after 1000 steps from setup (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
@@ -492,7 +537,12 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=100)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--calibrate-cpu", action="store_true",
+                    help="only time the CPU oracle on the survey's probe models (ratio to the reference's timings)")
     args = ap.parse_args()
+    if args.calibrate_cpu:
+        calibrate_cpu()
+        return
     if args.calib_copy_mib > 0:
         calib_copy(args.calib_copy_mib)
         return
@@ -523,17 +573,37 @@ def main():
     molsteps = d1["molsteps"] - d0["molsteps"]
     value = molsteps / wall
     # per-kernel device time with HIP events on the engine's stream
-    kt = sim.profile_steps(args.profile_steps) if args.profile_steps > 0 else np.zeros(6)
+    kt = sim.profile_steps(args.profile_steps) if args.profile_steps > 0 else np.zeros(len(KERNEL_NAMES))
     nmol = sim.diagnostics()["nmol"]
     times = dict(zip(KERNEL_NAMES, [float(x) for x in kt]))
     # each kernel's event interval includes one event overhead; the kernel's
     # own duration is the interval minus that overhead
     net = {k: max(v - times["event_overhead"], 0.0) for k, v in times.items() if k != "event_overhead"}
-    dom = max(ALGO_BYTES, key=lambda k: net[k])
+    dom = max(net, key=lambda k: net[k])  # the longest kernel of the step
     achieved = ALGO_BYTES[dom] * nmol / (net[dom] * 1e-3) / 1e9 if net[dom] > 0 else 0.0
-    step_bytes = sum(ALGO_BYTES.values()) * nmol
+    step_bytes = STEP_ALGO_BYTES * nmol
     counts = {s: sim.count(s) for s in ("A", "B", "C")}
     tr = measured_traffic(HIP_KERNEL_SYMBOL[dom])
+    per_kernel = {k: {"ms": net[k], "algo_bytes_per_molecule": ALGO_BYTES[k],
+                      "design_bytes_per_molecule": DESIGN_BYTES[k],
+                      "algo_GBs": ALGO_BYTES[k] * nmol / (net[k] * 1e-3) / 1e9 if net[k] > 0 else 0.0,
+                      "frac": (ALGO_BYTES[k] * nmol / (net[k] * 1e-3) / 1e9 / HBM_PEAK_GBS) if net[k] > 0 else 0.0}
+                  for k in net}
+    # steady state: continue to 1000 steps after setup, then time 500 more
+    steady = None
+    if args.steady_steps > 0:
+        done = args.warmup + args.steps + args.profile_steps
+        if done < 1000:
+            sim.run_steps(1000 - done, graph=use_graph)
+            done = 1000
+        s0 = sim.diagnostics()
+        t0 = time.perf_counter()
+        sim.run_steps(args.steady_steps, graph=use_graph)
+        sw = time.perf_counter() - t0
+        s1 = sim.diagnostics()
+        steady = {"after_steps": done, "timed_steps": args.steady_steps,
+                  "value": (s1["molsteps"] - s0["molsteps"]) / sw, "ms_per_step": sw * 1e3 / args.steady_steps,
+                  "live_molecules": s1["nmol"]}
     out = {
         "metric": "molecule-steps/s at 1M molecules",
         "value": value,
@@ -556,12 +626,15 @@ def main():
         "roofline": {"bound": "hbm", "kernel": HIP_KERNEL_SYMBOL[dom], "kernel_ms": net[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "algo_bytes_per_molecule": ALGO_BYTES[dom], "molecules_per_launch": nmol,
+                     "algo_bytes_per_molecule": ALGO_BYTES[dom], "design_bytes_per_molecule": DESIGN_BYTES[dom],
+                     "molecules_per_launch": nmol,
                      "traffic": tr["bytes_per_launch"] if tr else None,
                      "traffic_note": ("rocprofv3 FETCH_SIZE+WRITE_SIZE per launch at %d molecules, scaled by a "
                                       "plain-copy calibration (%s)" % (tr["molecules"], tr["source"])) if tr else None},
+        "kernels": per_kernel,
         "step_algo_bytes": step_bytes,
         "step_frac_of_hbm": step_bytes / (dev_ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "steady_state": steady,
         "final_counts": counts,
         "setup_s": setup_s,
     }
@@ -587,10 +660,16 @@ def main():
     if not args.no_cpu:
         log("CPU oracle baseline, %d steps" % args.cpu_steps)
         cv, cdt = cpu_baseline(spec, args.cpu_steps)
+        ratio = ORACLE_OVER_REFERENCE["c2_scaled_100k"]
         out["cpu_baseline"] = {"value": cv, "unit": "molecule-steps/s", "cores": 1, "kind": "port",
                                "sample": "same 1M-molecule workload, %d steps after setup (%.1f s), CPU oracle "
-                                         "(C restatement of the reference path), 1 thread" % (args.cpu_steps, cdt)}
+                                         "(C restatement of the reference path), 1 thread" % (args.cpu_steps, cdt),
+                               "calibration": {"oracle_over_reference": ratio,
+                                               "probe": "reversible workload scaled to 100k molecules, against the "
+                                                        "compiled reference's 0.52e6 molecule-steps/s",
+                                               "reference_estimate": cv / ratio}}
         out["speedup_vs_cpu"] = value / cv
+        out["speedup_vs_reference_estimate"] = value / (cv / ratio)
     print(json.dumps(out))
 
 

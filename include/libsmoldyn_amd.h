@@ -227,9 +227,10 @@ int smolamdCommandTimeline(const char* script, int maxev, double* ev_time, int* 
    device time (HIP events). */
 enum ErrorCode smolamdRunSteps(simptr sim, long nsteps, int use_graph, double* ms);
 /* Per-kernel average device time (ms) over nsteps timed steps, from HIP
-   events on the engine's stream: out[0..5] = diffuse, box scan, scatter,
+   events on the engine's stream: out[0..6] = diffuse, box scan, scatter,
    event overhead (two back-to-back events with no kernel between them),
-   pair search, resolve.  Each kernel interval includes one event overhead. */
+   pair screen, exact pair test, resolve.  Each kernel interval includes one
+   event overhead. */
 enum ErrorCode smolamdProfileSteps(simptr sim, long nsteps, double* out);
 /* Engine diagnostics: out[0]=live molecules, [1]=edges max, [2]=matching rounds
    of last step, [3]=error bits, [4]=next serial, [5]=step index,

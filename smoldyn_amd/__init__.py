@@ -387,7 +387,9 @@ class Simulation:
         return ms.value
 
     def profile_steps(self, n):
-        out = np.zeros(6)
+        """Per-kernel ms: diffuse, scan, scatter, event overhead, pair screen,
+        exact test, resolve."""
+        out = np.zeros(7)
         _check(self.L.smolamdProfileSteps(self.p, n, _dp(out)), "smolamdProfileSteps")
         return out
 

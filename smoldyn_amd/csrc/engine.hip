@@ -2545,7 +2545,7 @@ int engine_set_counters(DeviceState* s, uint64_t serno_next, uint32_t step, cons
 
 // Launch one step from buffer src into 1-src on s->stream.  Grids cover the
 // record capacity; kernels read the exact counts from device memory.
-// timed: events between the kernels into ev[0..6].
+// timed: events between the kernels into ev[0..7].
 static void launch_step(DeviceState* s, int src, const double* dinj, int timed) {
   int nb = (s->reccap + kBlock - 1) / kBlock;
   hipStream_t st = s->stream;
@@ -2592,12 +2592,13 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   if (timed) (void)hipEventRecord(s->ev[4], st);
   hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, B, s->bn, s->nscan, s->cand,
                      s->candcap);
+  if (timed) (void)hipEventRecord(s->ev[5], st);
   hipLaunchKernelGGL(kexact, dim3(kExactBlocks), dim3(kBlock), 0, st, s->dinst, s->glob, B, s->bn, s->cand,
                      s->candcap);
-  if (timed) (void)hipEventRecord(s->ev[5], st);
+  if (timed) (void)hipEventRecord(s->ev[6], st);
   hipLaunchKernelGGL(kres, dim3(s->ninst), dim3(kResolveThreads), 0, st, s->dinst, s->glob, A, B, s->inv, s->bn,
                      s->nscan, s->bestkey, s->killed, s->reccap);
-  if (timed) (void)hipEventRecord(s->ev[6], st);
+  if (timed) (void)hipEventRecord(s->ev[7], st);
 }
 
 // OR of the error bits of all simulations
@@ -2769,20 +2770,23 @@ int engine_read_obs(DeviceState* s, int nrow, int* out) {
 }
 
 // Steps with events between the kernels; per-kernel average ms in out[]:
-// diffuse, box scan, scatter, event overhead, pairs, resolve.
+// diffuse, box scan, scatter (and the first molsort with surfaces), event
+// overhead (two events with no kernel between them), pair screen, exact
+// test, resolve.
 int engine_profile(DeviceState* s, long nsteps, double* out) {
-  double acc[6] = {0, 0, 0, 0, 0, 0};
+  constexpr int kN = 7;
+  double acc[kN] = {0, 0, 0, 0, 0, 0, 0};
   for (long k = 0; k < nsteps; k++) {
     launch_step(s, s->cur, nullptr, 1);
     HIPOK(hipStreamSynchronize(s->stream));
     s->cur ^= 1;
-    for (int j = 0; j < 6; j++) {
+    for (int j = 0; j < kN; j++) {
       float ms = 0;
       HIPOK(hipEventElapsedTime(&ms, s->ev[j], s->ev[j + 1]));
       acc[j] += ms;
     }
   }
-  for (int j = 0; j < 6; j++) out[j] = acc[j] / (double)(nsteps > 0 ? nsteps : 1);
+  for (int j = 0; j < kN; j++) out[j] = acc[j] / (double)(nsteps > 0 ? nsteps : 1);
   return read_errors(s);
 }
 
