@@ -174,6 +174,14 @@ enum ErrorCode smolSetReactionProducts(simptr sim, const char* reaction, enum Re
  * Inspection and parity hooks used by callers that compare the device engine
  * with a CPU path, and the engine's own controls. */
 
+/* The reaction_log statement (smolsim.cpp:2016-2040, RxnSetLog
+   smolreact.c:2013-2061; libsmoldyn has no call for it): log the events of
+   `reaction` ("all": every reaction) to the output file `filename` when a
+   product's serial number is in serno[0..nserno-1] (nserno < 0: every event);
+   turnon = 0 removes those serial numbers (all of them for nserno < 0).
+   Lines are written by smolRunSim, smolRunSimUntil and smolRunTimeStep. */
+enum ErrorCode smolamdSetReactionLog(simptr sim, const char* filename, const char* reaction, const long* serno,
+                                     int nserno, int turnon);
 /* Initial molecules at given positions (n*dim doubles), serial numbers in order. */
 enum ErrorCode smolamdAddMoleculesAt(simptr sim, const char* species, int number, const double* positions);
 /* Replace the diffusion draws of the next step by caller-supplied standard

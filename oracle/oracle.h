@@ -113,6 +113,7 @@ double omath_erfn(double x);
 
 int osim_set_confspread(osim* sim, int r, double radius);
 int osim_set_bindrad(osim* sim, int r, double radius);
+int osim_rxnlog_pos(osim* sim, double* pos, int maxrows);
 int osim_set_prdpos(osim* sim, int order, int r, int prd, const double* pos);
 void osim_rng_rotation(uint32_t k0, uint32_t k1, uint64_t serial, uint32_t step, double* m);
 

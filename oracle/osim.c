@@ -1817,6 +1817,14 @@ int osim_rxnlog(osim* sim, int64_t* rows, int maxrows) {
   return sim->nlog;
 }
 
+/* reaction positions of the logged events (dim values each) */
+int osim_rxnlog_pos(osim* sim, double* pos, int maxrows) {
+  int k, d;
+  for (k = 0; k < sim->nlog && k < maxrows; k++)
+    for (d = 0; d < sim->dim; d++) pos[sim->dim * k + d] = sim->log[k].pos[d];
+  return sim->nlog;
+}
+
 uint64_t osim_next_serno(osim* sim) { return sim->serno; }
 
 const double* osim_gausstable(osim* sim, int flat) { return flat ? sim->gtable_flat : sim->gtable; }

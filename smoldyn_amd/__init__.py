@@ -111,6 +111,7 @@ def lib():
         "smolamdAddMoleculesAt": (I, [P, cp, I, dp]),
         "smolamdInjectDisplacements": (I, [P, dp, L64]),
         "smolamdInjectDisplacementsNoAlloc": (I, [P, dp, L64]),
+        "smolamdSetReactionLog": (I, [P, C.c_char_p, C.c_char_p, C.POINTER(C.c_long), I, I]),
         "smolamdSetPrecision": (I, [P, I]),
         "smolamdGetMolecules": (I, [P, I, ip, dp, C.POINTER(C.c_ulonglong), ip, I]),
         "smolamdGetListCount": (I, [P]),
@@ -374,6 +375,17 @@ class Simulation:
     def inject_step(self, g):
         g = np.ascontiguousarray(g, np.float64).reshape(-1, 3)
         return _check(self.L.smolamdInjectDisplacements(self.p, _dp(g), len(g)), "smolamdInjectDisplacements")
+
+    def set_reaction_log(self, filename, reaction="all", serials=None, on=True):
+        """reaction_log statement: log the events of `reaction` whose products'
+        serial numbers are listed (None: every event) to output file `filename`."""
+        if serials is None:
+            arr, n = None, -1
+        else:
+            arr = (C.c_long * max(len(serials), 1))(*serials)
+            n = len(serials)
+        _check(self.L.smolamdSetReactionLog(self.p, filename.encode(), reaction.encode(), arr, n, 1 if on else 0),
+               "smolamdSetReactionLog")
 
     def inject_step_noalloc(self, g, nmol=None):
         """inject_step through the entry point that allocates no device memory."""

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <array>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -58,6 +59,11 @@ struct Rxn {
   double rparam = 0, unbindrad = -1;
   double prdpos[kMaxProd][3] = {{0}};
   int disable = 0;
+  // reaction log (RxnSetLog, smolreact.c:2013-2061): output file name and the
+  // serial numbers whose products are logged (logall: every event)
+  std::string logfile;
+  std::vector<long> logserno;
+  bool logall = false;
 };
 
 struct PendingMol {
@@ -916,6 +922,50 @@ ErrorCode engine_error(const char* fn, int bits) {
 }
 
 // one simulatetimestep; returns the reference's code (0, 1 at tmax, 10 at tbreak) or <0
+bool rxnlog_on(simptr sim) {
+  for (int o = 0; o < 3; o++)
+    for (auto& r : sim->rxn[o])
+      if (!r.logfile.empty()) return true;
+  return false;
+}
+
+// The reaction log lines of the step that started at `time` (doreact,
+// smolreact.c:2879-2896): for each event of a logged reaction, in creation
+// order, the line starts at the first product whose serial number (its low
+// 32 bits, or its high ones for a compound serial) is listed, printing "?"
+// for the products before it: time, reaction name, reaction position,
+// reactant serials, then the product serials.
+void write_rxnlog(simptr sim, double time) {
+  std::vector<LogRow> rows;
+  int n = engine_get_log(sim->dev, nullptr, 0);
+  if (n <= 0) return;
+  rows.resize((size_t)n);
+  engine_get_log(sim->dev, rows.data(), n);
+  auto listed = [](const Rxn& rx, uint64_t s) {
+    if (rx.logall) return true;
+    for (long v : rx.logserno)
+      if (v == (long)(s & 0xFFFFFFFFull) || (s > 0xFFFFFFFull && v == (long)(s >> 32))) return true;
+    return false;
+  };
+  for (const LogRow& row : rows) {
+    if (row.order < 0 || row.order > 2 || row.r < 0 || row.r >= (int)sim->rxn[row.order].size()) continue;
+    const Rxn& rx = sim->rxn[row.order][(size_t)row.r];
+    if (rx.logfile.empty()) continue;
+    int first = -1;
+    for (int q = 0; q < rx.nprod && q < kMaxProd; q++)
+      if (listed(rx, row.prd[q])) {
+        first = q;
+        break;
+      }
+    if (first < 0) continue;
+    FILE* fp = nullptr;
+    int data = -1;
+    if (cmd::target(sim->cmds, rx.logfile.c_str(), 1, &fp, &data) < 0 || !fp) continue;
+    cmd::write_rxnlog_line(sim->cmds, fp, time, rx.name.c_str(), row.pos, sim->dim, row.order, row.rct, first,
+                           rx.nprod, row.prd);
+  }
+}
+
 int run_one(simptr sim, const double* inject, long ninject, const char* fn, ErrorCode* ec) {
   *ec = do_update(sim, fn);
   if (*ec != ECok) return -1;
@@ -925,6 +975,7 @@ int run_one(simptr sim, const double* inject, long ninject, const char* fn, Erro
     *ec = engine_error(fn, bits);
     return -1;
   }
+  if (rxnlog_on(sim)) write_rxnlog(sim, sim->time);
   sim->time += sim->dt;
   sim->stepindex++;
   if (sim->time >= sim->tmax) return 1;
@@ -1039,6 +1090,8 @@ int simulate_device(simptr sim, const char* fn, ErrorCode* ec) {
     if (code == cmd::kWarn) fprintf(stderr, "command '%s' error: %s\n", c.str.c_str(), err.c_str());
     return code;
   };
+  const bool logging = rxnlog_on(sim);
+  double tstep = sim->time;  // start time of the next step, accumulated as the clock does
   auto step = [&]() -> int {
     if (bits) return -1;
     if (engine_enqueue(sim->dev, 1)) {
@@ -1046,6 +1099,15 @@ int simulate_device(simptr sim, const char* fn, ErrorCode* ec) {
       return -1;
     }
     sim->stepindex++;
+    if (logging) {  // the step's events are written before the commands that follow it
+      int b = flush_outputs(sim);
+      if (b) {
+        bits = b;
+        return -1;
+      }
+      write_rxnlog(sim, tstep);
+    }
+    tstep += sim->dt;
     if (++since >= 256) {
       since = 0;
       int b = flush_outputs(sim);
@@ -2008,6 +2070,40 @@ enum ErrorCode smolSetReactionRate(simptr sim, const char* reaction, double rate
 }
 
 // libsmoldyn.cpp:1990-2020 with RxnSetRevparam (smolreact.c:1854-1880)
+enum ErrorCode smolamdSetReactionLog(simptr sim, const char* filename, const char* reaction, const long* serno,
+                                     int nserno, int turnon) {
+  const char* fn = "smolamdSetReactionLog";
+  LCHECK(sim, fn, ECmissing, "missing sim");
+  LCHECK(filename && filename[0], fn, ECmissing, "missing file name");
+  LCHECK(reaction && reaction[0], fn, ECmissing, "missing reaction name");
+  LCHECK(nserno < 0 || serno, fn, ECmissing, "missing serial numbers");
+  bool found = false;
+  for (int o = 0; o < 3; o++)
+    for (auto& rx : sim->rxn[o]) {
+      if (strcmp(reaction, "all") && rx.name != reaction) continue;
+      found = true;
+      if (turnon) {
+        if (nserno < 0) {
+          rx.logall = true;
+          rx.logserno.clear();
+        } else {
+          rx.logserno.insert(rx.logserno.end(), serno, serno + nserno);
+        }
+        rx.logfile = filename;
+      } else if (nserno < 0) {
+        rx.logall = false;
+        rx.logserno.clear();
+        rx.logfile.clear();
+      } else {
+        for (int k = 0; k < nserno; k++)
+          rx.logserno.erase(std::remove(rx.logserno.begin(), rx.logserno.end(), serno[k]), rx.logserno.end());
+        if (rx.logserno.empty() && !rx.logall) rx.logfile.clear();
+      }
+    }
+  LCHECK(found, fn, ECnonexist, "reaction '%s' not recognized", reaction);
+  return ECok;
+}
+
 enum ErrorCode smolSetReactionProducts(simptr sim, const char* reaction, enum RevParam method, double parameter,
                                        const char* product, double* position) {
   const char* fn = "smolSetReactionProducts";

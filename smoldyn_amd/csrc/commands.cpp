@@ -684,5 +684,26 @@ int flush(Commands& c, const std::vector<ObsRow>& rows, const std::vector<std::s
   return written;
 }
 
+void write_rxnlog_line(const Commands& c, FILE* fp, double time, const char* rname, const double* pos, int dim,
+                       int order, const uint64_t* rct, int first, int nprod, const uint64_t* prd) {
+  if (!fp) return;
+  write_num(c, fp, time);
+  fprintf(fp, " %s", rname);
+  for (int d = 0; d < dim; d++) {
+    fputc(' ', fp);
+    write_num(c, fp, pos[d]);
+  }
+  for (int k = 0; k < order; k++) {
+    fputc(' ', fp);
+    write_serno(fp, rct[k]);
+  }
+  for (int q = 0; q < first; q++) fprintf(fp, " ?");
+  for (int q = first; q < nprod; q++) {
+    fputc(' ', fp);
+    write_serno(fp, prd[q]);
+  }
+  fputc('\n', fp);
+}
+
 }  // namespace cmd
 }  // namespace smolamd

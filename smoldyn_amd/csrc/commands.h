@@ -140,6 +140,13 @@ int find_data(const Commands& c, const char* name);
 // written.
 int flush(Commands& c, const std::vector<ObsRow>& rows, const std::vector<std::string>& spname);
 
+// One reaction log line (doreact, smolreact.c:2883-2896): time, reaction
+// name, reaction position (dim values, %g with the output precision), the
+// reactant serial numbers (order of them), "?" for the products before
+// product `first`, then the serial numbers of products first..nprod-1.
+void write_rxnlog_line(const Commands& c, FILE* fp, double time, const char* rname, const double* pos, int dim,
+                       int order, const uint64_t* rct, int first, int nprod, const uint64_t* prd);
+
 // Parse "name" or "name(state)" (molstring2pattern): species index and state
 // (0 soln, 1 front, 2 back, 6 all); 0 for "all" species, -1 when not recognized.
 int parse_species(const std::vector<std::string>& spname, const char* word, int* ms);

@@ -443,6 +443,24 @@ int config_statement(simptr sim, const std::string& word, const std::string& res
       return 1;
     return failed(smolSetReactionRate(sim, w[0].c_str(), v, 1), word.c_str(), err) ? 1 : 0;
   }
+  if (word == "reaction_log") {
+    // reaction_log filename rxnname|all serial_numbers|all (smolsim.cpp:2016-2040)
+    if (w.size() < 3) return fail("reaction_log format: filename rxnname serial_numbers");
+    std::vector<long> sn;
+    int n = -1;
+    if (w[2] != "all") {
+      for (size_t k = 2; k < w.size(); k++) {
+        long v;
+        if (!inum(w[k], &v)) return fail("unable to read serial numbers");
+        sn.push_back(v);
+      }
+      n = (int)sn.size();
+    } else if (w.size() > 3) {
+      return fail("unexpected text following reaction_log");
+    }
+    return failed(smolamdSetReactionLog(sim, w[0].c_str(), w[1].c_str(), sn.data(), n, 1), word.c_str(), err) ? 1
+                                                                                                               : 0;
+  }
   if (word == "product_placement") {
     // product_placement rname type [parameter] / offset|fixed product x y z
     if (w.size() < 2) return fail("product_placement format: rname type parameters");

@@ -200,11 +200,13 @@ struct UniEvent {
   int box;   // virtual box the record's position is stored relative to (FP32-offset records)
 };
 
-// One reaction event in creation order (for the reaction log)
+// One reaction event in creation order (the reaction log): reactant and
+// product serial numbers (0 where there is none) and the reaction position
 struct LogRow {
   int order, r;
   uint64_t rct[2];
-  uint64_t prd[2];
+  uint64_t prd[kMaxProd];
+  double pos[3];
 };
 
 struct DeviceState;  // engine.hip

@@ -1961,8 +1961,12 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     row.rct[1] = sB;
     int pfirst = pc - rx->nprod;
     const bool cs = (rx->confspread || rx->bounce) && order == 2;
-    row.prd[0] = rx->nprod > 0 ? (cs ? sA : ser0 + (uint64_t)pfirst) : 0;
-    row.prd[1] = rx->nprod > 1 ? (cs ? sB : ser0 + (uint64_t)pfirst + 1) : 0;
+#pragma unroll
+    for (int q = 0; q < kMaxProd; q++)
+      row.prd[q] = q < rx->nprod ? (cs && q < 2 ? (q == 0 ? sA : sB) : ser0 + (uint64_t)(pfirst + q)) : 0;
+    row.pos[0] = rxnpos[0];
+    row.pos[1] = rxnpos[1];
+    row.pos[2] = rxnpos[2];
     I.log[e] = row;
   }
   __syncthreads();

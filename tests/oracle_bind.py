@@ -47,6 +47,7 @@ def lib():
             "ocmd_timeline": (I, [C.c_char_p, I, dp, ip, C.POINTER(C.c_longlong)]),
             "osim_set_confspread": (I, [P, I, D]),
             "osim_set_bindrad": (I, [P, I, D]),
+            "osim_rxnlog_pos": (I, [P, C.POINTER(C.c_double), I]),
             "osim_set_prdpos": (I, [P, I, I, I, dp]),
             "osim_rng_rotation": (None, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, dp]),
             "osim_new": (P, [I, dp, dp]),
@@ -297,6 +298,12 @@ class OSim:
         rows = np.zeros((max(n, 1), 8), np.int64)
         self.L.osim_rxnlog(self.p, rows.ctypes.data_as(C.POINTER(C.c_int64)), n)
         return rows[:n]
+
+    def rxnlog_pos(self):
+        n = self.L.osim_rxnlog_pos(self.p, None, 0)
+        pos = np.zeros((max(n, 1), self.dim))
+        self.L.osim_rxnlog_pos(self.p, _dp(pos), n)
+        return pos[:n]
 
     def add_surface(self, name):
         s = self.L.osim_add_surface(self.p, name.encode())

@@ -185,3 +185,55 @@ def test_listmols_equals_oracle_listmols(tmp_path):
                     rows7.append([1, L["ident"][j], L["mstate"][j], *L["pos"][j], L["serno"][j]])
     assert (tmp_path / "lm.txt").read_text() == text
     assert np.array_equal(d, np.array(rows7, dtype=np.float64))
+
+
+def _serno(s):
+    """molserno2string (smolmolec.c:625-630)"""
+    s = int(s)
+    return "%d" % s if s < 0xFFFFFFFF else "%d.%d" % (s >> 32, s & 0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("serials", [None, "list"])
+def test_reaction_log_lines_equal_oracle_events(tmp_path, serials):
+    """reaction_log (RxnSetLog smolreact.c:2013-2061, doreact's log lines
+    smolreact.c:2879-2896) through smolRunSim on the reversible workload at
+    6000 molecules: every logged line is "time name x y z reactant-serials
+    [?] product-serials" of an event the oracle makes in the same step, in
+    the same order, with the step's start time, the reaction position and the
+    serial numbers bit for bit (%g).  With a serial list, only events that
+    make a listed product are logged, the products before the first listed
+    one printed as '?'."""
+    dt = 0.005
+    spec = ss.c2_spec(n=3000, side=1000.0 * (6000 / 1e6) ** (1 / 3), kback=20.0, seed=29, tmax=60 * dt)
+    osim, rx = ss.build_oracle(spec)
+    names = {rx[k][0] * 100 + rx[k][1]: k for k in rx}
+    lines, t = [], 0.0
+    allrows = []
+    for k in range(_steps_to(0.0, spec["tmax"], dt)):
+        osim.step()
+        rows, pos = osim.rxnlog(), osim.rxnlog_pos()
+        allrows.extend((t, r, p) for r, p in zip(rows, pos))
+        t += dt
+    listed = None
+    if serials:
+        prods = sorted({int(r[5]) for _, r, _ in allrows if r[0] == 1})  # second products of dissociations
+        listed = prods[::3][:40]
+    for t, r, p in allrows:
+        order = int(r[0])
+        nprod = 1 if order == 2 else 2
+        prd = [int(r[4]), int(r[5])][:nprod]
+        first = 0 if listed is None else next((q for q in range(nprod) if (prd[q] & 0xFFFFFFFF) in listed), -1)
+        if first < 0:
+            continue
+        parts = ["%g" % t, names[order * 100 + int(r[1])]] + ["%g" % v for v in p]
+        parts += [_serno(s) for s in r[2:2 + order]] + ["?"] * first + [_serno(s) for s in prd[first:]]
+        lines.append(" ".join(parts))
+    dsim = ss.build_device(spec, update=False)
+    dsim.set_output_path(str(tmp_path) + "/")
+    dsim.add_output_file("rxn.log")
+    dsim.set_reaction_log("rxn.log", "all", listed)
+    assert dsim.run() == sa.ECok
+    dsim.close()
+    got = (tmp_path / "rxn.log").read_text().splitlines()
+    assert len(lines) > (20 if listed is None else 5)
+    assert got == lines
