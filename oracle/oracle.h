@@ -81,6 +81,12 @@ int osurf_geo_sphsxaabb3(const double* cent, double rad, const double* bpt1, con
 int osurf_geo_diskxaabb3(const double* cent, double rad, const double* norm, const double* bpt1,
                          const double* bpt2);
 double osurf_desorbdist_y(double step, double y, int revads);
+/* osurfparam.c: rates -> collision probabilities (SurfaceParam.c) */
+double osurf_surfaceprob(double k1, double k2, double dt, double difc, double* p2ptr, int algo);
+int osurf_transmit(double kap1, double kap2, double difc1, double difc2, double dt, double* p1ptr, double* p2ptr);
+double osurf_lookupirrevadsorb(double value);
+double osurf_lookuprevads(double value1, double value2, double* ans2ptr);
+double osurf_experfcm1(double x);
 int osurf_geo_linexaabb2(const double* pt1, const double* pt2, const double* norm, const double* bpt1,
                          const double* bpt2);
 int osurf_geo_circlexaabb2(const double* cent, double rad, const double* bpt1, const double* bpt2);

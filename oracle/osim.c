@@ -1643,9 +1643,11 @@ int osim_step(osim* sim) {
   if (o_diffuse(sim)) return 9;
   sim->inject = NULL;
   sim->ninject = 0;
-  if (sim->srfss) {
-    for (ll = 0; ll < sim->nlist; ll++) osurf_checksurfaces(sim, ll, 0);
-    for (ll = 0; ll < sim->nlist; ll++) osurf_checksurfacebound(sim, ll);
+  if (sim->srfss) { /* an interaction the oracle does not restate ends the step with 13 */
+    for (ll = 0; ll < sim->nlist; ll++)
+      if (osurf_checksurfaces(sim, ll, 0) < 0) return 13;
+    for (ll = 0; ll < sim->nlist; ll++)
+      if (osurf_checksurfacebound(sim, ll) < 0) return 13;
   } else {
     for (ll = 0; ll < sim->nlist; ll++)
       if (sim->diffuselist[ll]) o_checkwalls(sim, ll, 0);
@@ -1659,7 +1661,8 @@ int osim_step(osim* sim) {
   if (o_bireact(sim, 1)) return 5;
   o_molsort(sim, 0);
   if (sim->srfss) {
-    for (ll = 0; ll < sim->nlist; ll++) osurf_checksurfaces(sim, ll, 1);
+    for (ll = 0; ll < sim->nlist; ll++)
+      if (osurf_checksurfaces(sim, ll, 1) < 0) return 13;
   } else {
     for (ll = 0; ll < sim->nlist; ll++) o_checkwalls(sim, ll, 1);
   }

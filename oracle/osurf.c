@@ -592,13 +592,6 @@ int osurf_geo_hemisxaabb3(const double* cent, double rad, const double* outvect,
   return osurf_geo_rectxaabb3(r[0], r[1], r[2], r[3], bpt1, bpt2);
 }
 
-/* surfaceprob, IrrDes branch (SurfaceParam.c:159-163) */
-static double os_surfaceprob_irrdes(double k1, double k2, double dt) {
-  if (k1 == -1) return 1;
-  if (k2 <= 0) k2 = k1;
-  return k1 / k2 * (1.0 - exp(-k2 * dt));
-}
-
 /* desorbdist (SurfaceParam.c:202-220) for the two algorithms desorption uses */
 double osurf_desorbdist_y(double step, double y, int revads) {
   double y2 = y * y, x;
@@ -1464,8 +1457,13 @@ static int os_dosurfinteract(osim* sim, omol* mp, int ll, int mi, int p, int fac
     osim_molkill(sim, mp, ll, mi);
     if (os_checksurfaces1mol(sim, m2, 0, SMOLRNG_SURF_DESORBED) < 0) return -2;
     done = 1;
+  } else if (act == OSA_FLIP) { /* on-surface flip (smolsurface.c:5044-5048) */
+    os_molchangeident(sim, mp, ll, mi, i2, ms2, p, crsspt);
+    if (ms2 == OMS_FRONT) os_fixpt2panel(sim, mp->pos, &ss->pnl[mp->pnl], OPF_FRONT, epsilon);
+    else if (ms2 == OMS_BACK) os_fixpt2panel(sim, mp->pos, &ss->pnl[mp->pnl], OPF_BACK, epsilon);
+    done = 1;
   } else {
-    return -4; /* flip, port, emitters: not restated */
+    return -4; /* port, emitters: not restated */
   }
   return done;
 }
@@ -1652,8 +1650,12 @@ int osurf_checksurfacebound(osim* sim, int ll) {
         if (os_checksurfaces1mol(sim, m2, 0, SMOLRNG_SURF_DESORBED) < 0) return -2;
       } else if (act == OSA_ABSORB) {
         osim_molkill(sim, mp, ll, m);
+      } else if (act == OSA_FLIP) { /* on-surface flip (smolsurface.c:5044-5048), crsspt = posx */
+        os_molchangeident(sim, mp, ll, m, i2, ms2, mp->pnl, mp->posx);
+        if (ms2 == OMS_FRONT) os_fixpt2panel(sim, mp->pos, &ss->pnl[mp->pnl], OPF_FRONT, ss->epsilon);
+        else if (ms2 == OMS_BACK) os_fixpt2panel(sim, mp->pos, &ss->pnl[mp->pnl], OPF_BACK, ss->epsilon);
       } else {
-        return -4; /* flips between surface states: not restated */
+        return -4;
       }
     }
   }
@@ -1797,12 +1799,13 @@ static void os_reverseaction(int ms1, int face1, int ms2, int* ms3p, int* face2p
   if (actp) *actp = act;
 }
 
-/* srfcalcprob (smolsurface.c:3070-3146) for the cases restated: given
-   probabilities, and desorption / flips by rate without a reverse rate.
-   Returns -3 for a rate case that needs the reference's lookup tables. */
+/* srfcalcprob (smolsurface.c:3070-3146).  Returns -3 where the reference
+   leaves the probability undefined or finds it with random draws: a
+   two-sided transmission given by rates with one side maximal and the other
+   finite, or with a zero diffusion coefficient (surfacetransmit). */
 static double os_calcprob(osim* sim, const osurface* srf, int i, int ms1, int face, int ms2) {
   const oactdet* a;
-  double rate, raterev, sum = 0;
+  double rate, raterev, sum, dt, difc, difcrev, prob;
   int ms3, face2, ms4, actrev, i2, ms5;
   if (ms1 == OMS_SOLN && face == OPF_NONE) return 0;
   if (srf->action[i][ms1][face] != OSA_MULT) return -1;
@@ -1825,16 +1828,45 @@ static double os_calcprob(osim* sim, const osurface* srf, int i, int ms1, int fa
   } else if (srf->action[i2][ms3][face2] == actrev) raterev = -1;
   else raterev = 0;
   if (raterev < 0 && raterev != -1) raterev = 0;
+  dt = sim->dt;
+  difc = sim->difc[i][OMS_SOLN];
+  difcrev = sim->difc[i2][OMS_SOLN];
+  sum = 0;
   if (ms1 != OMS_SOLN && face == OPF_NONE)
     for (ms5 = 0; ms5 < O_MSMAX1; ms5++)
       if (ms5 != ms1 && a->datasrc[ms5] == 1 && a->rate[ms5] >= 0) sum += a->rate[ms5];
-  if (ms1 != OMS_SOLN && face == OPF_NONE && (ms2 == OMS_SOLN || ms2 == OMS_BSOLN) && raterev == 0)
-    return os_surfaceprob_irrdes(rate, sum, sim->dt);
-  if (ms1 != OMS_SOLN && face == OPF_NONE && ms2 != OMS_SOLN && ms2 != OMS_BSOLN) {
-    if (ms1 == ms2) return 0;
-    if (raterev == 0) return os_surfaceprob_irrdes(rate, sum, sim->dt); /* IrrFlip has the IrrDes form */
+  if (ms1 == OMS_SOLN) {
+    if ((face == OPF_FRONT && ms2 == OMS_SOLN) || (face == OPF_BACK && ms2 == OMS_BSOLN)) prob = 0;
+    else if (ms2 == OMS_SOLN || ms2 == OMS_BSOLN) {
+      if (raterev == 0) prob = osurf_surfaceprob(rate, 0, dt, difc, NULL, 0 /* SPAirrTrans */);
+      else {
+        prob = -2;
+        if (osurf_transmit(rate, raterev, difc, difcrev, dt, &prob, NULL)) return -3;
+      }
+    } else {
+      if (raterev == 0) prob = osurf_surfaceprob(rate, 0, dt, difc, NULL, 4 /* SPAirrAds */);
+      else prob = osurf_surfaceprob(rate, raterev, dt, difc, NULL, 8 /* SPArevAds */);
+    }
+  } else if (face == OPF_NONE) {
+    if (ms2 == OMS_SOLN || ms2 == OMS_BSOLN) {
+      if (raterev == 0) prob = osurf_surfaceprob(rate, sum, dt, difc, NULL, 10 /* SPAirrDes */);
+      else prob = osurf_surfaceprob(rate, raterev, dt, difc, NULL, 11 /* SPArevDes */);
+    } else {
+      if (ms1 == ms2) prob = 0;
+      else if (raterev == 0) prob = osurf_surfaceprob(rate, sum, dt, difc, NULL, 12 /* SPAirrFlip */);
+      else prob = osurf_surfaceprob(rate, raterev, dt, difc, NULL, 13 /* SPArevFlip */);
+    }
+  } else {
+    if ((face == OPF_FRONT && ms2 == OMS_SOLN) || (face == OPF_BACK && ms2 == OMS_BSOLN)) prob = 0;
+    else if (ms2 == OMS_SOLN || ms2 == OMS_BSOLN) {
+      if (raterev == 0) prob = osurf_surfaceprob(rate, 0, dt, difc, NULL, 0 /* SPAirrTrans */);
+      else {
+        prob = -2;
+        if (osurf_transmit(rate, raterev, difc, difcrev, dt, &prob, NULL)) return -3;
+      }
+    } else prob = osurf_surfaceprob(rate, 0, dt, difc, NULL, 4 /* SPAirrAds */);
   }
-  return -3;
+  return prob;
 }
 
 /* panelinbox (smolboxes.c:95-152) */

@@ -353,9 +353,11 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       int mkind = 0, newll = ll;  // list change queued for the first molsort (surfaces)
       int obox = -1, rbox = -1;    // desorption: box in the old list, box the reference files it in
       const int bold = kF32 ? a.box[r] : 0;
+      const int bprev = kSurf ? a.box[r] : -1;  // the box the record was filed in last step
       double pos[3];
       load_pos<kF32>(a, g, r, bold, pos);
       const bool moved = tab->diffuselist[ll];
+      bool flipped = false;  // a bound molecule moved to the other face of its panel
       if (moved) {
         const double posx[3] = {pos[0], pos[1], pos[2]};
         double ds = tab->difstep[i];
@@ -402,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
           }
         }
       } else if (kSurf && sb && I.sf.listbound[ll]) {
-        // checksurfacebound (smolsurface.c:5184-5199): desorption
+        // checksurfacebound (smolsurface.c:5184-5199): desorption or a flip
         int i2 = i, rb = 0;
         obox = pos2box(g, pos[0], pos[1], pos[2]);
         const int st = srf_bound<(kSurf > 1)>(I.sf, g, tab->difc, tab->difstep, pos, &i2, sb, ser, step, I.key0, I.key1,
@@ -422,6 +424,24 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
           newll = rb ? I.sf.blist[i][bstate(rb)] : tab->listof[i];
           if (rb) atomicAdd((rb & 3) == 1 ? &ctl->countf[i] : &ctl->countk[i], 1);
           mkind = kMoveDesorb;
+        } else if (st == kSrfFlipped && I.sf.blist[i2][bstate(rb)] >= 0) {
+          // a flip: the state's counter, the species and, when the new
+          // state lives in another list, a list change for the first molsort
+          atomicSub((sb & 3) == 1 ? &ctl->countf[i] : &ctl->countk[i], 1);
+          atomicAdd((rb & 3) == 1 ? &ctl->countf[i2] : &ctl->countk[i2], 1);
+          if (i2 != i) {
+            a.ident[r] = i2;
+            atomicSub(&ctl->count[i], 1);
+            atomicAdd(&ctl->count[i2], 1);
+            i = i2;
+          }
+          sb = rb;
+          a.sb[r] = rb;
+          flipped = true;
+          if (I.sf.blist[i][bstate(rb)] != ll) {
+            newll = I.sf.blist[i][bstate(rb)];
+            mkind = kMoveList;
+          }
         } else if (st != kSrfOk) {
           atomicOr(&ctl->err, kErrSurface);
           atomicOr((int*)&gl->halt, 1);
@@ -451,12 +471,16 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       }
       int b = pos2box(g, pos[0], pos[1], pos[2]);
       if (kSurf && mkind == kMoveDesorb) b = rbox;  // the reference's (possibly stale) box for this step
+      // a bound molecule in a list that does not diffuse keeps the box it was
+      // filed in: reassignmolecs skips such lists (smolboxes.c:896-954), so a
+      // flip to the other face of a panel on a box boundary leaves it stale
+      else if (kSurf && sb && !moved && bprev >= 0) b = bprev;
       if (kSurf && mev >= 0) {  // boxes of the list change: a moving molecule keeps its box
         if (mkind != kMoveDesorb) I.moves[mev].obox = b;
         I.moves[mev].nbox = b;
       }
       // FP32-offset records are stored relative to the new box
-      if (moved || mkind == kMoveDesorb || (kF32 && b != bold)) store_pos<kF32>(a, g, r, b, pos);
+      if (moved || flipped || mkind == kMoveDesorb || (kF32 && b != bold)) store_pos<kF32>(a, g, r, b, pos);
       // z sub-bin: floor of the box-relative z coordinate times zsub (exact
       // for a power of two), clamped like the box index
       int zs = 0;

@@ -15,8 +15,8 @@ namespace rxnparam {
 #include "rxnparam_tables.inc"
 
 // complementary error function, Numerical Recipes form used by the reference
-// (rxnparam.c:719-725)
-static double erfcc(double x) {
+// (rxnparam.c:719-725; erfccD of math2.c:408-414 is the same expression)
+double erfcc(double x) {
   double t, z, ans;
   z = fabs(x);
   t = 1.0 / (1.0 + 0.5 * z);

@@ -3,6 +3,7 @@
 
 namespace smolamd {
 namespace rxnparam {
+double erfcc(double x);
 double modelrxnrate(double a, double b, double difc, double chi);
 double numrxnrate(double step, double a, double b);
 double numrxnrateprob(double step, double a, double b, double prob);

@@ -487,7 +487,9 @@ __device__ __forceinline__ int srf_jump(const DevSurf& sf, const DevPanel& pn, d
   return 1;
 }
 
-enum SrfStatus : int { kSrfOk = 0, kSrfAbsorbed = 1, kSrfUnsupported = 2, kSrfAdsorbed = 3, kSrfDesorbed = 4 };
+enum SrfStatus : int {
+  kSrfOk = 0, kSrfAbsorbed = 1, kSrfUnsupported = 2, kSrfAdsorbed = 3, kSrfDesorbed = 4, kSrfFlipped = 5
+};
 
 // checksurfaces (smolsurface.c:5118-5180) for one solution-phase molecule:
 // segment posx -> pos, up to 100 interactions.  ident may change (species
@@ -598,6 +600,8 @@ __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, c
 // position before the surface check).  bound = (panel << 3) | state.  The
 // check may adsorb the new molecule again (dosurfinteract's adsorb branch for
 // the resurrected molecule): then *rebound = its (panel << 3) | state, else 0.
+// A flip to the other bound state returns kSrfFlipped with *rebound = the
+// new (panel << 3) | state and ident updated.
 template <bool kCv>
 __device__ __forceinline__ int srf_bound(const DevSurf& sf, const Geometry& g, const double* difc,
                                          const double* difstep, double* pos, int* ident, int bound, uint64_t serial,
@@ -617,6 +621,19 @@ __device__ __forceinline__ int srf_bound(const DevSurf& sf, const Geometry& g, c
     }
   const int o = a.outcome[ms2], i2 = a.newspec[ms2];
   if (o == kSaNo) return kSrfOk;
+  if (o == kSaFlip) {
+    // on-surface flip (smolsurface.c:5044-5048): molchangeident puts the
+    // molecule on the face of its new state (smolmolec.c:2898-2901) and
+    // dosurfinteract does so again; bound states do not diffuse on this
+    // path, so molchangeident makes no motion correction
+    if (ms2 != 1 && ms2 != 2) return kSrfUnsupported;
+    const int face = ms2 == 1 ? kFaceFront : kFaceBack;
+    srf_fixpt2panel<kCv>(pn, pos, face, sf.epsilon);
+    srf_fixpt2panel<kCv>(pn, pos, face, sf.epsilon);
+    *ident = i2;
+    *rebound = (p << 3) | ms2;
+    return kSrfFlipped;
+  }
   if (o != kSaRevdes && o != kSaIrrevdes) return kSrfUnsupported;
   const int face = ms2 == 0 ? kFaceFront : kFaceBack;
   double crsspt[3] = {pos[0], pos[1], pos[2]};

@@ -6,6 +6,8 @@
 // order (surfaces, then shapes, then panels).
 #include "surfaces_host.h"
 
+#include "surfparam.h"
+
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
@@ -387,6 +389,43 @@ void reverse_action(int ms1, int face1, int ms2, int* ms3p, int* face2p, int* ms
   if (actp) *actp = act;
 }
 
+// the rate branch of srfcalcprob (smolsurface.c:3097-3144): the probability
+// that a molecule in state ms1 at face ends in ms2 when that transition is
+// given by a rate, the reverse transition by raterev (0: none, -1: maximal).
+// difc and difcrev are the solution diffusion coefficients of the species
+// and of the one it becomes.  *st is set when the reference would leave the
+// probability undefined (transmit_probs' codes).
+double rate_prob(int ms1, int face, int ms2, double rate, double raterev, const SrfDetail& det, double difc,
+                 double difcrev, double dt, int* st) {
+  namespace sp = surfparam;
+  bool to_soln = ms2 == MSsoln_ || ms2 == MSbsoln_;
+  if (ms1 == MSsoln_ || face != PFnone_) {
+    // at a face: a solution molecule, or a surface-bound one meeting a panel
+    if ((face == PFfront_ && ms2 == MSsoln_) || (face == PFback_ && ms2 == MSbsoln_)) return 0;  // staying
+    if (to_soln) {  // transmission
+      if (raterev == 0) return sp::surfaceprob(rate, 0, dt, difc, nullptr, sp::kIrrTrans);
+      double p = -2;
+      *st = sp::transmit_probs(rate, raterev, difc, difcrev, dt, &p, nullptr);
+      return p;
+    }
+    if (ms1 != MSsoln_) return sp::surfaceprob(rate, 0, dt, difc, nullptr, sp::kIrrAds);  // hop to another surface
+    if (raterev == 0) return sp::surfaceprob(rate, 0, dt, difc, nullptr, sp::kIrrAds);  // adsorption
+    return sp::surfaceprob(rate, raterev, dt, difc, nullptr, sp::kRevAds);
+  }
+  // surface-bound, leaving by desorption or a flip: first-order loss over the
+  // sum of every exit rate when irreversible
+  double sum = 0;
+  for (int ms5 = 0; ms5 < kSrfMS1; ms5++)
+    if (ms5 != ms1 && det.datasrc[ms5] == 1 && det.rate[ms5] >= 0) sum += det.rate[ms5];
+  if (to_soln) {
+    if (raterev == 0) return sp::surfaceprob(rate, sum, dt, difc, nullptr, sp::kIrrDes);
+    return sp::surfaceprob(rate, raterev, dt, difc, nullptr, sp::kRevDes);
+  }
+  if (ms1 == ms2) return 0;
+  if (raterev == 0) return sp::surfaceprob(rate, sum, dt, difc, nullptr, sp::kIrrFlip);
+  return sp::surfaceprob(rate, raterev, dt, difc, nullptr, sp::kRevFlip);
+}
+
 std::unique_ptr<SrfDetail> new_detail(int species) {  // surfaceactionalloc
   std::unique_ptr<SrfDetail> a(new SrfDetail());
   memset(a.get(), 0, sizeof(SrfDetail));
@@ -712,20 +751,13 @@ int SurfaceSet::build(const Geometry& g, int nspecies, const double (*difc_ms)[k
                 }
               }
               if (raterev < 0 && raterev != -1) raterev = 0;
-              if (ms != MSsoln_ && f == PFnone_ && ms2 == ms) {
-                p = 0;
-              } else if (ms != MSsoln_ && f == PFnone_ && raterev == 0) {
-                // desorption or a flip, irreversible (surfaceprob, SurfaceParam.c:159-163)
-                double sum = 0;
-                for (int ms5 = 0; ms5 < kSrfMS1; ms5++)
-                  if (ms5 != ms && det->datasrc[ms5] == 1 && det->rate[ms5] >= 0) sum += det->rate[ms5];
-                if (rate == -1) p = 1;
-                else p = rate / (sum > 0 ? sum : rate) * (1.0 - exp(-(sum > 0 ? sum : rate) * dt));
-              } else {
+              int st = 0;
+              p = rate_prob(ms, f, ms2, rate, raterev, *det, difc_ms[i][MSsoln_], difc_ms[i2][MSsoln_], dt, &st);
+              if (st) {
                 snprintf(msg, msglen,
-                         "surface %s: collision probabilities from rates use the reference's adsorption lookup "
-                         "tables, which the device path does not restate; give them as probabilities "
-                         "(rate_internal)",
+                         st == 3 ? "surface %s: a two-sided transmission with one side at the maximal rate is "
+                                   "converted by the reference with a randomized search, which is not restated"
+                                 : "surface %s: transmission probability undefined (zero diffusion coefficient)",
                          S.name.c_str());
                 return 3;
               }
@@ -750,6 +782,15 @@ int SurfaceSet::build(const Geometry& g, int nspecies, const double (*difc_ms)[k
             sum += q.prob[ms2];
             q.cum[ms2] = sum;
           }
+          // the reference's parameter check rejects these (checksurfaceparams,
+          // smolsurface.c:1940-1943); a rate-given reverse desorption at the
+          // maximal rate gives one (SurfaceParam.c:170-172)
+          for (int ms2 = 0; ms2 < kSrfMS1; ms2++)
+            if (!(q.prob[ms2] >= 0 && q.prob[ms2] <= 1)) {
+              snprintf(msg, msglen, "surface %s: interaction probability %g of species %d is outside [0, 1]",
+                       S.name.c_str(), q.prob[ms2], i);
+              return 3;
+            }
         }
     for (int i = 1; i < nspecies; i++)
       for (int ms = 0; ms < kSrfMS; ms++)
@@ -851,9 +892,11 @@ int SurfaceSet::build(const Geometry& g, int nspecies, const double (*difc_ms)[k
           else if (ms2 == MSsoln_ || ms2 == MSbsoln_) o = q.rev[ms2] > 0 ? SArevdes_ : SAirrevdes_;
           else o = SAflip_;
           a.outcome[ms2] = o;
-          if (q.prob[ms2] > 0 && (o == SAabsorb_ || o == SAflip_)) {
-            snprintf(msg, msglen, "surface %s: absorption and state flips of surface-bound molecules are not "
-                     "implemented on the device path", S.name.c_str());
+          // flips between the front and back states run on the device;
+          // absorption and flips to the up and down states do not
+          if (q.prob[ms2] > 0 && (o == SAabsorb_ || (o == SAflip_ && ms2 != MSfront_ && ms2 != MSback_))) {
+            snprintf(msg, msglen, "surface %s: absorption of surface-bound molecules and flips to the up or down "
+                     "state are not implemented on the device path", S.name.c_str());
             return 2;
           }
         }

@@ -114,6 +114,11 @@ def lib():
             "osurf_geo_cylsxaabb3": (I, [dp, dp, D, dp, dp]),
             "osurf_geo_hemisxaabb3": (I, [dp, D, dp, dp, dp]),
             "osurf_desorbdist_y": (D, [D, D, I]),
+            "osurf_surfaceprob": (D, [D, D, D, D, dp, I]),
+            "osurf_transmit": (I, [D, D, D, D, D, dp, dp]),
+            "osurf_lookupirrevadsorb": (D, [D]),
+            "osurf_lookuprevads": (D, [D, D, dp]),
+            "osurf_experfcm1": (D, [D]),
             "omath_randtable": (None, [dp, I]),
             "omath_erfn": (D, [D]),
         }
@@ -390,6 +395,10 @@ def reflib():
         ("Geo_CylsXaabb3", I, [dp, dp, D, dp, dp]),
         ("Geo_HemisXaabb3", I, [dp, D, dp, dp, dp]),
         ("surfaceprob", D, [D, D, D, D, dp, I]),
+        ("surfacetransmit", I, [dp, dp, dp, dp, D, D, D]),
+        ("lookupirrevadsorb", D, [D, I]),
+        ("lookuprevads", D, [D, D, I, dp]),
+        ("experfcm1D", D, [D]),
         ("desorbdist", D, [D, I]),
         ("ref_randCCD", D, []),
     ]:

@@ -1,7 +1,7 @@
 """Build the same simulation in the CPU oracle and in the device engine.
 
 A spec is a dict: low/high bounds, walls ('r','p','t' per dimension), dt,
-seed, species {name: difc}, optional lists {list: [species]}, reactions
+seed, species {name: difc}, optional lists {list: [species or (species, state)]}, reactions
 [(name, [reactants], [products], rate)], product placements
 {name: (kind, param)}, molecules [(species, n)] placed uniformly from the
 counter stream, or explicit positions [(species, array)], and optional
@@ -13,6 +13,9 @@ libsmoldyn calls set them (smolSetSurfaceAction passes no species change).
 import numpy as np
 
 import oracle_bind as ob
+
+
+_STATES = {"soln": 0, "front": 1, "back": 2, "up": 3, "down": 4}  # MolecState codes (smoldyn.h)
 
 
 def build_oracle(spec, sfmt=False):
@@ -35,7 +38,10 @@ def build_oracle(spec, sfmt=False):
     for ll, (lname, members) in enumerate(spec.get("lists", {}).items()):
         sim.add_list(lname)
         for m in members:
-            sim.set_list(idx[m], ll, 6)
+            if isinstance(m, tuple):  # (species, state): one state's list
+                sim.set_list(idx[m[0]], ll, _STATES[m[1]])
+            else:
+                sim.set_list(idx[m], ll, 6)
     if "molperbox" in spec:
         sim.partitions("molperbox", spec["molperbox"])
     if "boxsize" in spec:
@@ -92,7 +98,10 @@ def build_device(spec, update=True):
         sim.mobility(name, d)
     for lname, members in spec.get("lists", {}).items():
         for m in members:
-            sim.set_list(m, lname)
+            if isinstance(m, tuple):
+                sim.set_list(m[0], lname, _STATES[m[1]])
+            else:
+                sim.set_list(m, lname)
     if "molperbox" in spec:
         sim.partitions("molperbox", spec["molperbox"])
     if "boxsize" in spec:
@@ -219,6 +228,26 @@ def c3_adsorb_spec(n=2000, side=100.0, seed=33, ndisk=8, p_ads=0.3, k_des=2.0, k
                         ("A", "soln", "bsoln", "back", p_ads, True),
                         ("A", "front", "front", "fsoln", k_des, False),
                         ("A", "back", "back", "bsoln", k_des, False)]
+    return spec
+
+
+def c3_rates_spec(n=2000, side=60.0, seed=47, kback=20.0, ndisk=4, rdisk=6.0, k_ads=8.0, k_des=20.0,
+                  k_flip=(10.0, 30.0), k_trans=(5.0, 2.0)):
+    """Surface interactions given as rates (`rate`, converted to collision
+    probabilities at setup as srfcalcprob does): c3_spec's membrane transmits
+    A both ways at rates k_trans (reversible transmission, surfacetransmit);
+    its disk patches adsorb A from the front reversibly (k_ads, with
+    desorption back to the front at k_des: lookuprevads) and from the back
+    irreversibly (lookupirrevadsorb), and the bound states flip between front
+    and back at rates k_flip (reversible flips)."""
+    spec = c3_spec(n=n, side=side, seed=seed, kback=kback, ndisk=ndisk, rdisk=rdisk)
+    spec["surfaces"][1]["rates"] = [("A", "soln", "fsoln", "bsoln", k_trans[0], False),
+                                    ("A", "soln", "bsoln", "fsoln", k_trans[1], False)]
+    spec["surfaces"][2]["rates"] = [("A", "soln", "fsoln", "front", k_ads, False),
+                                    ("A", "soln", "bsoln", "back", k_ads, False),
+                                    ("A", "front", "front", "fsoln", k_des, False),
+                                    ("A", "front", "front", "back", k_flip[0], False),
+                                    ("A", "back", "back", "front", k_flip[1], False)]
     return spec
 
 

@@ -168,6 +168,39 @@ def test_surfaces_adsorption_injected_steps():
     _run_pair(spec, 40, check_every=4, inject=True)
 
 
+def test_surfaces_rate_given_interactions_bit_exact():
+    """Surface actions given by rates: the host converts them to collision
+    probabilities as srfcalcprob does (reversible transmission through the
+    membrane, reversible adsorption from the front with reversible
+    desorption, irreversible adsorption from the back, reversible flips of
+    the bound states); trajectories, bound states and lists equal the
+    oracle's, whose conversion is pinned to the reference's surfaceprob."""
+    import smoldyn_amd as sa
+    spec = ss.c3_rates_spec(n=3000, side=60.0, seed=47)
+    _run_pair(spec, 300, check_every=10)
+    dsim = ss.build_device(spec)
+    dsim.run_steps(300)
+    assert dsim.count("A", sa.MSfront) > 0 and dsim.count("A", sa.MSback) > 0
+
+
+def test_surfaces_flips_between_lists_bit_exact():
+    """The front and back states of A in lists of their own: a flip moves the
+    molecule between the two lists at the first molsort (molchangeident's
+    list change, smolmolec.c:2902-2906), like adsorption does."""
+    spec = ss.c3_rates_spec(n=3000, side=60.0, seed=49, k_flip=(60.0, 90.0))
+    sp = list(spec["species"])
+    spec["lists"] = {"sol": [(x, "soln") for x in sp], "Afront": [("A", "front")], "Aback": [("A", "back")],
+                     "fixed": [("A", "up"), ("A", "down")] + [(x, m) for x in sp[1:]
+                                                               for m in ("front", "back", "up", "down")]}
+    _run_pair(spec, 200, check_every=10)
+
+
+def test_surfaces_rate_given_injected_large_steps():
+    spec = ss.c3_rates_spec(n=1500, side=40.0, seed=48, rdisk=4.0)
+    spec["dt"] = 0.1
+    _run_pair(spec, 40, check_every=4, inject=True)
+
+
 def test_surfaces_injected_large_steps():
     """Large injected steps make molecules cross several boxes and several
     panels per step (line2nextbox walk, repeated interactions)."""

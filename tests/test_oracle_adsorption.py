@@ -178,3 +178,96 @@ def test_desorption_probability_is_one_minus_exp():
     left = sum(1 for s in bound if after[s]["ms"] == 0)
     p = 1 - np.exp(-k * DT)
     assert abs(left - n * p) < 4 * np.sqrt(n * p * (1 - p)), (left, n * p)
+
+
+def _sp(k1, k2, algo):
+    """The oracle's surfaceprob (pinned to the reference's) for A in solution."""
+    import ctypes as C
+    from oracle_bind import lib
+    p2 = C.c_double(0)
+    return lib().osurf_surfaceprob(k1, k2, DT, D_A, C.byref(p2), algo)
+
+
+def _rate_spec(n, k_ads, k_des, seed):
+    """Rates (`rate`, not `rate_internal`): adsorption from the front is
+    reversible (the front state desorbs back to the front at k_des),
+    adsorption from the back is irreversible (nothing leaves the back state)."""
+    spec, h = _spec(n, 0.0, 0.0, seed=seed)
+    spec["surfaces"][0]["rates"] = [("A", "soln", "fsoln", "front", k_ads, False),
+                                    ("A", "soln", "bsoln", "back", k_ads, False),
+                                    ("A", "front", "front", "fsoln", k_des, False)]
+    return spec, h
+
+
+def test_rate_given_adsorption_picks_reversible_or_irreversible_lookup():
+    """srfcalcprob (smolsurface.c:3122-3124): a solution molecule adsorbs with
+    the reversible-adsorption probability (SPArevAds, lookuprevads) when its
+    bound state desorbs back to that face, else with the irreversible one
+    (SPAirrAds, lookupirrevadsorb); kappa = k dt / sqrt(2 D dt)."""
+    k_ads, k_des, n = 3.0, 100.0, 12000
+    p_rev, p_irr = _sp(k_ads, k_des, 8), _sp(k_ads, 0.0, 4)
+    assert abs(p_rev - 0.2885) < 1e-3 and abs(p_irr - 0.4107) < 1e-3  # far apart at these rates
+    spec, h = _rate_spec(n, k_ads, k_des, seed=37)
+    osim, before, after, _ = _cross_once(spec, h, seed=8)
+    nf = nb = kf = kb = 0
+    for s, b in before.items():
+        up = b["pos"][2] > Z0
+        nf += up
+        nb += not up
+        kf += up and after[s]["ms"] == 1
+        kb += (not up) and after[s]["ms"] == 2
+    for k, m, p in ((kf, nf, p_rev), (kb, nb, p_irr)):
+        assert abs(k - m * p) < 4 * np.sqrt(m * p * (1 - p)), (k, m, p)
+
+
+def test_rate_given_reversible_desorption_probability():
+    """A bound state that desorbs to the face its adsorption came from leaves
+    with SPArevDes's probability (the reversible problem read backwards:
+    surfaceprob(k_des, k_ads), SurfaceParam.c:165-174), not 1 - exp(-k dt)."""
+    k_ads, k_des, n = 3.0, 100.0, 12000
+    p_des = _sp(k_des, k_ads, 11)
+    assert abs(p_des - 0.5425) < 1e-3 and abs((1 - np.exp(-k_des * DT)) - p_des) > 0.08
+    spec, h = _rate_spec(n, k_ads, k_des, seed=39)
+    osim, _, bound, _ = _cross_once(spec, h, seed=9)
+    osim.inject(np.zeros((max(len(osim.get_list(0)["ident"]), 1), 3)))  # solution molecules stay put
+    assert osim.step() == 0
+    after = _state(osim)
+    front = [s for s in bound if bound[s]["ms"] == 1]
+    back = [s for s in bound if bound[s]["ms"] == 2]
+    left = sum(1 for s in front if after[s]["ms"] == 0)
+    m = len(front)
+    assert abs(left - m * p_des) < 4 * np.sqrt(m * p_des * (1 - p_des)), (left, m, p_des)
+    assert all(after[s]["ms"] == 2 for s in back)  # the back state has no exit
+
+
+def test_flip_between_bound_states():
+    """A bound molecule flips to the other face's state with the two-state
+    relaxation probability k1/(k1+k2) (1 - exp(-(k1+k2) dt)) when both flips
+    are given (SPArevFlip, SurfaceParam.c:186-196; srfcalcprob
+    smolsurface.c:3131-3134) and moves to that face of its panel
+    (molchangeident + fixpt2panel, smolsurface.c:5044-5048): same place in
+    the plane, on the other side of z = 50 by at most a few ulps."""
+    n, k_fb, k_bf = 8000, 40.0, 10.0
+    spec, h = _spec(n, 1.0, 1.0, seed=41)
+    spec["surfaces"][0]["rates"] += [("A", "front", "front", "back", k_fb, False),
+                                     ("A", "back", "back", "front", k_bf, False)]
+    osim, _, bound, _ = _cross_once(spec, h, seed=10)
+    assert all(v["ms"] != 0 for v in bound.values())
+    osim.inject(np.zeros((1, 3)))
+    assert osim.step() == 0
+    after = _state(osim)
+    kt = k_fb + k_bf
+    for src, dst, k in ((1, 2, k_fb), (2, 1, k_bf)):
+        mols = [s for s in bound if bound[s]["ms"] == src]
+        flipped = [s for s in mols if after[s]["ms"] == dst]
+        p = k / kt * (1 - np.exp(-kt * DT))
+        m = len(mols)
+        assert abs(len(flipped) - m * p) < 4 * np.sqrt(m * p * (1 - p)), (src, len(flipped), m * p)
+        for s in flipped:
+            a, b = after[s]["pos"], bound[s]["pos"]
+            assert np.array_equal(a[:2], b[:2])
+            dz = a[2] - Z0
+            assert (dz > 0 if dst == 1 else dz < 0) and abs(dz) < 1e-12
+            assert after[s]["list"] == bound[s]["list"]  # both bound states share a list
+        for s in set(mols) - set(flipped):
+            assert np.array_equal(after[s]["pos"], bound[s]["pos"]) and after[s]["ms"] == src
