@@ -174,7 +174,8 @@ struct DevSurf {
 // its bound state, or a desorbing bound molecule whose list slot dies while
 // the record is appended, as a new solution molecule with the same serial,
 // to the end of its solution list.
-enum MoveKind : int { kMoveList = 1, kMoveDesorb = 2 };
+// kMoveKill: a molecule absorbed by a surface; its slot dies in the first molsort
+enum MoveKind : int { kMoveList = 1, kMoveDesorb = 2, kMoveKill = 3 };
 struct MoveEv {
   int rec;   // record in the source buffer of the step
   int dst;   // record in the destination buffer (k_scatter), -1 when not binned (a first-order reactant)

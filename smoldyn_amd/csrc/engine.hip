@@ -95,7 +95,7 @@ struct Prod {
   int ident;
   int list;
   double x, y, z;
-  int sb;  // surfaces: a product adsorbed by the reborn check (pending its list change), else 0
+  int sb;  // surfaces: a product adsorbed by the reborn check (pending its list change), kProdKilled, else 0
   int pad;
   uint64_t serno;  // a confspread product keeps its reactant's serial number; 0 for a new serial
   double sx, sy, sz;  // surfaces: where the product was made (start of its reborn surface check)
@@ -358,6 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       load_pos<kF32>(a, g, r, bold, pos);
       const bool moved = tab->diffuselist[ll];
       bool flipped = false;  // a bound molecule moved to the other face of its panel
+      bool dead = false;     // absorbed by a surface: not binned, its slot dies in the first molsort
       if (moved) {
         const double posx[3] = {pos[0], pos[1], pos[2]};
         double ds = tab->difstep[i];
@@ -389,10 +390,17 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
             newll = I.sf.blist[i2][bstate(bnd)];
             mkind = kMoveList;
             atomicAdd((bnd & 3) == 1 ? &ctl->countf[i2] : &ctl->countk[i2], 1);
+          } else if (st == kSrfAbsorbed) {
+            // molkill (smolsurface.c:4994-4996): the first molsort removes
+            // the slot; the molecule counts no more, under any species
+            dead = true;
+            mkind = kMoveKill;
+            atomicSub(&ctl->count[i], 1);
+            i2 = i;
           } else if (st != kSrfOk || (i2 != i && tab->listof[i2] != ll)) {
-            // an outcome the device path does not run (absorption, adsorption
-            // into a state without a list, a species change into another
-            // list): report it and stop stepping
+            // an outcome the device path does not run (adsorption into a
+            // state without a list, a species change into another list):
+            // report it and stop stepping
             atomicOr(&ctl->err, kErrSurface);
             atomicOr((int*)&gl->halt, 1);
           }
@@ -404,9 +412,11 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
           }
         }
       } else if (kSurf && sb && I.sf.listbound[ll]) {
-        // checksurfacebound (smolsurface.c:5184-5199): desorption or a flip
+        // checksurfacebound (smolsurface.c:5184-5199): desorption, a flip or
+        // absorption.  The molecule leaves the box it was filed in (bound
+        // lists are not reassigned, so that box may be stale).
         int i2 = i, rb = 0;
-        obox = pos2box(g, pos[0], pos[1], pos[2]);
+        obox = bprev >= 0 ? bprev : pos2box(g, pos[0], pos[1], pos[2]);
         const int st = srf_bound<(kSurf > 1)>(I.sf, g, tab->difc, tab->difstep, pos, &i2, sb, ser, step, I.key0, I.key1,
                                               &rbox, &rb);
         if (st == kSrfDesorbed && (!rb || I.sf.blist[i2][bstate(rb)] >= 0)) {
@@ -442,6 +452,11 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
             newll = I.sf.blist[i][bstate(rb)];
             mkind = kMoveList;
           }
+        } else if (st == kSrfAbsorbed) {
+          dead = true;
+          mkind = kMoveKill;
+          atomicSub((sb & 3) == 1 ? &ctl->countf[i] : &ctl->countk[i], 1);
+          atomicSub(&ctl->count[i], 1);
         } else if (st != kSrfOk) {
           atomicOr(&ctl->err, kErrSurface);
           atomicOr((int*)&gl->halt, 1);
@@ -476,15 +491,17 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       // flip to the other face of a panel on a box boundary leaves it stale
       else if (kSurf && sb && !moved && bprev >= 0) b = bprev;
       if (kSurf && mev >= 0) {  // boxes of the list change: a moving molecule keeps its box
-        if (mkind != kMoveDesorb) I.moves[mev].obox = b;
+        // a molecule of a diffusing list was filed in the box of its new
+        // position by reassignmolecs before the molsort
+        if (mkind == kMoveList || (mkind == kMoveKill && moved)) I.moves[mev].obox = b;
         I.moves[mev].nbox = b;
       }
       // FP32-offset records are stored relative to the new box
-      if (moved || flipped || mkind == kMoveDesorb || (kF32 && b != bold)) store_pos<kF32>(a, g, r, b, pos);
+      if (!dead && (moved || flipped || mkind == kMoveDesorb || (kF32 && b != bold))) store_pos<kF32>(a, g, r, b, pos);
       // z sub-bin: floor of the box-relative z coordinate times zsub (exact
       // for a power of two), clamped like the box index
       int zs = 0;
-      if (I.zsub > 1) {
+      if (!dead && I.zsub > 1) {
         const double zrel = (pos[2] - g.bmin[2]) / g.bsize[2];
         int iz = (int)zrel;
         if (iz < 0) iz = 0;
@@ -493,12 +510,12 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
         if (zs < 0) zs = 0;
         else if (zs >= I.zsub) zs = I.zsub - 1;
       }
-      hkey = I.keybase + (ll * g.nbox + b) * I.zsub + zs;
+      hkey = dead ? -1 : I.keybase + (ll * g.nbox + b) * I.zsub + zs;
       zsbits = zs << kRankBits;
       // unireact: coin flips in table order, first success reacts; a
       // molecule that reacts takes no part in the pair search, so it is not
       // binned
-      int nu = sb ? 0 : tab->nuni[i];  // surface-bound molecules do not react
+      int nu = (sb || dead) ? 0 : tab->nuni[i];  // surface-bound molecules do not react
       if (nu > 0) {
         smolrng4 rb;
         int blk = -1;
@@ -1235,7 +1252,8 @@ __device__ int wg_scan(int v, int* s, int* tot) {
 
 constexpr int kLdsEvents = 1024;  // sorted event indices kept in LDS up to this count
 constexpr int kLdsKills = 2048;   // sorted kill slots kept in LDS up to this count
-constexpr int kMaxAds = 256;      // products adsorbed by their reborn surface check, per step
+constexpr int kMaxAds = 256;      // products adsorbed or absorbed by their reborn surface check, per step
+constexpr int kProdKilled = -1;   // Prod::sb of a product its reborn surface check absorbed
 
 // Sorted kill slots, read from LDS when they fit there, else from global.
 struct KillView {
@@ -1255,8 +1273,10 @@ struct KillView {
 // one per removal (the scan's second swap); removed molecules are met in the
 // scan's order and a moving one is appended to its new list then.  Last, the
 // desorbed molecules (the reference's resurrected ones, one serial number
-// each taken from the counter) are appended in reverse creation order (the
-// dead list's order after as many kills as desorptions).  One workgroup per
+// each taken from the counter) are appended in the dead list's order:
+// reverse creation order rotated by the number of absorbed molecules (every
+// kill of the scan rotates the resurrected entries by one; the desorbed
+// originals' kills rotate them back to where they were).  One workgroup per
 // simulation; the placement itself is sequential, in LDS, on few events.
 constexpr int kRelistMax = 512;
 constexpr int kAppendOv = 1 << 26;  // order values of box-list appends: above every live position
@@ -1383,14 +1403,28 @@ __global__ __launch_bounds__(kResolveThreads) void k_relist(const Inst* __restri
       }
       s_nl[ll] = b + a;
     }
-    // desorbed molecules: appended in reverse creation order (creation order =
-    // checksurfacebound's, i.e. by old slot)
-    for (int k = M - 1; k >= 0; k--) {
-      const int e = s_ev[k];
-      if (I.moves[e].kind != kMoveDesorb) continue;
-      place(ref(e), I.moves[e].list, s_nl[I.moves[e].list]++);
-      boxop(I.moves[e].list, I.moves[e].nbox, e, 1);
-      ndes++;
+    // desorbed molecules: the resurrected region of the dead list holds them
+    // in reverse creation order (creation order = checksurfacebound's, i.e.
+    // by old slot); every kill of the scan (desorbed originals and absorbed
+    // molecules) moves the region's first entry to its end, so they are
+    // appended rotated by the absorbed count
+    int nkill = 0;
+    for (int k = 0; k < M; k++) {
+      const int kd = I.moves[s_ev[k]].kind;
+      ndes += kd == kMoveDesorb;
+      nkill += kd == kMoveKill;
+    }
+    if (ndes > 0) {
+      int j = 0, first = nkill % ndes;  // index in reverse creation order of the first one appended
+      for (int pass = 0; pass < 2; pass++)
+        for (int k = M - 1; k >= 0; k--) {
+          const int e = s_ev[k];
+          if (I.moves[e].kind != kMoveDesorb) continue;
+          const int idx = j++ - pass * ndes;  // position in reverse creation order, once per pass
+          if (pass == 0 ? idx < first : idx >= first) continue;
+          place(ref(e), I.moves[e].list, s_nl[I.moves[e].list]++);
+          boxop(I.moves[e].list, I.moves[e].nbox, e, 1);
+        }
     }
     s_np = np;
     s_nop = nop;
@@ -1491,10 +1525,11 @@ __global__ __launch_bounds__(kResolveThreads) void k_relist(const Inst* __restri
 // The list sort that ends every step (simdocommands, smolsim.cpp:2571-2581,
 // a molsort of smolmolec.c:2943-3023) for products that their reborn surface
 // check adsorbed (molchangeident, smolmolec.c:2902-2906, put them in the list
-// of their bound state).  Those products sit among the products just
-// appended to the end of their solution lists, and nothing else changes list
-// in this sort.  Per list in order, the scan from the lowest such slot meets
-// each one, appends it to its bound list and fills its hole from the top of
+// of their bound state) or absorbed (molkill).  Those products sit among the
+// products just appended to the end of their solution lists, and nothing
+// else changes list in this sort.  Per list in order, the scan from the
+// lowest such slot meets each one, appends it to its bound list (an absorbed
+// one goes nowhere) and fills its hole from the top of
 // the list (swap from tail); holes are met in ascending order, each one
 // followed by the removed molecules the swaps bring into it; when the
 // remaining top slots are all removed they are met from the lowest upward
@@ -1573,9 +1608,9 @@ __device__ void end_of_step_sort(const Inst& I, Control* ctl, Glob* gl, const Re
       s_ppos[np] = pos;
       np++;
     };
-    auto meet = [&](int e) {  // append to the list of its bound state
+    auto meet = [&](int e) {  // append to the list of its bound state, or die
       const int L = s_alist[e];
-      place(s_arec[e], L, s_len[L]++);
+      if (L >= 0) place(s_arec[e], L, s_len[L]++);
     };
     for (int ll = 0; ll < nlist; ll++) {
       const int lo = s_lo[ll], r = s_cnt[ll];
@@ -2009,6 +2044,9 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
         // step (below); bit 2 marks it until then
         pr.sb = bnd | 4;
         atomicAdd((bnd & 3) == 1 ? &ctl->countf[i2] : &ctl->countk[i2], 1);
+      } else if (st == kSrfAbsorbed) {
+        // molkill: the list sort that ends the step removes it (below)
+        pr.sb = kProdKilled;
       } else if (st != kSrfOk || tab->listof[i2] != pr.list) {
         atomicOr(&ctl->err, kErrSurface);
         atomicOr(&gl->halt, 1);
@@ -2017,7 +2055,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       pr.x = pp[0];
       pr.y = pp[1];
       pr.z = pp[2];
-      atomicAdd(&s_dcount[pr.ident], 1);
+      if (pr.sb != kProdKilled) atomicAdd(&s_dcount[pr.ident], 1);
       I.prods[c] = pr;
     }
     __syncthreads();
@@ -2121,14 +2159,15 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
         const int pbx = pos2box(I.g, pr.x, pr.y, pr.z);
         const double pp[3] = {pr.x, pr.y, pr.z};
         store_pos<kF32>(B, I.g, rec, pbx, pp);
-        B.ident[rec] = pr.ident;
-        if (B.sb) B.sb[rec] = pr.sb & ~4;
-        if (kSurf && (pr.sb & 4)) {
+        const bool pkilled = kSurf && pr.sb == kProdKilled;
+        B.ident[rec] = pkilled ? 0 : pr.ident;  // an absorbed product stays as a dead record
+        if (B.sb) B.sb[rec] = pkilled ? 0 : pr.sb & ~4;
+        if (kSurf && (pkilled || (pr.sb & 4))) {
           const int a = atomicAdd(&s_nads, 1);
           if (a < kMaxAds) {
             s_aslot[a] = slot;
             s_arec[a] = rec;
-            s_alist[a] = I.sf.blist[pr.ident][bstate(pr.sb)];
+            s_alist[a] = pkilled ? -1 : I.sf.blist[pr.ident][bstate(pr.sb)];
           }
         }
         if (B.inst) B.inst[rec] = blockIdx.x;  // batch member (multi-simulation records)

@@ -601,7 +601,7 @@ __device__ __forceinline__ int srf_check(const DevSurf& sf, const Geometry& g, c
 // check may adsorb the new molecule again (dosurfinteract's adsorb branch for
 // the resurrected molecule): then *rebound = its (panel << 3) | state, else 0.
 // A flip to the other bound state returns kSrfFlipped with *rebound = the
-// new (panel << 3) | state and ident updated.
+// new (panel << 3) | state and ident updated; absorption returns kSrfAbsorbed.
 template <bool kCv>
 __device__ __forceinline__ int srf_bound(const DevSurf& sf, const Geometry& g, const double* difc,
                                          const double* difstep, double* pos, int* ident, int bound, uint64_t serial,
@@ -634,6 +634,7 @@ __device__ __forceinline__ int srf_bound(const DevSurf& sf, const Geometry& g, c
     *rebound = (p << 3) | ms2;
     return kSrfFlipped;
   }
+  if (o == kSaAbsorb) return kSrfAbsorbed;  // molkill (smolsurface.c:4994-4996)
   if (o != kSaRevdes && o != kSaIrrevdes) return kSrfUnsupported;
   const int face = ms2 == 0 ? kFaceFront : kFaceBack;
   double crsspt[3] = {pos[0], pos[1], pos[2]};

@@ -823,8 +823,8 @@ int SurfaceSet::build(const Geometry& g, int nspecies, const double (*difc_ms)[k
         const SrfDetail* det = S.det[i][MSsoln_][f].get();
         if (act != SAmult_) {
           if (det && det->datasrc[MSsoln_] == 3) a.newspec_fixed = det->newspec[MSsoln_];
-          if (act == SAabsorb_ || act == SAport_) {
-            snprintf(msg, msglen, "surface %s: absorbing and port actions are not implemented on the device path",
+          if (act == SAport_) {
+            snprintf(msg, msglen, "surface %s: port actions are not implemented on the device path",
                      S.name.c_str());
             return 2;
           }
@@ -844,11 +844,6 @@ int SurfaceSet::build(const Geometry& g, int nspecies, const double (*difc_ms)[k
           else o = ms2 == MSsoln_ ? SAtrans_ : ms2 == MSbsoln_ ? SAreflect_ : SAadsorb_;
           a.outcome[ms2] = o;
           bool possible = q.prob[ms2] > 0;
-          if (possible && o == SAabsorb_) {
-            snprintf(msg, msglen, "surface %s: absorption outcomes are not implemented on the device path",
-                     S.name.c_str());
-            return 2;
-          }
           if (possible && o == SAadsorb_) {
             int i2 = det->newspec[ms2];
             if (ms2 != MSfront_ && ms2 != MSback_) {
@@ -893,10 +888,10 @@ int SurfaceSet::build(const Geometry& g, int nspecies, const double (*difc_ms)[k
           else o = SAflip_;
           a.outcome[ms2] = o;
           // flips between the front and back states run on the device;
-          // absorption and flips to the up and down states do not
-          if (q.prob[ms2] > 0 && (o == SAabsorb_ || (o == SAflip_ && ms2 != MSfront_ && ms2 != MSback_))) {
-            snprintf(msg, msglen, "surface %s: absorption of surface-bound molecules and flips to the up or down "
-                     "state are not implemented on the device path", S.name.c_str());
+          // flips to the up and down states do not
+          if (q.prob[ms2] > 0 && o == SAflip_ && ms2 != MSfront_ && ms2 != MSback_) {
+            snprintf(msg, msglen, "surface %s: flips to the up or down state are not implemented on the device path",
+                     S.name.c_str());
             return 2;
           }
         }

@@ -231,6 +231,19 @@ def c3_adsorb_spec(n=2000, side=100.0, seed=33, ndisk=8, p_ads=0.3, k_des=2.0, k
     return spec
 
 
+def c3_absorb_spec(n=3000, side=60.0, seed=51, p_ads=0.5, k_des=20.0, kback=20.0, ndisk=4, rdisk=6.0):
+    """c3_adsorb_spec with absorbing faces: C that reaches the sphere from
+    inside dies, and so does B that reaches the membrane from below (B
+    products placed across the membrane by the back reaction included).
+    Desorption is fast, so absorptions and desorptions share steps (the
+    first molsort's dead-list order)."""
+    spec = c3_adsorb_spec(n=n, side=side, seed=seed, p_ads=p_ads, k_des=k_des, kback=kback, ndisk=ndisk,
+                          rdisk=rdisk)
+    spec["surfaces"][0]["actions"] = spec["surfaces"][0]["actions"] + [("back", "C", "absorb")]
+    spec["surfaces"][1]["actions"] = spec["surfaces"][1]["actions"] + [("back", "B", "absorb")]
+    return spec
+
+
 def c3_rates_spec(n=2000, side=60.0, seed=47, kback=20.0, ndisk=4, rdisk=6.0, k_ads=8.0, k_des=20.0,
                   k_flip=(10.0, 30.0), k_trans=(5.0, 2.0)):
     """Surface interactions given as rates (`rate`, converted to collision

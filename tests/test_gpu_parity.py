@@ -195,6 +195,27 @@ def test_surfaces_flips_between_lists_bit_exact():
     _run_pair(spec, 200, check_every=10)
 
 
+def test_surfaces_absorption_bit_exact():
+    """Absorbing faces: molecules killed by their surface check after
+    diffusion leave their lists in the first molsort (swap from tail), which
+    also orders the desorbed molecules appended in the same step (every kill
+    rotates the dead list's resurrected entries); products absorbed by their
+    own surface check leave in the list sort that ends the step."""
+    import smoldyn_amd as sa
+    spec = ss.c3_absorb_spec()
+    n0 = sum(c for _, c in spec["molecules"])
+    _run_pair(spec, 300, check_every=10)
+    dsim = ss.build_device(spec)
+    dsim.run_steps(300)
+    assert dsim.count("B", sa.MSall) < n0 // 2  # B below the membrane is absorbed
+
+
+def test_surfaces_absorption_injected_large_steps():
+    spec = ss.c3_absorb_spec(n=1500, side=40.0, seed=52, rdisk=4.0)
+    spec["dt"] = 0.1
+    _run_pair(spec, 40, check_every=4, inject=True)
+
+
 def test_surfaces_rate_given_injected_large_steps():
     spec = ss.c3_rates_spec(n=1500, side=40.0, seed=48, rdisk=4.0)
     spec["dt"] = 0.1
