@@ -539,8 +539,6 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   LCHECK(sim->dim == 3, fn, ECerror, "the device engine runs 3-D systems");
   int nsp = (int)sim->spname.size();
   LCHECK(nsp <= kMaxSpecies, fn, ECbounds, "too many species for the device tables (%d)", kMaxSpecies);
-  for (int w = 0; w < 6; w++)
-    LCHECK(sim->wtype[w] != 'a', fn, ECerror, "absorbing walls are not implemented on the device path");
   // molsupdatelists (smolmolec.c:2587-2651)
   std::vector<std::array<int, kMS>> lookup = sim->listlookup;
   std::vector<std::string> lists = sim->listname;
@@ -616,7 +614,10 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   LCHECK(nbox < (1L << 26), fn, ECbounds, "virtual box grid too large (%ld boxes)", nbox);
   g.nbox = (int)nbox;
   for (int w = 0; w < 6; w++)
-    g.wall[w] = sim->wtype[w] == 'r' ? kWallReflect : sim->wtype[w] == 'p' ? kWallPeriodic : kWallTransparent;
+    g.wall[w] = sim->wtype[w] == 'r'   ? kWallReflect
+                : sim->wtype[w] == 'p' ? kWallPeriodic
+                : sim->wtype[w] == 'a' ? kWallAbsorb
+                                       : kWallTransparent;
   // neighborZV type 7 reads the first dim entries of the per-wall periodic
   // flags as per-dimension flags (Zn.c:263, smolboxes.c:772-776)
   LCHECK(sim->accur >= 9, fn, ECerror, "accuracy below 9 selects neighbor lists the device path does not build");
@@ -792,8 +793,14 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
   // FP32 offsets round a stored position by up to 2^-24 of the box size,
   // far more than fixpt2panel's nudges off a panel (multiples of DBL_EPSILON):
   // a molecule stored next to a reflective panel could land on its far side
-  LCHECK(!(cfg.f32 && sim->surf.enabled()), fn, ECerror,
-         "FP32 box-offset records cannot be combined with surfaces (use precision 64)");
+  // absorbing walls without surfaces run on the surface variants with no
+  // panels (checkwalls kills need the molsorts those variants carry); with
+  // surfaces the walls are not applied at all (smolsim.cpp:2618-2630)
+  bool wall_absorb = false;
+  for (int w = 0; w < 6; w++) wall_absorb |= sim->wtype[w] == 'a';
+  wall_absorb = wall_absorb && !sim->surf.enabled();
+  LCHECK(!(cfg.f32 && (sim->surf.enabled() || wall_absorb)), fn, ECerror,
+         "FP32 box-offset records cannot be combined with surfaces or absorbing walls (use precision 64)");
   {  // tuning override of the pair search's z sub-bins per box (1, 2, 4 or 8)
     const char* e = getenv("SMOLAMD_ZSUB_MAX");
     cfg.zsub_max = e ? atoi(e) : 0;
@@ -874,8 +881,18 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
     for (int ll = 0; ll < kMaxList; ll++) hs.listdiffuse[ll] = st.listdiffuse[ll];
     hs.epsilon = sim->surf.epsilon;
   }
+  std::vector<int> wall_bpstart;  // absorbing walls: every box lists no panel
+  if (wall_absorb) {
+    wall_bpstart.assign((size_t)g.nbox + 1, 0);
+    hs.on = 1;
+    hs.walls = 1;
+    hs.nbox = g.nbox;
+    hs.bpstart = wall_bpstart.data();
+    for (int i = 0; i < kMaxSpecies; i++) hs.blist[i][0] = hs.blist[i][1] = -1;
+  }
   if (for_batch) {
-    LCHECK(!sim->surf.enabled(), fn, ECerror, "ensemble members cannot have surfaces");
+    LCHECK(!sim->surf.enabled() && !wall_absorb, fn, ECerror,
+           "ensemble members cannot have surfaces or absorbing walls");
     sim->prep.cfg = cfg;
     sim->prep.tab = t;
     sim->prep.geo = g;

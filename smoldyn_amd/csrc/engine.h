@@ -103,6 +103,7 @@ struct Control {
   int countf[kMaxSpecies];  // of which bound in the front state
   int countk[kMaxSpecies];  // of which bound in the back state
   int nmove;               // list changes queued by k_diffuse for the first molsort (surfaces)
+  int nwall;               // absorbing walls: survivors the second wall pass kills (k_diffuse -> k_resolve)
   int maxedges_seen;
   int rounds;              // matching rounds of the last step
   unsigned long long molsteps;  // sum over steps of the live molecules at step start
@@ -167,6 +168,10 @@ struct DevSurf {
   const DevSrfAct* actb;    // surface-bound molecules: [surface][species][front/back state], no face
   int listbound[kMaxList];  // lists holding surface-bound states (checksurfacebound)
   int blist[kMaxSpecies][2];  // list of each species in the front / back state (-1: none)
+  // no surfaces but absorbing walls: checkwalls with kills instead of the
+  // surface checks (the surface variants carry the molsorts a kill needs)
+  int walls;
+  int* wq;  // walls: destination records of survivors the step's second wall pass kills
 };
 
 // A list change queued by k_diffuse and applied by the first molsort of the
@@ -240,6 +245,7 @@ struct HostSurf {
   int listbound[kMaxList];
   int blist[kMaxSpecies][2];
   double epsilon;
+  int walls;  // no panels: absorbing walls run on the surface variants
 };
 
 DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const Geometry& geo,

@@ -190,6 +190,7 @@ struct DeviceState {
   DevSrfAct* d_act = nullptr;
   DevSrfAct* d_actb = nullptr;
   MoveEv* moves = nullptr;
+  int* wq = nullptr;  // absorbing walls: survivors the second wall pass kills
   double* inj = nullptr;
   long injcap = 0;
   int hnrec = 0;                     // records uploaded so far
@@ -286,6 +287,36 @@ __device__ __forceinline__ void apply_walls(const Geometry& g, double* p) {
   }
 }
 
+// checkwalls (smolwall.c:368-424) with absorbing walls, for one molecule:
+// the walls in the reference's order (low then high side of each axis),
+// reflective and periodic ones as apply_walls; an absorbing one kills a
+// molecule beyond it, or one inside with the Brownian-bridge probability
+// exp(-2 d_i d / step^2) that its path touched the wall (d_i, d: distances of
+// the step's start and end from the wall).  Draws: Philox keyed by (serial,
+// step, SMOLRNG_WALL, wall + 8 * pass), pass 1 for the wall pass after the
+// reactions.  Returns 1 when killed (the later walls no longer matter).
+__device__ __forceinline__ int walls_absorb(const Geometry& g, double* p, const double* px, double step,
+                                            uint64_t ser, uint32_t stepidx, uint32_t k0, uint32_t k1, int pass) {
+#pragma unroll
+  for (int w = 0; w < 6; w++) {
+    const int d = w >> 1, side = w & 1, t = g.wall[w];
+    const double wp = side ? g.high[d] : g.low[d];
+    if (t == kWallReflect) {
+      if (side ? p[d] > wp : p[d] < wp) p[d] = 2 * wp - p[d];
+    } else if (t == kWallPeriodic) {
+      if (side ? p[d] > wp : p[d] < wp) p[d] += (side ? g.low[d] : g.high[d]) - wp;
+    } else if (t == kWallAbsorb) {
+      const double diff = wp - p[d], difi = wp - px[d];
+      if ((!side && diff > 0) || (side && diff < 0)) return 1;
+      const smolrng4 r = smolrng_mol(k0, k1, ser, stepidx, SMOLRNG_WALL, (uint32_t)(w + 8 * pass));
+      if (smolrng_u01(r.v[0]) < exp(-2 * difi * diff / step / step)) return 1;
+    }
+  }
+  return 0;
+}
+
+constexpr int kSbWallMark = 1 << 30;  // Recs::sb of a survivor the second wall pass kills (| its queue index)
+
 __device__ __forceinline__ uint64_t atomic_load_u64(uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -359,6 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       const bool moved = tab->diffuselist[ll];
       bool flipped = false;  // a bound molecule moved to the other face of its panel
       bool dead = false;     // absorbed by a surface: not binned, its slot dies in the first molsort
+      bool wall2 = false;    // absorbing walls: killed by the second wall pass unless it reacts
       if (moved) {
         const double posx[3] = {pos[0], pos[1], pos[2]};
         double ds = tab->difstep[i];
@@ -379,6 +411,26 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
         pos[2] += ds * gg[2];
         if (!kSurf) {
           apply_walls(g, pos);
+        } else if (I.sf.walls) {
+          // no surfaces: checkwalls with absorbing walls; a killed molecule
+          // leaves its list in the first molsort
+          if (walls_absorb(g, pos, posx, ds, ser, step, I.key0, I.key1, 0)) {
+            dead = true;
+            mkind = kMoveKill;
+            atomicSub(&ctl->count[i], 1);
+          } else {
+            // the wall pass after the reactions (smolsim.cpp:2660-2666)
+            // flips the absorbing coins of every molecule again; one that
+            // does not react is still where this pass left it, so its coin
+            // is drawn here and the kill made at the end of the step
+            double p2[3] = {pos[0], pos[1], pos[2]};
+            wall2 = walls_absorb(g, p2, posx, ds, ser, step, I.key0, I.key1, 1);
+            if (p2[0] != pos[0] || p2[1] != pos[1] || p2[2] != pos[2]) {
+              // still beyond a wall after the first pass: not run here
+              atomicOr(&ctl->err, kErrSurface);
+              atomicOr((int*)&gl->halt, 1);
+            }
+          }
         } else if (I.sf.listdiffuse[ll]) {
           // with surfaces the walls are not applied (smolsim.cpp:2618-2630)
           int i2 = i, bnd = 0;
@@ -545,6 +597,16 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
           }
         }
       }
+      if (kSurf && wall2 && hkey >= 0) {  // (a first-order reactant dies anyway)
+        const int q = atomicAdd(&ctl->nwall, 1);
+        if (q < I.maxevents) {
+          I.sf.wq[q] = -1;
+          a.sb[r] = kSbWallMark | q;
+        } else {
+          atomicOr(&ctl->err, kErrEventOverflow);
+          atomicOr((int*)&gl->halt, 1);
+        }
+      }
       a.box[r] = hkey < 0 ? -1 : b;
     }
   }
@@ -704,7 +766,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ ins
   const Inst& I = insts[in];
   int i = a.ident[r];
   int ll = I.tab->listof[i];
-  const int sb = kSurf ? a.sb[r] : 0;
+  int sb = kSurf ? a.sb[r] : 0;
+  int wmark = -1;  // queue index of a survivor the second wall pass kills
+  if (kSurf && (sb & kSbWallMark)) {
+    wmark = sb & (kSbWallMark - 1);
+    sb = 0;
+  }
   if (kSurf && sb) ll = I.sf.blist[i][bstate(sb)];
   const int rk = a.rank[r];
   const int zs = rk >> kRankBits;
@@ -743,6 +810,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ ins
                             __int_as_float(i | (far << 16) | (zs << 17) | ((sb != 0) << 20)));
   if (kSurf) {
     b.sb[dst] = sb;
+    if (wmark >= 0) I.sf.wq[wmark] = dst;
     b.ov[dst] = lp < 0 ? -1 : lp;  // box lists are in live order until the first molsort changes them
     if (lp < 0) {  // list change of this step: the first molsort gives the slot
       I.moves[-lp - 2].dst = dst;
@@ -803,7 +871,8 @@ __device__ __forceinline__ int first_reaction(const DevTables* tab, int i1, int 
       if (!(smolrng_u01(rr.v[0]) < rx.prob)) continue;
     }
     // rxnXsurface (smolreact.c:3242,3289; not for wrapped pairs)
-    if (kSurf && !wrap && srf_rxnblocked<(kSurf > 1)>(sf, g, tab->difc, p1, p2, i1, i2, ser1, ser2, step, j, key0, key1))
+    if (kSurf && !wrap && sf.npnl > 0 &&
+        srf_rxnblocked<(kSurf > 1)>(sf, g, tab->difc, p1, p2, i1, i2, ser1, ser2, step, j, key0, key1))
       continue;
     if (!rx.permit_soln) continue;
     return r;
@@ -1252,7 +1321,7 @@ __device__ int wg_scan(int v, int* s, int* tot) {
 
 constexpr int kLdsEvents = 1024;  // sorted event indices kept in LDS up to this count
 constexpr int kLdsKills = 2048;   // sorted kill slots kept in LDS up to this count
-constexpr int kMaxAds = 256;      // products adsorbed or absorbed by their reborn surface check, per step
+constexpr int kMaxAds = 512;      // molecules leaving their lists in the sort that ends a step
 constexpr int kProdKilled = -1;   // Prod::sb of a product its reborn surface check absorbed
 
 // Sorted kill slots, read from LDS when they fit there, else from global.
@@ -1535,7 +1604,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_relist(const Inst* __restri
 // remaining top slots are all removed they are met from the lowest upward
 // and then downward from the top.  One thread runs the scan on the few
 // adsorbed products; the writes are spread over the workgroup.
-// ws_key/ws_val/ws_aux: LDS scratch of at least 2 * kMaxAds ints each.
+// ws_key, ws_val: LDS scratch of at least 2 * kMaxAds ints each; ws_aux: 6 * kMaxAds.
 constexpr int kNoSlot = -1;
 __device__ void end_of_step_sort(const Inst& I, Control* ctl, Glob* gl, const Recs& B, int* __restrict__ inv,
                                  int na, const int* s_aslot, const int* s_arec, const int* s_alist, const int* s_nf,
@@ -2037,8 +2106,15 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       double pp[3] = {pr.x, pr.y, pr.z};
       const double px0[3] = {pr.sx, pr.sy, pr.sz};
       int i2 = pr.ident, bnd = 0;
-      const int st = srf_check<(kSurf > 1)>(I.sf, I.g, tab->difc, pp, px0, &i2, ser0 + (uint64_t)c, step, SMOLRNG_SURF_REBORN,
-                               I.key0, I.key1, &bnd);
+      // draws keyed by the product's serial number (a reactant's, for
+      // products that keep them)
+      const uint64_t pser = pr.serno ? pr.serno : ser0 + (uint64_t)c;
+      // without panels: the wall pass after the reactions (absorbing walls)
+      const int st = I.sf.walls ? (walls_absorb(I.g, pp, px0, tab->difstep[i2], pser, step, I.key0, I.key1, 1)
+                                       ? kSrfAbsorbed
+                                       : kSrfOk)
+                                : srf_check<(kSurf > 1)>(I.sf, I.g, tab->difc, pp, px0, &i2, pser, step,
+                                                         SMOLRNG_SURF_REBORN, I.key0, I.key1, &bnd);
       if (st == kSrfAdsorbed && I.sf.blist[i2][bstate(bnd)] >= 0) {
         // bound; it leaves its solution list in the list sort that ends the
         // step (below); bit 2 marks it until then
@@ -2189,9 +2265,29 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     __syncthreads();
   }
   if (overflow) atomicOr(&ctl->err, kErrCapacity);
+  if (kSurf && I.sf.walls) {
+    // survivors the second wall pass kills (k_diffuse drew their coins): the
+    // ones that did not react die here and leave in the sort below
+    __syncthreads();
+    const int nq = min(ctl->nwall, I.maxevents);
+    for (int q = tid; q < nq; q += NT) {
+      const int rec = I.sf.wq[q];
+      if (rec < 0) continue;
+      const int i = B.ident[rec];
+      if (i <= 0) continue;  // a bimolecular reactant of this step
+      const int a = atomicAdd(&s_nads, 1);
+      if (a < kMaxAds) {
+        s_aslot[a] = I.L.base[tab->listof[i]] + B.lpos[rec];
+        s_arec[a] = rec;
+        s_alist[a] = -1;
+      }
+      B.ident[rec] = 0;
+      atomicSub(&s_dcount[i], 1);
+    }
+  }
   if (kSurf) {
     __syncthreads();
-    end_of_step_sort(I, ctl, gl, B, inv, s_nads, s_aslot, s_arec, s_alist, s_nf, s_pl, lv, s_ev, s_kill);
+    end_of_step_sort(I, ctl, gl, B, inv, s_nads, s_aslot, s_arec, s_alist, s_nf, s_pl, lv, s_kill, (int*)lk);
   }
   // ---- species counts (accumulated in LDS by the product loop), list
   // lengths, counters
@@ -2225,6 +2321,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     ctl->rounds = rounds;
     ctl->nedges = 0;
     ctl->nuni = 0;
+    ctl->nwall = 0;
     ctl->step = step + 1;
   }
 }
@@ -2412,7 +2509,7 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
     ok |= dalloc(&s->d_pnl, surf->npnl > 0 ? surf->npnl : 1) | dalloc(&s->d_bpstart, (size_t)surf->nbox + 1) |
           dalloc(&s->d_bplist, surf->bpstart[surf->nbox] > 0 ? surf->bpstart[surf->nbox] : 1) |
           dalloc(&s->d_act, (size_t)surf->nsrf * kMaxSpecies * 2) |
-          dalloc(&s->d_actb, (size_t)surf->nsrf * kMaxSpecies * 2) | dalloc(&s->moves, nev);
+          dalloc(&s->d_actb, (size_t)surf->nsrf * kMaxSpecies * 2) | dalloc(&s->moves, nev) | dalloc(&s->wq, nev);
   }
   if (ok) {
     snprintf(err, errlen, "device allocation failed (%d molecule records)", s->reccap);
@@ -2439,8 +2536,12 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
     (void)hipMemcpy(s->d_bpstart, surf->bpstart, sizeof(int) * (surf->nbox + 1), hipMemcpyHostToDevice);
     if (surf->bpstart[surf->nbox] > 0)
       (void)hipMemcpy(s->d_bplist, surf->bplist, sizeof(int) * surf->bpstart[surf->nbox], hipMemcpyHostToDevice);
-    (void)hipMemcpy(s->d_act, surf->act, sizeof(DevSrfAct) * surf->nsrf * kMaxSpecies * 2, hipMemcpyHostToDevice);
-    (void)hipMemcpy(s->d_actb, surf->actb, sizeof(DevSrfAct) * surf->nsrf * kMaxSpecies * 2, hipMemcpyHostToDevice);
+    if (surf->nsrf > 0) {
+      (void)hipMemcpy(s->d_act, surf->act, sizeof(DevSrfAct) * surf->nsrf * kMaxSpecies * 2, hipMemcpyHostToDevice);
+      (void)hipMemcpy(s->d_actb, surf->actb, sizeof(DevSrfAct) * surf->nsrf * kMaxSpecies * 2, hipMemcpyHostToDevice);
+    }
+    I.sf.walls = surf->walls;
+    I.sf.wq = s->wq;
     I.sf.on = 1;
     for (int ll = 0; ll < kMaxList; ll++) I.sf.listbound[ll] = surf->listbound[ll];
     for (int i = 0; i < kMaxSpecies; i++) {
@@ -2499,7 +2600,8 @@ void engine_destroy(DeviceState* s) {
   void* ptrs[] = {s->inv,     s->bn.count, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges,  s->cand,  s->estate,
                   s->accepted, s->bestkey, s->killed,  s->uniev,   s->log,       s->sortkey, s->sortval, s->killslot,
                   s->ctl,     s->tab,     s->gtab,    s->inj,     s->prods,     s->d_pnl,   s->d_bpstart,
-                  s->d_bplist, s->d_act,  s->d_actb,  s->moves,   s->dinst,   s->glob,    s->obs};
+                  s->d_bplist, s->d_act,  s->d_actb,  s->moves,   s->dinst,   s->glob,    s->obs,
+                  s->wq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; k++)

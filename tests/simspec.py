@@ -1,6 +1,7 @@
 """Build the same simulation in the CPU oracle and in the device engine.
 
-A spec is a dict: low/high bounds, walls ('r','p','t' per dimension), dt,
+A spec is a dict: low/high bounds, walls ('r','p','t','a' per dimension, or
+'lh' for the low and high sides), dt,
 seed, species {name: difc}, optional lists {list: [species or (species, state)]}, reactions
 [(name, [reactants], [products], rate)], product placements
 {name: (kind, param)}, molecules [(species, n)] placed uniformly from the
@@ -23,8 +24,8 @@ def build_oracle(spec, sfmt=False):
     program order) instead of the counter stream the device uses."""
     sim = ob.OSim(3, spec["low"], spec["high"])
     for d, w in enumerate(spec["walls"]):
-        sim.wall(d, 0, w)
-        sim.wall(d, 1, w)
+        sim.wall(d, 0, w[0])
+        sim.wall(d, 1, w[-1])
     sim.times(0.0, spec.get("tmax", 1e9), spec["dt"])
     if sfmt:
         sim.rngmode(ob.RNG_SFMT, 0)
@@ -85,8 +86,8 @@ def build_device(spec, update=True):
     import smoldyn_amd as sa
     sim = sa.Simulation(spec["low"], spec["high"])
     for d, w in enumerate(spec["walls"]):
-        sim.boundary(d, 0, w)
-        sim.boundary(d, 1, w)
+        sim.boundary(d, 0, w[0])
+        sim.boundary(d, 1, w[-1])
     sim.times(0.0, spec.get("tmax", 1e9), spec["dt"])
     sim.seed(spec["seed"])
     if "precision" in spec:
@@ -262,6 +263,19 @@ def c3_rates_spec(n=2000, side=60.0, seed=47, kback=20.0, ndisk=4, rdisk=6.0, k_
                                     ("A", "front", "front", "back", k_flip[0], False),
                                     ("A", "back", "back", "front", k_flip[1], False)]
     return spec
+
+
+def walls_absorb_spec(n=4000, side=40.0, seed=61, kback=20.0, dt=0.02):
+    """Absorbing walls (no surfaces): A+B<->C in a box whose x-low wall and
+    both z walls absorb, x-high reflects and y is periodic; molecules die
+    beyond an absorbing wall, or inside it with the Brownian-bridge
+    probability, in both wall passes of a step (after diffusion, and after
+    the reactions for every molecule)."""
+    return dict(low=[0.0] * 3, high=[side] * 3, walls=["ar", "p", "aa"], dt=dt, seed=seed,
+                species={"A": 2.0, "B": 2.0, "C": 1.0},
+                reactions=[("fwd", ["A", "B"], ["C"], 20.0), ("back", ["C"], ["A", "B"], kback)],
+                placement={"back": ("ratio", 1.5)},
+                molecules=[("A", n // 2), ("B", n // 2)])
 
 
 def c4_member(k, seed=500, n_lo=1000, n_hi=100000, side_lo=50.0, side_hi=400.0, dt=0.01, scale=1.0):

@@ -216,6 +216,24 @@ def test_surfaces_absorption_injected_large_steps():
     _run_pair(spec, 40, check_every=4, inject=True)
 
 
+def test_absorbing_walls_bit_exact():
+    """Absorbing walls: kills of the wall pass after diffusion leave in the
+    first molsort, kills of the pass after the reactions (products and every
+    molecule that did not react) in the list sort that ends the step."""
+    import smoldyn_amd as sa
+    spec = ss.walls_absorb_spec()
+    _run_pair(spec, 150, check_every=10)
+    dsim = ss.build_device(spec)
+    dsim.run_steps(150)
+    assert dsim.count("A", sa.MSall) + dsim.count("C", sa.MSall) < 2000
+
+
+def test_absorbing_walls_injected_large_steps():
+    spec = ss.walls_absorb_spec(n=2000, side=20.0, seed=62)
+    spec["dt"] = 0.2
+    _run_pair(spec, 30, check_every=3, inject=True)
+
+
 def test_surfaces_rate_given_injected_large_steps():
     spec = ss.c3_rates_spec(n=1500, side=40.0, seed=48, rdisk=4.0)
     spec["dt"] = 0.1
