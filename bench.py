@@ -44,12 +44,12 @@ def log(msg):
 # 153 B per molecule-step in all.  The layout's own bytes (design bytes: the
 # record also carries its serial number, live-list position and rank, and the
 # pair screen's packed FP32 copy) are reported beside them.
-ALGO_BYTES = {"diffuse": 54, "scan": 4, "scatter": 64, "pairs": 31, "exact": 0, "resolve": 0}
-DESIGN_BYTES = {"diffuse": 72, "scan": 16, "scatter": 112, "pairs": 24, "exact": 0, "resolve": 0}
+ALGO_BYTES = {"diffuse": 54, "scan": 4, "scatter": 64, "relist": 0, "pairs": 31, "exact": 0, "resolve": 0}
+DESIGN_BYTES = {"diffuse": 72, "scan": 16, "scatter": 112, "relist": 0, "pairs": 24, "exact": 0, "resolve": 0}
 STEP_ALGO_BYTES = 153
-KERNEL_NAMES = ["diffuse", "scan", "scatter", "event_overhead", "pairs", "exact", "resolve"]
-HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_hsum+k_hscan", "scatter": "k_scatter", "pairs": "k_pairs",
-                     "exact": "k_exact", "resolve": "k_resolve"}
+KERNEL_NAMES = ["diffuse", "scan", "scatter", "relist", "pairs", "exact", "resolve"]
+HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_hsum+k_hscan", "scatter": "k_scatter", "relist": "k_relist",
+                     "pairs": "k_pairs", "exact": "k_exact", "resolve": "k_resolve"}
 
 
 def c2_spec(n_per_species=500000, side=1000.0, seed=2026, kback=1.0):
@@ -179,7 +179,7 @@ def surfaces_leg(steps, warmup):
         raise SystemExit("device engine error bits 0x%x (surfaces workload)" % d1["err"])
     kt = sim.profile_steps(100)
     times = dict(zip(KERNEL_NAMES, [float(x) for x in kt]))
-    net = {k: max(v - times["event_overhead"], 0.0) for k, v in times.items() if k != "event_overhead"}
+    net = dict(times)  # each kernel's own dispatch start -> end
     return {"workload": "500k molecules (250k A + 250k B) in a reflecting sphere r=200, transmitting membrane "
                         "with 64 disk patches (A adsorbs with probability 0.2 per crossing, desorbs at rate 0.1), "
                         "A+B->C a=0.5, dt 0.002",
@@ -213,7 +213,7 @@ def build_member(spec, molperbox=None, update=False):
     return sim
 
 
-def ensemble_leg(members, steps, warmup, tune=True, sim_steps=2000):
+def ensemble_leg(members, steps, warmup, tune=True, sim_steps=2000, plain=True):
     """Batched sweep (one set of launches per step for all members) against
     the plain-GPU form (the same members, each its own engine, run one after
     another), both with HIP-graph stepping.  Throughput in simulations per
@@ -242,7 +242,7 @@ def ensemble_leg(members, steps, warmup, tune=True, sim_steps=2000):
     del sims
     log("batched %.2f ms/step; plain-GPU members next" % (t_batch * 1e3 / steps))
     t_plain = 0.0
-    for k in range(members):
+    for k in range(members if plain else 0):
         sim = build_member(specs[k], mpb[k], update=True)
         sim.run_steps(warmup)
         t0 = time.perf_counter()
@@ -251,10 +251,12 @@ def ensemble_leg(members, steps, warmup, tune=True, sim_steps=2000):
         del sim
     scale = sim_steps / steps
     sph = members * 3600.0 / (t_batch * scale)
-    sph_plain = members * 3600.0 / (t_plain * scale)
+    sph_plain = members * 3600.0 / (t_plain * scale) if plain else None
     return {"members": members, "molecules": int(sum(counts)), "timed_steps": steps, "sim_steps": sim_steps,
-            "sims_per_hour": sph, "plain_gpu_sims_per_hour": sph_plain, "speedup_vs_plain_gpu": sph / sph_plain,
-            "batched_ms_per_step": t_batch * 1e3 / steps, "plain_gpu_ms_per_member_step": t_plain * 1e3 / steps / members,
+            "sims_per_hour": sph, "plain_gpu_sims_per_hour": sph_plain,
+            "speedup_vs_plain_gpu": sph / sph_plain if plain else None,
+            "batched_ms_per_step": t_batch * 1e3 / steps,
+            "plain_gpu_ms_per_member_step": t_plain * 1e3 / steps / members if plain else None,
             "molecule_steps_per_s": molsteps / t_batch, "setup_s": setup, "err": err,
             "molperbox_by_size_class": {str(c): v for c, v in best.items()},
             "autotune_ms_per_step": {str(c): {str(m): t for m, t in tv.items()} for c, tv in timing.items()}}
@@ -386,7 +388,7 @@ def precision_leg(spec, steps, warmup, check_steps=100):
         raise SystemExit("device engine error bits 0x%x (FP32 records)" % d1["err"])
     kt = sim.profile_steps(100)
     times = dict(zip(KERNEL_NAMES, [float(x) for x in kt]))
-    net = {k: max(v - times["event_overhead"], 0.0) for k, v in times.items() if k != "event_overhead"}
+    net = dict(times)  # each kernel's own dispatch start -> end
     value = (d1["molsteps"] - d0["molsteps"]) / wall
     del sim
     s64, s32 = build_device(spec, 64), build_device(spec, 32)
@@ -526,6 +528,8 @@ def main():
                     help="members of the batched parameter sweep (0: skip the ensemble leg)")
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--ensemble-batched-only", action="store_true",
+                    help="skip the one-engine-per-member comparison (profiling runs)")
     ap.add_argument("--surface-steps", type=int, default=1000,
                     help="timed steps of the surfaces workload (0: skip)")
     ap.add_argument("--command-steps", type=int, default=2000,
@@ -572,14 +576,17 @@ def main():
         raise SystemExit("device engine error bits 0x%x" % d1["err"])
     molsteps = d1["molsteps"] - d0["molsteps"]
     value = molsteps / wall
-    # per-kernel device time with HIP events on the engine's stream
+    # per-kernel device time: each kernel's own start / stop HIP events
+    # (stamped at its dispatch's start and end) on the engine's stream
     kt = sim.profile_steps(args.profile_steps) if args.profile_steps > 0 else np.zeros(len(KERNEL_NAMES))
     nmol = sim.diagnostics()["nmol"]
     times = dict(zip(KERNEL_NAMES, [float(x) for x in kt]))
-    # each kernel's event interval includes one event overhead; the kernel's
-    # own duration is the interval minus that overhead
-    net = {k: max(v - times["event_overhead"], 0.0) for k, v in times.items() if k != "event_overhead"}
-    dom = max(net, key=lambda k: net[k])  # the longest kernel of the step
+    net = dict(times)
+    # the roofline's kernel: the longest of the per-molecule (HBM-streaming)
+    # kernels; the per-event kernels (exact test, resolve) have no
+    # per-molecule bytes and are reported beside it
+    longest = max(net, key=lambda k: net[k])
+    dom = max((k for k in net if ALGO_BYTES[k] > 0), key=lambda k: net[k])
     achieved = ALGO_BYTES[dom] * nmol / (net[dom] * 1e-3) / 1e9 if net[dom] > 0 else 0.0
     step_bytes = STEP_ALGO_BYTES * nmol
     counts = {s: sim.count(s) for s in ("A", "B", "C")}
@@ -621,7 +628,6 @@ def main():
                                "a~1.0, b/a 1.2, ~4 molecules per virtual box, FP64",
                    "molecules": 2 * args.n, "graph": use_graph},
         "device_ms_per_step": dev_ms / args.steps,
-        "kernel_ms_events": times,
         "kernel_ms": net,
         "roofline": {"bound": "hbm", "kernel": HIP_KERNEL_SYMBOL[dom], "kernel_ms": net[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -630,7 +636,10 @@ def main():
                      "molecules_per_launch": nmol,
                      "traffic": tr["bytes_per_launch"] if tr else None,
                      "traffic_note": ("rocprofv3 FETCH_SIZE+WRITE_SIZE per launch at %d molecules, scaled by a "
-                                      "plain-copy calibration (%s)" % (tr["molecules"], tr["source"])) if tr else None},
+                                      "plain-copy calibration (%s)" % (tr["molecules"], tr["source"])) if tr else None,
+                     "longest_kernel": {"kernel": HIP_KERNEL_SYMBOL[longest], "ms": net[longest],
+                                        "bound": "hbm" if ALGO_BYTES[longest] > 0 else
+                                        "latency: per-event work, no per-molecule bytes"}},
         "kernels": per_kernel,
         "step_algo_bytes": step_bytes,
         "step_frac_of_hbm": step_bytes / (dev_ms / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -651,7 +660,8 @@ def main():
     if args.ensemble_members > 0:
         del sim
         log("batched ensemble leg, %d members" % args.ensemble_members)
-        out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps, 20, tune=not args.no_autotune)
+        out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps, 20, tune=not args.no_autotune,
+                                       plain=not args.ensemble_batched_only)
     if args.sweep_members > 0:
         if "sim" in locals():
             del sim

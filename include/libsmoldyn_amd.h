@@ -234,11 +234,11 @@ int smolamdCommandTimeline(const char* script, int maxev, double* ev_time, int* 
    with the step captured in a HIP graph when use_graph != 0; *ms receives the
    device time (HIP events). */
 enum ErrorCode smolamdRunSteps(simptr sim, long nsteps, int use_graph, double* ms);
-/* Per-kernel average device time (ms) over nsteps timed steps, from HIP
-   events on the engine's stream: out[0..6] = diffuse, box scan, scatter,
-   event overhead (two back-to-back events with no kernel between them),
-   pair screen, exact pair test, resolve.  Each kernel interval includes one
-   event overhead. */
+/* Per-kernel average device time (ms) over nsteps timed steps: each kernel
+   is launched with its own start and stop HIP events, stamped at the
+   dispatch's start and end.  out[0..6] = diffuse, box scan (histogram sums
+   and scan), scatter, first molsort (models with surfaces or absorbing
+   walls, else 0), pair screen, exact pair test, resolve. */
 enum ErrorCode smolamdProfileSteps(simptr sim, long nsteps, double* out);
 /* Engine diagnostics: out[0]=live molecules, [1]=edges max, [2]=matching rounds
    of last step, [3]=error bits, [4]=next serial, [5]=step index,

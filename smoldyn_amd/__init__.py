@@ -399,8 +399,9 @@ class Simulation:
         return ms.value
 
     def profile_steps(self, n):
-        """Per-kernel ms: diffuse, scan, scatter, event overhead, pair screen,
-        exact test, resolve."""
+        """Per-kernel ms from each kernel's own dispatch start / stop events:
+        diffuse, scan, scatter, first molsort (relist), pair screen, exact
+        test, resolve."""
         out = np.zeros(7)
         _check(self.L.smolamdProfileSteps(self.p, n, _dp(out)), "smolamdProfileSteps")
         return out

@@ -280,8 +280,10 @@ int engine_step_noalloc(DeviceState* s, const double* inject, long ninject);
 long engine_noalloc_inject_max();
 // Many steps; returns 0 or an error bit set.
 int engine_run(DeviceState* s, long nsteps, int use_graph);
-// Steps with HIP events between the kernels; out[0..6] = average ms per step of
-// diffuse, box scan, scatter, event overhead, pair screen, exact test, resolve.
+// Steps whose kernels carry their own start / stop HIP events (stamped at
+// each dispatch's start and end); out[0..6] = average ms per step of
+// diffuse, box scan, scatter, first molsort (surfaces), pair screen, exact
+// test, resolve.
 int engine_profile(DeviceState* s, long nsteps, double* out);
 int engine_sync_control(DeviceState* s, Control* out);
 // Asynchronous stepping for the command loop: enqueue steps, enqueue a copy of

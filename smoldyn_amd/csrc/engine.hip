@@ -31,6 +31,7 @@
 // Arithmetic follows the reference operation by operation and the library is
 // built with -ffp-contract=off, so FP64 positions match the CPU path bit for
 // bit.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
@@ -149,6 +150,10 @@ struct Glob {
   int ncand;  // screened candidate pairs of the step (k_pairs -> k_exact), reset by k_resolve
 };
 
+// kernels timed by engine_profile, each with its own start / stop events
+enum TimedSlot : int { kTDiffuse, kTHsum, kTScan2, kTHscan, kTScatter, kTRelist, kTPairs, kTExact, kTResolve,
+                       kTimedSlots };
+
 struct DeviceState {
   int ninst = 0;
   bool multi = false;                // more than one simulation: records carry their instance
@@ -201,6 +206,8 @@ struct DeviceState {
   hipGraphExec_t graph[2] = {};
   int graph_ready[2] = {0, 0};
   hipEvent_t ev[16] = {};
+  hipEvent_t kev[2 * kTimedSlots] = {};  // start / stop of each timed kernel (engine_profile)
+  int timed_mask = 0;                    // slots the last timed step launched
   double last_ms = 0;
   int* obs = nullptr;                // observation rows (species counters between steps)
 };
@@ -2585,6 +2592,7 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
     return nullptr;
   }
   for (int k = 0; k < 16; k++) (void)hipEventCreate(&s->ev[k]);
+  for (int k = 0; k < 2 * kTimedSlots; k++) (void)hipEventCreate(&s->kev[k]);
   return s;
 }
 
@@ -2595,6 +2603,17 @@ DeviceState* engine_create(const EngineConfig& cfg, const DevTables& tab, const 
 
 void engine_destroy(DeviceState* s) {
   if (!s) return;
+  // the stream's work ends, its graphs and events go, then the memory they used
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (int k = 0; k < 2; k++)
+    if (s->graph_ready[k]) (void)hipGraphExecDestroy(s->graph[k]);
+  if (s->stream) {
+    for (int k = 0; k < 16; k++)
+      if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    for (int k = 0; k < 2 * kTimedSlots; k++)
+      if (s->kev[k]) (void)hipEventDestroy(s->kev[k]);
+    (void)hipStreamDestroy(s->stream);
+  }
   free_recs(&s->buf[0]);
   free_recs(&s->buf[1]);
   void* ptrs[] = {s->inv,     s->bn.count, s->bn.boff, s->bn.bstart, s->bn.spk, s->edges,  s->cand,  s->estate,
@@ -2604,13 +2623,6 @@ void engine_destroy(DeviceState* s) {
                   s->wq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  for (int k = 0; k < 2; k++)
-    if (s->graph_ready[k]) (void)hipGraphExecDestroy(s->graph[k]);
-  if (s->stream) {
-    for (int k = 0; k < 16; k++)
-      if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
-    (void)hipStreamDestroy(s->stream);
-  }
   delete s;
 }
 
@@ -2740,34 +2752,36 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   auto kres = sv == 2 ? (f ? k_resolve<true, 2> : k_resolve<false, 2>)
               : sv == 1 ? (f ? k_resolve<true, 1> : k_resolve<false, 1>)
                         : (f ? k_resolve<true, 0> : k_resolve<false, 0>);
-  if (timed) (void)hipEventRecord(s->ev[0], st);
-  hipLaunchKernelGGL(kdiff, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, s->bn.count, s->gtab,
-                     s->ngauss - 1, dinj, s->inv);
-  if (timed) (void)hipEventRecord(s->ev[1], st);
+  // timed: each kernel carries start / stop events stamped at its own
+  // dispatch start and end (the timestamps a kernel trace reads)
+  hipEvent_t* kev = s->kev;
+  s->timed_mask = 0;
+  auto go = [&](int slot, auto kernel, dim3 grid, dim3 block, auto... args) {
+    if (timed) {
+      hipExtLaunchKernelGGL(kernel, grid, block, 0, st, kev[2 * slot], kev[2 * slot + 1], 0, args...);
+      s->timed_mask |= 1 << slot;
+    } else {
+      hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+    }
+  };
+  go(kTDiffuse, kdiff, dim3(nb), dim3(kBlock), s->dinst, s->glob, A, s->bn.count, (const double*)s->gtab,
+     s->ngauss - 1, dinj, s->inv);
   {
     const int ntile = (s->nscan + kHTile - 1) / kHTile;
-    hipLaunchKernelGGL(k_hsum, dim3(ntile), dim3(kHThreads), 0, st, s->bn.count, s->bn.boff, s->nscan);
+    go(kTHsum, k_hsum, dim3(ntile), dim3(kHThreads), (const int*)s->bn.count, s->bn.boff, s->nscan);
     const int scanned = ntile > kHFusedTiles;
-    if (scanned) hipLaunchKernelGGL(k_scan2, dim3(1), dim3(kScanBlock), 0, st, s->bn.boff, ntile);
-    hipLaunchKernelGGL(k_hscan, dim3(ntile), dim3(kHThreads), 0, st, s->bn.count, s->bn.boff, s->bn.bstart, s->nscan,
-                       scanned);
+    if (scanned) go(kTScan2, k_scan2, dim3(1), dim3(kScanBlock), s->bn.boff, ntile);
+    go(kTHscan, k_hscan, dim3(ntile), dim3(kHThreads), s->bn.count, (const int*)s->bn.boff, s->bn.bstart, s->nscan,
+       scanned);
   }
-  if (timed) (void)hipEventRecord(s->ev[2], st);
-  hipLaunchKernelGGL(kscat, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, A, B, s->bn, s->inv);
+  go(kTScatter, kscat, dim3(nb), dim3(kBlock), s->dinst, s->glob, A, B, s->bn, s->inv);
   if (s->surf_on)  // first molsort: list changes from adsorption and desorption
-    hipLaunchKernelGGL(k_relist, dim3(s->ninst), dim3(kResolveThreads), 0, st, s->dinst, s->glob, A, B, s->inv,
-                       s->reccap, s->bn);
-  if (timed) (void)hipEventRecord(s->ev[3], st);
-  if (timed) (void)hipEventRecord(s->ev[4], st);
-  hipLaunchKernelGGL(kpair, dim3(nb), dim3(kBlock), 0, st, s->dinst, s->glob, B, s->bn, s->nscan, s->cand,
-                     s->candcap);
-  if (timed) (void)hipEventRecord(s->ev[5], st);
-  hipLaunchKernelGGL(kexact, dim3(kExactBlocks), dim3(kBlock), 0, st, s->dinst, s->glob, B, s->bn, s->cand,
-                     s->candcap);
-  if (timed) (void)hipEventRecord(s->ev[6], st);
-  hipLaunchKernelGGL(kres, dim3(s->ninst), dim3(kResolveThreads), 0, st, s->dinst, s->glob, A, B, s->inv, s->bn,
-                     s->nscan, s->bestkey, s->killed, s->reccap);
-  if (timed) (void)hipEventRecord(s->ev[7], st);
+    go(kTRelist, k_relist, dim3(s->ninst), dim3(kResolveThreads), s->dinst, s->glob, A, B, s->inv, s->reccap,
+       s->bn);
+  go(kTPairs, kpair, dim3(nb), dim3(kBlock), s->dinst, s->glob, B, s->bn, s->nscan, s->cand, s->candcap);
+  go(kTExact, kexact, dim3(kExactBlocks), dim3(kBlock), s->dinst, s->glob, B, s->bn, s->cand, s->candcap);
+  go(kTResolve, kres, dim3(s->ninst), dim3(kResolveThreads), s->dinst, s->glob, A, B, s->inv, s->bn, s->nscan,
+     s->bestkey, s->killed, s->reccap);
 }
 
 // OR of the error bits of all simulations
@@ -2943,19 +2957,22 @@ int engine_read_obs(DeviceState* s, int nrow, int* out) {
 // overhead (two events with no kernel between them), pair screen, exact
 // test, resolve.
 int engine_profile(DeviceState* s, long nsteps, double* out) {
-  constexpr int kN = 7;
-  double acc[kN] = {0, 0, 0, 0, 0, 0, 0};
+  // outputs: diffuse, scan (k_hsum + k_scan2 + k_hscan), scatter, relist,
+  // pairs, exact, resolve; kernels a step did not launch count 0
+  static const int group[kTimedSlots] = {0, 1, 1, 1, 2, 3, 4, 5, 6};
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   for (long k = 0; k < nsteps; k++) {
     launch_step(s, s->cur, nullptr, 1);
     HIPOK(hipStreamSynchronize(s->stream));
     s->cur ^= 1;
-    for (int j = 0; j < kN; j++) {
+    for (int j = 0; j < kTimedSlots; j++) {
+      if (!(s->timed_mask & (1 << j))) continue;
       float ms = 0;
-      HIPOK(hipEventElapsedTime(&ms, s->ev[j], s->ev[j + 1]));
-      acc[j] += ms;
+      HIPOK(hipEventElapsedTime(&ms, s->kev[2 * j], s->kev[2 * j + 1]));
+      acc[group[j]] += ms;
     }
   }
-  for (int j = 0; j < kN; j++) out[j] = acc[j] / (double)(nsteps > 0 ? nsteps : 1);
+  for (int j = 0; j < 7; j++) out[j] = acc[j] / (double)(nsteps > 0 ? nsteps : 1);
   return read_errors(s);
 }
 
