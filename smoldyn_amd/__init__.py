@@ -407,7 +407,7 @@ class Simulation:
         return out
 
     def count(self, species=None, state=MSsoln):
-        n = self.L.smolGetMoleculeCount(self.p, species.encode() if species else None, state)
+        n = self.L.smolGetMoleculeCount(self.p, species.encode() if species else b"all", state)
         _check(n if n < 0 else 0, "smolGetMoleculeCount")
         return n
 

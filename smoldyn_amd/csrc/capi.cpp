@@ -125,6 +125,25 @@ void set_error(const char* func, ErrorCode code, const char* fmt, ...) {
   smolSetError(func, code, msg, "");
 }
 
+// smolSetErrorNT (libsmoldyn.cpp:111-122): the error recorded without the
+// debug report; returns the code
+int set_error_nt(const char* func, ErrorCode code, const char* msg) {
+  if (code != ECsame) {
+    g_errcode = code;
+    g_warncode = code >= ECwarning ? code : ECok;
+    snprintf(g_errstr, kStrLen, "%s", msg ? msg : "");
+  }
+  snprintf(g_errfunc, kStrLen, "%s", func ? func : "");
+  return (int)code;
+}
+
+// LCHECK(..., ECsame, NULL) of the reference: the error an inner lookup
+// recorded stays (code and message), now named for the calling function
+ErrorCode same_error(const char* func) {
+  smolSetError(func, ECsame, nullptr, "");
+  return g_errcode;
+}
+
 #define LCHECK(cond, func, code, ...)     \
   do {                                    \
     if (!(cond)) {                        \
@@ -1387,16 +1406,28 @@ enum ErrorCode smolAddSpecies(simptr sim, const char* species, const char* molli
   return ECok;
 }
 
+// smolGetSpeciesIndexNT (libsmoldyn.cpp:851-866): the index, or the error
+// code, recorded with the reference's messages and no debug report
 int smolGetSpeciesIndexNT(simptr sim, const char* species) {
-  if (!sim || !species) return (int)ECmissing;
-  if (!strcmp(species, "all")) return (int)ECall;
+  const char* fn = "smolGetSpeciesIndexNT";
+  if (!sim) return set_error_nt(fn, ECmissing, "missing sim");
+  if (!species) return set_error_nt(fn, ECmissing, "missing species name");
+  if (sim->spname.size() <= 1) return set_error_nt(fn, ECnonexist, "no species defined");
+  if (!strcmp(species, "all")) return set_error_nt(fn, ECall, "species cannot be 'all'");
   int i = species_index(sim, species);
-  return i < 0 ? (int)ECnonexist : i;
+  if (i <= 0) {
+    char buf[kStrLen];
+    snprintf(buf, sizeof(buf), "species '%s' not found", species);
+    return set_error_nt(fn, ECnonexist, buf);
+  }
+  return i;
 }
 
+// smolGetSpeciesIndex (libsmoldyn.cpp:834-848): the NT lookup's error,
+// named for this function
 int smolGetSpeciesIndex(simptr sim, const char* species) {
   int i = smolGetSpeciesIndexNT(sim, species);
-  if (i < 0) smolSetError("smolGetSpeciesIndex", (ErrorCode)i, "species not found", "");
+  if (i <= 0) return (int)same_error("smolGetSpeciesIndex");
   return i;
 }
 
@@ -1418,12 +1449,13 @@ enum ErrorCode smolSetSpeciesMobility(simptr sim, const char* species, enum Mole
   LCHECK(!drift && !difmatrix, fn, ECerror, "drift and anisotropic diffusion are not implemented on the device path");
   LCHECK(!sim->started, fn, ECerror, "mobility is fixed once stepping started");
   std::vector<int> idx;
-  if (species && !strcmp(species, "all")) {
+  const int isp = smolGetSpeciesIndexNT(sim, species);
+  if (isp == (int)ECall) {
+    smolClearError();
     for (size_t i = 1; i < sim->spname.size(); i++) idx.push_back((int)i);
   } else {
-    int i = species_index(sim, species);
-    LCHECK(i > 0, fn, ECnonexist, "species not found");
-    idx.push_back(i);
+    if (isp <= 0) return same_error(fn);
+    idx.push_back(isp);
   }
   for (int i : idx) {
     if (state == MSall)
@@ -1450,29 +1482,43 @@ enum ErrorCode smolAddMolList(simptr sim, const char* mollist) {
   return ECok;
 }
 
+// smolGetMolListIndexNT (libsmoldyn.cpp:987-999)
 int smolGetMolListIndexNT(simptr sim, const char* mollist) {
-  if (!sim || !mollist) return (int)ECmissing;
+  const char* fn = "smolGetMolListIndexNT";
+  if (!sim) return set_error_nt(fn, ECmissing, "missing sim");
+  if (!mollist) return set_error_nt(fn, ECmissing, "missing mollist");
+  if (sim->listname.empty()) return set_error_nt(fn, ECnonexist, "no molecule lists defined");
+  if (!strcmp(mollist, "all")) return set_error_nt(fn, ECall, "molecule list cannot be 'all'");
   int l = list_index(sim, mollist);
-  return l < 0 ? (int)ECnonexist : l;
+  if (l < 0) return set_error_nt(fn, ECnonexist, "list name not recognized");
+  return l;
 }
 
 int smolGetMolListIndex(simptr sim, const char* mollist) {
   int l = smolGetMolListIndexNT(sim, mollist);
-  if (l < 0) smolSetError("smolGetMolListIndex", (ErrorCode)l, "list not found", "");
+  if (l < 0) return (int)same_error("smolGetMolListIndex");
   return l;
 }
 
 enum ErrorCode smolSetMolList(simptr sim, const char* species, enum MolecState state, const char* mollist) {
   const char* fn = "smolSetMolList";
   LCHECK(sim, fn, ECmissing, "missing sim");
-  int i = species_index(sim, species);
-  LCHECK(i > 0, fn, ECnonexist, "species not found");
-  int ll = list_index(sim, mollist);
-  LCHECK(ll >= 0, fn, ECnonexist, "molecule list not found");
-  if (state == MSall)
-    for (int ms = 0; ms < kMS; ms++) sim->listlookup[i][ms] = ll;
-  else
-    sim->listlookup[i][state == MSbsoln ? MSsoln : state] = ll;
+  int i = smolGetSpeciesIndexNT(sim, species);
+  if (i == (int)ECall) {
+    smolClearError();
+    i = -5;  // every species
+  } else if (i <= 0) {
+    return same_error(fn);
+  }
+  LCHECK((state >= 0 && state < kMS) || state == MSbsoln || state == MSall, fn, ECsyntax, "invalid state");
+  int ll = smolGetMolListIndexNT(sim, mollist);
+  if (ll < 0) return same_error(fn);
+  for (int k = (i == -5 ? 1 : i); k < (i == -5 ? (int)sim->spname.size() : i + 1); k++) {
+    if (state == MSall)
+      for (int ms = 0; ms < kMS; ms++) sim->listlookup[k][ms] = ll;
+    else
+      sim->listlookup[k][state == MSbsoln ? MSsoln : state] = ll;
+  }
   invalidate(sim);
   return ECok;
 }
@@ -1491,8 +1537,8 @@ enum ErrorCode smolAddSolutionMolecules(simptr sim, const char* species, int num
   const char* fn = "smolAddSolutionMolecules";
   LCHECK(sim, fn, ECmissing, "missing sim");
   LCHECK(!sim->started, fn, ECerror, "molecules can only be added before stepping starts");
-  int i = species_index(sim, species);
-  LCHECK(i > 0, fn, ECnonexist, "species not found");
+  int i = smolGetSpeciesIndexNT(sim, species);
+  if (i <= 0) return same_error(fn);
   LCHECK(number >= 0, fn, ECbounds, "number cannot be negative");
   double lo[3], hi[3];
   for (int d = 0; d < 3; d++) {
@@ -1538,16 +1584,14 @@ int smolGetMoleculeCount(simptr sim, const char* species, enum MolecState state)
   } else {
     for (auto& pm : sim->pending) counts[pm.ident]++;
   }
-  if (!species || !strcmp(species, "all")) {
+  const int i = smolGetSpeciesIndexNT(sim, species);
+  if (i == (int)ECall) {  // every species
+    smolClearError();
     int s = 0;
-    for (int i = 1; i < kMaxSpecies; i++) s += counts[i];
+    for (int k = 1; k < kMaxSpecies; k++) s += counts[k];
     return s;
   }
-  int i = species_index(sim, species);
-  if (i < 0) {
-    set_error(fn, ECnonexist, "species not found");
-    return (int)ECnonexist;
-  }
+  if (i <= 0) return (int)same_error(fn);
   return counts[i];
 }
 
@@ -1842,16 +1886,21 @@ enum ErrorCode smolAddSurface(simptr sim, const char* surface) {
   return ECok;
 }
 
+// smolGetSurfaceIndexNT (libsmoldyn.cpp:1274-1286)
 int smolGetSurfaceIndexNT(simptr sim, const char* surface) {
-  if (!sim || !surface) return (int)ECmissing;
-  if (!strcmp(surface, "all")) return (int)ECall;
+  const char* fn = "smolGetSurfaceIndexNT";
+  if (!sim) return set_error_nt(fn, ECmissing, "missing sim");
+  if (!surface) return set_error_nt(fn, ECmissing, "missing surface");
+  if (sim->surf.srf.empty()) return set_error_nt(fn, ECnonexist, "no surfaces defined");
+  if (!strcmp(surface, "all")) return set_error_nt(fn, ECall, "surface cannot be 'all'");
   int s = sim->surf.find_surface(surface);
-  return s < 0 ? (int)ECnonexist : s;
+  if (s < 0) return set_error_nt(fn, ECnonexist, "surface not found");
+  return s;
 }
 
 int smolGetSurfaceIndex(simptr sim, const char* surface) {
   int s = smolGetSurfaceIndexNT(sim, surface);
-  if (s < 0) smolSetError("smolGetSurfaceIndex", (ErrorCode)s, "surface not found", "");
+  if (s < 0) return (int)same_error("smolGetSurfaceIndex");
   return s;
 }
 
@@ -1861,14 +1910,20 @@ enum ErrorCode smolSetSurfaceAction(simptr sim, const char* surface, enum PanelF
   const char* fn = "smolSetSurfaceAction";
   LCHECK(sim, fn, ECmissing, "missing sim");
   int s = smolGetSurfaceIndexNT(sim, surface);
-  LCHECK(s >= 0 || s == (int)ECall, fn, ECnonexist, "surface not found");
+  if (s == (int)ECall) smolClearError();
+  else if (s < 0) return same_error(fn);
   LCHECK(face == PFfront || face == PFback || face == PFboth, fn, ECbounds, "invalid face");
   int i = smolGetSpeciesIndexNT(sim, species);
-  LCHECK(i > 0 || i == (int)ECall, fn, ECnonexist, "species not found");
-  if (i == (int)ECall) i = -5;
+  if (i == (int)ECall) {
+    smolClearError();
+    i = -5;
+  } else if (i <= 0) {
+    return same_error(fn);
+  }
   LCHECK((state >= 0 && state < kMS) || state == MSall, fn, ECbounds, "invalid state");
   LCHECK(action >= 0 && action <= SAmult, fn, ECbounds, "invalid action");
-  if (newspecies && newspecies[0]) LCHECK(species_index(sim, newspecies) > 0, fn, ECnonexist, "unrecognized new species name");
+  if (newspecies && newspecies[0])
+    LCHECK(smolGetSpeciesIndexNT(sim, newspecies) > 0, fn, ECnonexist, "unrecognized new species name");
   int nsp = (int)sim->spname.size();
   for (int k = 0; k < (int)sim->surf.srf.size(); k++)
     if (s < 0 || k == s) sim->surf.set_action(k, i, nsp, state, face, action);
@@ -1882,10 +1937,15 @@ enum ErrorCode smolSetSurfaceRate(simptr sim, const char* surface, const char* s
   const char* fn = "smolSetSurfaceRate";
   LCHECK(sim, fn, ECmissing, "missing sim");
   int s = smolGetSurfaceIndexNT(sim, surface);
-  LCHECK(s >= 0 || s == (int)ECall, fn, ECnonexist, "surface not found");
+  if (s == (int)ECall) smolClearError();
+  else if (s < 0) return same_error(fn);
   int i = smolGetSpeciesIndexNT(sim, species);
-  LCHECK(i > 0 || i == (int)ECall, fn, ECnonexist, "species not found");
-  if (i == (int)ECall) i = -5;
+  if (i == (int)ECall) {
+    smolClearError();
+    i = -5;
+  } else if (i <= 0) {
+    return same_error(fn);
+  }
   LCHECK(state >= 0 && state < kMS, fn, ECbounds, "invalid state");
   LCHECK(state1 >= 0 && state1 < kMS1, fn, ECbounds, "invalid state1");
   LCHECK(state == MSsoln || state1 == MSsoln || state1 == MSbsoln || state1 == state, fn, ECsyntax,
@@ -1894,7 +1954,7 @@ enum ErrorCode smolSetSurfaceRate(simptr sim, const char* surface, const char* s
   LCHECK(state1 != state2, fn, ECsyntax, "cannot set rate for state1 = state2");
   int i2 = -5;
   if (newspecies && newspecies[0]) {
-    i2 = species_index(sim, newspecies);
+    i2 = smolGetSpeciesIndexNT(sim, newspecies);
     LCHECK(i2 > 0, fn, ECerror, "invalid newspecies");
   }
   LCHECK(rate >= 0, fn, ECbounds, "rate needs to be non-negative");
@@ -1914,7 +1974,7 @@ enum ErrorCode smolAddPanel(simptr sim, const char* surface, enum PanelShape pan
   const char* fn = "smolAddPanel";
   LCHECK(sim, fn, ECmissing, "missing sim");
   int s = smolGetSurfaceIndexNT(sim, surface);
-  LCHECK(s >= 0, fn, ECnonexist, "surface not found");
+  if (s < 0) return same_error(fn);
   LCHECK(panelshape >= 0 && panelshape < PSall, fn, ECnonexist, "invalid panel shape");
   if (panelshape == PSrect) LCHECK(axisstring, fn, ECmissing, "missing axisstring");
   LCHECK(params, fn, ECmissing, "missing params");
@@ -1957,7 +2017,7 @@ enum ErrorCode smolSetPanelJump(simptr sim, const char* surface, const char* pan
   const char* fn = "smolSetPanelJump";
   LCHECK(sim, fn, ECmissing, "missing sim");
   int s = smolGetSurfaceIndexNT(sim, surface);
-  LCHECK(s >= 0, fn, ECnonexist, "surface not found");
+  if (s < 0) return same_error(fn);
   LCHECK(panel1 && panel2, fn, ECmissing, "missing panel name");
   int p1 = sim->surf.find_panel(s, panel1), p2 = sim->surf.find_panel(s, panel2);
   LCHECK(p1 >= 0 && p2 >= 0, fn, ECnonexist, "panel not found");
@@ -2001,15 +2061,15 @@ enum ErrorCode smolAddReaction(simptr sim, const char* reaction, const char* rea
   r.name = reaction;
   int order = 0;
   if (reactant1 && reactant1[0]) {
-    int i = species_index(sim, reactant1);
-    LCHECK(i > 0, fn, ECnonexist, "reactant 1 not found");
+    int i = smolGetSpeciesIndexNT(sim, reactant1);
+    if (i <= 0) return same_error(fn);
     r.rct[order] = i;
     r.rctstate[order] = rstate1;
     order++;
   }
   if (reactant2 && reactant2[0]) {
-    int i = species_index(sim, reactant2);
-    LCHECK(i > 0, fn, ECnonexist, "reactant 2 not found");
+    int i = smolGetSpeciesIndexNT(sim, reactant2);
+    if (i <= 0) return same_error(fn);
     r.rct[order] = i;
     r.rctstate[order] = rstate2;
     order++;
@@ -2018,8 +2078,8 @@ enum ErrorCode smolAddReaction(simptr sim, const char* reaction, const char* rea
   LCHECK(nproduct >= 0 && nproduct <= kMaxProd, fn, ECbounds, "nproduct out of range");
   r.nprod = nproduct;
   for (int p = 0; p < nproduct; p++) {
-    int i = species_index(sim, productspecies[p]);
-    LCHECK(i > 0, fn, ECnonexist, "product not found");
+    int i = smolGetSpeciesIndexNT(sim, productspecies[p]);
+    if (i <= 0) return same_error(fn);
     r.prd[p] = i;
     r.prdstate[p] = productstates ? productstates[p] : MSsoln;
   }
@@ -2044,22 +2104,31 @@ enum ErrorCode smolAddReaction(simptr sim, const char* reaction, const char* rea
   return ECok;
 }
 
+// smolGetReactionIndexNT (libsmoldyn.cpp:1846-1867)
 int smolGetReactionIndexNT(simptr sim, int* orderptr, const char* reaction) {
-  if (!sim || !reaction) return (int)ECmissing;
-  for (int o = 0; o < 3; o++) {
-    if (orderptr && *orderptr >= 0 && *orderptr != o) continue;
+  const char* fn = "smolGetReactionIndexNT";
+  if (!sim) return set_error_nt(fn, ECmissing, "missing sim");
+  if (!reaction) return set_error_nt(fn, ECmissing, "missing reaction");
+  if (!strcmp(reaction, "all")) return set_error_nt(fn, ECall, "reaction cannot be 'all'");
+  if (orderptr && *orderptr >= 0 && *orderptr < 3) {
+    const int o = *orderptr;
+    if (sim->rxn[o].empty()) return set_error_nt(fn, ECnonexist, "no reactions defined of this order");
+    for (size_t r = 0; r < sim->rxn[o].size(); r++)
+      if (sim->rxn[o][r].name == reaction) return (int)r;
+    return set_error_nt(fn, ECnonexist, "reaction not found");
+  }
+  for (int o = 0; o < 3; o++)
     for (size_t r = 0; r < sim->rxn[o].size(); r++)
       if (sim->rxn[o][r].name == reaction) {
         if (orderptr) *orderptr = o;
         return (int)r;
       }
-  }
-  return (int)ECnonexist;
+  return set_error_nt(fn, ECnonexist, "reaction not found");
 }
 
 int smolGetReactionIndex(simptr sim, int* orderptr, const char* reaction) {
   int r = smolGetReactionIndexNT(sim, orderptr, reaction);
-  if (r < 0) smolSetError("smolGetReactionIndex", (ErrorCode)r, "reaction not found", "");
+  if (r < 0) return (int)same_error("smolGetReactionIndex");
   return r;
 }
 
@@ -2069,7 +2138,7 @@ enum ErrorCode smolSetReactionRate(simptr sim, const char* reaction, double rate
   LCHECK(!sim->started, fn, ECerror, "rates are fixed once stepping started");
   int order = -1;
   int r = smolGetReactionIndexNT(sim, &order, reaction);
-  LCHECK(r >= 0, fn, ECnonexist, "reaction not found");
+  if (r < 0) return same_error(fn);
   Rxn& rx = sim->rxn[order][r];
   if (type == 0) {
     LCHECK(rate >= 0, fn, ECbounds, "rate must be >=0");
@@ -2127,12 +2196,12 @@ enum ErrorCode smolSetReactionProducts(simptr sim, const char* reaction, enum Re
   LCHECK(sim, fn, ECmissing, "missing sim");
   int order = -1;
   int r = smolGetReactionIndexNT(sim, &order, reaction);
-  LCHECK(r >= 0, fn, ECnonexist, "reaction not found");
+  if (r < 0) return same_error(fn);
   Rxn& rx = sim->rxn[order][r];
   int prd = -1;
   if (product && product[0]) {
-    int i = species_index(sim, product);
-    LCHECK(i > 0, fn, ECnonexist, "product species not found");
+    int i = smolGetSpeciesIndexNT(sim, product);
+    if (i <= 0) return same_error(fn);
     for (int p = 0; p < rx.nprod && prd < 0; p++)
       if (rx.prd[p] == i) prd = p;
     LCHECK(prd >= 0, fn, ECerror, "listed product is not a product of the listed reaction");
@@ -2174,8 +2243,8 @@ enum ErrorCode smolamdAddMoleculesAt(simptr sim, const char* species, int number
   const char* fn = "smolamdAddMoleculesAt";
   LCHECK(sim, fn, ECmissing, "missing sim");
   LCHECK(!sim->started, fn, ECerror, "molecules can only be added before stepping starts");
-  int i = species_index(sim, species);
-  LCHECK(i > 0, fn, ECnonexist, "species not found");
+  int i = smolGetSpeciesIndexNT(sim, species);
+  if (i <= 0) return same_error(fn);
   LCHECK(number >= 0 && (number == 0 || positions), fn, ECmissing, "missing positions");
   for (int k = 0; k < number; k++) {
     PendingMol pm;

@@ -51,6 +51,40 @@ def test_errors_follow_reference_codes():
     assert sim.reaction_index("zz")[1] == sa.ECnonexist
 
 
+def test_lookup_errors_propagate_as_the_reference_does():
+    """An inner lookup records its code and message (smolGetSpeciesIndexNT,
+    libsmoldyn.cpp:851-866) and the caller keeps them under its own name
+    (LCHECK with ECsame, e.g. smolAddSolutionMolecules :1057-1058); 'all'
+    selects every species where the reference allows it."""
+    L = sa.lib()
+    sim = sa.Simulation([0, 0, 0], [10, 10, 10])
+    sim.species("A")
+    sim.species("B")
+    lo = np.zeros(3)
+    dp = lo.ctypes.data_as(C.POINTER(C.c_double))
+    assert L.smolAddSolutionMolecules(sim.p, b"Q", 3, dp, dp) == sa.ECnonexist
+    assert sa.last_error() == (sa.ECnonexist, "smolAddSolutionMolecules", "species 'Q' not found")
+    assert L.smolAddSolutionMolecules(sim.p, b"all", 3, dp, dp) == sa.ECall
+    assert sa.last_error() == (sa.ECall, "smolAddSolutionMolecules", "species cannot be 'all'")
+    assert L.smolAddSolutionMolecules(sim.p, None, 3, dp, dp) == sa.ECmissing
+    assert sa.last_error() == (sa.ECmissing, "smolAddSolutionMolecules", "missing species name")
+    assert L.smolGetMoleculeCount(sim.p, b"Q", sa.MSsoln) == sa.ECnonexist
+    assert sa.last_error()[1] == "smolGetMoleculeCount"
+    assert L.smolGetSpeciesIndex(sim.p, b"Q") == sa.ECnonexist
+    assert sa.last_error() == (sa.ECnonexist, "smolGetSpeciesIndex", "species 'Q' not found")
+    # 'all' is every species for the list assignment and the count
+    sim.add_list("L1")
+    assert L.smolSetMolList(sim.p, b"all", sa.MSall, b"L1") == sa.ECok
+    assert L.smolGetMolListIndex(sim.p, b"L1") == 0
+    assert L.smolSetMolList(sim.p, b"A", sa.MSall, b"nolist") == sa.ECnonexist
+    assert sa.last_error() == (sa.ECnonexist, "smolSetMolList", "list name not recognized")
+    sim.add_molecules("A", 4)
+    sim.add_molecules("B", 5)
+    assert sim.count() == 9 and sim.count("B") == 5
+    assert L.smolAddReaction(sim.p, b"r", b"A", sa.MSsoln, b"Z", sa.MSsoln, 0, None, None, 1.0) == sa.ECnonexist
+    assert sa.last_error() == (sa.ECnonexist, "smolAddReaction", "species 'Z' not found")
+
+
 @pytest.mark.parametrize("rate,dt,difc,b,rel", [(9.640578597, 0.01, 2.0, -1, 0), (99.3218287, 0.005, 2.0, 1.2, 1),
                                                 (21.42267826, 0.002, 4.0, -1, 0), (8000.0, 0.001, 200.0, -1, 0),
                                                 (5.0, 0.0, 1.0, 2.0, 0)])
