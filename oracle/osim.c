@@ -245,9 +245,9 @@ int osim_pos2box(const osim* sim, const double* pos) {
   return b;
 }
 
-/* addmol (smolmolec.c:2717-2740): positions drawn with unirandOOD in every
-   dimension, also where lo==hi (the pointer test at :2726 is never true for a
-   'mol' statement, so fixed coordinates still consume a draw). */
+/* addmol (smolmolec.c:2717-2740) with sort 0: positions drawn with unirandOOD
+   in every dimension, also where lo==hi (the pointer test at :2726 is never
+   true for a 'mol' statement, so fixed coordinates still consume a draw). */
 int osim_add_mols(osim* sim, int n, int ident, const double* lo, const double* hi) {
   int k, d;
   for (k = 0; k < n; k++) {
@@ -265,7 +265,10 @@ int osim_add_mols(osim* sim, int n, int ident, const double* lo, const double* h
     m->box = sim->nbox ? osim_pos2box(sim, m->pos) : -1;
     m->pnl = m->pnlx = -1;
   }
-  sim->ok = 0;
+  /* before stepping: set up with the rest at the next update; once stepping
+     started the reference runs no update for them (addmol sorts nothing), so
+     they stay resurrected until the next step's first molsort */
+  if (sim->stepindex == 0) sim->ok = 0;
   return 0;
 }
 
@@ -281,7 +284,10 @@ int osim_add_mols_at(osim* sim, int n, int ident, const double* pos) {
     m->box = sim->nbox ? osim_pos2box(sim, m->pos) : -1;
     m->pnl = m->pnlx = -1;
   }
-  sim->ok = 0;
+  /* before stepping: set up with the rest at the next update; once stepping
+     started the reference runs no update for them (addmol sorts nothing), so
+     they stay resurrected until the next step's first molsort */
+  if (sim->stepindex == 0) sim->ok = 0;
   return 0;
 }
 

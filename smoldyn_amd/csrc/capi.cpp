@@ -1532,11 +1532,52 @@ enum ErrorCode smolSetMaxMolecules(simptr sim, int maxmolecules) {
   return ECok;
 }
 
+// Molecules added once stepping started (addmol with sort 0,
+// smolmolec.c:2717-2740): the reference resurrects them and runs no molsort,
+// so they join their list at the first molsort of the next step, after
+// diffusion, the wall or surface checks and reassignmolecs
+// (smolsim.cpp:2607-2636): appended in reverse creation order (the dead
+// list's order), undiffused in that step.  Serial numbers continue the
+// device's counter.  Models on the surface variants are refused (their first
+// molsort would interleave the new molecules with list changes).
+static ErrorCode add_running(simptr sim, const char* fn, int ident, int number, const double* lo, const double* hi,
+                      const double* positions) {
+  LCHECK(sim->dev, fn, ECerror, "no device engine");
+  Control c;
+  LCHECK(engine_get_control(sim->dev, 0, &c) == 0, fn, ECerror, "HIP runtime error");
+  std::vector<PendingMol> mols((size_t)number);
+  for (int k = 0; k < number; k++) {
+    PendingMol& pm = mols[(size_t)k];
+    pm.ident = ident;
+    pm.serno = c.serno_next + (uint64_t)k;
+    if (positions) {
+      for (int d = 0; d < 3; d++) pm.pos[d] = positions[3 * k + d];
+    } else {
+      smolrng4 r = smolrng_mol((uint32_t)sim->key, (uint32_t)(sim->key >> 32), pm.serno, 0, SMOLRNG_PLACE, 0);
+      for (int d = 0; d < 3; d++) pm.pos[d] = smolrng_u01open(r.v[d]) * (hi[d] - lo[d]) + lo[d];
+    }
+  }
+  const int ll = sim->listlookup[(size_t)ident][MSsoln];
+  std::vector<int> id((size_t)number, ident);
+  std::vector<double> pos((size_t)number * 3);
+  std::vector<uint64_t> ser((size_t)number);
+  for (int k = 0; k < number; k++) {  // reverse creation order
+    const PendingMol& pm = mols[(size_t)(number - 1 - k)];
+    for (int d = 0; d < 3; d++) pos[(size_t)k * 3 + d] = pm.pos[d];
+    ser[(size_t)k] = pm.serno;
+  }
+  const int er = engine_append_fresh(sim->dev, ll, number, id.data(), pos.data(), ser.data());
+  LCHECK(er != -1, fn, ECerror,
+         "molecules can be added after stepping started only to models without surfaces, absorbing walls, "
+         "ensembles or FP32 records");
+  LCHECK(er == 0, fn, ECmemory, "the molecule list is full (set max_mol)");
+  return ECok;
+}
+
 enum ErrorCode smolAddSolutionMolecules(simptr sim, const char* species, int number, double* lowposition,
                                         double* highposition) {
   const char* fn = "smolAddSolutionMolecules";
   LCHECK(sim, fn, ECmissing, "missing sim");
-  LCHECK(!sim->started, fn, ECerror, "molecules can only be added before stepping starts");
   int i = smolGetSpeciesIndexNT(sim, species);
   if (i <= 0) return same_error(fn);
   LCHECK(number >= 0, fn, ECbounds, "number cannot be negative");
@@ -1545,6 +1586,7 @@ enum ErrorCode smolAddSolutionMolecules(simptr sim, const char* species, int num
     lo[d] = lowposition ? lowposition[d] : sim->low[d];
     hi[d] = highposition ? highposition[d] : sim->high[d];
   }
+  if (sim->started) return add_running(sim, fn, i, number, lo, hi, nullptr);
   // placement in the counter stream: unirandOOD per coordinate, keyed by the
   // new molecule's serial number (addmol, smolmolec.c:2717-2740)
   for (int k = 0; k < number; k++) {
@@ -2242,10 +2284,10 @@ enum ErrorCode smolSetReactionProducts(simptr sim, const char* reaction, enum Re
 enum ErrorCode smolamdAddMoleculesAt(simptr sim, const char* species, int number, const double* positions) {
   const char* fn = "smolamdAddMoleculesAt";
   LCHECK(sim, fn, ECmissing, "missing sim");
-  LCHECK(!sim->started, fn, ECerror, "molecules can only be added before stepping starts");
   int i = smolGetSpeciesIndexNT(sim, species);
   if (i <= 0) return same_error(fn);
   LCHECK(number >= 0 && (number == 0 || positions), fn, ECmissing, "missing positions");
+  if (sim->started) return add_running(sim, fn, i, number, nullptr, nullptr, positions);
   for (int k = 0; k < number; k++) {
     PendingMol pm;
     pm.ident = i;

@@ -92,6 +92,7 @@ struct Geometry {
 struct Control {
   int nl[kMaxList];
   int pfx[kMaxList];       // diffuse-order offset of each list (injection index)
+  int nfresh[kMaxList];    // molecules appended since the last step at each list's end: they skip this step's diffusion
   uint64_t serno_next;
   uint32_t step;           // step index, a Philox counter word
   int nedges;
@@ -268,6 +269,11 @@ int engine_get_list_inst(DeviceState* s, int inst, int ll, int* ident, double* p
 void engine_destroy(DeviceState* s);
 // Upload molecules of list ll in live order (host arrays), replacing its content.
 int engine_upload_list(DeviceState* s, int ll, int n, const int* ident, const double* pos, const uint64_t* serno);
+// Molecules added between steps (one simulation, no surface variant): n
+// records appended to the end of list ll in the given order; they join the
+// next step's pair search but skip its diffusion.  -3 when the list or the
+// record buffer is full.
+int engine_append_fresh(DeviceState* s, int ll, int n, const int* ident, const double* pos, const uint64_t* serno);
 int engine_set_counters(DeviceState* s, uint64_t serno_next, uint32_t step, const int* counts);
 // One reference time step (simulatetimestep).  inject: optional host array of
 // 3 Gaussian deviates per diffusing molecule in diffuse order (NULL = Philox).

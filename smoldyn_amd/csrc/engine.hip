@@ -38,6 +38,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/smoldyn_amd_rng.h"
@@ -199,6 +200,11 @@ struct DeviceState {
   double* inj = nullptr;
   long injcap = 0;
   int hnrec = 0;                     // records uploaded so far
+  // molecules added between steps (engine_append_fresh): first record, list
+  // of each record since then, step they wait for
+  int fresh_rec0 = 0;
+  std::vector<int> fresh_list, fresh_lp;  // list and live position of each
+  uint32_t fresh_step = 0;
   int4* cand = nullptr;              // screened candidate pairs (k_pairs -> k_exact)
   int candcap = 0;
   bool half1 = false;                // every simulation reacts within one list only (half stencil)
@@ -394,7 +400,9 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       const int bprev = kSurf ? a.box[r] : -1;  // the box the record was filed in last step
       double pos[3];
       load_pos<kF32>(a, g, r, bold, pos);
-      const bool moved = tab->diffuselist[ll];
+      // molecules added since the last step sit at the end of their list and
+      // do not diffuse in this step
+      const bool moved = tab->diffuselist[ll] && lp < ctl->nl[ll] - ctl->nfresh[ll];
       bool flipped = false;  // a bound molecule moved to the other face of its panel
       bool dead = false;     // absorbed by a surface: not binned, its slot dies in the first molsort
       bool wall2 = false;    // absorbing walls: killed by the second wall pass unless it reacts
@@ -2310,6 +2318,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       int cap = I.L.base[ll + 1] - I.L.base[ll];
       if (nlnew > cap) nlnew = cap;
       ctl->nl[ll] = nlnew;
+      ctl->nfresh[ll] = 0;
       ctl->pfx[ll] = run;
       if (tab->diffuselist[ll]) run += nlnew;
     }
@@ -2703,6 +2712,77 @@ int engine_upload_list(DeviceState* s, int ll, int n, const int* ident, const do
   return engine_upload_list_inst(s, 0, ll, n, ident, pos, serno);
 }
 
+int engine_append_fresh(DeviceState* s, int ll, int n, const int* ident, const double* pos, const uint64_t* serno) {
+  if (s->multi || s->surf_on || s->f32) return -1;
+  const Inst& I = s->hinst[0];
+  if (ll < 0 || ll >= I.L.nlist || n < 0) return -2;
+  if (n == 0) return 0;
+  HIPOK(hipStreamSynchronize(s->stream));
+  Control c;
+  Glob g;
+  HIPOK(hipMemcpy(&c, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(&g, s->glob, sizeof(Glob), hipMemcpyDeviceToHost));
+  const int slot = (c.step + 1) & 1;  // the current buffer's record count (cur_nrec)
+  const int r0 = g.nrec[slot];
+  if (r0 + n > s->reccap || c.nl[ll] + n > I.L.base[ll + 1] - I.L.base[ll]) return -3;
+  // the molecules added since the last step: records [fresh_rec0, r0) with
+  // their lists; a step since the last addition clears them
+  bool any = false;
+  for (int l2 = 0; l2 < I.L.nlist; l2++) any |= c.nfresh[l2] > 0;
+  if (!any || s->fresh_step != c.step) {
+    s->fresh_rec0 = r0;
+    s->fresh_list.clear();
+    s->fresh_lp.clear();
+    s->fresh_step = c.step;
+  }
+  Recs& R = s->buf[s->cur];
+  std::vector<double> x(n), y(n), z(n);
+  std::vector<int> box(n, 0);
+  for (int k = 0; k < n; k++) {
+    x[k] = pos[3 * k];
+    y[k] = pos[3 * k + 1];
+    z[k] = pos[3 * k + 2];
+    s->fresh_list.push_back(ll);
+  }
+  HIPOK(hipMemcpy(R.px + r0, x.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(R.py + r0, y.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(R.pz + r0, z.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(R.ident + r0, ident, sizeof(int) * n, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(R.serno + r0, serno, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(R.box + r0, box.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  // live positions of list ll's added molecules: the dead list hands them to
+  // the molsort latest first, so this call's molecules (already in that
+  // order) go before the ones added earlier since the last step
+  const int nf = c.nfresh[ll] + n, first = c.nl[ll] + n - nf;
+  const int nold = r0 - s->fresh_rec0, nr = nold + n;
+  std::vector<int> inv(nf), prev;
+  s->fresh_lp.resize(nr);
+  for (int k = 0; k < nold; k++)
+    if (s->fresh_list[k] == ll) prev.push_back(k);
+  std::sort(prev.begin(), prev.end(), [&](int a, int b) { return s->fresh_lp[a] < s->fresh_lp[b]; });
+  int q = 0;
+  for (int k = nold; k < nr; k++) {  // this call's records
+    s->fresh_lp[k] = first + q;
+    inv[q++] = s->fresh_rec0 + k;
+  }
+  for (int k : prev) {  // earlier ones of list ll, shifted behind in their order
+    s->fresh_lp[k] = first + q;
+    inv[q++] = s->fresh_rec0 + k;
+  }
+  HIPOK(hipMemcpy(R.lpos + s->fresh_rec0, s->fresh_lp.data(), sizeof(int) * nr, hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(s->inv + I.L.base[ll] + first, inv.data(), sizeof(int) * nf, hipMemcpyHostToDevice));
+  c.nl[ll] += n;
+  c.nfresh[ll] = nf;
+  for (int k = 0; k < n; k++) {
+    c.count[ident[k]]++;
+    if (serno[k] >= c.serno_next) c.serno_next = serno[k] + 1;
+  }
+  g.nrec[slot] = r0 + n;
+  HIPOK(hipMemcpy(s->ctl, &c, sizeof(Control), hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(s->glob, &g, sizeof(Glob), hipMemcpyHostToDevice));
+  return 0;
+}
+
 // Counters of simulation inst; also sets the record count of the uploaded
 // state (every simulation of a batch starts at the same step).
 int engine_set_counters_inst(DeviceState* s, int inst, uint64_t serno_next, uint32_t step, const int* counts) {
@@ -2830,7 +2910,7 @@ int engine_step_noalloc(DeviceState* s, const double* inject, long ninject) {
   HIPOK(hipMemcpy(&c0, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
   long need = 0;
   for (int ll = 0; ll < s->hinst[0].L.nlist; ll++)
-    if (s->htab[0].diffuselist[ll]) need += c0.nl[ll];
+    if (s->htab[0].diffuselist[ll]) need += c0.nl[ll] - c0.nfresh[ll];
   if (ninject < need) return -2;
   if (need > 0) HIPOK(hipMemcpyAsync(ptr, inject, sizeof(double) * 3 * need, hipMemcpyHostToDevice, s->stream));
   launch_step(s, s->cur, ptr, 0);
@@ -2853,7 +2933,7 @@ int engine_step(DeviceState* s, const double* inject, long ninject) {
     HIPOK(hipMemcpy(&c0, s->ctl, sizeof(Control), hipMemcpyDeviceToHost));
     long need = 0;
     for (int ll = 0; ll < s->hinst[0].L.nlist; ll++)
-      if (s->htab[0].diffuselist[ll]) need += c0.nl[ll];
+      if (s->htab[0].diffuselist[ll]) need += c0.nl[ll] - c0.nfresh[ll];
     if (ninject < need) return -2;
     if (ninject * 3 > s->injcap) {
       if (s->inj) HIPOK(hipFree(s->inj));

@@ -4,6 +4,7 @@ ErrorCode behaviour, and the setup's reaction-parameter solver agrees with
 the reference (through the oracle, pinned to the compiled reference)."""
 import ctypes as C
 import re
+import subprocess
 from pathlib import Path
 
 import numpy as np
@@ -102,3 +103,23 @@ def test_setup_rate_tables_match_oracle():
             assert L.smolamdNumRxnRate(s, 1.0, b) == O.orxn_numrxnrate(s, 1.0, b)
     for pg in (0.1, 0.2, 0.6):
         assert L.smolamdUnbindingRadius(pg, 0.005, 3.0, 1.0) == O.orxn_unbindingradius(pg, 0.005, 3.0, 1.0)
+
+
+def build_c_caller(tmp):
+    """tests/libsmoldyn_caller.c compiled with gcc (C99, warnings as errors)
+    against include/libsmoldyn_amd.h and linked to the built library."""
+    exe = Path(tmp) / "libsmoldyn_caller"
+    lib = sa.LIBPATH.parent
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", str(ROOT / "include"), "-o", str(exe),
+                    str(ROOT / "tests" / "libsmoldyn_caller.c"), "-L", str(lib), "-l:" + sa.LIBPATH.name,
+                    "-Wl,-rpath," + str(lib)], check=True)
+    return exe
+
+
+def test_c_caller_compiles_against_the_header_and_sets_up(tmp_path):
+    exe = build_c_caller(tmp_path)
+    r = subprocess.run([str(exe), "setup"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert "lookup smolAddSolutionMolecules | species 'Q' not found" in lines
+    assert "species 1 2 3" in lines and lines[-1] == "setup ok"

@@ -237,3 +237,40 @@ def test_reaction_log_lines_equal_oracle_events(tmp_path, serials):
     got = (tmp_path / "rxn.log").read_text().splitlines()
     assert len(lines) > (20 if listed is None else 5)
     assert got == lines
+
+
+def test_c_caller_runs_the_reference_way_and_matches_the_oracle(tmp_path):
+    """tests/libsmoldyn_caller.c (a C program built with gcc against
+    include/libsmoldyn_amd.h) steps its model with smolRunTimeStep, runs it to
+    time 1 with smolRunSimUntil and to its end with smolRunSim, and reads a
+    molcount data table: its counts equal
+    the oracle's on the same model after the same steps."""
+    import subprocess
+    from test_capi_host import build_c_caller
+    exe = build_c_caller(tmp_path)
+    r = subprocess.run([str(exe), "run"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps, final, until, rows = {}, None, None, []
+    lines = r.stdout.splitlines()
+    for k, ln in enumerate(lines):
+        w = ln.split()
+        if w[0] == "step":
+            steps[int(w[1])] = tuple(int(x) for x in w[2:5])
+        elif w[0] == "final":
+            final = tuple(int(x) for x in w[1:4])
+        elif w[0] == "until":
+            until = tuple(int(x) for x in w[1:4])
+        elif w[0] == "data":
+            rows = [[float(x) for x in lines[k + 1 + j].split()] for j in range(int(w[1]))]
+    spec = dict(low=[0.0] * 3, high=[40.0] * 3, walls="rrr", dt=0.01, seed=17, molperbox=4,
+                species={"A": 1.0, "B": 1.0, "C": 0.5},
+                reactions=[("fwd", ["A", "B"], ["C"], 20.0), ("back", ["C"], ["A", "B"], 2.0)],
+                placement={"back": ("ratio", 1.5)},
+                surfaces=[{"name": "ball", "actions": [("both", "all", "reflect")],
+                           "panels": [("sph", None, [20.0, 20.0, 20.0, 15.0, 20, 20], "s0")]}],
+                molecules=[("A", 2000), ("B", 2000)])
+    want = _oracle_counts(spec, 200)
+    for k, c in steps.items():
+        assert c == tuple(want[k]), (k, c, want[k])
+    assert len(steps) == 20 and until == tuple(want[_steps_to(0.0, 1.0, 0.01)]) and final == tuple(want[200])
+    assert len(rows) >= 20 and all(r_[1] + r_[3] == 2000 and r_[2] + r_[3] == 2000 for r_ in rows)

@@ -131,6 +131,51 @@ def test_c2_full_size_invariants_over_many_steps():
     assert d["err"] == 0
 
 
+def test_molecules_added_between_steps():
+    """smolAddSolutionMolecules and smolamdAddMoleculesAt after stepping
+    started (addmol without a molsort, smolmolec.c:2717-2740): the new
+    molecules take the next serial numbers, join their list at the next
+    step's first molsort in reverse creation order, skip that step's
+    diffusion and react from then on; the injected step after an addition
+    takes deviates for the old molecules only.  Identical to the oracle."""
+    spec = ss.c2_spec(n=6000, side=40.0, seed=71, kback=20.0)
+    osim, _ = ss.build_oracle(spec)
+    dsim = ss.build_device(spec)
+    rng = np.random.default_rng(5)
+    lo, hi = [5.0, 5.0, 5.0], [20.0, 35.0, 30.0]
+
+    def steps(n):
+        for _ in range(n):
+            osim.step()
+            dsim.run_time_step()
+            assert ss.canonical_pairs(osim.rxnlog()[:, :6]) == ss.canonical_pairs(dsim.reaction_log())
+
+    steps(10)
+    ss.compare_states(ss.oracle_state(osim), ss.device_state(dsim))
+    osim.add_mols(50, osim.index("A"), lo, hi)
+    dsim.add_molecules("A", 50, lo, hi)
+    pos = rng.uniform(0, 40.0, (20, 3))
+    osim.add_mols_at(osim.index("B"), pos)
+    dsim.add_molecules_at("B", pos)
+    assert dsim.count("A") == osim.count(osim.index("A"))
+    steps(1)
+    ss.compare_states(ss.oracle_state(osim), ss.device_state(dsim))
+    steps(15)
+    ss.compare_states(ss.oracle_state(osim), ss.device_state(dsim))
+    # an injected step right after an addition
+    dsim.add_molecules("C", 30)
+    osim.add_mols(30, osim.index("C"), spec["low"], spec["high"])
+    n = sum(len(x["ident"]) for x in ss.oracle_state(osim))  # live molecules: the new ones are not yet
+    g = rng.standard_normal((n, 3))
+    osim.inject(g)
+    osim.step()
+    dsim.inject_step(g)
+    ss.compare_states(ss.oracle_state(osim), ss.device_state(dsim))
+    steps(10)
+    ss.compare_states(ss.oracle_state(osim), ss.device_state(dsim))
+    _compare_counts(spec, osim, dsim)
+
+
 # ---------------------------------------------------------------------------
 # surfaces
 
