@@ -149,6 +149,7 @@ struct Glob {
   int ninst;  // simulations in the batch
   int halt;   // set on an inconsistency: every later step does nothing
   int ncand;  // screened candidate pairs of the step (k_pairs -> k_exact), reset by k_resolve
+  int par;    // (s+1)&1 for the step s about to run: which nrec its kernels read
 };
 
 // kernels timed by engine_profile, each with its own start / stop events
@@ -362,7 +363,10 @@ __device__ __forceinline__ int wave_hist_rank(int* bcount, int key) {
 // Record count of the current buffer (see Glob), read by the kernels of a
 // step before k_resolve.
 __device__ __forceinline__ int cur_nrec(const Inst* insts, const Glob* gl) {
-  return gl->nrec[(insts[0].ctl->step + 1) & 1];
+  (void)insts;
+  // both counts and the parity load together (one scalar load, no chain)
+  const int n0 = gl->nrec[0], n1 = gl->nrec[1];
+  return gl->par ? n1 : n0;
 }
 
 template <int kSurf, bool kMulti, bool kF32>
@@ -2328,6 +2332,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       gl->nrec[(step + 1) & 1] = 0;
       gl->nprod[(step + 1) & 1] = 0;
       gl->ncand = 0;
+      gl->par = step & 1;
     }
     ctl->nbinned = nbinned;
     ctl->serno_next = ser0 + (uint64_t)K;
@@ -2586,6 +2591,7 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
     Glob g0;
     memset(&g0, 0, sizeof(g0));
     g0.ninst = n;
+    g0.par = 1;  // step 0 reads nrec[1]
     (void)hipMemcpy(s->glob, &g0, sizeof(Glob), hipMemcpyHostToDevice);
   }
   (void)hipMemcpy(s->tab, tab, sizeof(DevTables) * n, hipMemcpyHostToDevice);
@@ -2795,6 +2801,7 @@ int engine_set_counters_inst(DeviceState* s, int inst, uint64_t serno_next, uint
   Glob g;
   memset(&g, 0, sizeof(g));
   g.nrec[(step + 1) & 1] = s->hnrec;
+  g.par = (step + 1) & 1;
   g.ninst = s->ninst;
   HIPOK(hipMemcpy(s->glob, &g, sizeof(Glob), hipMemcpyHostToDevice));
   return 0;
