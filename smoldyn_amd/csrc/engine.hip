@@ -1340,6 +1340,10 @@ __device__ int wg_scan(int v, int* s, int* tot) {
 
 constexpr int kLdsEvents = 1024;  // sorted event indices kept in LDS up to this count
 constexpr int kLdsKills = 2048;   // sorted kill slots kept in LDS up to this count
+constexpr int kLdsMatch = 512;    // edges matched in LDS (more: the same rounds on global arrays)
+constexpr int kMatchHash = 2048;  // LDS slots of their molecules (at least twice the molecules)
+static_assert(kMatchHash <= kLdsSort && kMatchHash <= kLdsKills && kMatchHash >= 4 * kLdsMatch,
+              "LDS matching reuses the sort and kill arrays");
 constexpr int kMaxAds = 512;      // molecules leaving their lists in the sort that ends a step
 constexpr int kProdKilled = -1;   // Prod::sb of a product its reborn surface check absorbed
 
@@ -1783,9 +1787,68 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
     s_nads = 0;
   }
   if (tid < kMaxSpecies) s_dcount[tid] = 0;
+  int rounds = 0;
+  if (E <= kLdsMatch) {
+    // the same rounds in LDS: the edges' molecules hashed to LDS slots
+    // (open addressing on the record index), each slot with the smallest
+    // key offered to it this round and whether an accepted edge took it;
+    // one thread per edge keeps its edge in registers
+    uint64_t* best = lk;  // kMatchHash slots (the sort space, unused until the event sort)
+    int* srec = lv;
+    int* taken = s_kill;
+    for (int h = tid; h < kMatchHash; h += NT) {
+      srec[h] = -1;
+      taken[h] = 0;
+      best[h] = kNoKey;
+    }
+    __syncthreads();
+    Edge ed;
+    int h1 = 0, h2 = 0, st = tid < E ? 0 : 2;  // 0 open, 1 accepted, 2 out
+    auto slot = [&](int rec) {
+      int h = (int)(((uint32_t)rec * 2654435761u) >> 20) & (kMatchHash - 1);
+      for (;;) {
+        const int prev = atomicCAS(&srec[h], -1, rec);
+        if (prev == -1 || prev == rec) return h;
+        h = (h + 1) & (kMatchHash - 1);
+      }
+    };
+    if (st == 0) {
+      ed = I.edges[tid];
+      h1 = slot(ed.s1);
+      h2 = slot(ed.s2);
+    }
+    __syncthreads();
+    for (;;) {
+      if (tid == 0) s_flag = 0;
+      __syncthreads();
+      if (st == 0) {
+        if (taken[h1] || taken[h2]) {
+          st = 2;
+        } else {
+          atomicMin((unsigned long long*)&best[h1], (unsigned long long)ed.key);
+          atomicMin((unsigned long long*)&best[h2], (unsigned long long)ed.key);
+          s_flag = 1;
+        }
+      }
+      __syncthreads();
+      if (!s_flag) break;
+      rounds++;
+      if (st == 0 && best[h1] == ed.key && best[h2] == ed.key) {
+        st = 1;
+        taken[h1] = 1;
+        taken[h2] = 1;
+        I.accepted[atomicAdd(&s_nacc, 1)] = tid;
+      }
+      __syncthreads();
+      if (st == 0) {
+        best[h1] = kNoKey;
+        best[h2] = kNoKey;
+      }
+      __syncthreads();
+    }
+  } else {
   for (int e = tid; e < E; e += NT) I.estate[e] = 0;
   __syncthreads();
-  int rounds = 0;
   for (;;) {
     if (tid == 0) s_flag = 0;
     __syncthreads();
@@ -1822,6 +1885,7 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       atomicExch((unsigned long long*)&bestkey[ed.s2], (unsigned long long)kNoKey);
     }
     __syncthreads();
+  }
   }
   int B2 = s_nacc;
   __syncthreads();
