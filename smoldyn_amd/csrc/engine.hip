@@ -39,6 +39,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "../../include/smoldyn_amd_rng.h"
@@ -109,12 +110,24 @@ struct PairConst {
   float r2f[kMaxSpecies];           // (largest binding radius + FP32 margin)^2
 };
 
+// Per-species constants of the diffusion step and the first-order reaction
+// coins of one simulation, gathered from its tables when the engine is made.
+struct SpecRow {
+  double ds;                   // diffusion step length (difstep)
+  double prob[kMaxRxnPerSet];  // first-order reaction probabilities in table order; -1 for a
+                               // reaction not permitted to solution-phase reactants (never fires)
+  int ll;                      // list of the solution state
+  int nu;                      // first-order reactions
+  int rx[kMaxRxnPerSet];
+};
+
 // One simulation of the batch, as the kernels see it (device memory).
 struct Inst {
   Layout L;        // list slots in the batch-wide slot numbering
   Geometry g;
   DevSurf sf;
   const DevTables* tab;
+  const SpecRow* srow;  // kMaxSpecies rows
   Control* ctl;
   PairConst pc;
   uint32_t key0, key1;
@@ -179,6 +192,7 @@ struct DeviceState {
   // per-simulation arrays (the instance views point into them)
   Control* ctl = nullptr;
   DevTables* tab = nullptr;
+  SpecRow* srow = nullptr;  // kMaxSpecies rows per simulation
   UniEvent* uniev = nullptr;
   Edge* edges = nullptr;
   int* estate = nullptr;
@@ -283,21 +297,20 @@ __device__ __forceinline__ void store_pos(const Recs& a, const Geometry& g, int 
 #include "surface_dev.h"
 namespace smolamd {
 
-// checkwalls for one molecule, walls in the reference's order 0..5
+// checkwalls for one molecule, walls in the reference's order 0..5.  Both
+// outcomes of a wall are computed and one is selected, so the wall
+// parameters load together and no lane waits on a branch.
 __device__ __forceinline__ void apply_walls(const Geometry& g, double* p) {
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    int wl = g.wall[2 * d], wh = g.wall[2 * d + 1];
-    if (wl == kWallReflect) {
-      if (p[d] < g.low[d]) p[d] = 2 * g.low[d] - p[d];
-    } else if (wl == kWallPeriodic) {
-      if (p[d] < g.low[d]) p[d] += g.high[d] - g.low[d];
-    }
-    if (wh == kWallReflect) {
-      if (p[d] > g.high[d]) p[d] = 2 * g.high[d] - p[d];
-    } else if (wh == kWallPeriodic) {
-      if (p[d] > g.high[d]) p[d] += g.low[d] - g.high[d];
-    }
+    const int wl = g.wall[2 * d], wh = g.wall[2 * d + 1];
+    const double lo = g.low[d], hi = g.high[d];
+    double v = p[d];
+    const double vl = wl == kWallReflect ? 2 * lo - v : v + (hi - lo);
+    if ((wl == kWallReflect || wl == kWallPeriodic) && v < lo) v = vl;
+    const double vh = wh == kWallReflect ? 2 * hi - v : v + (lo - hi);
+    if ((wh == kWallReflect || wh == kWallPeriodic) && v > hi) v = vh;
+    p[d] = v;
   }
 }
 
@@ -369,6 +382,32 @@ __device__ __forceinline__ int cur_nrec(const Inst* insts, const Glob* gl) {
   return gl->par ? n1 : n0;
 }
 
+// The same count, -1 once stepping has halted: the halt flag, both counts and
+// the parity come in one scalar load at the start of a pass.
+__device__ __forceinline__ int live_nrec(const Glob* gl) {
+  const int n0 = gl->nrec[0], n1 = gl->nrec[1], h = gl->halt, p = gl->par;
+  return h ? -1 : (p ? n1 : n0);
+}
+
+// Wavefront-private copies of the species rows and of each list's diffusion
+// limit (list positions below it diffuse this step; INT_MIN for a list that
+// does not diffuse), for single-simulation passes: each lane loads one row
+// or one list's counters while the wave's record loads are in flight, so no
+// per-molecule table lookup waits on a dependent global load.  Every lane of
+// the wave must call this.
+__device__ __forceinline__ void fill_spec_rows(const Inst& I, SpecRow* S, int* lim) {
+  const int lane = threadIdx.x & 63;
+  const Control* ctl = I.ctl;
+  const int ls = lane & (kMaxList - 1);
+  const SpecRow row = I.srow[lane & (kMaxSpecies - 1)];
+  const int dl = I.tab->diffuselist[ls], nl = ctl->nl[ls], nf = ctl->nfresh[ls];
+  if (lane < kMaxSpecies) S[lane] = row;
+  if (lane < kMaxList) lim[lane] = dl ? nl - nf : INT_MIN;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int kSurf, bool kMulti, bool kF32>
 __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ insts, const Glob* __restrict__ gl,
                                                     Recs a, int* __restrict__ bcount,
@@ -377,10 +416,30 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   int hkey = -1;  // histogram key (simulation's first key + (list*nbox + box)*zsub + sub-bin), -1 when not binned
   int zsbits = 0;  // z sub-bin, kept above the rank for k_scatter
-  if (gl->halt) return;  // uniform over the grid: no wave reaches the histogram below
-  if (r < cur_nrec(insts, gl)) {
-    int i = a.ident[r];
-    int in = kMulti ? a.inst[r] : 0;
+  // uniform over the block: a block without records (or any block once
+  // stepping has halted) has nothing to bin
+  const int n = live_nrec(gl);
+  if ((int)(blockIdx.x * blockDim.x) >= n) return;
+  // the record's fields, loaded by every lane (a lane past the last record
+  // reads the last one and discards it) before anything waits on them
+  const int rr = r < n ? r : n - 1;
+  const int i0 = a.ident[rr];
+  const int in0 = kMulti ? a.inst[rr] : 0;
+  const int sb0 = kSurf ? a.sb[rr] : 0;
+  const int lp0 = a.lpos[rr];
+  const uint64_t ser0 = a.serno[rr];
+  const int box0 = (kF32 || kSurf) ? a.box[rr] : 0;
+  double pos0[3];
+  load_pos<kF32>(a, insts[(in0 >= 0 && in0 < gl->ninst) ? in0 : 0].g, rr, box0, pos0);
+  __shared__ SpecRow s_spec[kMulti ? 1 : kBlock / 64][kMaxSpecies];
+  __shared__ int s_lim[kMulti ? 1 : kBlock / 64][kMaxList];
+  SpecRow* S = s_spec[kMulti ? 0 : threadIdx.x >> 6];
+  int* Lim = s_lim[kMulti ? 0 : threadIdx.x >> 6];
+  const uint32_t step0 = kMulti ? 0u : insts[0].ctl->step;
+  if (!kMulti) fill_spec_rows(insts[0], S, Lim);
+  if (r < n) {
+    int i = i0;
+    int in = in0;
     if (i <= 0 || i >= kMaxSpecies || in < 0 || in >= gl->ninst) {
       if (i != 0) atomicOr((int*)&gl->halt, 1);
       a.box[r] = -1;
@@ -389,30 +448,30 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       const DevTables* tab = I.tab;
       Control* ctl = I.ctl;
       const Geometry& g = I.g;
-      int ll = tab->listof[i];
+      int ll = kMulti ? tab->listof[i] : S[i].ll;
       int sb = 0;  // surface binding (kSurf): bound molecules live in the list of their state
       if (kSurf) {
-        sb = a.sb[r];
+        sb = sb0;
         if (sb) ll = I.sf.blist[i][bstate(sb)];
       }
-      int lp = a.lpos[r];
-      uint64_t ser = a.serno[r];
-      uint32_t step = ctl->step;
+      const int lp = lp0;
+      const uint64_t ser = ser0;
+      const uint32_t step = kMulti ? ctl->step : step0;
       int mkind = 0, newll = ll;  // list change queued for the first molsort (surfaces)
       int obox = -1, rbox = -1;    // desorption: box in the old list, box the reference files it in
-      const int bold = kF32 ? a.box[r] : 0;
-      const int bprev = kSurf ? a.box[r] : -1;  // the box the record was filed in last step
-      double pos[3];
-      load_pos<kF32>(a, g, r, bold, pos);
+      const int bold = kF32 ? box0 : 0;
+      const int bprev = kSurf ? box0 : -1;  // the box the record was filed in last step
+      double pos[3] = {pos0[0], pos0[1], pos0[2]};
       // molecules added since the last step sit at the end of their list and
       // do not diffuse in this step
-      const bool moved = tab->diffuselist[ll] && lp < ctl->nl[ll] - ctl->nfresh[ll];
+      const bool moved =
+          kMulti ? (tab->diffuselist[ll] && lp < ctl->nl[ll] - ctl->nfresh[ll]) : lp < Lim[ll];
       bool flipped = false;  // a bound molecule moved to the other face of its panel
       bool dead = false;     // absorbed by a surface: not binned, its slot dies in the first molsort
       bool wall2 = false;    // absorbing walls: killed by the second wall pass unless it reacts
       if (moved) {
         const double posx[3] = {pos[0], pos[1], pos[2]};
-        double ds = tab->difstep[i];
+        double ds = kMulti ? tab->difstep[i] : S[i].ds;
         double gg[3];
         if (inj) {
           long k = (long)ctl->pfx[ll] + lp;
@@ -586,19 +645,27 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       // unireact: coin flips in table order, first success reacts; a
       // molecule that reacts takes no part in the pair search, so it is not
       // binned
-      int nu = (sb || dead) ? 0 : tab->nuni[i];  // surface-bound molecules do not react
+      int nu = (sb || dead) ? 0 : (kMulti ? tab->nuni[i] : S[i].nu);  // surface-bound molecules do not react
       if (nu > 0) {
         smolrng4 rb;
         int blk = -1;
         for (int j = 0; j < nu; j++) {
-          int rx = tab->uni[i][j];
-          const DevRxn& d = tab->rxn1[rx];
-          if (!d.permit_soln) continue;
+          int rx;
+          double prob;
+          if (kMulti) {
+            rx = tab->uni[i][j];
+            const DevRxn& d = tab->rxn1[rx];
+            if (!d.permit_soln) continue;
+            prob = d.prob;
+          } else {
+            rx = S[i].rx[j];
+            prob = S[i].prob[j];
+          }
           if (blk != j / 4) {
             blk = j / 4;
             rb = smolrng_mol(I.key0, I.key1, ser, step, SMOLRNG_UNIRXN, (uint32_t)blk);
           }
-          if (smolrng_u01(rb.v[j & 3]) < d.prob) {
+          if (smolrng_u01(rb.v[j & 3]) < prob) {
             int e = atomicAdd(&ctl->nuni, 1);
             if (e < I.maxevents) {
               I.uniev[e].rec = r;
@@ -2448,6 +2515,24 @@ static int upload_instances(DeviceState* s) {
   return 0;
 }
 
+// Species rows of one simulation's tables (the fields k_diffuse reads per
+// molecule; species without a list or reactions get harmless rows).
+static void spec_rows(const DevTables& t, SpecRow* out) {
+  for (int i = 0; i < kMaxSpecies; i++) {
+    SpecRow& r = out[i];
+    memset(&r, 0, sizeof(r));
+    r.ds = t.difstep[i];
+    r.ll = t.listof[i];
+    r.nu = t.nuni[i];
+    for (int j = 0; j < kMaxRxnPerSet; j++) {
+      const int x = t.uni[i][j];
+      const bool ok = j < r.nu && x >= 0 && x < kMaxRxn;
+      r.rx[j] = ok ? x : -1;
+      r.prob[j] = ok && t.rxn1[x].permit_soln ? t.rxn1[x].prob : -1.0;
+    }
+  }
+}
+
 DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables* tab, const Geometry* geo,
                                  const uint32_t* key0, const uint32_t* key1, const double* gtable, int ngauss,
                                  const HostSurf* surf, char* err, int errlen) {
@@ -2589,6 +2674,7 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
   ok |= dalloc(&s->sortkey, nsort) | dalloc(&s->sortval, nsort) | dalloc(&s->killslot, nsort);
   ok |= dalloc(&s->prods, nsort * kMaxProd);
   ok |= dalloc(&s->ctl, (size_t)n) | dalloc(&s->tab, (size_t)n) | dalloc(&s->gtab, ngauss);
+  ok |= dalloc(&s->srow, (size_t)n * kMaxSpecies);
   ok |= dalloc(&s->dinst, (size_t)n) | dalloc(&s->glob, 1);
   if (surf && surf->on) {
     ok |= dalloc(&s->d_pnl, surf->npnl > 0 ? surf->npnl : 1) | dalloc(&s->d_bpstart, (size_t)surf->nbox + 1) |
@@ -2604,6 +2690,7 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
   for (int k = 0; k < n; k++) {
     Inst& I = s->hinst[k];
     I.tab = s->tab + k;
+    I.srow = s->srow + (size_t)k * kMaxSpecies;
     I.ctl = s->ctl + k;
     I.uniev = s->uniev + evoff[k];
     I.edges = s->edges + edoff[k];
@@ -2659,6 +2746,11 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
     (void)hipMemcpy(s->glob, &g0, sizeof(Glob), hipMemcpyHostToDevice);
   }
   (void)hipMemcpy(s->tab, tab, sizeof(DevTables) * n, hipMemcpyHostToDevice);
+  {
+    std::vector<SpecRow> rows((size_t)n * kMaxSpecies);
+    for (int k = 0; k < n; k++) spec_rows(tab[k], &rows[(size_t)k * kMaxSpecies]);
+    (void)hipMemcpy(s->srow, rows.data(), sizeof(SpecRow) * rows.size(), hipMemcpyHostToDevice);
+  }
   (void)hipMemcpy(s->gtab, gtable, sizeof(double) * ngauss, hipMemcpyHostToDevice);
   if (upload_instances(s)) {
     snprintf(err, errlen, "device copy failed");
@@ -2699,7 +2791,7 @@ void engine_destroy(DeviceState* s) {
                   s->accepted, s->bestkey, s->killed,  s->uniev,   s->log,       s->sortkey, s->sortval, s->killslot,
                   s->ctl,     s->tab,     s->gtab,    s->inj,     s->prods,     s->d_pnl,   s->d_bpstart,
                   s->d_bplist, s->d_act,  s->d_actb,  s->moves,   s->dinst,   s->glob,    s->obs,
-                  s->wq};
+                  s->wq,      s->srow};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
