@@ -1295,6 +1295,109 @@ __device__ void wg_sort(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv) {
 // LDS: each key's destination is the number of keys before it (ties by
 // index).  Two barriers instead of the log^2 stages of a bitonic network.
 constexpr int kRankSort = 512;
+// Sort n <= blockDim int keys of k_in (LDS) into k_out (LDS), ascending.
+// Keys are spread over kBuckets buckets by their place in [min, max]; the
+// bucket counts (LDS atomics) are scanned, each key is placed in its bucket,
+// and its rank is the bucket's start plus the keys of the bucket below it
+// (equal keys by their place in the bucket).  Five barriers and, for keys
+// spread like list slots, about n / kBuckets comparisons per key.
+// scratch: LDS, kBuckets + 1 + n + 2 * (blockDim / 64) ints.
+constexpr int kBuckets = 256;
+__device__ void wg_bucket_sort(const int* k_in, int n, int* k_out, int* scratch) {
+  int* cnt = scratch;                 // kBuckets + 1
+  int* tmp = scratch + kBuckets + 1;  // n
+  int* red = tmp + n;                 // min, max of each wave
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
+  int key = 0, mn = INT_MAX, mx = INT_MIN;
+  if (tid < n) {
+    key = k_in[tid];
+    mn = key;
+    mx = key;
+  }
+  for (int i = tid; i <= kBuckets; i += blockDim.x) cnt[i] = 0;
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = min(mn, __shfl_xor(mn, o));
+    mx = max(mx, __shfl_xor(mx, o));
+  }
+  if (lane == 0) {
+    red[2 * w] = mn;
+    red[2 * w + 1] = mx;
+  }
+  __syncthreads();
+  mn = red[0];
+  mx = red[1];
+  for (int v = 1; v < nw; v++) {
+    mn = min(mn, red[2 * v]);
+    mx = max(mx, red[2 * v + 1]);
+  }
+  const long long span = (long long)mx - mn + 1;
+  int b = 0, pos = 0;
+  if (tid < n) {
+    b = (int)(((long long)key - mn) * kBuckets / span);
+    pos = atomicAdd(&cnt[b], 1);
+  }
+  __syncthreads();
+  if (w == 0) {  // exclusive scan of the counts, kBuckets / 64 per lane
+    constexpr int per = kBuckets / 64;
+    int c[per], t = 0;
+#pragma unroll
+    for (int u = 0; u < per; u++) {
+      c[u] = cnt[per * lane + u];
+      t += c[u];
+    }
+    int inc = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(inc, o);
+      if (lane >= o) inc += y;
+    }
+    int run = inc - t;
+#pragma unroll
+    for (int u = 0; u < per; u++) {
+      cnt[per * lane + u] = run;
+      run += c[u];
+    }
+    if (lane == 63) cnt[kBuckets] = run;
+  }
+  __syncthreads();
+  int st = 0;
+  if (tid < n) {
+    st = cnt[b];
+    tmp[st + pos] = key;
+  }
+  __syncthreads();
+  if (tid < n) {
+    const int en = cnt[b + 1], me = st + pos;
+    int r = st;
+    for (int j = st; j < en; j++) {
+      const int x = tmp[j];
+      r += (x < key) | ((x == key) & (j < me));
+    }
+    k_out[r] = key;
+  }
+  __syncthreads();
+}
+
+// Rank of key k (element i) among lk[j0, j1): keys below it, and equal keys
+// at lower indices.  The LDS loads go out eight at a time, so the loop waits
+// once per eight keys instead of once per key.
+template <typename K>
+__device__ __forceinline__ int rank_in(const K* lk, int j0, int j1, int i, K k) {
+  int r = 0, j = j0;
+  for (; j + 8 <= j1; j += 8) {
+    K x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = lk[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; u++) r += (x[u] < k) | ((x[u] == k) & (j + u < i));
+  }
+  for (; j < j1; j++) {
+    const K x = lk[j];
+    r += (x < k) | ((x == k) & (j < i));
+  }
+  return r;
+}
+
 __device__ void wg_rank_sort(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv, int* ok32, int* ov) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     lk[i] = gk[i];
@@ -1303,11 +1406,7 @@ __device__ void wg_rank_sort(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const uint64_t k = lk[i];
-    int r = 0;
-    for (int j = 0; j < n; j++) {
-      const uint64_t x = lk[j];
-      r += (x < k) | ((x == k) & (j < i));
-    }
+    const int r = rank_in<uint64_t>(lk, 0, n, i, k);
     gk[r] = k;
     gv[r] = lv[i];
     if (ok32) ok32[r] = (int)k;
@@ -1334,11 +1433,11 @@ __device__ void wg_rank_sort2(const K* lk, const int* lv, int n0, int n1, int* o
     K k = 0;
     if (i < n) {
       k = lk[i];
+      // this thread's contiguous share of the element's group
       const int lo = i < n0 ? 0 : n0, hi = i < n0 ? n0 : n;
-      for (int j = lo + q; j < hi; j += tpe) {
-        const K x = lk[j];
-        r += (x < k) | ((x == k) & (j < i));
-      }
+      const int c = (hi - lo + tpe - 1) / tpe;
+      const int j0 = lo + q * c;
+      r = rank_in<K>(lk, j0, j0 + c < hi ? j0 + c : hi, i, k);
     }
     for (int o = tpe >> 1; o > 0; o >>= 1) r += __shfl_down(r, o, tpe);
     if (i < n && q == 0) {
@@ -1411,6 +1510,9 @@ constexpr int kLdsMatch = 512;    // edges matched in LDS (more: the same rounds
 constexpr int kMatchHash = 2048;  // LDS slots of their molecules (at least twice the molecules)
 static_assert(kMatchHash <= kLdsSort && kMatchHash <= kLdsKills && kMatchHash >= 4 * kLdsMatch,
               "LDS matching reuses the sort and kill arrays");
+static_assert(kRankSort <= kResolveThreads && kRankSort <= kLdsKills && kBuckets % 64 == 0 &&
+                  kBuckets + 1 + kRankSort + 2 * (kResolveThreads / 64) <= 2 * kLdsSort,
+              "the kill sort takes one key per thread and its scratch from the sort space");
 constexpr int kMaxAds = 512;      // molecules leaving their lists in the sort that ends a step
 constexpr int kProdKilled = -1;   // Prod::sb of a product its reborn surface check absorbed
 
@@ -2292,7 +2394,8 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
   // the reference produces: creation order reversed, rotated by the kills.
   const bool kill_lds = D <= kLdsKills;
   if (kill_small) {
-    wg_rank_sort2<int>(lv, lv, D, 0, nullptr, s_kill);
+    // the sort space lk is free here
+    wg_bucket_sort(lv, D, s_kill, reinterpret_cast<int*>(lk));
   } else {
     for (int i = tid; i < D; i += NT) {
       I.sk[i] = (uint64_t)I.killslot[i];
