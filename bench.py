@@ -41,15 +41,21 @@ def log(msg):
 #                    write the 30 B payload reordered                      = 64
 #   k_pairs          pass C: read own 26 B + cell offsets 1, write result 4 = 31
 #   k_exact, k_resolve  work per candidate pair and per event, not per molecule
-# 153 B per molecule-step in all.  The layout's own bytes (design bytes: the
-# record also carries its serial number, live-list position and rank, and the
-# pair screen's packed FP32 copy) are reported beside them.
-ALGO_BYTES = {"diffuse": 54, "scan": 4, "scatter": 64, "relist": 0, "pairs": 31, "exact": 0, "resolve": 0}
-DESIGN_BYTES = {"diffuse": 72, "scan": 16, "scatter": 112, "relist": 0, "pairs": 24, "exact": 0, "resolve": 0}
+# 153 B per molecule-step in all.  The tile binning (one simulation without
+# surfaces) runs pass A as k_tile1 (54 B) and pass B with the cell scan as
+# k_tile2 (64 B; its per-key scan is in LDS) with k_tile_scan between them
+# (per tile of ~500 records, no per-molecule bytes).  The layout's own bytes (design bytes: the record also carries its
+# serial number, live-list position and rank, and the pair screen's packed
+# FP32 copy) are reported beside them.
+ALGO_BYTES = {"diffuse": 54, "scan": 4, "scatter": 64, "relist": 0, "pairs": 31, "exact": 0, "resolve": 0,
+              "tile1": 54, "tilescan": 0, "tile2": 64}
+DESIGN_BYTES = {"diffuse": 72, "scan": 16, "scatter": 112, "relist": 0, "pairs": 24, "exact": 0, "resolve": 0,
+                "tile1": 72, "tilescan": 0, "tile2": 112}
 STEP_ALGO_BYTES = 153
-KERNEL_NAMES = ["diffuse", "scan", "scatter", "relist", "pairs", "exact", "resolve"]
+KERNEL_NAMES = ["diffuse", "scan", "scatter", "relist", "pairs", "exact", "resolve", "tile1", "tilescan", "tile2"]
 HIP_KERNEL_SYMBOL = {"diffuse": "k_diffuse", "scan": "k_hsum+k_hscan", "scatter": "k_scatter", "relist": "k_relist",
-                     "pairs": "k_pairs", "exact": "k_exact", "resolve": "k_resolve"}
+                     "pairs": "k_pairs", "exact": "k_exact", "resolve": "k_resolve", "tile1": "k_tile1",
+                     "tilescan": "k_tile_scan", "tile2": "k_tile2"}
 
 
 def c2_spec(n_per_species=500000, side=1000.0, seed=2026, kback=1.0):

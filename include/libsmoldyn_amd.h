@@ -236,9 +236,12 @@ int smolamdCommandTimeline(const char* script, int maxev, double* ev_time, int* 
 enum ErrorCode smolamdRunSteps(simptr sim, long nsteps, int use_graph, double* ms);
 /* Per-kernel average device time (ms) over nsteps timed steps: each kernel
    is launched with its own start and stop HIP events, stamped at the
-   dispatch's start and end.  out[0..6] = diffuse, box scan (histogram sums
+   dispatch's start and end.  out[0..9] = diffuse, box scan (histogram sums
    and scan), scatter, first molsort (models with surfaces or absorbing
-   walls, else 0), pair screen, exact pair test, resolve. */
+   walls, else 0), pair screen, exact pair test, resolve, tile binning pass 1,
+   tile offsets, tile binning pass 2.  A step bins either with the counting
+   sort (diffuse, scan, scatter) or with the tile passes; the other entries
+   are 0. */
 enum ErrorCode smolamdProfileSteps(simptr sim, long nsteps, double* out);
 /* Engine diagnostics: out[0]=live molecules, [1]=edges max, [2]=matching rounds
    of last step, [3]=error bits, [4]=next serial, [5]=step index,

@@ -401,8 +401,8 @@ class Simulation:
     def profile_steps(self, n):
         """Per-kernel ms from each kernel's own dispatch start / stop events:
         diffuse, scan, scatter, first molsort (relist), pair screen, exact
-        test, resolve."""
-        out = np.zeros(7)
+        test, resolve, tile binning pass 1, tile offsets, tile pass 2."""
+        out = np.zeros(10)
         _check(self.L.smolamdProfileSteps(self.p, n, _dp(out)), "smolamdProfileSteps")
         return out
 

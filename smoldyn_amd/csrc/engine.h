@@ -287,9 +287,9 @@ long engine_noalloc_inject_max();
 // Many steps; returns 0 or an error bit set.
 int engine_run(DeviceState* s, long nsteps, int use_graph);
 // Steps whose kernels carry their own start / stop HIP events (stamped at
-// each dispatch's start and end); out[0..6] = average ms per step of
+// each dispatch's start and end); out[0..9] = average ms per step of
 // diffuse, box scan, scatter, first molsort (surfaces), pair screen, exact
-// test, resolve.
+// test, resolve, tile binning pass 1, tile offsets, tile binning pass 2.
 int engine_profile(DeviceState* s, long nsteps, double* out);
 int engine_sync_control(DeviceState* s, Control* out);
 // Asynchronous stepping for the command loop: enqueue steps, enqueue a copy of

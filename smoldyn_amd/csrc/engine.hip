@@ -167,7 +167,7 @@ struct Glob {
 
 // kernels timed by engine_profile, each with its own start / stop events
 enum TimedSlot : int { kTDiffuse, kTHsum, kTScan2, kTHscan, kTScatter, kTRelist, kTPairs, kTExact, kTResolve,
-                       kTimedSlots };
+                       kTTile1, kTTileScan, kTTile2, kTimedSlots };
 
 struct DeviceState {
   int ninst = 0;
@@ -220,6 +220,15 @@ struct DeviceState {
   int fresh_rec0 = 0;
   std::vector<int> fresh_list, fresh_lp;  // list and live position of each
   uint32_t fresh_step = 0;
+  // tile binning (one simulation without surfaces); `sorted`: the current
+  // buffer holds a binned step's records in key order with its box starts
+  bool tile_ok = false;
+  bool sorted = false;
+  struct TileArrays {
+    int ntile = 0, nkeys = 0, tshift = 0;
+    int *stay = nullptr, *imm = nullptr, *toff = nullptr, *nkey = nullptr, *bucket = nullptr, *nspill = nullptr;
+    int2 *range = nullptr, *spill = nullptr;
+  } ta;
   int4* cand = nullptr;              // screened candidate pairs (k_pairs -> k_exact)
   int candcap = 0;
   bool half1 = false;                // every simulation reacts within one list only (half stencil)
@@ -760,6 +769,17 @@ __device__ __forceinline__ void load16(const int* __restrict__ p, int i0, int n,
   }
 }
 
+// inclusive scan of one value per lane within a wavefront
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int t = __shfl_up(v, off);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
 // block-wide sum of one value per thread (kHThreads threads)
 __device__ __forceinline__ int block_sum(int x, int* s_w) {
   for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o);
@@ -861,7 +881,11 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ ins
   if (kSurf && sb) ll = I.sf.blist[i][bstate(sb)];
   const int rk = a.rank[r];
   const int zs = rk >> kRankBits;
-  int dst = bn.bstart[I.keybase + (ll * I.g.nbox + bx) * I.zsub + zs] + (rk & kRankMask);
+  const int key = I.keybase + (ll * I.g.nbox + bx) * I.zsub + zs;
+  int dst = bn.bstart[key] + (rk & kRankMask);
+  // the key a record was binned with, read by the tile binning of the next
+  // step (one simulation without surfaces)
+  if (!kMulti && !kSurf) b.rank[dst] = key;
   double x, y, z;
   if (kF32) {
     const float fx = a.ox[r], fy = a.oy[r], fz = a.oz[r];
@@ -904,6 +928,455 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ ins
     }
   }
   if (lp >= I.ctl->nl[ll] - I.sortcap) inv[I.L.base[ll] + lp] = dst;
+}
+
+// ---------------------------------------------------------------------------
+// Tile binning (one simulation without surfaces): the diffusion stage and the
+// counting sort into (list, box, z sub-bin) order in two launches.
+//
+// The key space is cut into tiles of 2^tshift consecutive keys.  The source
+// buffer holds the last step's binned records in key order, so each tile's
+// records are one range of it (read from the last step's box starts),
+// followed by an unsorted tail: the last step's products and molecules added
+// since.  Most records keep their tile from one step to the next; the few
+// that leave it (a box change across the tile's edge) and the tail are
+// queued for the tile of their new key.
+//   k_tile1      advances every record (diffuse, checkwalls, pos2box, z
+//                sub-bin, unireact coins; first-order events are queued
+//                here), one thread per record; stores its new position and
+//                key in the source buffer, counts the records that keep their
+//                tile and queues the others;
+//   k_tile_scan  one workgroup: each tile's first destination record (the
+//                records of the tiles before it) and its source range;
+//   k_tile2      reads the keys of the tile's records and of its queued
+//                immigrants, ranks them per key in LDS, scans the tile,
+//                writes the tile's box starts and copies the records to
+//                their places.
+// A record's `rank` field holds the key it was binned with (written by
+// k_tile2 or k_scatter), from which k_tile1 knows its tile; k_tile1 leaves
+// the new key in a separate array, so the old keys stay readable to the
+// neighbors that compare with them.  The stay counts are added per run of records of one
+// tile within a wavefront (records are in key order), so the global atomics
+// of a step number about one per wavefront and hardly ever meet on one
+// address.
+constexpr int kTileMinShift = 8;   // at least kBlock keys per tile (the scan takes whole keys per thread)
+constexpr int kTileMaxShift = 11;
+constexpr int kTileMaxKeys = 1 << kTileMaxShift;
+constexpr int kTileBucket = 64;     // queued immigrants per tile; more go to the shared spill list
+constexpr int kTileCache = 4;       // records per thread that k_tile2 keeps in registers between its passes
+
+struct Tiles {
+  int ntile, nkeys, tshift;
+  int* stay;    // [2 * ntile] records that keep their tile, by step parity (k_tile2 clears the other parity)
+  int* imm;     // [2 * ntile] immigrants per tile, by step parity
+  int* toff;    // [ntile] first destination record of each tile
+  int2* range;  // [2 * ntile] source records [x, y) of the tile by step parity ((0, 0): none), set by k_tile1
+  int* nkey;    // [record capacity] new key of each source record (-1: not binned), set by k_tile1
+  int* bucket;  // [ntile * kTileBucket] source records of the immigrants
+  int2* spill;  // [record capacity] (record, key) of immigrants beyond a full bucket
+  int* nspill;  // [2] by step parity
+};
+
+// The fields of one source record that the diffusion stage reads.
+struct TileIn {
+  double pos[3];
+  uint64_t ser;
+  int i, lp, bold, okey;
+};
+
+template <bool kF32>
+__device__ __forceinline__ void tile_fetch(const Recs& a, const Geometry& g, int r, bool sorted, TileIn& in) {
+  in.i = a.ident[r];
+  in.lp = a.lpos[r];
+  in.ser = a.serno[r];
+  in.okey = sorted ? a.rank[r] : -1;
+  in.bold = kF32 ? a.box[r] : 0;
+  load_pos<kF32>(a, g, r, in.bold, in.pos);
+}
+
+// diffuse (smolmolec.c:3115-3168), checkwalls (smolwall.c:368-424),
+// pos2box (smolboxes.c:67-79), the z sub-bin and the unireact coins
+// (smolreact.c:2997-3054) of record r of the source buffer (fields in `in`),
+// as k_diffuse computes them for a simulation without surfaces.  The new
+// position (and, for FP32-offset records, the box it is stored relative to)
+// goes back to the record; the return value is the record's new key, -1 for
+// a record that is not binned (no molecule, or a first-order reactant, whose
+// event is queued here).
+template <bool kF32>
+__device__ __forceinline__ int tile_advance(const Inst& I, const SpecRow* S, const int* Lim, const Recs& a, int r,
+                                            const TileIn& in, const double* __restrict__ gtab, int gmask,
+                                            const double* __restrict__ inj, uint32_t step, Glob* gl) {
+  const int i = in.i, lp = in.lp, bold = in.bold;
+  const uint64_t ser = in.ser;
+  const Geometry& g = I.g;
+  double pos[3] = {in.pos[0], in.pos[1], in.pos[2]};
+  if (i <= 0 || i >= kMaxSpecies) {
+    if (i != 0) atomicOr((int*)&gl->halt, 1);
+    return -1;
+  }
+  const int ll = S[i].ll;
+  const bool moved = lp < Lim[ll];
+  if (moved) {
+    const double ds = S[i].ds;
+    double gg[3];
+    if (inj) {
+      const long k = (long)I.ctl->pfx[ll] + lp;
+      gg[0] = inj[3 * k];
+      gg[1] = inj[3 * k + 1];
+      gg[2] = inj[3 * k + 2];
+    } else {
+      const smolrng4 rn = smolrng_mol(I.key0, I.key1, ser, step, SMOLRNG_DIFFUSE, 0);
+      gg[0] = gtab[rn.v[0] & (uint32_t)gmask];
+      gg[1] = gtab[rn.v[1] & (uint32_t)gmask];
+      gg[2] = gtab[rn.v[2] & (uint32_t)gmask];
+    }
+    pos[0] += ds * gg[0];
+    pos[1] += ds * gg[1];
+    pos[2] += ds * gg[2];
+    apply_walls(g, pos);
+  }
+  const int b = pos2box(g, pos[0], pos[1], pos[2]);
+  // FP32-offset records are stored relative to the new box
+  if (moved || (kF32 && b != bold)) {
+    store_pos<kF32>(a, g, r, b, pos);
+    if (kF32) a.box[r] = b;
+  }
+  int zs = 0;
+  if (I.zsub > 1) {
+    const double zrel = (pos[2] - g.bmin[2]) / g.bsize[2];
+    int iz = (int)zrel;
+    if (iz < 0) iz = 0;
+    else if (iz >= g.side[2]) iz = g.side[2] - 1;
+    zs = (int)((zrel - (double)iz) * (double)I.zsub);
+    if (zs < 0) zs = 0;
+    else if (zs >= I.zsub) zs = I.zsub - 1;
+  }
+  const int nu = S[i].nu;
+  if (nu > 0) {
+    smolrng4 rb;
+    int blk = -1;
+    for (int j = 0; j < nu; j++) {
+      if (blk != j / 4) {
+        blk = j / 4;
+        rb = smolrng_mol(I.key0, I.key1, ser, step, SMOLRNG_UNIRXN, (uint32_t)blk);
+      }
+      if (smolrng_u01(rb.v[j & 3]) < S[i].prob[j]) {
+        Control* ctl = I.ctl;
+        const int e = atomicAdd(&ctl->nuni, 1);
+        if (e < I.maxevents) {
+          I.uniev[e].rec = r;
+          I.uniev[e].slot = I.L.base[ll] + lp;
+          I.uniev[e].r = S[i].rx[j];
+          I.uniev[e].box = b;
+        } else {
+          atomicOr(&ctl->err, kErrEventOverflow);
+        }
+        return -1;
+      }
+    }
+  }
+  return I.keybase + (ll * g.nbox + b) * I.zsub + zs;
+}
+
+// Queue record r for the tile of its new key.
+__device__ __forceinline__ void tile_queue(const Tiles& tl, int* imm, int* nspill, int key, int r) {
+  const int dt = key >> tl.tshift;
+  const int q = atomicAdd(&imm[dt], 1);
+  if (q < kTileBucket) {
+    tl.bucket[dt * kTileBucket + q] = r;
+  } else {  // every record is queued at most once a step: the list never fills
+    const int j = atomicAdd(nspill, 1);
+    tl.spill[j] = make_int2(r, key);
+  }
+}
+
+// Adds, for each run of lanes of one tile (tile >= 0) in the wavefront, the
+// number of its lanes with flag set to cnt[tile].  Every lane of the wave
+// must call this.
+__device__ __forceinline__ void wave_run_count(int* cnt, int tile, bool flag) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(tile, 1);
+  const bool start = lane == 0 || prev != tile;
+  const unsigned long long starts = __ballot(start), fl = __ballot(flag);
+  if (start && tile >= 0) {
+    const unsigned long long above = starts & ~((2ull << lane) - 1);
+    const int nxt = above ? __ffsll((long long)above) - 1 : 64;
+    const unsigned long long run = (nxt == 64 ? ~0ull : ((1ull << nxt) - 1)) & ~((1ull << lane) - 1);
+    const int c = __popcll(fl & run);
+    if (c) atomicAdd(&cnt[tile], c);
+  }
+}
+
+// kPer records per thread, a block's records in kPer runs of kBlock: every
+// record's fields are loaded before the first one is advanced.
+constexpr int kTile1Per = 2;
+template <bool kF32>
+__global__ __launch_bounds__(kBlock) void k_tile1(const Inst* __restrict__ insts, Glob* __restrict__ gl, Recs a,
+                                                  const int* __restrict__ bstart, Tiles tl,
+                                                  const double* __restrict__ gtab, int gmask,
+                                                  const double* __restrict__ inj) {
+  const int n = live_nrec(gl);
+  const int r0 = blockIdx.x * kBlock * kTile1Per;
+  if (n < 0 || r0 >= n) return;  // uniform over the block
+  const Inst& I = insts[0];
+  __shared__ SpecRow s_spec[kBlock / 64][kMaxSpecies];
+  __shared__ int s_lim[kBlock / 64][kMaxList];
+  SpecRow* S = s_spec[threadIdx.x >> 6];
+  int* Lim = s_lim[threadIdx.x >> 6];
+  const uint32_t step = I.ctl->step;
+  const int par = step & 1;
+  const int nb0 = bstart[tl.nkeys];  // records binned by the last step; the tail follows
+  TileIn in[kTile1Per];
+#pragma unroll
+  for (int j = 0; j < kTile1Per; j++) {
+    const int r = min(r0 + j * kBlock + (int)threadIdx.x, n - 1);  // a lane past the end reads the last record
+    tile_fetch<kF32>(a, I.g, r, r < nb0, in[j]);
+  }
+  fill_spec_rows(I, S, Lim);
+  const int lane = threadIdx.x & 63;
+  int2* range = tl.range + par * tl.ntile;
+#pragma unroll
+  for (int j = 0; j < kTile1Per; j++) {
+    const int r = r0 + j * kBlock + threadIdx.x;
+    const int otile = r < min(n, nb0) ? in[j].okey >> tl.tshift : -1;
+    // the first and the last record of each tile's source range: its
+    // neighbors in the buffer (the next lanes of the wave) belong to another
+    // tile, or to none
+    int prev = __shfl_up(otile, 1), next = __shfl_down(otile, 1);
+    if (lane == 0) prev = (otile >= 0 && r > 0) ? a.rank[r - 1] >> tl.tshift : -1;
+    if (lane == 63) next = (otile >= 0 && r + 1 < nb0) ? a.rank[r + 1] >> tl.tshift : -1;
+    if (otile >= 0 && prev != otile) range[otile].x = r;
+    if (otile >= 0 && next != otile) range[otile].y = r + 1;
+    int key = -1;
+    if (r < n) {
+      key = tile_advance<kF32>(I, S, Lim, a, r, in[j], gtab, gmask, inj, step, gl);
+      tl.nkey[r] = key;
+    }
+    const bool stay = key >= 0 && (key >> tl.tshift) == otile;
+    if (key >= 0 && !stay) tile_queue(tl, tl.imm + par * tl.ntile, tl.nspill + par, key, r);
+    wave_run_count(tl.stay + par * tl.ntile, otile, stay);
+  }
+}
+
+// Each tile's first destination record (one workgroup; every thread takes a
+// run of consecutive tiles, eight at a time, with all of a batch's loads
+// issued before they are used).
+constexpr int kTileScanThreads = 1024;
+constexpr int kTileScanBatch = 8;
+__global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(const Inst* __restrict__ insts,
+                                                                const Glob* __restrict__ gl, Tiles tl) {
+  if (live_nrec(gl) < 0) return;
+  __shared__ int s_w[kTileScanThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int par = insts[0].ctl->step & 1;
+  const int* stay = tl.stay + par * tl.ntile;
+  const int* imm = tl.imm + par * tl.ntile;
+  const int chunk = (tl.ntile + kTileScanThreads - 1) / kTileScanThreads;
+  const int t0 = min(tid * chunk, tl.ntile), t1 = min(t0 + chunk, tl.ntile);
+  int v[kTileScanBatch];
+  int sum = 0;
+  for (int t = t0; t < t1; t += kTileScanBatch) {
+#pragma unroll
+    for (int j = 0; j < kTileScanBatch; j++) v[j] = t + j < t1 ? stay[t + j] + imm[t + j] : 0;
+#pragma unroll
+    for (int j = 0; j < kTileScanBatch; j++) sum += v[j];
+  }
+  const int inc = wave_incl_scan(sum);
+  if (lane == 63) s_w[wv] = inc;
+  __syncthreads();
+  int run = inc - sum;
+  for (int w = 0; w < wv; w++) run += s_w[w];
+  for (int t = t0; t < t1; t += kTileScanBatch) {
+    if (chunk > kTileScanBatch) {  // the counts of this batch again (the first loop kept only the last batch)
+#pragma unroll
+      for (int j = 0; j < kTileScanBatch; j++) v[j] = t + j < t1 ? stay[t + j] + imm[t + j] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kTileScanBatch; j++) {
+      if (t + j < t1) {
+        tl.toff[t + j] = run;
+        run += v[j];
+      }
+    }
+  }
+}
+
+// One record as k_tile2 copies it.
+struct TileRec {
+  double p[3];  // FP64 position
+  float f[3];   // FP32 offsets from the origin of the record's new box
+  uint64_t ser;
+  int key, ident, lp;
+};
+
+template <bool kF32>
+__device__ __forceinline__ void tile_load(const Recs& a, const int* __restrict__ nkey, int r, TileRec& o) {
+  o.key = nkey[r];
+  o.ident = a.ident[r];
+  o.lp = a.lpos[r];
+  o.ser = a.serno[r];
+  if (kF32) {
+    o.f[0] = a.ox[r];
+    o.f[1] = a.oy[r];
+    o.f[2] = a.oz[r];
+  } else {
+    o.p[0] = a.px[r];
+    o.p[1] = a.py[r];
+    o.p[2] = a.pz[r];
+  }
+}
+
+// Place a record at dst of the destination buffer, with its packed FP32
+// screen copy (as k_scatter writes it) and, for the last slots of its list,
+// the inverse map.
+template <bool kF32>
+__device__ __forceinline__ void tile_place(const Inst& I, const TileRec& o, int dst, const Recs& b,
+                                           const Binned& bn, int* __restrict__ inv) {
+  const Geometry& g = I.g;
+  const int zsub = I.zsub;
+  const int kk = o.key - I.keybase;
+  const int zs = kk & (zsub - 1);
+  const int bx = (kk / zsub) % g.nbox;
+  const int ll = kk / (zsub * g.nbox);
+  double x, y, z;
+  if (kF32) {
+    b.ox[dst] = o.f[0];
+    b.oy[dst] = o.f[1];
+    b.oz[dst] = o.f[2];
+    double og[3];
+    box_origin(g, bx, og);
+    x = og[0] + (double)o.f[0];
+    y = og[1] + (double)o.f[1];
+    z = og[2] + (double)o.f[2];
+  } else {
+    x = o.p[0];
+    y = o.p[1];
+    z = o.p[2];
+    b.px[dst] = x;
+    b.py[dst] = y;
+    b.pz[dst] = z;
+  }
+  b.ident[dst] = o.ident;
+  b.serno[dst] = o.ser;
+  b.lpos[dst] = o.lp;
+  b.box[dst] = bx;
+  b.rank[dst] = o.key;
+  const double lim = I.screen_lim;
+  const int far = !(fabs(x) <= lim && fabs(y) <= lim && fabs(z) <= lim);
+  bn.spk[dst] = make_float4((float)x, (float)y, (float)z, __int_as_float(o.ident | (far << 16) | (zs << 17)));
+  if (o.lp >= I.ctl->nl[ll] - I.sortcap) inv[I.L.base[ll] + o.lp] = dst;
+}
+
+template <bool kF32>
+__global__ __launch_bounds__(kBlock) void k_tile2(const Inst* __restrict__ insts, Glob* __restrict__ gl, Recs a,
+                                                  Recs b, Binned bn, Tiles tl, int* __restrict__ inv) {
+  const int n = live_nrec(gl);
+  if (n < 0) return;
+  const Inst& I = insts[0];
+  __shared__ int s_w[kBlock / 64];
+  __shared__ int s_cnt[kTileMaxKeys];   // records per key, then each key's first destination record
+  __shared__ int s_more[kTileMaxKeys];  // records per key among the cached ones, then the next free place
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int par = I.ctl->step & 1;
+  const int* imm = tl.imm + par * tl.ntile;
+  const int t = blockIdx.x;
+  const int tk = 1 << tl.tshift, k0 = t << tl.tshift;
+  const int off = tl.toff[t];
+  for (int k = tid; k < tk; k += kBlock) s_cnt[k] = 0;
+  const int2 rg = tl.range[par * tl.ntile + t];
+  const int nin = imm[t];
+  const int nbk = min(nin, kTileBucket);
+  const int nsp = nin > kTileBucket ? tl.nspill[par] : 0;
+  const int ns = rg.y - rg.x;
+  const int nitems = ns + nbk + nsp;
+  __syncthreads();
+  // the next step's queue counters (every tile has read this step's above)
+  if (tid == 0) {
+    tl.imm[(par ^ 1) * tl.ntile + t] = 0;
+    tl.stay[(par ^ 1) * tl.ntile + t] = 0;
+    tl.range[(par ^ 1) * tl.ntile + t] = make_int2(0, 0);
+    if (t == 0) tl.nspill[par ^ 1] = 0;
+  }
+  // item idx: the tile's source range, then the bucket, then the spill list
+  // (entries of other tiles there give -1)
+  auto item_rec = [&](int idx) -> int {
+    if (idx < ns) return rg.x + idx;
+    idx -= ns;
+    if (idx < nbk) return tl.bucket[t * kTileBucket + idx];
+    const int2 sp = tl.spill[idx - nbk];
+    return (sp.y >> tl.tshift) == t ? sp.x : -1;
+  };
+  TileRec c[kTileCache];
+  int rk[kTileCache];
+#pragma unroll
+  for (int j = 0; j < kTileCache; j++) {
+    const int idx = tid + j * kBlock;
+    rk[j] = -1;
+    const int r = idx < nitems ? item_rec(idx) : -1;
+    if (r >= 0) {
+      tile_load<kF32>(a, tl.nkey, r, c[j]);
+      if (c[j].key >= 0 && (c[j].key >> tl.tshift) == t) rk[j] = atomicAdd(&s_cnt[c[j].key - k0], 1);
+    }
+  }
+  const bool more = nitems > kTileCache * kBlock;  // uniform over the block
+  __syncthreads();
+  if (more) {  // a crowded tile: the records beyond the cached ones are read again to be placed
+    for (int k = tid; k < tk; k += kBlock) s_more[k] = s_cnt[k];
+    __syncthreads();
+    for (int idx = kTileCache * kBlock + tid; idx < nitems; idx += kBlock) {
+      const int r = item_rec(idx);
+      if (r < 0) continue;
+      const int key = tl.nkey[r];
+      if (key >= 0 && (key >> tl.tshift) == t) atomicAdd(&s_cnt[key - k0], 1);
+    }
+    __syncthreads();
+  }
+  // exclusive scan of the tile's counts: tk / kBlock consecutive keys per thread
+  {
+    const int per = tk / kBlock;
+    int v[kTileMaxKeys / kBlock];
+    int tsum = 0;
+#pragma unroll
+    for (int q = 0; q < kTileMaxKeys / kBlock; q++) {
+      v[q] = q < per ? s_cnt[tid * per + q] : 0;
+      tsum += v[q];
+    }
+    const int inc = wave_incl_scan(tsum);
+    if (lane == 63) s_w[wv] = inc;
+    __syncthreads();
+    int base = off + inc - tsum, total = 0;
+    for (int w = 0; w < kBlock / 64; w++) {
+      const int x = s_w[w];
+      if (w < wv) base += x;
+      total += x;
+    }
+    const int nk = min(tk, tl.nkeys - k0);
+#pragma unroll
+    for (int q = 0; q < kTileMaxKeys / kBlock; q++) {
+      if (q < per) {
+        const int kk = tid * per + q;
+        s_cnt[kk] = base;
+        if (kk < nk) bn.bstart[k0 + kk] = base;
+        base += v[q];
+      }
+    }
+    if (t == tl.ntile - 1 && tid == 0) bn.bstart[tl.nkeys] = off + total;  // records binned
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kTileCache; j++)
+    if (rk[j] >= 0) tile_place<kF32>(I, c[j], s_cnt[c[j].key - k0] + rk[j], b, bn, inv);
+  if (more) {
+    for (int k = tid; k < tk; k += kBlock) s_more[k] += s_cnt[k];
+    __syncthreads();
+    for (int idx = kTileCache * kBlock + tid; idx < nitems; idx += kBlock) {
+      const int r = item_rec(idx);
+      if (r < 0) continue;
+      TileRec o;
+      tile_load<kF32>(a, tl.nkey, r, o);
+      if (o.key >= 0 && (o.key >> tl.tshift) == t) tile_place<kF32>(I, o, atomicAdd(&s_more[o.key - k0], 1), b, bn, inv);
+    }
+  }
 }
 
 // Full inverse map of the current buffer, for reading the state back.
@@ -1473,16 +1946,6 @@ __device__ void wg_sort_any(uint64_t* gk, int* gv, int n, uint64_t* lk, int* lv,
     }
     __syncthreads();
   }
-}
-
-__device__ __forceinline__ int wave_incl_scan(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    int t = __shfl_up(v, off);
-    if (lane >= off) v += t;
-  }
-  return v;
 }
 
 // exclusive scan of one value per thread (wave scans by shuffles, then the
@@ -2785,6 +3248,27 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
           dalloc(&s->d_act, (size_t)surf->nsrf * kMaxSpecies * 2) |
           dalloc(&s->d_actb, (size_t)surf->nsrf * kMaxSpecies * 2) | dalloc(&s->moves, nev) | dalloc(&s->wq, nev);
   }
+  // tile binning: one simulation without surfaces (SMOLDYN_AMD_TILE=0 keeps
+  // the four-launch counting sort for every step).  Tiles are sized for about
+  // 640 records, so most fit the records k_tile2 keeps in registers.
+  {
+    const char* env = getenv("SMOLDYN_AMD_TILE");
+    if (!s->multi && !(surf && surf->on) && !(env && env[0] == '0')) {
+      const int nkeys = s->nscan - 1;
+      const double per_key = fmax(cfg[0].expected_per_box, 1e-3) / (double)s->hinst[0].zsub;
+      int sh = kTileMinShift;
+      while (sh < kTileMaxShift && (double)(1 << (sh + 1)) * per_key <= 640.0) sh++;
+      auto& T = s->ta;
+      T.tshift = sh;
+      T.nkeys = nkeys;
+      T.ntile = (nkeys + (1 << sh) - 1) >> sh;
+      ok |= dalloc(&T.toff, T.ntile) | dalloc(&T.nkey, rc);
+      ok |= dalloc(&T.stay, 2 * (size_t)T.ntile) | dalloc(&T.range, 2 * (size_t)T.ntile) |
+            dalloc(&T.imm, 2 * (size_t)T.ntile) |
+            dalloc(&T.bucket, (size_t)T.ntile * kTileBucket) | dalloc(&T.spill, rc) | dalloc(&T.nspill, 2);
+      s->tile_ok = true;
+    }
+  }
   if (ok) {
     snprintf(err, errlen, "device allocation failed (%d molecule records)", s->reccap);
     engine_destroy(s);
@@ -2838,6 +3322,12 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
       if (surf->pnl[k].ps == kPsTri || surf->pnl[k].ps == kPsCyl || surf->pnl[k].ps == kPsHemi) s->surf_curved = true;
   }
   (void)hipMemset(s->bn.count, 0, sizeof(int) * s->nscan);
+  if (s->tile_ok) {
+    (void)hipMemset(s->ta.imm, 0, sizeof(int) * 2 * s->ta.ntile);
+    (void)hipMemset(s->ta.stay, 0, sizeof(int) * 2 * s->ta.ntile);
+    (void)hipMemset(s->ta.range, 0, sizeof(int2) * 2 * s->ta.ntile);
+    (void)hipMemset(s->ta.nspill, 0, sizeof(int) * 2);
+  }
   (void)hipMemset(s->bestkey, 0xFF, sizeof(uint64_t) * rc);
   (void)hipMemset(s->killed, 0, sizeof(int) * rc);
   (void)hipMemset(s->ctl, 0, sizeof(Control) * n);
@@ -2894,7 +3384,8 @@ void engine_destroy(DeviceState* s) {
                   s->accepted, s->bestkey, s->killed,  s->uniev,   s->log,       s->sortkey, s->sortval, s->killslot,
                   s->ctl,     s->tab,     s->gtab,    s->inj,     s->prods,     s->d_pnl,   s->d_bpstart,
                   s->d_bplist, s->d_act,  s->d_actb,  s->moves,   s->dinst,   s->glob,    s->obs,
-                  s->wq,      s->srow};
+                  s->wq,      s->srow,    s->ta.stay, s->ta.range, s->ta.imm, s->ta.bucket, s->ta.spill,
+                  s->ta.nspill, s->ta.toff,   s->ta.nkey};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -2963,6 +3454,7 @@ int engine_upload_list_inst(DeviceState* s, int inst, int ll, int n, const int* 
     HIPOK(hipMemcpy(s->inv + I.L.base[ll], inv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   }
   s->hnrec = r0 + n;
+  s->sorted = false;
   c.nl[ll] = n;
   int run = 0;
   for (int l2 = 0; l2 < I.L.nlist; l2++) {
@@ -3043,6 +3535,7 @@ int engine_append_fresh(DeviceState* s, int ll, int n, const int* ident, const d
     if (serno[k] >= c.serno_next) c.serno_next = serno[k] + 1;
   }
   g.nrec[slot] = r0 + n;
+  s->sorted = false;  // the next step bins the added records with the counting sort
   HIPOK(hipMemcpy(s->ctl, &c, sizeof(Control), hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(s->glob, &g, sizeof(Glob), hipMemcpyHostToDevice));
   return 0;
@@ -3073,7 +3566,7 @@ int engine_set_counters(DeviceState* s, uint64_t serno_next, uint32_t step, cons
 // Launch one step from buffer src into 1-src on s->stream.  Grids cover the
 // record capacity; kernels read the exact counts from device memory.
 // timed: events between the kernels into ev[0..7].
-static void launch_step(DeviceState* s, int src, const double* dinj, int timed) {
+static void launch_step(DeviceState* s, int src, const double* dinj, int timed, bool tile) {
   int nb = (s->reccap + kBlock - 1) / kBlock;
   hipStream_t st = s->stream;
   Recs A = s->buf[src], B = s->buf[1 - src];
@@ -3110,17 +3603,36 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
       hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
     }
   };
-  go(kTDiffuse, kdiff, dim3(nb), dim3(kBlock), s->dinst, s->glob, A, s->bn.count, (const double*)s->gtab,
-     s->ngauss - 1, dinj, s->inv);
-  {
+  if (tile) {
+    Tiles tl;
+    tl.ntile = s->ta.ntile;
+    tl.toff = s->ta.toff;
+    tl.nkey = s->ta.nkey;
+    tl.nkeys = s->ta.nkeys;
+    tl.tshift = s->ta.tshift;
+    tl.stay = s->ta.stay;
+    tl.range = s->ta.range;
+    tl.imm = s->ta.imm;
+    tl.bucket = s->ta.bucket;
+    tl.spill = s->ta.spill;
+    tl.nspill = s->ta.nspill;
+    go(kTTile1, f ? k_tile1<true> : k_tile1<false>, dim3((nb + kTile1Per - 1) / kTile1Per), dim3(kBlock), s->dinst,
+       s->glob, A,
+       (const int*)s->bn.bstart, tl, (const double*)s->gtab, s->ngauss - 1, dinj);
+    go(kTTileScan, k_tile_scan, dim3(1), dim3(kTileScanThreads), s->dinst, (const Glob*)s->glob, tl);
+    go(kTTile2, f ? k_tile2<true> : k_tile2<false>, dim3(tl.ntile), dim3(kBlock), s->dinst, s->glob, A, B, s->bn,
+       tl, s->inv);
+  } else {
+    go(kTDiffuse, kdiff, dim3(nb), dim3(kBlock), s->dinst, s->glob, A, s->bn.count, (const double*)s->gtab,
+       s->ngauss - 1, dinj, s->inv);
     const int ntile = (s->nscan + kHTile - 1) / kHTile;
     go(kTHsum, k_hsum, dim3(ntile), dim3(kHThreads), (const int*)s->bn.count, s->bn.boff, s->nscan);
     const int scanned = ntile > kHFusedTiles;
     if (scanned) go(kTScan2, k_scan2, dim3(1), dim3(kScanBlock), s->bn.boff, ntile);
     go(kTHscan, k_hscan, dim3(ntile), dim3(kHThreads), s->bn.count, (const int*)s->bn.boff, s->bn.bstart, s->nscan,
        scanned);
+    go(kTScatter, kscat, dim3(nb), dim3(kBlock), s->dinst, s->glob, A, B, s->bn, s->inv);
   }
-  go(kTScatter, kscat, dim3(nb), dim3(kBlock), s->dinst, s->glob, A, B, s->bn, s->inv);
   if (s->surf_on)  // first molsort: list changes from adsorption and desorption
     go(kTRelist, k_relist, dim3(s->ninst), dim3(kResolveThreads), s->dinst, s->glob, A, B, s->inv, s->reccap,
        s->bn);
@@ -3128,6 +3640,50 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed) 
   go(kTExact, kexact, dim3(kExactBlocks), dim3(kBlock), s->dinst, s->glob, B, s->bn, s->cand, s->candcap);
   go(kTResolve, kres, dim3(s->ninst), dim3(kResolveThreads), s->dinst, s->glob, A, B, s->inv, s->bn, s->nscan,
      s->bestkey, s->killed, s->reccap);
+}
+
+// One step from the current buffer on the stream.  The tile binning needs the
+// source buffer in key order; right after an upload or an addition of
+// molecules it is not, and the step bins with the counting sort, which leaves
+// its destination in order (and the tile queue counters are cleared for the
+// tile steps that follow).
+static void launch_any(DeviceState* s, const double* dinj, int timed) {
+  const bool tile = s->tile_ok && s->sorted;
+  launch_step(s, s->cur, dinj, timed, tile);
+  if (s->tile_ok && !tile) {
+    (void)hipMemsetAsync(s->ta.imm, 0, sizeof(int) * 2 * s->ta.ntile, s->stream);
+    (void)hipMemsetAsync(s->ta.stay, 0, sizeof(int) * 2 * s->ta.ntile, s->stream);
+    (void)hipMemsetAsync(s->ta.range, 0, sizeof(int2) * 2 * s->ta.ntile, s->stream);
+    (void)hipMemsetAsync(s->ta.nspill, 0, sizeof(int) * 2, s->stream);
+  }
+  s->sorted = true;
+}
+
+// Captured step graphs of both buffer parities (with the tile binning when
+// the engine has it).
+static int capture_graphs(DeviceState* s) {
+  for (int k = 0; k < 2; k++) {
+    if (s->graph_ready[k]) continue;
+    hipGraph_t g;
+    HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    launch_step(s, k, nullptr, 0, s->tile_ok);
+    HIPOK(hipStreamEndCapture(s->stream, &g));
+    HIPOK(hipGraphInstantiate(&s->graph[k], g, nullptr, nullptr, 0));
+    HIPOK(hipGraphDestroy(g));
+    s->graph_ready[k] = 1;
+  }
+  return 0;
+}
+
+// Before graph replays: a step that must bin with the counting sort runs
+// eagerly; returns the steps it took.
+static int sort_step_before_graphs(DeviceState* s, long nsteps) {
+  if (nsteps > 0 && s->tile_ok && !s->sorted) {
+    launch_any(s, nullptr, 0);
+    s->cur ^= 1;
+    return 1;
+  }
+  return 0;
 }
 
 // OR of the error bits of all simulations
@@ -3179,7 +3735,7 @@ int engine_step_noalloc(DeviceState* s, const double* inject, long ninject) {
     if (s->htab[0].diffuselist[ll]) need += c0.nl[ll] - c0.nfresh[ll];
   if (ninject < need) return -2;
   if (need > 0) HIPOK(hipMemcpyAsync(ptr, inject, sizeof(double) * 3 * need, hipMemcpyHostToDevice, s->stream));
-  launch_step(s, s->cur, ptr, 0);
+  launch_any(s, ptr, 0);
   HIPOK(hipGetLastError());
   HIPOK(hipStreamSynchronize(s->stream));
   s->cur ^= 1;
@@ -3210,7 +3766,7 @@ int engine_step(DeviceState* s, const double* inject, long ninject) {
     HIPOK(hipMemcpy(s->inj, inject, sizeof(double) * 3 * ninject, hipMemcpyHostToDevice));
     dinj = s->inj;
   }
-  launch_step(s, s->cur, dinj, 0);
+  launch_any(s, dinj, 0);
   HIPOK(hipGetLastError());
   HIPOK(hipStreamSynchronize(s->stream));
   s->cur ^= 1;
@@ -3221,23 +3777,17 @@ int engine_run(DeviceState* s, long nsteps, int use_graph) {
   hipEvent_t e0 = s->ev[8], e1 = s->ev[9];
   // one captured graph per buffer parity: the launch sequence reads every
   // count from device memory, so the graphs replay for all steps
-  if (use_graph) {
-    for (int k = 0; k < 2; k++) {
-      if (s->graph_ready[k]) continue;
-      hipGraph_t g;
-      HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-      launch_step(s, k, nullptr, 0);
-      HIPOK(hipStreamEndCapture(s->stream, &g));
-      HIPOK(hipGraphInstantiate(&s->graph[k], g, nullptr, nullptr, 0));
-      HIPOK(hipGraphDestroy(g));
-      s->graph_ready[k] = 1;
-    }
-  }
+  if (use_graph && capture_graphs(s)) return -1;
   HIPOK(hipEventRecord(e0, s->stream));
-  for (long k = 0; k < nsteps; k++) {
-    if (use_graph) HIPOK(hipGraphLaunch(s->graph[s->cur], s->stream));
-    else launch_step(s, s->cur, nullptr, 0);
-    s->cur ^= 1;
+  long k = use_graph ? sort_step_before_graphs(s, nsteps) : 0;
+  for (; k < nsteps; k++) {
+    if (use_graph) {
+      HIPOK(hipGraphLaunch(s->graph[s->cur], s->stream));
+      s->cur ^= 1;
+    } else {
+      launch_any(s, nullptr, 0);
+      s->cur ^= 1;
+    }
   }
   HIPOK(hipEventRecord(e1, s->stream));
   HIPOK(hipStreamSynchronize(s->stream));
@@ -3260,17 +3810,8 @@ __global__ __launch_bounds__(64) void k_observe(const Control* __restrict__ ctl,
 }
 
 int engine_enqueue(DeviceState* s, long nsteps) {
-  for (int k = 0; k < 2; k++) {
-    if (s->graph_ready[k]) continue;
-    hipGraph_t g;
-    HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    launch_step(s, k, nullptr, 0);
-    HIPOK(hipStreamEndCapture(s->stream, &g));
-    HIPOK(hipGraphInstantiate(&s->graph[k], g, nullptr, nullptr, 0));
-    HIPOK(hipGraphDestroy(g));
-    s->graph_ready[k] = 1;
-  }
-  for (long k = 0; k < nsteps; k++) {
+  if (capture_graphs(s)) return -1;
+  for (long k = sort_step_before_graphs(s, nsteps); k < nsteps; k++) {
     HIPOK(hipGraphLaunch(s->graph[s->cur], s->stream));
     s->cur ^= 1;
   }
@@ -3304,11 +3845,13 @@ int engine_read_obs(DeviceState* s, int nrow, int* out) {
 // test, resolve.
 int engine_profile(DeviceState* s, long nsteps, double* out) {
   // outputs: diffuse, scan (k_hsum + k_scan2 + k_hscan), scatter, relist,
-  // pairs, exact, resolve; kernels a step did not launch count 0
-  static const int group[kTimedSlots] = {0, 1, 1, 1, 2, 3, 4, 5, 6};
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  // pairs, exact, resolve, tile pass 1, tile offsets, tile pass 2; kernels a
+  // step did not launch count 0
+  static const int group[kTimedSlots] = {0, 1, 1, 1, 2, 3, 4, 5, 6, 7, 8, 9};
+  constexpr int kGroups = 10;
+  double acc[kGroups] = {};
   for (long k = 0; k < nsteps; k++) {
-    launch_step(s, s->cur, nullptr, 1);
+    launch_any(s, nullptr, 1);
     HIPOK(hipStreamSynchronize(s->stream));
     s->cur ^= 1;
     for (int j = 0; j < kTimedSlots; j++) {
@@ -3318,7 +3861,7 @@ int engine_profile(DeviceState* s, long nsteps, double* out) {
       acc[group[j]] += ms;
     }
   }
-  for (int j = 0; j < 7; j++) out[j] = acc[j] / (double)(nsteps > 0 ? nsteps : 1);
+  for (int j = 0; j < kGroups; j++) out[j] = acc[j] / (double)(nsteps > 0 ? nsteps : 1);
   return read_errors(s);
 }
 
