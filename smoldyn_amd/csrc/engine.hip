@@ -122,13 +122,27 @@ struct SpecRow {
 };
 
 // One simulation of the batch, as the kernels see it (device memory).
+// Device arrays reached through a simulation's view are declared in the
+// global address space for the device compilation: loads and stores through
+// them are global memory instructions, whose waits are separate from LDS
+// waits (through a generic pointer they would be flat instructions, which
+// every LDS wait also waits for).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SMOLAMD_GLOBAL __attribute__((address_space(1)))
+#else
+#define SMOLAMD_GLOBAL
+#endif
+#define GAS SMOLAMD_GLOBAL
+// host pointer -> the view's global pointer type
+#define AS_VIEW(T, p) ((GAS T*)(p))
+
 struct Inst {
   Layout L;        // list slots in the batch-wide slot numbering
   Geometry g;
   DevSurf sf;
-  const DevTables* tab;
-  const SpecRow* srow;  // kMaxSpecies rows
-  Control* ctl;
+  GAS const DevTables* tab;
+  GAS const SpecRow* srow;  // kMaxSpecies rows
+  GAS Control* ctl;
   PairConst pc;
   uint32_t key0, key1;
   int keybase;     // first histogram key of this simulation
@@ -138,19 +152,19 @@ struct Inst {
   // in the same or an adjacent sub-bin of a neighbor column
   int zsub;
   double screen_lim;  // coordinates beyond this bypass the FP32 pair screen
-  UniEvent* uniev;
+  GAS UniEvent* uniev;
   int maxevents;
-  Edge* edges;
+  GAS Edge* edges;
   int maxedges;
-  int* estate;
-  int* accepted;
-  LogRow* log;
-  uint64_t* sk;
-  int* sv;
+  GAS int* estate;
+  GAS int* accepted;
+  GAS LogRow* log;
+  GAS uint64_t* sk;
+  GAS int* sv;
   int sortcap;
-  MoveEv* moves;  // list changes of the step (surfaces), capacity maxevents
-  int* killslot;
-  Prod* prods;
+  GAS MoveEv* moves;  // list changes of the step (surfaces), capacity maxevents
+  GAS int* killslot;
+  GAS Prod* prods;
 };
 
 // Batch-wide counters.  The record count of the current buffer is kept per
@@ -966,7 +980,7 @@ constexpr int kTileBucket = 64;     // queued immigrants per tile; more go to th
 constexpr int kTileCache = 4;       // records per thread that k_tile2 keeps in registers between its passes
 
 struct Tiles {
-  int ntile, nkeys, tshift;
+  int ntile, nkeys, tshift, reccap;
   int* stay;    // [2 * ntile] records that keep their tile, by step parity (k_tile2 clears the other parity)
   int* imm;     // [2 * ntile] immigrants per tile, by step parity
   int* toff;    // [ntile] first destination record of each tile
@@ -985,51 +999,76 @@ struct TileIn {
 };
 
 template <bool kF32>
-__device__ __forceinline__ void tile_fetch(const Recs& a, const Geometry& g, int r, bool sorted, TileIn& in) {
+__device__ __forceinline__ void tile_fetch(const Recs& a, const Geometry& g, int r, TileIn& in) {
   in.i = a.ident[r];
   in.lp = a.lpos[r];
   in.ser = a.serno[r];
-  in.okey = sorted ? a.rank[r] : -1;
+  in.okey = a.rank[r];  // meaningful only for a record binned by the last step
   in.bold = kF32 ? a.box[r] : 0;
   load_pos<kF32>(a, g, r, in.bold, in.pos);
 }
 
-// diffuse (smolmolec.c:3115-3168), checkwalls (smolwall.c:368-424),
-// pos2box (smolboxes.c:67-79), the z sub-bin and the unireact coins
-// (smolreact.c:2997-3054) of record r of the source buffer (fields in `in`),
-// as k_diffuse computes them for a simulation without surfaces.  The new
-// position (and, for FP32-offset records, the box it is stored relative to)
-// goes back to the record; the return value is the record's new key, -1 for
-// a record that is not binned (no molecule, or a first-order reactant, whose
-// event is queued here).
+// The diffusion stage of one record in three phases, so that a thread's
+// records wait on their table loads together: tile_draw (the Philox draw of
+// the displacement, smolmolec.c:3115-3168 diffuse, as table indices),
+// tile_table (the Gaussian table loads, or the injected deviates) and
+// tile_finish (the step, checkwalls smolwall.c:368-424, pos2box
+// smolboxes.c:67-79, the z sub-bin and the unireact coins smolreact.c:
+// 2997-3054), as k_diffuse computes them for a simulation without surfaces.
+struct TileDraw {
+  uint32_t gi[3];  // table indices of the three deviates
+  int ll;          // list, -1: no molecule
+  bool moved;      // diffuses this step
+};
+
+__device__ __forceinline__ void tile_draw(const Inst& I, const SpecRow* S, const int* Lim, const TileIn& in,
+                                          uint32_t step, bool inj, int gmask, TileDraw& d) {
+  const int i = in.i;
+  d.ll = (i > 0 && i < kMaxSpecies) ? S[i].ll : -1;
+  d.moved = d.ll >= 0 && in.lp < Lim[d.ll];
+  d.gi[0] = d.gi[1] = d.gi[2] = 0;
+  if (d.moved && !inj) {
+    const smolrng4 rn = smolrng_mol(I.key0, I.key1, in.ser, step, SMOLRNG_DIFFUSE, 0);
+    d.gi[0] = rn.v[0] & (uint32_t)gmask;
+    d.gi[1] = rn.v[1] & (uint32_t)gmask;
+    d.gi[2] = rn.v[2] & (uint32_t)gmask;
+  }
+}
+
+__device__ __forceinline__ void tile_table(const Inst& I, const TileIn& in, const TileDraw& d,
+                                           const double* __restrict__ gtab, const double* __restrict__ inj,
+                                           double* gg) {
+  gg[0] = gg[1] = gg[2] = 0.0;
+  if (!d.moved) return;
+  if (inj) {
+    const long k = (long)I.ctl->pfx[d.ll] + in.lp;
+    gg[0] = inj[3 * k];
+    gg[1] = inj[3 * k + 1];
+    gg[2] = inj[3 * k + 2];
+  } else {
+    gg[0] = gtab[d.gi[0]];
+    gg[1] = gtab[d.gi[1]];
+    gg[2] = gtab[d.gi[2]];
+  }
+}
+
+// The new position (and, for FP32-offset records, the box it is stored
+// relative to) goes back to the record; the return value is the record's new
+// key, -1 for a record that is not binned (no molecule, or a first-order
+// reactant, whose event is queued here).
 template <bool kF32>
-__device__ __forceinline__ int tile_advance(const Inst& I, const SpecRow* S, const int* Lim, const Recs& a, int r,
-                                            const TileIn& in, const double* __restrict__ gtab, int gmask,
-                                            const double* __restrict__ inj, uint32_t step, Glob* gl) {
-  const int i = in.i, lp = in.lp, bold = in.bold;
+__device__ __forceinline__ int tile_finish(const Inst& I, const SpecRow* S, const Recs& a, int r, const TileIn& in,
+                                           const TileDraw& d, const double* gg, uint32_t step, Glob* gl) {
+  const int i = in.i, lp = in.lp, bold = in.bold, ll = d.ll;
   const uint64_t ser = in.ser;
   const Geometry& g = I.g;
-  double pos[3] = {in.pos[0], in.pos[1], in.pos[2]};
-  if (i <= 0 || i >= kMaxSpecies) {
+  if (ll < 0) {
     if (i != 0) atomicOr((int*)&gl->halt, 1);
     return -1;
   }
-  const int ll = S[i].ll;
-  const bool moved = lp < Lim[ll];
-  if (moved) {
+  double pos[3] = {in.pos[0], in.pos[1], in.pos[2]};
+  if (d.moved) {
     const double ds = S[i].ds;
-    double gg[3];
-    if (inj) {
-      const long k = (long)I.ctl->pfx[ll] + lp;
-      gg[0] = inj[3 * k];
-      gg[1] = inj[3 * k + 1];
-      gg[2] = inj[3 * k + 2];
-    } else {
-      const smolrng4 rn = smolrng_mol(I.key0, I.key1, ser, step, SMOLRNG_DIFFUSE, 0);
-      gg[0] = gtab[rn.v[0] & (uint32_t)gmask];
-      gg[1] = gtab[rn.v[1] & (uint32_t)gmask];
-      gg[2] = gtab[rn.v[2] & (uint32_t)gmask];
-    }
     pos[0] += ds * gg[0];
     pos[1] += ds * gg[1];
     pos[2] += ds * gg[2];
@@ -1037,7 +1076,7 @@ __device__ __forceinline__ int tile_advance(const Inst& I, const SpecRow* S, con
   }
   const int b = pos2box(g, pos[0], pos[1], pos[2]);
   // FP32-offset records are stored relative to the new box
-  if (moved || (kF32 && b != bold)) {
+  if (d.moved || (kF32 && b != bold)) {
     store_pos<kF32>(a, g, r, b, pos);
     if (kF32) a.box[r] = b;
   }
@@ -1107,33 +1146,51 @@ __device__ __forceinline__ void wave_run_count(int* cnt, int tile, bool flag) {
   }
 }
 
-// kPer records per thread, a block's records in kPer runs of kBlock: every
-// record's fields are loaded before the first one is advanced.
+// kTile1Per records per thread, a block's records in kTile1Per runs of
+// kBlock.  Every record load of the block goes out before anything waits
+// (before the record count is known: lanes past the end read records that
+// exist but are not used), then the draws of all of a thread's records, their
+// table loads together, and the rest of the stage.
 constexpr int kTile1Per = 2;
 template <bool kF32>
 __global__ __launch_bounds__(kBlock) void k_tile1(const Inst* __restrict__ insts, Glob* __restrict__ gl, Recs a,
                                                   const int* __restrict__ bstart, Tiles tl,
                                                   const double* __restrict__ gtab, int gmask,
                                                   const double* __restrict__ inj) {
-  const int n = live_nrec(gl);
-  const int r0 = blockIdx.x * kBlock * kTile1Per;
-  if (n < 0 || r0 >= n) return;  // uniform over the block
   const Inst& I = insts[0];
   __shared__ SpecRow s_spec[kBlock / 64][kMaxSpecies];
   __shared__ int s_lim[kBlock / 64][kMaxList];
   SpecRow* S = s_spec[threadIdx.x >> 6];
   int* Lim = s_lim[threadIdx.x >> 6];
-  const uint32_t step = I.ctl->step;
-  const int par = step & 1;
-  const int nb0 = bstart[tl.nkeys];  // records binned by the last step; the tail follows
+  const int lane = threadIdx.x & 63;
+  const int r0 = blockIdx.x * kBlock * kTile1Per;
+  const int rlast = tl.reccap - 1;
   TileIn in[kTile1Per];
+  int nbk[kTile1Per];
 #pragma unroll
   for (int j = 0; j < kTile1Per; j++) {
-    const int r = min(r0 + j * kBlock + (int)threadIdx.x, n - 1);  // a lane past the end reads the last record
-    tile_fetch<kF32>(a, I.g, r, r < nb0, in[j]);
+    const int r = min(r0 + j * kBlock + (int)threadIdx.x, rlast);
+    tile_fetch<kF32>(a, I.g, r, in[j]);
+    // the old keys of the records just outside the wave (first and last lane)
+    nbk[j] = -1;
+    if (lane == 0) nbk[j] = a.rank[r > 0 ? r - 1 : 0];
+    if (lane == 63) nbk[j] = a.rank[min(r + 1, rlast)];
   }
+  const int nb0 = bstart[tl.nkeys];  // records binned by the last step; the tail follows
+  const int n = live_nrec(gl);
+  if (n < 0 || r0 >= n) return;  // uniform over the block
   fill_spec_rows(I, S, Lim);
-  const int lane = threadIdx.x & 63;
+  const uint32_t step = I.ctl->step;
+  const int par = step & 1;
+  TileDraw d[kTile1Per];
+#pragma unroll
+  for (int j = 0; j < kTile1Per; j++) {
+    tile_draw(I, S, Lim, in[j], step, inj != nullptr, gmask, d[j]);
+    if (r0 + j * kBlock + (int)threadIdx.x >= n) d[j].ll = -2;  // past the last record
+  }
+  double gg[kTile1Per][3];
+#pragma unroll
+  for (int j = 0; j < kTile1Per; j++) tile_table(I, in[j], d[j], gtab, inj, gg[j]);
   int2* range = tl.range + par * tl.ntile;
 #pragma unroll
   for (int j = 0; j < kTile1Per; j++) {
@@ -1143,13 +1200,13 @@ __global__ __launch_bounds__(kBlock) void k_tile1(const Inst* __restrict__ insts
     // neighbors in the buffer (the next lanes of the wave) belong to another
     // tile, or to none
     int prev = __shfl_up(otile, 1), next = __shfl_down(otile, 1);
-    if (lane == 0) prev = (otile >= 0 && r > 0) ? a.rank[r - 1] >> tl.tshift : -1;
-    if (lane == 63) next = (otile >= 0 && r + 1 < nb0) ? a.rank[r + 1] >> tl.tshift : -1;
+    if (lane == 0) prev = (otile >= 0 && r > 0) ? nbk[j] >> tl.tshift : -1;
+    if (lane == 63) next = (otile >= 0 && r + 1 < nb0) ? nbk[j] >> tl.tshift : -1;
     if (otile >= 0 && prev != otile) range[otile].x = r;
     if (otile >= 0 && next != otile) range[otile].y = r + 1;
     int key = -1;
     if (r < n) {
-      key = tile_advance<kF32>(I, S, Lim, a, r, in[j], gtab, gmask, inj, step, gl);
+      key = tile_finish<kF32>(I, S, a, r, in[j], d[j], gg[j], step, gl);
       tl.nkey[r] = key;
     }
     const bool stay = key >= 0 && (key >> tl.tshift) == otile;
@@ -1226,45 +1283,67 @@ __device__ __forceinline__ void tile_load(const Recs& a, const int* __restrict__
   }
 }
 
-// Place a record at dst of the destination buffer, with its packed FP32
-// screen copy (as k_scatter writes it) and, for the last slots of its list,
-// the inverse map.
+// The fields a record is written with: its box, list and z sub-bin from its
+// key, the position (and the packed FP32 screen copy k_scatter writes).
+struct TileOut {
+  double x, y, z;  // position (FP64 records: the stored value)
+  float4 spk;
+  int bx, ll;
+};
+
 template <bool kF32>
-__device__ __forceinline__ void tile_place(const Inst& I, const TileRec& o, int dst, const Recs& b,
-                                           const Binned& bn, int* __restrict__ inv) {
+__device__ __forceinline__ void tile_out(const Inst& I, const TileRec& o, TileOut& v) {
   const Geometry& g = I.g;
   const int zsub = I.zsub;
   const int kk = o.key - I.keybase;
   const int zs = kk & (zsub - 1);
-  const int bx = (kk / zsub) % g.nbox;
-  const int ll = kk / (zsub * g.nbox);
-  double x, y, z;
+  v.bx = (kk / zsub) % g.nbox;
+  v.ll = kk / (zsub * g.nbox);
+  if (kF32) {
+    double og[3];
+    box_origin(g, v.bx, og);
+    v.x = og[0] + (double)o.f[0];
+    v.y = og[1] + (double)o.f[1];
+    v.z = og[2] + (double)o.f[2];
+  } else {
+    v.x = o.p[0];
+    v.y = o.p[1];
+    v.z = o.p[2];
+  }
+  const double lim = I.screen_lim;
+  const int far = !(fabs(v.x) <= lim && fabs(v.y) <= lim && fabs(v.z) <= lim);
+  v.spk = make_float4((float)v.x, (float)v.y, (float)v.z, __int_as_float(o.ident | (far << 16) | (zs << 17)));
+}
+
+// The inverse map (list slot -> record) of the last slots of a list.
+__device__ __forceinline__ void tile_inv(const Inst& I, const TileRec& o, const TileOut& v, int dst,
+                                         int* __restrict__ inv) {
+  if (o.lp >= I.ctl->nl[v.ll] - I.sortcap) inv[I.L.base[v.ll] + o.lp] = dst;
+}
+
+// Place a record at dst of the destination buffer, with its packed FP32
+// screen copy and, for the last slots of its list, the inverse map.
+template <bool kF32>
+__device__ __forceinline__ void tile_place(const Inst& I, const TileRec& o, int dst, const Recs& b,
+                                           const Binned& bn, int* __restrict__ inv) {
+  TileOut v;
+  tile_out<kF32>(I, o, v);
   if (kF32) {
     b.ox[dst] = o.f[0];
     b.oy[dst] = o.f[1];
     b.oz[dst] = o.f[2];
-    double og[3];
-    box_origin(g, bx, og);
-    x = og[0] + (double)o.f[0];
-    y = og[1] + (double)o.f[1];
-    z = og[2] + (double)o.f[2];
   } else {
-    x = o.p[0];
-    y = o.p[1];
-    z = o.p[2];
-    b.px[dst] = x;
-    b.py[dst] = y;
-    b.pz[dst] = z;
+    b.px[dst] = v.x;
+    b.py[dst] = v.y;
+    b.pz[dst] = v.z;
   }
   b.ident[dst] = o.ident;
   b.serno[dst] = o.ser;
   b.lpos[dst] = o.lp;
-  b.box[dst] = bx;
+  b.box[dst] = v.bx;
   b.rank[dst] = o.key;
-  const double lim = I.screen_lim;
-  const int far = !(fabs(x) <= lim && fabs(y) <= lim && fabs(z) <= lim);
-  bn.spk[dst] = make_float4((float)x, (float)y, (float)z, __int_as_float(o.ident | (far << 16) | (zs << 17)));
-  if (o.lp >= I.ctl->nl[ll] - I.sortcap) inv[I.L.base[ll] + o.lp] = dst;
+  bn.spk[dst] = v.spk;
+  tile_inv(I, o, v, dst, inv);
 }
 
 template <bool kF32>
@@ -1277,16 +1356,20 @@ __global__ __launch_bounds__(kBlock) void k_tile2(const Inst* __restrict__ insts
   __shared__ int s_cnt[kTileMaxKeys];   // records per key, then each key's first destination record
   __shared__ int s_more[kTileMaxKeys];  // records per key among the cached ones, then the next free place
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int par = I.ctl->step & 1;
-  const int* imm = tl.imm + par * tl.ntile;
   const int t = blockIdx.x;
-  const int tk = 1 << tl.tshift, k0 = t << tl.tshift;
+  // the tile's counters of both step parities load with the step counter
+  // (no load waits on another)
+  const int2 rg0 = tl.range[t], rg1 = tl.range[tl.ntile + t];
+  const int in0 = tl.imm[t], in1 = tl.imm[tl.ntile + t];
+  const int sp0 = tl.nspill[0], sp1 = tl.nspill[1];
   const int off = tl.toff[t];
+  const int par = I.ctl->step & 1;
+  const int tk = 1 << tl.tshift, k0 = t << tl.tshift;
   for (int k = tid; k < tk; k += kBlock) s_cnt[k] = 0;
-  const int2 rg = tl.range[par * tl.ntile + t];
-  const int nin = imm[t];
+  const int2 rg = par ? rg1 : rg0;
+  const int nin = par ? in1 : in0;
   const int nbk = min(nin, kTileBucket);
-  const int nsp = nin > kTileBucket ? tl.nspill[par] : 0;
+  const int nsp = nin > kTileBucket ? (par ? sp1 : sp0) : 0;
   const int ns = rg.y - rg.x;
   const int nitems = ns + nbk + nsp;
   __syncthreads();
@@ -3276,18 +3359,18 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
   }
   for (int k = 0; k < n; k++) {
     Inst& I = s->hinst[k];
-    I.tab = s->tab + k;
-    I.srow = s->srow + (size_t)k * kMaxSpecies;
-    I.ctl = s->ctl + k;
-    I.uniev = s->uniev + evoff[k];
-    I.edges = s->edges + edoff[k];
-    I.estate = s->estate + edoff[k];
-    I.accepted = s->accepted + edoff[k];
-    I.log = s->log + sortoff[k];
-    I.sk = s->sortkey + sortoff[k];
-    I.sv = s->sortval + sortoff[k];
-    I.killslot = s->killslot + sortoff[k];
-    I.prods = s->prods + sortoff[k] * kMaxProd;
+    I.tab = AS_VIEW(const DevTables, s->tab + k);
+    I.srow = AS_VIEW(const SpecRow, s->srow + (size_t)k * kMaxSpecies);
+    I.ctl = AS_VIEW(Control, s->ctl + k);
+    I.uniev = AS_VIEW(UniEvent, s->uniev + evoff[k]);
+    I.edges = AS_VIEW(Edge, s->edges + edoff[k]);
+    I.estate = AS_VIEW(int, s->estate + edoff[k]);
+    I.accepted = AS_VIEW(int, s->accepted + edoff[k]);
+    I.log = AS_VIEW(LogRow, s->log + sortoff[k]);
+    I.sk = AS_VIEW(uint64_t, s->sortkey + sortoff[k]);
+    I.sv = AS_VIEW(int, s->sortval + sortoff[k]);
+    I.killslot = AS_VIEW(int, s->killslot + sortoff[k]);
+    I.prods = AS_VIEW(Prod, s->prods + sortoff[k] * kMaxProd);
   }
   if (surf && surf->on) {
     Inst& I = s->hinst[0];
@@ -3308,7 +3391,7 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
       I.sf.blist[i][1] = surf->blist[i][1];
     }
     I.sf.actb = s->d_actb;
-    I.moves = s->moves;
+    I.moves = AS_VIEW(MoveEv, s->moves);
     I.sf.npnl = surf->npnl;
     for (int ll = 0; ll < kMaxList; ll++) I.sf.listdiffuse[ll] = surf->listdiffuse[ll];
     I.sf.epsilon = surf->epsilon;
@@ -3608,6 +3691,7 @@ static void launch_step(DeviceState* s, int src, const double* dinj, int timed, 
     tl.ntile = s->ta.ntile;
     tl.toff = s->ta.toff;
     tl.nkey = s->ta.nkey;
+    tl.reccap = s->reccap;
     tl.nkeys = s->ta.nkeys;
     tl.tshift = s->ta.tshift;
     tl.stay = s->ta.stay;
