@@ -10,6 +10,18 @@
 
 #include <stdint.h>
 
+// Device arrays reached through pointers kept in device memory (a
+// simulation's view, its surface tables) are declared in the global address
+// space for the device compilation: loads and stores through them are global
+// memory instructions, whose waits are separate from LDS waits (through a
+// generic pointer they would be flat instructions, which every LDS wait also
+// waits for).  Host code sees plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SMOLAMD_GLOBAL __attribute__((address_space(1)))
+#else
+#define SMOLAMD_GLOBAL
+#endif
+
 namespace smolamd {
 
 constexpr int kMaxSpecies = 32;
@@ -162,17 +174,17 @@ struct DevSurf {
   int npnl;
   int listdiffuse[kMaxList];  // lists whose molecules are checked after diffusion
   double epsilon;
-  const DevPanel* pnl;
-  const int* bpstart;       // panels of box b: bplist[bpstart[b] .. bpstart[b+1])
-  const int* bplist;
-  const DevSrfAct* act;     // [surface][species][face]
-  const DevSrfAct* actb;    // surface-bound molecules: [surface][species][front/back state], no face
+  SMOLAMD_GLOBAL const DevPanel* pnl;
+  SMOLAMD_GLOBAL const int* bpstart;       // panels of box b: bplist[bpstart[b] .. bpstart[b+1])
+  SMOLAMD_GLOBAL const int* bplist;
+  SMOLAMD_GLOBAL const DevSrfAct* act;     // [surface][species][face]
+  SMOLAMD_GLOBAL const DevSrfAct* actb;    // surface-bound molecules: [surface][species][front/back state], no face
   int listbound[kMaxList];  // lists holding surface-bound states (checksurfacebound)
   int blist[kMaxSpecies][2];  // list of each species in the front / back state (-1: none)
   // no surfaces but absorbing walls: checkwalls with kills instead of the
   // surface checks (the surface variants carry the molsorts a kill needs)
   int walls;
-  int* wq;  // walls: destination records of survivors the step's second wall pass kills
+  SMOLAMD_GLOBAL int* wq;  // walls: destination records of survivors the step's second wall pass kills
 };
 
 // A list change queued by k_diffuse and applied by the first molsort of the

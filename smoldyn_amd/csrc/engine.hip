@@ -122,18 +122,8 @@ struct SpecRow {
 };
 
 // One simulation of the batch, as the kernels see it (device memory).
-// Device arrays reached through a simulation's view are declared in the
-// global address space for the device compilation: loads and stores through
-// them are global memory instructions, whose waits are separate from LDS
-// waits (through a generic pointer they would be flat instructions, which
-// every LDS wait also waits for).
-#if defined(__HIP_DEVICE_COMPILE__)
-#define SMOLAMD_GLOBAL __attribute__((address_space(1)))
-#else
-#define SMOLAMD_GLOBAL
-#endif
+// host pointer -> the view's global pointer type (SMOLAMD_GLOBAL, engine.h)
 #define GAS SMOLAMD_GLOBAL
-// host pointer -> the view's global pointer type
 #define AS_VIEW(T, p) ((GAS T*)(p))
 
 struct Inst {
@@ -3383,22 +3373,22 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
       (void)hipMemcpy(s->d_actb, surf->actb, sizeof(DevSrfAct) * surf->nsrf * kMaxSpecies * 2, hipMemcpyHostToDevice);
     }
     I.sf.walls = surf->walls;
-    I.sf.wq = s->wq;
+    I.sf.wq = AS_VIEW(int, s->wq);
     I.sf.on = 1;
     for (int ll = 0; ll < kMaxList; ll++) I.sf.listbound[ll] = surf->listbound[ll];
     for (int i = 0; i < kMaxSpecies; i++) {
       I.sf.blist[i][0] = surf->blist[i][0];
       I.sf.blist[i][1] = surf->blist[i][1];
     }
-    I.sf.actb = s->d_actb;
+    I.sf.actb = AS_VIEW(const DevSrfAct, s->d_actb);
     I.moves = AS_VIEW(MoveEv, s->moves);
     I.sf.npnl = surf->npnl;
     for (int ll = 0; ll < kMaxList; ll++) I.sf.listdiffuse[ll] = surf->listdiffuse[ll];
     I.sf.epsilon = surf->epsilon;
-    I.sf.pnl = s->d_pnl;
-    I.sf.bpstart = s->d_bpstart;
-    I.sf.bplist = s->d_bplist;
-    I.sf.act = s->d_act;
+    I.sf.pnl = AS_VIEW(const DevPanel, s->d_pnl);
+    I.sf.bpstart = AS_VIEW(const int, s->d_bpstart);
+    I.sf.bplist = AS_VIEW(const int, s->d_bplist);
+    I.sf.act = AS_VIEW(const DevSrfAct, s->d_act);
     s->surf_on = true;
     s->surf_curved = false;
     for (int k = 0; k < surf->npnl; k++)
