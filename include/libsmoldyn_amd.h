@@ -245,7 +245,11 @@ enum ErrorCode smolamdRunSteps(simptr sim, long nsteps, int use_graph, double* m
 enum ErrorCode smolamdProfileSteps(simptr sim, long nsteps, double* out);
 /* Engine diagnostics: out[0]=live molecules, [1]=edges max, [2]=matching rounds
    of last step, [3]=error bits, [4]=next serial, [5]=step index,
-   [6]=molecule-steps run on the device so far. */
+   [6]=molecule-steps run on the device so far, [7]=engine rebuilds for lists
+   that filled up (smolRunSim and smolRunTimeStep grow a list that is more
+   than half full to twice its capacity, as molexpandlist does,
+   smolmolec.c:1911-1950; not for models with surfaces or absorbing walls, nor
+   under smolSetMaxMolecules). */
 enum ErrorCode smolamdGetDiagnostics(simptr sim, double* out);
 
 /* Ensembles: no counterpart in the reference, which runs replicates as

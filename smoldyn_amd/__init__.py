@@ -450,7 +450,7 @@ class Simulation:
         out = np.zeros(8)
         _check(self.L.smolamdGetDiagnostics(self.p, _dp(out)), "smolamdGetDiagnostics")
         return dict(nmol=int(out[0]), maxedges=int(out[1]), rounds=int(out[2]), err=int(out[3]),
-                    serno_next=int(out[4]), step=int(out[5]), molsteps=int(out[6]))
+                    serno_next=int(out[4]), step=int(out[5]), molsteps=int(out[6]), rebuilds=int(out[7]))
 
 
 class Ensemble:

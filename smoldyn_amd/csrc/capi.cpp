@@ -34,6 +34,7 @@ namespace {
 constexpr int kMS = 5;  // MSMAX
 constexpr int kMS1 = 6; // MSMAX1
 constexpr int kStrLen = 512;
+constexpr int kGrowCheckSteps = 64;  // smolRunSim checks the lists' fill every this many steps
 
 // last-error state, as the reference keeps it in process globals
 // (libsmoldyn.cpp:33-38)
@@ -83,6 +84,15 @@ struct smolamd_sim {
   uint64_t key = 0;
   double mpbox = -1, boxsize = -1, accur = 10;
   int maxmol = -1;
+  // list capacity of the current engine, and the capacity a rebuild asks for
+  // (lists that fill up double, as molexpandlist does)
+  long devcap = 0, growcap = 0;
+  int rebuilds = 0;                       // engine rebuilds for full lists
+  // the reaction events of the step before a rebuild (the old engine's log),
+  // served by smolamdGetReactionLog until the next step
+  std::vector<LogRow> heldlog;
+  bool holdlog = false;
+  unsigned long long molsteps_base = 0;   // molecule-steps of engines replaced by a rebuild
   int precision = 64;  // record precision: 64 (FP64 positions) or 32 (FP32 box offsets)
   std::vector<std::string> spname{"empty"};
   std::vector<std::array<double, kMS>> difc{{0, 0, 0, 0, 0}};
@@ -552,9 +562,11 @@ void gauss_table(std::vector<double>& a, int n) {
 // Setup of simupdate (smolsim.cpp:2367-2473) for the device path.  With
 // for_batch the device inputs are kept in sim->prep and no engine is
 // created (the simulation becomes one member of an ensemble).
-ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
-  if (sim->ok && !for_batch) return ECok;
-  LCHECK(!sim->started, fn, ECerror, "the simulation definition changed after stepping started");
+// rebuild: a new engine for a running simulation (grow_lists): the box grid
+// is kept, the molecules are the current ones in sim->pending
+ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false, bool rebuild = false) {
+  if (sim->ok && !for_batch && !rebuild) return ECok;
+  LCHECK(!sim->started || rebuild, fn, ECerror, "the simulation definition changed after stepping started");
   LCHECK(sim->dim == 3, fn, ECerror, "the device engine runs 3-D systems");
   int nsp = (int)sim->spname.size();
   LCHECK(nsp <= kMaxSpecies, fn, ECbounds, "too many species for the device tables (%d)", kMaxSpecies);
@@ -606,6 +618,7 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
 
   // boxesupdatelists (smolboxes.c:717-819)
   Geometry& g = sim->geo;
+  const Geometry kept = g;  // a rebuild keeps the virtual boxes of the first update
   memset(&g, 0, sizeof(g));
   double mpbox = sim->mpbox;
   if (mpbox < 0 && sim->boxsize < 0) mpbox = 4;
@@ -627,6 +640,7 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
     g.bmin[d] = sim->low[d];
     g.side[d] = (int)ceil((sim->high[d] - sim->low[d]) * flt1);
     if (!g.side[d]) g.side[d] = 1;
+    if (rebuild) g.side[d] = kept.side[d];
     g.bsize[d] = (sim->high[d] - sim->low[d]) / g.side[d];
     nbox *= g.side[d];
   }
@@ -838,6 +852,8 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false) {
       for (int ms = MSfront; ms <= MSback; ms++)
         if (lookup[i][ms] >= 0) reachable[lookup[i][ms]] = 1;
   long capl = sim->maxmol > 0 ? sim->maxmol : ntot + (ntot / 4 > 4096 ? ntot / 4 : 4096);
+  if (sim->growcap > capl) capl = sim->growcap;
+  sim->devcap = capl;
   long captot = 0;
   for (int l = 0; l < nlist; l++) {
     cfg.cap[l] = reachable[l] ? (int)capl : 256;
@@ -1002,10 +1018,64 @@ void write_rxnlog(simptr sim, double time) {
   }
 }
 
+// Lists that fill up grow (molexpandlist, smolmolec.c:1911-1950, doubles a
+// full list).  The device's lists have a fixed capacity, so when a list is
+// more than half full after a step the engine is rebuilt from its own state
+// with twice the capacity: the same virtual boxes, the molecules in their
+// live order with their serial numbers, the serial counter and the step
+// index.  Every draw is keyed by serial number and step, so the steps that
+// follow are the ones the old engine would have taken.  Not for models with
+// surfaces or absorbing walls (their molecules' surface states are not
+// carried) nor under a maximum set by smolSetMaxMolecules.
+ErrorCode grow_lists(simptr sim, const char* fn) {
+  if (!sim->dev || sim->maxmol > 0 || sim->surf.enabled()) return ECok;
+  for (int w = 0; w < 6; w++)
+    if (sim->wtype[w] == 'a') return ECok;
+  Control c;
+  LCHECK(engine_sync_control(sim->dev, &c) == 0, fn, ECerror, "HIP runtime error");
+  const int nlist = (int)sim->listname.size();
+  bool full = false;
+  for (int l = 0; l < nlist; l++) full |= 2L * c.nl[l] > sim->devcap;
+  if (!full) return ECok;
+  // the live lists, each pushed last molecule first: the update reads the
+  // pending molecules from the end (the order molsort gives newest first)
+  std::vector<PendingMol> pend;
+  for (int l = 0; l < nlist; l++) {
+    const int n = c.nl[l];
+    std::vector<int> id((size_t)n), box((size_t)n);
+    std::vector<double> pos((size_t)n * 3);
+    std::vector<uint64_t> ser((size_t)n);
+    LCHECK(engine_get_list(sim->dev, l, id.data(), pos.data(), ser.data(), box.data(), n) == n, fn, ECerror,
+           "HIP runtime error");
+    for (int k = n - 1; k >= 0; k--) {
+      PendingMol pm;
+      pm.ident = id[(size_t)k];
+      pm.pos[0] = pos[3 * (size_t)k];
+      pm.pos[1] = pos[3 * (size_t)k + 1];
+      pm.pos[2] = pos[3 * (size_t)k + 2];
+      pm.serno = ser[(size_t)k];
+      pend.push_back(pm);
+    }
+  }
+  sim->molsteps_base += c.molsteps;
+  const int nlog = engine_get_log(sim->dev, nullptr, 0);
+  sim->heldlog.resize(nlog > 0 ? (size_t)nlog : 0);
+  if (nlog > 0) engine_get_log(sim->dev, sim->heldlog.data(), nlog);
+  sim->holdlog = true;
+  sim->pending.swap(pend);
+  sim->serno = c.serno_next;
+  sim->growcap = 2 * sim->devcap;
+  sim->ok = false;
+  const ErrorCode ec = do_update(sim, fn, false, true);
+  if (ec == ECok) sim->rebuilds++;
+  return ec;
+}
+
 int run_one(simptr sim, const double* inject, long ninject, const char* fn, ErrorCode* ec) {
   *ec = do_update(sim, fn);
   if (*ec != ECok) return -1;
   sim->started = true;
+  sim->holdlog = false;
   int bits = engine_step(sim->dev, inject, ninject);
   if (bits) {
     *ec = engine_error(fn, bits);
@@ -1014,6 +1084,8 @@ int run_one(simptr sim, const double* inject, long ninject, const char* fn, Erro
   if (rxnlog_on(sim)) write_rxnlog(sim, sim->time);
   sim->time += sim->dt;
   sim->stepindex++;
+  *ec = grow_lists(sim, fn);
+  if (*ec != ECok) return -1;
   if (sim->time >= sim->tmax) return 1;
   if (sim->time >= sim->tbreak) return 10;
   return 0;
@@ -1105,7 +1177,8 @@ int simulate_device(simptr sim, const char* fn, ErrorCode* ec) {
   const double t0 = sim->time;
   const uint32_t s0 = sim->stepindex;
   int bits = 0;
-  long since = 0;
+  long since = 0, sincegrow = 0;
+  ErrorCode growerr = ECok;
   auto observe = [&](uint32_t* step) -> int {
     if (sim->obsrows >= kObsRows) {
       int b = flush_outputs(sim);
@@ -1130,6 +1203,7 @@ int simulate_device(simptr sim, const char* fn, ErrorCode* ec) {
   double tstep = sim->time;  // start time of the next step, accumulated as the clock does
   auto step = [&]() -> int {
     if (bits) return -1;
+    sim->holdlog = false;
     if (engine_enqueue(sim->dev, 1)) {
       bits = -1;
       return -1;
@@ -1152,12 +1226,27 @@ int simulate_device(simptr sim, const char* fn, ErrorCode* ec) {
         return -1;
       }
     }
+    if (++sincegrow >= kGrowCheckSteps) {  // full lists grow between steps
+      sincegrow = 0;
+      int b = flush_outputs(sim);
+      if (b) {
+        bits = b;
+        return -1;
+      }
+      growerr = grow_lists(sim, fn);
+      if (growerr != ECok) return -1;
+    }
     return 0;
   };
   int code = cmd::simulate(sim->cmds, k, step, ex);
   int b = flush_outputs(sim);
   if (b && !bits) bits = b;
   sim->time = k.time;
+  if (growerr == ECok && !bits) growerr = grow_lists(sim, fn);  // the device is idle after the flush
+  if (growerr != ECok) {
+    *ec = growerr;
+    return -1;
+  }
   if (bits) {
     // the device stops at the failing step: the clock follows its step counter
     Control c;
@@ -2352,7 +2441,13 @@ int smolamdGetMolecules(simptr sim, int list, int* species, double* pos, unsigne
 int smolamdGetReactionLog(simptr sim, long long* rows, int maxrows) {
   if (!sim || !sim->dev) return 0;
   std::vector<LogRow> lr(maxrows > 0 ? maxrows : 1);
-  int n = engine_get_log(sim->dev, lr.data(), maxrows);
+  int n;
+  if (sim->holdlog) {  // the last step ran on an engine since rebuilt
+    n = (int)sim->heldlog.size();
+    for (int k = 0; k < n && k < maxrows; k++) lr[(size_t)k] = sim->heldlog[(size_t)k];
+  } else {
+    n = engine_get_log(sim->dev, lr.data(), maxrows);
+  }
   for (int k = 0; k < n && k < maxrows; k++) {
     rows[6 * k + 0] = lr[k].order;
     rows[6 * k + 1] = lr[k].r;
@@ -2433,7 +2528,8 @@ enum ErrorCode smolamdGetDiagnostics(simptr sim, double* out) {
   out[3] = c.err;
   out[4] = (double)c.serno_next;
   out[5] = c.step;
-  out[6] = (double)c.molsteps;
+  out[6] = (double)(sim->molsteps_base + c.molsteps);
+  out[7] = sim->rebuilds;
   return ECok;
 }
 

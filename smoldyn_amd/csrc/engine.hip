@@ -3488,21 +3488,26 @@ int engine_upload_list_inst(DeviceState* s, int inst, int ll, int n, const int* 
       lp[k] = k;
       inv[k] = r0 + k;
     }
+    // each molecule's virtual box (pos2box, smolboxes.c:67-79), as a read
+    // of the state before the first step reports it
+    const Geometry& g = I.g;
+    for (int k = 0; k < n; k++) {
+      int b = 0;
+      for (int d = 0; d < 3; d++) {
+        int indx = (int)((pos[3 * k + d] - g.bmin[d]) / g.bsize[d]);
+        if (indx < 0) indx = 0;
+        else if (indx >= g.side[d]) indx = g.side[d] - 1;
+        b = g.side[d] * b + indx;
+      }
+      box[k] = b;
+    }
     if (s->f32) {
       // FP32 offsets from the origin of each molecule's virtual box
       std::vector<float> fx(n), fy(n), fz(n);
-      const Geometry& g = I.g;
       for (int k = 0; k < n; k++) {
-        int b = 0;
-        for (int d = 0; d < 3; d++) {
-          int indx = (int)((pos[3 * k + d] - g.bmin[d]) / g.bsize[d]);
-          if (indx < 0) indx = 0;
-          else if (indx >= g.side[d]) indx = g.side[d] - 1;
-          b = g.side[d] * b + indx;
-        }
+        const int b = box[k];
         double o[3];
         box_origin(g, b, o);
-        box[k] = b;
         fx[k] = (float)(x[k] - o[0]);
         fy[k] = (float)(y[k] - o[1]);
         fz[k] = (float)(z[k] - o[2]);
