@@ -149,3 +149,41 @@ void omath_dircos2(double* c, double theta) {
   c[1] = sin(theta);
   c[2] = -c[1];
 }
+
+/* Geo_SphereReflectSphere (source/libSteve/Geometry.c:1264-1293): two
+   molecules move on straight lines a0->a1 and b0->b1; p is the fraction of
+   the step at which their separation first equals the binding radius
+   (radius2 = radius squared, the smaller root of the quadratic in p).  At
+   that moment each molecule's remaining motion is reflected off the plane
+   normal to their separation (deltap); the reflected end points go to a1p and
+   b1p.  Returns p (the caller uses the result only for 0 < p < 1). */
+double omath_sphere_reflect_sphere(const double* a0, const double* a1, const double* b0, const double* b1, int dim,
+                                   double radius2, double* a1p, double* b1p) {
+  double qa = 0, qb = 0, qc = 0, p, f1, f2, k, deltap[3];
+  int d;
+  for (d = 0; d < dim; d++) {
+    const double rel = b1[d] - a1[d] - b0[d] + a0[d]; /* change of the separation over the step */
+    const double sep0 = b0[d] - a0[d];                 /* separation at the start */
+    qa += rel * rel;
+    qb += 2 * sep0 * rel;
+    qc += sep0 * sep0;
+  }
+  qc -= radius2;
+  p = (-qb - sqrt(qb * qb - 4 * qa * qc)) / (2 * qa);
+  f1 = f2 = 0;
+  for (d = 0; d < dim; d++) {
+    deltap[d] = (1 - p) * (b0[d] - a0[d]) + p * (b1[d] - a1[d]);
+    f1 += (a1[d] - a0[d]) * deltap[d];
+    f2 += (b1[d] - b0[d]) * deltap[d];
+  }
+  k = 2 * (1 - p) / radius2;
+  if (f1 >= 0)
+    for (d = 0; d < dim; d++) a1p[d] = a1[d] - k * f1 * deltap[d];
+  else
+    for (d = 0; d < dim; d++) a1p[d] = 2 * (1 - p) * a0[d] + (2 * p - 1) * a1[d] + k * f1 * deltap[d];
+  if (f2 <= 0)
+    for (d = 0; d < dim; d++) b1p[d] = b1[d] - k * f2 * deltap[d];
+  else
+    for (d = 0; d < dim; d++) b1p[d] = 2 * (1 - p) * b0[d] + (2 * p - 1) * b1[d] + k * f2 * deltap[d];
+  return p;
+}

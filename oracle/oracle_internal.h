@@ -22,6 +22,8 @@ double omath_inversefn(double (*fn)(double), double y, double x1, double x2, int
 void omath_randtable(double* a, int n);
 void omath_dircos(double* c, double theta, double phi, double chi);
 void omath_dircos2(double* c, double theta);
+double omath_sphere_reflect_sphere(const double* a0, const double* a1, const double* b0, const double* b1, int dim,
+                                   double radius2, double* a1p, double* b1p);
 
 /* orxnparam.c: Andrews-Bray reaction parameter solver (source/libSteve/rxnparam.c). */
 double orxn_modelrxnrate(double a, double b, double difc, double chi);

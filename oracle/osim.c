@@ -1000,11 +1000,10 @@ static int o_rxnsetproduct(osim* sim, int order, int r) {
       return 2;
   } else if (rparamt == ORP_BOUNCE) {
     /* nprod >= 2, bounce (smolreact.c:1470-1494): offsets from the reactants'
-       relative diffusion coefficients; the reflection method (rparam -2)
-       needs the pre-diffusion positions and is not restated */
+       relative diffusion coefficients (the reflection method, rparam -2,
+       takes the rparam < 0 offsets for its fallback) */
     double dc1, dc2, dsum;
     int ms1 = rxn->rctstate[0], ms2 = rxn->rctstate[1];
-    if (rpar == -2) return 11;
     if (ms1 == OMS_BSOLN || ms1 == OMS_NONE || ms1 == OMS_ALL || ms1 == OMS_SOME) ms1 = OMS_SOLN;
     if (ms2 == OMS_BSOLN || ms2 == OMS_NONE || ms2 == OMS_ALL || ms2 == OMS_SOME) ms2 = OMS_SOLN;
     dc1 = o_difcsum(sim, rxn->rct[0], ms1, 0, OMS_NONE);
@@ -1277,6 +1276,7 @@ static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, i
   int dim = sim->dim, d, prd, calc = 0, rxnbox, offaxis = 0;
   double rxnpos[O_DIMMAX], m3[9], v1[O_DIMMAX], u[O_DIMMAX];
   double bv[O_DIMMAX] = {0}, bdist = 0, bx = 0; /* bounce: reactant separation vector, length, scale */
+  double pvalue = -1, rv1[3] = {0, 0, 0}, rv2[3] = {0, 0, 0}; /* bounce by reflection: contact fraction, ends */
   omol* prods[O_MAXPRD];
   if (order == 0) {
     for (d = 0; d < dim; d++) rxnpos[d] = pos0[d];
@@ -1301,24 +1301,35 @@ static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, i
     mp->box = rxnbox;
     mp->pnl = mp->pnlx = -1;
     if (rxn->rparamt == ORP_BOUNCE && order == 2) {
-      /* bounce (smolreact.c:2753-2805), scaling methods: the products keep
-         the reactants' serial numbers and move apart along the line joining
+      /* bounce (smolreact.c:2753-2805): the products keep the reactants'
+         serial numbers; by reflection (rparam -2) their paths are reflected
+         at the moment of contact, else they move apart along the line joining
          them (rparam >= 0: to distance rparam; otherwise reflected off the
          binding radius) */
       if (prd == 0) {
-        bdist = 0;
-        for (d = 0; d < dim; d++) {
-          bv[d] = m2->pos[d] - m1->pos[d];
-          bdist += bv[d] * bv[d];
+        pvalue = -1;
+        if (rxn->rparam == -2) { /* reflection method: the pair's motion reflected at contact */
+          pvalue = omath_sphere_reflect_sphere(m1->posx, m1->pos, m2->posx, m2->pos, dim, rxn->bindrad2, rv1, rv2);
+          if (pvalue > 0 && pvalue < 1)
+            for (d = 0; d < dim; d++) mp->pos[d] = rv1[d];
+          else
+            pvalue = -1; /* failed: the other methods */
         }
-        if (bdist == 0) {
-          bdist = sqrt(rxn->bindrad2);
-          bv[0] = bdist;
-        } else
-          bdist = sqrt(bdist);
-        if (rxn->rparam >= 0) bx = 1.0 / bdist - (1.0 / rxn->rparam);
-        else bx = 2.0 / bdist - 2.0 / (rxn->prdpos[0][0] + rxn->prdpos[1][0]);
-        for (d = 0; d < dim; d++) mp->pos[d] = m1->pos[d] - bx * rxn->prdpos[0][0] * bv[d];
+        if (pvalue == -1) {
+          bdist = 0;
+          for (d = 0; d < dim; d++) {
+            bv[d] = m2->pos[d] - m1->pos[d];
+            bdist += bv[d] * bv[d];
+          }
+          if (bdist == 0) {
+            bdist = sqrt(rxn->bindrad2);
+            bv[0] = bdist;
+          } else
+            bdist = sqrt(bdist);
+          if (rxn->rparam >= 0) bx = 1.0 / bdist - (1.0 / rxn->rparam);
+          else bx = 2.0 / bdist - 2.0 / (rxn->prdpos[0][0] + rxn->prdpos[1][0]);
+          for (d = 0; d < dim; d++) mp->pos[d] = m1->pos[d] - bx * rxn->prdpos[0][0] * bv[d];
+        }
         mp->box = m1->box;
         mp->serno = m1->serno;
         for (d = 0; d < dim; d++) {
@@ -1328,7 +1339,10 @@ static int o_doreact(osim* sim, int order, int r, omol* m1, omol* m2, int ll1, i
       } else if (prd == 1) {
         mp->box = m2->box;
         mp->serno = m2->serno;
-        for (d = 0; d < dim; d++) mp->pos[d] = m2->pos[d] + bx * rxn->prdpos[1][0] * bv[d];
+        if (pvalue != -1)
+          for (d = 0; d < dim; d++) mp->pos[d] = rv2[d];
+        else
+          for (d = 0; d < dim; d++) mp->pos[d] = m2->pos[d] + bx * rxn->prdpos[1][0] * bv[d];
         for (d = 0; d < dim; d++) {
           mp->posx[d] = m2->pos[d];
           mp->posoffset[d] = m2->posoffset[d];

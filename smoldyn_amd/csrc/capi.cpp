@@ -407,8 +407,7 @@ int set_product(simptr sim, int order, int r) {
       return 2;
   } else if (rparamt == RPbounce) {
     // nprod >= 2, bounce (smolreact.c:1470-1494); the reflection method
-    // (rparam -2) needs the pre-diffusion positions, which the device does not keep
-    if (rpar == -2) return 11;
+    // (rparam -2) keeps the rparam < 0 offsets for its fallback
     int ms1 = rx.rctstate[0], ms2 = rx.rctstate[1];
     if (ms1 == MSbsoln || ms1 == MSnone || ms1 == MSall || ms1 == MSsome) ms1 = MSsoln;
     if (ms2 == MSbsoln || ms2 == MSnone || ms2 == MSall || ms2 == MSsome) ms2 = MSsoln;
@@ -838,6 +837,13 @@ ErrorCode do_update(simptr sim, const char* fn, bool for_batch = false, bool reb
     const char* e = getenv("SMOLAMD_ZSUB_MAX");
     cfg.zsub_max = e ? atoi(e) : 0;
   }
+  // bounce by reflection (rparam -2, doreact smolreact.c:2756-2762) reads the
+  // reactants' positions before the step's diffusion: the engine keeps them
+  for (auto& rx : sim->rxn[2])
+    if (rx.rparamt == RPbounce && rx.rparam == -2) cfg.reflect = 1;
+  LCHECK(!(cfg.reflect && (sim->surf.enabled() || wall_absorb || for_batch)), fn, ECerror,
+         "the reflection method of bounce placement is implemented on the device path for solution models "
+         "without surfaces, absorbing walls or ensembles");
   cfg.nlist = nlist;
   // a list can only hold molecules placed at setup or created by reactions;
   // lists that receive neither keep a token capacity

@@ -241,6 +241,10 @@ struct EngineConfig {
   int f32;
   // largest z sub-bin count per virtual box for the pair search (0: automatic)
   int zsub_max;
+  // a bimolecular reaction places its products by bounce reflection: keep
+  // each record's position before the step's diffusion (one simulation
+  // without surfaces)
+  int reflect;
 };
 
 // Host description of the surfaces handed to engine_create (arrays copied).

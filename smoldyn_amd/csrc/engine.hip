@@ -155,6 +155,11 @@ struct Inst {
   GAS MoveEv* moves;  // list changes of the step (surfaces), capacity maxevents
   GAS int* killslot;
   GAS Prod* prods;
+  // bounce by reflection: each source record's position before the step's
+  // diffusion (3 per record) and, per destination record, its source record
+  // (null when the model has no such reaction)
+  GAS double* qpos;
+  GAS int* src;
 };
 
 // Batch-wide counters.  The record count of the current buffer is kept per
@@ -233,6 +238,8 @@ struct DeviceState {
     int *stay = nullptr, *imm = nullptr, *toff = nullptr, *nkey = nullptr, *bucket = nullptr, *nspill = nullptr;
     int2 *range = nullptr, *spill = nullptr;
   } ta;
+  double* qpos = nullptr;            // bounce by reflection: pre-diffusion positions, source-record order
+  int* srcmap = nullptr;             // bounce by reflection: destination record -> source record
   int4* cand = nullptr;              // screened candidate pairs (k_pairs -> k_exact)
   int candcap = 0;
   bool half1 = false;                // every simulation reacts within one list only (half stencil)
@@ -475,6 +482,11 @@ __global__ __launch_bounds__(kBlock) void k_diffuse(const Inst* __restrict__ ins
       const int bold = kF32 ? box0 : 0;
       const int bprev = kSurf ? box0 : -1;  // the box the record was filed in last step
       double pos[3] = {pos0[0], pos0[1], pos0[2]};
+      if (!kSurf && !kMulti && I.qpos) {  // bounce by reflection: the position before diffusion (posx)
+        I.qpos[3 * (long)r] = pos[0];
+        I.qpos[3 * (long)r + 1] = pos[1];
+        I.qpos[3 * (long)r + 2] = pos[2];
+      }
       // molecules added since the last step sit at the end of their list and
       // do not diffuse in this step
       const bool moved =
@@ -888,8 +900,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Inst* __restrict__ ins
   const int key = I.keybase + (ll * I.g.nbox + bx) * I.zsub + zs;
   int dst = bn.bstart[key] + (rk & kRankMask);
   // the key a record was binned with, read by the tile binning of the next
-  // step (one simulation without surfaces)
-  if (!kMulti && !kSurf) b.rank[dst] = key;
+  // step (one simulation without surfaces), and for bounce by reflection the
+  // record's source record
+  if (!kMulti && !kSurf) {
+    b.rank[dst] = key;
+    if (I.src) I.src[dst] = r;
+  }
   double x, y, z;
   if (kF32) {
     const float fx = a.ox[r], fy = a.oy[r], fz = a.oz[r];
@@ -1057,6 +1073,11 @@ __device__ __forceinline__ int tile_finish(const Inst& I, const SpecRow* S, cons
     return -1;
   }
   double pos[3] = {in.pos[0], in.pos[1], in.pos[2]};
+  if (I.qpos) {  // the position before the step's diffusion (posx)
+    I.qpos[3 * (long)r] = pos[0];
+    I.qpos[3 * (long)r + 1] = pos[1];
+    I.qpos[3 * (long)r + 2] = pos[2];
+  }
   if (d.moved) {
     const double ds = S[i].ds;
     pos[0] += ds * gg[0];
@@ -1254,11 +1275,13 @@ struct TileRec {
   float f[3];   // FP32 offsets from the origin of the record's new box
   uint64_t ser;
   int key, ident, lp;
+  int rec;  // source record
 };
 
 template <bool kF32>
 __device__ __forceinline__ void tile_load(const Recs& a, const int* __restrict__ nkey, int r, TileRec& o) {
   o.key = nkey[r];
+  o.rec = r;
   o.ident = a.ident[r];
   o.lp = a.lpos[r];
   o.ser = a.serno[r];
@@ -1333,6 +1356,7 @@ __device__ __forceinline__ void tile_place(const Inst& I, const TileRec& o, int 
   b.box[dst] = v.bx;
   b.rank[dst] = o.key;
   bn.spk[dst] = v.spk;
+  if (I.src) I.src[dst] = o.rec;
   tile_inv(I, o, v, dst, inv);
 }
 
@@ -1776,6 +1800,41 @@ __global__ __launch_bounds__(kBlock) void k_exact(const Inst* __restrict__ insts
 // position after the step's first molsort (the record of the source buffer).
 __device__ __forceinline__ int uni_slot(const Inst& I, const Recs& RA, int rec) {
   return I.L.base[I.tab->listof[RA.ident[rec]]] + RA.lpos[rec];
+}
+
+// Geo_SphereReflectSphere (source/libSteve/Geometry.c:1264-1293) in three
+// dimensions: two molecules move on straight lines a0->a1 and b0->b1; p is
+// the fraction of the step at which their separation first equals the
+// binding radius (radius2 = radius squared); each molecule's remaining motion
+// is reflected off the plane normal to their separation at that moment, the
+// reflected end points in a1p and b1p.  Returns p.
+__device__ __forceinline__ double sphere_reflect_sphere(const double* a0, const double* a1, const double* b0,
+                                                        const double* b1, double radius2, double* a1p, double* b1p) {
+  double qa = 0, qb = 0, qc = 0, deltap[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double rel = b1[d] - a1[d] - b0[d] + a0[d];  // change of the separation over the step
+    const double sep0 = b0[d] - a0[d];                  // separation at the start
+    qa += rel * rel;
+    qb += 2 * sep0 * rel;
+    qc += sep0 * sep0;
+  }
+  qc -= radius2;
+  const double p = (-qb - sqrt(qb * qb - 4 * qa * qc)) / (2 * qa);
+  double f1 = 0, f2 = 0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    deltap[d] = (1 - p) * (b0[d] - a0[d]) + p * (b1[d] - a1[d]);
+    f1 += (a1[d] - a0[d]) * deltap[d];
+    f2 += (b1[d] - b0[d]) * deltap[d];
+  }
+  const double k = 2 * (1 - p) / radius2;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    a1p[d] = f1 >= 0 ? a1[d] - k * f1 * deltap[d] : 2 * (1 - p) * a0[d] + (2 * p - 1) * a1[d] + k * f1 * deltap[d];
+    b1p[d] = f2 <= 0 ? b1[d] - k * f2 * deltap[d] : 2 * (1 - p) * b0[d] + (2 * p - 1) * b1[d] + k * f2 * deltap[d];
+  }
+  return p;
 }
 
 // ---------------------------------------------------------------------------
@@ -2731,6 +2790,18 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       load_pos<kF32>(B, I.g, ed.s1, kF32 ? B.box[ed.s1] : 0, p1);
       load_pos<kF32>(B, I.g, ed.s2, kF32 ? B.box[ed.s2] : 0, p2);
       int id1 = B.ident[ed.s1], id2 = B.ident[ed.s2];
+      // bounce by reflection: the reactants' positions before this step's
+      // diffusion (posx), by their source records
+      const bool refl = rx->bounce && rx->bparam == -2.0 && I.qpos;
+      double x1[3] = {0, 0, 0}, x2[3] = {0, 0, 0};
+      if (refl) {
+        const long q1 = I.src[ed.s1], q2 = I.src[ed.s2];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          x1[d] = I.qpos[3 * q1 + d];
+          x2[d] = I.qpos[3 * q2 + d];
+        }
+      }
       if (ed.wrap && !rx->confspread) {  // morebireact: move the faster molecule next to its partner
         double v[3];
 #pragma unroll
@@ -2745,12 +2816,14 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
           for (int d = 0; d < 3; d++) {
             double shift = p1[d] + v[d] - p2[d];
             p2[d] += shift;
+            x2[d] += shift;  // posx moves with the molecule
           }
         } else {
 #pragma unroll
           for (int d = 0; d < 3; d++) {
             double shift = p2[d] - v[d] - p1[d];
             p1[d] += shift;
+            x1[d] += shift;
           }
         }
       }
@@ -2768,7 +2841,20 @@ __global__ __launch_bounds__(kResolveThreads) void k_resolve(const Inst* __restr
       }
       sA = B.serno[swap ? ed.s2 : ed.s1];
       sB = B.serno[swap ? ed.s1 : ed.s2];
-      if (rx->bounce) {
+      bool reflected = false;
+      if (refl) {  // doreact's reflection method (smolreact.c:2756-2762)
+        double v1[3], v2[3];
+        const double pv = sphere_reflect_sphere(swap ? x2 : x1, pAv, swap ? x1 : x2, pBv, rx->bindrad2, v1, v2);
+        if (pv > 0 && pv < 1) {
+          reflected = true;
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            bq[0][d] = v1[d];
+            bq[1][d] = v2[d];
+          }
+        }
+      }
+      if (rx->bounce && !reflected) {
         // doreact's bounce scaling methods (smolreact.c:2766-2779): move apart along
         // the line joining the reactants
         double bv[3], dist = 0;
@@ -3342,6 +3428,7 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
       s->tile_ok = true;
     }
   }
+  if (!s->multi && cfg[0].reflect) ok |= dalloc(&s->qpos, 3 * rc) | dalloc(&s->srcmap, rc);
   if (ok) {
     snprintf(err, errlen, "device allocation failed (%d molecule records)", s->reccap);
     engine_destroy(s);
@@ -3361,6 +3448,8 @@ DeviceState* engine_create_batch(int n, const EngineConfig* cfg, const DevTables
     I.sv = AS_VIEW(int, s->sortval + sortoff[k]);
     I.killslot = AS_VIEW(int, s->killslot + sortoff[k]);
     I.prods = AS_VIEW(Prod, s->prods + sortoff[k] * kMaxProd);
+    I.qpos = AS_VIEW(double, s->qpos);
+    I.src = AS_VIEW(int, s->srcmap);
   }
   if (surf && surf->on) {
     Inst& I = s->hinst[0];
@@ -3458,7 +3547,7 @@ void engine_destroy(DeviceState* s) {
                   s->ctl,     s->tab,     s->gtab,    s->inj,     s->prods,     s->d_pnl,   s->d_bpstart,
                   s->d_bplist, s->d_act,  s->d_actb,  s->moves,   s->dinst,   s->glob,    s->obs,
                   s->wq,      s->srow,    s->ta.stay, s->ta.range, s->ta.imm, s->ta.bucket, s->ta.spill,
-                  s->ta.nspill, s->ta.toff,   s->ta.nkey};
+                  s->ta.nspill, s->ta.toff,   s->ta.nkey, s->qpos,   s->srcmap};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;

@@ -491,6 +491,36 @@ def test_bounce_placement():
     assert nev > 100
 
 
+@pytest.mark.parametrize("walls,precision", [("ppp", 64), ("rrr", 64), ("ppp", 32)])
+def test_bounce_reflection_method(walls, precision):
+    """Bounce by reflection (rparam -2): each pair's paths from the places
+    before diffusion (posx, kept per record for this) are reflected at the
+    moment of contact; pairs across the periodic boundary move posx with
+    the faster molecule (morebireact).  Bit-exact with the oracle over the
+    tile and counting-sort steps.  FP32-offset records: the first step's
+    events equal the FP64 run's (bounced products sit at the binding radius,
+    so later steps part at the FP32 rounding), and the species counts of the
+    conserving reactions hold."""
+    spec = dict(low=[0, 0, 0], high=[25, 25, 25], walls=walls, dt=0.01, seed=61,
+                species={"A": 1.0, "B": 0.5},
+                reactions=[("hitAB", ["A", "B"], ["A", "B"], 40.0), ("hitAA", ["A", "A"], ["A", "A"], 15.0)],
+                placement={"hitAB": ("bounce", -2.0), "hitAA": ("bounce", -2.0)},
+                molecules=[("A", 1500), ("B", 1500)])
+    if precision == 64:
+        nev = _run_pair(spec, 60, check_every=10)
+        assert nev > 100
+        return
+    d64, d32 = ss.build_device(spec), ss.build_device(dict(spec, precision=32))
+    d64.run_time_step()
+    d32.run_time_step()
+    assert ss.canonical_pairs(d64.reaction_log()) == ss.canonical_pairs(d32.reaction_log())
+    nev = 0
+    for _ in range(30):
+        d32.run_time_step()
+        nev += len(d32.reaction_log())
+    assert nev > 50 and d32.count("A") == 1500 and d32.count("B") == 1500
+
+
 @pytest.mark.parametrize("kind,param", [("pgem", 0.3), ("pgemmax", 0.2), ("unbindrad", 1.4)])
 def test_geminate_and_unbinding_placements(kind, param):
     """C -> A + B reversible with A + B -> C, the unbinding separation from
