@@ -45,6 +45,28 @@ def test_reference_model_file_on_device(tmp_path, name, extra):
     assert got == expect
 
 
+def test_reference_surf3_unmodified_on_device(tmp_path):
+    """examples/S95_regression surf3 as the reference ships it, absorbing
+    action for green included, read from the file and run for 2500 steps on
+    the device: the listmols output equals the oracle's on the same counter
+    stream."""
+    text = (S95 / "surf3.txt").read_text()
+    assert "action both green absorb" in text
+    f = tmp_path / "surf3.txt"
+    f.write_text("define OUTFILE lm.txt\n" + text)
+    dsim = sa.Simulation.from_file(f)
+    dsim.set_output_path(str(tmp_path) + "/")
+    assert dsim.run() == sa.ECok
+    dsim.close()
+    osim = read_config(text, lambda d, lo, hi: _CounterOSim(d, lo, hi))
+    osim.update()
+    assert osim.run() == 1
+    expect = osim.listmols()
+    got = (tmp_path / "lm.txt").read_text()
+    assert len(expect.splitlines()) < 1000  # green molecules were absorbed
+    assert got == expect
+
+
 def test_reference_surf3_model_on_device(tmp_path):
     """examples/S95_regression surf3: rectangles, triangles, a sphere,
     cylinders, hemispheres and disks, with reflect, transmit and jump
