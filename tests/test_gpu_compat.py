@@ -132,3 +132,32 @@ def test_runtimestep_through_low_level_class():
         assert s.runTimeStep() == S.ErrorCode.ok
         osim.step()
         ss.compare_states(ss.oracle_state(osim), ss.device_state(s.engine))
+
+
+def test_model_file_through_fromfile(tmp_path):
+    """Simulation.fromFile on the reference's regression model 'molecule'
+    (uniform, ranged and fixed placements in a reflective box), then
+    s.run(stop, dt) with the file's own times: the listmols file its
+    end-of-run command writes equals the oracle's byte for byte."""
+    from smolcfg import read_config
+
+    print("compat: fromFile molecule", flush=True)
+    text = (Path(__file__).resolve().parent / "golden" / "s95" / "molecule.txt").read_text()
+    f = tmp_path / "molecule.txt"
+    f.write_text("define OUTFILE lm.txt\n" + text)
+    s = smoldyn.Simulation.fromFile(f)
+    assert s.setOutputPath(str(tmp_path) + "/") == S.ErrorCode.ok
+    s.run(stop=10, dt=0.01, overwrite=True)
+    s.engine.close()
+
+    class _CounterOSim(ob.OSim):
+        def seed(self, seed):
+            self.rngmode(ob.RNG_COUNTER, seed)
+
+    osim = read_config(text, lambda d, lo, hi: _CounterOSim(d, lo, hi))
+    osim.update()
+    assert osim.run() == 1
+    expect = osim.listmols()
+    got = (tmp_path / "lm.txt").read_text()
+    assert len(expect.splitlines()) > 0
+    assert got == expect
